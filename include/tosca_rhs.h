@@ -1,0 +1,160 @@
+/*
+ * tosca_rhs.h -- C ABI of the MI355X (gfx950) momentum right-hand-side + LES library.
+ *
+ * The entry points replace, one for one, the reference solver routines that evaluate the
+ * momentum-predictor right-hand side of TOSCA (sebastipa/TOSCA):
+ *
+ *   tosca_formu       <-> PetscErrorCode FormU(ueqn_*, Vec &Rhs, PetscReal scale, PetscInt formMode)
+ *                         src/solvers/ueqn_solvers.h:10, body src/solvers/ueqn_solvers.c:6-1083
+ *   tosca_update_cs   <-> PetscErrorCode UpdateCs(les_*)        src/include/les.h:72, src/les.c:165-2115
+ *   tosca_update_nut  <-> PetscErrorCode UpdateNut(les_*)       src/include/les.h:75, src/les.c:2119-2578
+ *                         (includes UpdateNutBCs, src/boundary.c:2630-2929)
+ *   tosca_buoyancy    <-> PetscErrorCode Buoyancy(ueqn_*, PetscReal scale)
+ *                         src/include/ueqn_sources.h:22, src/sources/ueqn_sources.c:2726-2880
+ *
+ * Storage contract (the reference's PETSc DMDA layout, src/mesh.c:136-203):
+ *   - every array is [k][j][i] with i fastest; k <-> x (streamwise), j <-> z (vertical), i <-> y;
+ *   - "local" arrays are the ghosted DMDA local vectors (ghost width 3): gxm*gym*gzm points,
+ *     the pointer addresses element [gzs][gys][gxs], i.e. what VecGetArray returns;
+ *   - "global" arrays (Rhs, bTheta, gCont) hold the owned range only: xm*ym*zm points,
+ *     pointer at element [zs][ys][xs];
+ *   - vectors are AoS {x,y,z} doubles (Cmpnts, src/include/base.h:81-84).
+ * One GPU owns the whole domain: xs=ys=zs=0, xm=mx, ym=my, zm=mz, gxs=gys=gzs=-3,
+ * gxm=mx+6, gym=my+6, gzm=mz+6.  Anything else returns TOSCA_ERR_LAYOUT.
+ *
+ * All pointers are DEVICE pointers (hipMalloc'd or torch CUDA tensors); the caller owns them.
+ * Calls are asynchronous on the context's HIP stream (default: the null stream); results are
+ * ready after tosca_ctx_synchronize() or any later work ordered on that stream.
+ * Errors are returned as non-zero codes (never exit), with a message in tosca_last_error().
+ */
+#ifndef TOSCA_RHS_H
+#define TOSCA_RHS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ------------------------------------------------------------------------ */
+#define TOSCA_OK               0
+#define TOSCA_ERR_ARG          1   /* null / misaligned pointer, bad enum value, bad form_mode   */
+#define TOSCA_ERR_UNSUPPORTED  2   /* IBM, overset, structural LES, schemes other than central/QUICK */
+#define TOSCA_ERR_LAYOUT       3   /* ghost width != 3 or not a whole-domain single-GPU layout   */
+#define TOSCA_ERR_DEVICE       4   /* HIP runtime error or device unavailable                   */
+#define TOSCA_ERR_MESH         5   /* tosca_mesh_check: non-zero nvert/meshTag, bad volume/area */
+
+/* ---- layout: mirrors PETSc's DMDALocalInfo (mesh->info, src/include/mesh.h:40) ----------- */
+typedef struct {
+    int64_t xs, ys, zs, xm, ym, zm;        /* owned range                                 */
+    int64_t gxs, gys, gzs, gxm, gym, gzm;  /* ghosted range                               */
+    int64_t mx, my, mz;                    /* global DMDA sizes (cells + 2 boundary ghosts) */
+} tosca_dmda_info;
+
+/* ---- static mesh arrays (mesh_ fields, src/include/mesh.h:44-60), all local/ghosted ----- */
+typedef struct {
+    const double *csi, *eta, *zet;         /* cell-centred contravariant basis, AoS        */
+    const double *icsi, *ieta, *izet;      /* i-face basis                                 */
+    const double *jcsi, *jeta, *jzet;      /* j-face basis                                 */
+    const double *kcsi, *keta, *kzet;      /* k-face basis                                 */
+    const double *aj, *iaj, *jaj, *kaj;    /* inverse cell / face Jacobians (scalars)      */
+    const double *nvert, *meshtag;         /* IBM / overset markers; only read by tosca_mesh_check */
+} tosca_mesh;
+
+/* periodic connectivity per axis (src/boundary.c:356-612): type 1 = i_periodic, type 2 = ii_periodic */
+enum { TOSCA_PERIODIC_NONE = 0, TOSCA_PERIODIC_COPY = 1, TOSCA_PERIODIC_WRAP = 2 };
+
+/* convective schemes accepted by tosca_formu (ueqn->centralDiv / ueqn->quickDiv, src/ueqn.c:81-110) */
+enum { TOSCA_DIV_CENTRAL = 0, TOSCA_DIV_QUICK = 1 };
+
+/* boundaryU patch kinds that change FormU's viscous flux (ueqn_solvers.c:336-356,548-568,758-768).
+ * Every other velocity BC (noSlip, fixedValue, zeroGradient, inletFunction, ...) is GENERIC.     */
+enum { TOSCA_BCU_GENERIC = 0, TOSCA_BCU_SLIP = 1, TOSCA_BCU_WALLFUNCTION = 2, TOSCA_BCU_PERIODIC = 3 };
+
+/* boundaryNut patch kinds applied by UpdateNutBCs (src/boundary.c:2812-2917) */
+enum { TOSCA_BCNUT_NONE = 0, TOSCA_BCNUT_ZEROGRADIENT = 1, TOSCA_BCNUT_FIXEDVALUE = 2, TOSCA_BCNUT_PERIODIC = 3 };
+
+/* LES model numbering of the reference's LesModelNames (src/les.c:16-28) */
+enum {
+    TOSCA_LES_SMAGORINSKY = 0, TOSCA_LES_STABILITY = 1, TOSCA_LES_DSM = 2, TOSCA_LES_DLASI = 3,
+    TOSCA_LES_DLASD = 4, TOSCA_LES_DPASD = 5, TOSCA_LES_AMD = 6, TOSCA_LES_VREMAN = 7,
+    TOSCA_LES_BV = 8, TOSCA_LES_BAMD = 9
+};
+
+/* patch order used by bc_u / bc_nut / nut_value: iLeft, iRight, jLeft, jRight, kLeft, kRight */
+typedef struct {
+    int    periodic[3];      /* i, j, k: TOSCA_PERIODIC_*                                       */
+    int    bc_u[6];          /* TOSCA_BCU_*                                                     */
+    int    bc_nut[6];        /* TOSCA_BCNUT_*                                                   */
+    double nut_value[6];     /* fixedValue nut per patch (boundaryNut.iLval ...)                */
+    int    div_scheme;       /* TOSCA_DIV_*                                                     */
+    int    les_active;       /* flags->isLesActive                                              */
+    int    les_model;        /* TOSCA_LES_*; supported: SMAGORINSKY, DSM, DPASD                 */
+    int    ibm_active;       /* flags->isIBMActive: must be 0                                   */
+    int    overset_active;   /* flags->isOversetActive: must be 0                               */
+    double nu;               /* constants->nu                                                   */
+    double max_cs;           /* les->maxCs (default 0.5, src/les.c:49)                          */
+    double t_ref;            /* abl->tRef or constants->tRef (ueqn_sources.c:2749-2751)         */
+    /* wall-function shear stress (ueqn->iLWM / jLWM ->tauWall), used only where a patch is
+     * TOSCA_BCU_WALLFUNCTION: three planes x|y|z, each [zm][ym] (i patches) or [zm][xm]
+     * (j patches), device pointers; the i pair and the j pair each share one array, as in
+     * the reference (ueqn_solvers.c:336-345, 548-556).                                      */
+    const double *tau_wall_i;
+    const double *tau_wall_j;
+} tosca_params;
+
+/* ---- context ------------------------------------------------------------------------------ */
+typedef struct tosca_ctx tosca_ctx;
+
+int         tosca_ctx_create(tosca_ctx **ctx, const tosca_dmda_info *info, int device);
+int         tosca_ctx_destroy(tosca_ctx *ctx);
+int         tosca_ctx_set_stream(tosca_ctx *ctx, void *hip_stream);   /* hipStream_t, NULL = default */
+int         tosca_ctx_synchronize(tosca_ctx *ctx);
+const char *tosca_last_error(const tosca_ctx *ctx);
+const char *tosca_version(void);
+
+/* Validates the static mesh once (replaces the checks of SetMeshMetrics, src/mesh.c:1306-1346,
+ * and the IBM/overset flags): nvert and meshtag all zero, aj > 1e-20 and |csi|,|eta|,|zet| >= 1e-20
+ * on non-corner cells.  Synchronous.  Returns TOSCA_ERR_MESH with a message on failure. */
+int tosca_mesh_check(tosca_ctx *ctx, const tosca_mesh *mesh);
+
+/* FormU (src/solvers/ueqn_solvers.c:6-1083).
+ *   ucont: lUcont (local AoS), ucat: lUcat (local AoS), nut: les->lNu_t (local scalar; read only
+ *   when params->les_active), rhs: global AoS, ACCUMULATED (rhs += scale * ...).
+ *   form_mode: 0 = convective + viscous, 1 = convective only, 2 = viscous only. */
+int tosca_formu(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_params *params,
+                const double *ucont, const double *ucat, const double *nut,
+                double *rhs, double scale, int form_mode);
+
+/* UpdateCs (src/les.c:165-2115): writes les->lCs (local scalar) for SMAGORINSKY (Cs = 0.0289
+ * everywhere), DSM (test filter + Germano + 3x3x3 box average + clip) and DPASD (3- and 5-point
+ * test filters + j-plane averages + clip).  Uses context workspace. */
+int tosca_update_cs(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_params *params,
+                    const double *ucat, double *cs);
+
+/* UpdateNut (src/les.c:2119-2578, Smagorinsky/dynamic branch :2510-2535) followed by
+ * UpdateNutBCs (src/boundary.c:2630-2929): writes lNu_t (local scalar, all 6 boundary patches)
+ * and, when ksgs != NULL, lKsgs. */
+int tosca_update_nut(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_params *params,
+                     const double *ucat, const double *cs, double *nut, double *ksgs);
+
+/* Buoyancy (src/sources/ueqn_sources.c:2726-2880): theta = teqn->lTmprt (local scalar);
+ * btheta = ueqn->bTheta (global AoS) is zeroed and filled; gcont (global AoS, may be NULL)
+ * receives gravity . face area vector on the faces the reference computes it on. */
+int tosca_buoyancy(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_params *params,
+                   const double *theta, double scale, double *btheta, double *gcont);
+
+/* One full right-hand-side + LES evaluation, the unit of work of the benchmark:
+ * UpdateCs -> UpdateNut -> Buoyancy(buoy_scale) -> FormU(rhs, scale, form_mode 0).
+ * Outputs are exactly those of the four calls above (cs, nut, btheta written, rhs accumulated);
+ * the kernels are fused where the data flow allows. */
+int tosca_rhs_les_step(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_params *params,
+                       const double *ucont, const double *ucat, const double *theta,
+                       double *rhs, double scale, double *cs, double *nut,
+                       double *btheta, double buoy_scale);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TOSCA_RHS_H */

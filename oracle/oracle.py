@@ -1,0 +1,109 @@
+"""ctypes wrapper of the CPU oracle (oracle/tosca_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke check and the benchmark's CPU-baseline
+leg, never by the product library.  Arrays are numpy float64 in the DMDA local layout of
+include/tosca_rhs.h (host memory).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+from typing import Mapping, Optional
+
+import numpy as np
+
+from tosca_amd import abi
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+
+_lib = None
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB_PATH
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        build()
+    lib = C.CDLL(str(LIB_PATH))
+    P, I, D = C.c_void_p, C.c_int, C.c_double
+    M, Pa, In = C.POINTER(abi.Mesh), C.POINTER(abi.Params), C.POINTER(abi.DmdaInfo)
+    sigs = {
+        "oracle_last_error": (C.c_char_p, []),
+        "oracle_set_threads": (None, [I]),
+        "oracle_max_threads": (I, []),
+        "oracle_formu": (I, [In, M, Pa, P, P, P, P, D, I]),
+        "oracle_update_cs": (I, [In, M, Pa, P, P]),
+        "oracle_update_nut": (I, [In, M, Pa, P, P, P, P]),
+        "oracle_buoyancy": (I, [In, M, Pa, P, D, P, P]),
+        "oracle_rhs_les_step": (I, [In, M, Pa, P, P, P, P, D, P, P, P, D]),
+    }
+    for n, (r, a) in sigs.items():
+        f = getattr(lib, n)
+        f.restype = r
+        f.argtypes = a
+    _lib = lib
+    return lib
+
+
+def _p(a: Optional[np.ndarray]):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data
+
+
+def _mesh(mesh: Mapping[str, np.ndarray]):
+    return abi.make_mesh({n: _p(mesh[n]) for n in abi.MESH_FIELDS if n in mesh})
+
+
+def _check(rc):
+    if rc != 0:
+        raise abi.ToscaError(rc, load().oracle_last_error().decode())
+
+
+def set_threads(n: int):
+    load().oracle_set_threads(int(n))
+
+
+def max_threads() -> int:
+    return load().oracle_max_threads()
+
+
+def formu(info, mesh, params, ucont, ucat, nut, rhs, scale=1.0, form_mode=0):
+    lib = load()
+    inf, m, p = abi.make_info(info), _mesh(mesh), abi.make_params(params)
+    _check(lib.oracle_formu(C.byref(inf), C.byref(m), C.byref(p), _p(ucont), _p(ucat), _p(nut), _p(rhs),
+                            float(scale), int(form_mode)))
+
+
+def update_cs(info, mesh, params, ucat, cs):
+    lib = load()
+    inf, m, p = abi.make_info(info), _mesh(mesh), abi.make_params(params)
+    _check(lib.oracle_update_cs(C.byref(inf), C.byref(m), C.byref(p), _p(ucat), _p(cs)))
+
+
+def update_nut(info, mesh, params, ucat, cs, nut, ksgs=None):
+    lib = load()
+    inf, m, p = abi.make_info(info), _mesh(mesh), abi.make_params(params)
+    _check(lib.oracle_update_nut(C.byref(inf), C.byref(m), C.byref(p), _p(ucat), _p(cs), _p(nut), _p(ksgs)))
+
+
+def buoyancy(info, mesh, params, theta, scale, btheta, gcont=None):
+    lib = load()
+    inf, m, p = abi.make_info(info), _mesh(mesh), abi.make_params(params)
+    _check(lib.oracle_buoyancy(C.byref(inf), C.byref(m), C.byref(p), _p(theta), float(scale), _p(btheta), _p(gcont)))
+
+
+def rhs_les_step(info, mesh, params, ucont, ucat, theta, rhs, scale, cs, nut, btheta, buoy_scale=1.0):
+    lib = load()
+    inf, m, p = abi.make_info(info), _mesh(mesh), abi.make_params(params)
+    _check(lib.oracle_rhs_les_step(C.byref(inf), C.byref(m), C.byref(p), _p(ucont), _p(ucat), _p(theta), _p(rhs),
+                                   float(scale), _p(cs), _p(nut), _p(btheta), float(buoy_scale)))

@@ -1,0 +1,1028 @@
+// tosca_oracle.cpp -- CPU oracle for the momentum right-hand side + LES path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the checker the HIP library is compared against; it is
+// never linked into, called by, or used as a fallback for the product library.  Only tests/,
+// the smoke check and the CPU-baseline leg of the benchmark load it.
+//
+// It restates, pass for pass and with the same workspace arrays, loop order and FP64 expression
+// order, the reference routines of sebastipa/TOSCA (paths relative to the reference root):
+//   FormU            src/solvers/ueqn_solvers.c:6-1083
+//   UpdateCs         src/les.c:165-2115 (Smagorinsky :172-179, DSM :197-811 + :1779-1977,
+//                                        DPASD second filter :812-1174, plane averages :1979-2048)
+//   UpdateNut        src/les.c:2119-2578 (Smagorinsky / dynamic branch :2510-2535)
+//   UpdateNutBCs     src/boundary.c:2630-2929 (zeroGradient / fixedValue / periodic subset)
+//   Buoyancy         src/sources/ueqn_sources.c:2726-2880
+// and the helpers of src/include/inline.h they call (cited at each helper below).  PETSc's
+// single-rank DMLocalToLocal scatter is restated from PETSc's documented DMDA semantics:
+// a ghost point of a DM_BOUNDARY_PERIODIC axis holds the owned value at (index mod m); ghost
+// points of a DM_BOUNDARY_GHOSTED axis are left untouched.
+//
+// Build: g++ -O3 -ffp-contract=off -fopenmp -shared -fPIC (see oracle/Makefile).  FP contraction
+// is off so every product and sum is rounded exactly as the reference's gcc build without FMA.
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#include <algorithm>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "../include/tosca_rhs.h"
+
+namespace {
+
+struct V3 { double x, y, z; };
+static_assert(sizeof(V3) == 24, "Cmpnts layout");
+
+thread_local std::string g_err;
+
+struct Lay {
+    int64_t mx, my, mz;
+    int64_t gxs, gys, gzs, gxm, gym, gzm;
+    int per[3];      // periodic type per axis i, j, k
+    inline int64_t L(int64_t k, int64_t j, int64_t i) const {
+        return ((k - gzs) * gym + (j - gys)) * gxm + (i - gxs);
+    }
+    inline int64_t G(int64_t k, int64_t j, int64_t i) const { return (k * my + j) * mx + i; }
+    int64_t nloc() const { return gxm * gym * gzm; }
+    bool i_per() const { return per[0] == TOSCA_PERIODIC_COPY; }
+    bool ii_per() const { return per[0] == TOSCA_PERIODIC_WRAP; }
+    bool j_per() const { return per[1] == TOSCA_PERIODIC_COPY; }
+    bool jj_per() const { return per[1] == TOSCA_PERIODIC_WRAP; }
+    bool k_per() const { return per[2] == TOSCA_PERIODIC_COPY; }
+    bool kk_per() const { return per[2] == TOSCA_PERIODIC_WRAP; }
+    bool anyper() const { return per[0] || per[1] || per[2]; }
+};
+
+int check_layout(const tosca_dmda_info *in, const tosca_params *p, Lay &l) {
+    if (!in || !p) { g_err = "null info/params"; return TOSCA_ERR_ARG; }
+    if (in->xs != 0 || in->ys != 0 || in->zs != 0 || in->xm != in->mx || in->ym != in->my ||
+        in->zm != in->mz || in->gxs != -3 || in->gys != -3 || in->gzs != -3 ||
+        in->gxm != in->mx + 6 || in->gym != in->my + 6 || in->gzm != in->mz + 6 ||
+        in->mx < 3 || in->my < 3 || in->mz < 3) {
+        g_err = "layout must be a whole-domain DMDA with ghost width 3";
+        return TOSCA_ERR_LAYOUT;
+    }
+    for (int a = 0; a < 3; a++)
+        if (p->periodic[a] < 0 || p->periodic[a] > 2) { g_err = "bad periodic type"; return TOSCA_ERR_ARG; }
+    if (p->ibm_active || p->overset_active) { g_err = "IBM / overset not supported"; return TOSCA_ERR_UNSUPPORTED; }
+    l.mx = in->mx; l.my = in->my; l.mz = in->mz;
+    l.gxs = in->gxs; l.gys = in->gys; l.gzs = in->gzs;
+    l.gxm = in->gxm; l.gym = in->gym; l.gzm = in->gzm;
+    for (int a = 0; a < 3; a++) l.per[a] = p->periodic[a];
+    return TOSCA_OK;
+}
+
+// ---- PETSc single-rank DMLocalToLocal (INSERT_VALUES) ------------------------------------------
+// Ghost points whose every out-of-range axis is DM_BOUNDARY_PERIODIC (type-2 periodicity) take the
+// owned value at the wrapped index; other ghost points are not touched.
+template <typename T>
+void local_to_local(const Lay &l, T *a) {
+    const int64_t m[3] = {l.mx, l.my, l.mz};
+    for (int64_t k = l.gzs; k < l.gzs + l.gzm; k++)
+        for (int64_t j = l.gys; j < l.gys + l.gym; j++)
+            for (int64_t i = l.gxs; i < l.gxs + l.gxm; i++) {
+                int64_t idx[3] = {i, j, k};
+                bool ghost = false, fill = true;
+                for (int ax = 0; ax < 3; ax++) {
+                    if (idx[ax] < 0 || idx[ax] >= m[ax]) {
+                        ghost = true;
+                        if (l.per[ax] != TOSCA_PERIODIC_WRAP) fill = false;
+                        idx[ax] = ((idx[ax] % m[ax]) + m[ax]) % m[ax];
+                    }
+                }
+                if (ghost && fill) a[l.L(k, j, i)] = a[l.L(idx[2], idx[1], idx[0])];
+            }
+}
+
+// ---- resetFacePeriodicFluxesVector, localToLocal (src/include/inline.h:5357-5451) ---------------
+void reset_face_periodic(const Lay &l, V3 *v) {
+    if (l.anyper()) {
+        for (int64_t k = 0; k < l.mz; k++)
+            for (int64_t j = 0; j < l.my; j++)
+                for (int64_t i = 0; i < l.mx; i++) {
+                    int flag = 0; int64_t a = i, b = j, c = k;
+                    if (l.i_per() && i == 0) a = l.mx - 2, flag = 1;
+                    if (l.j_per() && j == 0) b = l.my - 2, flag = 1;
+                    if (l.k_per() && k == 0) c = l.mz - 2, flag = 1;
+                    if (l.ii_per() && i == 0) a = -2, flag = 1;
+                    if (l.jj_per() && j == 0) b = -2, flag = 1;
+                    if (l.kk_per() && k == 0) c = -2, flag = 1;
+                    if (flag) v[l.L(k, j, i)] = v[l.L(c, b, a)];
+                }
+    }
+    local_to_local(l, v);
+}
+
+// ---- resetCellPeriodicFluxes, localToLocal (src/include/inline.h:5453-5606) ---------------------
+template <typename T>
+void reset_cell_periodic(const Lay &l, T *v) {
+    if (l.anyper()) {
+        for (int64_t k = 0; k < l.mz; k++)
+            for (int64_t j = 0; j < l.my; j++)
+                for (int64_t i = 0; i < l.mx; i++) {
+                    int flag = 0; int64_t a = i, b = j, c = k;
+                    if (l.i_per() && i == 0) a = l.mx - 2, flag = 1;
+                    else if (l.i_per() && i == l.mx - 1) a = 1, flag = 1;
+                    if (l.j_per() && j == 0) b = l.my - 2, flag = 1;
+                    else if (l.j_per() && j == l.my - 1) b = 1, flag = 1;
+                    if (l.k_per() && k == 0) c = l.mz - 2, flag = 1;
+                    else if (l.k_per() && k == l.mz - 1) c = 1, flag = 1;
+                    if (l.ii_per() && i == 0) a = -2, flag = 1;
+                    else if (l.ii_per() && i == l.mx - 1) a = l.mx + 1, flag = 1;
+                    if (l.jj_per() && j == 0) b = -2, flag = 1;
+                    else if (l.jj_per() && j == l.my - 1) b = l.my + 1, flag = 1;
+                    if (l.kk_per() && k == 0) c = -2, flag = 1;
+                    else if (l.kk_per() && k == l.mz - 1) c = l.mz + 1, flag = 1;
+                    if (flag) v[l.L(k, j, i)] = v[l.L(c, b, a)];
+                }
+    }
+    local_to_local(l, v);
+}
+
+// ---- convective interpolants (src/include/inline.h:4314-4324, 5089-5109, 683) --------------------
+inline V3 centralVec(V3 f0, V3 f1) { return {0.5 * (f0.x + f1.x), 0.5 * (f0.y + f1.y), 0.5 * (f0.z + f1.z)}; }
+
+inline V3 quadraticUpwindVec(V3 f0, V3 f1, V3 f2, V3 f3, double wavespeed) {
+    V3 fUU, fU, fD;
+    if (wavespeed > 0) { fUU = f0; fU = f1; fD = f2; }
+    else               { fUU = f3; fU = f2; fD = f1; }
+    // nSum(nScale(0.5, nSum(fU, fD)), nScale(1./8., nSub(nScale(2.0, fU), nSum(fD, fUU))))
+    V3 r;
+    r.x = 0.5 * (fU.x + fD.x) + (1. / 8.) * (2.0 * fU.x - (fD.x + fUU.x));
+    r.y = 0.5 * (fU.y + fD.y) + (1. / 8.) * (2.0 * fU.y - (fD.y + fUU.y));
+    r.z = 0.5 * (fU.z + fD.z) + (1. / 8.) * (2.0 * fU.z - (fD.z + fUU.z));
+    return r;
+}
+
+inline V3 nScale(double c, V3 v) { return {c * v.x, c * v.y, c * v.z}; }
+
+// ---- getFace2Cell4Stencil{Csi,Eta,Zet} with nvert = meshTag = 0 (src/include/inline.h:2190-2380)
+inline void face2cell4(int ptype, int64_t f, int64_t m, int64_t *L, int64_t *R) {
+    if (f == 0 || f == m - 2) {
+        if (ptype == TOSCA_PERIODIC_COPY) { *L = m - 3; *R = 2; }
+        else if (ptype == TOSCA_PERIODIC_WRAP && f == m - 2) { *R = m + 2; *L = f - 1; }
+        else if (ptype == TOSCA_PERIODIC_WRAP && f == 0) { *L = -3; *R = f + 2; }
+        else {
+            if (f == 0) { *L = f; *R = f + 2; }
+            if (f == m - 2) { *L = f - 1; *R = f + 1; }
+        }
+    } else {
+        *L = f - 1; *R = f + 2;
+    }
+}
+
+struct Grad { double dudc, dvdc, dwdc, dude, dvde, dwde, dudz, dvdz, dwdz; };
+
+// ---- Compute_du_i / _j / _k with nvert = meshTag = 0 (src/include/inline.h:2748-2931) ------------
+inline Grad du_i(const Lay &l, const V3 *u, int64_t i, int64_t j, int64_t k) {
+    const bool iN = !l.i_per() && !l.ii_per(), jN = !l.j_per() && !l.jj_per(), kN = !l.k_per() && !l.kk_per();
+    auto U = [&](int64_t kk, int64_t jj, int64_t ii) { return u[l.L(kk, jj, ii)]; };
+    Grad g;
+    g.dudc = U(k, j, i + 1).x - U(k, j, i).x;
+    g.dvdc = U(k, j, i + 1).y - U(k, j, i).y;
+    g.dwdc = U(k, j, i + 1).z - U(k, j, i).z;
+    if (j == l.my - 2 && (i == 0 || i == l.mx - 2) && (jN || iN)) {
+        g.dude = (U(k, j, i + 1).x + U(k, j, i).x - U(k, j - 1, i + 1).x - U(k, j - 1, i).x) * 0.5;
+        g.dvde = (U(k, j, i + 1).y + U(k, j, i).y - U(k, j - 1, i + 1).y - U(k, j - 1, i).y) * 0.5;
+        g.dwde = (U(k, j, i + 1).z + U(k, j, i).z - U(k, j - 1, i + 1).z - U(k, j - 1, i).z) * 0.5;
+    } else if (j == 1 && (i == 0 || i == l.mx - 2) && (jN || iN)) {
+        g.dude = (U(k, j + 1, i + 1).x + U(k, j + 1, i).x - U(k, j, i + 1).x - U(k, j, i).x) * 0.5;
+        g.dvde = (U(k, j + 1, i + 1).y + U(k, j + 1, i).y - U(k, j, i + 1).y - U(k, j, i).y) * 0.5;
+        g.dwde = (U(k, j + 1, i + 1).z + U(k, j + 1, i).z - U(k, j, i + 1).z - U(k, j, i).z) * 0.5;
+    } else {
+        g.dude = (U(k, j + 1, i + 1).x + U(k, j + 1, i).x - U(k, j - 1, i + 1).x - U(k, j - 1, i).x) * 0.25;
+        g.dvde = (U(k, j + 1, i + 1).y + U(k, j + 1, i).y - U(k, j - 1, i + 1).y - U(k, j - 1, i).y) * 0.25;
+        g.dwde = (U(k, j + 1, i + 1).z + U(k, j + 1, i).z - U(k, j - 1, i + 1).z - U(k, j - 1, i).z) * 0.25;
+    }
+    if (k == l.mz - 2 && (i == 0 || i == l.mx - 2) && (kN || iN)) {
+        g.dudz = (U(k, j, i + 1).x + U(k, j, i).x - U(k - 1, j, i + 1).x - U(k - 1, j, i).x) * 0.5;
+        g.dvdz = (U(k, j, i + 1).y + U(k, j, i).y - U(k - 1, j, i + 1).y - U(k - 1, j, i).y) * 0.5;
+        g.dwdz = (U(k, j, i + 1).z + U(k, j, i).z - U(k - 1, j, i + 1).z - U(k - 1, j, i).z) * 0.5;
+    } else if (k == 1 && (i == 0 || i == l.mx - 2) && (kN || iN)) {
+        g.dudz = (U(k + 1, j, i + 1).x + U(k + 1, j, i).x - U(k, j, i + 1).x - U(k, j, i).x) * 0.5;
+        g.dvdz = (U(k + 1, j, i + 1).y + U(k + 1, j, i).y - U(k, j, i + 1).y - U(k, j, i).y) * 0.5;
+        g.dwdz = (U(k + 1, j, i + 1).z + U(k + 1, j, i).z - U(k, j, i + 1).z - U(k, j, i).z) * 0.5;
+    } else {
+        g.dudz = (U(k + 1, j, i + 1).x + U(k + 1, j, i).x - U(k - 1, j, i + 1).x - U(k - 1, j, i).x) * 0.25;
+        g.dvdz = (U(k + 1, j, i + 1).y + U(k + 1, j, i).y - U(k - 1, j, i + 1).y - U(k - 1, j, i).y) * 0.25;
+        g.dwdz = (U(k + 1, j, i + 1).z + U(k + 1, j, i).z - U(k - 1, j, i + 1).z - U(k - 1, j, i).z) * 0.25;
+    }
+    return g;
+}
+
+inline Grad du_j(const Lay &l, const V3 *u, int64_t i, int64_t j, int64_t k) {
+    const bool iN = !l.i_per() && !l.ii_per(), jN = !l.j_per() && !l.jj_per(), kN = !l.k_per() && !l.kk_per();
+    auto U = [&](int64_t kk, int64_t jj, int64_t ii) { return u[l.L(kk, jj, ii)]; };
+    Grad g;
+    if (i == l.mx - 2 && (j == 0 || j == l.my - 2) && (iN || jN)) {
+        g.dudc = (U(k, j + 1, i).x + U(k, j, i).x - U(k, j + 1, i - 1).x - U(k, j, i - 1).x) * 0.5;
+        g.dvdc = (U(k, j + 1, i).y + U(k, j, i).y - U(k, j + 1, i - 1).y - U(k, j, i - 1).y) * 0.5;
+        g.dwdc = (U(k, j + 1, i).z + U(k, j, i).z - U(k, j + 1, i - 1).z - U(k, j, i - 1).z) * 0.5;
+    } else if (i == 1 && (j == 0 || j == l.my - 2) && (iN || jN)) {
+        g.dudc = (U(k, j + 1, i + 1).x + U(k, j, i + 1).x - U(k, j + 1, i).x - U(k, j, i).x) * 0.5;
+        g.dvdc = (U(k, j + 1, i + 1).y + U(k, j, i + 1).y - U(k, j + 1, i).y - U(k, j, i).y) * 0.5;
+        g.dwdc = (U(k, j + 1, i + 1).z + U(k, j, i + 1).z - U(k, j + 1, i).z - U(k, j, i).z) * 0.5;
+    } else {
+        g.dudc = (U(k, j + 1, i + 1).x + U(k, j, i + 1).x - U(k, j + 1, i - 1).x - U(k, j, i - 1).x) * 0.25;
+        g.dvdc = (U(k, j + 1, i + 1).y + U(k, j, i + 1).y - U(k, j + 1, i - 1).y - U(k, j, i - 1).y) * 0.25;
+        g.dwdc = (U(k, j + 1, i + 1).z + U(k, j, i + 1).z - U(k, j + 1, i - 1).z - U(k, j, i - 1).z) * 0.25;
+    }
+    g.dude = U(k, j + 1, i).x - U(k, j, i).x;
+    g.dvde = U(k, j + 1, i).y - U(k, j, i).y;
+    g.dwde = U(k, j + 1, i).z - U(k, j, i).z;
+    if (k == l.mz - 2 && (j == 0 || j == l.my - 2) && (kN || jN)) {
+        g.dudz = (U(k, j + 1, i).x + U(k, j, i).x - U(k - 1, j + 1, i).x - U(k - 1, j, i).x) * 0.5;
+        g.dvdz = (U(k, j + 1, i).y + U(k, j, i).y - U(k - 1, j + 1, i).y - U(k - 1, j, i).y) * 0.5;
+        g.dwdz = (U(k, j + 1, i).z + U(k, j, i).z - U(k - 1, j + 1, i).z - U(k - 1, j, i).z) * 0.5;
+    } else if (k == 1 && (j == 0 || j == l.my - 2) && (kN || jN)) {
+        g.dudz = (U(k + 1, j + 1, i).x + U(k + 1, j, i).x - U(k, j + 1, i).x - U(k, j, i).x) * 0.5;
+        g.dvdz = (U(k + 1, j + 1, i).y + U(k + 1, j, i).y - U(k, j + 1, i).y - U(k, j, i).y) * 0.5;
+        g.dwdz = (U(k + 1, j + 1, i).z + U(k + 1, j, i).z - U(k, j + 1, i).z - U(k, j, i).z) * 0.5;
+    } else {
+        g.dudz = (U(k + 1, j + 1, i).x + U(k + 1, j, i).x - U(k - 1, j + 1, i).x - U(k - 1, j, i).x) * 0.25;
+        g.dvdz = (U(k + 1, j + 1, i).y + U(k + 1, j, i).y - U(k - 1, j + 1, i).y - U(k - 1, j, i).y) * 0.25;
+        g.dwdz = (U(k + 1, j + 1, i).z + U(k + 1, j, i).z - U(k - 1, j + 1, i).z - U(k - 1, j, i).z) * 0.25;
+    }
+    return g;
+}
+
+inline Grad du_k(const Lay &l, const V3 *u, int64_t i, int64_t j, int64_t k) {
+    const bool iN = !l.i_per() && !l.ii_per(), jN = !l.j_per() && !l.jj_per(), kN = !l.k_per() && !l.kk_per();
+    auto U = [&](int64_t kk, int64_t jj, int64_t ii) { return u[l.L(kk, jj, ii)]; };
+    Grad g;
+    if (i == l.mx - 2 && (k == 0 || k == l.mz - 2) && (iN || kN)) {
+        g.dudc = (U(k + 1, j, i).x + U(k, j, i).x - U(k + 1, j, i - 1).x - U(k, j, i - 1).x) * 0.5;
+        g.dvdc = (U(k + 1, j, i).y + U(k, j, i).y - U(k + 1, j, i - 1).y - U(k, j, i - 1).y) * 0.5;
+        g.dwdc = (U(k + 1, j, i).z + U(k, j, i).z - U(k + 1, j, i - 1).z - U(k, j, i - 1).z) * 0.5;
+    } else if (i == 1 && (k == 0 || k == l.mz - 2) && (iN || kN)) {
+        g.dudc = (U(k + 1, j, i + 1).x + U(k, j, i + 1).x - U(k + 1, j, i).x - U(k, j, i).x) * 0.5;
+        g.dvdc = (U(k + 1, j, i + 1).y + U(k, j, i + 1).y - U(k + 1, j, i).y - U(k, j, i).y) * 0.5;
+        g.dwdc = (U(k + 1, j, i + 1).z + U(k, j, i + 1).z - U(k + 1, j, i).z - U(k, j, i).z) * 0.5;
+    } else {
+        g.dudc = (U(k + 1, j, i + 1).x + U(k, j, i + 1).x - U(k + 1, j, i - 1).x - U(k, j, i - 1).x) * 0.25;
+        g.dvdc = (U(k + 1, j, i + 1).y + U(k, j, i + 1).y - U(k + 1, j, i - 1).y - U(k, j, i - 1).y) * 0.25;
+        g.dwdc = (U(k + 1, j, i + 1).z + U(k, j, i + 1).z - U(k + 1, j, i - 1).z - U(k, j, i - 1).z) * 0.25;
+    }
+    if (j == l.my - 2 && (k == 0 || k == l.mz - 2) && (jN || kN)) {
+        g.dude = (U(k + 1, j, i).x + U(k, j, i).x - U(k + 1, j - 1, i).x - U(k, j - 1, i).x) * 0.5;
+        g.dvde = (U(k + 1, j, i).y + U(k, j, i).y - U(k + 1, j - 1, i).y - U(k, j - 1, i).y) * 0.5;
+        g.dwde = (U(k + 1, j, i).z + U(k, j, i).z - U(k + 1, j - 1, i).z - U(k, j - 1, i).z) * 0.5;
+    } else if (j == 1 && (k == 0 || k == l.mz - 2) && (jN || kN)) {
+        g.dude = (U(k + 1, j + 1, i).x + U(k, j + 1, i).x - U(k + 1, j, i).x - U(k, j, i).x) * 0.5;
+        g.dvde = (U(k + 1, j + 1, i).y + U(k, j + 1, i).y - U(k + 1, j, i).y - U(k, j, i).y) * 0.5;
+        g.dwde = (U(k + 1, j + 1, i).z + U(k, j + 1, i).z - U(k + 1, j, i).z - U(k, j, i).z) * 0.5;
+    } else {
+        g.dude = (U(k + 1, j + 1, i).x + U(k, j + 1, i).x - U(k + 1, j - 1, i).x - U(k, j - 1, i).x) * 0.25;
+        g.dvde = (U(k + 1, j + 1, i).y + U(k, j + 1, i).y - U(k + 1, j - 1, i).y - U(k, j - 1, i).y) * 0.25;
+        g.dwde = (U(k + 1, j + 1, i).z + U(k, j + 1, i).z - U(k + 1, j - 1, i).z - U(k, j - 1, i).z) * 0.25;
+    }
+    g.dudz = U(k + 1, j, i).x - U(k, j, i).x;
+    g.dvdz = U(k + 1, j, i).y - U(k, j, i).y;
+    g.dwdz = U(k + 1, j, i).z - U(k, j, i).z;
+    return g;
+}
+
+// ---- Compute_du_center with nvert = meshTag = 0 (src/include/inline.h:3344-3418) ----------------
+inline Grad du_center(const Lay &l, const V3 *u, int64_t i, int64_t j, int64_t k) {
+    auto U = [&](int64_t kk, int64_t jj, int64_t ii) { return u[l.L(kk, jj, ii)]; };
+    const bool iN = !l.i_per() && !l.ii_per(), jN = !l.j_per() && !l.jj_per(), kN = !l.k_per() && !l.kk_per();
+    Grad g;
+    if (iN && i == l.mx - 2) {
+        g.dudc = (U(k, j, i).x - U(k, j, i - 1).x); g.dvdc = (U(k, j, i).y - U(k, j, i - 1).y); g.dwdc = (U(k, j, i).z - U(k, j, i - 1).z);
+    } else if (iN && i == 1) {
+        g.dudc = (U(k, j, i + 1).x - U(k, j, i).x); g.dvdc = (U(k, j, i + 1).y - U(k, j, i).y); g.dwdc = (U(k, j, i + 1).z - U(k, j, i).z);
+    } else {
+        g.dudc = (U(k, j, i + 1).x - U(k, j, i - 1).x) * 0.5; g.dvdc = (U(k, j, i + 1).y - U(k, j, i - 1).y) * 0.5; g.dwdc = (U(k, j, i + 1).z - U(k, j, i - 1).z) * 0.5;
+    }
+    if (jN && j == l.my - 2) {
+        g.dude = (U(k, j, i).x - U(k, j - 1, i).x); g.dvde = (U(k, j, i).y - U(k, j - 1, i).y); g.dwde = (U(k, j, i).z - U(k, j - 1, i).z);
+    } else if (jN && j == 1) {
+        g.dude = (U(k, j + 1, i).x - U(k, j, i).x); g.dvde = (U(k, j + 1, i).y - U(k, j, i).y); g.dwde = (U(k, j + 1, i).z - U(k, j, i).z);
+    } else {
+        g.dude = (U(k, j + 1, i).x - U(k, j - 1, i).x) * 0.5; g.dvde = (U(k, j + 1, i).y - U(k, j - 1, i).y) * 0.5; g.dwde = (U(k, j + 1, i).z - U(k, j - 1, i).z) * 0.5;
+    }
+    if (kN && k == l.mz - 2) {
+        g.dudz = (U(k, j, i).x - U(k - 1, j, i).x); g.dvdz = (U(k, j, i).y - U(k - 1, j, i).y); g.dwdz = (U(k, j, i).z - U(k - 1, j, i).z);
+    } else if (kN && k == 1) {
+        g.dudz = (U(k + 1, j, i).x - U(k, j, i).x); g.dvdz = (U(k + 1, j, i).y - U(k, j, i).y); g.dwdz = (U(k + 1, j, i).z - U(k, j, i).z);
+    } else {
+        g.dudz = (U(k + 1, j, i).x - U(k - 1, j, i).x) * 0.5; g.dvdz = (U(k + 1, j, i).y - U(k - 1, j, i).y) * 0.5; g.dwdz = (U(k + 1, j, i).z - U(k - 1, j, i).z) * 0.5;
+    }
+    return g;
+}
+
+struct Cart { double du_dx, du_dy, du_dz, dv_dx, dv_dy, dv_dz, dw_dx, dw_dy, dw_dz; };
+
+// ---- Compute_du_dxyz (src/include/inline.h:3635-3663) --------------------------------------------
+inline Cart du_dxyz(V3 c, V3 e, V3 z, double ajc, const Grad &g) {
+    Cart r;
+    r.du_dx = (g.dudc * c.x + g.dude * e.x + g.dudz * z.x) * ajc;
+    r.du_dy = (g.dudc * c.y + g.dude * e.y + g.dudz * z.y) * ajc;
+    r.du_dz = (g.dudc * c.z + g.dude * e.z + g.dudz * z.z) * ajc;
+    r.dv_dx = (g.dvdc * c.x + g.dvde * e.x + g.dvdz * z.x) * ajc;
+    r.dv_dy = (g.dvdc * c.y + g.dvde * e.y + g.dvdz * z.y) * ajc;
+    r.dv_dz = (g.dvdc * c.z + g.dvde * e.z + g.dvdz * z.z) * ajc;
+    r.dw_dx = (g.dwdc * c.x + g.dwde * e.x + g.dwdz * z.x) * ajc;
+    r.dw_dy = (g.dwdc * c.y + g.dwde * e.y + g.dwdz * z.y) * ajc;
+    r.dw_dz = (g.dwdc * c.z + g.dwde * e.z + g.dwdz * z.z) * ajc;
+    return r;
+}
+
+// |S| = sqrt(2 S:S) (src/les.c:342-354, 2265-2271)
+inline double strain_mag(const Cart &d) {
+    double Sxx = 0.5 * (d.du_dx + d.du_dx), Sxy = 0.5 * (d.du_dy + d.dv_dx), Sxz = 0.5 * (d.du_dz + d.dw_dx);
+    double Syx = Sxy, Syy = 0.5 * (d.dv_dy + d.dv_dy), Syz = 0.5 * (d.dv_dz + d.dw_dy);
+    double Szx = Sxz, Szy = Syz, Szz = 0.5 * (d.dw_dz + d.dw_dz);
+    return std::sqrt(2.0 * (Sxx * Sxx + Sxy * Sxy + Sxz * Sxz + Syx * Syx + Syy * Syy + Syz * Syz +
+                            Szx * Szx + Szy * Szy + Szz * Szz));
+}
+
+inline double nMag(V3 v) { return std::sqrt(v.x * v.x + v.y * v.y + v.z * v.z); }
+
+// PetscMin / PetscMax
+inline double pmax(double a, double b) { return (a < b) ? b : a; }
+inline double pmin(double a, double b) { return (a < b) ? a : b; }
+
+// ---- integrateTestfilterSimpson / 5x5 (src/include/inline.h:3592-3633) ---------------------------
+double simpson3(const double val[3][3][3], const double w[3][3][3]) {
+    double wsum = 0, valsum = 0;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            for (int k = 0; k < 3; k++) {
+                double sw = 1.0;
+                if (i == 1) sw *= 4.;
+                if (j == 1) sw *= 4.;
+                if (k == 1) sw *= 4.;
+                wsum += sw * w[i][j][k];
+                valsum += sw * w[i][j][k] * val[i][j][k];
+            }
+    return valsum / wsum;
+}
+
+double simpson5(const double val[5][5][5], const double w[5][5][5]) {
+    double wsum = 0, valsum = 0;
+    for (int i = 0; i < 5; i++)
+        for (int j = 0; j < 5; j++)
+            for (int k = 0; k < 5; k++) {
+                double sw = 1.0;
+                sw *= (i == 0 || i == 4) ? 1 : (i % 2 == 0 ? 2 : 4);
+                sw *= (j == 0 || j == 4) ? 1 : (j % 2 == 0 ? 2 : 4);
+                sw *= (k == 0 || k == 4) ? 1 : (k % 2 == 0 ? 2 : 4);
+                wsum += sw * w[i][j][k];
+                valsum += sw * w[i][j][k] * val[i][j][k];
+            }
+    return valsum / wsum;
+}
+
+// ---- Calculate_Covariant_metrics (src/include/inline.h:1210-1229) --------------------------------
+void covariant(const double g[3][3], double G[3][3]) {
+    const double a11 = g[0][0], a12 = g[0][1], a13 = g[0][2];
+    const double a21 = g[1][0], a22 = g[1][1], a23 = g[1][2];
+    const double a31 = g[2][0], a32 = g[2][1], a33 = g[2][2];
+    double det = a11 * (a33 * a22 - a32 * a23) - a21 * (a33 * a12 - a32 * a13) + a31 * (a23 * a12 - a22 * a13);
+    G[0][0] = (a33 * a22 - a32 * a23) / det, G[0][1] = -(a33 * a12 - a32 * a13) / det, G[0][2] = (a23 * a12 - a22 * a13) / det;
+    G[1][0] = -(a33 * a21 - a31 * a23) / det, G[1][1] = (a33 * a11 - a31 * a13) / det, G[1][2] = -(a23 * a11 - a21 * a13) / det;
+    G[2][0] = (a32 * a21 - a31 * a22) / det, G[2][1] = -(a32 * a11 - a31 * a12) / det, G[2][2] = (a22 * a11 - a21 * a12) / det;
+}
+
+// Binomial test-filter kernels of src/les.c:597-612 (n=2) and :961-997 (n=4): products of the 1-D
+// rows (1 2 1)/4 and (1 4 6 4 1)/16; every entry is a dyadic rational, equal to the printed tables.
+const double bin3[3] = {0.25, 0.5, 0.25};
+const double bin5[5] = {1.0 / 16, 4.0 / 16, 6.0 / 16, 4.0 / 16, 1.0 / 16};
+
+// Germano contraction shared by the 3- and 5-point test filters (src/les.c:715-799, :1089-1164):
+// returns num = L_ba M_aq G_bq and denom = M_nm M_nl G_ml.
+void germano(const double L[3][3], const double Shat_ij[3][3], const double SShat_ij[3][3],
+             double filter, double test_filter, V3 c0, V3 e0, V3 z0, double *num_out, double *den_out) {
+    double S_hat = 0;
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) S_hat += std::pow(Shat_ij[a][b], 2.);
+    S_hat = std::sqrt(2 * S_hat);
+    double gg[3][3] = {{c0.x, c0.y, c0.z}, {e0.x, e0.y, e0.z}, {z0.x, z0.y, z0.z}}, ggc[3][3], G[3][3];
+    covariant(gg, ggc);
+    double xcsi = ggc[0][0], xeta = ggc[0][1], xzet = ggc[0][2];
+    double ycsi = ggc[1][0], yeta = ggc[1][1], yzet = ggc[1][2];
+    double zcsi = ggc[2][0], zeta = ggc[2][1], zzet = ggc[2][2];
+    G[0][0] = xcsi * xcsi + ycsi * ycsi + zcsi * zcsi;
+    G[1][1] = xeta * xeta + yeta * yeta + zeta * zeta;
+    G[2][2] = xzet * xzet + yzet * yzet + zzet * zzet;
+    G[0][1] = G[1][0] = xeta * xcsi + yeta * ycsi + zeta * zcsi;
+    G[0][2] = G[2][0] = xzet * xcsi + yzet * ycsi + zzet * zcsi;
+    G[1][2] = G[2][1] = xeta * xzet + yeta * yzet + zeta * zzet;
+    double Mc[3][3], M[3][3];
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++)
+            Mc[a][b] = (-std::pow(test_filter, 2.) * S_hat * Shat_ij[a][b] + std::pow(filter, 2.) * SShat_ij[a][b]);
+    for (int a = 0; a < 3; a++) {
+        M[a][0] = Mc[a][0] * c0.x + Mc[a][1] * c0.y + Mc[a][2] * c0.z;
+        M[a][1] = Mc[a][0] * e0.x + Mc[a][1] * e0.y + Mc[a][2] * e0.z;
+        M[a][2] = Mc[a][0] * z0.x + Mc[a][1] * z0.y + Mc[a][2] * z0.z;
+    }
+    double num = 0, denom = 0;
+    for (int q = 0; q < 3; q++)
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) num += L[b][a] * M[a][q] * G[b][q];
+    for (int m = 0; m < 3; m++)
+        for (int n = 0; n < 3; n++)
+            for (int l = 0; l < 3; l++) denom += M[n][m] * M[n][l] * G[m][l];
+    *num_out = num;
+    *den_out = denom;
+}
+
+// DSM / DPASD test filter at one interior cell, NP = 3 (src/les.c:466-801) or 5 (:820-1164).
+template <int NP>
+void test_filter_cell(const Lay &l, const tosca_mesh *m, const V3 *ucat, const V3 *Ax, const V3 *Ay,
+                      const V3 *Az, const double *Sabs, int64_t i, int64_t j, int64_t k,
+                      double *num, double *den) {
+    const V3 *csi = (const V3 *)m->csi, *eta = (const V3 *)m->eta, *zet = (const V3 *)m->zet;
+    const double *aj = m->aj;
+    constexpr int H = NP / 2;
+    // 40 stencil quantities (the reference keeps each in its own [NP][NP][NP] array)
+    static thread_local double q[40][NP][NP][NP];
+    double w[NP][NP][NP];
+    for (int p = -H; p <= H; p++)
+        for (int qq = -H; qq <= H; qq++)
+            for (int r = -H; r <= H; r++) {
+                int R = r + H, Q = qq + H, P = p + H;
+                int64_t K = k + r, J = j + qq, I = i + p;
+                int64_t c = l.L(K, J, I);
+                double u = ucat[c].x, v = ucat[c].y, ww = ucat[c].z;
+                double U = u * csi[c].x + v * csi[c].y + ww * csi[c].z;
+                double V = u * eta[c].x + v * eta[c].y + ww * eta[c].z;
+                double W = u * zet[c].x + v * zet[c].y + ww * zet[c].z;
+                double du_dx = Ax[c].x, du_dy = Ax[c].y, du_dz = Ax[c].z;
+                double dv_dx = Ay[c].x, dv_dy = Ay[c].y, dv_dz = Ay[c].z;
+                double dw_dx = Az[c].x, dw_dy = Az[c].y, dw_dz = Az[c].z;
+                double Sxx = 0.5 * (du_dx + du_dx), Sxy = 0.5 * (du_dy + dv_dx), Sxz = 0.5 * (du_dz + dw_dx);
+                double Syx = Sxy, Syy = 0.5 * (dv_dy + dv_dy), Syz = 0.5 * (dv_dz + dw_dy);
+                double Szx = Sxz, Szy = Syz, Szz = 0.5 * (dw_dz + dw_dz);
+                double S = Sabs[c];
+                double vals[40] = {u, v, ww, U, V, W,
+                                   U * u, U * v, U * ww, V * u, V * v, V * ww, W * u, W * v, W * ww,
+                                   Sxx, Sxy, Sxz, Syx, Syy, Syz, Szx, Szy, Szz,
+                                   Sxx * S, Sxy * S, Sxz * S, Syx * S, Syy * S, Syz * S, Szx * S, Szy * S, Szz * S,
+                                   S, 0, 0, 0, 0, 0, 0};
+                for (int n = 0; n < 34; n++) q[n][R][Q][P] = vals[n];
+            }
+    double sum_vol = 0;
+    for (int p = -H; p <= H; p++)
+        for (int qq = -H; qq <= H; qq++)
+            for (int r = -H; r <= H; r++) {
+                int R = r + H, Q = qq + H, P = p + H;
+                int64_t K = k + r, J = j + qq, I = i + p;
+                // nvert = meshTag = 0: fluid and calculated everywhere; exclude DMDA boundary ghosts
+                bool inside = (NP == 3) ? (I != 0 && I != l.mx - 1 && J != 0 && J != l.my - 1 && K != 0 && K != l.mz - 1)
+                                        : (I > 0 && I < l.mx - 1 && J > 0 && J < l.my - 1 && K > 0 && K < l.mz - 1);
+                if (inside) {
+                    double coef = (NP == 3) ? bin3[R] * bin3[Q] * bin3[P] : bin5[R] * bin5[Q] * bin5[P];
+                    double scale = (NP == 3) ? 8.0 : 64.0;
+                    sum_vol += 1. / aj[l.L(K, J, I)] * scale * coef;
+                    w[R][Q][P] = 1;
+                } else {
+                    w[R][Q][P] = 0;
+                }
+            }
+    double filter = std::pow(1. / aj[l.L(k, j, i)], 1. / 3.);
+    double test_filter = std::pow(sum_vol, 1. / 3.);
+    auto F = [&](int n) {
+        if (NP == 3) return simpson3(reinterpret_cast<const double(*)[3][3]>(q[n]), reinterpret_cast<const double(*)[3][3]>(w));
+        return simpson5(reinterpret_cast<const double(*)[5][5]>(q[n]), reinterpret_cast<const double(*)[5][5]>(w));
+    };
+    double _u = 0, _v = 0, _w = 0, _U = 0, _V = 0, _W = 0;
+    _U = F(3); _V = F(4); _W = F(5);
+    _u = F(0); _v = F(1); _w = F(2);
+    double L[3][3];
+    L[0][0] = F(6) - _U * _u; L[0][1] = F(7) - _U * _v; L[0][2] = F(8) - _U * _w;
+    L[1][0] = F(9) - _V * _u; L[1][1] = F(10) - _V * _v; L[1][2] = F(11) - _V * _w;
+    L[2][0] = F(12) - _W * _u; L[2][1] = F(13) - _W * _v; L[2][2] = F(14) - _W * _w;
+    double Sh[3][3], SSh[3][3];
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) Sh[a][b] = F(15 + 3 * a + b);
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) SSh[a][b] = F(24 + 3 * a + b);
+    int64_t c0 = l.L(k, j, i);
+    germano(L, Sh, SSh, filter, test_filter, csi[c0], eta[c0], zet[c0], num, den);
+}
+
+#ifdef _OPENMP
+#define OMP_FOR _Pragma("omp parallel for schedule(static)")
+#else
+#define OMP_FOR
+#endif
+
+}  // namespace
+
+extern "C" {
+
+const char *oracle_last_error(void) { return g_err.c_str(); }
+
+void oracle_set_threads(int n) {
+#ifdef _OPENMP
+    omp_set_num_threads(n > 0 ? n : 1);
+#else
+    (void)n;
+#endif
+}
+
+int oracle_max_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+// ================================================================================================
+// FormU (src/solvers/ueqn_solvers.c:6-1083)
+// ================================================================================================
+int oracle_formu(const tosca_dmda_info *info, const tosca_mesh *m, const tosca_params *p,
+                 const double *ucont_, const double *ucat_, const double *nut, double *rhs_,
+                 double scale, int formMode) {
+    Lay l;
+    int rc = check_layout(info, p, l);
+    if (rc) return rc;
+    if (formMode < 0 || formMode > 2) { g_err = "bad formMode"; return TOSCA_ERR_ARG; }
+    if (p->div_scheme != TOSCA_DIV_CENTRAL && p->div_scheme != TOSCA_DIV_QUICK) { g_err = "scheme"; return TOSCA_ERR_UNSUPPORTED; }
+    const int64_t mx = l.mx, my = l.my, mz = l.mz, n = l.nloc();
+    const V3 *ucont = (const V3 *)ucont_, *ucat = (const V3 *)ucat_;
+    const V3 *csi = (const V3 *)m->csi, *eta = (const V3 *)m->eta, *zet = (const V3 *)m->zet;
+    const V3 *icsi = (const V3 *)m->icsi, *ieta = (const V3 *)m->ieta, *izet = (const V3 *)m->izet;
+    const V3 *jcsi = (const V3 *)m->jcsi, *jeta = (const V3 *)m->jeta, *jzet = (const V3 *)m->jzet;
+    const V3 *kcsi = (const V3 *)m->kcsi, *keta = (const V3 *)m->keta, *kzet = (const V3 *)m->kzet;
+    const double *iaj = m->iaj, *jaj = m->jaj, *kaj = m->kaj;
+    V3 *rhs = (V3 *)rhs_;
+    const bool les = p->les_active != 0;
+    const double nu = p->nu;
+    const bool quick = p->div_scheme == TOSCA_DIV_QUICK;
+
+    std::vector<V3> fp(n, V3{0, 0, 0});
+    std::vector<V3> div1(n, V3{0, 0, 0}), div2(n, V3{0, 0, 0}), div3(n, V3{0, 0, 0});
+    std::vector<V3> visc1(n, V3{0, 0, 0}), visc2(n, V3{0, 0, 0}), visc3(n, V3{0, 0, 0});
+
+    const double *tauI = p->tau_wall_i, *tauJ = p->tau_wall_j;
+    const int64_t tIx = 0, tIy = mz * my, tIz = 2 * mz * my;   // plane offsets [x|y|z][zm][ym]
+    const int64_t tJx = 0, tJy = mz * mx, tJz = 2 * mz * mx;   // plane offsets [x|y|z][zm][xm]
+
+    // viscous flux of one face (shared expression of :196-215, :410-429, :624-643, :386-392 ...)
+    auto visc_flux = [](const Grad &g, V3 c, V3 e, V3 z, int dir, double ajc, double nuEff, V3 &out) {
+        double csi0 = c.x, csi1 = c.y, csi2 = c.z, eta0 = e.x, eta1 = e.y, eta2 = e.z, zet0 = z.x, zet1 = z.y, zet2 = z.z;
+        double g11, g21, g31;
+        if (dir == 0) { g11 = csi0 * csi0 + csi1 * csi1 + csi2 * csi2; g21 = eta0 * csi0 + eta1 * csi1 + eta2 * csi2; g31 = zet0 * csi0 + zet1 * csi1 + zet2 * csi2; }
+        else if (dir == 1) { g11 = csi0 * eta0 + csi1 * eta1 + csi2 * eta2; g21 = eta0 * eta0 + eta1 * eta1 + eta2 * eta2; g31 = zet0 * eta0 + zet1 * eta1 + zet2 * eta2; }
+        else { g11 = csi0 * zet0 + csi1 * zet1 + csi2 * zet2; g21 = eta0 * zet0 + eta1 * zet1 + eta2 * zet2; g31 = zet0 * zet0 + zet1 * zet1 + zet2 * zet2; }
+        double r11 = g.dudc * csi0 + g.dude * eta0 + g.dudz * zet0;
+        double r21 = g.dvdc * csi0 + g.dvde * eta0 + g.dvdz * zet0;
+        double r31 = g.dwdc * csi0 + g.dwde * eta0 + g.dwdz * zet0;
+        double r12 = g.dudc * csi1 + g.dude * eta1 + g.dudz * zet1;
+        double r22 = g.dvdc * csi1 + g.dvde * eta1 + g.dvdz * zet1;
+        double r32 = g.dwdc * csi1 + g.dwde * eta1 + g.dwdz * zet1;
+        double r13 = g.dudc * csi2 + g.dude * eta2 + g.dudz * zet2;
+        double r23 = g.dvdc * csi2 + g.dvde * eta2 + g.dvdz * zet2;
+        double r33 = g.dwdc * csi2 + g.dwde * eta2 + g.dwdz * zet2;
+        double n0, n1, n2;   // the face's own normal: csi (i), eta (j) or zet (k)
+        if (dir == 0) { n0 = csi0; n1 = csi1; n2 = csi2; }
+        else if (dir == 1) { n0 = eta0; n1 = eta1; n2 = eta2; }
+        else { n0 = zet0; n1 = zet1; n2 = zet2; }
+        out.x += (g11 * g.dudc + g21 * g.dude + g31 * g.dudz + r11 * n0 + r21 * n1 + r31 * n2) * ajc * (nuEff);
+        out.y += (g11 * g.dvdc + g21 * g.dvde + g31 * g.dvdz + r12 * n0 + r22 * n1 + r32 * n2) * ajc * (nuEff);
+        out.z += (g11 * g.dwdc + g21 * g.dwde + g31 * g.dwdz + r13 * n0 + r23 * n1 + r33 * n2) * ajc * (nuEff);
+    };
+
+    // ---- face pass (:173-808) ---------------------------------------------------------------
+    OMP_FOR
+    for (int64_t k = 0; k < mz; k++)
+        for (int64_t j = 0; j < my; j++)
+            for (int64_t i = 0; i < mx; i++) {
+                if (i == mx - 1 || j == my - 1 || k == mz - 1) continue;
+                const int64_t c = l.L(k, j, i);
+                // i faces
+                if (j != 0 && k != 0) {
+                    Grad g{};
+                    double ajc = 0.0;
+                    if (formMode != 1) { ajc = iaj[c]; g = du_i(l, ucat, i, j, k); }
+                    if (formMode != 2) {
+                        if (quick) {
+                            int64_t iL, iR; face2cell4(l.per[0], i, mx, &iL, &iR);
+                            div1[c] = nScale(-ucont[c].x, quadraticUpwindVec(ucat[l.L(k, j, iL)], ucat[c], ucat[l.L(k, j, i + 1)], ucat[l.L(k, j, iR)], ucont[c].x));
+                        } else {
+                            div1[c] = nScale(-ucont[c].x, centralVec(ucat[c], ucat[l.L(k, j, i + 1)]));
+                        }
+                    }
+                    if (formMode != 1) {
+                        double nuEff;
+                        if (les) {
+                            double nt = 0.5 * (nut[c] + nut[l.L(k, j, i + 1)]);
+                            if ((p->bc_u[0] == TOSCA_BCU_WALLFUNCTION && i == 0) || (p->bc_u[1] == TOSCA_BCU_WALLFUNCTION && i == mx - 2)) {
+                                int64_t t = k * my + j;
+                                visc1[c].x = -tauI[tIx + t]; visc1[c].y = -tauI[tIy + t]; visc1[c].z = -tauI[tIz + t];
+                                nuEff = nu;
+                            } else if ((p->bc_u[0] == TOSCA_BCU_SLIP && i == 0) || (p->bc_u[1] == TOSCA_BCU_SLIP && i == mx - 2)) {
+                                nuEff = 0.0;
+                            } else {
+                                nuEff = nu + nt;
+                            }
+                        } else {
+                            nuEff = nu;
+                        }
+                        visc_flux(g, icsi[c], ieta[c], izet[c], 0, ajc, nuEff, visc1[c]);
+                    }
+                }
+                // j faces
+                if (i != 0 && k != 0) {
+                    Grad g{};
+                    double ajc = 0.0;
+                    if (formMode != 1) { ajc = jaj[c]; g = du_j(l, ucat, i, j, k); }
+                    if (formMode != 2) {
+                        if (quick) {
+                            int64_t jL, jR; face2cell4(l.per[1], j, my, &jL, &jR);
+                            div2[c] = nScale(-ucont[c].y, quadraticUpwindVec(ucat[l.L(k, jL, i)], ucat[c], ucat[l.L(k, j + 1, i)], ucat[l.L(k, jR, i)], ucont[c].y));
+                        } else {
+                            div2[c] = nScale(-ucont[c].y, centralVec(ucat[c], ucat[l.L(k, j + 1, i)]));
+                        }
+                    }
+                    if (formMode != 1) {
+                        double nuEff;
+                        if (les) {
+                            double nt = 0.5 * (nut[c] + nut[l.L(k, j + 1, i)]);
+                            if ((p->bc_u[2] == TOSCA_BCU_WALLFUNCTION && j == 0) || (p->bc_u[3] == TOSCA_BCU_WALLFUNCTION && j == my - 2)) {
+                                int64_t t = k * mx + i;
+                                visc2[c].x = -tauJ[tJx + t]; visc2[c].y = -tauJ[tJy + t]; visc2[c].z = -tauJ[tJz + t];
+                                nuEff = nu;
+                            } else if ((p->bc_u[2] == TOSCA_BCU_SLIP && j == 0) || (p->bc_u[3] == TOSCA_BCU_SLIP && j == my - 2)) {
+                                nuEff = 0.0;
+                            } else {
+                                nuEff = nu + nt;
+                            }
+                        } else {
+                            nuEff = nu;
+                        }
+                        visc_flux(g, jcsi[c], jeta[c], jzet[c], 1, ajc, nuEff, visc2[c]);
+                    }
+                }
+                // k faces
+                if (i != 0 && j != 0) {
+                    Grad g{};
+                    double ajc = 0.0;
+                    if (formMode != 1) { ajc = kaj[c]; g = du_k(l, ucat, i, j, k); }
+                    if (formMode != 2) {
+                        if (quick) {
+                            int64_t kL, kR; face2cell4(l.per[2], k, mz, &kL, &kR);
+                            div3[c] = nScale(-ucont[c].z, quadraticUpwindVec(ucat[l.L(kL, j, i)], ucat[c], ucat[l.L(k + 1, j, i)], ucat[l.L(kR, j, i)], ucont[c].z));
+                        } else {
+                            div3[c] = nScale(-ucont[c].z, centralVec(ucat[c], ucat[l.L(k + 1, j, i)]));
+                        }
+                    }
+                    if (formMode != 1) {
+                        double nuEff;
+                        if (les) {
+                            double nt = 0.5 * (nut[c] + nut[l.L(k + 1, j, i)]);
+                            if ((p->bc_u[4] == TOSCA_BCU_SLIP && k == 0) || (p->bc_u[5] == TOSCA_BCU_SLIP && k == mz - 2)) nuEff = 0.0;
+                            else nuEff = nu + nt;
+                        } else {
+                            nuEff = nu;
+                        }
+                        visc_flux(g, kcsi[c], keta[c], kzet[c], 2, ajc, nuEff, visc3[c]);
+                    }
+                }
+            }
+
+    // ---- ghost scatter + periodic face resets (:829-857) -------------------------------------
+    V3 *arrs[6] = {div1.data(), div2.data(), div3.data(), visc1.data(), visc2.data(), visc3.data()};
+    for (auto *a : arrs) local_to_local(l, a);
+    for (auto *a : arrs) reset_face_periodic(l, a);
+
+    // ---- cell pass (:871-993) ----------------------------------------------------------------
+    const bool iN = !l.i_per() && !l.ii_per(), jN = !l.j_per() && !l.jj_per(), kN = !l.k_per() && !l.kk_per();
+    OMP_FOR
+    for (int64_t k = 1; k < mz - 1; k++)
+        for (int64_t j = 1; j < my - 1; j++)
+            for (int64_t i = 1; i < mx - 1; i++) {
+                const int64_t c = l.L(k, j, i), cim = l.L(k, j, i - 1), cjm = l.L(k, j - 1, i), ckm = l.L(k - 1, j, i);
+                V3 &f = fp[c];
+                if (formMode != 2) {
+                    f.x += (div1[c].x - div1[cim].x + div2[c].x - div2[cjm].x + div3[c].x - div3[ckm].x);
+                    f.y += (div1[c].y - div1[cim].y + div2[c].y - div2[cjm].y + div3[c].y - div3[ckm].y);
+                    f.z += (div1[c].z - div1[cim].z + div2[c].z - div2[cjm].z + div3[c].z - div3[ckm].z);
+                }
+                if (formMode != 1) {
+                    f.x += (visc1[c].x - visc1[cim].x + visc2[c].x - visc2[cjm].x + visc3[c].x - visc3[ckm].x);
+                    f.y += (visc1[c].y - visc1[cim].y + visc2[c].y - visc2[cjm].y + visc3[c].y - visc3[ckm].y);
+                    f.z += (visc1[c].z - visc1[cim].z + visc2[c].z - visc2[cjm].z + visc3[c].z - visc3[ckm].z);
+                }
+            }
+    // boundary-ghost copies of the same pass (:949-990), serial: they read finished interior cells
+    for (int64_t k = 1; k < mz - 1; k++)
+        for (int64_t j = 1; j < my - 1; j++)
+            for (int64_t i = 1; i < mx - 1; i++) {
+                const int64_t c = l.L(k, j, i);
+                if (i == 1 && iN) fp[l.L(k, j, i - 1)] = fp[c];
+                if (j == 1 && jN) fp[l.L(k, j - 1, i)] = fp[c];
+                if (k == 1 && kN) fp[l.L(k - 1, j, i)] = fp[c];
+                if (i == mx - 2 && iN) fp[l.L(k, j, i + 1)] = fp[c];
+                if (j == my - 2 && jN) fp[l.L(k, j + 1, i)] = fp[c];
+                if (k == mz - 2 && kN) fp[l.L(k + 1, j, i)] = fp[c];
+            }
+    local_to_local(l, fp.data());
+    reset_cell_periodic(l, fp.data());
+
+    // ---- face-dot pass (:1006-1035) ----------------------------------------------------------
+    OMP_FOR
+    for (int64_t k = 1; k < mz - 1; k++)
+        for (int64_t j = 1; j < my - 1; j++)
+            for (int64_t i = 1; i < mx - 1; i++) {
+                const int64_t c = l.L(k, j, i);
+                const int64_t ci = l.L(k, j, i + 1), cj = l.L(k, j + 1, i), ck = l.L(k + 1, j, i);
+                V3 &r = rhs[l.G(k, j, i)];
+                double v1, v2;
+                v1 = csi[c].x * fp[c].x + csi[c].y * fp[c].y + csi[c].z * fp[c].z;
+                v2 = csi[ci].x * fp[ci].x + csi[ci].y * fp[ci].y + csi[ci].z * fp[ci].z;
+                r.x += scale * iaj[c] * (0.5 * (v1 + v2));
+                v1 = eta[c].x * fp[c].x + eta[c].y * fp[c].y + eta[c].z * fp[c].z;
+                v2 = eta[cj].x * fp[cj].x + eta[cj].y * fp[cj].y + eta[cj].z * fp[cj].z;
+                r.y += scale * jaj[c] * (0.5 * (v1 + v2));
+                v1 = zet[c].x * fp[c].x + zet[c].y * fp[c].y + zet[c].z * fp[c].z;
+                v2 = zet[ck].x * fp[ck].x + zet[ck].y * fp[ck].y + zet[ck].z * fp[ck].z;
+                r.z += scale * kaj[c] * (0.5 * (v1 + v2));
+            }
+    return TOSCA_OK;
+}
+
+// ================================================================================================
+// UpdateCs (src/les.c:165-2115): SMAGORINSKY, DSM, DPASD
+// ================================================================================================
+int oracle_update_cs(const tosca_dmda_info *info, const tosca_mesh *m, const tosca_params *p,
+                     const double *ucat_, double *Cs) {
+    Lay l;
+    int rc = check_layout(info, p, l);
+    if (rc) return rc;
+    const int64_t mx = l.mx, my = l.my, mz = l.mz, n = l.nloc();
+    if (p->les_model == TOSCA_LES_SMAGORINSKY) {
+        // VecSet(lCs, std_cs) over the whole local vector, then a ghost scatter (:172-179)
+        for (int64_t c = 0; c < n; c++) Cs[c] = 0.0289;
+        return TOSCA_OK;
+    }
+    if (p->les_model != TOSCA_LES_DSM && p->les_model != TOSCA_LES_DPASD) {
+        g_err = "LES model not supported";
+        return TOSCA_ERR_UNSUPPORTED;
+    }
+    const V3 *ucat = (const V3 *)ucat_;
+    const V3 *csi = (const V3 *)m->csi, *eta = (const V3 *)m->eta, *zet = (const V3 *)m->zet;
+    const double *aj = m->aj;
+
+    std::vector<V3> Ax(n, V3{0, 0, 0}), Ay(n, V3{0, 0, 0}), Az(n, V3{0, 0, 0});
+    std::vector<double> Sabs(n, 0.0);
+
+    // 1 - strain rate (:277-374)
+    OMP_FOR
+    for (int64_t k = 1; k < mz - 1; k++)
+        for (int64_t j = 1; j < my - 1; j++)
+            for (int64_t i = 1; i < mx - 1; i++) {
+                const int64_t c = l.L(k, j, i);
+                Grad g = du_center(l, ucat, i, j, k);
+                Cart d = du_dxyz(csi[c], eta[c], zet[c], aj[c], g);
+                Sabs[c] = strain_mag(d);
+                Ax[c] = {d.du_dx, d.du_dy, d.du_dz};
+                Ay[c] = {d.dv_dx, d.dv_dy, d.dv_dz};
+                Az[c] = {d.dw_dx, d.dw_dy, d.dw_dz};
+            }
+    local_to_local(l, Ax.data()); local_to_local(l, Ay.data());
+    local_to_local(l, Az.data()); local_to_local(l, Sabs.data());
+    reset_cell_periodic(l, Ax.data()); reset_cell_periodic(l, Ay.data());
+    reset_cell_periodic(l, Az.data()); reset_cell_periodic(l, Sabs.data());
+
+    // 2 - first test filter -> LM, MM (:452-811)
+    std::vector<double> LM(n, 0.0), MM(n, 0.0);
+    OMP_FOR
+    for (int64_t k = 1; k < mz - 1; k++)
+        for (int64_t j = 1; j < my - 1; j++)
+            for (int64_t i = 1; i < mx - 1; i++) {
+                const int64_t c = l.L(k, j, i);
+                test_filter_cell<3>(l, m, ucat, Ax.data(), Ay.data(), Az.data(), Sabs.data(), i, j, k, &LM[c], &MM[c]);
+            }
+    local_to_local(l, LM.data()); local_to_local(l, MM.data());
+
+    std::vector<double> QN, NN;
+    if (p->les_model == TOSCA_LES_DPASD) {
+        // second test filter -> QN, NN (:812-1174)
+        QN.assign(n, 0.0); NN.assign(n, 0.0);
+        OMP_FOR
+        for (int64_t k = 1; k < mz - 1; k++)
+            for (int64_t j = 1; j < my - 1; j++)
+                for (int64_t i = 1; i < mx - 1; i++) {
+                    const int64_t c = l.L(k, j, i);
+                    test_filter_cell<5>(l, m, ucat, Ax.data(), Ay.data(), Az.data(), Sabs.data(), i, j, k, &QN[c], &NN[c]);
+                }
+        local_to_local(l, QN.data()); local_to_local(l, NN.data());
+    }
+    local_to_local(l, LM.data()); local_to_local(l, MM.data());
+    reset_cell_periodic(l, LM.data()); reset_cell_periodic(l, MM.data());
+
+    // 3 - Cs (:1779-1977)
+    OMP_FOR
+    for (int64_t k = 1; k < mz - 1; k++)
+        for (int64_t j = 1; j < my - 1; j++)
+            for (int64_t i = 1; i < mx - 1; i++) {
+                double C = 0.0;
+                if (p->les_model == TOSCA_LES_DSM) {
+                    double weight[3][3][3], LM0[3][3][3], MM0[3][3][3];
+                    for (int a = -1; a <= 1; a++)
+                        for (int b = -1; b <= 1; b++)
+                            for (int cc = -1; cc <= 1; cc++) {
+                                int R = cc + 1, Q = b + 1, P = a + 1;
+                                int64_t K = k + cc, J = j + b, I = i + a;
+                                weight[R][Q][P] = 1. / aj[l.L(K, J, I)];
+                                if (l.i_per()) { if (I == 0) I = mx - 2; else if (I == mx - 1) I = 1; }
+                                else if (l.ii_per()) { if (I == 0) I = -2; else if (I == mx - 1) I = mx + 1; }
+                                else if (I == 0 || I == mx - 1) weight[R][Q][P] = 0;
+                                if (l.j_per()) { if (J == 0) J = my - 2; else if (J == my - 1) J = 1; }
+                                else if (l.jj_per()) { if (J == 0) J = -2; else if (J == my - 1) J = my + 1; }
+                                else if (J == 0 || j == my - 1) weight[R][Q][P] = 0;   // lower-case j as in :1911
+                                if (l.k_per()) { if (K == 0) K = mz - 2; else if (K == mz - 1) K = 1; }
+                                else if (l.kk_per()) { if (K == 0) K = -2; else if (K == mz - 1) K = mz + 1; }
+                                else if (K == 0 || K == mz - 1) weight[R][Q][P] = 0;
+                                LM0[R][Q][P] = LM[l.L(K, J, I)];
+                                MM0[R][Q][P] = MM[l.L(K, J, I)];
+                            }
+                    double LM_avg = simpson3(LM0, weight), MM_avg = simpson3(MM0, weight);
+                    C = 0.5 * LM_avg / (MM_avg + 1e-10);
+                }
+                Cs[l.L(k, j, i)] = pmin(pmax(C, 0.0), p->max_cs);
+            }
+
+    if (p->les_model == TOSCA_LES_DPASD) {
+        // j-plane averages (:1979-2048); single rank, so MPI_Allreduce is the identity
+        std::vector<double> J_LM(my, 0.0), J_MM(my, 0.0), J_QN(my, 0.0), J_NN(my, 0.0);
+        std::vector<int64_t> count(my, 0);
+        for (int64_t k = 1; k < mz - 1; k++)
+            for (int64_t j = 1; j < my - 1; j++)
+                for (int64_t i = 1; i < mx - 1; i++) {
+                    const int64_t c = l.L(k, j, i);
+                    J_LM[j] += LM[c]; J_MM[j] += MM[c]; J_QN[j] += QN[c]; J_NN[j] += NN[c];
+                    count[j]++;
+                }
+        for (int64_t j = 0; j < my; j++)
+            if (count[j] > 0) {
+                J_LM[j] = J_LM[j] / count[j]; J_MM[j] = J_MM[j] / count[j];
+                J_QN[j] = J_QN[j] / count[j]; J_NN[j] = J_NN[j] / count[j];
+            }
+        for (int64_t k = 1; k < mz - 1; k++)
+            for (int64_t j = 1; j < my - 1; j++)
+                for (int64_t i = 1; i < mx - 1; i++) {
+                    double beta = (J_QN[j] * J_MM[j]) / (J_NN[j] * J_LM[j] + 1e-10);
+                    double v = 0.5 * (J_LM[j] / (J_MM[j] + 1e-10)) / beta;
+                    Cs[l.L(k, j, i)] = pmin(pmax(v, 0.0), p->max_cs);
+                }
+    }
+    local_to_local(l, Cs);
+    reset_cell_periodic(l, Cs);
+    return TOSCA_OK;
+}
+
+// ================================================================================================
+// UpdateNut (src/les.c:2119-2578) + UpdateNutBCs (src/boundary.c:2630-2929)
+// ================================================================================================
+int oracle_update_nut(const tosca_dmda_info *info, const tosca_mesh *m, const tosca_params *p,
+                      const double *ucat_, const double *Cs, double *nut, double *ksgs) {
+    Lay l;
+    int rc = check_layout(info, p, l);
+    if (rc) return rc;
+    if (p->les_model != TOSCA_LES_SMAGORINSKY && p->les_model != TOSCA_LES_DSM && p->les_model != TOSCA_LES_DPASD) {
+        g_err = "LES model not supported";
+        return TOSCA_ERR_UNSUPPORTED;
+    }
+    const int64_t mx = l.mx, my = l.my, mz = l.mz, n = l.nloc();
+    const V3 *ucat = (const V3 *)ucat_;
+    const V3 *csi = (const V3 *)m->csi, *eta = (const V3 *)m->eta, *zet = (const V3 *)m->zet;
+    const double *aj = m->aj;
+    std::vector<double> ksg_local;
+    double *ksg = ksgs;
+    if (!ksg) { ksg_local.assign(n, 0.0); ksg = ksg_local.data(); }
+    for (int64_t c = 0; c < n; c++) { nut[c] = 0.0; ksg[c] = 0.0; }
+    OMP_FOR
+    for (int64_t k = 1; k < mz - 1; k++)
+        for (int64_t j = 1; j < my - 1; j++)
+            for (int64_t i = 1; i < mx - 1; i++) {
+                const int64_t c = l.L(k, j, i);
+                Grad g = du_center(l, ucat, i, j, k);
+                Cart d = du_dxyz(csi[c], eta[c], zet[c], aj[c], g);
+                double S = strain_mag(d);
+                double filter = std::pow(1. / aj[c], 1. / 3.);
+                nut[c] = Cs[c] * std::pow(filter, 2.0) * S;
+                double Ck = 0.094;
+                double V = 1.0 / aj[c];
+                double A_x = nMag(zet[c]), A_y = nMag(csi[c]), A_z = nMag(eta[c]);
+                double delta_x = V / A_x, delta_y = V / A_y, delta_z = V / A_z;
+                double invDelta2Sum = 1.0 / (delta_x * delta_x) + 1.0 / (delta_y * delta_y) + 1.0 / (delta_z * delta_z);
+                double delta2_harm = 3.0 / invDelta2Sum;
+                ksg[c] = std::pow(nut[c], 2.0) / (std::pow(Ck, 2.0) * delta2_harm);
+            }
+    local_to_local(l, nut);
+    local_to_local(l, ksg);
+    // UpdateNutBCs (:2812-2917), face-adjacent ghosts of the interior cells
+    const int *bc = p->bc_nut;
+    const double *val = p->nut_value;
+    for (int64_t k = 1; k < mz - 1; k++)
+        for (int64_t j = 1; j < my - 1; j++)
+            for (int64_t i = 1; i < mx - 1; i++) {
+                const int64_t c = l.L(k, j, i);
+                if (bc[0] == TOSCA_BCNUT_ZEROGRADIENT && i == 1) nut[l.L(k, j, i - 1)] = nut[c];
+                if (bc[1] == TOSCA_BCNUT_ZEROGRADIENT && i == mx - 2) nut[l.L(k, j, i + 1)] = nut[c];
+                if (bc[2] == TOSCA_BCNUT_ZEROGRADIENT && j == 1) nut[l.L(k, j - 1, i)] = nut[c];
+                if (bc[3] == TOSCA_BCNUT_ZEROGRADIENT && j == my - 2) nut[l.L(k, j + 1, i)] = nut[c];
+                if (bc[4] == TOSCA_BCNUT_ZEROGRADIENT && k == 1) nut[l.L(k - 1, j, i)] = nut[c];
+                if (bc[5] == TOSCA_BCNUT_ZEROGRADIENT && k == mz - 2) nut[l.L(k + 1, j, i)] = nut[c];
+                if (bc[0] == TOSCA_BCNUT_FIXEDVALUE && i == 1) nut[l.L(k, j, i - 1)] = val[0];
+                if (bc[1] == TOSCA_BCNUT_FIXEDVALUE && i == mx - 2) nut[l.L(k, j, i + 1)] = val[1];
+                if (bc[2] == TOSCA_BCNUT_FIXEDVALUE && j == 1) nut[l.L(k, j - 1, i)] = val[2];
+                if (bc[3] == TOSCA_BCNUT_FIXEDVALUE && j == my - 2) nut[l.L(k, j + 1, i)] = val[3];
+                if (bc[4] == TOSCA_BCNUT_FIXEDVALUE && k == 1) nut[l.L(k - 1, j, i)] = val[4];
+                if (bc[5] == TOSCA_BCNUT_FIXEDVALUE && k == mz - 2) nut[l.L(k + 1, j, i)] = val[5];
+                if (bc[0] == TOSCA_BCNUT_PERIODIC && i == 1) {
+                    if (l.i_per()) nut[l.L(k, j, i - 1)] = nut[l.L(k, j, mx - 2)];
+                    else if (l.ii_per()) nut[l.L(k, j, i - 1)] = nut[l.L(k, j, -2)];
+                }
+                if (bc[1] == TOSCA_BCNUT_PERIODIC && i == mx - 2) {
+                    if (l.i_per()) nut[l.L(k, j, i + 1)] = nut[l.L(k, j, 1)];
+                    else if (l.ii_per()) nut[l.L(k, j, i + 1)] = nut[l.L(k, j, mx + 1)];
+                }
+                if (bc[2] == TOSCA_BCNUT_PERIODIC && j == 1) {
+                    if (l.j_per()) nut[l.L(k, j - 1, i)] = nut[l.L(k, my - 2, i)];
+                    else if (l.jj_per()) nut[l.L(k, j - 1, i)] = nut[l.L(k, -2, i)];
+                }
+                if (bc[3] == TOSCA_BCNUT_PERIODIC && j == my - 2) {
+                    if (l.j_per()) nut[l.L(k, j + 1, i)] = nut[l.L(k, 1, i)];
+                    else if (l.jj_per()) nut[l.L(k, j + 1, i)] = nut[l.L(k, my + 1, i)];
+                }
+                if (bc[4] == TOSCA_BCNUT_PERIODIC && k == 1) {
+                    if (l.k_per()) nut[l.L(k - 1, j, i)] = nut[l.L(mz - 2, j, i)];
+                    else if (l.kk_per()) nut[l.L(k - 1, j, i)] = nut[l.L(-2, j, i)];
+                }
+                if (bc[5] == TOSCA_BCNUT_PERIODIC && k == mz - 2) {
+                    if (l.k_per()) nut[l.L(k + 1, j, i)] = nut[l.L(1, j, i)];
+                    else if (l.kk_per()) nut[l.L(k + 1, j, i)] = nut[l.L(mz + 1, j, i)];
+                }
+            }
+    local_to_local(l, nut);
+    return TOSCA_OK;
+}
+
+// ================================================================================================
+// Buoyancy (src/sources/ueqn_sources.c:2726-2880)
+// ================================================================================================
+int oracle_buoyancy(const tosca_dmda_info *info, const tosca_mesh *m, const tosca_params *p,
+                    const double *tmprt, double scale, double *btheta_, double *gcont_) {
+    Lay l;
+    int rc = check_layout(info, p, l);
+    if (rc) return rc;
+    const int64_t mx = l.mx, my = l.my, mz = l.mz;
+    const V3 *icsi = (const V3 *)m->icsi, *jeta = (const V3 *)m->jeta, *kzet = (const V3 *)m->kzet;
+    V3 *btheta = (V3 *)btheta_;
+    std::vector<V3> gloc;
+    V3 *gcont = (V3 *)gcont_;
+    if (!gcont) { gloc.assign(mx * my * mz, V3{0, 0, 0}); gcont = gloc.data(); }
+    const V3 gravity{0, 0, -9.81};
+    const double tRef = p->t_ref;
+    for (int64_t k = 0; k < mz - 1; k++)
+        for (int64_t j = 0; j < my - 1; j++)
+            for (int64_t i = 0; i < mx - 1; i++) {
+                const int64_t c = l.L(k, j, i), g = l.G(k, j, i);
+                if (j > 0 && k > 0) gcont[g].x = (gravity.x * icsi[c].x + gravity.y * icsi[c].y + gravity.z * icsi[c].z);
+                if (i > 0 && k > 0) gcont[g].y = (gravity.x * jeta[c].x + gravity.y * jeta[c].y + gravity.z * jeta[c].z);
+                if (i > 0 && j > 0) gcont[g].z = (gravity.x * kzet[c].x + gravity.y * kzet[c].y + gravity.z * kzet[c].z);
+            }
+    for (int64_t c = 0; c < mx * my * mz; c++) btheta[c] = V3{0, 0, 0};
+    for (int64_t k = 1; k < mz - 1; k++)
+        for (int64_t j = 1; j < my - 1; j++)
+            for (int64_t i = 1; i < mx - 1; i++) {
+                const int64_t c = l.L(k, j, i), g = l.G(k, j, i);
+                double tempI = 0.5 * (tmprt[c] + tmprt[l.L(k, j, i + 1)]);
+                double tempJ = 0.5 * (tmprt[c] + tmprt[l.L(k, j + 1, i)]);
+                double tempK = 0.5 * (tmprt[c] + tmprt[l.L(k + 1, j, i)]);
+                btheta[g].x += scale * (gcont[g].x * (tRef - tempI) / tRef);
+                btheta[g].y += scale * (gcont[g].y * (tRef - tempJ) / tRef);
+                btheta[g].z += scale * (gcont[g].z * (tRef - tempK) / tRef);
+            }
+    return TOSCA_OK;
+}
+
+// One evaluation of the benchmark's unit of work, in the reference's call order:
+// UpdateCs -> UpdateNut -> Buoyancy -> FormU(formMode 0) (src/main.c:111-112,181,185).
+int oracle_rhs_les_step(const tosca_dmda_info *info, const tosca_mesh *m, const tosca_params *p,
+                        const double *ucont, const double *ucat, const double *theta,
+                        double *rhs, double scale, double *cs, double *nut, double *btheta,
+                        double buoy_scale) {
+    int rc = oracle_update_cs(info, m, p, ucat, cs);
+    if (rc) return rc;
+    rc = oracle_update_nut(info, m, p, ucat, cs, nut, nullptr);
+    if (rc) return rc;
+    rc = oracle_buoyancy(info, m, p, theta, buoy_scale, btheta, nullptr);
+    if (rc) return rc;
+    return oracle_formu(info, m, p, ucont, ucat, nut, rhs, scale, 0);
+}
+
+}  // extern "C"

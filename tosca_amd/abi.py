@@ -1,0 +1,125 @@
+"""ctypes view of the C ABI declared in include/tosca_rhs.h.
+
+The structures here mirror the header field for field; the loader binds libtosca_rhs.so built
+in-tree by tosca_amd/build.py and fails loudly when it is missing (there is no CPU fallback).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+from typing import Dict, Mapping, Optional
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "libtosca_rhs.so"
+
+TOSCA_OK, TOSCA_ERR_ARG, TOSCA_ERR_UNSUPPORTED, TOSCA_ERR_LAYOUT, TOSCA_ERR_DEVICE, TOSCA_ERR_MESH = range(6)
+ERROR_NAMES = {0: "OK", 1: "ERR_ARG", 2: "ERR_UNSUPPORTED", 3: "ERR_LAYOUT", 4: "ERR_DEVICE", 5: "ERR_MESH"}
+
+# every function the header declares, with its ctypes signature
+P = C.c_void_p
+I = C.c_int
+D = C.c_double
+
+
+class DmdaInfo(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in
+                ("xs", "ys", "zs", "xm", "ym", "zm", "gxs", "gys", "gzs", "gxm", "gym", "gzm", "mx", "my", "mz")]
+
+
+MESH_FIELDS = ("csi", "eta", "zet", "icsi", "ieta", "izet", "jcsi", "jeta", "jzet", "kcsi", "keta", "kzet",
+               "aj", "iaj", "jaj", "kaj", "nvert", "meshtag")
+
+
+class Mesh(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in MESH_FIELDS]
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("periodic", C.c_int * 3),
+        ("bc_u", C.c_int * 6),
+        ("bc_nut", C.c_int * 6),
+        ("nut_value", C.c_double * 6),
+        ("div_scheme", C.c_int),
+        ("les_active", C.c_int),
+        ("les_model", C.c_int),
+        ("ibm_active", C.c_int),
+        ("overset_active", C.c_int),
+        ("nu", C.c_double),
+        ("max_cs", C.c_double),
+        ("t_ref", C.c_double),
+        ("tau_wall_i", C.c_void_p),
+        ("tau_wall_j", C.c_void_p),
+    ]
+
+
+SIGNATURES = {
+    "tosca_version": (C.c_char_p, []),
+    "tosca_last_error": (C.c_char_p, [P]),
+    "tosca_ctx_create": (I, [C.POINTER(C.c_void_p), C.POINTER(DmdaInfo), I]),
+    "tosca_ctx_destroy": (I, [P]),
+    "tosca_ctx_set_stream": (I, [P, P]),
+    "tosca_ctx_synchronize": (I, [P]),
+    "tosca_mesh_check": (I, [P, C.POINTER(Mesh)]),
+    "tosca_formu": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P, P, D, I]),
+    "tosca_update_cs": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P]),
+    "tosca_update_nut": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P, P]),
+    "tosca_buoyancy": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, D, P, P]),
+    "tosca_rhs_les_step": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P, P, D, P, P, P, D]),
+}
+
+_lib: Optional[C.CDLL] = None
+
+
+def load(path: Optional[Path] = None) -> C.CDLL:
+    """Loads libtosca_rhs.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(f"{p} is missing: build the gfx950 library first (python -m tosca_amd.build)")
+    lib = C.CDLL(str(p))
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+class ToscaError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERROR_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+def make_info(d: Mapping[str, int]) -> DmdaInfo:
+    return DmdaInfo(**{k: int(v) for k, v in d.items()})
+
+
+def make_params(d: Mapping) -> Params:
+    p = Params()
+    p.periodic[:] = list(d["periodic"])
+    p.bc_u[:] = list(d["bc_u"])
+    p.bc_nut[:] = list(d["bc_nut"])
+    p.nut_value[:] = [float(v) for v in d.get("nut_value", (0.0,) * 6)]
+    p.div_scheme = int(d["div_scheme"])
+    p.les_active = int(d["les_active"])
+    p.les_model = int(d["les_model"])
+    p.ibm_active = int(d.get("ibm_active", 0))
+    p.overset_active = int(d.get("overset_active", 0))
+    p.nu = float(d["nu"])
+    p.max_cs = float(d.get("max_cs", 0.5))
+    p.t_ref = float(d["t_ref"])
+    p.tau_wall_i = d.get("tau_wall_i") or None
+    p.tau_wall_j = d.get("tau_wall_j") or None
+    return p
+
+
+def make_mesh(ptrs: Mapping[str, int]) -> Mesh:
+    m = Mesh()
+    for n in MESH_FIELDS:
+        setattr(m, n, ptrs.get(n) or None)
+    return m

@@ -1,0 +1,678 @@
+"""Synthetic TOSCA cases: structured meshes, metrics, initial fields and ghost cells.
+
+Everything here prepares INPUTS for the right-hand-side library in the exact storage the
+reference solver hands to FormU / UpdateCs / UpdateNut / Buoyancy: PETSc DMDA local vectors with
+ghost width 3, [k][j][i] with i fastest, vectors AoS {x,y,z}, and the axis convention of TOSCA
+(k <-> x streamwise, j <-> z vertical, i <-> y spanwise; docs/.../spatial_mesh.rst:215).
+
+It restates, in vectorised torch (so the same code runs on the CPU for tests and on the GPU for
+large benchmark cases):
+  - SetMeshMetrics (src/mesh.c:630-1976): cell centres, cell/face contravariant bases and
+    Jacobians, boundary mirroring and periodic ghost copies;
+  - SetTaylorGreenFieldU (src/initialField.c:1061-1204) and the face interpolation that builds
+    ucont from ucat;
+  - SetABLInitialFlowT (src/initialField.c:1767-1866), the Rampanelli-Zardi potential temperature;
+  - the subset of UpdateCartesianBCs (src/boundary.c:824-2050) the cases use: periodic, noSlip,
+    slip, fixedValue, zeroGradient;
+  - PETSc's single-rank DMLocalToLocal / DMGlobalToLocal ghost semantics (type-2 periodic axes are
+    DM_BOUNDARY_PERIODIC and wrap; every other axis is DM_BOUNDARY_GHOSTED and is not filled).
+
+None of this is on the timed path.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import Dict, Optional, Sequence, Tuple
+
+import numpy as np
+import torch
+
+G = 3  # DMDA ghost width (src/mesh.c:42)
+
+PERIODIC_NONE, PERIODIC_COPY, PERIODIC_WRAP = 0, 1, 2
+DIV_CENTRAL, DIV_QUICK = 0, 1
+BCU_GENERIC, BCU_SLIP, BCU_WALLFUNCTION, BCU_PERIODIC = 0, 1, 2, 3
+BCNUT_NONE, BCNUT_ZEROGRADIENT, BCNUT_FIXEDVALUE, BCNUT_PERIODIC = 0, 1, 2, 3
+LES_SMAGORINSKY, LES_DSM, LES_DPASD = 0, 2, 5
+
+MESH_VECTORS = ("csi", "eta", "zet", "icsi", "ieta", "izet", "jcsi", "jeta", "jzet",
+                "kcsi", "keta", "kzet")
+MESH_SCALARS = ("aj", "iaj", "jaj", "kaj", "nvert", "meshtag")
+
+
+# ------------------------------------------------------------------------------------------------
+# layout helpers
+# ------------------------------------------------------------------------------------------------
+@dataclass
+class Layout:
+    mx: int
+    my: int
+    mz: int
+
+    @property
+    def gshape(self) -> Tuple[int, int, int]:
+        return (self.mz + 2 * G, self.my + 2 * G, self.mx + 2 * G)
+
+    def info(self) -> Dict[str, int]:
+        return dict(xs=0, ys=0, zs=0, xm=self.mx, ym=self.my, zm=self.mz,
+                    gxs=-G, gys=-G, gzs=-G, gxm=self.mx + 2 * G, gym=self.my + 2 * G,
+                    gzm=self.mz + 2 * G, mx=self.mx, my=self.my, mz=self.mz)
+
+    @property
+    def ncells(self) -> int:
+        return (self.mx - 2) * (self.my - 2) * (self.mz - 2)
+
+
+def owned(t: torch.Tensor, lay: Layout) -> torch.Tensor:
+    """View of the owned range [0,m) of a local array."""
+    return t[G:G + lay.mz, G:G + lay.my, G:G + lay.mx]
+
+
+def interior(t: torch.Tensor, lay: Layout) -> torch.Tensor:
+    """View of the interior cells [1,m-1) of a local array."""
+    return t[G + 1:G + lay.mz - 1, G + 1:G + lay.my - 1, G + 1:G + lay.mx - 1]
+
+
+def local_to_local(t: torch.Tensor, lay: Layout, periodic: Sequence[int]) -> torch.Tensor:
+    """PETSc single-rank DMLocalToLocal: wrap the ghosts of DM_BOUNDARY_PERIODIC (type-2) axes.
+
+    A ghost point is filled iff every axis on which it lies outside [0,m) is periodic; the
+    axis-by-axis sweep below restricts each copy to the owned range of the non-periodic axes,
+    which gives exactly that set (edges and corners included)."""
+    m = {0: lay.mz, 1: lay.my, 2: lay.mx}                 # tensor dim -> size
+    per = {0: periodic[2], 1: periodic[1], 2: periodic[0]}  # tensor dim -> periodic type
+    full = {d: slice(0, m[d] + 2 * G) if per[d] == PERIODIC_WRAP else slice(G, G + m[d])
+            for d in range(3)}
+    for d in range(3):
+        if per[d] != PERIODIC_WRAP:
+            continue
+        n = m[d]
+        for dst, src in ((slice(0, G), slice(n, n + G)), (slice(n + G, n + 2 * G), slice(G, 2 * G))):
+            idx_d = [full[0], full[1], full[2]]
+            idx_s = [full[0], full[1], full[2]]
+            idx_d[d] = dst
+            idx_s[d] = src
+            t[tuple(idx_d)] = t[tuple(idx_s)]
+    return t
+
+
+def _lp(t, k, j, i):
+    """Local-array element access helper with DMDA indices (k, j, i may be slices or ints)."""
+    def s(a):
+        if isinstance(a, slice):
+            return slice(None if a.start is None else a.start + G, None if a.stop is None else a.stop + G)
+        return a + G
+    return t[s(k), s(j), s(i)]
+
+
+# ------------------------------------------------------------------------------------------------
+# nodes
+# ------------------------------------------------------------------------------------------------
+def cartesian_nodes(x: np.ndarray, y: np.ndarray, z: np.ndarray, device, dtype=torch.float64):
+    """Node coordinates of a 'cartesian' mesh file (src/mesh.c:58-96, 227-270):
+    x along k, y along i, z along j.  Returns (npx, npz, npy, 3) = node[k][j][i]."""
+    X = torch.as_tensor(x, dtype=dtype, device=device)
+    Y = torch.as_tensor(y, dtype=dtype, device=device)
+    Z = torch.as_tensor(z, dtype=dtype, device=device)
+    K, J, I = len(x), len(z), len(y)
+    n = torch.empty((K, J, I, 3), dtype=dtype, device=device)
+    n[..., 0] = X.view(K, 1, 1)
+    n[..., 1] = Y.view(1, 1, I)
+    n[..., 2] = Z.view(1, J, 1)
+    return n
+
+
+def geometric_axis(length: float, n_cells: int, ratio: float) -> np.ndarray:
+    """Nodes 0..length with geometric cell growth `ratio` (uniform if ratio == 1)."""
+    if abs(ratio - 1.0) < 1e-15:
+        return np.linspace(0.0, length, n_cells + 1)
+    d0 = length * (ratio - 1.0) / (ratio ** n_cells - 1.0)
+    widths = d0 * ratio ** np.arange(n_cells)
+    nodes = np.concatenate([[0.0], np.cumsum(widths)])
+    nodes[-1] = length
+    return nodes
+
+
+def hill_nodes(lx, ly, lz, nx, ny, nz, ratio, h0, half_width, device, dtype=torch.float64):
+    """Terrain-following curvilinear nodes over a cosine hill:
+    z = h(x,y) + (lz - h) * s(zeta) with geometric vertical spacing s, h = h0 cos^2(pi r / (4 hw))
+    for r <= 2 hw, centred in the domain.  Returns node[k][j][i] (k: x, j: z, i: y)."""
+    x = torch.as_tensor(np.linspace(0.0, lx, nx + 1), dtype=dtype, device=device)
+    y = torch.as_tensor(np.linspace(0.0, ly, ny + 1), dtype=dtype, device=device)
+    s = torch.as_tensor(geometric_axis(1.0, nz, ratio), dtype=dtype, device=device)
+    X = x.view(-1, 1, 1)
+    Y = y.view(1, 1, -1)
+    r = torch.sqrt((X - lx / 2) ** 2 + (Y - ly / 2) ** 2)
+    h = torch.where(r <= 2 * half_width, h0 * torch.cos(math.pi * r / (4 * half_width)) ** 2,
+                    torch.zeros_like(r))
+    Z = h + (lz - h) * s.view(1, -1, 1)
+    n = torch.empty((nx + 1, nz + 1, ny + 1, 3), dtype=dtype, device=device)
+    n[..., 0] = X.expand_as(Z)
+    n[..., 1] = Y.expand_as(Z)
+    n[..., 2] = Z
+    return n
+
+
+# ------------------------------------------------------------------------------------------------
+# metrics (src/mesh.c:630-1976)
+# ------------------------------------------------------------------------------------------------
+def _cross_rows(d_de, d_dz):
+    """(a x b) laid out as the reference writes csi = de x dz."""
+    ax, ay, az = d_de
+    bx, by, bz = d_dz
+    return torch.stack([ay * bz - az * by, -ax * bz + az * bx, ax * by - ay * bx], dim=-1)
+
+
+def mesh_metrics(nodes: torch.Tensor, periodic: Sequence[int]) -> Tuple[Layout, Dict[str, torch.Tensor]]:
+    """SetMeshMetrics restated: returns the layout and the local (ghosted) metric arrays."""
+    dev, dt = nodes.device, nodes.dtype
+    KN, JN, IN = nodes.shape[:3]
+    lay = Layout(mx=IN + 1, my=JN + 1, mz=KN + 1)
+    mx, my, mz = lay.mx, lay.my, lay.mz
+    gs = lay.gshape
+
+    # local coordinates: nodes at DMDA indices [0, m-2], zero at m-1 (VecSet(gCoor,0), mesh.c:225)
+    coor = torch.zeros(gs + (3,), dtype=dt, device=dev)
+    coor[G:G + KN, G:G + JN, G:G + IN] = nodes
+    local_to_local(coor, lay, periodic)
+    C = lambda k, j, i: _lp(coor, k, j, i)
+
+    # cell centres (mesh.c:705-737)
+    cent = torch.zeros(gs + (3,), dtype=dt, device=dev)
+    K, J, I = slice(1, mz - 1), slice(1, my - 1), slice(1, mx - 1)
+    Km, Jm, Im = slice(0, mz - 2), slice(0, my - 2), slice(0, mx - 2)
+    _lp(cent, K, J, I)[...] = 0.125 * (C(K, J, I) + C(K, Jm, I) + C(Km, J, I) + C(Km, Jm, I) +
+                                       C(K, J, Im) + C(K, Jm, Im) + C(Km, J, Im) + C(Km, Jm, Im))
+    local_to_local(cent, lay, periodic)
+
+    # dummy face bases (mesh.c:757-910), then cell-centred averages (mesh.c:930-960)
+    csitmp = torch.zeros(gs + (3,), dtype=dt, device=dev)
+    etatmp = torch.zeros_like(csitmp)
+    zettmp = torch.zeros_like(csitmp)
+    Ia = slice(0, mx - 1)   # i = xs .. lxe-1
+    Iam = slice(-1, mx - 2)
+    d = lambda a: [a[..., 0], a[..., 1], a[..., 2]]
+    # csi = de x dz on i-faces
+    de = 0.5 * (C(K, J, Ia) + C(Km, J, Ia) - C(K, Jm, Ia) - C(Km, Jm, Ia))
+    dz = 0.5 * (C(K, Jm, Ia) + C(K, J, Ia) - C(Km, Jm, Ia) - C(Km, J, Ia))
+    _lp(csitmp, K, J, Ia)[...] = _cross_rows(d(de), d(dz))
+    # eta = dz x dc on j-faces
+    Ja = slice(0, my - 1)
+    Jam = slice(-1, my - 2)
+    dc = 0.5 * (C(K, Ja, I) + C(Km, Ja, I) - C(K, Ja, Im) - C(Km, Ja, Im))
+    dz = 0.5 * (C(K, Ja, I) + C(K, Ja, Im) - C(Km, Ja, I) - C(Km, Ja, Im))
+    _lp(etatmp, K, Ja, I)[...] = _cross_rows(d(dz), d(dc))
+    # zet = dc x de on k-faces
+    Ka = slice(0, mz - 1)
+    Kam = slice(-1, mz - 2)
+    dc = 0.5 * (C(Ka, J, I) + C(Ka, Jm, I) - C(Ka, J, Im) - C(Ka, Jm, Im))
+    de = 0.5 * (C(Ka, J, I) + C(Ka, J, Im) - C(Ka, Jm, I) - C(Ka, Jm, Im))
+    _lp(zettmp, Ka, J, I)[...] = _cross_rows(d(dc), d(de))
+    for t in (csitmp, etatmp, zettmp):
+        local_to_local(t, lay, periodic)
+
+    csi = torch.zeros(gs + (3,), dtype=dt, device=dev)
+    eta = torch.zeros_like(csi)
+    zet = torch.zeros_like(csi)
+    aj = torch.zeros(gs, dtype=dt, device=dev)
+    _lp(csi, K, J, I)[...] = 0.5 * _lp(csitmp, K, J, I) + 0.5 * _lp(csitmp, K, J, Im)
+    _lp(eta, K, J, I)[...] = 0.5 * _lp(etatmp, K, J, I) + 0.5 * _lp(etatmp, K, Jm, I)
+    _lp(zet, K, J, I)[...] = 0.5 * _lp(zettmp, K, J, I) + 0.5 * _lp(zettmp, Km, J, I)
+
+    # Jacobian (mesh.c:975-1050)
+    dxc = 0.25 * (C(K, J, I) + C(K, Jm, I) + C(Km, J, I) + C(Km, Jm, I) - C(K, J, Im) - C(K, Jm, Im) - C(Km, J, Im) - C(Km, Jm, Im))
+    dxe = 0.25 * (C(K, J, I) + C(K, J, Im) + C(Km, J, I) + C(Km, J, Im) - C(K, Jm, I) - C(K, Jm, Im) - C(Km, Jm, I) - C(Km, Jm, Im))
+    dxz = 0.25 * (C(K, J, I) + C(K, Jm, I) + C(K, J, Im) + C(K, Jm, Im) - C(Km, J, I) - C(Km, Jm, I) - C(Km, J, Im) - C(Km, Jm, Im))
+    _lp(aj, K, J, I)[...] = 1.0 / _det3(dxc, dxe, dxz)
+
+    # mirror grid outside the boundaries (mesh.c:1053-1140), over the owned range
+    o = lambda t: owned(t, lay)
+    for t in (csi, eta, zet, aj):
+        ot = o(t)
+        ot[:, :, 0] = ot[:, :, 1]
+        ot[:, :, mx - 1] = ot[:, :, mx - 2]
+        ot[:, 0] = ot[:, 1]
+        ot[:, my - 1] = ot[:, my - 2]
+        ot[0] = ot[1]
+        ot[mz - 1] = ot[mz - 2]
+        local_to_local(t, lay, periodic)
+    # periodic ghost cells (mesh.c:1160-1260)
+    for t in (csi, eta, zet, aj):
+        reset_cell_periodic(t, lay, periodic)
+
+    # face bases (mesh.c:1352-1876)
+    icsi = torch.zeros(gs + (3,), dtype=dt, device=dev)
+    ieta, izet, jcsi, jeta, jzet, kcsi, keta, kzet = (torch.zeros_like(icsi) for _ in range(8))
+    iaj = torch.zeros(gs, dtype=dt, device=dev)
+    jaj, kaj = torch.zeros_like(iaj), torch.zeros_like(iaj)
+
+    # face centres over the local range where the reference defines them
+    centx = torch.zeros(gs + (3,), dtype=dt, device=dev)
+    centy = torch.zeros_like(centx)
+    centz = torch.zeros_like(centx)
+    centx[1:, 1:, :] = 0.25 * (coor[1:, 1:, :] + coor[:-1, 1:, :] + coor[1:, :-1, :] + coor[:-1, :-1, :])
+    centy[1:, :, 1:] = 0.25 * (coor[1:, :, 1:] + coor[:-1, :, 1:] + coor[1:, :, :-1] + coor[:-1, :, :-1])
+    centz[:, 1:, 1:] = 0.25 * (coor[:, 1:, 1:] + coor[:, :-1, 1:] + coor[:, 1:, :-1] + coor[:, :-1, :-1])
+    CX = lambda k, j, i: _lp(centx, k, j, i)
+    CY = lambda k, j, i: _lp(centy, k, j, i)
+    CZ = lambda k, j, i: _lp(centz, k, j, i)
+
+    def along(Cf, axis_len, ptype, k, j, i, axis):
+        """d/d(face-normal index) from face centres with the reference's boundary rules."""
+        def sl(a, b):
+            idx = [k, j, i]
+            idx[axis] = slice(a, b)
+            return Cf(*idx)
+        n = axis_len
+        out = torch.empty_like(sl(0, n - 1))
+        sel = [slice(None)] * 3
+        # interior faces 1..n-3: central
+        sel_int = list(sel); sel_int[axis] = slice(1, n - 2)
+        out[tuple(sel_int)] = 0.5 * (sl(2, n - 1) - sl(0, n - 3))
+        # face 0: forward
+        s0 = list(sel); s0[axis] = slice(0, 1)
+        out[tuple(s0)] = sl(1, 2) - sl(0, 1)
+        # face n-2
+        sl_last = list(sel); sl_last[axis] = slice(n - 2, n - 1)
+        if ptype == PERIODIC_COPY:
+            out[tuple(sl_last)] = 0.5 * (sl(1, 2) - sl(0, 1) + sl(n - 2, n - 1) - sl(n - 3, n - 2))
+        elif ptype == PERIODIC_WRAP:
+            out[tuple(sl_last)] = 0.5 * (sl(n + 1, n + 2) - sl(n, n + 1) + sl(n - 2, n - 1) - sl(n - 3, n - 2))
+        else:
+            out[tuple(sl_last)] = sl(n - 2, n - 1) - sl(n - 3, n - 2)
+        return out
+
+    # i faces: k,j interior; i = 0..mx-2
+    dc = along(CX, mx, periodic[0], K, J, None, 2)
+    de = 0.5 * (C(K, J, Ia) + C(Km, J, Ia) - C(K, Jm, Ia) - C(Km, Jm, Ia))
+    dz = 0.5 * (C(K, Jm, Ia) + C(K, J, Ia) - C(Km, Jm, Ia) - C(Km, J, Ia))
+    _face_basis(dc, de, dz, [_lp(t, K, J, Ia) for t in (icsi, ieta, izet)], _lp(iaj, K, J, Ia))
+    # j faces: k,i interior; j = 0..my-2
+    dc = 0.5 * (C(K, Ja, I) + C(Km, Ja, I) - C(K, Ja, Im) - C(Km, Ja, Im))
+    de = along(CY, my, periodic[1], K, None, I, 1)
+    dz = 0.5 * (C(K, Ja, I) + C(K, Ja, Im) - C(Km, Ja, I) - C(Km, Ja, Im))
+    _face_basis(dc, de, dz, [_lp(t, K, Ja, I) for t in (jcsi, jeta, jzet)], _lp(jaj, K, Ja, I))
+    # k faces: j,i interior; k = 0..mz-2
+    dc = 0.5 * (C(Ka, J, I) + C(Ka, Jm, I) - C(Ka, J, Im) - C(Ka, Jm, Im))
+    de = 0.5 * (C(Ka, J, I) + C(Ka, J, Im) - C(Ka, Jm, I) - C(Ka, Jm, Im))
+    dz = along(CZ, mz, periodic[2], None, J, I, 0)
+    _face_basis(dc, de, dz, [_lp(t, Ka, J, I) for t in (kcsi, keta, kzet)], _lp(kaj, Ka, J, I))
+    for t in (icsi, ieta, izet, jcsi, jeta, jzet, kcsi, keta, kzet, iaj, jaj, kaj):
+        local_to_local(t, lay, periodic)
+
+    out = dict(csi=csi, eta=eta, zet=zet, icsi=icsi, ieta=ieta, izet=izet, jcsi=jcsi, jeta=jeta,
+               jzet=jzet, kcsi=kcsi, keta=keta, kzet=kzet, aj=aj, iaj=iaj, jaj=jaj, kaj=kaj,
+               cent=cent, nvert=torch.zeros(gs, dtype=dt, device=dev),
+               meshtag=torch.zeros(gs, dtype=dt, device=dev))
+    return lay, out
+
+
+def _det3(dc, de, dz):
+    dxdc, dydc, dzdc = dc[..., 0], dc[..., 1], dc[..., 2]
+    dxde, dyde, dzde = de[..., 0], de[..., 1], de[..., 2]
+    dxdz, dydz, dzdz = dz[..., 0], dz[..., 1], dz[..., 2]
+    return (dxdc * (dyde * dzdz - dzde * dydz) - dydc * (dxde * dzdz - dzde * dxdz) +
+            dzdc * (dxde * dydz - dyde * dxdz))
+
+
+def _face_basis(dc, de, dz, outs, aj_out):
+    """csi = eta x zet, eta = zet x csi, zet = csi x eta at a face, and its inverse Jacobian."""
+    dxdc, dydc, dzdc = dc[..., 0], dc[..., 1], dc[..., 2]
+    dxde, dyde, dzde = de[..., 0], de[..., 1], de[..., 2]
+    dxdz, dydz, dzdz = dz[..., 0], dz[..., 1], dz[..., 2]
+    outs[0][...] = torch.stack([dyde * dzdz - dzde * dydz, -dxde * dzdz + dzde * dxdz, dxde * dydz - dyde * dxdz], -1)
+    outs[1][...] = torch.stack([dydz * dzdc - dzdz * dydc, -dxdz * dzdc + dzdz * dxdc, dxdz * dydc - dydz * dxdc], -1)
+    outs[2][...] = torch.stack([dydc * dzde - dzdc * dyde, -dxdc * dzde + dzdc * dxde, dxdc * dyde - dydc * dxde], -1)
+    aj_out[...] = 1.0 / _det3(dc, de, dz)
+
+
+def reset_cell_periodic(t: torch.Tensor, lay: Layout, periodic: Sequence[int]) -> torch.Tensor:
+    """resetCellPeriodicFluxes / the periodic block of SetMeshMetrics: on periodic axes the
+    boundary ghost cells 0 and m-1 take the values of cells m-2 and 1, then ghosts are wrapped."""
+    o = owned(t, lay)
+    m = (lay.mx, lay.my, lay.mz)
+    for axis, dim in ((0, 2), (1, 1), (2, 0)):
+        if periodic[axis] == PERIODIC_NONE:
+            continue
+        n = m[axis]
+        idx0 = [slice(None)] * 3
+        idx0[dim] = 0
+        idxs = [slice(None)] * 3
+        idxs[dim] = n - 2
+        o[tuple(idx0)] = o[tuple(idxs)]
+        idx0[dim] = n - 1
+        idxs[dim] = 1
+        o[tuple(idx0)] = o[tuple(idxs)]
+    return local_to_local(t, lay, periodic)
+
+
+# ------------------------------------------------------------------------------------------------
+# boundary conditions for cell-centred fields
+# ------------------------------------------------------------------------------------------------
+BC_PERIODIC, BC_NOSLIP, BC_SLIP, BC_FIXED, BC_ZEROGRAD = "periodic", "noSlip", "slip", "fixedValue", "zeroGradient"
+
+
+def apply_vector_bcs(u: torch.Tensor, lay: Layout, periodic, bcs, values, metrics=None):
+    """Ghost-cell values of a cell-centred vector (UpdateCartesianBCs subset, boundary.c:1632-1960).
+    bcs/values: 6 entries in patch order iLeft, iRight, jLeft, jRight, kLeft, kRight."""
+    o = owned(u, lay)
+    m = (lay.mx, lay.my, lay.mz)
+    for patch in range(6):
+        axis, side = patch // 2, patch % 2
+        dim = 2 - axis
+        n = m[axis]
+        bc = bcs[patch]
+        g_idx, c1, c2 = (0, 1, 2) if side == 0 else (n - 1, n - 2, n - 3)
+        # restrict the transverse axes to interior cells, as the reference's cell loop does
+        idx = [slice(1, lay.mz - 1), slice(1, lay.my - 1), slice(1, lay.mx - 1)]
+
+        def at(i):
+            s = list(idx)
+            s[dim] = i
+            return tuple(s)
+
+        if bc == BC_PERIODIC:
+            src = n - 2 if side == 0 else 1
+            o[at(g_idx)] = o[at(src)]
+        elif bc == BC_NOSLIP:
+            o[at(g_idx)] = -o[at(c1)]
+        elif bc == BC_FIXED:
+            o[at(g_idx)] = torch.as_tensor(values[patch], dtype=u.dtype, device=u.device)
+        elif bc == BC_ZEROGRAD:
+            o[at(g_idx)] = 2 * o[at(c1)] - o[at(c2)]
+        elif bc == BC_SLIP:
+            nrm = _patch_normal(metrics, lay, axis)[at(c1)]
+            uc = o[at(c1)]
+            o[at(g_idx)] = uc - 2.0 * (uc * nrm).sum(-1, keepdim=True) * nrm
+        else:
+            raise ValueError(f"unsupported velocity bc {bc}")
+    return local_to_local(u, lay, periodic)
+
+
+def _patch_normal(metrics, lay, axis):
+    """Unit normal of a constant-index surface from the covariant basis (calculateNormal,
+    src/include/inline.h:1231-1260)."""
+    csi, eta, zet = (owned(metrics[n], lay) for n in ("csi", "eta", "zet"))
+    g = torch.stack([csi, eta, zet], dim=-2)         # rows csi, eta, zet
+    Gc = torch.linalg.inv(g)                          # columns: x_csi, x_eta, x_zet
+    v = Gc[..., :, axis]
+    return v / torch.linalg.norm(v, dim=-1, keepdim=True)
+
+
+def apply_scalar_bcs(t: torch.Tensor, lay: Layout, periodic, bcs, values):
+    o = owned(t, lay)
+    m = (lay.mx, lay.my, lay.mz)
+    for patch in range(6):
+        axis, side = patch // 2, patch % 2
+        dim = 2 - axis
+        n = m[axis]
+        bc = bcs[patch]
+        g_idx, c1, c2 = (0, 1, 2) if side == 0 else (n - 1, n - 2, n - 3)
+        idx = [slice(1, lay.mz - 1), slice(1, lay.my - 1), slice(1, lay.mx - 1)]
+
+        def at(i):
+            s = list(idx)
+            s[dim] = i
+            return tuple(s)
+
+        if bc == BC_PERIODIC:
+            o[at(g_idx)] = o[at(n - 2 if side == 0 else 1)]
+        elif bc == BC_FIXED:
+            o[at(g_idx)] = values[patch]
+        elif bc == BC_ZEROGRAD:
+            o[at(g_idx)] = o[at(c1)]
+        else:
+            raise ValueError(f"unsupported scalar bc {bc}")
+    return local_to_local(t, lay, periodic)
+
+
+def ucont_from_ucat(ucat, metrics, lay: Layout, periodic, bcs):
+    """Contravariant face fluxes V^q = S^q . (U_L+U_R)/2 (initialField.c:1135-1188), then the
+    face BCs: periodic faces 0 copy face m-2, wall faces carry zero flux (boundary.c:614-822)."""
+    V = torch.zeros_like(ucat)
+    mx, my, mz = lay.mx, lay.my, lay.mz
+    K, J, I = slice(1, mz - 1), slice(1, my - 1), slice(1, mx - 1)
+    Ia, Ja, Ka = slice(0, mx - 1), slice(0, my - 1), slice(0, mz - 1)
+    U = lambda k, j, i: _lp(ucat, k, j, i)
+    uf = 0.5 * (U(K, J, Ia) + U(K, J, slice(1, mx)))
+    _lp(V, K, J, Ia)[..., 0] = (uf * _lp(metrics["icsi"], K, J, Ia)).sum(-1)
+    uf = 0.5 * (U(K, Ja, I) + U(K, slice(1, my), I))
+    _lp(V, K, Ja, I)[..., 1] = (uf * _lp(metrics["jeta"], K, Ja, I)).sum(-1)
+    uf = 0.5 * (U(Ka, J, I) + U(slice(1, mz), J, I))
+    _lp(V, Ka, J, I)[..., 2] = (uf * _lp(metrics["kzet"], Ka, J, I)).sum(-1)
+    o = owned(V, lay)
+    m = (mx, my, mz)
+    for patch in range(6):
+        axis, side = patch // 2, patch % 2
+        dim, n = 2 - axis, m[axis]
+        f = 0 if side == 0 else n - 2
+        s = [slice(None)] * 3
+        s[dim] = f
+        if bcs[patch] in (BC_NOSLIP, BC_SLIP):
+            o[tuple(s) + (axis,)] = 0.0
+        elif bcs[patch] == BC_PERIODIC and side == 0:
+            s2 = list(s)
+            s2[dim] = n - 2
+            o[tuple(s) + (axis,)] = o[tuple(s2) + (axis,)]
+    return local_to_local(V, lay, periodic)
+
+
+# ------------------------------------------------------------------------------------------------
+# initial fields
+# ------------------------------------------------------------------------------------------------
+def taylor_green(cent, lay: Layout, lengths, u0=1.0, freq=1.0):
+    """SetTaylorGreenFieldU (initialField.c:1085-1116) at interior cell centres."""
+    Lx, Ly, Lz = lengths
+    kx, ky, kz = (freq * 2.0 * math.pi / L for L in (Lx, Ly, Lz))
+    c = interior(cent, lay)
+    x, y, z = c[..., 0], c[..., 1], c[..., 2]
+    u = torch.zeros(lay.gshape + (3,), dtype=cent.dtype, device=cent.device)
+    ui = interior(u, lay)
+    ui[..., 0] = u0 * torch.sin(kx * x) * torch.cos(ky * y) * torch.cos(kz * z)
+    ui[..., 1] = -u0 * torch.cos(kx * x) * torch.sin(ky * y) * torch.cos(kz * z)
+    ui[..., 2] = 0.0
+    return u
+
+
+def log_law(cent, lay: Layout, ground, ustar=0.4, z0=0.1, kappa=0.4):
+    """Neutral log law U = (u*/kappa) ln((h + z0)/z0) along x at interior cell centres;
+    `ground` is the local terrain height under each cell centre (broadcastable)."""
+    c = interior(cent, lay)
+    h = torch.clamp(c[..., 2] - ground, min=0.0)
+    u = torch.zeros(lay.gshape + (3,), dtype=cent.dtype, device=cent.device)
+    interior(u, lay)[..., 0] = ustar / kappa * torch.log((h + z0) / z0)
+    return u
+
+
+def gaussian_noise(u, lay: Layout, rel_rms: float, seed: int):
+    """Adds N(0, (rel_rms*|U|)^2) per component (numpy PCG64, generated on the host)."""
+    ui = interior(u, lay)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    noise = torch.as_tensor(rng.standard_normal(tuple(ui.shape)), dtype=u.dtype).to(u.device)
+    mag = torch.linalg.norm(ui, dim=-1, keepdim=True)
+    ui += rel_rms * mag * noise
+    return u
+
+
+def random_fourier(u, cent, lay: Layout, lengths, dmin, rms, seed, n_modes=2000):
+    """Divergence-free random-Fourier turbulence: n_modes modes, |k| log-spaced in
+    [2 pi / L, pi / dmin], E(k) ~ k^-5/3, amplitude direction orthogonal to k, random phases,
+    rescaled to `rms`; added to u at interior cells."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    kmin = 2.0 * math.pi / max(lengths)
+    kmax = math.pi / dmin
+    kmag = np.exp(np.linspace(math.log(kmin), math.log(kmax), n_modes))
+    dk = np.gradient(kmag)
+    amp = np.sqrt(kmag ** (-5.0 / 3.0) * dk)
+    dirs = rng.standard_normal((n_modes, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    kvec = dirs * kmag[:, None]
+    sig = rng.standard_normal((n_modes, 3))
+    sig -= (sig * dirs).sum(1, keepdims=True) * dirs
+    sig /= np.linalg.norm(sig, axis=1, keepdims=True)
+    phase = rng.uniform(0, 2 * math.pi, n_modes)
+    c = interior(cent, lay)
+    acc = torch.zeros(c.shape, dtype=u.dtype, device=u.device)
+    kv = torch.as_tensor(kvec, dtype=u.dtype, device=u.device)
+    sg = torch.as_tensor(sig * amp[:, None], dtype=u.dtype, device=u.device)
+    ph = torch.as_tensor(phase, dtype=u.dtype, device=u.device)
+    for m in range(n_modes):
+        arg = c[..., 0] * kv[m, 0] + c[..., 1] * kv[m, 1] + c[..., 2] * kv[m, 2] + ph[m]
+        acc += torch.cos(arg).unsqueeze(-1) * sg[m]
+    cur = torch.sqrt((acc ** 2).sum(-1).mean() / 3.0)
+    interior(u, lay)[...] += acc * (rms / cur)
+    return u
+
+
+def rampanelli_zardi(cent, lay: Layout, ground, t_ref=300.0, h_inv=700.0, d_inv=100.0,
+                     g_inv=5.0, g_top=0.003, g_abl=0.0, smear=0.33):
+    """SetABLInitialFlowT (initialField.c:1803-1846) at interior cell centres."""
+    b = smear * g_top * d_inv
+    a = g_inv - b
+    c = smear * g_abl * d_inv
+    h = interior(cent, lay)[..., 2] - ground
+    eta = (h - h_inv) / smear / d_inv
+    f = (torch.tanh(eta) + 1.0) / 2.0
+    lc = torch.log(2.0 * torch.cosh(eta))
+    g = (lc + eta) / 2.0
+    hh = (eta - lc) / 2.0
+    t = torch.zeros(lay.gshape, dtype=cent.dtype, device=cent.device)
+    interior(t, lay)[...] = t_ref + a * f + b * g + c * hh + g_abl * h_inv
+    return t
+
+
+# ------------------------------------------------------------------------------------------------
+# the benchmark configurations
+# ------------------------------------------------------------------------------------------------
+@dataclass
+class Case:
+    name: str
+    lay: Layout
+    periodic: Tuple[int, int, int]
+    mesh: Dict[str, torch.Tensor]
+    ucat: torch.Tensor
+    ucont: torch.Tensor
+    theta: torch.Tensor
+    params: Dict
+    bcs_u: Tuple[str, ...]
+    meta: Dict = field(default_factory=dict)
+
+    @property
+    def ncells(self) -> int:
+        return self.lay.ncells
+
+
+def default_params(periodic, bcs_u, bcs_nut, nut_values=(0.0,) * 6, div_scheme=DIV_CENTRAL,
+                   les_model=LES_DSM, nu=1.5e-5, t_ref=300.0, max_cs=0.5, les_active=1):
+    bcu = []
+    for b in bcs_u:
+        bcu.append({BC_SLIP: BCU_SLIP, BC_PERIODIC: BCU_PERIODIC}.get(b, BCU_GENERIC))
+    bcn = []
+    for b in bcs_nut:
+        bcn.append({BC_PERIODIC: BCNUT_PERIODIC, BC_ZEROGRAD: BCNUT_ZEROGRADIENT,
+                    BC_FIXED: BCNUT_FIXEDVALUE}.get(b, BCNUT_NONE))
+    return dict(periodic=tuple(periodic), bc_u=tuple(bcu), bc_nut=tuple(bcn),
+                nut_value=tuple(float(v) for v in nut_values), div_scheme=div_scheme,
+                les_active=les_active, les_model=les_model, ibm_active=0, overset_active=0,
+                nu=nu, max_cs=max_cs, t_ref=t_ref)
+
+
+def _finish(name, lay, periodic, mesh, ucat, theta, bcs_u, u_values, bcs_t, params, meta):
+    apply_vector_bcs(ucat, lay, periodic, bcs_u, u_values, mesh)
+    apply_scalar_bcs(theta, lay, periodic, bcs_t, [params["t_ref"]] * 6)
+    ucont = ucont_from_ucat(ucat, mesh, lay, periodic, bcs_u)
+    return Case(name, lay, tuple(periodic), mesh, ucat, ucont, theta, params, tuple(bcs_u), meta)
+
+
+def tgv_case(n=64, ptype=PERIODIC_WRAP, device="cpu", les_model=LES_SMAGORINSKY,
+             div_scheme=DIV_CENTRAL) -> Case:
+    """n^3 periodic box of side 2 pi, Taylor-Green vortex, nu = 1/1600, theta = t_ref."""
+    L = 2.0 * math.pi
+    ax = np.linspace(0.0, L, n + 1)
+    periodic = (ptype, ptype, ptype)
+    nodes = cartesian_nodes(ax, ax, ax, device)
+    lay, mesh = mesh_metrics(nodes, periodic)
+    ucat = taylor_green(mesh["cent"], lay, (L, L, L))
+    theta = torch.full(lay.gshape, 300.0, dtype=torch.float64, device=device)
+    bcs = (BC_PERIODIC,) * 6
+    params = default_params(periodic, bcs, bcs, div_scheme=div_scheme, les_model=les_model,
+                            nu=1.0 / 1600.0)
+    return _finish("tgv", lay, periodic, mesh, ucat, theta, bcs, [None] * 6, bcs, params,
+                   dict(lengths=(L, L, L)))
+
+
+def hill_case(nx=256, ny=128, nz=96, lx=3000.0, ly=1500.0, lz=1000.0, ratio=1.05,
+              device="cpu", seed=1, les_model=LES_DSM, div_scheme=DIV_QUICK,
+              turbulence="noise", iperiod=PERIODIC_WRAP) -> Case:
+    """Terrain-following cosine hill (h = 100 m, half-width 250 m), log-law inflow + noise or
+    random-Fourier turbulence, Rampanelli-Zardi theta; i periodic, bottom noSlip, top slip,
+    k-left fixedValue, k-right zeroGradient."""
+    nodes = hill_nodes(lx, ly, lz, nx, ny, nz, ratio, 100.0, 250.0, device)
+    periodic = (iperiod, PERIODIC_NONE, PERIODIC_NONE)
+    lay, mesh = mesh_metrics(nodes, periodic)
+    # terrain height under each interior cell centre: bottom node layer averaged like the centre
+    ground = _ground_under_cells(nodes, lay)
+    ucat = log_law(mesh["cent"], lay, ground)
+    if turbulence == "noise":
+        gaussian_noise(ucat, lay, 0.05, seed)
+    else:
+        random_fourier(ucat, mesh["cent"], lay, (lx, ly, lz), min(lx / nx, ly / ny), 1.0, seed)
+    theta = rampanelli_zardi(mesh["cent"], lay, ground)
+    bcs_u = (BC_PERIODIC, BC_PERIODIC, BC_NOSLIP, BC_SLIP, BC_FIXED, BC_ZEROGRAD)
+    u_vals = [None, None, None, None, (1.0, 0.0, 0.0), None]
+    bcs_nut = (BC_PERIODIC, BC_PERIODIC, BC_FIXED, BC_FIXED, BC_ZEROGRAD, BC_ZEROGRAD)
+    bcs_t = (BC_PERIODIC, BC_PERIODIC, BC_ZEROGRAD, BC_ZEROGRAD, BC_ZEROGRAD, BC_ZEROGRAD)
+    params = default_params(periodic, bcs_u, bcs_nut, div_scheme=div_scheme, les_model=les_model)
+    return _finish("hill", lay, periodic, mesh, ucat, theta, bcs_u, u_vals, bcs_t, params,
+                   dict(lengths=(lx, ly, lz)))
+
+
+def abl_case(nx=384, ny=256, nz=128, lx=6000.0, ly=4000.0, lz=1500.0, ratio=1.05,
+             device="cpu", seed=3, les_model=LES_SMAGORINSKY, div_scheme=DIV_CENTRAL,
+             turbulence="fourier", n_modes=2000) -> Case:
+    """Stretched Cartesian ABL box, periodic in x and y (type 2), bottom noSlip, top slip;
+    log law + turbulence; capped neutral theta."""
+    x = np.linspace(0.0, lx, nx + 1)
+    y = np.linspace(0.0, ly, ny + 1)
+    z = geometric_axis(lz, nz, ratio)
+    nodes = cartesian_nodes(x, y, z, device)
+    periodic = (PERIODIC_WRAP, PERIODIC_NONE, PERIODIC_WRAP)
+    lay, mesh = mesh_metrics(nodes, periodic)
+    ucat = log_law(mesh["cent"], lay, 0.0)
+    if turbulence == "fourier":
+        random_fourier(ucat, mesh["cent"], lay, (lx, ly, lz), min(lx / nx, ly / ny), 1.0, seed, n_modes)
+    else:
+        gaussian_noise(ucat, lay, 0.05, seed)
+    theta = rampanelli_zardi(mesh["cent"], lay, 0.0)
+    bcs_u = (BC_PERIODIC, BC_PERIODIC, BC_NOSLIP, BC_SLIP, BC_PERIODIC, BC_PERIODIC)
+    bcs_nut = (BC_PERIODIC, BC_PERIODIC, BC_ZEROGRAD, BC_ZEROGRAD, BC_PERIODIC, BC_PERIODIC)
+    bcs_t = (BC_PERIODIC, BC_PERIODIC, BC_ZEROGRAD, BC_ZEROGRAD, BC_PERIODIC, BC_PERIODIC)
+    params = default_params(periodic, bcs_u, bcs_nut, div_scheme=div_scheme, les_model=les_model)
+    return _finish("abl", lay, periodic, mesh, ucat, theta, bcs_u, [None] * 6, bcs_t, params,
+                   dict(lengths=(lx, ly, lz)))
+
+
+def _ground_under_cells(nodes, lay):
+    zb = nodes[:, 0, :, 2]                         # bottom node layer, [k][i]
+    g = 0.25 * (zb[1:, 1:] + zb[:-1, 1:] + zb[1:, :-1] + zb[:-1, :-1])
+    return g.unsqueeze(1)                          # broadcast over j
+
+
+def shard_case(nx=512, ny=512, nz=192, device="cpu", seed=5, les_model=LES_DSM,
+               div_scheme=DIV_CENTRAL, dx=10.0, lz=1500.0, ratio=1.01) -> Case:
+    """Large single-GPU ABL shard: dx = dy = 10 m, geometric vertical stretching to lz, neutral
+    below 700 m with a 5 K inversion and 3 K/km above, log law + 5 % Gaussian noise."""
+    return abl_case(nx=nx, ny=ny, nz=nz, lx=nx * dx, ly=ny * dx, lz=lz, ratio=ratio,
+                    device=device, seed=seed, les_model=les_model, div_scheme=div_scheme,
+                    turbulence="noise")
+
+
+CONFIGS = {
+    1: dict(builder=tgv_case, kwargs=dict(n=64, les_model=LES_SMAGORINSKY, div_scheme=DIV_CENTRAL)),
+    2: dict(builder=hill_case, kwargs=dict(nx=256, ny=128, nz=96, les_model=LES_DSM, div_scheme=DIV_QUICK)),
+    3: dict(builder=abl_case, kwargs=dict(nx=384, ny=256, nz=128, les_model=LES_SMAGORINSKY, div_scheme=DIV_CENTRAL)),
+    4: dict(builder=hill_case, kwargs=dict(nx=192, ny=192, nz=128, lx=3000.0, ly=3000.0, les_model=LES_DSM,
+                                           div_scheme=DIV_QUICK, turbulence="fourier", seed=4)),
+    5: dict(builder=shard_case, kwargs=dict(nx=512, ny=512, nz=192, les_model=LES_DSM, div_scheme=DIV_CENTRAL)),
+}

@@ -1,0 +1,386 @@
+// api.hip -- the C ABI (include/tosca_rhs.h): context, argument validation and dispatch.
+//
+// Every entry point validates its arguments on the host before any kernel is queued: enums,
+// layout, and that every array is device memory large enough for the DMDA layout it is read or
+// written with (hipMemGetAddressRange), so a wrong size or a host pointer is an error code, not
+// a GPU memory fault.  Unsupported reference features (IBM, overset, structural LES models,
+// schemes other than central/QUICK) are rejected with TOSCA_ERR_UNSUPPORTED.
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+
+using tosca::Geo;
+
+struct tosca_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    tosca_dmda_info info{};
+    Geo geo{};
+    std::string err;
+    double *ws = nullptr;
+    size_t ws_elems = 0;
+    int *chk = nullptr;
+    size_t chk_elems = 0;
+};
+
+namespace {
+
+thread_local std::string g_err;   // last error of calls made without a usable context
+
+int set_err(tosca_ctx *c, int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    g_err = buf;
+    return code;
+}
+
+bool device_forced_off() {
+    const char *e = getenv("TOSCA_FORCE_NO_DEVICE");
+    return e && e[0] && strcmp(e, "0") != 0;
+}
+
+// `need` bytes starting at p must lie inside one device allocation.
+int check_dev(tosca_ctx *c, const void *p, size_t need, const char *name) {
+    if (!p) return set_err(c, TOSCA_ERR_ARG, "%s is NULL", name);
+    if (reinterpret_cast<uintptr_t>(p) % 8) return set_err(c, TOSCA_ERR_ARG, "%s is not 8-byte aligned", name);
+    hipPointerAttribute_t attr;
+    if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return set_err(c, TOSCA_ERR_ARG, "%s is not a device pointer", name);
+    }
+    if (attr.type != hipMemoryTypeDevice && attr.type != hipMemoryTypeManaged)
+        return set_err(c, TOSCA_ERR_ARG, "%s is not device memory", name);
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    if (hipMemGetAddressRange(&base, &size, const_cast<void *>(p)) == hipSuccess && base) {
+        const char *b = reinterpret_cast<const char *>(base), *q = reinterpret_cast<const char *>(p);
+        if (q + need > b + size)
+            return set_err(c, TOSCA_ERR_ARG, "%s: allocation too small (%zu bytes needed past the pointer, %zu available)",
+                           name, need, (size_t)(b + size - q));
+    } else {
+        (void)hipGetLastError();
+    }
+    return TOSCA_OK;
+}
+
+int check_ctx(tosca_ctx *c) {
+    if (!c) return set_err(nullptr, TOSCA_ERR_ARG, "context is NULL");
+    if (hipSetDevice(c->device) != hipSuccess) return set_err(c, TOSCA_ERR_DEVICE, "hipSetDevice(%d) failed", c->device);
+    return TOSCA_OK;
+}
+
+int check_params(tosca_ctx *c, const tosca_params *p) {
+    if (!p) return set_err(c, TOSCA_ERR_ARG, "params is NULL");
+    for (int a = 0; a < 3; a++)
+        if (p->periodic[a] < TOSCA_PERIODIC_NONE || p->periodic[a] > TOSCA_PERIODIC_WRAP)
+            return set_err(c, TOSCA_ERR_ARG, "periodic[%d] = %d is not a periodic type", a, p->periodic[a]);
+    for (int q = 0; q < 6; q++) {
+        if (p->bc_u[q] < TOSCA_BCU_GENERIC || p->bc_u[q] > TOSCA_BCU_PERIODIC)
+            return set_err(c, TOSCA_ERR_ARG, "bc_u[%d] = %d is not a TOSCA_BCU_* kind", q, p->bc_u[q]);
+        if (p->bc_nut[q] < TOSCA_BCNUT_NONE || p->bc_nut[q] > TOSCA_BCNUT_PERIODIC)
+            return set_err(c, TOSCA_ERR_ARG, "bc_nut[%d] = %d is not a TOSCA_BCNUT_* kind", q, p->bc_nut[q]);
+    }
+    if (p->ibm_active) return set_err(c, TOSCA_ERR_UNSUPPORTED, "immersed boundaries (IBM) are not supported");
+    if (p->overset_active) return set_err(c, TOSCA_ERR_UNSUPPORTED, "overset meshes are not supported");
+    return TOSCA_OK;
+}
+
+int check_les_model(tosca_ctx *c, const tosca_params *p) {
+    switch (p->les_model) {
+        case TOSCA_LES_SMAGORINSKY: case TOSCA_LES_DSM: case TOSCA_LES_DPASD: return TOSCA_OK;
+        case TOSCA_LES_BV: case TOSCA_LES_BAMD:
+            return set_err(c, TOSCA_ERR_UNSUPPORTED, "structural LES models (bardinaVreman/bardinaAMD) are not supported");
+        default:
+            if (p->les_model < 0 || p->les_model > TOSCA_LES_BAMD)
+                return set_err(c, TOSCA_ERR_ARG, "les_model = %d is not an LES model", p->les_model);
+            return set_err(c, TOSCA_ERR_UNSUPPORTED, "LES model %d is not supported (smagorinsky, dynamicSmagorinsky, dynamicPASD are)", p->les_model);
+    }
+}
+
+int check_mesh(tosca_ctx *c, const tosca_mesh *m, bool face, bool cell, bool jac) {
+    if (!m) return set_err(c, TOSCA_ERR_ARG, "mesh is NULL");
+    const size_t n = (size_t)c->geo.nloc(), v = 3 * n * sizeof(double), s = n * sizeof(double);
+    int rc;
+    if (cell) {
+        if ((rc = check_dev(c, m->csi, v, "mesh.csi")) || (rc = check_dev(c, m->eta, v, "mesh.eta")) ||
+            (rc = check_dev(c, m->zet, v, "mesh.zet")) || (rc = check_dev(c, m->aj, s, "mesh.aj")))
+            return rc;
+    }
+    if (face) {
+        const double *vv[9] = {m->icsi, m->ieta, m->izet, m->jcsi, m->jeta, m->jzet, m->kcsi, m->keta, m->kzet};
+        const char *nm[9] = {"mesh.icsi", "mesh.ieta", "mesh.izet", "mesh.jcsi", "mesh.jeta", "mesh.jzet", "mesh.kcsi", "mesh.keta", "mesh.kzet"};
+        for (int q = 0; q < 9; q++)
+            if ((rc = check_dev(c, vv[q], v, nm[q]))) return rc;
+    }
+    if (jac) {
+        if ((rc = check_dev(c, m->iaj, s, "mesh.iaj")) || (rc = check_dev(c, m->jaj, s, "mesh.jaj")) ||
+            (rc = check_dev(c, m->kaj, s, "mesh.kaj")))
+            return rc;
+    }
+    return TOSCA_OK;
+}
+
+int ensure_ws(tosca_ctx *c, size_t elems) {
+    if (c->ws_elems >= elems) return TOSCA_OK;
+    if (c->ws) { hipStreamSynchronize(c->stream); hipFree(c->ws); c->ws = nullptr; c->ws_elems = 0; }
+    if (hipMalloc(&c->ws, elems * sizeof(double)) != hipSuccess) {
+        (void)hipGetLastError();
+        return set_err(c, TOSCA_ERR_DEVICE, "workspace allocation of %zu bytes failed", elems * sizeof(double));
+    }
+    c->ws_elems = elems;
+    return TOSCA_OK;
+}
+
+int launched(tosca_ctx *c, const char *what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_err(c, TOSCA_ERR_DEVICE, "%s: %s", what, hipGetErrorString(e));
+    return TOSCA_OK;
+}
+
+void set_geo_per(tosca_ctx *c, const tosca_params *p) {
+    for (int a = 0; a < 3; a++) c->geo.per[a] = p->periodic[a];
+}
+
+size_t ws_need(const tosca_ctx *c) {
+    // 22 local fields (FormU: 6 face-flux vectors + fp; UpdateCs: strain 10 + LM/MM/QN/NN)
+    // plus the DPASD plane partials
+    return 22 * (size_t)c->geo.nloc() + (size_t)c->geo.my * 16 * 4 + 64;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *tosca_version(void) { return "tosca-mi355x 0.1 (gfx950)"; }
+
+const char *tosca_last_error(const tosca_ctx *ctx) { return ctx ? ctx->err.c_str() : g_err.c_str(); }
+
+int tosca_ctx_create(tosca_ctx **out, const tosca_dmda_info *in, int device) {
+    if (!out) return set_err(nullptr, TOSCA_ERR_ARG, "ctx output pointer is NULL");
+    *out = nullptr;
+    if (!in) return set_err(nullptr, TOSCA_ERR_ARG, "info is NULL");
+    if (in->xs != 0 || in->ys != 0 || in->zs != 0 || in->xm != in->mx || in->ym != in->my || in->zm != in->mz)
+        return set_err(nullptr, TOSCA_ERR_LAYOUT, "one GPU must own the whole DMDA (xs=ys=zs=0, xm=mx, ym=my, zm=mz)");
+    if (in->gxs != -3 || in->gys != -3 || in->gzs != -3 || in->gxm != in->mx + 6 || in->gym != in->my + 6 ||
+        in->gzm != in->mz + 6)
+        return set_err(nullptr, TOSCA_ERR_LAYOUT, "local arrays must carry ghost width 3 on every side");
+    if (in->mx < 4 || in->my < 4 || in->mz < 4)
+        return set_err(nullptr, TOSCA_ERR_LAYOUT, "each axis needs at least 2 interior cells (m >= 4)");
+    const double pts = (double)in->gxm * (double)in->gym * (double)in->gzm;
+    if (pts * 10.0 >= 2147483647.0)
+        return set_err(nullptr, TOSCA_ERR_LAYOUT, "local array too large for 32-bit element indexing (%.0f points)", pts);
+    if (device_forced_off()) return set_err(nullptr, TOSCA_ERR_DEVICE, "device disabled by TOSCA_FORCE_NO_DEVICE");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        (void)hipGetLastError();
+        return set_err(nullptr, TOSCA_ERR_DEVICE, "no HIP device available");
+    }
+    if (device < 0 || device >= ndev) return set_err(nullptr, TOSCA_ERR_ARG, "device %d out of range (%d devices)", device, ndev);
+    if (hipSetDevice(device) != hipSuccess) return set_err(nullptr, TOSCA_ERR_DEVICE, "hipSetDevice(%d) failed", device);
+    tosca_ctx *c = new tosca_ctx();
+    c->device = device;
+    c->info = *in;
+    c->geo.mx = (int)in->mx; c->geo.my = (int)in->my; c->geo.mz = (int)in->mz;
+    c->geo.gxm = (int)in->gxm; c->geo.gym = (int)in->gym; c->geo.gzm = (int)in->gzm;
+    *out = c;
+    return TOSCA_OK;
+}
+
+int tosca_ctx_destroy(tosca_ctx *c) {
+    if (!c) return TOSCA_OK;
+    hipSetDevice(c->device);
+    if (c->ws || c->chk) hipStreamSynchronize(c->stream);
+    if (c->ws) hipFree(c->ws);
+    if (c->chk) hipFree(c->chk);
+    delete c;
+    return TOSCA_OK;
+}
+
+int tosca_ctx_set_stream(tosca_ctx *c, void *stream) {
+    if (!c) return set_err(nullptr, TOSCA_ERR_ARG, "context is NULL");
+    c->stream = reinterpret_cast<hipStream_t>(stream);
+    return TOSCA_OK;
+}
+
+int tosca_ctx_synchronize(tosca_ctx *c) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return set_err(c, TOSCA_ERR_DEVICE, "stream synchronize: %s", hipGetErrorString(e));
+    return TOSCA_OK;
+}
+
+int tosca_mesh_check(tosca_ctx *c, const tosca_mesh *m) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if ((rc = check_mesh(c, m, false, true, false))) return rc;
+    const size_t s = (size_t)c->geo.nloc() * sizeof(double);
+    if ((rc = check_dev(c, m->nvert, s, "mesh.nvert")) || (rc = check_dev(c, m->meshtag, s, "mesh.meshtag"))) return rc;
+    const size_t need = 4 * (size_t)(c->geo.mz) + 8;
+    if (c->chk_elems < need) {
+        if (c->chk) hipFree(c->chk);
+        c->chk = nullptr;
+        if (hipMalloc(&c->chk, need * sizeof(int)) != hipSuccess) return set_err(c, TOSCA_ERR_DEVICE, "allocation failed");
+        c->chk_elems = need;
+    }
+    tosca::launch_mesh_check(c->geo, m, c->chk, c->stream);
+    if ((rc = launched(c, "mesh check"))) return rc;
+    std::vector<int> h(4 * c->geo.mz);
+    hipMemcpyAsync(h.data(), c->chk, h.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return set_err(c, TOSCA_ERR_DEVICE, "mesh check failed");
+    long cnt[4] = {0, 0, 0, 0};
+    for (int k = 0; k < c->geo.mz; k++)
+        for (int q = 0; q < 4; q++) cnt[q] += h[4 * k + q];
+    if (cnt[0]) return set_err(c, TOSCA_ERR_MESH, "%ld points with nvert != 0 (immersed boundaries are not supported)", cnt[0]);
+    if (cnt[1]) return set_err(c, TOSCA_ERR_MESH, "%ld points with meshTag != 0 (overset meshes are not supported)", cnt[1]);
+    if (cnt[2]) return set_err(c, TOSCA_ERR_MESH, "%ld interior cells with negative or close-to-zero volume (aj < 1e-20)", cnt[2]);
+    if (cnt[3]) return set_err(c, TOSCA_ERR_MESH, "%ld interior cells with a close-to-zero face area (|csi|,|eta|,|zet| < 1e-20)", cnt[3]);
+    return TOSCA_OK;
+}
+
+int tosca_formu(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const double *ucont, const double *ucat,
+                const double *nut, double *rhs, double scale, int form_mode) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if ((rc = check_params(c, p))) return rc;
+    if (form_mode < 0 || form_mode > 2) return set_err(c, TOSCA_ERR_ARG, "form_mode = %d (0, 1 or 2)", form_mode);
+    if (p->div_scheme != TOSCA_DIV_CENTRAL && p->div_scheme != TOSCA_DIV_QUICK)
+        return set_err(c, TOSCA_ERR_UNSUPPORTED, "div_scheme = %d: only central (0) and QUICK (1) are supported", p->div_scheme);
+    set_geo_per(c, p);
+    const Geo &g = c->geo;
+    const size_t n = (size_t)g.nloc();
+    if ((rc = check_mesh(c, m, true, true, true))) return rc;
+    if ((rc = check_dev(c, ucont, 3 * n * sizeof(double), "ucont")) || (rc = check_dev(c, ucat, 3 * n * sizeof(double), "ucat")))
+        return rc;
+    const bool visc = form_mode != 1;
+    if (p->les_active && visc) {
+        if ((rc = check_les_model(c, p))) return rc;
+        if ((rc = check_dev(c, nut, n * sizeof(double), "nut"))) return rc;
+        if (p->bc_u[0] == TOSCA_BCU_WALLFUNCTION || p->bc_u[1] == TOSCA_BCU_WALLFUNCTION)
+            if ((rc = check_dev(c, p->tau_wall_i, 3 * (size_t)g.mz * g.my * sizeof(double), "tau_wall_i"))) return rc;
+        if (p->bc_u[2] == TOSCA_BCU_WALLFUNCTION || p->bc_u[3] == TOSCA_BCU_WALLFUNCTION)
+            if ((rc = check_dev(c, p->tau_wall_j, 3 * (size_t)g.mz * g.mx * sizeof(double), "tau_wall_j"))) return rc;
+    }
+    if ((rc = check_dev(c, rhs, 3 * (size_t)g.mx * g.my * g.mz * sizeof(double), "rhs"))) return rc;
+    if ((rc = ensure_ws(c, ws_need(c)))) return rc;
+    tosca::ref_formu(g, m, p, ucont, ucat, (p->les_active && visc) ? nut : nullptr, rhs, scale, form_mode, c->ws, c->stream);
+    return launched(c, "FormU");
+}
+
+int tosca_update_cs(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const double *ucat, double *cs) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if ((rc = check_params(c, p)) || (rc = check_les_model(c, p))) return rc;
+    set_geo_per(c, p);
+    const size_t n = (size_t)c->geo.nloc();
+    if ((rc = check_dev(c, cs, n * sizeof(double), "cs"))) return rc;
+    if (p->les_model != TOSCA_LES_SMAGORINSKY) {
+        if ((rc = check_mesh(c, m, false, true, false))) return rc;
+        if ((rc = check_dev(c, ucat, 3 * n * sizeof(double), "ucat"))) return rc;
+    }
+    if ((rc = ensure_ws(c, ws_need(c)))) return rc;
+    tosca::ref_update_cs(c->geo, m, p, ucat, cs, c->ws, c->stream);
+    return launched(c, "UpdateCs");
+}
+
+int tosca_update_nut(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const double *ucat, const double *cs,
+                     double *nut, double *ksgs) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if ((rc = check_params(c, p)) || (rc = check_les_model(c, p))) return rc;
+    set_geo_per(c, p);
+    const size_t n = (size_t)c->geo.nloc();
+    if ((rc = check_mesh(c, m, false, true, false))) return rc;
+    if ((rc = check_dev(c, ucat, 3 * n * sizeof(double), "ucat")) || (rc = check_dev(c, cs, n * sizeof(double), "cs")) ||
+        (rc = check_dev(c, nut, n * sizeof(double), "nut")))
+        return rc;
+    if (ksgs && (rc = check_dev(c, ksgs, n * sizeof(double), "ksgs"))) return rc;
+    tosca::ref_update_nut(c->geo, m, p, ucat, cs, nut, ksgs, c->stream);
+    return launched(c, "UpdateNut");
+}
+
+int tosca_buoyancy(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const double *theta, double scale,
+                   double *btheta, double *gcont) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if ((rc = check_params(c, p))) return rc;
+    set_geo_per(c, p);
+    const Geo &g = c->geo;
+    const size_t n = (size_t)g.nloc(), ng = 3 * (size_t)g.mx * g.my * g.mz * sizeof(double);
+    if (!m) return set_err(c, TOSCA_ERR_ARG, "mesh is NULL");
+    if ((rc = check_dev(c, m->icsi, 3 * n * sizeof(double), "mesh.icsi")) ||
+        (rc = check_dev(c, m->jeta, 3 * n * sizeof(double), "mesh.jeta")) ||
+        (rc = check_dev(c, m->kzet, 3 * n * sizeof(double), "mesh.kzet")) ||
+        (rc = check_dev(c, theta, n * sizeof(double), "theta")) || (rc = check_dev(c, btheta, ng, "btheta")))
+        return rc;
+    if (gcont && (rc = check_dev(c, gcont, ng, "gcont"))) return rc;
+    if (!(p->t_ref != 0.0)) return set_err(c, TOSCA_ERR_ARG, "t_ref must be non-zero");
+    tosca::ref_buoyancy(g, m, p, theta, scale, btheta, gcont, c->stream);
+    return launched(c, "Buoyancy");
+}
+
+int tosca_rhs_les_step(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const double *ucont,
+                       const double *ucat, const double *theta, double *rhs, double scale, double *cs, double *nut,
+                       double *btheta, double buoy_scale) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if ((rc = check_params(c, p))) return rc;
+    if (!p->les_active) return set_err(c, TOSCA_ERR_ARG, "tosca_rhs_les_step needs les_active = 1");
+    if ((rc = tosca_update_cs(c, m, p, ucat, cs))) return rc;
+    if ((rc = tosca_update_nut(c, m, p, ucat, cs, nut, nullptr))) return rc;
+    if ((rc = tosca_buoyancy(c, m, p, theta, buoy_scale, btheta, nullptr))) return rc;
+    return tosca_formu(c, m, p, ucont, ucat, nut, rhs, scale, 0);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------------------
+// mesh check kernel: one block per k plane of the owned range, fixed-size per-plane counters
+// ---------------------------------------------------------------------------------------------
+namespace tosca {
+
+__global__ void k_mesh_check(Geo g, const double *nvert, const double *meshtag, const double *aj, const double *csi,
+                             const double *eta, const double *zet, int *out) {
+    const int k = blockIdx.x;
+    int cnt[4] = {0, 0, 0, 0};
+    const int n = g.my * g.mx;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+        int j = t / g.mx, i = t % g.mx;
+        int c = g.L(k, j, i);
+        if (nvert[c] != 0.0) cnt[0]++;
+        if (meshtag[c] != 0.0) cnt[1]++;
+        bool in = i > 0 && i < g.mx - 1 && j > 0 && j < g.my - 1 && k > 0 && k < g.mz - 1;
+        if (in) {
+            if (!(aj[c] >= 1e-20)) cnt[2]++;
+            V3 a = ld3(csi, c), b = ld3(eta, c), d = ld3(zet, c);
+            double na = sqrt(a.x * a.x + a.y * a.y + a.z * a.z), nb = sqrt(b.x * b.x + b.y * b.y + b.z * b.z),
+                   nd = sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
+            if (!(na >= 1e-20) || !(nb >= 1e-20) || !(nd >= 1e-20)) cnt[3]++;
+        }
+    }
+    __shared__ int red[4][256];
+    for (int q = 0; q < 4; q++) red[q][threadIdx.x] = cnt[q];
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+            for (int q = 0; q < 4; q++) red[q][threadIdx.x] += red[q][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 4; q++) out[4 * k + q] = red[q][0];
+}
+
+void launch_mesh_check(const Geo &g, const tosca_mesh *m, int *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_mesh_check, dim3(g.mz), dim3(256), 0, s, g, m->nvert, m->meshtag, m->aj, m->csi, m->eta, m->zet, out);
+}
+
+}  // namespace tosca

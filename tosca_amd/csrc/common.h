@@ -1,0 +1,231 @@
+// common.h -- device-side building blocks shared by the gfx950 kernels of the momentum
+// right-hand side + LES library.  Index conventions are those of the reference's PETSc DMDA
+// local arrays: [k][j][i], i fastest, ghost width 3, vectors AoS {x,y,z}.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/tosca_rhs.h"
+
+namespace tosca {
+
+constexpr int GW = 3;   // DMDA ghost width (src/mesh.c:42)
+
+struct V3 { double x, y, z; };
+
+// Grid geometry passed by value to every kernel.
+struct Geo {
+    int mx, my, mz;          // DMDA sizes
+    int gxm, gym, gzm;       // ghosted sizes (m + 6)
+    int per[3];              // periodic type on i, j, k
+    // local (ghosted) linear index of DMDA point (k, j, i)
+    __host__ __device__ __forceinline__ int L(int k, int j, int i) const {
+        return ((k + GW) * gym + (j + GW)) * gxm + (i + GW);
+    }
+    // global (owned range) linear index
+    __host__ __device__ __forceinline__ int G(int k, int j, int i) const { return (k * my + j) * mx + i; }
+    __host__ __device__ __forceinline__ bool nonper(int ax) const { return per[ax] == TOSCA_PERIODIC_NONE; }
+    __host__ __device__ __forceinline__ int nloc() const { return gxm * gym * gzm; }
+};
+
+__device__ __forceinline__ V3 ld3(const double *p, int c) { return V3{p[3 * c], p[3 * c + 1], p[3 * c + 2]}; }
+__device__ __forceinline__ void st3(double *p, int c, V3 v) { p[3 * c] = v.x; p[3 * c + 1] = v.y; p[3 * c + 2] = v.z; }
+
+struct Grad { double dudc, dvdc, dwdc, dude, dvde, dwde, dudz, dvdz, dwdz; };
+struct Cart { double du_dx, du_dy, du_dz, dv_dx, dv_dy, dv_dz, dw_dx, dw_dy, dw_dz; };
+
+// Velocity derivatives along the curvilinear directions at an i-face (Compute_du_i,
+// src/include/inline.h:2748-2806) with no IBM / overset cells.
+__device__ __forceinline__ Grad du_i(const Geo &g, const double *u, int i, int j, int k) {
+    const bool iN = g.nonper(0), jN = g.nonper(1), kN = g.nonper(2);
+    Grad r;
+    V3 a = ld3(u, g.L(k, j, i + 1)), b = ld3(u, g.L(k, j, i));
+    r.dudc = a.x - b.x; r.dvdc = a.y - b.y; r.dwdc = a.z - b.z;
+    const bool ib = (i == 0 || i == g.mx - 2);
+    if (j == g.my - 2 && ib && (jN || iN)) {
+        V3 p0 = ld3(u, g.L(k, j, i + 1)), p1 = ld3(u, g.L(k, j, i)), m0 = ld3(u, g.L(k, j - 1, i + 1)), m1 = ld3(u, g.L(k, j - 1, i));
+        r.dude = (p0.x + p1.x - m0.x - m1.x) * 0.5; r.dvde = (p0.y + p1.y - m0.y - m1.y) * 0.5; r.dwde = (p0.z + p1.z - m0.z - m1.z) * 0.5;
+    } else if (j == 1 && ib && (jN || iN)) {
+        V3 p0 = ld3(u, g.L(k, j + 1, i + 1)), p1 = ld3(u, g.L(k, j + 1, i)), m0 = ld3(u, g.L(k, j, i + 1)), m1 = ld3(u, g.L(k, j, i));
+        r.dude = (p0.x + p1.x - m0.x - m1.x) * 0.5; r.dvde = (p0.y + p1.y - m0.y - m1.y) * 0.5; r.dwde = (p0.z + p1.z - m0.z - m1.z) * 0.5;
+    } else {
+        V3 p0 = ld3(u, g.L(k, j + 1, i + 1)), p1 = ld3(u, g.L(k, j + 1, i)), m0 = ld3(u, g.L(k, j - 1, i + 1)), m1 = ld3(u, g.L(k, j - 1, i));
+        r.dude = (p0.x + p1.x - m0.x - m1.x) * 0.25; r.dvde = (p0.y + p1.y - m0.y - m1.y) * 0.25; r.dwde = (p0.z + p1.z - m0.z - m1.z) * 0.25;
+    }
+    if (k == g.mz - 2 && ib && (kN || iN)) {
+        V3 p0 = ld3(u, g.L(k, j, i + 1)), p1 = ld3(u, g.L(k, j, i)), m0 = ld3(u, g.L(k - 1, j, i + 1)), m1 = ld3(u, g.L(k - 1, j, i));
+        r.dudz = (p0.x + p1.x - m0.x - m1.x) * 0.5; r.dvdz = (p0.y + p1.y - m0.y - m1.y) * 0.5; r.dwdz = (p0.z + p1.z - m0.z - m1.z) * 0.5;
+    } else if (k == 1 && ib && (kN || iN)) {
+        V3 p0 = ld3(u, g.L(k + 1, j, i + 1)), p1 = ld3(u, g.L(k + 1, j, i)), m0 = ld3(u, g.L(k, j, i + 1)), m1 = ld3(u, g.L(k, j, i));
+        r.dudz = (p0.x + p1.x - m0.x - m1.x) * 0.5; r.dvdz = (p0.y + p1.y - m0.y - m1.y) * 0.5; r.dwdz = (p0.z + p1.z - m0.z - m1.z) * 0.5;
+    } else {
+        V3 p0 = ld3(u, g.L(k + 1, j, i + 1)), p1 = ld3(u, g.L(k + 1, j, i)), m0 = ld3(u, g.L(k - 1, j, i + 1)), m1 = ld3(u, g.L(k - 1, j, i));
+        r.dudz = (p0.x + p1.x - m0.x - m1.x) * 0.25; r.dvdz = (p0.y + p1.y - m0.y - m1.y) * 0.25; r.dwdz = (p0.z + p1.z - m0.z - m1.z) * 0.25;
+    }
+    return r;
+}
+
+// Compute_du_j (src/include/inline.h:2810-2868)
+__device__ __forceinline__ Grad du_j(const Geo &g, const double *u, int i, int j, int k) {
+    const bool iN = g.nonper(0), jN = g.nonper(1), kN = g.nonper(2);
+    Grad r;
+    const bool jb = (j == 0 || j == g.my - 2);
+    if (i == g.mx - 2 && jb && (iN || jN)) {
+        V3 p0 = ld3(u, g.L(k, j + 1, i)), p1 = ld3(u, g.L(k, j, i)), m0 = ld3(u, g.L(k, j + 1, i - 1)), m1 = ld3(u, g.L(k, j, i - 1));
+        r.dudc = (p0.x + p1.x - m0.x - m1.x) * 0.5; r.dvdc = (p0.y + p1.y - m0.y - m1.y) * 0.5; r.dwdc = (p0.z + p1.z - m0.z - m1.z) * 0.5;
+    } else if (i == 1 && jb && (iN || jN)) {
+        V3 p0 = ld3(u, g.L(k, j + 1, i + 1)), p1 = ld3(u, g.L(k, j, i + 1)), m0 = ld3(u, g.L(k, j + 1, i)), m1 = ld3(u, g.L(k, j, i));
+        r.dudc = (p0.x + p1.x - m0.x - m1.x) * 0.5; r.dvdc = (p0.y + p1.y - m0.y - m1.y) * 0.5; r.dwdc = (p0.z + p1.z - m0.z - m1.z) * 0.5;
+    } else {
+        V3 p0 = ld3(u, g.L(k, j + 1, i + 1)), p1 = ld3(u, g.L(k, j, i + 1)), m0 = ld3(u, g.L(k, j + 1, i - 1)), m1 = ld3(u, g.L(k, j, i - 1));
+        r.dudc = (p0.x + p1.x - m0.x - m1.x) * 0.25; r.dvdc = (p0.y + p1.y - m0.y - m1.y) * 0.25; r.dwdc = (p0.z + p1.z - m0.z - m1.z) * 0.25;
+    }
+    V3 a = ld3(u, g.L(k, j + 1, i)), b = ld3(u, g.L(k, j, i));
+    r.dude = a.x - b.x; r.dvde = a.y - b.y; r.dwde = a.z - b.z;
+    if (k == g.mz - 2 && jb && (kN || jN)) {
+        V3 p0 = ld3(u, g.L(k, j + 1, i)), p1 = ld3(u, g.L(k, j, i)), m0 = ld3(u, g.L(k - 1, j + 1, i)), m1 = ld3(u, g.L(k - 1, j, i));
+        r.dudz = (p0.x + p1.x - m0.x - m1.x) * 0.5; r.dvdz = (p0.y + p1.y - m0.y - m1.y) * 0.5; r.dwdz = (p0.z + p1.z - m0.z - m1.z) * 0.5;
+    } else if (k == 1 && jb && (kN || jN)) {
+        V3 p0 = ld3(u, g.L(k + 1, j + 1, i)), p1 = ld3(u, g.L(k + 1, j, i)), m0 = ld3(u, g.L(k, j + 1, i)), m1 = ld3(u, g.L(k, j, i));
+        r.dudz = (p0.x + p1.x - m0.x - m1.x) * 0.5; r.dvdz = (p0.y + p1.y - m0.y - m1.y) * 0.5; r.dwdz = (p0.z + p1.z - m0.z - m1.z) * 0.5;
+    } else {
+        V3 p0 = ld3(u, g.L(k + 1, j + 1, i)), p1 = ld3(u, g.L(k + 1, j, i)), m0 = ld3(u, g.L(k - 1, j + 1, i)), m1 = ld3(u, g.L(k - 1, j, i));
+        r.dudz = (p0.x + p1.x - m0.x - m1.x) * 0.25; r.dvdz = (p0.y + p1.y - m0.y - m1.y) * 0.25; r.dwdz = (p0.z + p1.z - m0.z - m1.z) * 0.25;
+    }
+    return r;
+}
+
+// Compute_du_k (src/include/inline.h:2872-2931)
+__device__ __forceinline__ Grad du_k(const Geo &g, const double *u, int i, int j, int k) {
+    const bool iN = g.nonper(0), jN = g.nonper(1), kN = g.nonper(2);
+    Grad r;
+    const bool kb = (k == 0 || k == g.mz - 2);
+    if (i == g.mx - 2 && kb && (iN || kN)) {
+        V3 p0 = ld3(u, g.L(k + 1, j, i)), p1 = ld3(u, g.L(k, j, i)), m0 = ld3(u, g.L(k + 1, j, i - 1)), m1 = ld3(u, g.L(k, j, i - 1));
+        r.dudc = (p0.x + p1.x - m0.x - m1.x) * 0.5; r.dvdc = (p0.y + p1.y - m0.y - m1.y) * 0.5; r.dwdc = (p0.z + p1.z - m0.z - m1.z) * 0.5;
+    } else if (i == 1 && kb && (iN || kN)) {
+        V3 p0 = ld3(u, g.L(k + 1, j, i + 1)), p1 = ld3(u, g.L(k, j, i + 1)), m0 = ld3(u, g.L(k + 1, j, i)), m1 = ld3(u, g.L(k, j, i));
+        r.dudc = (p0.x + p1.x - m0.x - m1.x) * 0.5; r.dvdc = (p0.y + p1.y - m0.y - m1.y) * 0.5; r.dwdc = (p0.z + p1.z - m0.z - m1.z) * 0.5;
+    } else {
+        V3 p0 = ld3(u, g.L(k + 1, j, i + 1)), p1 = ld3(u, g.L(k, j, i + 1)), m0 = ld3(u, g.L(k + 1, j, i - 1)), m1 = ld3(u, g.L(k, j, i - 1));
+        r.dudc = (p0.x + p1.x - m0.x - m1.x) * 0.25; r.dvdc = (p0.y + p1.y - m0.y - m1.y) * 0.25; r.dwdc = (p0.z + p1.z - m0.z - m1.z) * 0.25;
+    }
+    if (j == g.my - 2 && kb && (jN || kN)) {
+        V3 p0 = ld3(u, g.L(k + 1, j, i)), p1 = ld3(u, g.L(k, j, i)), m0 = ld3(u, g.L(k + 1, j - 1, i)), m1 = ld3(u, g.L(k, j - 1, i));
+        r.dude = (p0.x + p1.x - m0.x - m1.x) * 0.5; r.dvde = (p0.y + p1.y - m0.y - m1.y) * 0.5; r.dwde = (p0.z + p1.z - m0.z - m1.z) * 0.5;
+    } else if (j == 1 && kb && (jN || kN)) {
+        V3 p0 = ld3(u, g.L(k + 1, j + 1, i)), p1 = ld3(u, g.L(k, j + 1, i)), m0 = ld3(u, g.L(k + 1, j, i)), m1 = ld3(u, g.L(k, j, i));
+        r.dude = (p0.x + p1.x - m0.x - m1.x) * 0.5; r.dvde = (p0.y + p1.y - m0.y - m1.y) * 0.5; r.dwde = (p0.z + p1.z - m0.z - m1.z) * 0.5;
+    } else {
+        V3 p0 = ld3(u, g.L(k + 1, j + 1, i)), p1 = ld3(u, g.L(k, j + 1, i)), m0 = ld3(u, g.L(k + 1, j - 1, i)), m1 = ld3(u, g.L(k, j - 1, i));
+        r.dude = (p0.x + p1.x - m0.x - m1.x) * 0.25; r.dvde = (p0.y + p1.y - m0.y - m1.y) * 0.25; r.dwde = (p0.z + p1.z - m0.z - m1.z) * 0.25;
+    }
+    V3 a = ld3(u, g.L(k + 1, j, i)), b = ld3(u, g.L(k, j, i));
+    r.dudz = a.x - b.x; r.dvdz = a.y - b.y; r.dwdz = a.z - b.z;
+    return r;
+}
+
+// Cell-centred velocity derivatives (Compute_du_center, src/include/inline.h:3344-3418).
+__device__ __forceinline__ Grad du_center(const Geo &g, const double *u, int i, int j, int k) {
+    Grad r;
+    V3 c = ld3(u, g.L(k, j, i));
+    {
+        V3 p = ld3(u, g.L(k, j, i + 1)), m = ld3(u, g.L(k, j, i - 1));
+        if (g.nonper(0) && i == g.mx - 2) { r.dudc = c.x - m.x; r.dvdc = c.y - m.y; r.dwdc = c.z - m.z; }
+        else if (g.nonper(0) && i == 1) { r.dudc = p.x - c.x; r.dvdc = p.y - c.y; r.dwdc = p.z - c.z; }
+        else { r.dudc = (p.x - m.x) * 0.5; r.dvdc = (p.y - m.y) * 0.5; r.dwdc = (p.z - m.z) * 0.5; }
+    }
+    {
+        V3 p = ld3(u, g.L(k, j + 1, i)), m = ld3(u, g.L(k, j - 1, i));
+        if (g.nonper(1) && j == g.my - 2) { r.dude = c.x - m.x; r.dvde = c.y - m.y; r.dwde = c.z - m.z; }
+        else if (g.nonper(1) && j == 1) { r.dude = p.x - c.x; r.dvde = p.y - c.y; r.dwde = p.z - c.z; }
+        else { r.dude = (p.x - m.x) * 0.5; r.dvde = (p.y - m.y) * 0.5; r.dwde = (p.z - m.z) * 0.5; }
+    }
+    {
+        V3 p = ld3(u, g.L(k + 1, j, i)), m = ld3(u, g.L(k - 1, j, i));
+        if (g.nonper(2) && k == g.mz - 2) { r.dudz = c.x - m.x; r.dvdz = c.y - m.y; r.dwdz = c.z - m.z; }
+        else if (g.nonper(2) && k == 1) { r.dudz = p.x - c.x; r.dvdz = p.y - c.y; r.dwdz = p.z - c.z; }
+        else { r.dudz = (p.x - m.x) * 0.5; r.dvdz = (p.y - m.y) * 0.5; r.dwdz = (p.z - m.z) * 0.5; }
+    }
+    return r;
+}
+
+// Compute_du_dxyz (src/include/inline.h:3635-3663)
+__device__ __forceinline__ Cart du_dxyz(V3 c, V3 e, V3 z, double ajc, const Grad &g) {
+    Cart r;
+    r.du_dx = (g.dudc * c.x + g.dude * e.x + g.dudz * z.x) * ajc;
+    r.du_dy = (g.dudc * c.y + g.dude * e.y + g.dudz * z.y) * ajc;
+    r.du_dz = (g.dudc * c.z + g.dude * e.z + g.dudz * z.z) * ajc;
+    r.dv_dx = (g.dvdc * c.x + g.dvde * e.x + g.dvdz * z.x) * ajc;
+    r.dv_dy = (g.dvdc * c.y + g.dvde * e.y + g.dvdz * z.y) * ajc;
+    r.dv_dz = (g.dvdc * c.z + g.dvde * e.z + g.dvdz * z.z) * ajc;
+    r.dw_dx = (g.dwdc * c.x + g.dwde * e.x + g.dwdz * z.x) * ajc;
+    r.dw_dy = (g.dwdc * c.y + g.dwde * e.y + g.dwdz * z.y) * ajc;
+    r.dw_dz = (g.dwdc * c.z + g.dwde * e.z + g.dwdz * z.z) * ajc;
+    return r;
+}
+
+// |S| = sqrt(2 S_ij S_ij) (src/les.c:342-354)
+__device__ __forceinline__ double strain_mag(const Cart &d) {
+    double Sxx = 0.5 * (d.du_dx + d.du_dx), Sxy = 0.5 * (d.du_dy + d.dv_dx), Sxz = 0.5 * (d.du_dz + d.dw_dx);
+    double Syy = 0.5 * (d.dv_dy + d.dv_dy), Syz = 0.5 * (d.dv_dz + d.dw_dy), Szz = 0.5 * (d.dw_dz + d.dw_dz);
+    return sqrt(2.0 * (Sxx * Sxx + Sxy * Sxy + Sxz * Sxz + Sxy * Sxy + Syy * Syy + Syz * Syz +
+                       Sxz * Sxz + Syz * Syz + Szz * Szz));
+}
+
+// The viscous flux of one face (ueqn_solvers.c:196-215 + 386-392 and their j/k twins): dir is the
+// face direction (0 = i, 1 = j, 2 = k); csi/eta/zet are the face's contravariant basis.
+__device__ __forceinline__ V3 visc_flux(const Grad &g, V3 c, V3 e, V3 z, int dir, double ajc, double nuEff) {
+    double g11, g21, g31, n0, n1, n2;
+    if (dir == 0) {
+        g11 = c.x * c.x + c.y * c.y + c.z * c.z; g21 = e.x * c.x + e.y * c.y + e.z * c.z; g31 = z.x * c.x + z.y * c.y + z.z * c.z;
+        n0 = c.x; n1 = c.y; n2 = c.z;
+    } else if (dir == 1) {
+        g11 = c.x * e.x + c.y * e.y + c.z * e.z; g21 = e.x * e.x + e.y * e.y + e.z * e.z; g31 = z.x * e.x + z.y * e.y + z.z * e.z;
+        n0 = e.x; n1 = e.y; n2 = e.z;
+    } else {
+        g11 = c.x * z.x + c.y * z.y + c.z * z.z; g21 = e.x * z.x + e.y * z.y + e.z * z.z; g31 = z.x * z.x + z.y * z.y + z.z * z.z;
+        n0 = z.x; n1 = z.y; n2 = z.z;
+    }
+    double r11 = g.dudc * c.x + g.dude * e.x + g.dudz * z.x;
+    double r21 = g.dvdc * c.x + g.dvde * e.x + g.dvdz * z.x;
+    double r31 = g.dwdc * c.x + g.dwde * e.x + g.dwdz * z.x;
+    double r12 = g.dudc * c.y + g.dude * e.y + g.dudz * z.y;
+    double r22 = g.dvdc * c.y + g.dvde * e.y + g.dvdz * z.y;
+    double r32 = g.dwdc * c.y + g.dwde * e.y + g.dwdz * z.y;
+    double r13 = g.dudc * c.z + g.dude * e.z + g.dudz * z.z;
+    double r23 = g.dvdc * c.z + g.dvde * e.z + g.dvdz * z.z;
+    double r33 = g.dwdc * c.z + g.dwde * e.z + g.dwdz * z.z;
+    V3 o;
+    o.x = (g11 * g.dudc + g21 * g.dude + g31 * g.dudz + r11 * n0 + r21 * n1 + r31 * n2) * ajc * (nuEff);
+    o.y = (g11 * g.dvdc + g21 * g.dvde + g31 * g.dvdz + r12 * n0 + r22 * n1 + r32 * n2) * ajc * (nuEff);
+    o.z = (g11 * g.dwdc + g21 * g.dwde + g31 * g.dwdz + r13 * n0 + r23 * n1 + r33 * n2) * ajc * (nuEff);
+    return o;
+}
+
+// QUICK (quadraticUpwindVec, src/include/inline.h:5089-5109): upwind on the sign of the flux.
+__device__ __forceinline__ V3 quick(V3 f0, V3 f1, V3 f2, V3 f3, double ws) {
+    V3 UU, U, D;
+    if (ws > 0) { UU = f0; U = f1; D = f2; } else { UU = f3; U = f2; D = f1; }
+    V3 r;
+    r.x = 0.5 * (U.x + D.x) + 0.125 * (2.0 * U.x - (D.x + UU.x));
+    r.y = 0.5 * (U.y + D.y) + 0.125 * (2.0 * U.y - (D.y + UU.y));
+    r.z = 0.5 * (U.z + D.z) + 0.125 * (2.0 * U.z - (D.z + UU.z));
+    return r;
+}
+
+// Outer cells of the 4-cell QUICK stencil of face f (getFace2Cell4Stencil*, inline.h:2190-2380)
+__device__ __forceinline__ void face2cell4(int ptype, int f, int m, int &L, int &R) {
+    L = f - 1; R = f + 2;
+    if (f == 0 || f == m - 2) {
+        if (ptype == TOSCA_PERIODIC_COPY) { L = m - 3; R = 2; }
+        else if (ptype == TOSCA_PERIODIC_WRAP) { if (f == m - 2) { R = m + 2; L = f - 1; } else { L = -3; R = f + 2; } }
+        else { if (f == 0) { L = f; R = f + 2; } else { L = f - 1; R = f + 1; } }
+    }
+}
+
+__device__ __forceinline__ double pmax(double a, double b) { return (a < b) ? b : a; }
+__device__ __forceinline__ double pmin(double a, double b) { return (a < b) ? a : b; }
+
+}  // namespace tosca

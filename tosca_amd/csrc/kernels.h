@@ -1,0 +1,26 @@
+// kernels.h -- host-side launch entry points of the gfx950 kernels (internal to the library).
+#pragma once
+
+#include "common.h"
+
+namespace tosca {
+
+// ghost handling (ref_kernels.hip)
+void launch_wrap(double *a, int ncomp, const Geo &g, hipStream_t s);
+void launch_reset_cell_periodic(double *a, int ncomp, const Geo &g, hipStream_t s);
+void launch_fill(double *a, long n, double v, hipStream_t s);
+
+// reference-structured path (ref_kernels.hip); `ws` is a workspace of >= 22 * nloc doubles
+void ref_formu(const Geo &g, const tosca_mesh *m, const tosca_params *p, const double *ucont, const double *ucat,
+               const double *nut, double *rhs, double scale, int mode, double *ws, hipStream_t s);
+void ref_update_cs(const Geo &g, const tosca_mesh *m, const tosca_params *p, const double *ucat, double *cs,
+                   double *ws, hipStream_t s);
+void ref_update_nut(const Geo &g, const tosca_mesh *m, const tosca_params *p, const double *ucat, const double *cs,
+                    double *nut, double *ksgs, hipStream_t s);
+void ref_buoyancy(const Geo &g, const tosca_mesh *m, const tosca_params *p, const double *theta, double scale,
+                  double *btheta, double *gcont, hipStream_t s);
+
+// mesh validation (api.hip): counts of offending points, written to `out` (device, 4 ints)
+void launch_mesh_check(const Geo &g, const tosca_mesh *m, int *out, hipStream_t s);
+
+}  // namespace tosca

@@ -1,0 +1,104 @@
+"""Host-side mirror of the reference's momentum-RHS operator interface over the C ABI.
+
+    ctx = RhsContext(case.lay.info())
+    ctx.UpdateCs(mesh, params, ucat, cs)                          # src/les.c:165
+    ctx.UpdateNut(mesh, params, ucat, cs, nut)                    # src/les.c:2119
+    ctx.Buoyancy(mesh, params, theta, 1.0, btheta)                # src/sources/ueqn_sources.c:2726
+    ctx.FormU(mesh, params, ucont, ucat, nut, rhs, 1.0, 0)        # src/solvers/ueqn_solvers.c:6
+
+Arrays are torch CUDA float64 tensors in the DMDA local layout (see include/tosca_rhs.h); the
+calls queue work on torch's current stream and raise ToscaError on a non-zero return code.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Dict, Mapping, Optional
+
+import torch
+
+from . import abi
+
+
+def _ptr(t: Optional[torch.Tensor]) -> Optional[int]:
+    if t is None:
+        return None
+    if not isinstance(t, torch.Tensor):
+        raise TypeError("expected a torch tensor")
+    if t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous():
+        raise ValueError("arrays must be contiguous float64 CUDA tensors")
+    return t.data_ptr()
+
+
+def mesh_struct(mesh: Mapping[str, torch.Tensor]) -> abi.Mesh:
+    return abi.make_mesh({n: _ptr(mesh[n]) for n in abi.MESH_FIELDS if n in mesh})
+
+
+class RhsContext:
+    """One tosca_ctx (workspace + stream) for one DMDA layout on one GPU."""
+
+    def __init__(self, info: Mapping[str, int], device: int = 0, lib=None):
+        self.lib = lib or abi.load()
+        self.info = abi.make_info(info)
+        h = C.c_void_p()
+        rc = self.lib.tosca_ctx_create(C.byref(h), C.byref(self.info), int(device))
+        if rc != abi.TOSCA_OK:
+            raise abi.ToscaError(rc, self.lib.tosca_last_error(None).decode())
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.tosca_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int):
+        if rc != abi.TOSCA_OK:
+            raise abi.ToscaError(rc, self.lib.tosca_last_error(self.h).decode())
+
+    def _stream(self):
+        self._check(self.lib.tosca_ctx_set_stream(self.h, C.c_void_p(torch.cuda.current_stream().cuda_stream)))
+
+    def synchronize(self):
+        self._check(self.lib.tosca_ctx_synchronize(self.h))
+
+    # ---- the four reference operators ---------------------------------------------------------
+    def FormU(self, mesh, params, ucont, ucat, nut, rhs, scale: float = 1.0, form_mode: int = 0):
+        self._stream()
+        m, p = mesh_struct(mesh), abi.make_params(params)
+        self._check(self.lib.tosca_formu(self.h, C.byref(m), C.byref(p), _ptr(ucont), _ptr(ucat), _ptr(nut),
+                                         _ptr(rhs), float(scale), int(form_mode)))
+
+    def UpdateCs(self, mesh, params, ucat, cs):
+        self._stream()
+        m, p = mesh_struct(mesh), abi.make_params(params)
+        self._check(self.lib.tosca_update_cs(self.h, C.byref(m), C.byref(p), _ptr(ucat), _ptr(cs)))
+
+    def UpdateNut(self, mesh, params, ucat, cs, nut, ksgs=None):
+        self._stream()
+        m, p = mesh_struct(mesh), abi.make_params(params)
+        self._check(self.lib.tosca_update_nut(self.h, C.byref(m), C.byref(p), _ptr(ucat), _ptr(cs), _ptr(nut),
+                                              _ptr(ksgs)))
+
+    def Buoyancy(self, mesh, params, theta, scale, btheta, gcont=None):
+        self._stream()
+        m, p = mesh_struct(mesh), abi.make_params(params)
+        self._check(self.lib.tosca_buoyancy(self.h, C.byref(m), C.byref(p), _ptr(theta), float(scale),
+                                            _ptr(btheta), _ptr(gcont)))
+
+    def rhs_les_step(self, mesh, params, ucont, ucat, theta, rhs, scale, cs, nut, btheta, buoy_scale=1.0):
+        self._stream()
+        m, p = mesh_struct(mesh), abi.make_params(params)
+        self._check(self.lib.tosca_rhs_les_step(self.h, C.byref(m), C.byref(p), _ptr(ucont), _ptr(ucat), _ptr(theta),
+                                                _ptr(rhs), float(scale), _ptr(cs), _ptr(nut), _ptr(btheta),
+                                                float(buoy_scale)))
+
+    def mesh_check(self, mesh):
+        self._stream()
+        m = mesh_struct(mesh)
+        self._check(self.lib.tosca_mesh_check(self.h, C.byref(m)))
