@@ -1,0 +1,211 @@
+#!/usr/bin/env python3
+"""Benchmark: momentum right-hand side + LES cell-updates/s on one MI355X.
+
+One step = one evaluation of UpdateCs -> UpdateNut -> Buoyancy -> FormU (formMode 0) over every
+interior cell of the workload (tosca_rhs_les_step), with all inputs resident in HBM.
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--config 5] [--path fused|ref]
+
+Prints ONE JSON line (rank 0).  Only one GPU is supported: --gpus > 1 exits with a message.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "RHS+LES cell-updates/s (FP64 vs mixed) at 1 GPU, %HBM roofline, VJP/fwd time"
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes per interior cell of each kernel: compulsory traffic of the arrays the kernel
+# reads (once) and writes (once), 8-byte doubles, a read-modify-write output counted twice.
+# Cell metrics csi/eta/zet = 72 B, face metrics i/j/k x csi/eta/zet = 216 B, face Jacobians 24 B.
+KERNEL_BYTES = {
+    # reference-structured path
+    "k_formu_faces": 24 + 24 + 216 + 24 + 8 + 144,        # ucont, ucat, face metrics, iaj/jaj/kaj, nut -> 6 flux vectors
+    "k_formu_cells": 144 + 24,                            # 6 flux vectors -> fp
+    "k_formu_dot": 24 + 72 + 24 + 2 * 24,                 # fp, csi/eta/zet, iaj/jaj/kaj, rhs (+=)
+    "k_strain": 24 + 72 + 8 + 80,                         # ucat, cell metrics, aj -> grad u (9) + |S|
+    "k_test_filter3": 24 + 72 + 8 + 80 + 16,              # ucat, cell metrics, aj, strain -> LM, MM
+    "k_box_clip": 16 + 8 + 8,                             # LM, MM, aj -> Cs
+    "k_nut": 24 + 72 + 8 + 8 + 8,                         # ucat, cell metrics, aj, Cs -> nut
+    "k_buoyancy": 8 + 72 + 24,                            # theta, icsi/jeta/kzet -> bTheta
+}
+# Whole step: every ABI input read once, every output written once (rhs read + written);
+# nvert / meshTag are validated once per mesh (tosca_mesh_check) and not read per step.
+STEP_BYTES = (24 + 24 + 216 + 72 + 24 + 8 + 8 + 24) + (24 + 8 + 8 + 24)   # = 464
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", type=int, default=5)
+    ap.add_argument("--path", default=os.environ.get("TOSCA_PATH", "default"))
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU-oracle baseline")
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def build_case(cfg: int, device: str):
+    from tosca_amd import case as C
+    spec = C.CONFIGS[cfg]
+    return spec["builder"](device=device, **spec["kwargs"])
+
+
+def workload_name(cfg: int, case) -> str:
+    lay = case.lay
+    n = f"{lay.mz - 2}x{lay.mx - 2}x{lay.my - 2}"
+    names = {1: "TGV periodic box", 2: "cosine-hill curvilinear", 3: "stretched ABL box",
+             4: "cosine-hill curvilinear", 5: "ABL shard (dx=dy=10 m, stretched z)"}
+    return f"{names[cfg]} {n} cells (x,y,z)"
+
+
+def cpu_baseline(cfg_case_kwargs, seconds: float):
+    """Times the CPU oracle on a bounded sample of the same workload: the same case builder and
+    physics on a smaller grid, all host threads (OpenMP over k slabs) and one thread."""
+    import numpy as np
+    from tosca_amd import case as C
+    from oracle import oracle as O
+
+    builder, kw = cfg_case_kwargs
+    kw = dict(kw)
+    kw.update(nx=96, ny=96, nz=48)
+    case = builder(device="cpu", **kw)
+    lay = case.lay
+    mesh = {k: v.numpy().copy() for k, v in case.mesh.items() if k != "cent"}
+    ucont, ucat, theta = case.ucont.numpy().copy(), case.ucat.numpy().copy(), case.theta.numpy().copy()
+    gs, gl = lay.gshape, (lay.mz, lay.my, lay.mx, 3)
+    out = dict(rhs=np.zeros(gl), cs=np.zeros(gs), nut=np.zeros(gs), btheta=np.zeros(gl))
+
+    def run(threads, budget):
+        O.set_threads(threads)
+        O.rhs_les_step(lay.info(), mesh, case.params, ucont, ucat, theta, out["rhs"], 1.0, out["cs"], out["nut"],
+                       out["btheta"], 1.0)                            # warm-up
+        t0, reps = time.perf_counter(), 0
+        while True:
+            O.rhs_les_step(lay.info(), mesh, case.params, ucont, ucat, theta, out["rhs"], 1.0, out["cs"],
+                           out["nut"], out["btheta"], 1.0)
+            reps += 1
+            el = time.perf_counter() - t0
+            if el >= budget or reps >= 50:
+                return lay.ncells * reps / el, reps
+
+    ncores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    ncores = max(1, min(ncores, 16))
+    v_all, r_all = run(ncores, 0.6 * seconds)
+    v_one, r_one = run(1, 0.4 * seconds)
+    return {
+        "value": v_all, "unit": "cell-updates/s", "cores": ncores, "kind": "port",
+        "value_1core": v_one,
+        "sample": f"CPU oracle (reference-structured C++ restatement, g++ -O3 -ffp-contract=off, OpenMP over k) on "
+                  f"the same case builder and physics at {lay.mz - 2}x{lay.mx - 2}x{lay.my - 2} = {lay.ncells} cells; "
+                  f"{r_all} steps on {ncores} threads, {r_one} steps on 1 thread",
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
+              f"(the multi-GPU sharding is a design in DESIGN.md, not a measured path)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from tosca_amd import case as C
+    from tosca_amd.rhs import RhsContext
+
+    if not torch.cuda.is_available():
+        print("bench.py: no HIP device", file=sys.stderr)
+        sys.exit(1)
+    if args.path != "default":
+        os.environ["TOSCA_PATH"] = args.path
+    dev = "cuda"
+    t0 = time.time()
+    case = build_case(args.config, dev)
+    lay = case.lay
+    mesh = {k: v.contiguous() for k, v in case.mesh.items() if k != "cent"}
+    ucont, ucat, theta = case.ucont.contiguous(), case.ucat.contiguous(), case.theta.contiguous()
+    del case.mesh["cent"]
+    gs, gl = lay.gshape, (lay.mz, lay.my, lay.mx, 3)
+    rhs = torch.zeros(gl, dtype=torch.float64, device=dev)
+    cs = torch.zeros(gs, dtype=torch.float64, device=dev)
+    nut = torch.zeros(gs, dtype=torch.float64, device=dev)
+    btheta = torch.zeros(gl, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    setup_s = time.time() - t0
+    print(f"# case built in {setup_s:.1f} s: {lay} cells={lay.ncells}", file=sys.stderr, flush=True)
+
+    ctx = RhsContext(lay.info())
+    ctx.mesh_check(mesh)
+
+    def step():
+        ctx.rhs_les_step(mesh, case.params, ucont, ucat, theta, rhs, 1.0, cs, nut, btheta, 1.0)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    # timed region: exactly K steps between two synchronisations
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(args.steps):
+        step()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    value = lay.ncells / (ms * 1e-3)
+    print(f"# {ms:.3f} ms/step, {value:.3e} cells/s", file=sys.stderr, flush=True)
+
+    # per-kernel HIP-event timing on the launch stream (a second pass over K steps)
+    ctx.enable_timing(True)
+    ctx.reset_timing()
+    for _ in range(args.steps):
+        step()
+    tim = ctx.timings()
+    ctx.enable_timing(False)
+    per = {k: (v[0] / v[1], v[1] // args.steps) for k, v in tim.items()}
+    for k, (avg, n) in sorted(per.items(), key=lambda kv: -kv[1][0] * kv[1][1]):
+        print(f"#   {k:24s} {avg:9.4f} ms x {n}/step", file=sys.stderr, flush=True)
+    dom = max(per, key=lambda k: per[k][0] * per[k][1])
+    dom_ms = per[dom][0]
+    dom_bytes = KERNEL_BYTES.get(dom)
+    achieved = dom_bytes * lay.ncells / (dom_ms * 1e-3) / 1e9 if dom_bytes else None
+    traffic = None
+    tfile = ROOT / "profiles" / f"traffic_{dom}_config{args.config}.json"
+    if tfile.exists():
+        traffic = json.loads(tfile.read_text()).get("bytes_per_launch")
+    roof = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,
+            "algorithmic_bytes_per_cell": dom_bytes, "avg_launch_ms": dom_ms}
+    step_gbs = STEP_BYTES * value / 1e9
+
+    line = {
+        "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": workload_name(args.config, case), "cells": lay.ncells,
+                   "les_model": {0: "smagorinsky", 2: "dynamicSmagorinsky", 5: "dynamicPASD"}[case.params["les_model"]],
+                   "div_scheme": {0: "central", 1: "quick"}[case.params["div_scheme"]],
+                   "path": os.environ.get("TOSCA_PATH", "default"), "parallelism": "single GPU"},
+        "roofline": roof,
+        "step_roofline": {"bound": "hbm", "algorithmic_bytes_per_cell": STEP_BYTES, "achieved": step_gbs,
+                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": step_gbs / HBM_PEAK_GBS},
+        "kernels_ms": {k: v[0] for k, v in per.items()},
+    }
+    if not args.no_cpu:
+        spec = C.CONFIGS[args.config]
+        line["cpu_baseline"] = cpu_baseline((spec["builder"], spec["kwargs"]), args.cpu_seconds)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -113,6 +113,16 @@ int         tosca_ctx_synchronize(tosca_ctx *ctx);
 const char *tosca_last_error(const tosca_ctx *ctx);
 const char *tosca_version(void);
 
+/* Optional per-kernel timing (instrumentation; no counterpart in the reference, whose solvers
+ * print PetscTime wall timers).  When enabled, every kernel the context queues is bracketed by
+ * two hipEvents on its stream; the readers synchronise and return, per kernel name, the total
+ * event time in milliseconds and the number of launches since the last reset. */
+int tosca_ctx_enable_timing(tosca_ctx *ctx, int on);
+int tosca_ctx_timing_reset(tosca_ctx *ctx);
+int tosca_ctx_timing_count(tosca_ctx *ctx);   /* number of kernel names, or -1 on error */
+int tosca_ctx_timing_entry(tosca_ctx *ctx, int idx, char *name, int name_len, double *total_ms,
+                           int64_t *launches);
+
 /* Validates the static mesh once (replaces the checks of SetMeshMetrics, src/mesh.c:1306-1346,
  * and the IBM/overset flags): nvert and meshtag all zero, aj > 1e-20 and |csi|,|eta|,|zet| >= 1e-20
  * on non-corner cells.  Synchronous.  Returns TOSCA_ERR_MESH with a message on failure. */

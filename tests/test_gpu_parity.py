@@ -19,12 +19,21 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-12
 
 
-@pytest.fixture(scope="module")
-def rhs_mod():
+@pytest.fixture(scope="module", params=["fused", "ref"])
+def rhs_mod(request):
+    """The library module, with contexts created for the fused (default) or the
+    reference-structured kernel path."""
     if not torch.cuda.is_available():
         pytest.skip("no HIP device")
+    import os
     from tosca_amd import rhs
-    return rhs
+    old = os.environ.get("TOSCA_PATH")
+    os.environ["TOSCA_PATH"] = request.param
+    yield rhs
+    if old is None:
+        os.environ.pop("TOSCA_PATH", None)
+    else:
+        os.environ["TOSCA_PATH"] = old
 
 
 def _dev_mesh(mesh):
@@ -137,3 +146,32 @@ def test_step_hill(rhs_mod, model, iper):
 
 def test_step_abl(rhs_mod):
     _step_parity(rhs_mod, C.abl_case(nx=20, ny=12, nz=10, n_modes=64), "abl smag central")
+
+
+# multi-tile cases: several 62-wide i tiles, several j tiles and several k chunks, so the tile
+# seams, the periodic seams and the k-chunk priming of the fused kernels are all exercised
+@pytest.mark.parametrize("periodic", [(2, 0, 2), (1, 1, 0), (0, 0, 0)])
+@pytest.mark.parametrize("scheme", [C.DIV_CENTRAL, C.DIV_QUICK])
+def test_formu_random_multitile(rhs_mod, periodic, scheme):
+    lay, mesh, f, params = random_case(n=(37, 130, 14), periodic=periodic, seed=11, div_scheme=scheme, amp=0.1)
+    print(f"\nFormU multi-tile periodic={periodic} scheme={scheme} {lay}", flush=True)
+    out = out_arrays(lay)
+    nm = np_mesh(mesh)
+    rhs_o = out["rhs"].copy()
+    O.formu(lay.info(), nm, params, to_np(f["ucont"]), to_np(f["ucat"]), to_np(f["nut"]), rhs_o, 1.0, 0)
+    ctx = rhs_mod.RhsContext(lay.info())
+    rhs_g = to_dev(out["rhs"])
+    ctx.FormU(_dev_mesh(mesh), params, to_dev(to_np(f["ucont"])), to_dev(to_np(f["ucat"])), to_dev(to_np(f["nut"])),
+              rhs_g, 1.0, 0)
+    torch.cuda.synchronize()
+    assert _report("rhs", to_np(rhs_g), rhs_o) <= TOL
+
+
+@pytest.mark.parametrize("builder,kw", [
+    (C.tgv_case, dict(n=70, ptype=C.PERIODIC_WRAP, les_model=C.LES_DSM, div_scheme=C.DIV_QUICK)),
+    (C.tgv_case, dict(n=66, ptype=C.PERIODIC_COPY, les_model=C.LES_SMAGORINSKY, div_scheme=C.DIV_CENTRAL)),
+    (C.hill_case, dict(nx=40, ny=130, nz=20, les_model=C.LES_DSM)),
+    (C.abl_case, dict(nx=36, ny=70, nz=24, les_model=C.LES_DSM, n_modes=64)),
+])
+def test_step_multitile(rhs_mod, builder, kw):
+    _step_parity(rhs_mod, builder(**kw), f"{builder.__name__} {kw}")

@@ -60,6 +60,10 @@ SIGNATURES = {
     "tosca_ctx_destroy": (I, [P]),
     "tosca_ctx_set_stream": (I, [P, P]),
     "tosca_ctx_synchronize": (I, [P]),
+    "tosca_ctx_enable_timing": (I, [P, I]),
+    "tosca_ctx_timing_reset": (I, [P]),
+    "tosca_ctx_timing_count": (I, [P]),
+    "tosca_ctx_timing_entry": (I, [P, I, C.c_char_p, I, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "tosca_mesh_check": (I, [P, C.POINTER(Mesh)]),
     "tosca_formu": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P, P, D, I]),
     "tosca_update_cs": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P]),

@@ -31,6 +31,8 @@ COMMON = [
 SOURCES = {
     "api.hip": [],
     "ref_kernels.hip": ["-ffp-contract=off"],
+    "fused_formu.hip": [],
+    "fused_les.hip": [],
 }
 
 

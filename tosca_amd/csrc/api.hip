@@ -10,9 +10,10 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <iterator>
 #include <vector>
 
-#include "kernels.h"
+#include "fused.h"
 
 using tosca::Geo;
 
@@ -26,7 +27,13 @@ struct tosca_ctx {
     size_t ws_elems = 0;
     int *chk = nullptr;
     size_t chk_elems = 0;
+    tosca::KernelTimer timer;
+    bool ref_path = false;   // TOSCA_PATH=ref at creation: reference-structured kernels only
 };
+
+namespace tosca {
+thread_local KernelTimer *g_timer = nullptr;
+}
 
 namespace {
 
@@ -75,6 +82,7 @@ int check_dev(tosca_ctx *c, const void *p, size_t need, const char *name) {
 int check_ctx(tosca_ctx *c) {
     if (!c) return set_err(nullptr, TOSCA_ERR_ARG, "context is NULL");
     if (hipSetDevice(c->device) != hipSuccess) return set_err(c, TOSCA_ERR_DEVICE, "hipSetDevice(%d) failed", c->device);
+    tosca::g_timer = &c->timer;
     return TOSCA_OK;
 }
 
@@ -150,6 +158,25 @@ void set_geo_per(tosca_ctx *c, const tosca_params *p) {
     for (int a = 0; a < 3; a++) c->geo.per[a] = p->periodic[a];
 }
 
+tosca::FormUArgs formu_args(const tosca_mesh *m, const tosca_params *p, const double *ucont, const double *ucat,
+                            const double *nut, double *rhs, double scale, int mode) {
+    tosca::FormUArgs A{};
+    A.ucont = ucont; A.ucat = ucat; A.nut = nut;
+    A.csi = m->csi; A.eta = m->eta; A.zet = m->zet;
+    A.icsi = m->icsi; A.ieta = m->ieta; A.izet = m->izet;
+    A.jcsi = m->jcsi; A.jeta = m->jeta; A.jzet = m->jzet;
+    A.kcsi = m->kcsi; A.keta = m->keta; A.kzet = m->kzet;
+    A.iaj = m->iaj; A.jaj = m->jaj; A.kaj = m->kaj;
+    A.tauI = p->tau_wall_i; A.tauJ = p->tau_wall_j;
+    A.rhs = rhs; A.scale = scale; A.mode = mode;
+    A.quick = p->div_scheme == TOSCA_DIV_QUICK;
+    A.les = nut != nullptr;
+    A.nu = p->nu;
+    for (int q = 0; q < 6; q++) A.bc_u[q] = p->bc_u[q];
+    A.theta = nullptr; A.btheta = nullptr; A.bscale = 0.0; A.tRef = 1.0;
+    return A;
+}
+
 size_t ws_need(const tosca_ctx *c) {
     // 22 local fields (FormU: 6 face-flux vectors + fp; UpdateCs: strain 10 + LM/MM/QN/NN)
     // plus the DPASD plane partials
@@ -188,6 +215,8 @@ int tosca_ctx_create(tosca_ctx **out, const tosca_dmda_info *in, int device) {
     if (hipSetDevice(device) != hipSuccess) return set_err(nullptr, TOSCA_ERR_DEVICE, "hipSetDevice(%d) failed", device);
     tosca_ctx *c = new tosca_ctx();
     c->device = device;
+    const char *path = getenv("TOSCA_PATH");
+    c->ref_path = path && strcmp(path, "ref") == 0;
     c->info = *in;
     c->geo.mx = (int)in->mx; c->geo.my = (int)in->my; c->geo.mz = (int)in->mz;
     c->geo.gxm = (int)in->gxm; c->geo.gym = (int)in->gym; c->geo.gzm = (int)in->gzm;
@@ -198,6 +227,8 @@ int tosca_ctx_create(tosca_ctx **out, const tosca_dmda_info *in, int device) {
 int tosca_ctx_destroy(tosca_ctx *c) {
     if (!c) return TOSCA_OK;
     hipSetDevice(c->device);
+    c->timer.release();
+    if (tosca::g_timer == &c->timer) tosca::g_timer = nullptr;
     if (c->ws || c->chk) hipStreamSynchronize(c->stream);
     if (c->ws) hipFree(c->ws);
     if (c->chk) hipFree(c->chk);
@@ -216,6 +247,39 @@ int tosca_ctx_synchronize(tosca_ctx *c) {
     if (rc) return rc;
     hipError_t e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return set_err(c, TOSCA_ERR_DEVICE, "stream synchronize: %s", hipGetErrorString(e));
+    return TOSCA_OK;
+}
+
+int tosca_ctx_enable_timing(tosca_ctx *c, int on) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    c->timer.on = on != 0;
+    return TOSCA_OK;
+}
+
+int tosca_ctx_timing_reset(tosca_ctx *c) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    c->timer.reset();
+    return TOSCA_OK;
+}
+
+int tosca_ctx_timing_count(tosca_ctx *c) {
+    if (check_ctx(c)) return -1;
+    c->timer.collect();
+    return (int)c->timer.acc.size();
+}
+
+int tosca_ctx_timing_entry(tosca_ctx *c, int idx, char *name, int name_len, double *total_ms, int64_t *launches) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    c->timer.collect();
+    if (idx < 0 || idx >= (int)c->timer.acc.size()) return set_err(c, TOSCA_ERR_ARG, "timing index %d out of range", idx);
+    auto it = c->timer.acc.begin();
+    std::advance(it, idx);
+    if (name && name_len > 0) { strncpy(name, it->first.c_str(), name_len - 1); name[name_len - 1] = 0; }
+    if (total_ms) *total_ms = it->second.first;
+    if (launches) *launches = it->second.second;
     return TOSCA_OK;
 }
 
@@ -271,8 +335,13 @@ int tosca_formu(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const 
             if ((rc = check_dev(c, p->tau_wall_j, 3 * (size_t)g.mz * g.mx * sizeof(double), "tau_wall_j"))) return rc;
     }
     if ((rc = check_dev(c, rhs, 3 * (size_t)g.mx * g.my * g.mz * sizeof(double), "rhs"))) return rc;
-    if ((rc = ensure_ws(c, ws_need(c)))) return rc;
-    tosca::ref_formu(g, m, p, ucont, ucat, (p->les_active && visc) ? nut : nullptr, rhs, scale, form_mode, c->ws, c->stream);
+    if (c->ref_path) {
+        if ((rc = ensure_ws(c, ws_need(c)))) return rc;
+        tosca::ref_formu(g, m, p, ucont, ucat, (p->les_active && visc) ? nut : nullptr, rhs, scale, form_mode, c->ws, c->stream);
+    } else {
+        tosca::FormUArgs A = formu_args(m, p, ucont, ucat, (p->les_active && visc) ? nut : nullptr, rhs, scale, form_mode);
+        tosca::fused_formu(g, A, c->stream);
+    }
     return launched(c, "FormU");
 }
 
@@ -288,7 +357,13 @@ int tosca_update_cs(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, co
         if ((rc = check_dev(c, ucat, 3 * n * sizeof(double), "ucat"))) return rc;
     }
     if ((rc = ensure_ws(c, ws_need(c)))) return rc;
-    tosca::ref_update_cs(c->geo, m, p, ucat, cs, c->ws, c->stream);
+    if (!c->ref_path && p->les_model == TOSCA_LES_DSM) {
+        tosca::fused_dsm(c->geo, m, ucat, p->max_cs, cs, nullptr, c->ws, c->stream);
+        tosca::launch_wrap(cs, 1, c->geo, c->stream);
+        tosca::launch_reset_cell_periodic(cs, 1, c->geo, c->stream);
+    } else {
+        tosca::ref_update_cs(c->geo, m, p, ucat, cs, c->ws, c->stream);
+    }
     return launched(c, "UpdateCs");
 }
 
@@ -304,7 +379,15 @@ int tosca_update_nut(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, c
         (rc = check_dev(c, nut, n * sizeof(double), "nut")))
         return rc;
     if (ksgs && (rc = check_dev(c, ksgs, n * sizeof(double), "ksgs"))) return rc;
-    tosca::ref_update_nut(c->geo, m, p, ucat, cs, nut, ksgs, c->stream);
+    if (c->ref_path) {
+        tosca::ref_update_nut(c->geo, m, p, ucat, cs, nut, ksgs, c->stream);
+    } else {
+        tosca::fused_nut(c->geo, m, ucat, cs, nut, ksgs, c->stream);
+        tosca::zero_outside_interior(nut, c->geo, c->stream);
+        if (ksgs) tosca::zero_outside_interior(ksgs, c->geo, c->stream);
+        tosca::nut_bcs(nut, p, c->geo, c->stream);
+        if (ksgs) tosca::launch_wrap(ksgs, 1, c->geo, c->stream);
+    }
     return launched(c, "UpdateNut");
 }
 
@@ -335,10 +418,54 @@ int tosca_rhs_les_step(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p,
     if (rc) return rc;
     if ((rc = check_params(c, p))) return rc;
     if (!p->les_active) return set_err(c, TOSCA_ERR_ARG, "tosca_rhs_les_step needs les_active = 1");
-    if ((rc = tosca_update_cs(c, m, p, ucat, cs))) return rc;
-    if ((rc = tosca_update_nut(c, m, p, ucat, cs, nut, nullptr))) return rc;
-    if ((rc = tosca_buoyancy(c, m, p, theta, buoy_scale, btheta, nullptr))) return rc;
-    return tosca_formu(c, m, p, ucont, ucat, nut, rhs, scale, 0);
+    if (c->ref_path) {
+        if ((rc = tosca_update_cs(c, m, p, ucat, cs))) return rc;
+        if ((rc = tosca_update_nut(c, m, p, ucat, cs, nut, nullptr))) return rc;
+        if ((rc = tosca_buoyancy(c, m, p, theta, buoy_scale, btheta, nullptr))) return rc;
+        return tosca_formu(c, m, p, ucont, ucat, nut, rhs, scale, 0);
+    }
+    // fused path: validate everything the four calls would, then queue the fused kernels
+    if ((rc = check_les_model(c, p))) return rc;
+    if (p->div_scheme != TOSCA_DIV_CENTRAL && p->div_scheme != TOSCA_DIV_QUICK)
+        return set_err(c, TOSCA_ERR_UNSUPPORTED, "div_scheme = %d: only central (0) and QUICK (1) are supported", p->div_scheme);
+    if (!(p->t_ref != 0.0)) return set_err(c, TOSCA_ERR_ARG, "t_ref must be non-zero");
+    set_geo_per(c, p);
+    const Geo &g = c->geo;
+    const size_t n = (size_t)g.nloc(), v = 3 * n * sizeof(double), ng = 3 * (size_t)g.mx * g.my * g.mz * sizeof(double);
+    if ((rc = check_mesh(c, m, true, true, true))) return rc;
+    if ((rc = check_dev(c, ucont, v, "ucont")) || (rc = check_dev(c, ucat, v, "ucat")) ||
+        (rc = check_dev(c, theta, n * sizeof(double), "theta")) || (rc = check_dev(c, rhs, ng, "rhs")) ||
+        (rc = check_dev(c, cs, n * sizeof(double), "cs")) || (rc = check_dev(c, nut, n * sizeof(double), "nut")) ||
+        (rc = check_dev(c, btheta, ng, "btheta")))
+        return rc;
+    if (p->bc_u[0] == TOSCA_BCU_WALLFUNCTION || p->bc_u[1] == TOSCA_BCU_WALLFUNCTION)
+        if ((rc = check_dev(c, p->tau_wall_i, 3 * (size_t)g.mz * g.my * sizeof(double), "tau_wall_i"))) return rc;
+    if (p->bc_u[2] == TOSCA_BCU_WALLFUNCTION || p->bc_u[3] == TOSCA_BCU_WALLFUNCTION)
+        if ((rc = check_dev(c, p->tau_wall_j, 3 * (size_t)g.mz * g.mx * sizeof(double), "tau_wall_j"))) return rc;
+    if ((rc = ensure_ws(c, ws_need(c)))) return rc;
+    hipStream_t s = c->stream;
+    // UpdateCs (+ the interior of UpdateNut)
+    if (p->les_model == TOSCA_LES_DSM) {
+        tosca::fused_dsm(g, m, ucat, p->max_cs, cs, nut, c->ws, s);
+    } else {
+        tosca::ref_update_cs(g, m, p, ucat, cs, c->ws, s);
+        tosca::fused_nut(g, m, ucat, cs, nut, nullptr, s);
+    }
+    if (p->les_model == TOSCA_LES_DSM) {
+        tosca::launch_wrap(cs, 1, g, s);
+        tosca::launch_reset_cell_periodic(cs, 1, g, s);
+    }
+    // UpdateNut: zero outside the interior, boundary conditions, ghost wrap
+    tosca::zero_outside_interior(nut, g, s);
+    tosca::nut_bcs(nut, p, g, s);
+    // Buoyancy + FormU
+    tosca::FormUArgs A = formu_args(m, p, ucont, ucat, nut, rhs, scale, 0);
+    A.theta = theta;
+    A.btheta = btheta;
+    A.bscale = buoy_scale;
+    A.tRef = p->t_ref;
+    tosca::fused_formu(g, A, s);
+    return launched(c, "rhs_les_step");
 }
 
 }  // extern "C"
@@ -380,7 +507,7 @@ __global__ void k_mesh_check(Geo g, const double *nvert, const double *meshtag, 
 }
 
 void launch_mesh_check(const Geo &g, const tosca_mesh *m, int *out, hipStream_t s) {
-    hipLaunchKernelGGL(k_mesh_check, dim3(g.mz), dim3(256), 0, s, g, m->nvert, m->meshtag, m->aj, m->csi, m->eta, m->zet, out);
+    TOSCA_LAUNCH("k_mesh_check", k_mesh_check, dim3(g.mz), dim3(256), 0, s, g, m->nvert, m->meshtag, m->aj, m->csi, m->eta, m->zet, out);
 }
 
 }  // namespace tosca

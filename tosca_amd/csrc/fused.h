@@ -1,0 +1,47 @@
+// fused.h -- argument blocks and launchers of the fused gfx950 path (fused_formu.hip,
+// fused_les.hip).  The fused path serves tosca_formu, tosca_update_cs (DSM),
+// tosca_update_nut and tosca_rhs_les_step; DPASD and the standalone Buoyancy keep the
+// reference-structured kernels.
+#pragma once
+
+#include "kernels.h"
+
+namespace tosca {
+
+struct FormUArgs {
+    const double *ucont, *ucat, *nut;
+    const double *csi, *eta, *zet;
+    const double *icsi, *ieta, *izet, *jcsi, *jeta, *jzet, *kcsi, *keta, *kzet;
+    const double *iaj, *jaj, *kaj;
+    const double *tauI, *tauJ;
+    double *rhs;
+    double scale;
+    int mode, quick, les;
+    double nu;
+    int bc_u[6];
+    // fused Buoyancy (btheta == nullptr: not evaluated)
+    const double *theta;
+    double *btheta;
+    double bscale, tRef;
+};
+
+// FormU (+ Buoyancy when A.btheta != nullptr)
+void fused_formu(const Geo &g, const FormUArgs &A, hipStream_t s);
+
+// Dynamic Smagorinsky: LM, MM and |S| of every interior cell into ws (3 local scalar fields),
+// then the box average + clip into cs (interior) and, when nut != nullptr, nut = Cs Delta^2 |S|
+// (interior).  Ghost handling is left to the caller.
+void fused_dsm(const Geo &g, const tosca_mesh *m, const double *ucat, double maxCs, double *cs, double *nut,
+               double *ws, hipStream_t s);
+
+// nut = Cs Delta^2 |S| at interior cells from a given Cs (Smagorinsky, or any stored Cs)
+void fused_nut(const Geo &g, const tosca_mesh *m, const double *ucat, const double *cs, double *nut, double *ksgs,
+               hipStream_t s);
+
+// zero every point of a local scalar array outside the interior cells
+void zero_outside_interior(double *a, const Geo &g, hipStream_t s);
+
+// nut ghost cells: UpdateNutBCs subset + ghost wrap
+void nut_bcs(double *nut, const tosca_params *p, const Geo &g, hipStream_t s);
+
+}  // namespace tosca

@@ -2,6 +2,7 @@
 #pragma once
 
 #include "common.h"
+#include "timing.h"
 
 namespace tosca {
 
@@ -9,6 +10,7 @@ namespace tosca {
 void launch_wrap(double *a, int ncomp, const Geo &g, hipStream_t s);
 void launch_reset_cell_periodic(double *a, int ncomp, const Geo &g, hipStream_t s);
 void launch_fill(double *a, long n, double v, hipStream_t s);
+void launch_nut_bcs(double *nut, const tosca_params *p, const Geo &g, hipStream_t s);
 
 // reference-structured path (ref_kernels.hip); `ws` is a workspace of >= 22 * nloc doubles
 void ref_formu(const Geo &g, const tosca_mesh *m, const tosca_params *p, const double *ucont, const double *ucat,

@@ -232,50 +232,6 @@ __global__ void k_strain(const double *ucat, const double *csi, const double *et
     s[9] = strain_mag(d);
 }
 
-// Germano contraction (les.c:715-799): num = L_ba M_aq G_bq, denom = M_nm M_nl G_ml.
-__device__ void germano(const double L[3][3], const double Sh[3][3], const double SSh[3][3], double filter,
-                        double test_filter, V3 c0, V3 e0, V3 z0, double &num_out, double &den_out) {
-    double S_hat = 0;
-    for (int a = 0; a < 3; a++)
-        for (int b = 0; b < 3; b++) S_hat += pow(Sh[a][b], 2.);
-    S_hat = sqrt(2 * S_hat);
-    const double a11 = c0.x, a12 = c0.y, a13 = c0.z, a21 = e0.x, a22 = e0.y, a23 = e0.z, a31 = z0.x, a32 = z0.y, a33 = z0.z;
-    double det = a11 * (a33 * a22 - a32 * a23) - a21 * (a33 * a12 - a32 * a13) + a31 * (a23 * a12 - a22 * a13);
-    double gc[3][3];
-    gc[0][0] = (a33 * a22 - a32 * a23) / det, gc[0][1] = -(a33 * a12 - a32 * a13) / det, gc[0][2] = (a23 * a12 - a22 * a13) / det;
-    gc[1][0] = -(a33 * a21 - a31 * a23) / det, gc[1][1] = (a33 * a11 - a31 * a13) / det, gc[1][2] = -(a23 * a11 - a21 * a13) / det;
-    gc[2][0] = (a32 * a21 - a31 * a22) / det, gc[2][1] = -(a32 * a11 - a31 * a12) / det, gc[2][2] = (a22 * a11 - a21 * a12) / det;
-    double xcsi = gc[0][0], xeta = gc[0][1], xzet = gc[0][2];
-    double ycsi = gc[1][0], yeta = gc[1][1], yzet = gc[1][2];
-    double zcsi = gc[2][0], zeta = gc[2][1], zzet = gc[2][2];
-    double G[3][3];
-    G[0][0] = xcsi * xcsi + ycsi * ycsi + zcsi * zcsi;
-    G[1][1] = xeta * xeta + yeta * yeta + zeta * zeta;
-    G[2][2] = xzet * xzet + yzet * yzet + zzet * zzet;
-    G[0][1] = G[1][0] = xeta * xcsi + yeta * ycsi + zeta * zcsi;
-    G[0][2] = G[2][0] = xzet * xcsi + yzet * ycsi + zzet * zcsi;
-    G[1][2] = G[2][1] = xeta * xzet + yeta * yzet + zeta * zzet;
-    double tf2 = pow(test_filter, 2.), f2 = pow(filter, 2.);
-    double M[3][3];
-    for (int a = 0; a < 3; a++) {
-        double Mc0 = (-tf2 * S_hat * Sh[a][0] + f2 * SSh[a][0]);
-        double Mc1 = (-tf2 * S_hat * Sh[a][1] + f2 * SSh[a][1]);
-        double Mc2 = (-tf2 * S_hat * Sh[a][2] + f2 * SSh[a][2]);
-        M[a][0] = Mc0 * c0.x + Mc1 * c0.y + Mc2 * c0.z;
-        M[a][1] = Mc0 * e0.x + Mc1 * e0.y + Mc2 * e0.z;
-        M[a][2] = Mc0 * z0.x + Mc1 * z0.y + Mc2 * z0.z;
-    }
-    double num = 0, den = 0;
-    for (int q = 0; q < 3; q++)
-        for (int a = 0; a < 3; a++)
-            for (int b = 0; b < 3; b++) num += L[b][a] * M[a][q] * G[b][q];
-    for (int m = 0; m < 3; m++)
-        for (int n = 0; n < 3; n++)
-            for (int l = 0; l < 3; l++) den += M[n][m] * M[n][l] * G[m][l];
-    num_out = num;
-    den_out = den;
-}
-
 // Masked Simpson test filter + Germano at interior cells (les.c:466-801 for NP=3, :820-1164 for
 // NP=5).  The 34 filtered quantities are accumulated point by point in the reference's
 // [R][Q][P] order; masked points (DMDA boundary ghosts) contribute nothing.
@@ -527,7 +483,7 @@ void launch_wrap(double *a, int ncomp, const Geo &g, hipStream_t s) {
         int t1 = (axis == 0) ? 1 : 0, t2 = (axis == 2) ? 1 : 2;
         int n1 = m[t1] + ((g.per[t1] == TOSCA_PERIODIC_WRAP) ? 2 * GW : 0);
         int n2 = m[t2] + ((g.per[t2] == TOSCA_PERIODIC_WRAP) ? 2 * GW : 0);
-        hipLaunchKernelGGL(k_wrap_axis, dim3((n1 + 127) / 128, n2, 2 * GW), dim3(128), 0, s, a, ncomp, g, axis);
+        TOSCA_LAUNCH("k_wrap_axis", k_wrap_axis, dim3((n1 + 127) / 128, n2, 2 * GW), dim3(128), 0, s, a, ncomp, g, axis);
     }
 }
 
@@ -536,13 +492,13 @@ void launch_reset_cell_periodic(double *a, int ncomp, const Geo &g, hipStream_t 
     for (int axis = 0; axis < 3; axis++) {
         if (g.per[axis] == TOSCA_PERIODIC_NONE) continue;
         int t1 = (axis == 0) ? 1 : 0, t2 = (axis == 2) ? 1 : 2;
-        hipLaunchKernelGGL(k_reset_axis, dim3((m[t1] + 127) / 128, m[t2], 2), dim3(128), 0, s, a, ncomp, g, axis);
+        TOSCA_LAUNCH("k_reset_axis", k_reset_axis, dim3((m[t1] + 127) / 128, m[t2], 2), dim3(128), 0, s, a, ncomp, g, axis);
     }
     launch_wrap(a, ncomp, g, s);
 }
 
 void launch_fill(double *a, long n, double v, hipStream_t s) {
-    hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, n, v);
+    TOSCA_LAUNCH("k_fill", k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, n, v);
 }
 
 void ref_formu(const Geo &g, const tosca_mesh *m, const tosca_params *p, const double *ucont, const double *ucat,
@@ -564,9 +520,9 @@ void ref_formu(const Geo &g, const tosca_mesh *m, const tosca_params *p, const d
     double *fp = ws + 18 * n;
     CellArgs C;
     for (int q = 0; q < 3; q++) { C.div[q] = A.div[q]; C.visc[q] = A.visc[q]; }
-    hipLaunchKernelGGL(k_formu_faces, grd(g.mx - 1, g.my - 1, g.mz - 1), blk2(), 0, s, A, g);
-    hipLaunchKernelGGL(k_formu_cells, grd(g.mx - 2, g.my - 2, g.mz - 2), blk2(), 0, s, C, fp, mode, g);
-    hipLaunchKernelGGL(k_formu_dot, grd(g.mx - 2, g.my - 2, g.mz - 2), blk2(), 0, s, fp, m->csi, m->eta, m->zet,
+    TOSCA_LAUNCH("k_formu_faces", k_formu_faces, grd(g.mx - 1, g.my - 1, g.mz - 1), blk2(), 0, s, A, g);
+    TOSCA_LAUNCH("k_formu_cells", k_formu_cells, grd(g.mx - 2, g.my - 2, g.mz - 2), blk2(), 0, s, C, fp, mode, g);
+    TOSCA_LAUNCH("k_formu_dot", k_formu_dot, grd(g.mx - 2, g.my - 2, g.mz - 2), blk2(), 0, s, fp, m->csi, m->eta, m->zet,
                        m->iaj, m->jaj, m->kaj, rhs, scale, g);
 }
 
@@ -577,14 +533,14 @@ void ref_update_cs(const Geo &g, const tosca_mesh *m, const tosca_params *p, con
     double *S = ws, *LM = ws + 10 * n, *MM = ws + 11 * n, *QN = ws + 12 * n, *NN = ws + 13 * n;
     double *part = ws + 14 * n;
     dim3 gi = grd(g.mx - 2, g.my - 2, g.mz - 2);
-    hipLaunchKernelGGL(k_strain, gi, blk2(), 0, s, ucat, m->csi, m->eta, m->zet, m->aj, S, g);
-    hipLaunchKernelGGL(k_test_filter<3>, gi, blk2(), 0, s, ucat, m->csi, m->eta, m->zet, m->aj, S, LM, MM, g);
+    TOSCA_LAUNCH("k_strain", k_strain, gi, blk2(), 0, s, ucat, m->csi, m->eta, m->zet, m->aj, S, g);
+    TOSCA_LAUNCH("k_test_filter3", k_test_filter<3>, gi, blk2(), 0, s, ucat, m->csi, m->eta, m->zet, m->aj, S, LM, MM, g);
     if (p->les_model == TOSCA_LES_DSM) {
-        hipLaunchKernelGGL(k_box_clip, gi, blk2(), 0, s, LM, MM, m->aj, cs, p->max_cs, g);
+        TOSCA_LAUNCH("k_box_clip", k_box_clip, gi, blk2(), 0, s, LM, MM, m->aj, cs, p->max_cs, g);
     } else {
-        hipLaunchKernelGGL(k_test_filter<5>, gi, blk2(), 0, s, ucat, m->csi, m->eta, m->zet, m->aj, S, QN, NN, g);
-        hipLaunchKernelGGL(k_plane_partial, dim3(g.my - 2, PLANE_CHUNKS), dim3(256), 0, s, LM, MM, QN, NN, part, g);
-        hipLaunchKernelGGL(k_dpasd_cs, gi, blk2(), 0, s, part, cs, p->max_cs, g);
+        TOSCA_LAUNCH("k_test_filter5", k_test_filter<5>, gi, blk2(), 0, s, ucat, m->csi, m->eta, m->zet, m->aj, S, QN, NN, g);
+        TOSCA_LAUNCH("k_plane_partial", k_plane_partial, dim3(g.my - 2, PLANE_CHUNKS), dim3(256), 0, s, LM, MM, QN, NN, part, g);
+        TOSCA_LAUNCH("k_dpasd_cs", k_dpasd_cs, gi, blk2(), 0, s, part, cs, p->max_cs, g);
     }
     launch_wrap(cs, 1, g, s);
     launch_reset_cell_periodic(cs, 1, g, s);
@@ -595,22 +551,26 @@ void ref_update_nut(const Geo &g, const tosca_mesh *m, const tosca_params *p, co
     const long n = g.nloc();
     launch_fill(nut, n, 0.0, s);
     if (ksgs) launch_fill(ksgs, n, 0.0, s);
-    hipLaunchKernelGGL(k_nut, grd(g.mx - 2, g.my - 2, g.mz - 2), blk2(), 0, s, ucat, m->csi, m->eta, m->zet, m->aj,
+    TOSCA_LAUNCH("k_nut", k_nut, grd(g.mx - 2, g.my - 2, g.mz - 2), blk2(), 0, s, ucat, m->csi, m->eta, m->zet, m->aj,
                        cs, nut, ksgs, g);
+    launch_nut_bcs(nut, p, g, s);
+    if (ksgs) launch_wrap(ksgs, 1, g, s);
+}
+
+void launch_nut_bcs(double *nut, const tosca_params *p, const Geo &g, hipStream_t s) {
     NutBC B;
     for (int q = 0; q < 6; q++) { B.bc[q] = p->bc_nut[q]; B.val[q] = p->nut_value[q]; }
     const int mm[3] = {g.mx, g.my, g.mz};
     for (int axis = 0; axis < 3; axis++) {
         int t1 = (axis == 0) ? 1 : 0, t2 = (axis == 2) ? 1 : 2;
-        hipLaunchKernelGGL(k_nut_bc, dim3((mm[t1] + 127) / 128, mm[t2] - 2, 2), dim3(128), 0, s, nut, B, g, axis);
+        TOSCA_LAUNCH("k_nut_bc", k_nut_bc, dim3((mm[t1] + 127) / 128, mm[t2] - 2, 2), dim3(128), 0, s, nut, B, g, axis);
     }
     launch_wrap(nut, 1, g, s);
-    if (ksgs) launch_wrap(ksgs, 1, g, s);
 }
 
 void ref_buoyancy(const Geo &g, const tosca_mesh *m, const tosca_params *p, const double *theta, double scale,
                   double *btheta, double *gcont, hipStream_t s) {
-    hipLaunchKernelGGL(k_buoyancy, grd(g.mx, g.my, g.mz), blk2(), 0, s, theta, m->icsi, m->jeta, m->kzet,
+    TOSCA_LAUNCH("k_buoyancy", k_buoyancy, grd(g.mx, g.my, g.mz), blk2(), 0, s, theta, m->icsi, m->jeta, m->kzet,
                        p->t_ref, scale, btheta, gcont, g);
 }
 

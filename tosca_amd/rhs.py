@@ -12,7 +12,7 @@ calls queue work on torch's current stream and raise ToscaError on a non-zero re
 from __future__ import annotations
 
 import ctypes as C
-from typing import Dict, Mapping, Optional
+from typing import Dict, Mapping, Optional, Tuple
 
 import torch
 
@@ -97,6 +97,26 @@ class RhsContext:
         self._check(self.lib.tosca_rhs_les_step(self.h, C.byref(m), C.byref(p), _ptr(ucont), _ptr(ucat), _ptr(theta),
                                                 _ptr(rhs), float(scale), _ptr(cs), _ptr(nut), _ptr(btheta),
                                                 float(buoy_scale)))
+
+    # ---- instrumentation -----------------------------------------------------------------------
+    def enable_timing(self, on: bool = True):
+        self._check(self.lib.tosca_ctx_enable_timing(self.h, int(on)))
+
+    def reset_timing(self):
+        self._check(self.lib.tosca_ctx_timing_reset(self.h))
+
+    def timings(self) -> Dict[str, Tuple[float, int]]:
+        """{kernel name: (total event milliseconds, launches)} since the last reset."""
+        n = self.lib.tosca_ctx_timing_count(self.h)
+        if n < 0:
+            self._check(abi.TOSCA_ERR_ARG)
+        out = {}
+        buf = C.create_string_buffer(128)
+        ms, cnt = C.c_double(), C.c_int64()
+        for i in range(n):
+            self._check(self.lib.tosca_ctx_timing_entry(self.h, i, buf, 128, C.byref(ms), C.byref(cnt)))
+            out[buf.value.decode()] = (ms.value, cnt.value)
+        return out
 
     def mesh_check(self, mesh):
         self._stream()
