@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <algorithm>
 #include <iterator>
 #include <vector>
 
@@ -180,7 +181,8 @@ tosca::FormUArgs formu_args(const tosca_mesh *m, const tosca_params *p, const do
 size_t ws_need(const tosca_ctx *c) {
     // 22 local fields (FormU: 6 face-flux vectors + fp; UpdateCs: strain 10 + LM/MM/QN/NN)
     // plus the DPASD plane partials
-    return 22 * (size_t)c->geo.nloc() + (size_t)c->geo.my * 16 * 4 + 64;
+    return std::max(22 * (size_t)c->geo.nloc() + (size_t)c->geo.my * 16 * 4 + 64,
+                    3 * (size_t)c->geo.nloc() + tosca::formu_ws_doubles(c->geo) + 64);
 }
 
 }  // namespace
@@ -339,8 +341,9 @@ int tosca_formu(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const 
         if ((rc = ensure_ws(c, ws_need(c)))) return rc;
         tosca::ref_formu(g, m, p, ucont, ucat, (p->les_active && visc) ? nut : nullptr, rhs, scale, form_mode, c->ws, c->stream);
     } else {
+        if ((rc = ensure_ws(c, ws_need(c)))) return rc;
         tosca::FormUArgs A = formu_args(m, p, ucont, ucat, (p->les_active && visc) ? nut : nullptr, rhs, scale, form_mode);
-        tosca::fused_formu(g, A, c->stream);
+        tosca::fused_formu(g, A, c->ws + 3 * (size_t)g.nloc(), c->stream);
     }
     return launched(c, "FormU");
 }
@@ -464,7 +467,7 @@ int tosca_rhs_les_step(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p,
     A.btheta = btheta;
     A.bscale = buoy_scale;
     A.tRef = p->t_ref;
-    tosca::fused_formu(g, A, s);
+    tosca::fused_formu(g, A, c->ws + 3 * (size_t)g.nloc(), s);
     return launched(c, "rhs_les_step");
 }
 

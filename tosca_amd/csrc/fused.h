@@ -25,8 +25,10 @@ struct FormUArgs {
     double bscale, tRef;
 };
 
-// FormU (+ Buoyancy when A.btheta != nullptr)
-void fused_formu(const Geo &g, const FormUArgs &A, hipStream_t s);
+// FormU (+ Buoyancy when A.btheta != nullptr); ws holds formu_ws_doubles(g) doubles for the
+// boundary-face fluxes and periodic-seam fp
+void fused_formu(const Geo &g, const FormUArgs &A, double *ws, hipStream_t s);
+size_t formu_ws_doubles(const Geo &g);
 
 // Dynamic Smagorinsky: LM, MM and |S| of every interior cell into ws (3 local scalar fields),
 // then the box average + clip into cs (interior) and, when nut != nullptr, nut = Cs Delta^2 |S|
@@ -37,6 +39,10 @@ void fused_dsm(const Geo &g, const tosca_mesh *m, const double *ucat, double max
 // nut = Cs Delta^2 |S| at interior cells from a given Cs (Smagorinsky, or any stored Cs)
 void fused_nut(const Geo &g, const tosca_mesh *m, const double *ucat, const double *cs, double *nut, double *ksgs,
                hipStream_t s);
+
+// zero the shell of an nx*ny*nz array (ncomp doubles per point) outside its interior cells,
+// the interior being [1+lo, n-2-hi] on each axis in array coordinates
+void zero_shell(double *a, int ncomp, int nx, int ny, int nz, int lo, int hi, hipStream_t s);
 
 // zero every point of a local scalar array outside the interior cells
 void zero_outside_interior(double *a, const Geo &g, hipStream_t s);

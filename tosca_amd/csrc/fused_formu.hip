@@ -77,6 +77,118 @@ __device__ __forceinline__ Flux face_flux(const FormUArgs &A, const Geo &g, int 
     return r;
 }
 
+__device__ __forceinline__ V3 ldnt3(const double *p, int c) {
+    return V3{__builtin_nontemporal_load(p + 3 * c), __builtin_nontemporal_load(p + 3 * c + 1),
+              __builtin_nontemporal_load(p + 3 * c + 2)};
+}
+
+// Flux of a face away from the domain boundary (face index 1..m-3 along its normal): every
+// boundary rule of the general evaluation is inactive there, so the gradients are the central
+// four-point averages of Compute_du_i/j/k, the QUICK stencil is (f-1, f, f+1, f+2) and nuEff is
+// nu + <nu_t>.  c is the local index of the face's left cell, sj/sk the j and k strides.
+template <int DIR>
+__device__ __forceinline__ Flux face_fast(const FormUArgs &A, int c, int sj, int sk) {
+    const double *u = A.ucat;
+    const int s = (DIR == 0) ? 1 : (DIR == 1) ? sj : sk;
+    const V3 u0 = ld3(u, c), u1 = ld3(u, c + s);
+    Flux r;
+    r.d = V3{0, 0, 0};
+    r.v = V3{0, 0, 0};
+    if (A.mode != 2) {
+        const double V = __builtin_nontemporal_load(A.ucont + 3 * c + DIR);
+        V3 f;
+        if (A.quick) f = quick(ld3(u, c - s), u0, u1, ld3(u, c + 2 * s), V);
+        else f = V3{0.5 * (u0.x + u1.x), 0.5 * (u0.y + u1.y), 0.5 * (u0.z + u1.z)};
+        r.d = V3{-V * f.x, -V * f.y, -V * f.z};
+    }
+    if (A.mode != 1) {
+        Grad g;
+        // the two transverse strides of this face
+        const int a = (DIR == 0) ? sj : 1, b = (DIR == 2) ? sj : sk;
+        const V3 ap = ld3(u, c + a + s), a0 = ld3(u, c + a), am = ld3(u, c - a + s), an = ld3(u, c - a);
+        const V3 bp = ld3(u, c + b + s), b0 = ld3(u, c + b), bm = ld3(u, c - b + s), bn = ld3(u, c - b);
+        const double nx = u1.x - u0.x, ny = u1.y - u0.y, nz = u1.z - u0.z;
+        const double ax = (ap.x + a0.x - am.x - an.x) * 0.25, ay = (ap.y + a0.y - am.y - an.y) * 0.25,
+                     az = (ap.z + a0.z - am.z - an.z) * 0.25;
+        const double bx = (bp.x + b0.x - bm.x - bn.x) * 0.25, by = (bp.y + b0.y - bm.y - bn.y) * 0.25,
+                     bz = (bp.z + b0.z - bm.z - bn.z) * 0.25;
+        if (DIR == 0) {        // normal = csi; transverse a = eta (j), b = zet (k)
+            g.dudc = nx; g.dvdc = ny; g.dwdc = nz; g.dude = ax; g.dvde = ay; g.dwde = az; g.dudz = bx; g.dvdz = by; g.dwdz = bz;
+        } else if (DIR == 1) { // normal = eta; transverse a = csi (i), b = zet (k)
+            g.dudc = ax; g.dvdc = ay; g.dwdc = az; g.dude = nx; g.dvde = ny; g.dwde = nz; g.dudz = bx; g.dvdz = by; g.dwdz = bz;
+        } else {               // normal = zet; transverse a = csi (i), b = eta (j)
+            g.dudc = ax; g.dvdc = ay; g.dwdc = az; g.dude = bx; g.dvde = by; g.dwde = bz; g.dudz = nx; g.dvdz = ny; g.dwdz = nz;
+        }
+        double nuEff = A.nu;
+        if (A.les) nuEff = A.nu + 0.5 * (A.nut[c] + A.nut[c + s]);
+        const double *fc = (DIR == 0) ? A.icsi : (DIR == 1) ? A.jcsi : A.kcsi;
+        const double *fe = (DIR == 0) ? A.ieta : (DIR == 1) ? A.jeta : A.keta;
+        const double *fz = (DIR == 0) ? A.izet : (DIR == 1) ? A.jzet : A.kzet;
+        const double *fa = (DIR == 0) ? A.iaj : (DIR == 1) ? A.jaj : A.kaj;
+        r.v = visc_flux(g, ldnt3(fc, c), ldnt3(fe, c), ldnt3(fz, c), DIR, fa[c], nuEff);
+    }
+    return r;
+}
+
+// LDS ring of ucat planes: SoA [slot][component][row][col], rows j0-2 .. j0+NR, cols i0-2 .. i0+65
+template <int NR>
+struct Ring {
+    static constexpr int RI = 68, RJ = NR + 3, NS = 4;
+    double v[NS][3][RJ][RI];
+    __device__ __forceinline__ V3 at(int slot, int row, int col) const {
+        return V3{v[slot][0][row][col], v[slot][1][row][col], v[slot][2][row][col]};
+    }
+};
+
+// face_fast with ucat taken from the LDS ring.  (row, col) is the face's left cell in ring
+// coordinates and slot(t) the ring slot of plane t.
+template <int DIR, int NR>
+__device__ __forceinline__ Flux face_lds(const FormUArgs &A, const Ring<NR> &R, int c, int row, int col, int t,
+                                         int sj, int sk) {
+    auto U = [&](int dk, int dj, int di) { return R.at((t + dk) & 3, row + dj, col + di); };
+    const int s = (DIR == 0) ? 1 : (DIR == 1) ? sj : sk;
+    const int ok = (DIR == 2), oj = (DIR == 1), oi = (DIR == 0);
+    const V3 u0 = U(0, 0, 0), u1 = U(ok, oj, oi);
+    Flux r;
+    r.d = V3{0, 0, 0};
+    r.v = V3{0, 0, 0};
+    if (A.mode != 2) {
+        const double V = __builtin_nontemporal_load(A.ucont + 3 * c + DIR);
+        V3 f;
+        if (A.quick) f = quick(U(-ok, -oj, -oi), u0, u1, U(2 * ok, 2 * oj, 2 * oi), V);
+        else f = V3{0.5 * (u0.x + u1.x), 0.5 * (u0.y + u1.y), 0.5 * (u0.z + u1.z)};
+        r.d = V3{-V * f.x, -V * f.y, -V * f.z};
+    }
+    if (A.mode != 1) {
+        // transverse directions a and b of the face (see face_fast)
+        const int ak = 0, aj = (DIR == 0), ai = (DIR != 0);
+        const int bk = (DIR != 2), bj = (DIR == 2), bi = 0;
+        const V3 ap = U(ak + ok, aj + oj, ai + oi), a0 = U(ak, aj, ai), am = U(ok - ak, oj - aj, oi - ai), an = U(-ak, -aj, -ai);
+        const V3 bp = U(bk + ok, bj + oj, bi + oi), b0 = U(bk, bj, bi), bm = U(ok - bk, oj - bj, oi - bi), bn = U(-bk, -bj, -bi);
+        const double nx = u1.x - u0.x, ny = u1.y - u0.y, nz = u1.z - u0.z;
+        const double ax = (ap.x + a0.x - am.x - an.x) * 0.25, ay = (ap.y + a0.y - am.y - an.y) * 0.25,
+                     az = (ap.z + a0.z - am.z - an.z) * 0.25;
+        const double bx = (bp.x + b0.x - bm.x - bn.x) * 0.25, by = (bp.y + b0.y - bm.y - bn.y) * 0.25,
+                     bz = (bp.z + b0.z - bm.z - bn.z) * 0.25;
+        Grad g;
+        if (DIR == 0) {
+            g.dudc = nx; g.dvdc = ny; g.dwdc = nz; g.dude = ax; g.dvde = ay; g.dwde = az; g.dudz = bx; g.dvdz = by; g.dwdz = bz;
+        } else if (DIR == 1) {
+            g.dudc = ax; g.dvdc = ay; g.dwdc = az; g.dude = nx; g.dvde = ny; g.dwde = nz; g.dudz = bx; g.dvdz = by; g.dwdz = bz;
+        } else {
+            g.dudc = ax; g.dvdc = ay; g.dwdc = az; g.dude = bx; g.dvde = by; g.dwde = bz; g.dudz = nx; g.dvdz = ny; g.dwdz = nz;
+        }
+        double nuEff = A.nu;
+        if (A.les) nuEff = A.nu + 0.5 * (A.nut[c] + A.nut[c + s]);
+        const double *fc = (DIR == 0) ? A.icsi : (DIR == 1) ? A.jcsi : A.kcsi;
+        const double *fe = (DIR == 0) ? A.ieta : (DIR == 1) ? A.jeta : A.keta;
+        const double *fz = (DIR == 0) ? A.izet : (DIR == 1) ? A.jzet : A.kzet;
+        const double *fa = (DIR == 0) ? A.iaj : (DIR == 1) ? A.jaj : A.kaj;
+        r.v = visc_flux(g, ldnt3(fc, c), ldnt3(fe, c), ldnt3(fz, c), DIR, fa[c], nuEff);
+    }
+    return r;
+}
+
 __device__ __forceinline__ double shfl_up1(double x) { return __shfl_up(x, 1, 64); }
 __device__ __forceinline__ double shfl_dn1(double x) { return __shfl_down(x, 1, 64); }
 __device__ __forceinline__ V3 shfl_up3(V3 a) { return V3{shfl_up1(a.x), shfl_up1(a.y), shfl_up1(a.z)}; }
@@ -93,137 +205,423 @@ __device__ __forceinline__ double fpsum(double d1, double d1m, double d2, double
 
 }  // namespace
 
+// ------------------------------------------------------------------------------------------------
+// boundary faces and periodic seams (general evaluation, ~2 % of the faces)
+// ------------------------------------------------------------------------------------------------
+// Buffers (doubles): bF[d] holds, for the two boundary faces 0 and m-2 of direction d, the div
+// and visc flux (6 values) at every transverse position; on a periodic axis face 0 holds the flux
+// of face m-2 (resetFacePeriodicFluxesVector).  seam[d] holds fp at cell 1 of a periodic axis d
+// for every transverse interior position: what the ghost cell m-1 carries after
+// resetCellPeriodicFluxes.
+struct BoundaryBufs {
+    double *bF[3];
+    double *seam[3];
+};
+
+__host__ __device__ inline int tr_dims(const Geo &g, int d, int &n1, int &n2) {
+    // transverse sizes (fastest first) of direction d
+    if (d == 0) { n1 = g.my; n2 = g.mz; }
+    else if (d == 1) { n1 = g.mx; n2 = g.mz; }
+    else { n1 = g.mx; n2 = g.my; }
+    return n1 * n2;
+}
+
+template <int DIR>
+__device__ __forceinline__ Flux face_general(const FormUArgs &A, const Geo &g, int f, int a, int b) {
+    // face f along DIR at transverse (a, b) = (j, k) | (i, k) | (i, j); periodic face 0 is face m-2
+    const int m = (DIR == 0) ? g.mx : (DIR == 1) ? g.my : g.mz;
+    if (f == 0 && g.per[DIR] != TOSCA_PERIODIC_NONE) f = m - 2;
+    if (DIR == 0) return face_flux<0>(A, g, f, a, b);
+    if (DIR == 1) return face_flux<1>(A, g, a, f, b);
+    return face_flux<2>(A, g, a, b, f);
+}
+
+__global__ void __launch_bounds__(128) k_formu_boundary(FormUArgs A, Geo g, BoundaryBufs B) {
+    const int d = blockIdx.z / 3, what = blockIdx.z % 3;      // what: 0 face 0, 1 face m-2, 2 seam fp
+    int n1, n2;
+    const int n = tr_dims(g, d, n1, n2);
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int a = t % n1, b = t / n1;
+    const int ma = (d == 0) ? g.my : g.mx, mb = (d == 2) ? g.my : g.mz;
+    if (a < 1 || a > ma - 2 || b < 1 || b > mb - 2) return;     // transverse interior only
+    const int m = (d == 0) ? g.mx : (d == 1) ? g.my : g.mz;
+    if (what < 2) {
+        const int f = what == 0 ? 0 : m - 2;
+        Flux F = (d == 0) ? face_general<0>(A, g, f, a, b) : (d == 1) ? face_general<1>(A, g, f, a, b)
+                                                                    : face_general<2>(A, g, f, a, b);
+        double *o = B.bF[d] + ((long)what * n + t) * 6;
+        o[0] = F.d.x; o[1] = F.d.y; o[2] = F.d.z; o[3] = F.v.x; o[4] = F.v.y; o[5] = F.v.z;
+        return;
+    }
+    if (g.per[d] == TOSCA_PERIODIC_NONE) return;
+    // fp at cell 1 along d (transverse interior), from its six faces
+    int i, j, k;
+    if (d == 0) { i = 1; j = a; k = b; } else if (d == 1) { i = a; j = 1; k = b; } else { i = a; j = b; k = 1; }
+    const Flux f1 = face_general<0>(A, g, i, j, k), f1m = face_general<0>(A, g, i - 1, j, k);
+    const Flux f2 = face_general<1>(A, g, j, i, k), f2m = face_general<1>(A, g, j - 1, i, k);
+    const Flux f3 = face_general<2>(A, g, k, i, j), f3m = face_general<2>(A, g, k - 1, i, j);
+    double *o = B.seam[d] + (long)t * 3;
+    o[0] = fpsum(f1.d.x, f1m.d.x, f2.d.x, f2m.d.x, f3.d.x, f3m.d.x, f1.v.x, f1m.v.x, f2.v.x, f2m.v.x, f3.v.x, f3m.v.x, A.mode);
+    o[1] = fpsum(f1.d.y, f1m.d.y, f2.d.y, f2m.d.y, f3.d.y, f3m.d.y, f1.v.y, f1m.v.y, f2.v.y, f2m.v.y, f3.v.y, f3m.v.y, A.mode);
+    o[2] = fpsum(f1.d.z, f1m.d.z, f2.d.z, f2m.d.z, f3.d.z, f3m.d.z, f1.v.z, f1m.v.z, f2.v.z, f2m.v.z, f3.v.z, f3m.v.z, A.mode);
+}
+
+// ------------------------------------------------------------------------------------------------
+// main k-marching kernel: interior faces only; boundary faces and seams come from the buffers
+// ------------------------------------------------------------------------------------------------
+struct MainArgs {
+    const double *ucont, *ucat, *nut;
+    const double *csi, *eta, *zet;
+    const double *icsi, *ieta, *izet, *jcsi, *jeta, *jzet, *kcsi, *keta, *kzet;
+    const double *iaj, *jaj, *kaj;
+    double *rhs;
+    const double *theta;
+    double *btheta;
+    const double *bF1, *bF2, *bF3, *sI, *sJ, *sK;
+    double scale, nu, bscale, invTRef;
+    int mode, quick, les;
+};
+
+__device__ __forceinline__ Flux ldflux(const double *p) {
+    return Flux{V3{p[0], p[1], p[2]}, V3{p[3], p[4], p[5]}};
+}
+
+// per-thread column pointers: every array at the thread's (j, i) column, plane 0
+struct ColPtrs {
+    const double *ucont, *nut, *fc[3], *fe[3], *fz[3], *fa[3];
+};
+
+// metrics and fluxes of one interior face, loaded ahead of use
+struct FaceIn { V3 fc, fe, fz; double fa, V, nutR; };
+
+template <int DIR>
+__device__ __forceinline__ FaceIn face_in(const ColPtrs &P, long q, int s) {
+    FaceIn f;
+    f.fc = ldnt3(P.fc[DIR] + 3 * q, 0);
+    f.fe = ldnt3(P.fe[DIR] + 3 * q, 0);
+    f.fz = ldnt3(P.fz[DIR] + 3 * q, 0);
+    f.fa = P.fa[DIR][q];
+    f.V = __builtin_nontemporal_load(P.ucont + 3 * q + DIR);
+    f.nutR = P.nut[q + s];
+    return f;
+}
+
+// interior face flux from the LDS ring and preloaded face data (nut0: nu_t of the face's cell)
+template <int DIR, int NR>
+__device__ __forceinline__ Flux face_pre(const MainArgs &A, const FaceIn &in, double nut0, const Ring<NR> &R,
+                                         int row, int col, int t) {
+    auto U = [&](int dk, int dj, int di) { return R.at((t + dk) & 3, row + dj, col + di); };
+    const int ok = (DIR == 2), oj = (DIR == 1), oi = (DIR == 0);
+    const V3 u0 = U(0, 0, 0), u1 = U(ok, oj, oi);
+    Flux r;
+    r.d = V3{0, 0, 0};
+    r.v = V3{0, 0, 0};
+    if (A.mode != 2) {
+        V3 f;
+        if (A.quick) f = quick(U(-ok, -oj, -oi), u0, u1, U(2 * ok, 2 * oj, 2 * oi), in.V);
+        else f = V3{0.5 * (u0.x + u1.x), 0.5 * (u0.y + u1.y), 0.5 * (u0.z + u1.z)};
+        r.d = V3{-in.V * f.x, -in.V * f.y, -in.V * f.z};
+    }
+    if (A.mode != 1) {
+        const int ak = 0, aj = (DIR == 0), ai = (DIR != 0);
+        const int bk = (DIR != 2), bj = (DIR == 2), bi = 0;
+        const V3 ap = U(ak + ok, aj + oj, ai + oi), a0 = U(ak, aj, ai), am = U(ok - ak, oj - aj, oi - ai), an = U(-ak, -aj, -ai);
+        const V3 bp = U(bk + ok, bj + oj, bi + oi), b0 = U(bk, bj, bi), bm = U(ok - bk, oj - bj, oi - bi), bn = U(-bk, -bj, -bi);
+        const double nx = u1.x - u0.x, ny = u1.y - u0.y, nz = u1.z - u0.z;
+        const double ax = (ap.x + a0.x - am.x - an.x) * 0.25, ay = (ap.y + a0.y - am.y - an.y) * 0.25,
+                     az = (ap.z + a0.z - am.z - an.z) * 0.25;
+        const double bx = (bp.x + b0.x - bm.x - bn.x) * 0.25, by = (bp.y + b0.y - bm.y - bn.y) * 0.25,
+                     bz = (bp.z + b0.z - bm.z - bn.z) * 0.25;
+        Grad g;
+        if (DIR == 0) {
+            g.dudc = nx; g.dvdc = ny; g.dwdc = nz; g.dude = ax; g.dvde = ay; g.dwde = az; g.dudz = bx; g.dvdz = by; g.dwdz = bz;
+        } else if (DIR == 1) {
+            g.dudc = ax; g.dvdc = ay; g.dwdc = az; g.dude = nx; g.dvde = ny; g.dwde = nz; g.dudz = bx; g.dvdz = by; g.dwdz = bz;
+        } else {
+            g.dudc = ax; g.dvdc = ay; g.dwdc = az; g.dude = bx; g.dvde = by; g.dwde = bz; g.dudz = nx; g.dvdz = ny; g.dwdz = nz;
+        }
+        double nuEff = A.nu;
+        if (A.les) nuEff = A.nu + 0.5 * (nut0 + in.nutR);
+        r.v = visc_flux(g, in.fc, in.fe, in.fz, DIR, in.fa, nuEff);
+    }
+    return r;
+}
+
+// interior face flux from the LDS ring; q is the plane offset (in points) of the face's cell
+template <int DIR, int NR>
+__device__ __forceinline__ Flux face_main(const MainArgs &A, const ColPtrs &P, const Ring<NR> &R, long q, int row,
+                                          int col, int t, int s) {
+    auto U = [&](int dk, int dj, int di) { return R.at((t + dk) & 3, row + dj, col + di); };
+    const int ok = (DIR == 2), oj = (DIR == 1), oi = (DIR == 0);
+    const V3 u0 = U(0, 0, 0), u1 = U(ok, oj, oi);
+    Flux r;
+    r.d = V3{0, 0, 0};
+    r.v = V3{0, 0, 0};
+    if (A.mode != 2) {
+        const double V = __builtin_nontemporal_load(P.ucont + 3 * q + DIR);
+        V3 f;
+        if (A.quick) f = quick(U(-ok, -oj, -oi), u0, u1, U(2 * ok, 2 * oj, 2 * oi), V);
+        else f = V3{0.5 * (u0.x + u1.x), 0.5 * (u0.y + u1.y), 0.5 * (u0.z + u1.z)};
+        r.d = V3{-V * f.x, -V * f.y, -V * f.z};
+    }
+    if (A.mode != 1) {
+        const int ak = 0, aj = (DIR == 0), ai = (DIR != 0);
+        const int bk = (DIR != 2), bj = (DIR == 2), bi = 0;
+        const V3 ap = U(ak + ok, aj + oj, ai + oi), a0 = U(ak, aj, ai), am = U(ok - ak, oj - aj, oi - ai), an = U(-ak, -aj, -ai);
+        const V3 bp = U(bk + ok, bj + oj, bi + oi), b0 = U(bk, bj, bi), bm = U(ok - bk, oj - bj, oi - bi), bn = U(-bk, -bj, -bi);
+        const double nx = u1.x - u0.x, ny = u1.y - u0.y, nz = u1.z - u0.z;
+        const double ax = (ap.x + a0.x - am.x - an.x) * 0.25, ay = (ap.y + a0.y - am.y - an.y) * 0.25,
+                     az = (ap.z + a0.z - am.z - an.z) * 0.25;
+        const double bx = (bp.x + b0.x - bm.x - bn.x) * 0.25, by = (bp.y + b0.y - bm.y - bn.y) * 0.25,
+                     bz = (bp.z + b0.z - bm.z - bn.z) * 0.25;
+        Grad g;
+        if (DIR == 0) {
+            g.dudc = nx; g.dvdc = ny; g.dwdc = nz; g.dude = ax; g.dvde = ay; g.dwde = az; g.dudz = bx; g.dvdz = by; g.dwdz = bz;
+        } else if (DIR == 1) {
+            g.dudc = ax; g.dvdc = ay; g.dwdc = az; g.dude = nx; g.dvde = ny; g.dwde = nz; g.dudz = bx; g.dvdz = by; g.dwdz = bz;
+        } else {
+            g.dudc = ax; g.dvdc = ay; g.dwdc = az; g.dude = bx; g.dvde = by; g.dwde = bz; g.dudz = nx; g.dvdz = ny; g.dwdz = nz;
+        }
+        double nuEff = A.nu;
+        if (A.les) nuEff = A.nu + 0.5 * (P.nut[q] + P.nut[q + s]);
+        r.v = visc_flux(g, ldnt3(P.fc[DIR] + 3 * q, 0), ldnt3(P.fe[DIR] + 3 * q, 0), ldnt3(P.fz[DIR] + 3 * q, 0), DIR,
+                        P.fa[DIR][q], nuEff);
+    }
+    return r;
+}
+
 template <int NR>
-__global__ void __launch_bounds__(64 * NR) k_fused_formu(FormUArgs A, Geo g, int kchunk) {
+__global__ void __launch_bounds__(64 * NR) k_fused_formu(MainArgs A, Geo g, int kchunk) {
+    // Every thread evaluates its three faces and fp on every plane (threads outside the tile's
+    // useful range work on clamped, in-bounds addresses and their results are discarded), so the
+    // hot loop has no divergent branches; only the stores and the rare boundary-buffer loads are
+    // predicated.
     constexpr int TI = 62, TJ = NR - 2;
     __shared__ double sF2[6][NR][64];     // j-face fluxes of the plane (div xyz, visc xyz)
-    __shared__ double sFp[3][NR][64];     // fp of the previous plane
-
+    __shared__ double sFp[2][3][NR][64];  // fp of the previous plane (double-buffered)
+    __shared__ Ring<NR> ring;             // ucat planes t-1 .. t+2
+    using RingT = Ring<NR>;
+    constexpr int PLANE = RingT::RJ * RingT::RI * 3;
+    constexpr int PRE = (PLANE + 64 * NR - 1) / (64 * NR);
+    const int tid = threadIdx.x + 64 * threadIdx.y;
     const int lane = threadIdx.x, r = threadIdx.y;
     const int i0 = 1 + blockIdx.x * TI, j0 = 1 + blockIdx.y * TJ;
     const int kb = 1 + blockIdx.z * kchunk;
     const int ke = min(kb + kchunk, g.mz - 1);      // output planes [kb, ke)
-    const int iraw = i0 - 1 + lane, jraw = j0 - 1 + r;
+    const int i = i0 - 1 + lane, j = j0 - 1 + r;
     const bool iper = g.per[0] != TOSCA_PERIODIC_NONE, jper = g.per[1] != TOSCA_PERIODIC_NONE,
                kper = g.per[2] != TOSCA_PERIODIC_NONE;
-    // column this thread evaluates (periodic ghost m-1 acts as cell 1)
-    const int ic = (iraw == g.mx - 1 && iper) ? 1 : iraw;
-    const int jc = (jraw == g.my - 1 && jper) ? 1 : jraw;
-    const bool colI = ic >= 1 && ic <= g.mx - 2, colJ = jc >= 1 && jc <= g.my - 2;
-    // i-face of this lane (face 0 of a periodic axis is face m-2)
-    const int fi = (ic == 0 && iper) ? g.mx - 2 : ic;
-    const bool doF1 = r >= 1 && colJ && fi >= 0 && fi <= g.mx - 2;
-    const int fj = (jc == 0 && jper) ? g.my - 2 : jc;
-    const bool doF2 = lane >= 1 && colI && fj >= 0 && fj <= g.my - 2;
-    const bool cell = lane >= 1 && r >= 1 && colI && colJ;           // fp of (ic, jc) is formed
-    const bool out = cell && lane <= TI && r <= TJ && iraw <= g.mx - 2 && jraw <= g.my - 2;
-    // + side neighbours of an output cell on a non-periodic boundary read their own fp
-    const bool ownI = (iraw == g.mx - 2) && !iper, ownJ = (jraw == g.my - 2) && !jper;
+    const bool colI = i >= 1 && i <= g.mx - 2, colJ = j >= 1 && j <= g.my - 2;
+    const bool cell = lane >= 1 && r >= 1 && colI && colJ;
+    const bool bndI = i == 0 || i == g.mx - 2, bndJ = j == 0 || j == g.my - 2;
+    const bool seamI = colJ && i == g.mx - 1 && iper;       // ghost m-1 carries fp of cell 1
+    const bool seamJ = colI && j == g.my - 1 && jper;
+    const bool out = cell && lane <= TI && r <= TJ;
+    const bool ownI = (i == g.mx - 2) && !iper, ownJ = (j == g.my - 2) && !jper;
+    // clamped column for addressing
+    const int ia = min(i, g.mx - 1), ja = min(j, g.my - 1);
+    const int SJ = g.gxm, SK = g.gxm * g.gym;
+    const int c0 = g.L(0, ja, ia);
+    ColPtrs P;
+    P.ucont = A.ucont + 3 * (long)c0; P.nut = A.nut + c0;
+    P.fc[0] = A.icsi + 3 * (long)c0; P.fe[0] = A.ieta + 3 * (long)c0; P.fz[0] = A.izet + 3 * (long)c0;
+    P.fc[1] = A.jcsi + 3 * (long)c0; P.fe[1] = A.jeta + 3 * (long)c0; P.fz[1] = A.jzet + 3 * (long)c0;
+    P.fc[2] = A.kcsi + 3 * (long)c0; P.fe[2] = A.keta + 3 * (long)c0; P.fz[2] = A.kzet + 3 * (long)c0;
+    P.fa[0] = A.iaj + c0; P.fa[1] = A.jaj + c0; P.fa[2] = A.kaj + c0;
+    const double *pcsi = A.csi + 3 * (long)c0, *peta = A.eta + 3 * (long)c0, *pzet = A.zet + 3 * (long)c0;
+    const double *ptheta = A.theta + c0;
+    const long SG = (long)g.mx * g.my;
+    double *prhs = A.rhs + 3 * (long)(ja * g.mx + ia);
+    double *pbt = A.btheta + 3 * (long)(ja * g.mx + ia);
+    const bool buoy = A.btheta != A.rhs;   // the launcher passes btheta == rhs when Buoyancy is off
+    const int nI = g.my * g.mz, nJ = g.mx * g.mz, nK = g.mx * g.my;
+    const double *pF1 = A.bF1 + ((long)((i == 0) ? 0 : nI) + ja) * 6;       // + k * my * 6
+    const double *pF2 = A.bF2 + ((long)((j == 0) ? 0 : nJ) + ia) * 6;       // + k * mx * 6
+    const double *pF3 = A.bF3 + ((long)ja * g.mx + ia) * 6;                 // + (0 | nK) * 6
+    const double *pSI = A.sI + (long)ja * 3, *pSJ = A.sJ + (long)ia * 3, *pSK = A.sK + ((long)ja * g.mx + ia) * 3;
 
-    Flux F3p{V3{0, 0, 0}, V3{0, 0, 0}};
-    if (cell) {
-        const int kf = (kb - 1 == 0 && kper) ? g.mz - 2 : kb - 1;
-        F3p = face_flux<2>(A, g, ic, jc, kf);
+    // ucat ring: rows j0-2 .., cols i0-2 ..
+    const int ri0 = i0 - 2, rj0 = j0 - 2;
+    const int rowMax = min(RingT::RJ - 1, g.my + 2 - rj0), colMax = min(RingT::RI - 1, g.mx + 2 - ri0);
+    auto plane_load = [&](int k, double *buf) {
+        for (int n = 0; n < PRE; n++) {
+            const int idx = tid + n * 64 * NR;
+            buf[n] = 0.0;
+            if (idx < PLANE) {
+                const int row = idx / (3 * RingT::RI), e = idx % (3 * RingT::RI), col = e / 3;
+                if (row <= rowMax && col <= colMax) buf[n] = A.ucat[3 * (long)g.L(k, rj0 + row, ri0) + e];
+            }
+        }
+    };
+    auto plane_store = [&](int k, const double *buf) {
+        const int slot = k & 3;
+        for (int n = 0; n < PRE; n++) {
+            const int idx = tid + n * 64 * NR;
+            if (idx < PLANE) {
+                const int row = idx / (3 * RingT::RI), e = idx % (3 * RingT::RI);
+                ring.v[slot][e % 3][row][e / 3] = buf[n];
+            }
+        }
+    };
+    const int rrow = r + 1, rcol = lane + 1;
+    auto kface = [&](int k) -> Flux {          // k face k of this column
+        if (k == 0) return ldflux(pF3);
+        if (k == g.mz - 2) return ldflux(pF3 + (long)nK * 6);
+        return face_main<2, NR>(A, P, ring, (long)k * SK, rrow, rcol, k, SK);
+    };
+    // prime: k face kb-1 needs planes kb-2 .. kb+1, the first plane step kb-1 .. kb+2
+    Flux F3p;
+    {
+        double buf[PRE];
+        for (int k = kb - 2; k <= kb + 1; k++) { plane_load(k, buf); plane_store(k, buf); }
+        __syncthreads();
+        F3p = kface(kb - 1);
+        plane_load(kb + 2, buf);
+        __syncthreads();
+        plane_store(kb + 2, buf);
+        __syncthreads();
     }
     V3 fpPrev{0, 0, 0};
+    int buf = 0;
+    // values of plane t-1 the Rhs step reuses: inverse face Jacobians and the z components of the
+    // face bases (buoyancy), loaded with the face data of that plane
+    double fa0p = P.fa[0][(long)(kb - 1) * SK], fa1p = P.fa[1][(long)(kb - 1) * SK], fa2p = P.fa[2][(long)(kb - 1) * SK];
+    double gzIp = P.fc[0][3 * (long)(kb - 1) * SK + 2], gzJp = P.fe[1][3 * (long)(kb - 1) * SK + 2],
+           gzKp = P.fz[2][3 * (long)(kb - 1) * SK + 2];
     for (int t = kb; t <= ke; t++) {
-        // plane evaluated at this step: the periodic ghost plane m-1 is plane 1
+        // ---- every global load of this plane step, issued together ------------------------------
+        double pre[PRE];
+        const bool doPre = t + 3 <= ke + 2;
+        if (doPre) plane_load(t + 3, pre);
         const bool ghostK = (t == g.mz - 1);
-        const int kc = (ghostK && kper) ? 1 : t;
-        const bool evalPlane = !(ghostK && !kper);
+        const long q = (long)min(t, g.mz - 2) * SK;           // faces of plane t (clamped at the ghost plane)
+        const FaceIn in1 = face_in<0>(P, q, 1), in2 = face_in<1>(P, q, SJ), in3 = face_in<2>(P, q, SK);
+        const double nut0 = P.nut[q];
+        // ---- face fluxes ---------------------------------------------------------------------------
         Flux F1{V3{0, 0, 0}, V3{0, 0, 0}}, F2{V3{0, 0, 0}, V3{0, 0, 0}}, F3{V3{0, 0, 0}, V3{0, 0, 0}};
-        if (evalPlane) {
-            if (doF1) F1 = face_flux<0>(A, g, fi, jc, kc);
-            if (doF2) F2 = face_flux<1>(A, g, ic, fj, kc);
-            if (cell) F3 = face_flux<2>(A, g, ic, jc, kc);
+        if (!ghostK) {
+            F1 = face_pre<0, NR>(A, in1, nut0, ring, rrow, rcol, t);
+            F2 = face_pre<1, NR>(A, in2, nut0, ring, rrow, rcol, t);
+            if (t == g.mz - 2) F3 = ldflux(pF3 + (long)nK * 6);
+            else F3 = face_pre<2, NR>(A, in3, nut0, ring, rrow, rcol, t);
+            if (bndI) F1 = ldflux(pF1 + (long)t * g.my * 6);
+            if (bndJ) F2 = ldflux(pF2 + (long)t * g.mx * 6);
         }
-        // exchange: F1 of the lane below by shuffle (every lane takes part), F2 of the row below by LDS
-        V3 F1dm = shfl_up3(F1.d), F1vm = shfl_up3(F1.v);
+        // ---- loads of the Rhs / buoyancy step (cells of plane t-1), in flight across the exchange
+        const long qp = (long)(t - 1) * SK;
+        const V3 csA = ld3(pcsi + 3 * qp, 0), csB = ld3(pcsi + 3 * qp, 1);
+        const V3 etA = ld3(peta + 3 * qp, 0), etB = ld3(peta + 3 * qp, SJ);
+        const V3 zeA = ld3(pzet + 3 * qp, 0), zeB = ld3(pzet + 3 * qp, SK);
+        const long ro = 3 * (long)(t - 1) * SG;
+        const double r0 = prhs[ro], r1 = prhs[ro + 1], r2 = prhs[ro + 2];
+        double th = 0, thI = 0, thJ = 0, thK = 0;
+        if (buoy) { th = ptheta[qp]; thI = ptheta[qp + 1]; thJ = ptheta[qp + SJ]; thK = ptheta[qp + SK]; }
+        const V3 F1dm = shfl_up3(F1.d), F1vm = shfl_up3(F1.v);
         sF2[0][r][lane] = F2.d.x; sF2[1][r][lane] = F2.d.y; sF2[2][r][lane] = F2.d.z;
         sF2[3][r][lane] = F2.v.x; sF2[4][r][lane] = F2.v.y; sF2[5][r][lane] = F2.v.z;
         __syncthreads();
-        V3 fp{0, 0, 0};
-        if (cell) {
-            if (evalPlane) {
-                const int rm = r - 1;
-                V3 F2dm{sF2[0][rm][lane], sF2[1][rm][lane], sF2[2][rm][lane]};
-                V3 F2vm{sF2[3][rm][lane], sF2[4][rm][lane], sF2[5][rm][lane]};
-                fp.x = fpsum(F1.d.x, F1dm.x, F2.d.x, F2dm.x, F3.d.x, F3p.d.x, F1.v.x, F1vm.x, F2.v.x, F2vm.x, F3.v.x, F3p.v.x, A.mode);
-                fp.y = fpsum(F1.d.y, F1dm.y, F2.d.y, F2dm.y, F3.d.y, F3p.d.y, F1.v.y, F1vm.y, F2.v.y, F2vm.y, F3.v.y, F3p.v.y, A.mode);
-                fp.z = fpsum(F1.d.z, F1dm.z, F2.d.z, F2dm.z, F3.d.z, F3p.d.z, F1.v.z, F1vm.z, F2.v.z, F2vm.z, F3.v.z, F3p.v.z, A.mode);
-            } else {
-                fp = fpPrev;                      // non-periodic top: ghost fp copies cell m-2
-            }
+        const int rm = r > 0 ? r - 1 : 0;
+        const V3 F2dm{sF2[0][rm][lane], sF2[1][rm][lane], sF2[2][rm][lane]};
+        const V3 F2vm{sF2[3][rm][lane], sF2[4][rm][lane], sF2[5][rm][lane]};
+        V3 fp;
+        fp.x = fpsum(F1.d.x, F1dm.x, F2.d.x, F2dm.x, F3.d.x, F3p.d.x, F1.v.x, F1vm.x, F2.v.x, F2vm.x, F3.v.x, F3p.v.x, A.mode);
+        fp.y = fpsum(F1.d.y, F1dm.y, F2.d.y, F2dm.y, F3.d.y, F3p.d.y, F1.v.y, F1vm.y, F2.v.y, F2vm.y, F3.v.y, F3p.v.y, A.mode);
+        fp.z = fpsum(F1.d.z, F1dm.z, F2.d.z, F2dm.z, F3.d.z, F3p.d.z, F1.v.z, F1vm.z, F2.v.z, F2vm.z, F3.v.z, F3p.v.z, A.mode);
+        if (ghostK) {
+            if (kper) fp = V3{pSK[0], pSK[1], pSK[2]};
+            else fp = fpPrev;                        // non-periodic top: ghost fp copies cell m-2
+        } else if (seamI) {
+            const double *qq = pSI + (long)t * g.my * 3;
+            fp = V3{qq[0], qq[1], qq[2]};
+        } else if (seamJ) {
+            const double *qq = pSJ + (long)t * g.mx * 3;
+            fp = V3{qq[0], qq[1], qq[2]};
         }
         // Rhs of the previous plane (t-1): fp(t-1) own / i+1 (shuffle) / j+1 (LDS), fp(t) own
         V3 fpI = shfl_dn3(fpPrev);
+        const int rp = r + 1 < NR ? r + 1 : r;
+        V3 fJ{sFp[buf ^ 1][0][rp][lane], sFp[buf ^ 1][1][rp][lane], sFp[buf ^ 1][2][rp][lane]};
+        if (ownI) fpI = fpPrev;
+        if (ownJ) fJ = fpPrev;
         if (t > kb && out) {
-            const int k = t - 1;
-            const int c = g.L(k, jraw, iraw);
-            V3 fJ{sFp[0][r + 1][lane], sFp[1][r + 1][lane], sFp[2][r + 1][lane]};
-            if (ownI) fpI = fpPrev;
-            if (ownJ) fJ = fpPrev;
             const V3 fK = fp;
-            const int ro = g.G(k, jraw, iraw);
-            V3 a = ld3(A.csi, c), b = ld3(A.csi, g.L(k, jraw, iraw + 1));
-            double v1 = a.x * fpPrev.x + a.y * fpPrev.y + a.z * fpPrev.z;
-            double v2 = b.x * fpI.x + b.y * fpI.y + b.z * fpI.z;
-            double rx = A.scale * A.iaj[c] * (0.5 * (v1 + v2));
-            a = ld3(A.eta, c); b = ld3(A.eta, g.L(k, jraw + 1, iraw));
-            v1 = a.x * fpPrev.x + a.y * fpPrev.y + a.z * fpPrev.z;
-            v2 = b.x * fJ.x + b.y * fJ.y + b.z * fJ.z;
-            double ry = A.scale * A.jaj[c] * (0.5 * (v1 + v2));
-            a = ld3(A.zet, c); b = ld3(A.zet, g.L(k + 1, jraw, iraw));
-            v1 = a.x * fpPrev.x + a.y * fpPrev.y + a.z * fpPrev.z;
-            v2 = b.x * fK.x + b.y * fK.y + b.z * fK.z;
-            double rz = A.scale * A.kaj[c] * (0.5 * (v1 + v2));
-            A.rhs[3 * ro] += rx;
-            A.rhs[3 * ro + 1] += ry;
-            A.rhs[3 * ro + 2] += rz;
-            if (A.btheta) {
+            double v1 = csA.x * fpPrev.x + csA.y * fpPrev.y + csA.z * fpPrev.z;
+            double v2 = csB.x * fpI.x + csB.y * fpI.y + csB.z * fpI.z;
+            const double rx = A.scale * fa0p * (0.5 * (v1 + v2));
+            v1 = etA.x * fpPrev.x + etA.y * fpPrev.y + etA.z * fpPrev.z;
+            v2 = etB.x * fJ.x + etB.y * fJ.y + etB.z * fJ.z;
+            const double ry = A.scale * fa1p * (0.5 * (v1 + v2));
+            v1 = zeA.x * fpPrev.x + zeA.y * fpPrev.y + zeA.z * fpPrev.z;
+            v2 = zeB.x * fK.x + zeB.y * fK.y + zeB.z * fK.z;
+            const double rz = A.scale * fa2p * (0.5 * (v1 + v2));
+            prhs[ro] = r0 + rx;
+            prhs[ro + 1] = r1 + ry;
+            prhs[ro + 2] = r2 + rz;
+            if (buoy) {
                 // Boussinesq source on the three + faces of the cell (ueqn_sources.c:2812-2870):
                 // g . S with g = (0, 0, -9.81) is -9.81 * S.z exactly
-                const double th = A.theta[c], tr = A.tRef;
-                const double tI = 0.5 * (th + A.theta[g.L(k, jraw, iraw + 1)]);
-                const double tJ = 0.5 * (th + A.theta[g.L(k, jraw + 1, iraw)]);
-                const double tK = 0.5 * (th + A.theta[g.L(k + 1, jraw, iraw)]);
-                const double gx = -9.81 * A.icsi[3 * c + 2], gy = -9.81 * A.jeta[3 * c + 2], gz = -9.81 * A.kzet[3 * c + 2];
-                A.btheta[3 * ro] = A.bscale * (gx * (tr - tI) / tr);
-                A.btheta[3 * ro + 1] = A.bscale * (gy * (tr - tJ) / tr);
-                A.btheta[3 * ro + 2] = A.bscale * (gz * (tr - tK) / tr);
+                const double tI = 0.5 * (th + thI), tJ = 0.5 * (th + thJ), tK = 0.5 * (th + thK);
+                const double tr = 1.0 / A.invTRef;
+                pbt[ro] = A.bscale * ((-9.81 * gzIp) * (tr - tI) * A.invTRef);
+                pbt[ro + 1] = A.bscale * ((-9.81 * gzJp) * (tr - tJ) * A.invTRef);
+                pbt[ro + 2] = A.bscale * ((-9.81 * gzKp) * (tr - tK) * A.invTRef);
             }
         }
+        sFp[buf][0][r][lane] = fp.x; sFp[buf][1][r][lane] = fp.y; sFp[buf][2][r][lane] = fp.z;
+        if (doPre) plane_store(t + 3, pre);
         __syncthreads();
-        sFp[0][r][lane] = fp.x; sFp[1][r][lane] = fp.y; sFp[2][r][lane] = fp.z;
+        buf ^= 1;
         fpPrev = fp;
         F3p = F3;
+        fa0p = in1.fa; fa1p = in2.fa; fa2p = in3.fa;
+        gzIp = in1.fc.z; gzJp = in2.fe.z; gzKp = in3.fz.z;
     }
 }
 
-// zero bTheta on the boundary shell of the owned range (the reference zeroes the whole vector)
-__global__ void k_btheta_shell(double *bt, Geo g) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
-    if (i >= g.mx) return;
-    bool shell = i == 0 || i == g.mx - 1 || j == 0 || j == g.my - 1 || k == 0 || k == g.mz - 1;
-    if (!shell) return;
-    int r = g.G(k, j, i);
-    bt[3 * r] = 0.0; bt[3 * r + 1] = 0.0; bt[3 * r + 2] = 0.0;
-}
-
-void fused_formu(const Geo &g, const FormUArgs &A, hipStream_t s) {
+void fused_formu(const Geo &g, const FormUArgs &A, double *ws, hipStream_t s) {
     constexpr int NR = 8;
     const int TI = 62, TJ = NR - 2;
+    // boundary faces and seams first
+    BoundaryBufs B;
+    long off = 0;
+    int n1, n2;
+    for (int d = 0; d < 3; d++) { const long n = tr_dims(g, d, n1, n2); B.bF[d] = ws + off; off += 2 * n * 6; }
+    for (int d = 0; d < 3; d++) { const long n = tr_dims(g, d, n1, n2); B.seam[d] = ws + off; off += n * 3; }
+    const int nmax = max(g.my * g.mz, max(g.mx * g.mz, g.mx * g.my));
+    TOSCA_LAUNCH("k_formu_boundary", k_formu_boundary, dim3((nmax + 127) / 128, 1, 9), dim3(128), 0, s, A, g, B);
+
+    MainArgs M;
+    // the main kernel loads nu_t unconditionally; without LES (or without the viscous term) the
+    // caller passes no nu_t, so any valid local array stands in (its values are never used)
+    M.ucont = A.ucont; M.ucat = A.ucat; M.nut = A.nut ? A.nut : A.ucat;
+    M.csi = A.csi; M.eta = A.eta; M.zet = A.zet;
+    M.icsi = A.icsi; M.ieta = A.ieta; M.izet = A.izet; M.jcsi = A.jcsi; M.jeta = A.jeta; M.jzet = A.jzet;
+    M.kcsi = A.kcsi; M.keta = A.keta; M.kzet = A.kzet; M.iaj = A.iaj; M.jaj = A.jaj; M.kaj = A.kaj;
+    M.rhs = A.rhs;
+    M.theta = A.btheta ? A.theta : A.ucat;          // any valid local array when Buoyancy is off
+    M.btheta = A.btheta ? A.btheta : A.rhs;
+    M.bF1 = B.bF[0]; M.bF2 = B.bF[1]; M.bF3 = B.bF[2]; M.sI = B.seam[0]; M.sJ = B.seam[1]; M.sK = B.seam[2];
+    M.scale = A.scale; M.nu = A.nu; M.bscale = A.bscale; M.invTRef = 1.0 / (A.btheta ? A.tRef : 1.0);
+    M.mode = A.mode; M.quick = A.quick; M.les = A.les;
+
     const int nk = g.mz - 2;
-    // k chunks: enough workgroups to fill 256 CUs several times over
     const int tiles = ((g.mx - 2 + TI - 1) / TI) * ((g.my - 2 + TJ - 1) / TJ);
     int chunks = (2048 + tiles - 1) / tiles;
     chunks = max(1, min(chunks, nk / 8 > 0 ? nk / 8 : 1));
     const int kchunk = (nk + chunks - 1) / chunks;
     chunks = (nk + kchunk - 1) / kchunk;
     dim3 grid((g.mx - 2 + TI - 1) / TI, (g.my - 2 + TJ - 1) / TJ, chunks);
-    TOSCA_LAUNCH("k_fused_formu", k_fused_formu<NR>, grid, dim3(64, NR), 0, s, A, g, kchunk);
-    if (A.btheta)
-        TOSCA_LAUNCH("k_btheta_shell", k_btheta_shell, dim3((g.mx + 63) / 64, g.my, g.mz), dim3(64), 0, s, A.btheta, g);
+    TOSCA_LAUNCH("k_fused_formu", k_fused_formu<NR>, grid, dim3(64, NR), 0, s, M, g, kchunk);
+    // the reference zeroes all of bTheta; the kernel writes the interior cells, this the rest
+    if (A.btheta) zero_shell(A.btheta, 3, g.mx, g.my, g.mz, 0, 0, s);
+}
+
+size_t formu_ws_doubles(const Geo &g) {
+    long tot = 0;
+    int n1, n2;
+    for (int d = 0; d < 3; d++) tot += (long)tr_dims(g, d, n1, n2) * (12 + 3);
+    return (size_t)tot;
 }
 
 }  // namespace tosca

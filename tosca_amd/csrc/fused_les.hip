@@ -187,15 +187,33 @@ __global__ void __launch_bounds__(256) k_nut_fused(Geo g, const double *ucat, co
     }
 }
 
-__global__ void k_zero_outside(double *a, Geo g) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x - GW, j = blockIdx.y - GW, k = blockIdx.z - GW;
-    if (i >= g.mx + GW) return;
-    if (interior_cell(g, i, j, k)) return;
-    a[g.L(k, j, i)] = 0.0;
+// zero an ni x nj x nk box of an array with row length nx and plane size nx*ny (ncomp values per point)
+__global__ void k_zero_box(double *a, int ncomp, int nx, int ny, int i0, int j0, int k0, int ni) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ni * ncomp) return;
+    const long p = ((long)(k0 + blockIdx.z) * ny + (j0 + blockIdx.y)) * nx + i0;
+    a[p * ncomp + t] = 0.0;
+}
+
+void zero_shell(double *a, int ncomp, int nx, int ny, int nz, int lo, int hi, hipStream_t s) {
+    // boxes (in array coordinates) covering everything but [lo+1 .. n-hi-2] on every axis:
+    // `lo` / `hi` extra layers outside the boundary cells on the low / high side
+    const int w0 = lo + 1, w1 = hi + 1;            // shell thickness per side
+    auto box = [&](int i0, int j0, int k0, int ni, int nj, int nk) {
+        if (ni <= 0 || nj <= 0 || nk <= 0) return;
+        TOSCA_LAUNCH("k_zero_box", k_zero_box, dim3((ni * ncomp + 255) / 256, nj, nk), dim3(256), 0, s, a, ncomp, nx, ny,
+                     i0, j0, k0, ni);
+    };
+    box(0, 0, 0, nx, ny, w0);                                   // low k slab
+    box(0, 0, nz - w1, nx, ny, w1);                             // high k slab
+    box(0, 0, w0, nx, w0, nz - w0 - w1);                        // low j slab
+    box(0, ny - w1, w0, nx, w1, nz - w0 - w1);                  // high j slab
+    box(0, w0, w0, w0, ny - w0 - w1, nz - w0 - w1);             // low i slab
+    box(nx - w1, w0, w0, w1, ny - w0 - w1, nz - w0 - w1);       // high i slab
 }
 
 void zero_outside_interior(double *a, const Geo &g, hipStream_t s) {
-    TOSCA_LAUNCH("k_zero_outside", k_zero_outside, dim3((g.gxm + 63) / 64, g.gym, g.gzm), dim3(64), 0, s, a, g);
+    zero_shell(a, 1, g.gxm, g.gym, g.gzm, GW, GW, s);
 }
 
 void fused_dsm(const Geo &g, const tosca_mesh *m, const double *ucat, double maxCs, double *cs, double *nut,
