@@ -580,6 +580,9 @@ def default_params(periodic, bcs_u, bcs_nut, nut_values=(0.0,) * 6, div_scheme=D
 
 def _finish(name, lay, periodic, mesh, ucat, theta, bcs_u, u_values, bcs_t, params, meta):
     apply_vector_bcs(ucat, lay, periodic, bcs_u, u_values, mesh)
+    # ExplicitRhsU resets the periodic ghosts of lUcat, edges and corners included, before FormU
+    # (src/solvers/ueqn_solvers.c:1302, src/include/inline.h:5453-5606)
+    reset_cell_periodic(ucat, lay, periodic)
     apply_scalar_bcs(theta, lay, periodic, bcs_t, [params["t_ref"]] * 6)
     ucont = ucont_from_ucat(ucat, mesh, lay, periodic, bcs_u)
     return Case(name, lay, tuple(periodic), mesh, ucat, ucont, theta, params, tuple(bcs_u), meta)
