@@ -211,9 +211,12 @@ def main():
     dom_bytes = KERNEL_BYTES.get(dom)
     achieved = dom_bytes * lay.ncells / (dom_ms * 1e-3) / 1e9 if dom_bytes else None
     traffic = None
-    tfile = ROOT / "profiles" / f"traffic_{dom}_{WORKLOAD_TAG[args.config]}.json"
+    # HBM bytes per launch measured with rocprofv3 (FETCH_SIZE + WRITE_SIZE, calibrated on a copy)
+    tfile = ROOT / "profiles" / f"traffic_{dom}_{WORKLOAD_TAG[args.config]}.txt"
     if tfile.exists():
-        traffic = json.loads(tfile.read_text()).get("bytes_per_launch")
+        for ln in tfile.read_text().splitlines():
+            if ln.startswith("bytes_per_launch:"):
+                traffic = float(ln.split(":", 1)[1])
     copy_gbs = copy_peak_gbs(torch)
     print(f"# device copy: {copy_gbs:.0f} GB/s", file=sys.stderr, flush=True)
     roof = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
