@@ -175,3 +175,33 @@ def test_formu_random_multitile(rhs_mod, periodic, scheme):
 ])
 def test_step_multitile(rhs_mod, builder, kw):
     _step_parity(rhs_mod, builder(**kw), f"{builder.__name__} {kw}")
+
+
+@pytest.mark.parametrize("model", [C.LES_DSM, C.LES_DPASD])
+@pytest.mark.parametrize("periodic", [(2, 0, 1), (0, 0, 0)])
+def test_update_cs_uniform_velocity(rhs_mod, model, periodic):
+    """Uniform velocity: L = M = 0.  The dynamic coefficient is 0; the plane-averaged model divides
+    0 by beta = 0 and its clip (two plain comparisons) turns the NaN into maxCs."""
+    lay, mesh, f, params = random_case(n=(7, 6, 8), periodic=periodic, seed=2, les_model=model)
+    u = np.broadcast_to(np.array([1.5, -0.5, 0.25]), lay.gshape + (3,)).copy()
+    out = out_arrays(lay)
+    nm = np_mesh(mesh)
+    cs_o = out["cs"].copy()
+    O.update_cs(lay.info(), nm, params, u, cs_o)
+    ctx = rhs_mod.RhsContext(lay.info())
+    cs_g = to_dev(out["cs"])
+    ctx.UpdateCs(_dev_mesh(mesh), params, to_dev(u), cs_g)
+    torch.cuda.synchronize()
+    print(f"\nUpdateCs uniform velocity model={model} periodic={periodic}", flush=True)
+    assert np.array_equal(to_np(cs_g), cs_o)
+
+
+@pytest.mark.parametrize("t_ref", [300.0, 287.3])
+def test_step_buoyancy_near_tref(rhs_mod, t_ref):
+    """theta within 1e-3 K of t_ref: (t_ref - theta) cancels to a few digits, so any change in how
+    t_ref enters (a reciprocal, a re-derived t_ref) shows at the 1e-12 level."""
+    case = C.hill_case(nx=20, ny=12, nz=10, les_model=C.LES_DSM)
+    rng = np.random.Generator(np.random.PCG64(17))
+    case.params = dict(case.params, t_ref=t_ref)
+    case.theta = torch.as_tensor(t_ref + 1e-3 * rng.standard_normal(tuple(case.theta.shape)))
+    _step_parity(rhs_mod, case, f"buoyancy near t_ref={t_ref}")

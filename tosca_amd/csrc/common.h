@@ -126,29 +126,28 @@ __device__ __forceinline__ Grad du_k(const Geo &g, const double *u, int i, int j
     return r;
 }
 
-// Cell-centred velocity derivatives (Compute_du_center, src/include/inline.h:3344-3418).
-__device__ __forceinline__ Grad du_center(const Geo &g, const double *u, int i, int j, int k) {
+// Cell-centred velocity derivatives (Compute_du_center, src/include/inline.h:3344-3418), from the
+// cell's value c and its six neighbours: central differences, one-sided at the first and last
+// interior cell of a non-periodic direction.
+__device__ __forceinline__ Grad du_center_v(const Geo &g, int i, int j, int k, V3 c, V3 im, V3 ip, V3 jm, V3 jp,
+                                            V3 km, V3 kp) {
     Grad r;
-    V3 c = ld3(u, g.L(k, j, i));
-    {
-        V3 p = ld3(u, g.L(k, j, i + 1)), m = ld3(u, g.L(k, j, i - 1));
-        if (g.nonper(0) && i == g.mx - 2) { r.dudc = c.x - m.x; r.dvdc = c.y - m.y; r.dwdc = c.z - m.z; }
-        else if (g.nonper(0) && i == 1) { r.dudc = p.x - c.x; r.dvdc = p.y - c.y; r.dwdc = p.z - c.z; }
-        else { r.dudc = (p.x - m.x) * 0.5; r.dvdc = (p.y - m.y) * 0.5; r.dwdc = (p.z - m.z) * 0.5; }
-    }
-    {
-        V3 p = ld3(u, g.L(k, j + 1, i)), m = ld3(u, g.L(k, j - 1, i));
-        if (g.nonper(1) && j == g.my - 2) { r.dude = c.x - m.x; r.dvde = c.y - m.y; r.dwde = c.z - m.z; }
-        else if (g.nonper(1) && j == 1) { r.dude = p.x - c.x; r.dvde = p.y - c.y; r.dwde = p.z - c.z; }
-        else { r.dude = (p.x - m.x) * 0.5; r.dvde = (p.y - m.y) * 0.5; r.dwde = (p.z - m.z) * 0.5; }
-    }
-    {
-        V3 p = ld3(u, g.L(k + 1, j, i)), m = ld3(u, g.L(k - 1, j, i));
-        if (g.nonper(2) && k == g.mz - 2) { r.dudz = c.x - m.x; r.dvdz = c.y - m.y; r.dwdz = c.z - m.z; }
-        else if (g.nonper(2) && k == 1) { r.dudz = p.x - c.x; r.dvdz = p.y - c.y; r.dwdz = p.z - c.z; }
-        else { r.dudz = (p.x - m.x) * 0.5; r.dvdz = (p.y - m.y) * 0.5; r.dwdz = (p.z - m.z) * 0.5; }
-    }
+    if (g.nonper(0) && i == g.mx - 2) { r.dudc = c.x - im.x; r.dvdc = c.y - im.y; r.dwdc = c.z - im.z; }
+    else if (g.nonper(0) && i == 1) { r.dudc = ip.x - c.x; r.dvdc = ip.y - c.y; r.dwdc = ip.z - c.z; }
+    else { r.dudc = (ip.x - im.x) * 0.5; r.dvdc = (ip.y - im.y) * 0.5; r.dwdc = (ip.z - im.z) * 0.5; }
+    if (g.nonper(1) && j == g.my - 2) { r.dude = c.x - jm.x; r.dvde = c.y - jm.y; r.dwde = c.z - jm.z; }
+    else if (g.nonper(1) && j == 1) { r.dude = jp.x - c.x; r.dvde = jp.y - c.y; r.dwde = jp.z - c.z; }
+    else { r.dude = (jp.x - jm.x) * 0.5; r.dvde = (jp.y - jm.y) * 0.5; r.dwde = (jp.z - jm.z) * 0.5; }
+    if (g.nonper(2) && k == g.mz - 2) { r.dudz = c.x - km.x; r.dvdz = c.y - km.y; r.dwdz = c.z - km.z; }
+    else if (g.nonper(2) && k == 1) { r.dudz = kp.x - c.x; r.dvdz = kp.y - c.y; r.dwdz = kp.z - c.z; }
+    else { r.dudz = (kp.x - km.x) * 0.5; r.dvdz = (kp.y - km.y) * 0.5; r.dwdz = (kp.z - km.z) * 0.5; }
     return r;
+}
+
+__device__ __forceinline__ Grad du_center(const Geo &g, const double *u, int i, int j, int k) {
+    return du_center_v(g, i, j, k, ld3(u, g.L(k, j, i)), ld3(u, g.L(k, j, i - 1)), ld3(u, g.L(k, j, i + 1)),
+                       ld3(u, g.L(k, j - 1, i)), ld3(u, g.L(k, j + 1, i)), ld3(u, g.L(k - 1, j, i)),
+                       ld3(u, g.L(k + 1, j, i)));
 }
 
 // Compute_du_dxyz (src/include/inline.h:3635-3663)
@@ -228,16 +227,18 @@ __device__ __forceinline__ void face2cell4(int ptype, int f, int m, int &L, int 
 // Germano contraction (les.c:715-799): num = L_ba M_aq G_bq, denom = M_nm M_nl G_ml.
 __device__ __forceinline__ void germano(const double L[3][3], const double Sh[3][3], const double SSh[3][3], double filter,
                         double test_filter, V3 c0, V3 e0, V3 z0, double &num_out, double &den_out) {
+    // pow(x, 2.) of the reference is x * x (exactly rounded); the inverse uses one reciprocal of det
     double S_hat = 0;
     for (int a = 0; a < 3; a++)
-        for (int b = 0; b < 3; b++) S_hat += pow(Sh[a][b], 2.);
+        for (int b = 0; b < 3; b++) S_hat += Sh[a][b] * Sh[a][b];
     S_hat = sqrt(2 * S_hat);
     const double a11 = c0.x, a12 = c0.y, a13 = c0.z, a21 = e0.x, a22 = e0.y, a23 = e0.z, a31 = z0.x, a32 = z0.y, a33 = z0.z;
     double det = a11 * (a33 * a22 - a32 * a23) - a21 * (a33 * a12 - a32 * a13) + a31 * (a23 * a12 - a22 * a13);
+    const double rd = 1.0 / det;
     double gc[3][3];
-    gc[0][0] = (a33 * a22 - a32 * a23) / det, gc[0][1] = -(a33 * a12 - a32 * a13) / det, gc[0][2] = (a23 * a12 - a22 * a13) / det;
-    gc[1][0] = -(a33 * a21 - a31 * a23) / det, gc[1][1] = (a33 * a11 - a31 * a13) / det, gc[1][2] = -(a23 * a11 - a21 * a13) / det;
-    gc[2][0] = (a32 * a21 - a31 * a22) / det, gc[2][1] = -(a32 * a11 - a31 * a12) / det, gc[2][2] = (a22 * a11 - a21 * a12) / det;
+    gc[0][0] = (a33 * a22 - a32 * a23) * rd, gc[0][1] = -(a33 * a12 - a32 * a13) * rd, gc[0][2] = (a23 * a12 - a22 * a13) * rd;
+    gc[1][0] = -(a33 * a21 - a31 * a23) * rd, gc[1][1] = (a33 * a11 - a31 * a13) * rd, gc[1][2] = -(a23 * a11 - a21 * a13) * rd;
+    gc[2][0] = (a32 * a21 - a31 * a22) * rd, gc[2][1] = -(a32 * a11 - a31 * a12) * rd, gc[2][2] = (a22 * a11 - a21 * a12) * rd;
     double xcsi = gc[0][0], xeta = gc[0][1], xzet = gc[0][2];
     double ycsi = gc[1][0], yeta = gc[1][1], yzet = gc[1][2];
     double zcsi = gc[2][0], zeta = gc[2][1], zzet = gc[2][2];
@@ -248,7 +249,7 @@ __device__ __forceinline__ void germano(const double L[3][3], const double Sh[3]
     G[0][1] = G[1][0] = xeta * xcsi + yeta * ycsi + zeta * zcsi;
     G[0][2] = G[2][0] = xzet * xcsi + yzet * ycsi + zzet * zcsi;
     G[1][2] = G[2][1] = xeta * xzet + yeta * yzet + zeta * zzet;
-    double tf2 = pow(test_filter, 2.), f2 = pow(filter, 2.);
+    double tf2 = test_filter * test_filter, f2 = filter * filter;
     double M[3][3];
     for (int a = 0; a < 3; a++) {
         double Mc0 = (-tf2 * S_hat * Sh[a][0] + f2 * SSh[a][0]);
@@ -271,5 +272,21 @@ __device__ __forceinline__ void germano(const double L[3][3], const double Sh[3]
 
 __device__ __forceinline__ double pmax(double a, double b) { return (a < b) ? b : a; }
 __device__ __forceinline__ double pmin(double a, double b) { return (a < b) ? a : b; }
+
+// MI355X dispatches the workgroups of a launch to its 8 XCDs round-robin (linear id % 8), and each
+// XCD has its own L2.  xcd_tile maps the linear workgroup id to a tile id such that every XCD
+// receives one contiguous range of tiles: neighbouring tiles, which read each other's halos, then
+// run on the same XCD at about the same time and share those halos in its L2.
+#ifndef TOSCA_XCD_TILES
+#define TOSCA_XCD_TILES 1
+#endif
+constexpr int NUM_XCD = 8;
+__device__ __forceinline__ int xcd_tile() {
+    const int n = gridDim.x * gridDim.y * gridDim.z;
+    const int w = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    if (!TOSCA_XCD_TILES) return w;
+    const int q = n / NUM_XCD, r = n % NUM_XCD, x = w % NUM_XCD, slot = w / NUM_XCD;
+    return (x < r) ? x * (q + 1) + slot : r * (q + 1) + (x - r) * q + slot;
+}
 
 }  // namespace tosca

@@ -279,7 +279,7 @@ struct MainArgs {
     const double *theta;
     double *btheta;
     const double *bF1, *bF2, *bF3, *sI, *sJ, *sK;
-    double scale, nu, bscale, invTRef;
+    double scale, nu, bscale, tRef;
     int mode, quick, les;
 };
 
@@ -406,8 +406,11 @@ __global__ void __launch_bounds__(64 * NR) k_fused_formu(MainArgs A, Geo g, int 
     constexpr int PRE = (PLANE + 64 * NR - 1) / (64 * NR);
     const int tid = threadIdx.x + 64 * threadIdx.y;
     const int lane = threadIdx.x, r = threadIdx.y;
-    const int i0 = 1 + blockIdx.x * TI, j0 = 1 + blockIdx.y * TJ;
-    const int kb = 1 + blockIdx.z * kchunk;
+    // tile order: j fastest, then i, then the k chunk (a j column of tiles shares its j halos)
+    const int tile = xcd_tile();
+    const int bj = tile % gridDim.y, bi = (tile / gridDim.y) % gridDim.x, bk = tile / (gridDim.x * gridDim.y);
+    const int i0 = 1 + bi * TI, j0 = 1 + bj * TJ;
+    const int kb = 1 + bk * kchunk;
     const int ke = min(kb + kchunk, g.mz - 1);      // output planes [kb, ke)
     const int i = i0 - 1 + lane, j = j0 - 1 + r;
     const bool iper = g.per[0] != TOSCA_PERIODIC_NONE, jper = g.per[1] != TOSCA_PERIODIC_NONE,
@@ -562,10 +565,9 @@ __global__ void __launch_bounds__(64 * NR) k_fused_formu(MainArgs A, Geo g, int 
                 // Boussinesq source on the three + faces of the cell (ueqn_sources.c:2812-2870):
                 // g . S with g = (0, 0, -9.81) is -9.81 * S.z exactly
                 const double tI = 0.5 * (th + thI), tJ = 0.5 * (th + thJ), tK = 0.5 * (th + thK);
-                const double tr = 1.0 / A.invTRef;
-                pbt[ro] = A.bscale * ((-9.81 * gzIp) * (tr - tI) * A.invTRef);
-                pbt[ro + 1] = A.bscale * ((-9.81 * gzJp) * (tr - tJ) * A.invTRef);
-                pbt[ro + 2] = A.bscale * ((-9.81 * gzKp) * (tr - tK) * A.invTRef);
+                pbt[ro] = A.bscale * ((-9.81 * gzIp) * (A.tRef - tI) / A.tRef);
+                pbt[ro + 1] = A.bscale * ((-9.81 * gzJp) * (A.tRef - tJ) / A.tRef);
+                pbt[ro + 2] = A.bscale * ((-9.81 * gzKp) * (A.tRef - tK) / A.tRef);
             }
         }
         sFp[buf][0][r][lane] = fp.x; sFp[buf][1][r][lane] = fp.y; sFp[buf][2][r][lane] = fp.z;
@@ -602,7 +604,7 @@ void fused_formu(const Geo &g, const FormUArgs &A, double *ws, hipStream_t s) {
     M.theta = A.btheta ? A.theta : A.ucat;          // any valid local array when Buoyancy is off
     M.btheta = A.btheta ? A.btheta : A.rhs;
     M.bF1 = B.bF[0]; M.bF2 = B.bF[1]; M.bF3 = B.bF[2]; M.sI = B.seam[0]; M.sJ = B.seam[1]; M.sK = B.seam[2];
-    M.scale = A.scale; M.nu = A.nu; M.bscale = A.bscale; M.invTRef = 1.0 / (A.btheta ? A.tRef : 1.0);
+    M.scale = A.scale; M.nu = A.nu; M.bscale = A.bscale; M.tRef = A.btheta ? A.tRef : 1.0;
     M.mode = A.mode; M.quick = A.quick; M.les = A.les;
 
     const int nk = g.mz - 2;
