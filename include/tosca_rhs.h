@@ -163,6 +163,18 @@ int tosca_rhs_les_step(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_param
                        double *rhs, double scale, double *cs, double *nut,
                        double *btheta, double buoy_scale);
 
+/* Mixed-precision form of tosca_rhs_les_step (no reference counterpart: a new precision mode of
+ * the same unit of work).  The flow fields are FP32 (ucont, ucat, theta in; nut out); the metrics
+ * and Jacobians stay FP64 (tosca_mesh), as do rhs (accumulated), cs and btheta.  Arithmetic:
+ * velocity, strain and face-flux arithmetic in FP32; the cell-face flux differences (fp), the
+ * face dot into rhs, the buoyancy source, the test-filter sums, the Germano contraction, the box
+ * average and its denominator in FP64.  LES models: SMAGORINSKY and DSM (others:
+ * TOSCA_ERR_UNSUPPORTED). */
+int tosca_rhs_les_step_mixed(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_params *params,
+                             const float *ucont, const float *ucat, const float *theta,
+                             double *rhs, double scale, double *cs, float *nut,
+                             double *btheta, double buoy_scale);
+
 #ifdef __cplusplus
 }
 #endif

@@ -185,6 +185,52 @@ size_t ws_need(const tosca_ctx *c) {
                     3 * (size_t)c->geo.nloc() + tosca::formu_ws_doubles(c->geo) + 64);
 }
 
+// The fused UpdateCs -> UpdateNut -> Buoyancy -> FormU sequence on the context's stream, for
+// double (FP64 path) or float (mixed path) flow fields; arguments are validated by the caller.
+template <class T>
+void queue_step(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const T *ucont, const T *ucat,
+                const T *theta, double *rhs, double scale, double *cs, T *nut, double *btheta, double buoy_scale) {
+    const Geo &g = c->geo;
+    hipStream_t s = c->stream;
+    // UpdateCs (+ the interior of UpdateNut)
+    if (p->les_model == TOSCA_LES_DSM) {
+        tosca::fused_dsm<T>(g, m, ucat, p->max_cs, cs, nut, c->ws, s);
+    } else if (p->les_model == TOSCA_LES_SMAGORINSKY) {
+        tosca::launch_fill(cs, (long)g.nloc(), 0.0289, s);   // VecSet(lCs, std_cs), les.c:172-179
+        tosca::fused_nut<T>(g, m, ucat, cs, nut, nullptr, s);
+    } else {
+        // plane-averaged dynamic model: reference-structured kernels (FP64 flow fields only)
+        tosca::ref_update_cs(g, m, p, (const double *)ucat, cs, c->ws, s);
+        tosca::fused_nut<T>(g, m, ucat, cs, nut, nullptr, s);
+    }
+    if (p->les_model == TOSCA_LES_DSM) {
+        tosca::launch_wrap(cs, 1, g, s);
+        tosca::launch_reset_cell_periodic(cs, 1, g, s);
+    }
+    // UpdateNut: zero outside the interior, boundary conditions, ghost wrap
+    tosca::zero_outside_interior(nut, g, s);
+    tosca::nut_bcs(nut, p, g, s);
+    // Buoyancy + FormU
+    tosca::FormUArgsT<T> A{};
+    A.ucont = ucont; A.ucat = ucat; A.nut = nut;
+    A.csi = m->csi; A.eta = m->eta; A.zet = m->zet;
+    A.icsi = m->icsi; A.ieta = m->ieta; A.izet = m->izet;
+    A.jcsi = m->jcsi; A.jeta = m->jeta; A.jzet = m->jzet;
+    A.kcsi = m->kcsi; A.keta = m->keta; A.kzet = m->kzet;
+    A.iaj = m->iaj; A.jaj = m->jaj; A.kaj = m->kaj;
+    A.tauI = p->tau_wall_i; A.tauJ = p->tau_wall_j;
+    A.rhs = rhs; A.scale = scale; A.mode = 0;
+    A.quick = p->div_scheme == TOSCA_DIV_QUICK;
+    A.les = 1;
+    A.nu = p->nu;
+    for (int q = 0; q < 6; q++) A.bc_u[q] = p->bc_u[q];
+    A.theta = theta;
+    A.btheta = btheta;
+    A.bscale = buoy_scale;
+    A.tRef = p->t_ref;
+    tosca::fused_formu<T>(g, A, c->ws + 3 * (size_t)g.nloc(), s);
+}
+
 }  // namespace
 
 extern "C" {
@@ -361,7 +407,7 @@ int tosca_update_cs(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, co
     }
     if ((rc = ensure_ws(c, ws_need(c)))) return rc;
     if (!c->ref_path && p->les_model == TOSCA_LES_DSM) {
-        tosca::fused_dsm(c->geo, m, ucat, p->max_cs, cs, nullptr, c->ws, c->stream);
+        tosca::fused_dsm<double>(c->geo, m, ucat, p->max_cs, cs, nullptr, c->ws, c->stream);
         tosca::launch_wrap(cs, 1, c->geo, c->stream);
         tosca::launch_reset_cell_periodic(cs, 1, c->geo, c->stream);
     } else {
@@ -446,29 +492,39 @@ int tosca_rhs_les_step(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p,
     if (p->bc_u[2] == TOSCA_BCU_WALLFUNCTION || p->bc_u[3] == TOSCA_BCU_WALLFUNCTION)
         if ((rc = check_dev(c, p->tau_wall_j, 3 * (size_t)g.mz * g.mx * sizeof(double), "tau_wall_j"))) return rc;
     if ((rc = ensure_ws(c, ws_need(c)))) return rc;
-    hipStream_t s = c->stream;
-    // UpdateCs (+ the interior of UpdateNut)
-    if (p->les_model == TOSCA_LES_DSM) {
-        tosca::fused_dsm(g, m, ucat, p->max_cs, cs, nut, c->ws, s);
-    } else {
-        tosca::ref_update_cs(g, m, p, ucat, cs, c->ws, s);
-        tosca::fused_nut(g, m, ucat, cs, nut, nullptr, s);
-    }
-    if (p->les_model == TOSCA_LES_DSM) {
-        tosca::launch_wrap(cs, 1, g, s);
-        tosca::launch_reset_cell_periodic(cs, 1, g, s);
-    }
-    // UpdateNut: zero outside the interior, boundary conditions, ghost wrap
-    tosca::zero_outside_interior(nut, g, s);
-    tosca::nut_bcs(nut, p, g, s);
-    // Buoyancy + FormU
-    tosca::FormUArgs A = formu_args(m, p, ucont, ucat, nut, rhs, scale, 0);
-    A.theta = theta;
-    A.btheta = btheta;
-    A.bscale = buoy_scale;
-    A.tRef = p->t_ref;
-    tosca::fused_formu(g, A, c->ws + 3 * (size_t)g.nloc(), s);
+    queue_step<double>(c, m, p, ucont, ucat, theta, rhs, scale, cs, nut, btheta, buoy_scale);
     return launched(c, "rhs_les_step");
+}
+
+int tosca_rhs_les_step_mixed(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const float *ucont,
+                             const float *ucat, const float *theta, double *rhs, double scale, double *cs, float *nut,
+                             double *btheta, double buoy_scale) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if ((rc = check_params(c, p))) return rc;
+    if (!p->les_active) return set_err(c, TOSCA_ERR_ARG, "tosca_rhs_les_step_mixed needs les_active = 1");
+    if (p->les_model != TOSCA_LES_SMAGORINSKY && p->les_model != TOSCA_LES_DSM)
+        return set_err(c, TOSCA_ERR_UNSUPPORTED, "tosca_rhs_les_step_mixed: LES model %d (supported: Smagorinsky 0, "
+                                                 "dynamic Smagorinsky 2)", p->les_model);
+    if (p->div_scheme != TOSCA_DIV_CENTRAL && p->div_scheme != TOSCA_DIV_QUICK)
+        return set_err(c, TOSCA_ERR_UNSUPPORTED, "div_scheme = %d: only central (0) and QUICK (1) are supported", p->div_scheme);
+    if (!(p->t_ref != 0.0)) return set_err(c, TOSCA_ERR_ARG, "t_ref must be non-zero");
+    set_geo_per(c, p);
+    const Geo &g = c->geo;
+    const size_t n = (size_t)g.nloc(), v = 3 * n * sizeof(float), ng = 3 * (size_t)g.mx * g.my * g.mz * sizeof(double);
+    if ((rc = check_mesh(c, m, true, true, true))) return rc;
+    if ((rc = check_dev(c, ucont, v, "ucont")) || (rc = check_dev(c, ucat, v, "ucat")) ||
+        (rc = check_dev(c, theta, n * sizeof(float), "theta")) || (rc = check_dev(c, rhs, ng, "rhs")) ||
+        (rc = check_dev(c, cs, n * sizeof(double), "cs")) || (rc = check_dev(c, nut, n * sizeof(float), "nut")) ||
+        (rc = check_dev(c, btheta, ng, "btheta")))
+        return rc;
+    if (p->bc_u[0] == TOSCA_BCU_WALLFUNCTION || p->bc_u[1] == TOSCA_BCU_WALLFUNCTION)
+        if ((rc = check_dev(c, p->tau_wall_i, 3 * (size_t)g.mz * g.my * sizeof(double), "tau_wall_i"))) return rc;
+    if (p->bc_u[2] == TOSCA_BCU_WALLFUNCTION || p->bc_u[3] == TOSCA_BCU_WALLFUNCTION)
+        if ((rc = check_dev(c, p->tau_wall_j, 3 * (size_t)g.mz * g.mx * sizeof(double), "tau_wall_j"))) return rc;
+    if ((rc = ensure_ws(c, ws_need(c)))) return rc;
+    queue_step<float>(c, m, p, ucont, ucat, theta, rhs, scale, cs, nut, btheta, buoy_scale);
+    return launched(c, "rhs_les_step_mixed");
 }
 
 }  // extern "C"

@@ -12,7 +12,14 @@ namespace tosca {
 
 constexpr int GW = 3;   // DMDA ghost width (src/mesh.c:42)
 
-struct V3 { double x, y, z; };
+// vectors of the flow-field type T (double on the FP64 path, float for the FP32 flow fields of the
+// mixed path); metrics are always double
+template <class T> struct Vec3 { T x, y, z; };
+using V3 = Vec3<double>;
+using V3f = Vec3<float>;
+template <class T, class S> __host__ __device__ __forceinline__ Vec3<T> cvt3(Vec3<S> a) {
+    return Vec3<T>{static_cast<T>(a.x), static_cast<T>(a.y), static_cast<T>(a.z)};
+}
 
 // Grid geometry passed by value to every kernel.
 struct Geo {
@@ -29,99 +36,106 @@ struct Geo {
     __host__ __device__ __forceinline__ int nloc() const { return gxm * gym * gzm; }
 };
 
-__device__ __forceinline__ V3 ld3(const double *p, int c) { return V3{p[3 * c], p[3 * c + 1], p[3 * c + 2]}; }
+template <class T> __device__ __forceinline__ Vec3<T> ld3(const T *p, int c) {
+    return Vec3<T>{p[3 * c], p[3 * c + 1], p[3 * c + 2]};
+}
 __device__ __forceinline__ void st3(double *p, int c, V3 v) { p[3 * c] = v.x; p[3 * c + 1] = v.y; p[3 * c + 2] = v.z; }
 
-struct Grad { double dudc, dvdc, dwdc, dude, dvde, dwde, dudz, dvdz, dwdz; };
-struct Cart { double du_dx, du_dy, du_dz, dv_dx, dv_dy, dv_dz, dw_dx, dw_dy, dw_dz; };
+template <class T> struct GradT { T dudc, dvdc, dwdc, dude, dvde, dwde, dudz, dvdz, dwdz; };
+template <class T> struct CartT { T du_dx, du_dy, du_dz, dv_dx, dv_dy, dv_dz, dw_dx, dw_dy, dw_dz; };
+using Grad = GradT<double>;
+using Cart = CartT<double>;
 
 // Velocity derivatives along the curvilinear directions at an i-face (Compute_du_i,
 // src/include/inline.h:2748-2806) with no IBM / overset cells.
-__device__ __forceinline__ Grad du_i(const Geo &g, const double *u, int i, int j, int k) {
+template <class T>
+__device__ __forceinline__ GradT<T> du_i(const Geo &g, const T *u, int i, int j, int k) {
     const bool iN = g.nonper(0), jN = g.nonper(1), kN = g.nonper(2);
-    Grad r;
-    V3 a = ld3(u, g.L(k, j, i + 1)), b = ld3(u, g.L(k, j, i));
+    GradT<T> r;
+    Vec3<T> a = ld3(u, g.L(k, j, i + 1)), b = ld3(u, g.L(k, j, i));
     r.dudc = a.x - b.x; r.dvdc = a.y - b.y; r.dwdc = a.z - b.z;
     const bool ib = (i == 0 || i == g.mx - 2);
     if (j == g.my - 2 && ib && (jN || iN)) {
-        V3 p0 = ld3(u, g.L(k, j, i + 1)), p1 = ld3(u, g.L(k, j, i)), m0 = ld3(u, g.L(k, j - 1, i + 1)), m1 = ld3(u, g.L(k, j - 1, i));
-        r.dude = (p0.x + p1.x - m0.x - m1.x) * 0.5; r.dvde = (p0.y + p1.y - m0.y - m1.y) * 0.5; r.dwde = (p0.z + p1.z - m0.z - m1.z) * 0.5;
+        Vec3<T> p0 = ld3(u, g.L(k, j, i + 1)), p1 = ld3(u, g.L(k, j, i)), m0 = ld3(u, g.L(k, j - 1, i + 1)), m1 = ld3(u, g.L(k, j - 1, i));
+        r.dude = (p0.x + p1.x - m0.x - m1.x) * T(0.5); r.dvde = (p0.y + p1.y - m0.y - m1.y) * T(0.5); r.dwde = (p0.z + p1.z - m0.z - m1.z) * T(0.5);
     } else if (j == 1 && ib && (jN || iN)) {
-        V3 p0 = ld3(u, g.L(k, j + 1, i + 1)), p1 = ld3(u, g.L(k, j + 1, i)), m0 = ld3(u, g.L(k, j, i + 1)), m1 = ld3(u, g.L(k, j, i));
-        r.dude = (p0.x + p1.x - m0.x - m1.x) * 0.5; r.dvde = (p0.y + p1.y - m0.y - m1.y) * 0.5; r.dwde = (p0.z + p1.z - m0.z - m1.z) * 0.5;
+        Vec3<T> p0 = ld3(u, g.L(k, j + 1, i + 1)), p1 = ld3(u, g.L(k, j + 1, i)), m0 = ld3(u, g.L(k, j, i + 1)), m1 = ld3(u, g.L(k, j, i));
+        r.dude = (p0.x + p1.x - m0.x - m1.x) * T(0.5); r.dvde = (p0.y + p1.y - m0.y - m1.y) * T(0.5); r.dwde = (p0.z + p1.z - m0.z - m1.z) * T(0.5);
     } else {
-        V3 p0 = ld3(u, g.L(k, j + 1, i + 1)), p1 = ld3(u, g.L(k, j + 1, i)), m0 = ld3(u, g.L(k, j - 1, i + 1)), m1 = ld3(u, g.L(k, j - 1, i));
-        r.dude = (p0.x + p1.x - m0.x - m1.x) * 0.25; r.dvde = (p0.y + p1.y - m0.y - m1.y) * 0.25; r.dwde = (p0.z + p1.z - m0.z - m1.z) * 0.25;
+        Vec3<T> p0 = ld3(u, g.L(k, j + 1, i + 1)), p1 = ld3(u, g.L(k, j + 1, i)), m0 = ld3(u, g.L(k, j - 1, i + 1)), m1 = ld3(u, g.L(k, j - 1, i));
+        r.dude = (p0.x + p1.x - m0.x - m1.x) * T(0.25); r.dvde = (p0.y + p1.y - m0.y - m1.y) * T(0.25); r.dwde = (p0.z + p1.z - m0.z - m1.z) * T(0.25);
     }
     if (k == g.mz - 2 && ib && (kN || iN)) {
-        V3 p0 = ld3(u, g.L(k, j, i + 1)), p1 = ld3(u, g.L(k, j, i)), m0 = ld3(u, g.L(k - 1, j, i + 1)), m1 = ld3(u, g.L(k - 1, j, i));
-        r.dudz = (p0.x + p1.x - m0.x - m1.x) * 0.5; r.dvdz = (p0.y + p1.y - m0.y - m1.y) * 0.5; r.dwdz = (p0.z + p1.z - m0.z - m1.z) * 0.5;
+        Vec3<T> p0 = ld3(u, g.L(k, j, i + 1)), p1 = ld3(u, g.L(k, j, i)), m0 = ld3(u, g.L(k - 1, j, i + 1)), m1 = ld3(u, g.L(k - 1, j, i));
+        r.dudz = (p0.x + p1.x - m0.x - m1.x) * T(0.5); r.dvdz = (p0.y + p1.y - m0.y - m1.y) * T(0.5); r.dwdz = (p0.z + p1.z - m0.z - m1.z) * T(0.5);
     } else if (k == 1 && ib && (kN || iN)) {
-        V3 p0 = ld3(u, g.L(k + 1, j, i + 1)), p1 = ld3(u, g.L(k + 1, j, i)), m0 = ld3(u, g.L(k, j, i + 1)), m1 = ld3(u, g.L(k, j, i));
-        r.dudz = (p0.x + p1.x - m0.x - m1.x) * 0.5; r.dvdz = (p0.y + p1.y - m0.y - m1.y) * 0.5; r.dwdz = (p0.z + p1.z - m0.z - m1.z) * 0.5;
+        Vec3<T> p0 = ld3(u, g.L(k + 1, j, i + 1)), p1 = ld3(u, g.L(k + 1, j, i)), m0 = ld3(u, g.L(k, j, i + 1)), m1 = ld3(u, g.L(k, j, i));
+        r.dudz = (p0.x + p1.x - m0.x - m1.x) * T(0.5); r.dvdz = (p0.y + p1.y - m0.y - m1.y) * T(0.5); r.dwdz = (p0.z + p1.z - m0.z - m1.z) * T(0.5);
     } else {
-        V3 p0 = ld3(u, g.L(k + 1, j, i + 1)), p1 = ld3(u, g.L(k + 1, j, i)), m0 = ld3(u, g.L(k - 1, j, i + 1)), m1 = ld3(u, g.L(k - 1, j, i));
-        r.dudz = (p0.x + p1.x - m0.x - m1.x) * 0.25; r.dvdz = (p0.y + p1.y - m0.y - m1.y) * 0.25; r.dwdz = (p0.z + p1.z - m0.z - m1.z) * 0.25;
+        Vec3<T> p0 = ld3(u, g.L(k + 1, j, i + 1)), p1 = ld3(u, g.L(k + 1, j, i)), m0 = ld3(u, g.L(k - 1, j, i + 1)), m1 = ld3(u, g.L(k - 1, j, i));
+        r.dudz = (p0.x + p1.x - m0.x - m1.x) * T(0.25); r.dvdz = (p0.y + p1.y - m0.y - m1.y) * T(0.25); r.dwdz = (p0.z + p1.z - m0.z - m1.z) * T(0.25);
     }
     return r;
 }
 
 // Compute_du_j (src/include/inline.h:2810-2868)
-__device__ __forceinline__ Grad du_j(const Geo &g, const double *u, int i, int j, int k) {
+template <class T>
+__device__ __forceinline__ GradT<T> du_j(const Geo &g, const T *u, int i, int j, int k) {
     const bool iN = g.nonper(0), jN = g.nonper(1), kN = g.nonper(2);
-    Grad r;
+    GradT<T> r;
     const bool jb = (j == 0 || j == g.my - 2);
     if (i == g.mx - 2 && jb && (iN || jN)) {
-        V3 p0 = ld3(u, g.L(k, j + 1, i)), p1 = ld3(u, g.L(k, j, i)), m0 = ld3(u, g.L(k, j + 1, i - 1)), m1 = ld3(u, g.L(k, j, i - 1));
-        r.dudc = (p0.x + p1.x - m0.x - m1.x) * 0.5; r.dvdc = (p0.y + p1.y - m0.y - m1.y) * 0.5; r.dwdc = (p0.z + p1.z - m0.z - m1.z) * 0.5;
+        Vec3<T> p0 = ld3(u, g.L(k, j + 1, i)), p1 = ld3(u, g.L(k, j, i)), m0 = ld3(u, g.L(k, j + 1, i - 1)), m1 = ld3(u, g.L(k, j, i - 1));
+        r.dudc = (p0.x + p1.x - m0.x - m1.x) * T(0.5); r.dvdc = (p0.y + p1.y - m0.y - m1.y) * T(0.5); r.dwdc = (p0.z + p1.z - m0.z - m1.z) * T(0.5);
     } else if (i == 1 && jb && (iN || jN)) {
-        V3 p0 = ld3(u, g.L(k, j + 1, i + 1)), p1 = ld3(u, g.L(k, j, i + 1)), m0 = ld3(u, g.L(k, j + 1, i)), m1 = ld3(u, g.L(k, j, i));
-        r.dudc = (p0.x + p1.x - m0.x - m1.x) * 0.5; r.dvdc = (p0.y + p1.y - m0.y - m1.y) * 0.5; r.dwdc = (p0.z + p1.z - m0.z - m1.z) * 0.5;
+        Vec3<T> p0 = ld3(u, g.L(k, j + 1, i + 1)), p1 = ld3(u, g.L(k, j, i + 1)), m0 = ld3(u, g.L(k, j + 1, i)), m1 = ld3(u, g.L(k, j, i));
+        r.dudc = (p0.x + p1.x - m0.x - m1.x) * T(0.5); r.dvdc = (p0.y + p1.y - m0.y - m1.y) * T(0.5); r.dwdc = (p0.z + p1.z - m0.z - m1.z) * T(0.5);
     } else {
-        V3 p0 = ld3(u, g.L(k, j + 1, i + 1)), p1 = ld3(u, g.L(k, j, i + 1)), m0 = ld3(u, g.L(k, j + 1, i - 1)), m1 = ld3(u, g.L(k, j, i - 1));
-        r.dudc = (p0.x + p1.x - m0.x - m1.x) * 0.25; r.dvdc = (p0.y + p1.y - m0.y - m1.y) * 0.25; r.dwdc = (p0.z + p1.z - m0.z - m1.z) * 0.25;
+        Vec3<T> p0 = ld3(u, g.L(k, j + 1, i + 1)), p1 = ld3(u, g.L(k, j, i + 1)), m0 = ld3(u, g.L(k, j + 1, i - 1)), m1 = ld3(u, g.L(k, j, i - 1));
+        r.dudc = (p0.x + p1.x - m0.x - m1.x) * T(0.25); r.dvdc = (p0.y + p1.y - m0.y - m1.y) * T(0.25); r.dwdc = (p0.z + p1.z - m0.z - m1.z) * T(0.25);
     }
-    V3 a = ld3(u, g.L(k, j + 1, i)), b = ld3(u, g.L(k, j, i));
+    Vec3<T> a = ld3(u, g.L(k, j + 1, i)), b = ld3(u, g.L(k, j, i));
     r.dude = a.x - b.x; r.dvde = a.y - b.y; r.dwde = a.z - b.z;
     if (k == g.mz - 2 && jb && (kN || jN)) {
-        V3 p0 = ld3(u, g.L(k, j + 1, i)), p1 = ld3(u, g.L(k, j, i)), m0 = ld3(u, g.L(k - 1, j + 1, i)), m1 = ld3(u, g.L(k - 1, j, i));
-        r.dudz = (p0.x + p1.x - m0.x - m1.x) * 0.5; r.dvdz = (p0.y + p1.y - m0.y - m1.y) * 0.5; r.dwdz = (p0.z + p1.z - m0.z - m1.z) * 0.5;
+        Vec3<T> p0 = ld3(u, g.L(k, j + 1, i)), p1 = ld3(u, g.L(k, j, i)), m0 = ld3(u, g.L(k - 1, j + 1, i)), m1 = ld3(u, g.L(k - 1, j, i));
+        r.dudz = (p0.x + p1.x - m0.x - m1.x) * T(0.5); r.dvdz = (p0.y + p1.y - m0.y - m1.y) * T(0.5); r.dwdz = (p0.z + p1.z - m0.z - m1.z) * T(0.5);
     } else if (k == 1 && jb && (kN || jN)) {
-        V3 p0 = ld3(u, g.L(k + 1, j + 1, i)), p1 = ld3(u, g.L(k + 1, j, i)), m0 = ld3(u, g.L(k, j + 1, i)), m1 = ld3(u, g.L(k, j, i));
-        r.dudz = (p0.x + p1.x - m0.x - m1.x) * 0.5; r.dvdz = (p0.y + p1.y - m0.y - m1.y) * 0.5; r.dwdz = (p0.z + p1.z - m0.z - m1.z) * 0.5;
+        Vec3<T> p0 = ld3(u, g.L(k + 1, j + 1, i)), p1 = ld3(u, g.L(k + 1, j, i)), m0 = ld3(u, g.L(k, j + 1, i)), m1 = ld3(u, g.L(k, j, i));
+        r.dudz = (p0.x + p1.x - m0.x - m1.x) * T(0.5); r.dvdz = (p0.y + p1.y - m0.y - m1.y) * T(0.5); r.dwdz = (p0.z + p1.z - m0.z - m1.z) * T(0.5);
     } else {
-        V3 p0 = ld3(u, g.L(k + 1, j + 1, i)), p1 = ld3(u, g.L(k + 1, j, i)), m0 = ld3(u, g.L(k - 1, j + 1, i)), m1 = ld3(u, g.L(k - 1, j, i));
-        r.dudz = (p0.x + p1.x - m0.x - m1.x) * 0.25; r.dvdz = (p0.y + p1.y - m0.y - m1.y) * 0.25; r.dwdz = (p0.z + p1.z - m0.z - m1.z) * 0.25;
+        Vec3<T> p0 = ld3(u, g.L(k + 1, j + 1, i)), p1 = ld3(u, g.L(k + 1, j, i)), m0 = ld3(u, g.L(k - 1, j + 1, i)), m1 = ld3(u, g.L(k - 1, j, i));
+        r.dudz = (p0.x + p1.x - m0.x - m1.x) * T(0.25); r.dvdz = (p0.y + p1.y - m0.y - m1.y) * T(0.25); r.dwdz = (p0.z + p1.z - m0.z - m1.z) * T(0.25);
     }
     return r;
 }
 
 // Compute_du_k (src/include/inline.h:2872-2931)
-__device__ __forceinline__ Grad du_k(const Geo &g, const double *u, int i, int j, int k) {
+template <class T>
+__device__ __forceinline__ GradT<T> du_k(const Geo &g, const T *u, int i, int j, int k) {
     const bool iN = g.nonper(0), jN = g.nonper(1), kN = g.nonper(2);
-    Grad r;
+    GradT<T> r;
     const bool kb = (k == 0 || k == g.mz - 2);
     if (i == g.mx - 2 && kb && (iN || kN)) {
-        V3 p0 = ld3(u, g.L(k + 1, j, i)), p1 = ld3(u, g.L(k, j, i)), m0 = ld3(u, g.L(k + 1, j, i - 1)), m1 = ld3(u, g.L(k, j, i - 1));
-        r.dudc = (p0.x + p1.x - m0.x - m1.x) * 0.5; r.dvdc = (p0.y + p1.y - m0.y - m1.y) * 0.5; r.dwdc = (p0.z + p1.z - m0.z - m1.z) * 0.5;
+        Vec3<T> p0 = ld3(u, g.L(k + 1, j, i)), p1 = ld3(u, g.L(k, j, i)), m0 = ld3(u, g.L(k + 1, j, i - 1)), m1 = ld3(u, g.L(k, j, i - 1));
+        r.dudc = (p0.x + p1.x - m0.x - m1.x) * T(0.5); r.dvdc = (p0.y + p1.y - m0.y - m1.y) * T(0.5); r.dwdc = (p0.z + p1.z - m0.z - m1.z) * T(0.5);
     } else if (i == 1 && kb && (iN || kN)) {
-        V3 p0 = ld3(u, g.L(k + 1, j, i + 1)), p1 = ld3(u, g.L(k, j, i + 1)), m0 = ld3(u, g.L(k + 1, j, i)), m1 = ld3(u, g.L(k, j, i));
-        r.dudc = (p0.x + p1.x - m0.x - m1.x) * 0.5; r.dvdc = (p0.y + p1.y - m0.y - m1.y) * 0.5; r.dwdc = (p0.z + p1.z - m0.z - m1.z) * 0.5;
+        Vec3<T> p0 = ld3(u, g.L(k + 1, j, i + 1)), p1 = ld3(u, g.L(k, j, i + 1)), m0 = ld3(u, g.L(k + 1, j, i)), m1 = ld3(u, g.L(k, j, i));
+        r.dudc = (p0.x + p1.x - m0.x - m1.x) * T(0.5); r.dvdc = (p0.y + p1.y - m0.y - m1.y) * T(0.5); r.dwdc = (p0.z + p1.z - m0.z - m1.z) * T(0.5);
     } else {
-        V3 p0 = ld3(u, g.L(k + 1, j, i + 1)), p1 = ld3(u, g.L(k, j, i + 1)), m0 = ld3(u, g.L(k + 1, j, i - 1)), m1 = ld3(u, g.L(k, j, i - 1));
-        r.dudc = (p0.x + p1.x - m0.x - m1.x) * 0.25; r.dvdc = (p0.y + p1.y - m0.y - m1.y) * 0.25; r.dwdc = (p0.z + p1.z - m0.z - m1.z) * 0.25;
+        Vec3<T> p0 = ld3(u, g.L(k + 1, j, i + 1)), p1 = ld3(u, g.L(k, j, i + 1)), m0 = ld3(u, g.L(k + 1, j, i - 1)), m1 = ld3(u, g.L(k, j, i - 1));
+        r.dudc = (p0.x + p1.x - m0.x - m1.x) * T(0.25); r.dvdc = (p0.y + p1.y - m0.y - m1.y) * T(0.25); r.dwdc = (p0.z + p1.z - m0.z - m1.z) * T(0.25);
     }
     if (j == g.my - 2 && kb && (jN || kN)) {
-        V3 p0 = ld3(u, g.L(k + 1, j, i)), p1 = ld3(u, g.L(k, j, i)), m0 = ld3(u, g.L(k + 1, j - 1, i)), m1 = ld3(u, g.L(k, j - 1, i));
-        r.dude = (p0.x + p1.x - m0.x - m1.x) * 0.5; r.dvde = (p0.y + p1.y - m0.y - m1.y) * 0.5; r.dwde = (p0.z + p1.z - m0.z - m1.z) * 0.5;
+        Vec3<T> p0 = ld3(u, g.L(k + 1, j, i)), p1 = ld3(u, g.L(k, j, i)), m0 = ld3(u, g.L(k + 1, j - 1, i)), m1 = ld3(u, g.L(k, j - 1, i));
+        r.dude = (p0.x + p1.x - m0.x - m1.x) * T(0.5); r.dvde = (p0.y + p1.y - m0.y - m1.y) * T(0.5); r.dwde = (p0.z + p1.z - m0.z - m1.z) * T(0.5);
     } else if (j == 1 && kb && (jN || kN)) {
-        V3 p0 = ld3(u, g.L(k + 1, j + 1, i)), p1 = ld3(u, g.L(k, j + 1, i)), m0 = ld3(u, g.L(k + 1, j, i)), m1 = ld3(u, g.L(k, j, i));
-        r.dude = (p0.x + p1.x - m0.x - m1.x) * 0.5; r.dvde = (p0.y + p1.y - m0.y - m1.y) * 0.5; r.dwde = (p0.z + p1.z - m0.z - m1.z) * 0.5;
+        Vec3<T> p0 = ld3(u, g.L(k + 1, j + 1, i)), p1 = ld3(u, g.L(k, j + 1, i)), m0 = ld3(u, g.L(k + 1, j, i)), m1 = ld3(u, g.L(k, j, i));
+        r.dude = (p0.x + p1.x - m0.x - m1.x) * T(0.5); r.dvde = (p0.y + p1.y - m0.y - m1.y) * T(0.5); r.dwde = (p0.z + p1.z - m0.z - m1.z) * T(0.5);
     } else {
-        V3 p0 = ld3(u, g.L(k + 1, j + 1, i)), p1 = ld3(u, g.L(k, j + 1, i)), m0 = ld3(u, g.L(k + 1, j - 1, i)), m1 = ld3(u, g.L(k, j - 1, i));
-        r.dude = (p0.x + p1.x - m0.x - m1.x) * 0.25; r.dvde = (p0.y + p1.y - m0.y - m1.y) * 0.25; r.dwde = (p0.z + p1.z - m0.z - m1.z) * 0.25;
+        Vec3<T> p0 = ld3(u, g.L(k + 1, j + 1, i)), p1 = ld3(u, g.L(k, j + 1, i)), m0 = ld3(u, g.L(k + 1, j - 1, i)), m1 = ld3(u, g.L(k, j - 1, i));
+        r.dude = (p0.x + p1.x - m0.x - m1.x) * T(0.25); r.dvde = (p0.y + p1.y - m0.y - m1.y) * T(0.25); r.dwde = (p0.z + p1.z - m0.z - m1.z) * T(0.25);
     }
-    V3 a = ld3(u, g.L(k + 1, j, i)), b = ld3(u, g.L(k, j, i));
+    Vec3<T> a = ld3(u, g.L(k + 1, j, i)), b = ld3(u, g.L(k, j, i));
     r.dudz = a.x - b.x; r.dvdz = a.y - b.y; r.dwdz = a.z - b.z;
     return r;
 }
@@ -129,30 +143,33 @@ __device__ __forceinline__ Grad du_k(const Geo &g, const double *u, int i, int j
 // Cell-centred velocity derivatives (Compute_du_center, src/include/inline.h:3344-3418), from the
 // cell's value c and its six neighbours: central differences, one-sided at the first and last
 // interior cell of a non-periodic direction.
-__device__ __forceinline__ Grad du_center_v(const Geo &g, int i, int j, int k, V3 c, V3 im, V3 ip, V3 jm, V3 jp,
-                                            V3 km, V3 kp) {
-    Grad r;
+template <class T>
+__device__ __forceinline__ GradT<T> du_center_v(const Geo &g, int i, int j, int k, Vec3<T> c, Vec3<T> im, Vec3<T> ip,
+                                                Vec3<T> jm, Vec3<T> jp, Vec3<T> km, Vec3<T> kp) {
+    GradT<T> r;
     if (g.nonper(0) && i == g.mx - 2) { r.dudc = c.x - im.x; r.dvdc = c.y - im.y; r.dwdc = c.z - im.z; }
     else if (g.nonper(0) && i == 1) { r.dudc = ip.x - c.x; r.dvdc = ip.y - c.y; r.dwdc = ip.z - c.z; }
-    else { r.dudc = (ip.x - im.x) * 0.5; r.dvdc = (ip.y - im.y) * 0.5; r.dwdc = (ip.z - im.z) * 0.5; }
+    else { r.dudc = (ip.x - im.x) * T(0.5); r.dvdc = (ip.y - im.y) * T(0.5); r.dwdc = (ip.z - im.z) * T(0.5); }
     if (g.nonper(1) && j == g.my - 2) { r.dude = c.x - jm.x; r.dvde = c.y - jm.y; r.dwde = c.z - jm.z; }
     else if (g.nonper(1) && j == 1) { r.dude = jp.x - c.x; r.dvde = jp.y - c.y; r.dwde = jp.z - c.z; }
-    else { r.dude = (jp.x - jm.x) * 0.5; r.dvde = (jp.y - jm.y) * 0.5; r.dwde = (jp.z - jm.z) * 0.5; }
+    else { r.dude = (jp.x - jm.x) * T(0.5); r.dvde = (jp.y - jm.y) * T(0.5); r.dwde = (jp.z - jm.z) * T(0.5); }
     if (g.nonper(2) && k == g.mz - 2) { r.dudz = c.x - km.x; r.dvdz = c.y - km.y; r.dwdz = c.z - km.z; }
     else if (g.nonper(2) && k == 1) { r.dudz = kp.x - c.x; r.dvdz = kp.y - c.y; r.dwdz = kp.z - c.z; }
-    else { r.dudz = (kp.x - km.x) * 0.5; r.dvdz = (kp.y - km.y) * 0.5; r.dwdz = (kp.z - km.z) * 0.5; }
+    else { r.dudz = (kp.x - km.x) * T(0.5); r.dvdz = (kp.y - km.y) * T(0.5); r.dwdz = (kp.z - km.z) * T(0.5); }
     return r;
 }
 
-__device__ __forceinline__ Grad du_center(const Geo &g, const double *u, int i, int j, int k) {
+template <class T>
+__device__ __forceinline__ GradT<T> du_center(const Geo &g, const T *u, int i, int j, int k) {
     return du_center_v(g, i, j, k, ld3(u, g.L(k, j, i)), ld3(u, g.L(k, j, i - 1)), ld3(u, g.L(k, j, i + 1)),
                        ld3(u, g.L(k, j - 1, i)), ld3(u, g.L(k, j + 1, i)), ld3(u, g.L(k - 1, j, i)),
                        ld3(u, g.L(k + 1, j, i)));
 }
 
 // Compute_du_dxyz (src/include/inline.h:3635-3663)
-__device__ __forceinline__ Cart du_dxyz(V3 c, V3 e, V3 z, double ajc, const Grad &g) {
-    Cart r;
+template <class T>
+__device__ __forceinline__ CartT<T> du_dxyz(Vec3<T> c, Vec3<T> e, Vec3<T> z, T ajc, const GradT<T> &g) {
+    CartT<T> r;
     r.du_dx = (g.dudc * c.x + g.dude * e.x + g.dudz * z.x) * ajc;
     r.du_dy = (g.dudc * c.y + g.dude * e.y + g.dudz * z.y) * ajc;
     r.du_dz = (g.dudc * c.z + g.dude * e.z + g.dudz * z.z) * ajc;
@@ -166,17 +183,20 @@ __device__ __forceinline__ Cart du_dxyz(V3 c, V3 e, V3 z, double ajc, const Grad
 }
 
 // |S| = sqrt(2 S_ij S_ij) (src/les.c:342-354)
-__device__ __forceinline__ double strain_mag(const Cart &d) {
-    double Sxx = 0.5 * (d.du_dx + d.du_dx), Sxy = 0.5 * (d.du_dy + d.dv_dx), Sxz = 0.5 * (d.du_dz + d.dw_dx);
-    double Syy = 0.5 * (d.dv_dy + d.dv_dy), Syz = 0.5 * (d.dv_dz + d.dw_dy), Szz = 0.5 * (d.dw_dz + d.dw_dz);
-    return sqrt(2.0 * (Sxx * Sxx + Sxy * Sxy + Sxz * Sxz + Sxy * Sxy + Syy * Syy + Syz * Syz +
-                       Sxz * Sxz + Syz * Syz + Szz * Szz));
+template <class T>
+__device__ __forceinline__ T strain_mag(const CartT<T> &d) {
+    const T h = T(0.5);
+    T Sxx = h * (d.du_dx + d.du_dx), Sxy = h * (d.du_dy + d.dv_dx), Sxz = h * (d.du_dz + d.dw_dx);
+    T Syy = h * (d.dv_dy + d.dv_dy), Syz = h * (d.dv_dz + d.dw_dy), Szz = h * (d.dw_dz + d.dw_dz);
+    return sqrt(T(2.0) * (Sxx * Sxx + Sxy * Sxy + Sxz * Sxz + Sxy * Sxy + Syy * Syy + Syz * Syz +
+                          Sxz * Sxz + Syz * Syz + Szz * Szz));
 }
 
 // The viscous flux of one face (ueqn_solvers.c:196-215 + 386-392 and their j/k twins): dir is the
 // face direction (0 = i, 1 = j, 2 = k); csi/eta/zet are the face's contravariant basis.
-__device__ __forceinline__ V3 visc_flux(const Grad &g, V3 c, V3 e, V3 z, int dir, double ajc, double nuEff) {
-    double g11, g21, g31, n0, n1, n2;
+template <class T>
+__device__ __forceinline__ Vec3<T> visc_flux(const GradT<T> &g, Vec3<T> c, Vec3<T> e, Vec3<T> z, int dir, T ajc, T nuEff) {
+    T g11, g21, g31, n0, n1, n2;
     if (dir == 0) {
         g11 = c.x * c.x + c.y * c.y + c.z * c.z; g21 = e.x * c.x + e.y * c.y + e.z * c.z; g31 = z.x * c.x + z.y * c.y + z.z * c.z;
         n0 = c.x; n1 = c.y; n2 = c.z;
@@ -187,16 +207,16 @@ __device__ __forceinline__ V3 visc_flux(const Grad &g, V3 c, V3 e, V3 z, int dir
         g11 = c.x * z.x + c.y * z.y + c.z * z.z; g21 = e.x * z.x + e.y * z.y + e.z * z.z; g31 = z.x * z.x + z.y * z.y + z.z * z.z;
         n0 = z.x; n1 = z.y; n2 = z.z;
     }
-    double r11 = g.dudc * c.x + g.dude * e.x + g.dudz * z.x;
-    double r21 = g.dvdc * c.x + g.dvde * e.x + g.dvdz * z.x;
-    double r31 = g.dwdc * c.x + g.dwde * e.x + g.dwdz * z.x;
-    double r12 = g.dudc * c.y + g.dude * e.y + g.dudz * z.y;
-    double r22 = g.dvdc * c.y + g.dvde * e.y + g.dvdz * z.y;
-    double r32 = g.dwdc * c.y + g.dwde * e.y + g.dwdz * z.y;
-    double r13 = g.dudc * c.z + g.dude * e.z + g.dudz * z.z;
-    double r23 = g.dvdc * c.z + g.dvde * e.z + g.dvdz * z.z;
-    double r33 = g.dwdc * c.z + g.dwde * e.z + g.dwdz * z.z;
-    V3 o;
+    T r11 = g.dudc * c.x + g.dude * e.x + g.dudz * z.x;
+    T r21 = g.dvdc * c.x + g.dvde * e.x + g.dvdz * z.x;
+    T r31 = g.dwdc * c.x + g.dwde * e.x + g.dwdz * z.x;
+    T r12 = g.dudc * c.y + g.dude * e.y + g.dudz * z.y;
+    T r22 = g.dvdc * c.y + g.dvde * e.y + g.dvdz * z.y;
+    T r32 = g.dwdc * c.y + g.dwde * e.y + g.dwdz * z.y;
+    T r13 = g.dudc * c.z + g.dude * e.z + g.dudz * z.z;
+    T r23 = g.dvdc * c.z + g.dvde * e.z + g.dvdz * z.z;
+    T r33 = g.dwdc * c.z + g.dwde * e.z + g.dwdz * z.z;
+    Vec3<T> o;
     o.x = (g11 * g.dudc + g21 * g.dude + g31 * g.dudz + r11 * n0 + r21 * n1 + r31 * n2) * ajc * (nuEff);
     o.y = (g11 * g.dvdc + g21 * g.dvde + g31 * g.dvdz + r12 * n0 + r22 * n1 + r32 * n2) * ajc * (nuEff);
     o.z = (g11 * g.dwdc + g21 * g.dwde + g31 * g.dwdz + r13 * n0 + r23 * n1 + r33 * n2) * ajc * (nuEff);
@@ -204,13 +224,15 @@ __device__ __forceinline__ V3 visc_flux(const Grad &g, V3 c, V3 e, V3 z, int dir
 }
 
 // QUICK (quadraticUpwindVec, src/include/inline.h:5089-5109): upwind on the sign of the flux.
-__device__ __forceinline__ V3 quick(V3 f0, V3 f1, V3 f2, V3 f3, double ws) {
-    V3 UU, U, D;
+template <class T>
+__device__ __forceinline__ Vec3<T> quick(Vec3<T> f0, Vec3<T> f1, Vec3<T> f2, Vec3<T> f3, T ws) {
+    Vec3<T> UU, U, D;
     if (ws > 0) { UU = f0; U = f1; D = f2; } else { UU = f3; U = f2; D = f1; }
-    V3 r;
-    r.x = 0.5 * (U.x + D.x) + 0.125 * (2.0 * U.x - (D.x + UU.x));
-    r.y = 0.5 * (U.y + D.y) + 0.125 * (2.0 * U.y - (D.y + UU.y));
-    r.z = 0.5 * (U.z + D.z) + 0.125 * (2.0 * U.z - (D.z + UU.z));
+    const T h = T(0.5), e = T(0.125), two = T(2.0);
+    Vec3<T> r;
+    r.x = h * (U.x + D.x) + e * (two * U.x - (D.x + UU.x));
+    r.y = h * (U.y + D.y) + e * (two * U.y - (D.y + UU.y));
+    r.z = h * (U.z + D.z) + e * (two * U.z - (D.z + UU.z));
     return r;
 }
 

@@ -8,8 +8,11 @@
 
 namespace tosca {
 
-struct FormUArgs {
-    const double *ucont, *ucat, *nut;
+// T is the type of the flow fields (ucont, ucat, nut, theta): double on the FP64 path, float on
+// the mixed path.  Metrics, Jacobians, the wall stresses and the outputs are always double.
+template <class T>
+struct FormUArgsT {
+    const T *ucont, *ucat, *nut;
     const double *csi, *eta, *zet;
     const double *icsi, *ieta, *izet, *jcsi, *jeta, *jzet, *kcsi, *keta, *kzet;
     const double *iaj, *jaj, *kaj;
@@ -20,34 +23,40 @@ struct FormUArgs {
     double nu;
     int bc_u[6];
     // fused Buoyancy (btheta == nullptr: not evaluated)
-    const double *theta;
+    const T *theta;
     double *btheta;
     double bscale, tRef;
 };
+using FormUArgs = FormUArgsT<double>;
 
 // FormU (+ Buoyancy when A.btheta != nullptr); ws holds formu_ws_doubles(g) doubles for the
-// boundary-face fluxes and periodic-seam fp
-void fused_formu(const Geo &g, const FormUArgs &A, double *ws, hipStream_t s);
+// boundary-face fluxes and periodic-seam fp.  Instantiated for double and float flow fields.
+template <class T>
+void fused_formu(const Geo &g, const FormUArgsT<T> &A, double *ws, hipStream_t s);
 size_t formu_ws_doubles(const Geo &g);
 
 // Dynamic Smagorinsky: LM, MM and |S| of every interior cell into ws (3 local scalar fields),
 // then the box average + clip into cs (interior) and, when nut != nullptr, nut = Cs Delta^2 |S|
-// (interior).  Ghost handling is left to the caller.
-void fused_dsm(const Geo &g, const tosca_mesh *m, const double *ucat, double maxCs, double *cs, double *nut,
-               double *ws, hipStream_t s);
-
-// nut = Cs Delta^2 |S| at interior cells from a given Cs (Smagorinsky, or any stored Cs)
-void fused_nut(const Geo &g, const tosca_mesh *m, const double *ucat, const double *cs, double *nut, double *ksgs,
+// (interior).  Ghost handling is left to the caller.  T: type of ucat and nut.
+template <class T>
+void fused_dsm(const Geo &g, const tosca_mesh *m, const T *ucat, double maxCs, double *cs, T *nut, double *ws,
                hipStream_t s);
 
-// zero the shell of an nx*ny*nz array (ncomp doubles per point) outside its interior cells,
+// nut = Cs Delta^2 |S| at interior cells from a given Cs (Smagorinsky, or any stored Cs)
+template <class T>
+void fused_nut(const Geo &g, const tosca_mesh *m, const T *ucat, const double *cs, T *nut, T *ksgs, hipStream_t s);
+
+// zero the shell of an nx*ny*nz array (ncomp values per point) outside its interior cells,
 // the interior being [1+lo, n-2-hi] on each axis in array coordinates
-void zero_shell(double *a, int ncomp, int nx, int ny, int nz, int lo, int hi, hipStream_t s);
+template <class T>
+void zero_shell(T *a, int ncomp, int nx, int ny, int nz, int lo, int hi, hipStream_t s);
 
 // zero every point of a local scalar array outside the interior cells
-void zero_outside_interior(double *a, const Geo &g, hipStream_t s);
+template <class T>
+void zero_outside_interior(T *a, const Geo &g, hipStream_t s);
 
 // nut ghost cells: UpdateNutBCs subset + ghost wrap
-void nut_bcs(double *nut, const tosca_params *p, const Geo &g, hipStream_t s);
+template <class T>
+void nut_bcs(T *nut, const tosca_params *p, const Geo &g, hipStream_t s);
 
 }  // namespace tosca

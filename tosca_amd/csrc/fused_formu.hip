@@ -21,58 +21,60 @@ namespace tosca {
 
 namespace {
 
-struct Flux { V3 d, v; };
+template <class T> struct FluxT { Vec3<T> d, v; };
+using Flux = FluxT<double>;
 
-__device__ __forceinline__ V3 add3(V3 a, V3 b) { return V3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <class T> __device__ __forceinline__ Vec3<T> add3(Vec3<T> a, Vec3<T> b) { return Vec3<T>{a.x + b.x, a.y + b.y, a.z + b.z}; }
 
 // convective (div) and viscous (visc) flux of one face.  dir: 0 = i face (k,j,i)|(k,j,i+1),
 // 1 = j face, 2 = k face.  (i,j,k) is the face index after the periodic remap.
-template <int DIR>
-__device__ __forceinline__ Flux face_flux(const FormUArgs &A, const Geo &g, int i, int j, int k) {
+template <int DIR, class T>
+__device__ __forceinline__ FluxT<T> face_flux(const FormUArgsT<T> &A, const Geo &g, int i, int j, int k) {
     const int c = g.L(k, j, i);
-    const double *u = A.ucat;
-    Flux r;
-    r.d = V3{0, 0, 0};
-    r.v = V3{0, 0, 0};
+    const T *u = A.ucat;
+    FluxT<T> r;
+    r.d = Vec3<T>{0, 0, 0};
+    r.v = Vec3<T>{0, 0, 0};
     const int up = (DIR == 0) ? g.L(k, j, i + 1) : (DIR == 1) ? g.L(k, j + 1, i) : g.L(k + 1, j, i);
     if (A.mode != 2) {
-        const double V = A.ucont[3 * c + DIR];
-        V3 f;
+        const T V = A.ucont[3 * c + DIR];
+        Vec3<T> f;
         if (A.quick) {
             int L, R;
             if (DIR == 0) { face2cell4(g.per[0], i, g.mx, L, R); f = quick(ld3(u, g.L(k, j, L)), ld3(u, c), ld3(u, up), ld3(u, g.L(k, j, R)), V); }
             else if (DIR == 1) { face2cell4(g.per[1], j, g.my, L, R); f = quick(ld3(u, g.L(k, L, i)), ld3(u, c), ld3(u, up), ld3(u, g.L(k, R, i)), V); }
             else { face2cell4(g.per[2], k, g.mz, L, R); f = quick(ld3(u, g.L(L, j, i)), ld3(u, c), ld3(u, up), ld3(u, g.L(R, j, i)), V); }
         } else {
-            V3 a = ld3(u, c), b = ld3(u, up);
-            f = V3{0.5 * (a.x + b.x), 0.5 * (a.y + b.y), 0.5 * (a.z + b.z)};
+            Vec3<T> a = ld3(u, c), b = ld3(u, up);
+            const T h = T(0.5);
+            f = Vec3<T>{h * (a.x + b.x), h * (a.y + b.y), h * (a.z + b.z)};
         }
-        r.d = V3{-V * f.x, -V * f.y, -V * f.z};
+        r.d = Vec3<T>{-V * f.x, -V * f.y, -V * f.z};
     }
     if (A.mode != 1) {
-        Grad gr = (DIR == 0) ? du_i(g, u, i, j, k) : (DIR == 1) ? du_j(g, u, i, j, k) : du_k(g, u, i, j, k);
-        double nuEff = A.nu;
-        V3 base{0, 0, 0};
+        GradT<T> gr = (DIR == 0) ? du_i(g, u, i, j, k) : (DIR == 1) ? du_j(g, u, i, j, k) : du_k(g, u, i, j, k);
+        T nuEff = T(A.nu);
+        Vec3<T> base{0, 0, 0};
         if (A.les) {
-            const double nt = 0.5 * (A.nut[c] + A.nut[up]);
+            const T nt = T(0.5) * (A.nut[c] + A.nut[up]);
             const int f = (DIR == 0) ? i : (DIR == 1) ? j : k;
             const int m = (DIR == 0) ? g.mx : (DIR == 1) ? g.my : g.mz;
             const int bl = A.bc_u[2 * DIR], br = A.bc_u[2 * DIR + 1];
             if (DIR < 2 && ((bl == TOSCA_BCU_WALLFUNCTION && f == 0) || (br == TOSCA_BCU_WALLFUNCTION && f == m - 2))) {
-                if (DIR == 0) { int t = k * g.my + j, P = g.mz * g.my; base = V3{-A.tauI[t], -A.tauI[P + t], -A.tauI[2 * P + t]}; }
-                else { int t = k * g.mx + i, P = g.mz * g.mx; base = V3{-A.tauJ[t], -A.tauJ[P + t], -A.tauJ[2 * P + t]}; }
-                nuEff = A.nu;
+                if (DIR == 0) { int t = k * g.my + j, P = g.mz * g.my; base = Vec3<T>{T(-A.tauI[t]), T(-A.tauI[P + t]), T(-A.tauI[2 * P + t])}; }
+                else { int t = k * g.mx + i, P = g.mz * g.mx; base = Vec3<T>{T(-A.tauJ[t]), T(-A.tauJ[P + t]), T(-A.tauJ[2 * P + t])}; }
+                nuEff = T(A.nu);
             } else if ((bl == TOSCA_BCU_SLIP && f == 0) || (br == TOSCA_BCU_SLIP && f == m - 2)) {
-                nuEff = 0.0;
+                nuEff = T(0.0);
             } else {
-                nuEff = A.nu + nt;
+                nuEff = T(A.nu) + nt;
             }
         }
         const double *fc = (DIR == 0) ? A.icsi : (DIR == 1) ? A.jcsi : A.kcsi;
         const double *fe = (DIR == 0) ? A.ieta : (DIR == 1) ? A.jeta : A.keta;
         const double *fz = (DIR == 0) ? A.izet : (DIR == 1) ? A.jzet : A.kzet;
         const double *fa = (DIR == 0) ? A.iaj : (DIR == 1) ? A.jaj : A.kaj;
-        r.v = add3(base, visc_flux(gr, ld3(fc, c), ld3(fe, c), ld3(fz, c), DIR, fa[c], nuEff));
+        r.v = add3(base, visc_flux<T>(gr, cvt3<T>(ld3(fc, c)), cvt3<T>(ld3(fe, c)), cvt3<T>(ld3(fz, c)), DIR, T(fa[c]), nuEff));
     }
     return r;
 }
@@ -82,117 +84,20 @@ __device__ __forceinline__ V3 ldnt3(const double *p, int c) {
               __builtin_nontemporal_load(p + 3 * c + 2)};
 }
 
-// Flux of a face away from the domain boundary (face index 1..m-3 along its normal): every
-// boundary rule of the general evaluation is inactive there, so the gradients are the central
-// four-point averages of Compute_du_i/j/k, the QUICK stencil is (f-1, f, f+1, f+2) and nuEff is
-// nu + <nu_t>.  c is the local index of the face's left cell, sj/sk the j and k strides.
-template <int DIR>
-__device__ __forceinline__ Flux face_fast(const FormUArgs &A, int c, int sj, int sk) {
-    const double *u = A.ucat;
-    const int s = (DIR == 0) ? 1 : (DIR == 1) ? sj : sk;
-    const V3 u0 = ld3(u, c), u1 = ld3(u, c + s);
-    Flux r;
-    r.d = V3{0, 0, 0};
-    r.v = V3{0, 0, 0};
-    if (A.mode != 2) {
-        const double V = __builtin_nontemporal_load(A.ucont + 3 * c + DIR);
-        V3 f;
-        if (A.quick) f = quick(ld3(u, c - s), u0, u1, ld3(u, c + 2 * s), V);
-        else f = V3{0.5 * (u0.x + u1.x), 0.5 * (u0.y + u1.y), 0.5 * (u0.z + u1.z)};
-        r.d = V3{-V * f.x, -V * f.y, -V * f.z};
-    }
-    if (A.mode != 1) {
-        Grad g;
-        // the two transverse strides of this face
-        const int a = (DIR == 0) ? sj : 1, b = (DIR == 2) ? sj : sk;
-        const V3 ap = ld3(u, c + a + s), a0 = ld3(u, c + a), am = ld3(u, c - a + s), an = ld3(u, c - a);
-        const V3 bp = ld3(u, c + b + s), b0 = ld3(u, c + b), bm = ld3(u, c - b + s), bn = ld3(u, c - b);
-        const double nx = u1.x - u0.x, ny = u1.y - u0.y, nz = u1.z - u0.z;
-        const double ax = (ap.x + a0.x - am.x - an.x) * 0.25, ay = (ap.y + a0.y - am.y - an.y) * 0.25,
-                     az = (ap.z + a0.z - am.z - an.z) * 0.25;
-        const double bx = (bp.x + b0.x - bm.x - bn.x) * 0.25, by = (bp.y + b0.y - bm.y - bn.y) * 0.25,
-                     bz = (bp.z + b0.z - bm.z - bn.z) * 0.25;
-        if (DIR == 0) {        // normal = csi; transverse a = eta (j), b = zet (k)
-            g.dudc = nx; g.dvdc = ny; g.dwdc = nz; g.dude = ax; g.dvde = ay; g.dwde = az; g.dudz = bx; g.dvdz = by; g.dwdz = bz;
-        } else if (DIR == 1) { // normal = eta; transverse a = csi (i), b = zet (k)
-            g.dudc = ax; g.dvdc = ay; g.dwdc = az; g.dude = nx; g.dvde = ny; g.dwde = nz; g.dudz = bx; g.dvdz = by; g.dwdz = bz;
-        } else {               // normal = zet; transverse a = csi (i), b = eta (j)
-            g.dudc = ax; g.dvdc = ay; g.dwdc = az; g.dude = bx; g.dvde = by; g.dwde = bz; g.dudz = nx; g.dvdz = ny; g.dwdz = nz;
-        }
-        double nuEff = A.nu;
-        if (A.les) nuEff = A.nu + 0.5 * (A.nut[c] + A.nut[c + s]);
-        const double *fc = (DIR == 0) ? A.icsi : (DIR == 1) ? A.jcsi : A.kcsi;
-        const double *fe = (DIR == 0) ? A.ieta : (DIR == 1) ? A.jeta : A.keta;
-        const double *fz = (DIR == 0) ? A.izet : (DIR == 1) ? A.jzet : A.kzet;
-        const double *fa = (DIR == 0) ? A.iaj : (DIR == 1) ? A.jaj : A.kaj;
-        r.v = visc_flux(g, ldnt3(fc, c), ldnt3(fe, c), ldnt3(fz, c), DIR, fa[c], nuEff);
-    }
-    return r;
-}
-
 // LDS ring of ucat planes: SoA [slot][component][row][col], rows j0-2 .. j0+NR, cols i0-2 .. i0+65
-template <int NR>
+template <int NR, class T>
 struct Ring {
     static constexpr int RI = 68, RJ = NR + 3, NS = 4;
-    double v[NS][3][RJ][RI];
-    __device__ __forceinline__ V3 at(int slot, int row, int col) const {
-        return V3{v[slot][0][row][col], v[slot][1][row][col], v[slot][2][row][col]};
+    T v[NS][3][RJ][RI];
+    __device__ __forceinline__ Vec3<T> at(int slot, int row, int col) const {
+        return Vec3<T>{v[slot][0][row][col], v[slot][1][row][col], v[slot][2][row][col]};
     }
 };
 
-// face_fast with ucat taken from the LDS ring.  (row, col) is the face's left cell in ring
-// coordinates and slot(t) the ring slot of plane t.
-template <int DIR, int NR>
-__device__ __forceinline__ Flux face_lds(const FormUArgs &A, const Ring<NR> &R, int c, int row, int col, int t,
-                                         int sj, int sk) {
-    auto U = [&](int dk, int dj, int di) { return R.at((t + dk) & 3, row + dj, col + di); };
-    const int s = (DIR == 0) ? 1 : (DIR == 1) ? sj : sk;
-    const int ok = (DIR == 2), oj = (DIR == 1), oi = (DIR == 0);
-    const V3 u0 = U(0, 0, 0), u1 = U(ok, oj, oi);
-    Flux r;
-    r.d = V3{0, 0, 0};
-    r.v = V3{0, 0, 0};
-    if (A.mode != 2) {
-        const double V = __builtin_nontemporal_load(A.ucont + 3 * c + DIR);
-        V3 f;
-        if (A.quick) f = quick(U(-ok, -oj, -oi), u0, u1, U(2 * ok, 2 * oj, 2 * oi), V);
-        else f = V3{0.5 * (u0.x + u1.x), 0.5 * (u0.y + u1.y), 0.5 * (u0.z + u1.z)};
-        r.d = V3{-V * f.x, -V * f.y, -V * f.z};
-    }
-    if (A.mode != 1) {
-        // transverse directions a and b of the face (see face_fast)
-        const int ak = 0, aj = (DIR == 0), ai = (DIR != 0);
-        const int bk = (DIR != 2), bj = (DIR == 2), bi = 0;
-        const V3 ap = U(ak + ok, aj + oj, ai + oi), a0 = U(ak, aj, ai), am = U(ok - ak, oj - aj, oi - ai), an = U(-ak, -aj, -ai);
-        const V3 bp = U(bk + ok, bj + oj, bi + oi), b0 = U(bk, bj, bi), bm = U(ok - bk, oj - bj, oi - bi), bn = U(-bk, -bj, -bi);
-        const double nx = u1.x - u0.x, ny = u1.y - u0.y, nz = u1.z - u0.z;
-        const double ax = (ap.x + a0.x - am.x - an.x) * 0.25, ay = (ap.y + a0.y - am.y - an.y) * 0.25,
-                     az = (ap.z + a0.z - am.z - an.z) * 0.25;
-        const double bx = (bp.x + b0.x - bm.x - bn.x) * 0.25, by = (bp.y + b0.y - bm.y - bn.y) * 0.25,
-                     bz = (bp.z + b0.z - bm.z - bn.z) * 0.25;
-        Grad g;
-        if (DIR == 0) {
-            g.dudc = nx; g.dvdc = ny; g.dwdc = nz; g.dude = ax; g.dvde = ay; g.dwde = az; g.dudz = bx; g.dvdz = by; g.dwdz = bz;
-        } else if (DIR == 1) {
-            g.dudc = ax; g.dvdc = ay; g.dwdc = az; g.dude = nx; g.dvde = ny; g.dwde = nz; g.dudz = bx; g.dvdz = by; g.dwdz = bz;
-        } else {
-            g.dudc = ax; g.dvdc = ay; g.dwdc = az; g.dude = bx; g.dvde = by; g.dwde = bz; g.dudz = nx; g.dvdz = ny; g.dwdz = nz;
-        }
-        double nuEff = A.nu;
-        if (A.les) nuEff = A.nu + 0.5 * (A.nut[c] + A.nut[c + s]);
-        const double *fc = (DIR == 0) ? A.icsi : (DIR == 1) ? A.jcsi : A.kcsi;
-        const double *fe = (DIR == 0) ? A.ieta : (DIR == 1) ? A.jeta : A.keta;
-        const double *fz = (DIR == 0) ? A.izet : (DIR == 1) ? A.jzet : A.kzet;
-        const double *fa = (DIR == 0) ? A.iaj : (DIR == 1) ? A.jaj : A.kaj;
-        r.v = visc_flux(g, ldnt3(fc, c), ldnt3(fe, c), ldnt3(fz, c), DIR, fa[c], nuEff);
-    }
-    return r;
-}
-
-__device__ __forceinline__ double shfl_up1(double x) { return __shfl_up(x, 1, 64); }
-__device__ __forceinline__ double shfl_dn1(double x) { return __shfl_down(x, 1, 64); }
-__device__ __forceinline__ V3 shfl_up3(V3 a) { return V3{shfl_up1(a.x), shfl_up1(a.y), shfl_up1(a.z)}; }
-__device__ __forceinline__ V3 shfl_dn3(V3 a) { return V3{shfl_dn1(a.x), shfl_dn1(a.y), shfl_dn1(a.z)}; }
+template <class T> __device__ __forceinline__ T shfl_up1(T x) { return __shfl_up(x, 1, 64); }
+template <class T> __device__ __forceinline__ T shfl_dn1(T x) { return __shfl_down(x, 1, 64); }
+template <class T> __device__ __forceinline__ Vec3<T> shfl_up3(Vec3<T> a) { return Vec3<T>{shfl_up1(a.x), shfl_up1(a.y), shfl_up1(a.z)}; }
+template <class T> __device__ __forceinline__ Vec3<T> shfl_dn3(Vec3<T> a) { return Vec3<T>{shfl_dn1(a.x), shfl_dn1(a.y), shfl_dn1(a.z)}; }
 
 // fp differences in the reference's order: ((((a - b) + c) - d) + e) - f, div and visc apart
 __device__ __forceinline__ double fpsum(double d1, double d1m, double d2, double d2m, double d3, double d3m,
@@ -226,8 +131,8 @@ __host__ __device__ inline int tr_dims(const Geo &g, int d, int &n1, int &n2) {
     return n1 * n2;
 }
 
-template <int DIR>
-__device__ __forceinline__ Flux face_general(const FormUArgs &A, const Geo &g, int f, int a, int b) {
+template <int DIR, class T>
+__device__ __forceinline__ FluxT<T> face_general(const FormUArgsT<T> &A, const Geo &g, int f, int a, int b) {
     // face f along DIR at transverse (a, b) = (j, k) | (i, k) | (i, j); periodic face 0 is face m-2
     const int m = (DIR == 0) ? g.mx : (DIR == 1) ? g.my : g.mz;
     if (f == 0 && g.per[DIR] != TOSCA_PERIODIC_NONE) f = m - 2;
@@ -236,7 +141,8 @@ __device__ __forceinline__ Flux face_general(const FormUArgs &A, const Geo &g, i
     return face_flux<2>(A, g, a, b, f);
 }
 
-__global__ void __launch_bounds__(128) k_formu_boundary(FormUArgs A, Geo g, BoundaryBufs B) {
+template <class T>
+__global__ void __launch_bounds__(128) k_formu_boundary(FormUArgsT<T> A, Geo g, BoundaryBufs B) {
     const int d = blockIdx.z / 3, what = blockIdx.z % 3;      // what: 0 face 0, 1 face m-2, 2 seam fp
     int n1, n2;
     const int n = tr_dims(g, d, n1, n2);
@@ -248,8 +154,8 @@ __global__ void __launch_bounds__(128) k_formu_boundary(FormUArgs A, Geo g, Boun
     const int m = (d == 0) ? g.mx : (d == 1) ? g.my : g.mz;
     if (what < 2) {
         const int f = what == 0 ? 0 : m - 2;
-        Flux F = (d == 0) ? face_general<0>(A, g, f, a, b) : (d == 1) ? face_general<1>(A, g, f, a, b)
-                                                                    : face_general<2>(A, g, f, a, b);
+        FluxT<T> F = (d == 0) ? face_general<0>(A, g, f, a, b) : (d == 1) ? face_general<1>(A, g, f, a, b)
+                                                                        : face_general<2>(A, g, f, a, b);
         double *o = B.bF[d] + ((long)what * n + t) * 6;
         o[0] = F.d.x; o[1] = F.d.y; o[2] = F.d.z; o[3] = F.v.x; o[4] = F.v.y; o[5] = F.v.z;
         return;
@@ -258,9 +164,9 @@ __global__ void __launch_bounds__(128) k_formu_boundary(FormUArgs A, Geo g, Boun
     // fp at cell 1 along d (transverse interior), from its six faces
     int i, j, k;
     if (d == 0) { i = 1; j = a; k = b; } else if (d == 1) { i = a; j = 1; k = b; } else { i = a; j = b; k = 1; }
-    const Flux f1 = face_general<0>(A, g, i, j, k), f1m = face_general<0>(A, g, i - 1, j, k);
-    const Flux f2 = face_general<1>(A, g, j, i, k), f2m = face_general<1>(A, g, j - 1, i, k);
-    const Flux f3 = face_general<2>(A, g, k, i, j), f3m = face_general<2>(A, g, k - 1, i, j);
+    const FluxT<T> f1 = face_general<0>(A, g, i, j, k), f1m = face_general<0>(A, g, i - 1, j, k);
+    const FluxT<T> f2 = face_general<1>(A, g, j, i, k), f2m = face_general<1>(A, g, j - 1, i, k);
+    const FluxT<T> f3 = face_general<2>(A, g, k, i, j), f3m = face_general<2>(A, g, k - 1, i, j);
     double *o = B.seam[d] + (long)t * 3;
     o[0] = fpsum(f1.d.x, f1m.d.x, f2.d.x, f2m.d.x, f3.d.x, f3m.d.x, f1.v.x, f1m.v.x, f2.v.x, f2m.v.x, f3.v.x, f3m.v.x, A.mode);
     o[1] = fpsum(f1.d.y, f1m.d.y, f2.d.y, f2m.d.y, f3.d.y, f3m.d.y, f1.v.y, f1m.v.y, f2.v.y, f2m.v.y, f3.v.y, f3m.v.y, A.mode);
@@ -270,34 +176,40 @@ __global__ void __launch_bounds__(128) k_formu_boundary(FormUArgs A, Geo g, Boun
 // ------------------------------------------------------------------------------------------------
 // main k-marching kernel: interior faces only; boundary faces and seams come from the buffers
 // ------------------------------------------------------------------------------------------------
+template <class T>
 struct MainArgs {
-    const double *ucont, *ucat, *nut;
+    const T *ucont, *ucat, *nut;
     const double *csi, *eta, *zet;
     const double *icsi, *ieta, *izet, *jcsi, *jeta, *jzet, *kcsi, *keta, *kzet;
     const double *iaj, *jaj, *kaj;
     double *rhs;
-    const double *theta;
+    const T *theta;
     double *btheta;
     const double *bF1, *bF2, *bF3, *sI, *sJ, *sK;
     double scale, nu, bscale, tRef;
     int mode, quick, les;
 };
 
-__device__ __forceinline__ Flux ldflux(const double *p) {
-    return Flux{V3{p[0], p[1], p[2]}, V3{p[3], p[4], p[5]}};
+// boundary-buffer fluxes are stored as double; on the mixed path they hold float values exactly
+template <class T>
+__device__ __forceinline__ FluxT<T> ldflux(const double *p) {
+    return FluxT<T>{Vec3<T>{T(p[0]), T(p[1]), T(p[2])}, Vec3<T>{T(p[3]), T(p[4]), T(p[5])}};
 }
 
 // per-thread column pointers: every array at the thread's (j, i) column, plane 0
+template <class T>
 struct ColPtrs {
-    const double *ucont, *nut, *fc[3], *fe[3], *fz[3], *fa[3];
+    const T *ucont, *nut;
+    const double *fc[3], *fe[3], *fz[3], *fa[3];
 };
 
 // metrics and fluxes of one interior face, loaded ahead of use
-struct FaceIn { V3 fc, fe, fz; double fa, V, nutR; };
+template <class T>
+struct FaceIn { V3 fc, fe, fz; double fa; T V, nutR; };
 
-template <int DIR>
-__device__ __forceinline__ FaceIn face_in(const ColPtrs &P, long q, int s) {
-    FaceIn f;
+template <int DIR, class T>
+__device__ __forceinline__ FaceIn<T> face_in(const ColPtrs<T> &P, long q, int s) {
+    FaceIn<T> f;
     f.fc = ldnt3(P.fc[DIR] + 3 * q, 0);
     f.fe = ldnt3(P.fe[DIR] + 3 * q, 0);
     f.fz = ldnt3(P.fz[DIR] + 3 * q, 0);
@@ -307,101 +219,91 @@ __device__ __forceinline__ FaceIn face_in(const ColPtrs &P, long q, int s) {
     return f;
 }
 
+// viscous flux of an interior face from the LDS ring: the normal difference and the two four-point
+// transverse averages of Compute_du_i/j/k (inline.h:2748-2931, no boundary rule applies), then the
+// flux with the face's metrics converted to T
+template <int DIR, int NR, class T>
+__device__ __forceinline__ Vec3<T> ring_visc(const Ring<NR, T> &R, int row, int col, int t, V3 fc, V3 fe, V3 fz,
+                                             double fa, T nuEff) {
+    auto U = [&](int dk, int dj, int di) { return R.at((t + dk) & 3, row + dj, col + di); };
+    const int ok = (DIR == 2), oj = (DIR == 1), oi = (DIR == 0);
+    const int ak = 0, aj = (DIR == 0), ai = (DIR != 0);
+    const int bk = (DIR != 2), bj = (DIR == 2), bi = 0;
+    const Vec3<T> u0 = U(0, 0, 0), u1 = U(ok, oj, oi);
+    const Vec3<T> ap = U(ak + ok, aj + oj, ai + oi), a0 = U(ak, aj, ai), am = U(ok - ak, oj - aj, oi - ai), an = U(-ak, -aj, -ai);
+    const Vec3<T> bp = U(bk + ok, bj + oj, bi + oi), b0 = U(bk, bj, bi), bm = U(ok - bk, oj - bj, oi - bi), bn = U(-bk, -bj, -bi);
+    const T q4 = T(0.25);
+    const T nx = u1.x - u0.x, ny = u1.y - u0.y, nz = u1.z - u0.z;
+    const T ax = (ap.x + a0.x - am.x - an.x) * q4, ay = (ap.y + a0.y - am.y - an.y) * q4, az = (ap.z + a0.z - am.z - an.z) * q4;
+    const T bx = (bp.x + b0.x - bm.x - bn.x) * q4, by = (bp.y + b0.y - bm.y - bn.y) * q4, bz = (bp.z + b0.z - bm.z - bn.z) * q4;
+    GradT<T> g;
+    if (DIR == 0) {        // normal csi; transverse a = eta (j), b = zet (k)
+        g.dudc = nx; g.dvdc = ny; g.dwdc = nz; g.dude = ax; g.dvde = ay; g.dwde = az; g.dudz = bx; g.dvdz = by; g.dwdz = bz;
+    } else if (DIR == 1) { // normal eta; transverse a = csi (i), b = zet (k)
+        g.dudc = ax; g.dvdc = ay; g.dwdc = az; g.dude = nx; g.dvde = ny; g.dwde = nz; g.dudz = bx; g.dvdz = by; g.dwdz = bz;
+    } else {               // normal zet; transverse a = csi (i), b = eta (j)
+        g.dudc = ax; g.dvdc = ay; g.dwdc = az; g.dude = bx; g.dvde = by; g.dwde = bz; g.dudz = nx; g.dvdz = ny; g.dwdz = nz;
+    }
+    return visc_flux<T>(g, cvt3<T>(fc), cvt3<T>(fe), cvt3<T>(fz), DIR, T(fa), nuEff);
+}
+
+// convective flux of an interior face from the LDS ring (central or QUICK)
+template <int DIR, int NR, class T>
+__device__ __forceinline__ Vec3<T> ring_conv(const Ring<NR, T> &R, int row, int col, int t, int quick_, T V) {
+    auto U = [&](int dk, int dj, int di) { return R.at((t + dk) & 3, row + dj, col + di); };
+    const int ok = (DIR == 2), oj = (DIR == 1), oi = (DIR == 0);
+    const Vec3<T> u0 = U(0, 0, 0), u1 = U(ok, oj, oi);
+    Vec3<T> f;
+    const T h = T(0.5);
+    if (quick_) f = quick(U(-ok, -oj, -oi), u0, u1, U(2 * ok, 2 * oj, 2 * oi), V);
+    else f = Vec3<T>{h * (u0.x + u1.x), h * (u0.y + u1.y), h * (u0.z + u1.z)};
+    return Vec3<T>{-V * f.x, -V * f.y, -V * f.z};
+}
+
 // interior face flux from the LDS ring and preloaded face data (nut0: nu_t of the face's cell)
-template <int DIR, int NR>
-__device__ __forceinline__ Flux face_pre(const MainArgs &A, const FaceIn &in, double nut0, const Ring<NR> &R,
-                                         int row, int col, int t) {
-    auto U = [&](int dk, int dj, int di) { return R.at((t + dk) & 3, row + dj, col + di); };
-    const int ok = (DIR == 2), oj = (DIR == 1), oi = (DIR == 0);
-    const V3 u0 = U(0, 0, 0), u1 = U(ok, oj, oi);
-    Flux r;
-    r.d = V3{0, 0, 0};
-    r.v = V3{0, 0, 0};
-    if (A.mode != 2) {
-        V3 f;
-        if (A.quick) f = quick(U(-ok, -oj, -oi), u0, u1, U(2 * ok, 2 * oj, 2 * oi), in.V);
-        else f = V3{0.5 * (u0.x + u1.x), 0.5 * (u0.y + u1.y), 0.5 * (u0.z + u1.z)};
-        r.d = V3{-in.V * f.x, -in.V * f.y, -in.V * f.z};
-    }
+template <int DIR, int NR, class T>
+__device__ __forceinline__ FluxT<T> face_pre(const MainArgs<T> &A, const FaceIn<T> &in, T nut0, const Ring<NR, T> &R,
+                                             int row, int col, int t) {
+    FluxT<T> r;
+    r.d = Vec3<T>{0, 0, 0};
+    r.v = Vec3<T>{0, 0, 0};
+    if (A.mode != 2) r.d = ring_conv<DIR, NR, T>(R, row, col, t, A.quick, in.V);
     if (A.mode != 1) {
-        const int ak = 0, aj = (DIR == 0), ai = (DIR != 0);
-        const int bk = (DIR != 2), bj = (DIR == 2), bi = 0;
-        const V3 ap = U(ak + ok, aj + oj, ai + oi), a0 = U(ak, aj, ai), am = U(ok - ak, oj - aj, oi - ai), an = U(-ak, -aj, -ai);
-        const V3 bp = U(bk + ok, bj + oj, bi + oi), b0 = U(bk, bj, bi), bm = U(ok - bk, oj - bj, oi - bi), bn = U(-bk, -bj, -bi);
-        const double nx = u1.x - u0.x, ny = u1.y - u0.y, nz = u1.z - u0.z;
-        const double ax = (ap.x + a0.x - am.x - an.x) * 0.25, ay = (ap.y + a0.y - am.y - an.y) * 0.25,
-                     az = (ap.z + a0.z - am.z - an.z) * 0.25;
-        const double bx = (bp.x + b0.x - bm.x - bn.x) * 0.25, by = (bp.y + b0.y - bm.y - bn.y) * 0.25,
-                     bz = (bp.z + b0.z - bm.z - bn.z) * 0.25;
-        Grad g;
-        if (DIR == 0) {
-            g.dudc = nx; g.dvdc = ny; g.dwdc = nz; g.dude = ax; g.dvde = ay; g.dwde = az; g.dudz = bx; g.dvdz = by; g.dwdz = bz;
-        } else if (DIR == 1) {
-            g.dudc = ax; g.dvdc = ay; g.dwdc = az; g.dude = nx; g.dvde = ny; g.dwde = nz; g.dudz = bx; g.dvdz = by; g.dwdz = bz;
-        } else {
-            g.dudc = ax; g.dvdc = ay; g.dwdc = az; g.dude = bx; g.dvde = by; g.dwde = bz; g.dudz = nx; g.dvdz = ny; g.dwdz = nz;
-        }
-        double nuEff = A.nu;
-        if (A.les) nuEff = A.nu + 0.5 * (nut0 + in.nutR);
-        r.v = visc_flux(g, in.fc, in.fe, in.fz, DIR, in.fa, nuEff);
+        T nuEff = T(A.nu);
+        if (A.les) nuEff = T(A.nu) + T(0.5) * (nut0 + in.nutR);
+        r.v = ring_visc<DIR, NR, T>(R, row, col, t, in.fc, in.fe, in.fz, in.fa, nuEff);
     }
     return r;
 }
 
-// interior face flux from the LDS ring; q is the plane offset (in points) of the face's cell
-template <int DIR, int NR>
-__device__ __forceinline__ Flux face_main(const MainArgs &A, const ColPtrs &P, const Ring<NR> &R, long q, int row,
-                                          int col, int t, int s) {
-    auto U = [&](int dk, int dj, int di) { return R.at((t + dk) & 3, row + dj, col + di); };
-    const int ok = (DIR == 2), oj = (DIR == 1), oi = (DIR == 0);
-    const V3 u0 = U(0, 0, 0), u1 = U(ok, oj, oi);
-    Flux r;
-    r.d = V3{0, 0, 0};
-    r.v = V3{0, 0, 0};
-    if (A.mode != 2) {
-        const double V = __builtin_nontemporal_load(P.ucont + 3 * q + DIR);
-        V3 f;
-        if (A.quick) f = quick(U(-ok, -oj, -oi), u0, u1, U(2 * ok, 2 * oj, 2 * oi), V);
-        else f = V3{0.5 * (u0.x + u1.x), 0.5 * (u0.y + u1.y), 0.5 * (u0.z + u1.z)};
-        r.d = V3{-V * f.x, -V * f.y, -V * f.z};
-    }
+// interior face flux from the LDS ring, loading its face data; q is the plane offset of the face's cell
+template <int DIR, int NR, class T>
+__device__ __forceinline__ FluxT<T> face_main(const MainArgs<T> &A, const ColPtrs<T> &P, const Ring<NR, T> &R, long q,
+                                              int row, int col, int t, int s) {
+    FluxT<T> r;
+    r.d = Vec3<T>{0, 0, 0};
+    r.v = Vec3<T>{0, 0, 0};
+    if (A.mode != 2) r.d = ring_conv<DIR, NR, T>(R, row, col, t, A.quick, P.ucont[3 * q + DIR]);
     if (A.mode != 1) {
-        const int ak = 0, aj = (DIR == 0), ai = (DIR != 0);
-        const int bk = (DIR != 2), bj = (DIR == 2), bi = 0;
-        const V3 ap = U(ak + ok, aj + oj, ai + oi), a0 = U(ak, aj, ai), am = U(ok - ak, oj - aj, oi - ai), an = U(-ak, -aj, -ai);
-        const V3 bp = U(bk + ok, bj + oj, bi + oi), b0 = U(bk, bj, bi), bm = U(ok - bk, oj - bj, oi - bi), bn = U(-bk, -bj, -bi);
-        const double nx = u1.x - u0.x, ny = u1.y - u0.y, nz = u1.z - u0.z;
-        const double ax = (ap.x + a0.x - am.x - an.x) * 0.25, ay = (ap.y + a0.y - am.y - an.y) * 0.25,
-                     az = (ap.z + a0.z - am.z - an.z) * 0.25;
-        const double bx = (bp.x + b0.x - bm.x - bn.x) * 0.25, by = (bp.y + b0.y - bm.y - bn.y) * 0.25,
-                     bz = (bp.z + b0.z - bm.z - bn.z) * 0.25;
-        Grad g;
-        if (DIR == 0) {
-            g.dudc = nx; g.dvdc = ny; g.dwdc = nz; g.dude = ax; g.dvde = ay; g.dwde = az; g.dudz = bx; g.dvdz = by; g.dwdz = bz;
-        } else if (DIR == 1) {
-            g.dudc = ax; g.dvdc = ay; g.dwdc = az; g.dude = nx; g.dvde = ny; g.dwde = nz; g.dudz = bx; g.dvdz = by; g.dwdz = bz;
-        } else {
-            g.dudc = ax; g.dvdc = ay; g.dwdc = az; g.dude = bx; g.dvde = by; g.dwde = bz; g.dudz = nx; g.dvdz = ny; g.dwdz = nz;
-        }
-        double nuEff = A.nu;
-        if (A.les) nuEff = A.nu + 0.5 * (P.nut[q] + P.nut[q + s]);
-        r.v = visc_flux(g, ldnt3(P.fc[DIR] + 3 * q, 0), ldnt3(P.fe[DIR] + 3 * q, 0), ldnt3(P.fz[DIR] + 3 * q, 0), DIR,
-                        P.fa[DIR][q], nuEff);
+        T nuEff = T(A.nu);
+        if (A.les) nuEff = T(A.nu) + T(0.5) * (P.nut[q] + P.nut[q + s]);
+        r.v = ring_visc<DIR, NR, T>(R, row, col, t, ldnt3(P.fc[DIR] + 3 * q, 0), ldnt3(P.fe[DIR] + 3 * q, 0),
+                                    ldnt3(P.fz[DIR] + 3 * q, 0), P.fa[DIR][q], nuEff);
     }
     return r;
 }
 
-template <int NR>
-__global__ void __launch_bounds__(64 * NR) k_fused_formu(MainArgs A, Geo g, int kchunk) {
+template <int NR, class T>
+__global__ void __launch_bounds__(64 * NR) k_fused_formu(MainArgs<T> A, Geo g, int kchunk) {
     // Every thread evaluates its three faces and fp on every plane (threads outside the tile's
     // useful range work on clamped, in-bounds addresses and their results are discarded), so the
     // hot loop has no divergent branches; only the stores and the rare boundary-buffer loads are
     // predicated.
     constexpr int TI = 62, TJ = NR - 2;
-    __shared__ double sF2[6][NR][64];     // j-face fluxes of the plane (div xyz, visc xyz)
+    __shared__ T sF2[6][NR][64];          // j-face fluxes of the plane (div xyz, visc xyz)
     __shared__ double sFp[2][3][NR][64];  // fp of the previous plane (double-buffered)
-    __shared__ Ring<NR> ring;             // ucat planes t-1 .. t+2
-    using RingT = Ring<NR>;
+    __shared__ Ring<NR, T> ring;          // ucat planes t-1 .. t+2
+    using RingT = Ring<NR, T>;
     constexpr int PLANE = RingT::RJ * RingT::RI * 3;
     constexpr int PRE = (PLANE + 64 * NR - 1) / (64 * NR);
     const int tid = threadIdx.x + 64 * threadIdx.y;
@@ -426,14 +328,14 @@ __global__ void __launch_bounds__(64 * NR) k_fused_formu(MainArgs A, Geo g, int 
     const int ia = min(i, g.mx - 1), ja = min(j, g.my - 1);
     const int SJ = g.gxm, SK = g.gxm * g.gym;
     const int c0 = g.L(0, ja, ia);
-    ColPtrs P;
+    ColPtrs<T> P;
     P.ucont = A.ucont + 3 * (long)c0; P.nut = A.nut + c0;
     P.fc[0] = A.icsi + 3 * (long)c0; P.fe[0] = A.ieta + 3 * (long)c0; P.fz[0] = A.izet + 3 * (long)c0;
     P.fc[1] = A.jcsi + 3 * (long)c0; P.fe[1] = A.jeta + 3 * (long)c0; P.fz[1] = A.jzet + 3 * (long)c0;
     P.fc[2] = A.kcsi + 3 * (long)c0; P.fe[2] = A.keta + 3 * (long)c0; P.fz[2] = A.kzet + 3 * (long)c0;
     P.fa[0] = A.iaj + c0; P.fa[1] = A.jaj + c0; P.fa[2] = A.kaj + c0;
     const double *pcsi = A.csi + 3 * (long)c0, *peta = A.eta + 3 * (long)c0, *pzet = A.zet + 3 * (long)c0;
-    const double *ptheta = A.theta + c0;
+    const T *ptheta = A.theta + c0;
     const long SG = (long)g.mx * g.my;
     double *prhs = A.rhs + 3 * (long)(ja * g.mx + ia);
     double *pbt = A.btheta + 3 * (long)(ja * g.mx + ia);
@@ -447,7 +349,7 @@ __global__ void __launch_bounds__(64 * NR) k_fused_formu(MainArgs A, Geo g, int 
     // ucat ring: rows j0-2 .., cols i0-2 ..
     const int ri0 = i0 - 2, rj0 = j0 - 2;
     const int rowMax = min(RingT::RJ - 1, g.my + 2 - rj0), colMax = min(RingT::RI - 1, g.mx + 2 - ri0);
-    auto plane_load = [&](int k, double *buf) {
+    auto plane_load = [&](int k, T *buf) {
         for (int n = 0; n < PRE; n++) {
             const int idx = tid + n * 64 * NR;
             buf[n] = 0.0;
@@ -457,7 +359,7 @@ __global__ void __launch_bounds__(64 * NR) k_fused_formu(MainArgs A, Geo g, int 
             }
         }
     };
-    auto plane_store = [&](int k, const double *buf) {
+    auto plane_store = [&](int k, const T *buf) {
         const int slot = k & 3;
         for (int n = 0; n < PRE; n++) {
             const int idx = tid + n * 64 * NR;
@@ -468,15 +370,15 @@ __global__ void __launch_bounds__(64 * NR) k_fused_formu(MainArgs A, Geo g, int 
         }
     };
     const int rrow = r + 1, rcol = lane + 1;
-    auto kface = [&](int k) -> Flux {          // k face k of this column
-        if (k == 0) return ldflux(pF3);
-        if (k == g.mz - 2) return ldflux(pF3 + (long)nK * 6);
-        return face_main<2, NR>(A, P, ring, (long)k * SK, rrow, rcol, k, SK);
+    auto kface = [&](int k) -> FluxT<T> {      // k face k of this column
+        if (k == 0) return ldflux<T>(pF3);
+        if (k == g.mz - 2) return ldflux<T>(pF3 + (long)nK * 6);
+        return face_main<2, NR, T>(A, P, ring, (long)k * SK, rrow, rcol, k, SK);
     };
     // prime: k face kb-1 needs planes kb-2 .. kb+1, the first plane step kb-1 .. kb+2
-    Flux F3p;
+    FluxT<T> F3p;
     {
-        double buf[PRE];
+        T buf[PRE];
         for (int k = kb - 2; k <= kb + 1; k++) { plane_load(k, buf); plane_store(k, buf); }
         __syncthreads();
         F3p = kface(kb - 1);
@@ -494,22 +396,23 @@ __global__ void __launch_bounds__(64 * NR) k_fused_formu(MainArgs A, Geo g, int 
            gzKp = P.fz[2][3 * (long)(kb - 1) * SK + 2];
     for (int t = kb; t <= ke; t++) {
         // ---- every global load of this plane step, issued together ------------------------------
-        double pre[PRE];
+        T pre[PRE];
         const bool doPre = t + 3 <= ke + 2;
         if (doPre) plane_load(t + 3, pre);
         const bool ghostK = (t == g.mz - 1);
         const long q = (long)min(t, g.mz - 2) * SK;           // faces of plane t (clamped at the ghost plane)
-        const FaceIn in1 = face_in<0>(P, q, 1), in2 = face_in<1>(P, q, SJ), in3 = face_in<2>(P, q, SK);
-        const double nut0 = P.nut[q];
+        const FaceIn<T> in1 = face_in<0>(P, q, 1), in2 = face_in<1>(P, q, SJ), in3 = face_in<2>(P, q, SK);
+        const T nut0 = P.nut[q];
         // ---- face fluxes ---------------------------------------------------------------------------
-        Flux F1{V3{0, 0, 0}, V3{0, 0, 0}}, F2{V3{0, 0, 0}, V3{0, 0, 0}}, F3{V3{0, 0, 0}, V3{0, 0, 0}};
+        const Vec3<T> z3{0, 0, 0};
+        FluxT<T> F1{z3, z3}, F2{z3, z3}, F3{z3, z3};
         if (!ghostK) {
-            F1 = face_pre<0, NR>(A, in1, nut0, ring, rrow, rcol, t);
-            F2 = face_pre<1, NR>(A, in2, nut0, ring, rrow, rcol, t);
-            if (t == g.mz - 2) F3 = ldflux(pF3 + (long)nK * 6);
-            else F3 = face_pre<2, NR>(A, in3, nut0, ring, rrow, rcol, t);
-            if (bndI) F1 = ldflux(pF1 + (long)t * g.my * 6);
-            if (bndJ) F2 = ldflux(pF2 + (long)t * g.mx * 6);
+            F1 = face_pre<0, NR, T>(A, in1, nut0, ring, rrow, rcol, t);
+            F2 = face_pre<1, NR, T>(A, in2, nut0, ring, rrow, rcol, t);
+            if (t == g.mz - 2) F3 = ldflux<T>(pF3 + (long)nK * 6);
+            else F3 = face_pre<2, NR, T>(A, in3, nut0, ring, rrow, rcol, t);
+            if (bndI) F1 = ldflux<T>(pF1 + (long)t * g.my * 6);
+            if (bndJ) F2 = ldflux<T>(pF2 + (long)t * g.mx * 6);
         }
         // ---- loads of the Rhs / buoyancy step (cells of plane t-1), in flight across the exchange
         const long qp = (long)(t - 1) * SK;
@@ -520,13 +423,13 @@ __global__ void __launch_bounds__(64 * NR) k_fused_formu(MainArgs A, Geo g, int 
         const double r0 = prhs[ro], r1 = prhs[ro + 1], r2 = prhs[ro + 2];
         double th = 0, thI = 0, thJ = 0, thK = 0;
         if (buoy) { th = ptheta[qp]; thI = ptheta[qp + 1]; thJ = ptheta[qp + SJ]; thK = ptheta[qp + SK]; }
-        const V3 F1dm = shfl_up3(F1.d), F1vm = shfl_up3(F1.v);
+        const Vec3<T> F1dm = shfl_up3(F1.d), F1vm = shfl_up3(F1.v);
         sF2[0][r][lane] = F2.d.x; sF2[1][r][lane] = F2.d.y; sF2[2][r][lane] = F2.d.z;
         sF2[3][r][lane] = F2.v.x; sF2[4][r][lane] = F2.v.y; sF2[5][r][lane] = F2.v.z;
         __syncthreads();
         const int rm = r > 0 ? r - 1 : 0;
-        const V3 F2dm{sF2[0][rm][lane], sF2[1][rm][lane], sF2[2][rm][lane]};
-        const V3 F2vm{sF2[3][rm][lane], sF2[4][rm][lane], sF2[5][rm][lane]};
+        const Vec3<T> F2dm{sF2[0][rm][lane], sF2[1][rm][lane], sF2[2][rm][lane]};
+        const Vec3<T> F2vm{sF2[3][rm][lane], sF2[4][rm][lane], sF2[5][rm][lane]};
         V3 fp;
         fp.x = fpsum(F1.d.x, F1dm.x, F2.d.x, F2dm.x, F3.d.x, F3p.d.x, F1.v.x, F1vm.x, F2.v.x, F2vm.x, F3.v.x, F3p.v.x, A.mode);
         fp.y = fpsum(F1.d.y, F1dm.y, F2.d.y, F2dm.y, F3.d.y, F3p.d.y, F1.v.y, F1vm.y, F2.v.y, F2vm.y, F3.v.y, F3p.v.y, A.mode);
@@ -581,7 +484,8 @@ __global__ void __launch_bounds__(64 * NR) k_fused_formu(MainArgs A, Geo g, int 
     }
 }
 
-void fused_formu(const Geo &g, const FormUArgs &A, double *ws, hipStream_t s) {
+template <class T>
+void fused_formu(const Geo &g, const FormUArgsT<T> &A, double *ws, hipStream_t s) {
     constexpr int NR = 8;
     const int TI = 62, TJ = NR - 2;
     // boundary faces and seams first
@@ -591,9 +495,9 @@ void fused_formu(const Geo &g, const FormUArgs &A, double *ws, hipStream_t s) {
     for (int d = 0; d < 3; d++) { const long n = tr_dims(g, d, n1, n2); B.bF[d] = ws + off; off += 2 * n * 6; }
     for (int d = 0; d < 3; d++) { const long n = tr_dims(g, d, n1, n2); B.seam[d] = ws + off; off += n * 3; }
     const int nmax = max(g.my * g.mz, max(g.mx * g.mz, g.mx * g.my));
-    TOSCA_LAUNCH("k_formu_boundary", k_formu_boundary, dim3((nmax + 127) / 128, 1, 9), dim3(128), 0, s, A, g, B);
+    TOSCA_LAUNCH(sizeof(T) == 8 ? "k_formu_boundary" : "k_formu_boundary_f32", k_formu_boundary<T>, dim3((nmax + 127) / 128, 1, 9), dim3(128), 0, s, A, g, B);
 
-    MainArgs M;
+    MainArgs<T> M;
     // the main kernel loads nu_t unconditionally; without LES (or without the viscous term) the
     // caller passes no nu_t, so any valid local array stands in (its values are never used)
     M.ucont = A.ucont; M.ucat = A.ucat; M.nut = A.nut ? A.nut : A.ucat;
@@ -614,10 +518,13 @@ void fused_formu(const Geo &g, const FormUArgs &A, double *ws, hipStream_t s) {
     const int kchunk = (nk + chunks - 1) / chunks;
     chunks = (nk + kchunk - 1) / kchunk;
     dim3 grid((g.mx - 2 + TI - 1) / TI, (g.my - 2 + TJ - 1) / TJ, chunks);
-    TOSCA_LAUNCH("k_fused_formu", k_fused_formu<NR>, grid, dim3(64, NR), 0, s, M, g, kchunk);
+    TOSCA_LAUNCH(sizeof(T) == 8 ? "k_fused_formu" : "k_fused_formu_f32", (k_fused_formu<NR, T>), grid, dim3(64, NR), 0, s, M, g, kchunk);
     // the reference zeroes all of bTheta; the kernel writes the interior cells, this the rest
     if (A.btheta) zero_shell(A.btheta, 3, g.mx, g.my, g.mz, 0, 0, s);
 }
+
+template void fused_formu<double>(const Geo &, const FormUArgsT<double> &, double *, hipStream_t);
+template void fused_formu<float>(const Geo &, const FormUArgsT<float> &, double *, hipStream_t);
 
 size_t formu_ws_doubles(const Geo &g) {
     long tot = 0;

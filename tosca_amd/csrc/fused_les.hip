@@ -25,10 +25,10 @@ constexpr int NQ = 28;   // 27 Germano quantities + the binomial-weighted volume
 constexpr int DSM_NR = 8;
 constexpr int DSM_TI = 62, DSM_TJ = DSM_NR - 2;
 
-__device__ __forceinline__ double shup(double x) { return __shfl_up(x, 1, 64); }
-__device__ __forceinline__ double shdn(double x) { return __shfl_down(x, 1, 64); }
-__device__ __forceinline__ V3 shup3(V3 a) { return V3{shup(a.x), shup(a.y), shup(a.z)}; }
-__device__ __forceinline__ V3 shdn3(V3 a) { return V3{shdn(a.x), shdn(a.y), shdn(a.z)}; }
+template <class T> __device__ __forceinline__ T shup(T x) { return __shfl_up(x, 1, 64); }
+template <class T> __device__ __forceinline__ T shdn(T x) { return __shfl_down(x, 1, 64); }
+template <class T> __device__ __forceinline__ Vec3<T> shup3(Vec3<T> a) { return Vec3<T>{shup(a.x), shup(a.y), shup(a.z)}; }
+template <class T> __device__ __forceinline__ Vec3<T> shdn3(Vec3<T> a) { return Vec3<T>{shdn(a.x), shdn(a.y), shdn(a.z)}; }
 
 // sum of the 1-D Simpson weights (1, 4, 1) over the interior points c-1, c, c+1 of an axis of size m
 __device__ __forceinline__ double simpson_mask_sum(int c, int m) {
@@ -36,10 +36,11 @@ __device__ __forceinline__ double simpson_mask_sum(int c, int m) {
 }
 }  // namespace
 
-__global__ void __launch_bounds__(64 * DSM_NR) k_dsm_lm(Geo g, const double *ucat, const double *csi,
+template <class T>
+__global__ void __launch_bounds__(64 * DSM_NR) k_dsm_lm(Geo g, const T *ucat, const double *csi,
                                                         const double *eta, const double *zet, const double *aj,
                                                         double *LM, double *MM, double *Sabs, int kchunk) {
-    __shared__ double sU[3][DSM_NR][64];       // ucat of the plane, for the j neighbours
+    __shared__ T sU[3][DSM_NR][64];            // ucat of the plane, for the j neighbours
     __shared__ double sH[NQ][DSM_NR][64];      // i-filtered quantities of the plane
     const int lane = threadIdx.x, r = threadIdx.y;
     const int tile = xcd_tile();                // j fastest, then i, then the k chunk
@@ -54,10 +55,10 @@ __global__ void __launch_bounds__(64 * DSM_NR) k_dsm_lm(Geo g, const double *uca
     const double dij = mi && mj ? simpson_mask_sum(i, g.mx) * simpson_mask_sum(j, g.my) : 1.0;
     const int SK = g.gxm * g.gym;
     const long c0 = g.L(0, ja, ia);
-    const double *pu = ucat + 3 * c0;
+    const T *pu = ucat + 3 * c0;
 
     // k-ring of ucat for this column: planes K-1, K, K+1 of the current point plane K
-    V3 uKm = ld3(pu, (kb - 2) * SK), uK = ld3(pu, (kb - 1) * SK);
+    Vec3<T> uKm = ld3(pu, (kb - 2) * SK), uK = ld3(pu, (kb - 1) * SK);
     const double *pc = csi + 3 * c0, *pe = eta + 3 * c0, *pz = zet + 3 * c0, *pa = aj + c0;
 
     // partial sums along k: A for output K-1 (terms K-2, K-1 in), B for output K (term K-1 in)
@@ -69,20 +70,20 @@ __global__ void __launch_bounds__(64 * DSM_NR) k_dsm_lm(Geo g, const double *uca
     for (int K = kb - 1; K <= ke; K++) {
         // loads of this plane, issued before the exchange barrier (a register prefetch of the next
         // plane spills at this register budget and measured slower)
-        const V3 uKp = ld3(pu, (K + 1) * SK);
+        const Vec3<T> uKp = ld3(pu, (K + 1) * SK);
         const V3 cC = ld3(pc, K * SK), eC = ld3(pe, K * SK), zC = ld3(pz, K * SK);
         const double aC = pa[(long)K * SK];
         // ---- neighbours of the point (i, j, K) ----------------------------------------------------
         sU[0][r][lane] = uK.x; sU[1][r][lane] = uK.y; sU[2][r][lane] = uK.z;
         const bool pt = mi && mj && K >= 1 && K <= g.mz - 2;     // point inside the filter mask
-        V3 uim = shup3(uK), uip = shdn3(uK);
+        Vec3<T> uim = shup3(uK), uip = shdn3(uK);
         if (lane == 0) uim = ld3(ucat, g.L(K, ja, ima));
         if (lane == 63) uip = ld3(ucat, g.L(K, ja, ipa));
         __syncthreads();
-        V3 ujm, ujp;
-        if (r > 0) ujm = V3{sU[0][r - 1][lane], sU[1][r - 1][lane], sU[2][r - 1][lane]};
+        Vec3<T> ujm, ujp;
+        if (r > 0) ujm = Vec3<T>{sU[0][r - 1][lane], sU[1][r - 1][lane], sU[2][r - 1][lane]};
         else ujm = ld3(ucat, g.L(K, jma, ia));
-        if (r < DSM_NR - 1) ujp = V3{sU[0][r + 1][lane], sU[1][r + 1][lane], sU[2][r + 1][lane]};
+        if (r < DSM_NR - 1) ujp = Vec3<T>{sU[0][r + 1][lane], sU[1][r + 1][lane], sU[2][r + 1][lane]};
         else ujp = ld3(ucat, g.L(K, jpa, ia));
         // ---- point quantities (les.c:466-611), zero outside the mask -------------------------------
         // base values b: u v w, U V W, Sxx Sxy Sxz Syy Syz Szz, |S|, 1/aj; the 28 filtered quantities
@@ -91,13 +92,17 @@ __global__ void __launch_bounds__(64 * DSM_NR) k_dsm_lm(Geo g, const double *uca
 #pragma unroll
         for (int n = 0; n < 14; n++) b[n] = 0.0;
         if (pt) {
-            const Cart d = du_dxyz(cC, eC, zC, aC, du_center_v(g, i, j, K, uK, uim, uip, ujm, ujp, uKm, uKp));
+            // velocity arithmetic in T (the strain, |S| and the contravariant velocity); the
+            // filtered quantities below are formed and summed in double
+            const Vec3<T> cT = cvt3<T>(cC), eT = cvt3<T>(eC), zT = cvt3<T>(zC);
+            const CartT<T> d = du_dxyz(cT, eT, zT, T(aC), du_center_v(g, i, j, K, uK, uim, uip, ujm, ujp, uKm, uKp));
+            const T h = T(0.5);
             b[0] = uK.x; b[1] = uK.y; b[2] = uK.z;
-            b[3] = uK.x * cC.x + uK.y * cC.y + uK.z * cC.z;
-            b[4] = uK.x * eC.x + uK.y * eC.y + uK.z * eC.z;
-            b[5] = uK.x * zC.x + uK.y * zC.y + uK.z * zC.z;
-            b[6] = 0.5 * (d.du_dx + d.du_dx); b[7] = 0.5 * (d.du_dy + d.dv_dx); b[8] = 0.5 * (d.du_dz + d.dw_dx);
-            b[9] = 0.5 * (d.dv_dy + d.dv_dy); b[10] = 0.5 * (d.dv_dz + d.dw_dy); b[11] = 0.5 * (d.dw_dz + d.dw_dz);
+            b[3] = uK.x * cT.x + uK.y * cT.y + uK.z * cT.z;
+            b[4] = uK.x * eT.x + uK.y * eT.y + uK.z * eT.z;
+            b[5] = uK.x * zT.x + uK.y * zT.y + uK.z * zT.z;
+            b[6] = h * (d.du_dx + d.du_dx); b[7] = h * (d.du_dy + d.dv_dx); b[8] = h * (d.du_dz + d.dw_dx);
+            b[9] = h * (d.dv_dy + d.dv_dy); b[10] = h * (d.dv_dz + d.dw_dy); b[11] = h * (d.dw_dz + d.dw_dz);
             b[12] = strain_mag(d);
             b[13] = 1. / aC;
         }
@@ -189,8 +194,9 @@ __device__ __forceinline__ bool remap_axis(int &I, int centre, int m, int ptype,
 constexpr int BX = 32, BY = 8, BZ = 4;
 constexpr int HX = BX + 2, HY = BY + 2, HZ = BZ + 2, HN = HX * HY * HZ;
 
+template <class T>
 __global__ void __launch_bounds__(BX * BY) k_dsm_cs_nut(Geo g, const double *LM, const double *MM, const double *Sabs,
-                                                        const double *aj, double maxCs, double *Cs, double *nut) {
+                                                        const double *aj, double maxCs, double *Cs, T *nut) {
     __shared__ double W[HN], Lw[HN], Mw[HN];
     const int tid = threadIdx.x + BX * threadIdx.y;
     const int tile = xcd_tile();     // i fastest, then j, then k
@@ -236,48 +242,51 @@ __global__ void __launch_bounds__(BX * BY) k_dsm_cs_nut(Geo g, const double *LM,
         Cs[c] = cs;
         if (nut) {
             const double filter = pow(1. / aj[c], 1. / 3.);
-            nut[c] = cs * (filter * filter) * Sabs[c];
+            nut[c] = T(cs * (filter * filter) * Sabs[c]);
         }
     }
 }
 
-__global__ void __launch_bounds__(256) k_nut_fused(Geo g, const double *ucat, const double *csi, const double *eta, const double *zet,
-                            const double *aj, const double *Cs, double *nut, double *ksgs) {
+template <class T>
+__global__ void __launch_bounds__(256) k_nut_fused(Geo g, const T *ucat, const double *csi, const double *eta, const double *zet,
+                            const double *aj, const double *Cs, T *nut, T *ksgs) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y * blockDim.y + threadIdx.y + 1,
               k = blockIdx.z + 1;
     if (i > g.mx - 2 || j > g.my - 2 || k > g.mz - 2) return;
     const int c = g.L(k, j, i);
     const V3 cc = ld3(csi, c), ee = ld3(eta, c), zz = ld3(zet, c);
     const double a = aj[c];
-    const double S = strain_mag(du_dxyz(cc, ee, zz, a, du_center(g, ucat, i, j, k)));
+    const double S = strain_mag(du_dxyz(cvt3<T>(cc), cvt3<T>(ee), cvt3<T>(zz), T(a), du_center(g, ucat, i, j, k)));
     const double filter = pow(1. / a, 1. / 3.);
     const double nt = Cs[c] * pow(filter, 2.0) * S;
-    nut[c] = nt;
+    nut[c] = T(nt);
     if (ksgs) {
         const double V = 1.0 / a;
         const double dx = V / sqrt(zz.x * zz.x + zz.y * zz.y + zz.z * zz.z);
         const double dy = V / sqrt(cc.x * cc.x + cc.y * cc.y + cc.z * cc.z);
         const double dz = V / sqrt(ee.x * ee.x + ee.y * ee.y + ee.z * ee.z);
         const double d2 = 3.0 / (1.0 / (dx * dx) + 1.0 / (dy * dy) + 1.0 / (dz * dz));
-        ksgs[c] = pow(nt, 2.0) / (pow(0.094, 2.0) * d2);
+        ksgs[c] = T(pow(nt, 2.0) / (pow(0.094, 2.0) * d2));
     }
 }
 
 // zero an ni x nj x nk box of an array with row length nx and plane size nx*ny (ncomp values per point)
-__global__ void k_zero_box(double *a, int ncomp, int nx, int ny, int i0, int j0, int k0, int ni) {
+template <class T>
+__global__ void k_zero_box(T *a, int ncomp, int nx, int ny, int i0, int j0, int k0, int ni) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ni * ncomp) return;
     const long p = ((long)(k0 + blockIdx.z) * ny + (j0 + blockIdx.y)) * nx + i0;
-    a[p * ncomp + t] = 0.0;
+    a[p * ncomp + t] = T(0);
 }
 
-void zero_shell(double *a, int ncomp, int nx, int ny, int nz, int lo, int hi, hipStream_t s) {
+template <class T>
+void zero_shell(T *a, int ncomp, int nx, int ny, int nz, int lo, int hi, hipStream_t s) {
     // boxes (in array coordinates) covering everything but [lo+1 .. n-hi-2] on every axis:
     // `lo` / `hi` extra layers outside the boundary cells on the low / high side
     const int w0 = lo + 1, w1 = hi + 1;            // shell thickness per side
     auto box = [&](int i0, int j0, int k0, int ni, int nj, int nk) {
         if (ni <= 0 || nj <= 0 || nk <= 0) return;
-        TOSCA_LAUNCH("k_zero_box", k_zero_box, dim3((ni * ncomp + 255) / 256, nj, nk), dim3(256), 0, s, a, ncomp, nx, ny,
+        TOSCA_LAUNCH("k_zero_box", k_zero_box<T>, dim3((ni * ncomp + 255) / 256, nj, nk), dim3(256), 0, s, a, ncomp, nx, ny,
                      i0, j0, k0, ni);
     };
     box(0, 0, 0, nx, ny, w0);                                   // low k slab
@@ -288,12 +297,14 @@ void zero_shell(double *a, int ncomp, int nx, int ny, int nz, int lo, int hi, hi
     box(nx - w1, w0, w0, w1, ny - w0 - w1, nz - w0 - w1);       // high i slab
 }
 
-void zero_outside_interior(double *a, const Geo &g, hipStream_t s) {
+template <class T>
+void zero_outside_interior(T *a, const Geo &g, hipStream_t s) {
     zero_shell(a, 1, g.gxm, g.gym, g.gzm, GW, GW, s);
 }
 
-void fused_dsm(const Geo &g, const tosca_mesh *m, const double *ucat, double maxCs, double *cs, double *nut,
-               double *ws, hipStream_t s) {
+template <class T>
+void fused_dsm(const Geo &g, const tosca_mesh *m, const T *ucat, double maxCs, double *cs, T *nut, double *ws,
+               hipStream_t s) {
     const long n = g.nloc();
     double *LM = ws, *MM = ws + n, *Sabs = ws + 2 * n;
     const int nk = g.mz - 2;
@@ -303,18 +314,30 @@ void fused_dsm(const Geo &g, const tosca_mesh *m, const double *ucat, double max
     const int kchunk = (nk + chunks - 1) / chunks;
     chunks = (nk + kchunk - 1) / kchunk;
     dim3 grid((g.mx - 2 + DSM_TI - 1) / DSM_TI, (g.my - 2 + DSM_TJ - 1) / DSM_TJ, chunks);
-    TOSCA_LAUNCH("k_dsm_lm", k_dsm_lm, grid, dim3(64, DSM_NR), 0, s, g, ucat, m->csi, m->eta, m->zet, m->aj, LM, MM,
+    TOSCA_LAUNCH(sizeof(T) == 8 ? "k_dsm_lm" : "k_dsm_lm_f32", k_dsm_lm<T>, grid, dim3(64, DSM_NR), 0, s, g, ucat, m->csi, m->eta, m->zet, m->aj, LM, MM,
                  Sabs, kchunk);
-    TOSCA_LAUNCH("k_dsm_cs_nut", k_dsm_cs_nut, dim3((g.mx - 2 + BX - 1) / BX, (g.my - 2 + BY - 1) / BY, (g.mz - 2 + BZ - 1) / BZ),
+    TOSCA_LAUNCH(sizeof(T) == 8 ? "k_dsm_cs_nut" : "k_dsm_cs_nut_f32", k_dsm_cs_nut<T>, dim3((g.mx - 2 + BX - 1) / BX, (g.my - 2 + BY - 1) / BY, (g.mz - 2 + BZ - 1) / BZ),
                  dim3(BX, BY), 0, s, g, LM, MM, Sabs, m->aj, maxCs, cs, nut);
 }
 
-void fused_nut(const Geo &g, const tosca_mesh *m, const double *ucat, const double *cs, double *nut, double *ksgs,
-               hipStream_t s) {
-    TOSCA_LAUNCH("k_nut_fused", k_nut_fused, dim3((g.mx - 2 + 63) / 64, (g.my - 2 + 3) / 4, g.mz - 2), dim3(64, 4), 0,
+template <class T>
+void fused_nut(const Geo &g, const tosca_mesh *m, const T *ucat, const double *cs, T *nut, T *ksgs, hipStream_t s) {
+    TOSCA_LAUNCH(sizeof(T) == 8 ? "k_nut_fused" : "k_nut_fused_f32", k_nut_fused<T>, dim3((g.mx - 2 + 63) / 64, (g.my - 2 + 3) / 4, g.mz - 2), dim3(64, 4), 0,
                  s, g, ucat, m->csi, m->eta, m->zet, m->aj, cs, nut, ksgs);
 }
 
-void nut_bcs(double *nut, const tosca_params *p, const Geo &g, hipStream_t s) { launch_nut_bcs(nut, p, g, s); }
+template <class T>
+void nut_bcs(T *nut, const tosca_params *p, const Geo &g, hipStream_t s) { launch_nut_bcs(nut, p, g, s); }
+
+#define TOSCA_LES_INSTANTIATE(T)                                                                            \
+    template void zero_shell<T>(T *, int, int, int, int, int, int, hipStream_t);                            \
+    template void zero_outside_interior<T>(T *, const Geo &, hipStream_t);                                  \
+    template void fused_dsm<T>(const Geo &, const tosca_mesh *, const T *, double, double *, T *, double *,   \
+                               hipStream_t);                                                                 \
+    template void fused_nut<T>(const Geo &, const tosca_mesh *, const T *, const double *, T *, T *,         \
+                               hipStream_t);                                                                 \
+    template void nut_bcs<T>(T *, const tosca_params *, const Geo &, hipStream_t);
+TOSCA_LES_INSTANTIATE(double)
+TOSCA_LES_INSTANTIATE(float)
 
 }  // namespace tosca

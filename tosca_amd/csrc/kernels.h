@@ -7,10 +7,10 @@
 namespace tosca {
 
 // ghost handling (ref_kernels.hip)
-void launch_wrap(double *a, int ncomp, const Geo &g, hipStream_t s);
+template <class T> void launch_wrap(T *a, int ncomp, const Geo &g, hipStream_t s);
 void launch_reset_cell_periodic(double *a, int ncomp, const Geo &g, hipStream_t s);
 void launch_fill(double *a, long n, double v, hipStream_t s);
-void launch_nut_bcs(double *nut, const tosca_params *p, const Geo &g, hipStream_t s);
+template <class T> void launch_nut_bcs(T *nut, const tosca_params *p, const Geo &g, hipStream_t s);
 
 // reference-structured path (ref_kernels.hip); `ws` is a workspace of >= 22 * nloc doubles
 void ref_formu(const Geo &g, const tosca_mesh *m, const tosca_params *p, const double *ucont, const double *ucat,

@@ -18,7 +18,8 @@ namespace tosca {
 // ---------------------------------------------------------------------------------------------
 // DMLocalToLocal along one DM_BOUNDARY_PERIODIC axis: ghost slabs take the wrapped owned values.
 // The transverse ranges cover the ghosts of periodic-wrap axes and the owned range otherwise.
-__global__ void k_wrap_axis(double *a, int ncomp, Geo g, int axis) {
+template <class T>
+__global__ void k_wrap_axis(T *a, int ncomp, Geo g, int axis) {
     const int m[3] = {g.mx, g.my, g.mz};
     int t1 = (axis == 0) ? 1 : 0, t2 = (axis == 2) ? 1 : 2;
     int lo1 = (g.per[t1] == TOSCA_PERIODIC_WRAP) ? -GW : 0, n1 = m[t1] + ((g.per[t1] == TOSCA_PERIODIC_WRAP) ? 2 * GW : 0);
@@ -418,7 +419,8 @@ struct NutBC { int bc[6]; double val[6]; };
 
 // UpdateNutBCs (boundary.c:2812-2917) along one axis: ghost cells next to the interior cells of
 // both patches of `axis`.  Reads interior values only.
-__global__ void k_nut_bc(double *nut, NutBC B, Geo g, int axis) {
+template <class T>
+__global__ void k_nut_bc(T *nut, NutBC B, Geo g, int axis) {
     const int m[3] = {g.mx, g.my, g.mz};
     int t1 = (axis == 0) ? 1 : 0, t2 = (axis == 2) ? 1 : 2;
     int a1 = blockIdx.x * blockDim.x + threadIdx.x + 1, a2 = blockIdx.y + 1, side = blockIdx.z;
@@ -429,9 +431,9 @@ __global__ void k_nut_bc(double *nut, NutBC B, Geo g, int axis) {
     idx[t2] = src[t2] = a2;
     idx[axis] = side ? n - 1 : 0;
     int bc = B.bc[patch];
-    double v;
+    T v;
     if (bc == TOSCA_BCNUT_ZEROGRADIENT) { src[axis] = side ? n - 2 : 1; v = nut[g.L(src[2], src[1], src[0])]; }
-    else if (bc == TOSCA_BCNUT_FIXEDVALUE) v = B.val[patch];
+    else if (bc == TOSCA_BCNUT_FIXEDVALUE) v = T(B.val[patch]);
     else if (bc == TOSCA_BCNUT_PERIODIC && g.per[axis] != TOSCA_PERIODIC_NONE) { src[axis] = side ? 1 : n - 2; v = nut[g.L(src[2], src[1], src[0])]; }
     else return;
     nut[g.L(idx[2], idx[1], idx[0])] = v;
@@ -476,16 +478,19 @@ __global__ void k_buoyancy(const double *tmprt, const double *icsi, const double
 static inline dim3 blk2() { return dim3(64, 4, 1); }
 static inline dim3 grd(int ni, int nj, int nk) { return dim3((ni + 63) / 64, (nj + 3) / 4, nk); }
 
-void launch_wrap(double *a, int ncomp, const Geo &g, hipStream_t s) {
+template <class T>
+void launch_wrap(T *a, int ncomp, const Geo &g, hipStream_t s) {
     const int m[3] = {g.mx, g.my, g.mz};
     for (int axis = 0; axis < 3; axis++) {
         if (g.per[axis] != TOSCA_PERIODIC_WRAP) continue;
         int t1 = (axis == 0) ? 1 : 0, t2 = (axis == 2) ? 1 : 2;
         int n1 = m[t1] + ((g.per[t1] == TOSCA_PERIODIC_WRAP) ? 2 * GW : 0);
         int n2 = m[t2] + ((g.per[t2] == TOSCA_PERIODIC_WRAP) ? 2 * GW : 0);
-        TOSCA_LAUNCH("k_wrap_axis", k_wrap_axis, dim3((n1 + 127) / 128, n2, 2 * GW), dim3(128), 0, s, a, ncomp, g, axis);
+        TOSCA_LAUNCH("k_wrap_axis", k_wrap_axis<T>, dim3((n1 + 127) / 128, n2, 2 * GW), dim3(128), 0, s, a, ncomp, g, axis);
     }
 }
+template void launch_wrap<double>(double *, int, const Geo &, hipStream_t);
+template void launch_wrap<float>(float *, int, const Geo &, hipStream_t);
 
 void launch_reset_cell_periodic(double *a, int ncomp, const Geo &g, hipStream_t s) {
     const int m[3] = {g.mx, g.my, g.mz};
@@ -557,16 +562,19 @@ void ref_update_nut(const Geo &g, const tosca_mesh *m, const tosca_params *p, co
     if (ksgs) launch_wrap(ksgs, 1, g, s);
 }
 
-void launch_nut_bcs(double *nut, const tosca_params *p, const Geo &g, hipStream_t s) {
+template <class T>
+void launch_nut_bcs(T *nut, const tosca_params *p, const Geo &g, hipStream_t s) {
     NutBC B;
     for (int q = 0; q < 6; q++) { B.bc[q] = p->bc_nut[q]; B.val[q] = p->nut_value[q]; }
     const int mm[3] = {g.mx, g.my, g.mz};
     for (int axis = 0; axis < 3; axis++) {
         int t1 = (axis == 0) ? 1 : 0, t2 = (axis == 2) ? 1 : 2;
-        TOSCA_LAUNCH("k_nut_bc", k_nut_bc, dim3((mm[t1] + 127) / 128, mm[t2] - 2, 2), dim3(128), 0, s, nut, B, g, axis);
+        TOSCA_LAUNCH("k_nut_bc", k_nut_bc<T>, dim3((mm[t1] + 127) / 128, mm[t2] - 2, 2), dim3(128), 0, s, nut, B, g, axis);
     }
     launch_wrap(nut, 1, g, s);
 }
+template void launch_nut_bcs<double>(double *, const tosca_params *, const Geo &, hipStream_t);
+template void launch_nut_bcs<float>(float *, const tosca_params *, const Geo &, hipStream_t);
 
 void ref_buoyancy(const Geo &g, const tosca_mesh *m, const tosca_params *p, const double *theta, double scale,
                   double *btheta, double *gcont, hipStream_t s) {

@@ -19,14 +19,18 @@ import torch
 from . import abi
 
 
-def _ptr(t: Optional[torch.Tensor]) -> Optional[int]:
+def _ptr(t: Optional[torch.Tensor], dtype=torch.float64) -> Optional[int]:
     if t is None:
         return None
     if not isinstance(t, torch.Tensor):
         raise TypeError("expected a torch tensor")
-    if t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous():
-        raise ValueError("arrays must be contiguous float64 CUDA tensors")
+    if t.dtype != dtype or not t.is_cuda or not t.is_contiguous():
+        raise ValueError(f"arrays must be contiguous {dtype} CUDA tensors")
     return t.data_ptr()
+
+
+def _p32(t: Optional[torch.Tensor]) -> Optional[int]:
+    return _ptr(t, torch.float32)
 
 
 def mesh_struct(mesh: Mapping[str, torch.Tensor]) -> abi.Mesh:
@@ -97,6 +101,14 @@ class RhsContext:
         self._check(self.lib.tosca_rhs_les_step(self.h, C.byref(m), C.byref(p), _ptr(ucont), _ptr(ucat), _ptr(theta),
                                                 _ptr(rhs), float(scale), _ptr(cs), _ptr(nut), _ptr(btheta),
                                                 float(buoy_scale)))
+
+    def rhs_les_step_mixed(self, mesh, params, ucont, ucat, theta, rhs, scale, cs, nut, btheta, buoy_scale=1.0):
+        """Mixed-precision step: ucont, ucat, theta, nut float32; mesh, rhs, cs, btheta float64."""
+        self._stream()
+        m, p = mesh_struct(mesh), abi.make_params(params)
+        self._check(self.lib.tosca_rhs_les_step_mixed(self.h, C.byref(m), C.byref(p), _p32(ucont), _p32(ucat),
+                                                      _p32(theta), _ptr(rhs), float(scale), _ptr(cs), _p32(nut),
+                                                      _ptr(btheta), float(buoy_scale)))
 
     # ---- instrumentation -----------------------------------------------------------------------
     def enable_timing(self, on: bool = True):
