@@ -143,6 +143,51 @@ def mixed_leg(torch, ctx, mesh, case, ucont, ucat, theta, lay, args, fp64_ms):
             "dtype": "f32 flow fields / f64 metrics and reductions", "errors_vs_fp64": err, "kernels_ms": kms}
 
 
+def vjp_leg(torch, ctx, mesh, case, ucont, ucat, theta, cs, lay, args, step_ms):
+    """Adjoint cost: tosca_rhs_vjp against the forward it differentiates (UpdateNut + FormU +
+    Buoyancy with the step's Cs) and against the whole step (which also runs the dynamic model)."""
+    dev = "cuda"
+    gs, gl = lay.gshape, (lay.mz, lay.my, lay.mx, 3)
+    nut = torch.zeros(gs, dtype=torch.float64, device=dev)
+    r = torch.zeros(gl, dtype=torch.float64, device=dev)
+    bt = torch.zeros_like(r)
+    rbar = torch.randn(gl, dtype=torch.float64, device=dev, generator=torch.Generator(device=dev).manual_seed(4))
+    bars = [torch.empty_like(t) for t in (ucont, ucat, theta, cs)]
+
+    def fwd():
+        ctx.UpdateNut(mesh, case.params, ucat, cs, nut)
+        ctx.FormU(mesh, case.params, ucont, ucat, nut, r, 1.0, 0)
+        ctx.Buoyancy(mesh, case.params, theta, 1.0, bt)
+
+    def vjp():
+        ctx.rhs_vjp(mesh, case.params, ucont, ucat, theta, cs, rbar, 1.0, 1.0, *bars)
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(args.steps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / args.steps
+
+    fwd_ms, vjp_ms = timed(fwd), timed(vjp)
+    ctx.enable_timing(True)
+    ctx.reset_timing()
+    for _ in range(args.steps):
+        vjp()
+    tim = ctx.timings()
+    ctx.enable_timing(False)
+    print(f"# vjp: {vjp_ms:.3f} ms vs forward {fwd_ms:.3f} ms (x{vjp_ms / fwd_ms:.2f}), vs step x{vjp_ms / step_ms:.2f}",
+          file=sys.stderr, flush=True)
+    return {"vjp_ms": vjp_ms, "forward_ms": fwd_ms, "vjp_over_forward": vjp_ms / fwd_ms,
+            "vjp_over_step": vjp_ms / step_ms, "kernels_ms": {k: v[0] / v[1] for k, v in tim.items()},
+            "forward": "UpdateNut + FormU + Buoyancy (FP64) with the step's Cs"}
+
+
 def workload_name(cfg: int, case) -> str:
     lay = case.lay
     n = f"{lay.mz - 2}x{lay.mx - 2}x{lay.my - 2}"
@@ -294,6 +339,7 @@ def main():
         "kernels_ms": {k: v[0] for k, v in per.items()},
     }
     line["mixed"] = mixed_leg(torch, ctx, mesh, case, ucont, ucat, theta, lay, args, ms)
+    line["vjp"] = vjp_leg(torch, ctx, mesh, case, ucont, ucat, theta, cs, lay, args, ms)
     if not args.no_cpu:
         spec = C.CONFIGS[args.config]
         line["cpu_baseline"] = cpu_baseline((spec["builder"], spec["kwargs"]), args.cpu_seconds)

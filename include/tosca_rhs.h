@@ -175,6 +175,21 @@ int tosca_rhs_les_step_mixed(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca
                              double *rhs, double scale, double *cs, float *nut,
                              double *btheta, double buoy_scale);
 
+/* Adjoint (vector-Jacobian product) of the right-hand side with respect to velocity, temperature
+ * and Cs (no reference counterpart: the reference has no adjoint).  The function is
+ *   R(ucont, ucat, theta, cs) = FormU(ucont, ucat, nu_t, scale, form_mode 0) + Buoyancy(theta, buoy_scale),
+ *   nu_t = UpdateNut(ucat, cs)  (Cs Delta^2 |S| at interior cells, UpdateNutBCs ghosts),
+ * i.e. what tosca_update_nut + tosca_formu (rhs from zero) + tosca_buoyancy produce, summed.
+ * Every input is a full local array (ghost cells included); R and rbar are global AoS arrays.
+ * Given rbar, writes (overwrites) ucont_bar, ucat_bar (local AoS), theta_bar, cs_bar (local
+ * scalars) = J^T rbar.  Forward intermediates are recomputed, not stored.  The sums use atomic
+ * adds, so repeated calls agree to round-off, not bit for bit.  Where |S| = 0 the subgradient 0
+ * is used; QUICK's upwind switch is differentiated with the branch held. */
+int tosca_rhs_vjp(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_params *params,
+                  const double *ucont, const double *ucat, const double *theta, const double *cs,
+                  const double *rbar, double scale, double buoy_scale,
+                  double *ucont_bar, double *ucat_bar, double *theta_bar, double *cs_bar);
+
 #ifdef __cplusplus
 }
 #endif

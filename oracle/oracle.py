@@ -44,6 +44,7 @@ def load() -> C.CDLL:
         "oracle_update_nut": (I, [In, M, Pa, P, P, P, P]),
         "oracle_buoyancy": (I, [In, M, Pa, P, D, P, P]),
         "oracle_rhs_les_step": (I, [In, M, Pa, P, P, P, P, D, P, P, P, D]),
+        "oracle_rhs_jvp": (I, [In, M, Pa, P, P, P, P, P, P, P, P, D, D, P, P]),
     }
     for n, (r, a) in sigs.items():
         f = getattr(lib, n)
@@ -107,3 +108,33 @@ def rhs_les_step(info, mesh, params, ucont, ucat, theta, rhs, scale, cs, nut, bt
     inf, m, p = abi.make_info(info), _mesh(mesh), abi.make_params(params)
     _check(lib.oracle_rhs_les_step(C.byref(inf), C.byref(m), C.byref(p), _p(ucont), _p(ucat), _p(theta), _p(rhs),
                                    float(scale), _p(cs), _p(nut), _p(btheta), float(buoy_scale)))
+
+
+def rhs_jvp(info, mesh, params, ucont, ucat, theta, cs, d_ucont=None, d_ucat=None, d_theta=None, d_cs=None,
+            scale=1.0, buoy_scale=1.0):
+    """R = FormU(ucont, ucat, UpdateNut(ucat, cs)) + Buoyancy(theta) on the owned faces and its exact
+    tangent J d (oracle/tosca_oracle_tangent.cpp).  Returns (R, dR), arrays of shape (mz, my, mx, 3)."""
+    lib = load()
+    inf, m, p = abi.make_info(info), _mesh(mesh), abi.make_params(params)
+    shape = (info["mz"], info["my"], info["mx"], 3)
+    R, dR = np.zeros(shape), np.zeros(shape)
+    rc = lib.oracle_rhs_jvp(C.byref(inf), C.byref(m), C.byref(p), _p(ucont), _p(ucat), _p(theta), _p(cs),
+                            _p(d_ucont), _p(d_ucat), _p(d_theta), _p(d_cs), float(scale), float(buoy_scale),
+                            _p(R), _p(dR))
+    if rc != 0:
+        raise abi.ToscaError(rc, "oracle_rhs_jvp failed")
+    return R, dR
+
+
+def rhs_forward(info, mesh, params, ucont, ucat, theta, cs, scale=1.0, buoy_scale=1.0):
+    """The function rhs_jvp differentiates, from the double oracle: UpdateNut, FormU onto a zero rhs,
+    Buoyancy, summed."""
+    shape = (info["mz"], info["my"], info["mx"], 3)
+    gs = (info["gzm"], info["gym"], info["gxm"])
+    nut = np.zeros(gs)
+    update_nut(info, mesh, params, ucat, cs, nut, None)
+    rhs = np.zeros(shape)
+    formu(info, mesh, params, ucont, ucat, nut, rhs, scale, 0)
+    bt = np.zeros(shape)
+    buoyancy(info, mesh, params, theta, buoy_scale, bt, None)
+    return rhs + bt

@@ -33,10 +33,24 @@
 
 #include "../include/tosca_rhs.h"
 
+// The FormU / UpdateNut / Buoyancy part is written over a scalar type Real: double here, and a
+// dual number in tosca_oracle_tangent.cpp, which includes this file to obtain the exact FP64
+// tangent (JVP) of the same code.
+#ifndef ORACLE_TANGENT
+using Real = double;
+#define ORACLE_MESH tosca_mesh
+#define ORACLE_API(name) name
+#endif
+
 namespace {
 
-struct V3 { double x, y, z; };
+using std::pow;
+using std::sqrt;
+
+struct V3 { Real x, y, z; };
+#ifndef ORACLE_TANGENT
 static_assert(sizeof(V3) == 24, "Cmpnts layout");
+#endif
 
 thread_local std::string g_err;
 
@@ -147,7 +161,7 @@ void reset_cell_periodic(const Lay &l, T *v) {
 // ---- convective interpolants (src/include/inline.h:4314-4324, 5089-5109, 683) --------------------
 inline V3 centralVec(V3 f0, V3 f1) { return {0.5 * (f0.x + f1.x), 0.5 * (f0.y + f1.y), 0.5 * (f0.z + f1.z)}; }
 
-inline V3 quadraticUpwindVec(V3 f0, V3 f1, V3 f2, V3 f3, double wavespeed) {
+inline V3 quadraticUpwindVec(V3 f0, V3 f1, V3 f2, V3 f3, Real wavespeed) {
     V3 fUU, fU, fD;
     if (wavespeed > 0) { fUU = f0; fU = f1; fD = f2; }
     else               { fUU = f3; fU = f2; fD = f1; }
@@ -159,7 +173,7 @@ inline V3 quadraticUpwindVec(V3 f0, V3 f1, V3 f2, V3 f3, double wavespeed) {
     return r;
 }
 
-inline V3 nScale(double c, V3 v) { return {c * v.x, c * v.y, c * v.z}; }
+inline V3 nScale(Real c, V3 v) { return {c * v.x, c * v.y, c * v.z}; }
 
 // ---- getFace2Cell4Stencil{Csi,Eta,Zet} with nvert = meshTag = 0 (src/include/inline.h:2190-2380)
 inline void face2cell4(int ptype, int64_t f, int64_t m, int64_t *L, int64_t *R) {
@@ -176,7 +190,7 @@ inline void face2cell4(int ptype, int64_t f, int64_t m, int64_t *L, int64_t *R) 
     }
 }
 
-struct Grad { double dudc, dvdc, dwdc, dude, dvde, dwde, dudz, dvdz, dwdz; };
+struct Grad { Real dudc, dvdc, dwdc, dude, dvde, dwde, dudz, dvdz, dwdz; };
 
 // ---- Compute_du_i / _j / _k with nvert = meshTag = 0 (src/include/inline.h:2748-2931) ------------
 inline Grad du_i(const Lay &l, const V3 *u, int64_t i, int64_t j, int64_t k) {
@@ -316,10 +330,10 @@ inline Grad du_center(const Lay &l, const V3 *u, int64_t i, int64_t j, int64_t k
     return g;
 }
 
-struct Cart { double du_dx, du_dy, du_dz, dv_dx, dv_dy, dv_dz, dw_dx, dw_dy, dw_dz; };
+struct Cart { Real du_dx, du_dy, du_dz, dv_dx, dv_dy, dv_dz, dw_dx, dw_dy, dw_dz; };
 
 // ---- Compute_du_dxyz (src/include/inline.h:3635-3663) --------------------------------------------
-inline Cart du_dxyz(V3 c, V3 e, V3 z, double ajc, const Grad &g) {
+inline Cart du_dxyz(V3 c, V3 e, V3 z, Real ajc, const Grad &g) {
     Cart r;
     r.du_dx = (g.dudc * c.x + g.dude * e.x + g.dudz * z.x) * ajc;
     r.du_dy = (g.dudc * c.y + g.dude * e.y + g.dudz * z.y) * ajc;
@@ -334,20 +348,21 @@ inline Cart du_dxyz(V3 c, V3 e, V3 z, double ajc, const Grad &g) {
 }
 
 // |S| = sqrt(2 S:S) (src/les.c:342-354, 2265-2271)
-inline double strain_mag(const Cart &d) {
-    double Sxx = 0.5 * (d.du_dx + d.du_dx), Sxy = 0.5 * (d.du_dy + d.dv_dx), Sxz = 0.5 * (d.du_dz + d.dw_dx);
-    double Syx = Sxy, Syy = 0.5 * (d.dv_dy + d.dv_dy), Syz = 0.5 * (d.dv_dz + d.dw_dy);
-    double Szx = Sxz, Szy = Syz, Szz = 0.5 * (d.dw_dz + d.dw_dz);
-    return std::sqrt(2.0 * (Sxx * Sxx + Sxy * Sxy + Sxz * Sxz + Syx * Syx + Syy * Syy + Syz * Syz +
+inline Real strain_mag(const Cart &d) {
+    Real Sxx = 0.5 * (d.du_dx + d.du_dx), Sxy = 0.5 * (d.du_dy + d.dv_dx), Sxz = 0.5 * (d.du_dz + d.dw_dx);
+    Real Syx = Sxy, Syy = 0.5 * (d.dv_dy + d.dv_dy), Syz = 0.5 * (d.dv_dz + d.dw_dy);
+    Real Szx = Sxz, Szy = Syz, Szz = 0.5 * (d.dw_dz + d.dw_dz);
+    return sqrt(2.0 * (Sxx * Sxx + Sxy * Sxy + Sxz * Sxz + Syx * Syx + Syy * Syy + Syz * Syz +
                             Szx * Szx + Szy * Szy + Szz * Szz));
 }
 
-inline double nMag(V3 v) { return std::sqrt(v.x * v.x + v.y * v.y + v.z * v.z); }
+inline Real nMag(V3 v) { return sqrt(v.x * v.x + v.y * v.y + v.z * v.z); }
 
 // PetscMin / PetscMax
 inline double pmax(double a, double b) { return (a < b) ? b : a; }
 inline double pmin(double a, double b) { return (a < b) ? a : b; }
 
+#ifndef ORACLE_TANGENT
 // ---- integrateTestfilterSimpson / 5x5 (src/include/inline.h:3592-3633) ---------------------------
 double simpson3(const double val[3][3][3], const double w[3][3][3]) {
     double wsum = 0, valsum = 0;
@@ -509,6 +524,8 @@ void test_filter_cell(const Lay &l, const tosca_mesh *m, const V3 *ucat, const V
     germano(L, Sh, SSh, filter, test_filter, csi[c0], eta[c0], zet[c0], num, den);
 }
 
+#endif  // ORACLE_TANGENT
+
 #ifdef _OPENMP
 #define OMP_FOR _Pragma("omp parallel for schedule(static)")
 #else
@@ -517,8 +534,13 @@ void test_filter_cell(const Lay &l, const tosca_mesh *m, const V3 *ucat, const V
 
 }  // namespace
 
+#ifndef ORACLE_TANGENT
 extern "C" {
+#else
+namespace {
+#endif
 
+#ifndef ORACLE_TANGENT
 const char *oracle_last_error(void) { return g_err.c_str(); }
 
 void oracle_set_threads(int n) {
@@ -536,13 +558,14 @@ int oracle_max_threads(void) {
     return 1;
 #endif
 }
+#endif  // ORACLE_TANGENT
 
 // ================================================================================================
 // FormU (src/solvers/ueqn_solvers.c:6-1083)
 // ================================================================================================
-int oracle_formu(const tosca_dmda_info *info, const tosca_mesh *m, const tosca_params *p,
-                 const double *ucont_, const double *ucat_, const double *nut, double *rhs_,
-                 double scale, int formMode) {
+int ORACLE_API(oracle_formu)(const tosca_dmda_info *info, const ORACLE_MESH *m, const tosca_params *p,
+                             const Real *ucont_, const Real *ucat_, const Real *nut, Real *rhs_,
+                             double scale, int formMode) {
     Lay l;
     int rc = check_layout(info, p, l);
     if (rc) return rc;
@@ -554,10 +577,10 @@ int oracle_formu(const tosca_dmda_info *info, const tosca_mesh *m, const tosca_p
     const V3 *icsi = (const V3 *)m->icsi, *ieta = (const V3 *)m->ieta, *izet = (const V3 *)m->izet;
     const V3 *jcsi = (const V3 *)m->jcsi, *jeta = (const V3 *)m->jeta, *jzet = (const V3 *)m->jzet;
     const V3 *kcsi = (const V3 *)m->kcsi, *keta = (const V3 *)m->keta, *kzet = (const V3 *)m->kzet;
-    const double *iaj = m->iaj, *jaj = m->jaj, *kaj = m->kaj;
+    const Real *iaj = m->iaj, *jaj = m->jaj, *kaj = m->kaj;
     V3 *rhs = (V3 *)rhs_;
     const bool les = p->les_active != 0;
-    const double nu = p->nu;
+    const Real nu = p->nu;
     const bool quick = p->div_scheme == TOSCA_DIV_QUICK;
 
     std::vector<V3> fp(n, V3{0, 0, 0});
@@ -569,22 +592,22 @@ int oracle_formu(const tosca_dmda_info *info, const tosca_mesh *m, const tosca_p
     const int64_t tJx = 0, tJy = mz * mx, tJz = 2 * mz * mx;   // plane offsets [x|y|z][zm][xm]
 
     // viscous flux of one face (shared expression of :196-215, :410-429, :624-643, :386-392 ...)
-    auto visc_flux = [](const Grad &g, V3 c, V3 e, V3 z, int dir, double ajc, double nuEff, V3 &out) {
-        double csi0 = c.x, csi1 = c.y, csi2 = c.z, eta0 = e.x, eta1 = e.y, eta2 = e.z, zet0 = z.x, zet1 = z.y, zet2 = z.z;
-        double g11, g21, g31;
+    auto visc_flux = [](const Grad &g, V3 c, V3 e, V3 z, int dir, Real ajc, Real nuEff, V3 &out) {
+        Real csi0 = c.x, csi1 = c.y, csi2 = c.z, eta0 = e.x, eta1 = e.y, eta2 = e.z, zet0 = z.x, zet1 = z.y, zet2 = z.z;
+        Real g11, g21, g31;
         if (dir == 0) { g11 = csi0 * csi0 + csi1 * csi1 + csi2 * csi2; g21 = eta0 * csi0 + eta1 * csi1 + eta2 * csi2; g31 = zet0 * csi0 + zet1 * csi1 + zet2 * csi2; }
         else if (dir == 1) { g11 = csi0 * eta0 + csi1 * eta1 + csi2 * eta2; g21 = eta0 * eta0 + eta1 * eta1 + eta2 * eta2; g31 = zet0 * eta0 + zet1 * eta1 + zet2 * eta2; }
         else { g11 = csi0 * zet0 + csi1 * zet1 + csi2 * zet2; g21 = eta0 * zet0 + eta1 * zet1 + eta2 * zet2; g31 = zet0 * zet0 + zet1 * zet1 + zet2 * zet2; }
-        double r11 = g.dudc * csi0 + g.dude * eta0 + g.dudz * zet0;
-        double r21 = g.dvdc * csi0 + g.dvde * eta0 + g.dvdz * zet0;
-        double r31 = g.dwdc * csi0 + g.dwde * eta0 + g.dwdz * zet0;
-        double r12 = g.dudc * csi1 + g.dude * eta1 + g.dudz * zet1;
-        double r22 = g.dvdc * csi1 + g.dvde * eta1 + g.dvdz * zet1;
-        double r32 = g.dwdc * csi1 + g.dwde * eta1 + g.dwdz * zet1;
-        double r13 = g.dudc * csi2 + g.dude * eta2 + g.dudz * zet2;
-        double r23 = g.dvdc * csi2 + g.dvde * eta2 + g.dvdz * zet2;
-        double r33 = g.dwdc * csi2 + g.dwde * eta2 + g.dwdz * zet2;
-        double n0, n1, n2;   // the face's own normal: csi (i), eta (j) or zet (k)
+        Real r11 = g.dudc * csi0 + g.dude * eta0 + g.dudz * zet0;
+        Real r21 = g.dvdc * csi0 + g.dvde * eta0 + g.dvdz * zet0;
+        Real r31 = g.dwdc * csi0 + g.dwde * eta0 + g.dwdz * zet0;
+        Real r12 = g.dudc * csi1 + g.dude * eta1 + g.dudz * zet1;
+        Real r22 = g.dvdc * csi1 + g.dvde * eta1 + g.dvdz * zet1;
+        Real r32 = g.dwdc * csi1 + g.dwde * eta1 + g.dwdz * zet1;
+        Real r13 = g.dudc * csi2 + g.dude * eta2 + g.dudz * zet2;
+        Real r23 = g.dvdc * csi2 + g.dvde * eta2 + g.dvdz * zet2;
+        Real r33 = g.dwdc * csi2 + g.dwde * eta2 + g.dwdz * zet2;
+        Real n0, n1, n2;   // the face's own normal: csi (i), eta (j) or zet (k)
         if (dir == 0) { n0 = csi0; n1 = csi1; n2 = csi2; }
         else if (dir == 1) { n0 = eta0; n1 = eta1; n2 = eta2; }
         else { n0 = zet0; n1 = zet1; n2 = zet2; }
@@ -603,7 +626,7 @@ int oracle_formu(const tosca_dmda_info *info, const tosca_mesh *m, const tosca_p
                 // i faces
                 if (j != 0 && k != 0) {
                     Grad g{};
-                    double ajc = 0.0;
+                    Real ajc = 0.0;
                     if (formMode != 1) { ajc = iaj[c]; g = du_i(l, ucat, i, j, k); }
                     if (formMode != 2) {
                         if (quick) {
@@ -614,9 +637,9 @@ int oracle_formu(const tosca_dmda_info *info, const tosca_mesh *m, const tosca_p
                         }
                     }
                     if (formMode != 1) {
-                        double nuEff;
+                        Real nuEff;
                         if (les) {
-                            double nt = 0.5 * (nut[c] + nut[l.L(k, j, i + 1)]);
+                            Real nt = 0.5 * (nut[c] + nut[l.L(k, j, i + 1)]);
                             if ((p->bc_u[0] == TOSCA_BCU_WALLFUNCTION && i == 0) || (p->bc_u[1] == TOSCA_BCU_WALLFUNCTION && i == mx - 2)) {
                                 int64_t t = k * my + j;
                                 visc1[c].x = -tauI[tIx + t]; visc1[c].y = -tauI[tIy + t]; visc1[c].z = -tauI[tIz + t];
@@ -635,7 +658,7 @@ int oracle_formu(const tosca_dmda_info *info, const tosca_mesh *m, const tosca_p
                 // j faces
                 if (i != 0 && k != 0) {
                     Grad g{};
-                    double ajc = 0.0;
+                    Real ajc = 0.0;
                     if (formMode != 1) { ajc = jaj[c]; g = du_j(l, ucat, i, j, k); }
                     if (formMode != 2) {
                         if (quick) {
@@ -646,9 +669,9 @@ int oracle_formu(const tosca_dmda_info *info, const tosca_mesh *m, const tosca_p
                         }
                     }
                     if (formMode != 1) {
-                        double nuEff;
+                        Real nuEff;
                         if (les) {
-                            double nt = 0.5 * (nut[c] + nut[l.L(k, j + 1, i)]);
+                            Real nt = 0.5 * (nut[c] + nut[l.L(k, j + 1, i)]);
                             if ((p->bc_u[2] == TOSCA_BCU_WALLFUNCTION && j == 0) || (p->bc_u[3] == TOSCA_BCU_WALLFUNCTION && j == my - 2)) {
                                 int64_t t = k * mx + i;
                                 visc2[c].x = -tauJ[tJx + t]; visc2[c].y = -tauJ[tJy + t]; visc2[c].z = -tauJ[tJz + t];
@@ -667,7 +690,7 @@ int oracle_formu(const tosca_dmda_info *info, const tosca_mesh *m, const tosca_p
                 // k faces
                 if (i != 0 && j != 0) {
                     Grad g{};
-                    double ajc = 0.0;
+                    Real ajc = 0.0;
                     if (formMode != 1) { ajc = kaj[c]; g = du_k(l, ucat, i, j, k); }
                     if (formMode != 2) {
                         if (quick) {
@@ -678,9 +701,9 @@ int oracle_formu(const tosca_dmda_info *info, const tosca_mesh *m, const tosca_p
                         }
                     }
                     if (formMode != 1) {
-                        double nuEff;
+                        Real nuEff;
                         if (les) {
-                            double nt = 0.5 * (nut[c] + nut[l.L(k + 1, j, i)]);
+                            Real nt = 0.5 * (nut[c] + nut[l.L(k + 1, j, i)]);
                             if ((p->bc_u[4] == TOSCA_BCU_SLIP && k == 0) || (p->bc_u[5] == TOSCA_BCU_SLIP && k == mz - 2)) nuEff = 0.0;
                             else nuEff = nu + nt;
                         } else {
@@ -738,7 +761,7 @@ int oracle_formu(const tosca_dmda_info *info, const tosca_mesh *m, const tosca_p
                 const int64_t c = l.L(k, j, i);
                 const int64_t ci = l.L(k, j, i + 1), cj = l.L(k, j + 1, i), ck = l.L(k + 1, j, i);
                 V3 &r = rhs[l.G(k, j, i)];
-                double v1, v2;
+                Real v1, v2;
                 v1 = csi[c].x * fp[c].x + csi[c].y * fp[c].y + csi[c].z * fp[c].z;
                 v2 = csi[ci].x * fp[ci].x + csi[ci].y * fp[ci].y + csi[ci].z * fp[ci].z;
                 r.x += scale * iaj[c] * (0.5 * (v1 + v2));
@@ -752,6 +775,7 @@ int oracle_formu(const tosca_dmda_info *info, const tosca_mesh *m, const tosca_p
     return TOSCA_OK;
 }
 
+#ifndef ORACLE_TANGENT
 // ================================================================================================
 // UpdateCs (src/les.c:165-2115): SMAGORINSKY, DSM, DPASD
 // ================================================================================================
@@ -883,11 +907,13 @@ int oracle_update_cs(const tosca_dmda_info *info, const tosca_mesh *m, const tos
     return TOSCA_OK;
 }
 
+#endif  // ORACLE_TANGENT
+
 // ================================================================================================
 // UpdateNut (src/les.c:2119-2578) + UpdateNutBCs (src/boundary.c:2630-2929)
 // ================================================================================================
-int oracle_update_nut(const tosca_dmda_info *info, const tosca_mesh *m, const tosca_params *p,
-                      const double *ucat_, const double *Cs, double *nut, double *ksgs) {
+int ORACLE_API(oracle_update_nut)(const tosca_dmda_info *info, const ORACLE_MESH *m, const tosca_params *p,
+                                  const Real *ucat_, const Real *Cs, Real *nut, Real *ksgs) {
     Lay l;
     int rc = check_layout(info, p, l);
     if (rc) return rc;
@@ -898,9 +924,9 @@ int oracle_update_nut(const tosca_dmda_info *info, const tosca_mesh *m, const to
     const int64_t mx = l.mx, my = l.my, mz = l.mz, n = l.nloc();
     const V3 *ucat = (const V3 *)ucat_;
     const V3 *csi = (const V3 *)m->csi, *eta = (const V3 *)m->eta, *zet = (const V3 *)m->zet;
-    const double *aj = m->aj;
-    std::vector<double> ksg_local;
-    double *ksg = ksgs;
+    const Real *aj = m->aj;
+    std::vector<Real> ksg_local;
+    Real *ksg = ksgs;
     if (!ksg) { ksg_local.assign(n, 0.0); ksg = ksg_local.data(); }
     for (int64_t c = 0; c < n; c++) { nut[c] = 0.0; ksg[c] = 0.0; }
     OMP_FOR
@@ -910,16 +936,16 @@ int oracle_update_nut(const tosca_dmda_info *info, const tosca_mesh *m, const to
                 const int64_t c = l.L(k, j, i);
                 Grad g = du_center(l, ucat, i, j, k);
                 Cart d = du_dxyz(csi[c], eta[c], zet[c], aj[c], g);
-                double S = strain_mag(d);
-                double filter = std::pow(1. / aj[c], 1. / 3.);
-                nut[c] = Cs[c] * std::pow(filter, 2.0) * S;
-                double Ck = 0.094;
-                double V = 1.0 / aj[c];
-                double A_x = nMag(zet[c]), A_y = nMag(csi[c]), A_z = nMag(eta[c]);
-                double delta_x = V / A_x, delta_y = V / A_y, delta_z = V / A_z;
-                double invDelta2Sum = 1.0 / (delta_x * delta_x) + 1.0 / (delta_y * delta_y) + 1.0 / (delta_z * delta_z);
-                double delta2_harm = 3.0 / invDelta2Sum;
-                ksg[c] = std::pow(nut[c], 2.0) / (std::pow(Ck, 2.0) * delta2_harm);
+                Real S = strain_mag(d);
+                Real filter = pow(1. / aj[c], 1. / 3.);
+                nut[c] = Cs[c] * pow(filter, 2.0) * S;
+                Real Ck = 0.094;
+                Real V = 1.0 / aj[c];
+                Real A_x = nMag(zet[c]), A_y = nMag(csi[c]), A_z = nMag(eta[c]);
+                Real delta_x = V / A_x, delta_y = V / A_y, delta_z = V / A_z;
+                Real invDelta2Sum = 1.0 / (delta_x * delta_x) + 1.0 / (delta_y * delta_y) + 1.0 / (delta_z * delta_z);
+                Real delta2_harm = 3.0 / invDelta2Sum;
+                ksg[c] = pow(nut[c], 2.0) / (pow(Ck, 2.0) * delta2_harm);
             }
     local_to_local(l, nut);
     local_to_local(l, ksg);
@@ -974,8 +1000,8 @@ int oracle_update_nut(const tosca_dmda_info *info, const tosca_mesh *m, const to
 // ================================================================================================
 // Buoyancy (src/sources/ueqn_sources.c:2726-2880)
 // ================================================================================================
-int oracle_buoyancy(const tosca_dmda_info *info, const tosca_mesh *m, const tosca_params *p,
-                    const double *tmprt, double scale, double *btheta_, double *gcont_) {
+int ORACLE_API(oracle_buoyancy)(const tosca_dmda_info *info, const ORACLE_MESH *m, const tosca_params *p,
+                                const Real *tmprt, double scale, Real *btheta_, Real *gcont_) {
     Lay l;
     int rc = check_layout(info, p, l);
     if (rc) return rc;
@@ -986,7 +1012,7 @@ int oracle_buoyancy(const tosca_dmda_info *info, const tosca_mesh *m, const tosc
     V3 *gcont = (V3 *)gcont_;
     if (!gcont) { gloc.assign(mx * my * mz, V3{0, 0, 0}); gcont = gloc.data(); }
     const V3 gravity{0, 0, -9.81};
-    const double tRef = p->t_ref;
+    const Real tRef = p->t_ref;
     for (int64_t k = 0; k < mz - 1; k++)
         for (int64_t j = 0; j < my - 1; j++)
             for (int64_t i = 0; i < mx - 1; i++) {
@@ -1000,9 +1026,9 @@ int oracle_buoyancy(const tosca_dmda_info *info, const tosca_mesh *m, const tosc
         for (int64_t j = 1; j < my - 1; j++)
             for (int64_t i = 1; i < mx - 1; i++) {
                 const int64_t c = l.L(k, j, i), g = l.G(k, j, i);
-                double tempI = 0.5 * (tmprt[c] + tmprt[l.L(k, j, i + 1)]);
-                double tempJ = 0.5 * (tmprt[c] + tmprt[l.L(k, j + 1, i)]);
-                double tempK = 0.5 * (tmprt[c] + tmprt[l.L(k + 1, j, i)]);
+                Real tempI = 0.5 * (tmprt[c] + tmprt[l.L(k, j, i + 1)]);
+                Real tempJ = 0.5 * (tmprt[c] + tmprt[l.L(k, j + 1, i)]);
+                Real tempK = 0.5 * (tmprt[c] + tmprt[l.L(k + 1, j, i)]);
                 btheta[g].x += scale * (gcont[g].x * (tRef - tempI) / tRef);
                 btheta[g].y += scale * (gcont[g].y * (tRef - tempJ) / tRef);
                 btheta[g].z += scale * (gcont[g].z * (tRef - tempK) / tRef);
@@ -1010,6 +1036,7 @@ int oracle_buoyancy(const tosca_dmda_info *info, const tosca_mesh *m, const tosc
     return TOSCA_OK;
 }
 
+#ifndef ORACLE_TANGENT
 // One evaluation of the benchmark's unit of work, in the reference's call order:
 // UpdateCs -> UpdateNut -> Buoyancy -> FormU(formMode 0) (src/main.c:111-112,181,185).
 int oracle_rhs_les_step(const tosca_dmda_info *info, const tosca_mesh *m, const tosca_params *p,
@@ -1024,5 +1051,10 @@ int oracle_rhs_les_step(const tosca_dmda_info *info, const tosca_mesh *m, const 
     if (rc) return rc;
     return oracle_formu(info, m, p, ucont, ucat, nut, rhs, scale, 0);
 }
+#endif  // ORACLE_TANGENT
 
+#ifndef ORACLE_TANGENT
 }  // extern "C"
+#else
+}  // namespace
+#endif

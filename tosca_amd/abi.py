@@ -71,6 +71,7 @@ SIGNATURES = {
     "tosca_buoyancy": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, D, P, P]),
     "tosca_rhs_les_step": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P, P, D, P, P, P, D]),
     "tosca_rhs_les_step_mixed": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P, P, D, P, P, P, D]),
+    "tosca_rhs_vjp": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P, P, P, D, D, P, P, P, P]),
 }
 
 _lib: Optional[C.CDLL] = None

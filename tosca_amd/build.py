@@ -33,6 +33,7 @@ SOURCES = {
     "ref_kernels.hip": ["-ffp-contract=off"],
     "fused_formu.hip": [],
     "fused_les.hip": [],
+    "vjp.hip": [],
 }
 
 

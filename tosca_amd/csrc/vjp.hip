@@ -1,0 +1,371 @@
+// vjp.hip -- hand-written adjoint (vector-Jacobian product) of the momentum right-hand side.
+//
+// Function (no counterpart in the reference; it is the adjoint of the reference's forward path):
+//   R(ucont, ucat, theta, Cs) = FormU(ucont, ucat, nu_t; scale, formMode 0) + Buoyancy(theta; b)
+//   nu_t = UpdateNut(ucat, Cs): Cs Delta^2 |S(ucat)| at interior cells, UpdateNutBCs ghosts
+// with every input a full DMDA local array (ghost cells are inputs too) and R on the owned faces.
+// Given the cotangent Rbar, the kernels accumulate ucont_bar, ucat_bar, theta_bar, Cs_bar, the
+// transpose of each forward stage in reverse order:
+//   k_vjp_fp      face dot (ueqn_solvers.c:1006-1035) -> fpbar per interior cell (a gather: the
+//                 cell's own faces, the faces of its lower neighbours, and the ghost m-1 aliases:
+//                 cell 1 on a periodic axis, the copied cell m-2 otherwise)
+//   k_vjp_face    cell divergence (ueqn_solvers.c:871-993) -> face-flux cotangent, then the
+//                 convective flux (central / QUICK, inline.h:4314-4324, 5089-5109) -> ucont_bar and
+//                 ucat_bar, the viscous flux (ueqn_solvers.c:184-216...) -> its nine gradient
+//                 cotangents -> ucat_bar through the Compute_du_i/j/k stencils (inline.h:2748-2931),
+//                 and nu_eff -> nu_t_bar.  Scatter by atomic adds.
+//   k_vjp_theta   Buoyancy (ueqn_sources.c:2812-2870) -> theta_bar
+//   k_vjp_nut_bc  UpdateNutBCs (boundary.c:2812-2917) transposed: ghost nu_t_bar -> its source cell
+//   k_vjp_nut     nu_t = Cs Delta^2 |S| (les.c:2510-2514) -> Cs_bar, and |S| -> ucat_bar through
+//                 Compute_du_dxyz / Compute_du_center (inline.h:3344-3418, 3635-3663)
+// The forward intermediates the adjoint needs (nu_t, the face gradients, the interpolated face
+// velocities) are recomputed from the inputs instead of being stored by the forward pass.
+#include "fused.h"
+
+namespace tosca {
+
+namespace {
+
+__device__ __forceinline__ void aadd(double *p, double v) { unsafeAtomicAdd(p, v); }
+__device__ __forceinline__ void aadd3(double *p, int c, V3 v) {
+    aadd(p + 3 * c, v.x);
+    aadd(p + 3 * c + 1, v.y);
+    aadd(p + 3 * c + 2, v.z);
+}
+__device__ __forceinline__ V3 sc3(double s, V3 v) { return V3{s * v.x, s * v.y, s * v.z}; }
+__device__ __forceinline__ V3 add3v(V3 a, V3 b) { return V3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ double dot3(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+// w * (p0 + p1 - m0 - m1): adjoint of the four-point differences of Compute_du_*
+__device__ __forceinline__ void scat4(double *ub, int p0, int p1, int m0, int m1, double w, V3 gb) {
+    const V3 a = sc3(w, gb), b = sc3(-w, gb);
+    aadd3(ub, p0, a); aadd3(ub, p1, a); aadd3(ub, m0, b); aadd3(ub, m1, b);
+}
+__device__ __forceinline__ void scat2(double *ub, int p, int m, double w, V3 gb) {
+    aadd3(ub, p, sc3(w, gb)); aadd3(ub, m, sc3(-w, gb));
+}
+
+// Cotangents of the nine curvilinear derivatives, grouped by derivative direction: gc = (dudc,
+// dvdc, dwdc), ge = (dude, ...), gz = (dudz, ...).  Adjoint of Compute_du_i (same branches).
+__device__ void du_i_adj(const Geo &g, double *ub, int i, int j, int k, V3 gc, V3 ge, V3 gz) {
+    const bool iN = g.nonper(0), jN = g.nonper(1), kN = g.nonper(2);
+    scat2(ub, g.L(k, j, i + 1), g.L(k, j, i), 1.0, gc);
+    const bool ib = (i == 0 || i == g.mx - 2);
+    if (j == g.my - 2 && ib && (jN || iN)) scat4(ub, g.L(k, j, i + 1), g.L(k, j, i), g.L(k, j - 1, i + 1), g.L(k, j - 1, i), 0.5, ge);
+    else if (j == 1 && ib && (jN || iN)) scat4(ub, g.L(k, j + 1, i + 1), g.L(k, j + 1, i), g.L(k, j, i + 1), g.L(k, j, i), 0.5, ge);
+    else scat4(ub, g.L(k, j + 1, i + 1), g.L(k, j + 1, i), g.L(k, j - 1, i + 1), g.L(k, j - 1, i), 0.25, ge);
+    if (k == g.mz - 2 && ib && (kN || iN)) scat4(ub, g.L(k, j, i + 1), g.L(k, j, i), g.L(k - 1, j, i + 1), g.L(k - 1, j, i), 0.5, gz);
+    else if (k == 1 && ib && (kN || iN)) scat4(ub, g.L(k + 1, j, i + 1), g.L(k + 1, j, i), g.L(k, j, i + 1), g.L(k, j, i), 0.5, gz);
+    else scat4(ub, g.L(k + 1, j, i + 1), g.L(k + 1, j, i), g.L(k - 1, j, i + 1), g.L(k - 1, j, i), 0.25, gz);
+}
+
+// adjoint of Compute_du_j
+__device__ void du_j_adj(const Geo &g, double *ub, int i, int j, int k, V3 gc, V3 ge, V3 gz) {
+    const bool iN = g.nonper(0), jN = g.nonper(1), kN = g.nonper(2);
+    const bool jb = (j == 0 || j == g.my - 2);
+    if (i == g.mx - 2 && jb && (iN || jN)) scat4(ub, g.L(k, j + 1, i), g.L(k, j, i), g.L(k, j + 1, i - 1), g.L(k, j, i - 1), 0.5, gc);
+    else if (i == 1 && jb && (iN || jN)) scat4(ub, g.L(k, j + 1, i + 1), g.L(k, j, i + 1), g.L(k, j + 1, i), g.L(k, j, i), 0.5, gc);
+    else scat4(ub, g.L(k, j + 1, i + 1), g.L(k, j, i + 1), g.L(k, j + 1, i - 1), g.L(k, j, i - 1), 0.25, gc);
+    scat2(ub, g.L(k, j + 1, i), g.L(k, j, i), 1.0, ge);
+    if (k == g.mz - 2 && jb && (kN || jN)) scat4(ub, g.L(k, j + 1, i), g.L(k, j, i), g.L(k - 1, j + 1, i), g.L(k - 1, j, i), 0.5, gz);
+    else if (k == 1 && jb && (kN || jN)) scat4(ub, g.L(k + 1, j + 1, i), g.L(k + 1, j, i), g.L(k, j + 1, i), g.L(k, j, i), 0.5, gz);
+    else scat4(ub, g.L(k + 1, j + 1, i), g.L(k + 1, j, i), g.L(k - 1, j + 1, i), g.L(k - 1, j, i), 0.25, gz);
+}
+
+// adjoint of Compute_du_k
+__device__ void du_k_adj(const Geo &g, double *ub, int i, int j, int k, V3 gc, V3 ge, V3 gz) {
+    const bool iN = g.nonper(0), jN = g.nonper(1), kN = g.nonper(2);
+    const bool kb = (k == 0 || k == g.mz - 2);
+    if (i == g.mx - 2 && kb && (iN || kN)) scat4(ub, g.L(k + 1, j, i), g.L(k, j, i), g.L(k + 1, j, i - 1), g.L(k, j, i - 1), 0.5, gc);
+    else if (i == 1 && kb && (iN || kN)) scat4(ub, g.L(k + 1, j, i + 1), g.L(k, j, i + 1), g.L(k + 1, j, i), g.L(k, j, i), 0.5, gc);
+    else scat4(ub, g.L(k + 1, j, i + 1), g.L(k, j, i + 1), g.L(k + 1, j, i - 1), g.L(k, j, i - 1), 0.25, gc);
+    if (j == g.my - 2 && kb && (jN || kN)) scat4(ub, g.L(k + 1, j, i), g.L(k, j, i), g.L(k + 1, j - 1, i), g.L(k, j - 1, i), 0.5, ge);
+    else if (j == 1 && kb && (jN || kN)) scat4(ub, g.L(k + 1, j + 1, i), g.L(k, j + 1, i), g.L(k + 1, j, i), g.L(k, j, i), 0.5, ge);
+    else scat4(ub, g.L(k + 1, j + 1, i), g.L(k, j + 1, i), g.L(k + 1, j - 1, i), g.L(k, j - 1, i), 0.25, ge);
+    scat2(ub, g.L(k + 1, j, i), g.L(k, j, i), 1.0, gz);
+}
+
+// adjoint of Compute_du_center at cell (i, j, k)
+__device__ void du_center_adj(const Geo &g, double *ub, int i, int j, int k, V3 gc, V3 ge, V3 gz) {
+    const int c = g.L(k, j, i);
+    if (g.nonper(0) && i == g.mx - 2) scat2(ub, c, g.L(k, j, i - 1), 1.0, gc);
+    else if (g.nonper(0) && i == 1) scat2(ub, g.L(k, j, i + 1), c, 1.0, gc);
+    else scat2(ub, g.L(k, j, i + 1), g.L(k, j, i - 1), 0.5, gc);
+    if (g.nonper(1) && j == g.my - 2) scat2(ub, c, g.L(k, j - 1, i), 1.0, ge);
+    else if (g.nonper(1) && j == 1) scat2(ub, g.L(k, j + 1, i), c, 1.0, ge);
+    else scat2(ub, g.L(k, j + 1, i), g.L(k, j - 1, i), 0.5, ge);
+    if (g.nonper(2) && k == g.mz - 2) scat2(ub, c, g.L(k - 1, j, i), 1.0, gz);
+    else if (g.nonper(2) && k == 1) scat2(ub, g.L(k + 1, j, i), c, 1.0, gz);
+    else scat2(ub, g.L(k + 1, j, i), g.L(k - 1, j, i), 0.5, gz);
+}
+
+struct NutBCv { int bc[6]; };
+
+struct VjpArgs {
+    const double *ucont, *ucat, *theta, *cs, *nut, *rbar;
+    const double *csi, *eta, *zet, *icsi, *ieta, *izet, *jcsi, *jeta, *jzet, *kcsi, *keta, *kzet;
+    const double *aj, *iaj, *jaj, *kaj;
+    const double *tauI, *tauJ;
+    double *fpbar, *nutbar;
+    double *vbar, *ubar, *tbar, *csbar;
+    double scale, bscale, tRef, nu;
+    int quick, les;
+    int bc_u[6];
+};
+
+__device__ __forceinline__ V3 cellS(const VjpArgs &A, int q, int c) {
+    return ld3(q == 0 ? A.csi : (q == 1 ? A.eta : A.zet), c);
+}
+__device__ __forceinline__ double faceA(const VjpArgs &A, int q, int c) {
+    return (q == 0 ? A.iaj : (q == 1 ? A.jaj : A.kaj))[c];
+}
+
+}  // namespace
+
+// fpbar at every interior cell (gather; no atomics)
+__global__ void __launch_bounds__(256) k_vjp_fp(VjpArgs A, Geo g) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y * blockDim.y + threadIdx.y + 1,
+              k = blockIdx.z + 1;
+    if (i > g.mx - 2 || j > g.my - 2 || k > g.mz - 2) return;
+    const int idx[3] = {i, j, k}, m[3] = {g.mx, g.my, g.mz};
+    const int c = g.L(k, j, i);
+    V3 fb{0, 0, 0};
+    for (int q = 0; q < 3; q++) {
+        const int dq[3] = {q == 0, q == 1, q == 2};
+        const V3 Sc = cellS(A, q, c);
+        // face q of this cell (this cell on the left)
+        double w = A.scale * faceA(A, q, c) * 0.5 * A.rbar[3 * g.G(k, j, i) + q];
+        fb = add3v(fb, sc3(w, Sc));
+        // face q of the lower neighbour (this cell on the right)
+        if (idx[q] >= 2) {
+            const int kk = k - dq[2], jj = j - dq[1], ii = i - dq[0];
+            const double w2 = A.scale * faceA(A, q, g.L(kk, jj, ii)) * 0.5 * A.rbar[3 * g.G(kk, jj, ii) + q];
+            fb = add3v(fb, sc3(w2, Sc));
+        }
+        // the ghost cell m-1 to the right of face m-2 carries fp of cell 1 (periodic) or of m-2
+        if (g.per[q] != TOSCA_PERIODIC_NONE && idx[q] == 1) {
+            int t[3] = {i, j, k};
+            t[q] = m[q] - 2;
+            const int cf = g.L(t[2], t[1], t[0]);
+            t[q] = m[q] - 1;
+            const V3 Sg = cellS(A, q, g.L(t[2], t[1], t[0]));
+            t[q] = m[q] - 2;
+            const double w3 = A.scale * faceA(A, q, cf) * 0.5 * A.rbar[3 * g.G(t[2], t[1], t[0]) + q];
+            fb = add3v(fb, sc3(w3, Sg));
+        }
+        if (g.per[q] == TOSCA_PERIODIC_NONE && idx[q] == m[q] - 2) {
+            int t[3] = {i, j, k};
+            t[q] = m[q] - 1;
+            const V3 Sg = cellS(A, q, g.L(t[2], t[1], t[0]));
+            fb = add3v(fb, sc3(w, Sg));
+        }
+    }
+    st3(A.fpbar, c, fb);
+}
+
+// face-flux cotangent of the faces of direction DIR, and everything it feeds
+template <int DIR>
+__global__ void __launch_bounds__(256) k_vjp_face(VjpArgs A, Geo g) {
+    // thread -> (f along DIR, transverse a, b); faces f = 0 .. m-2, transverse interior
+    const int m[3] = {g.mx, g.my, g.mz};
+    const int ta = (DIR == 0) ? 1 : 0, tb = (DIR == 2) ? 1 : 2;
+    const int f = blockIdx.x * blockDim.x + threadIdx.x, a = blockIdx.y + 1, b = blockIdx.z + 1;
+    if (f > m[DIR] - 2 || a > m[ta] - 2 || b > m[tb] - 2) return;
+    const bool per = g.per[DIR] != TOSCA_PERIODIC_NONE;
+    if (per && f == 0) return;                     // face 0 of a periodic axis is face m-2
+    int x[3];
+    x[DIR] = f; x[ta] = a; x[tb] = b;
+    const int i = x[0], j = x[1], k = x[2];
+    const int c = g.L(k, j, i);
+    const int up = (DIR == 0) ? g.L(k, j, i + 1) : (DIR == 1) ? g.L(k, j + 1, i) : g.L(k + 1, j, i);
+    // cotangent of the face flux: + fp of the left cell, - fp of the right cell (interior cells)
+    V3 Fb{0, 0, 0};
+    if (f >= 1) Fb = add3v(Fb, ld3(A.fpbar, c));
+    if (f + 1 <= m[DIR] - 2) Fb = add3v(Fb, sc3(-1.0, ld3(A.fpbar, up)));
+    if (per && f == m[DIR] - 2) {                  // face 0 aliases this face: it is cell 1's left face
+        int t[3] = {i, j, k};
+        t[DIR] = 1;
+        Fb = add3v(Fb, sc3(-1.0, ld3(A.fpbar, g.L(t[2], t[1], t[0]))));
+    }
+    if (Fb.x == 0.0 && Fb.y == 0.0 && Fb.z == 0.0) return;
+    const double *u = A.ucat;
+    // ---- convective: div = -V * u_face ---------------------------------------------------------
+    {
+        const double V = A.ucont[3 * c + DIR];
+        const V3 Vfb = sc3(-V, Fb);
+        if (A.quick) {
+            int L, R;
+            face2cell4(g.per[DIR], f, m[DIR], L, R);
+            int xl[3] = {i, j, k}, xr[3] = {i, j, k};
+            xl[DIR] = L; xr[DIR] = R;
+            const int cLL = g.L(xl[2], xl[1], xl[0]), cRR = g.L(xr[2], xr[1], xr[0]);
+            const V3 f0 = ld3(u, cLL), f1 = ld3(u, c), f2 = ld3(u, up), f3 = ld3(u, cRR);
+            const V3 uf = quick(f0, f1, f2, f3, V);
+            A.vbar[3 * c + DIR] = -dot3(Fb, uf);
+            // d/dfU = 1/2 + 2/8, d/dfD = 1/2 - 1/8, d/dfUU = -1/8
+            if (V > 0) { aadd3(A.ubar, c, sc3(0.75, Vfb)); aadd3(A.ubar, up, sc3(0.375, Vfb)); aadd3(A.ubar, cLL, sc3(-0.125, Vfb)); }
+            else { aadd3(A.ubar, up, sc3(0.75, Vfb)); aadd3(A.ubar, c, sc3(0.375, Vfb)); aadd3(A.ubar, cRR, sc3(-0.125, Vfb)); }
+        } else {
+            const V3 a0 = ld3(u, c), a1 = ld3(u, up);
+            const V3 uf{0.5 * (a0.x + a1.x), 0.5 * (a0.y + a1.y), 0.5 * (a0.z + a1.z)};
+            A.vbar[3 * c + DIR] = -dot3(Fb, uf);
+            aadd3(A.ubar, c, sc3(0.5, Vfb));
+            aadd3(A.ubar, up, sc3(0.5, Vfb));
+        }
+    }
+    // ---- viscous ------------------------------------------------------------------------------------
+    double nuEff = A.nu;
+    bool dep = false;                               // nu_eff depends on nu_t
+    if (A.les) {
+        const int bl = A.bc_u[2 * DIR], br = A.bc_u[2 * DIR + 1];
+        if (DIR < 2 && ((bl == TOSCA_BCU_WALLFUNCTION && f == 0) || (br == TOSCA_BCU_WALLFUNCTION && f == m[DIR] - 2))) {
+            nuEff = A.nu;
+        } else if ((bl == TOSCA_BCU_SLIP && f == 0) || (br == TOSCA_BCU_SLIP && f == m[DIR] - 2)) {
+            nuEff = 0.0;
+        } else {
+            nuEff = A.nu + 0.5 * (A.nut[c] + A.nut[up]);
+            dep = true;
+        }
+    }
+    const double *fcp = (DIR == 0) ? A.icsi : (DIR == 1) ? A.jcsi : A.kcsi;
+    const double *fep = (DIR == 0) ? A.ieta : (DIR == 1) ? A.jeta : A.keta;
+    const double *fzp = (DIR == 0) ? A.izet : (DIR == 1) ? A.jzet : A.kzet;
+    const double *fap = (DIR == 0) ? A.iaj : (DIR == 1) ? A.jaj : A.kaj;
+    const V3 fc = ld3(fcp, c), fe = ld3(fep, c), fz = ld3(fzp, c);
+    const double ajc = fap[c];
+    if (dep) {
+        const Grad gr = (DIR == 0) ? du_i(g, u, i, j, k) : (DIR == 1) ? du_j(g, u, i, j, k) : du_k(g, u, i, j, k);
+        const V3 unit = visc_flux(gr, fc, fe, fz, DIR, ajc, 1.0);
+        const double nb = 0.5 * dot3(Fb, unit);
+        aadd(A.nutbar + c, nb);
+        aadd(A.nutbar + up, nb);
+    }
+    if (nuEff != 0.0) {
+        // visc_m = a nu (g_X1 d_X u_m + (M_X . d_X u) n_m) summed over X = csi, eta, zet
+        const V3 n = (DIR == 0) ? fc : (DIR == 1) ? fe : fz;
+        const double g11 = dot3(fc, n), g21 = dot3(fe, n), g31 = dot3(fz, n);
+        const double an = ajc * nuEff;
+        const V3 Fa = sc3(an, Fb);
+        auto gx = [&](double gX1, V3 MX) {
+            const double d = dot3(MX, Fa);
+            return V3{gX1 * Fa.x + d * n.x, gX1 * Fa.y + d * n.y, gX1 * Fa.z + d * n.z};
+        };
+        const V3 gc = gx(g11, fc), ge = gx(g21, fe), gz = gx(g31, fz);
+        if (DIR == 0) du_i_adj(g, A.ubar, i, j, k, gc, ge, gz);
+        else if (DIR == 1) du_j_adj(g, A.ubar, i, j, k, gc, ge, gz);
+        else du_k_adj(g, A.ubar, i, j, k, gc, ge, gz);
+    }
+}
+
+// theta_bar: the buoyancy source on the three + faces of every interior cell
+__global__ void __launch_bounds__(256) k_vjp_theta(VjpArgs A, Geo g) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y * blockDim.y + threadIdx.y + 1,
+              k = blockIdx.z + 1;
+    if (i > g.mx - 2 || j > g.my - 2 || k > g.mz - 2) return;
+    const int c = g.L(k, j, i), gi = 3 * g.G(k, j, i);
+    const int nb[3] = {g.L(k, j, i + 1), g.L(k, j + 1, i), g.L(k + 1, j, i)};
+    const double gz[3] = {A.icsi[3 * c + 2], A.jeta[3 * c + 2], A.kzet[3 * c + 2]};
+    for (int q = 0; q < 3; q++) {
+        const double w = -A.bscale * (-9.81 * gz[q]) * 0.5 / A.tRef * A.rbar[gi + q];
+        aadd(A.tbar + c, w);
+        aadd(A.tbar + nb[q], w);
+    }
+}
+
+// nu_t ghosts of UpdateNutBCs, transposed: a ghost's cotangent moves to the interior cell it copies
+__global__ void k_vjp_nut_bc(VjpArgs A, NutBCv B, Geo g, int axis) {
+    const int m[3] = {g.mx, g.my, g.mz};
+    const int t1 = (axis == 0) ? 1 : 0, t2 = (axis == 2) ? 1 : 2;
+    const int a1 = blockIdx.x * blockDim.x + threadIdx.x + 1, a2 = blockIdx.y + 1, side = blockIdx.z;
+    if (a1 >= m[t1] - 1 || a2 >= m[t2] - 1) return;
+    const int patch = 2 * axis + side, n = m[axis];
+    int idx[3], src[3];
+    idx[t1] = src[t1] = a1;
+    idx[t2] = src[t2] = a2;
+    idx[axis] = side ? n - 1 : 0;
+    const int bc = B.bc[patch];
+    if (bc == TOSCA_BCNUT_ZEROGRADIENT) src[axis] = side ? n - 2 : 1;
+    else if (bc == TOSCA_BCNUT_PERIODIC && g.per[axis] != TOSCA_PERIODIC_NONE) src[axis] = side ? 1 : n - 2;
+    else return;
+    const double v = A.nutbar[g.L(idx[2], idx[1], idx[0])];
+    if (v != 0.0) aadd(A.nutbar + g.L(src[2], src[1], src[0]), v);
+}
+
+// Cs_bar and the |S| path of nu_t = Cs Delta^2 |S| at every interior cell
+__global__ void __launch_bounds__(256) k_vjp_nut(VjpArgs A, Geo g) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y * blockDim.y + threadIdx.y + 1,
+              k = blockIdx.z + 1;
+    if (i > g.mx - 2 || j > g.my - 2 || k > g.mz - 2) return;
+    const int c = g.L(k, j, i);
+    const double nb = A.nutbar[c];
+    if (nb == 0.0) return;
+    const V3 cc = ld3(A.csi, c), ee = ld3(A.eta, c), zz = ld3(A.zet, c);
+    const double a = A.aj[c];
+    const Cart d = du_dxyz(cc, ee, zz, a, du_center(g, A.ucat, i, j, k));
+    const double S = strain_mag(d);
+    const double filter = pow(1. / a, 1. / 3.), d2 = filter * filter;
+    A.csbar[c] += nb * d2 * S;
+    if (S == 0.0) return;                          // |S| is not differentiable at 0: subgradient 0
+    // d|S| / dD_mn = 2 S_mn / |S|, S = sym(D)
+    const double sb = nb * A.cs[c] * d2 * 2.0 / S;
+    const double Sxx = d.du_dx, Sxy = 0.5 * (d.du_dy + d.dv_dx), Sxz = 0.5 * (d.du_dz + d.dw_dx);
+    const double Syy = d.dv_dy, Syz = 0.5 * (d.dv_dz + d.dw_dy), Szz = d.dw_dz;
+    // Dbar rows (u, v, w) x columns (x, y, z)
+    const V3 Du{sb * Sxx, sb * Sxy, sb * Sxz}, Dv{sb * Sxy, sb * Syy, sb * Syz}, Dw{sb * Sxz, sb * Syz, sb * Szz};
+    // D_m. = aj (d_c u_m csi + d_e u_m eta + d_z u_m zet)  ->  d_X u_m bar = aj (Dbar_m . M_X)
+    const V3 gc{a * dot3(Du, cc), a * dot3(Dv, cc), a * dot3(Dw, cc)};
+    const V3 ge{a * dot3(Du, ee), a * dot3(Dv, ee), a * dot3(Dw, ee)};
+    const V3 gz{a * dot3(Du, zz), a * dot3(Dv, zz), a * dot3(Dw, zz)};
+    du_center_adj(g, A.ubar, i, j, k, gc, ge, gz);
+}
+
+void vjp_rhs(const Geo &g, const tosca_mesh *mesh, const tosca_params *p, const double *ucont, const double *ucat,
+             const double *theta, const double *cs, const double *rbar, double scale, double buoy_scale,
+             double *ucont_bar, double *ucat_bar, double *theta_bar, double *cs_bar, double *ws, hipStream_t s) {
+    const long n = g.nloc();
+    double *nut = ws, *nutbar = ws + n, *fpbar = ws + 2 * n;
+    launch_fill(ucont_bar, 3 * n, 0.0, s);
+    launch_fill(ucat_bar, 3 * n, 0.0, s);
+    launch_fill(theta_bar, n, 0.0, s);
+    launch_fill(cs_bar, n, 0.0, s);
+    launch_fill(nutbar, n, 0.0, s);
+    const bool les = p->les_active != 0;
+    if (les) {   // recompute nu_t as the forward step does
+        fused_nut<double>(g, mesh, ucat, cs, nut, nullptr, s);
+        zero_outside_interior(nut, g, s);
+        nut_bcs(nut, p, g, s);
+    }
+    VjpArgs A;
+    A.ucont = ucont; A.ucat = ucat; A.theta = theta; A.cs = cs; A.nut = les ? nut : ucat; A.rbar = rbar;
+    A.csi = mesh->csi; A.eta = mesh->eta; A.zet = mesh->zet;
+    A.icsi = mesh->icsi; A.ieta = mesh->ieta; A.izet = mesh->izet;
+    A.jcsi = mesh->jcsi; A.jeta = mesh->jeta; A.jzet = mesh->jzet;
+    A.kcsi = mesh->kcsi; A.keta = mesh->keta; A.kzet = mesh->kzet;
+    A.aj = mesh->aj; A.iaj = mesh->iaj; A.jaj = mesh->jaj; A.kaj = mesh->kaj;
+    A.tauI = p->tau_wall_i; A.tauJ = p->tau_wall_j;
+    A.fpbar = fpbar; A.nutbar = nutbar;
+    A.vbar = ucont_bar; A.ubar = ucat_bar; A.tbar = theta_bar; A.csbar = cs_bar;
+    A.scale = scale; A.bscale = buoy_scale; A.tRef = p->t_ref; A.nu = p->nu;
+    A.quick = p->div_scheme == TOSCA_DIV_QUICK;
+    A.les = les;
+    for (int q = 0; q < 6; q++) A.bc_u[q] = p->bc_u[q];
+    const dim3 cb(64, 4), cg((g.mx - 2 + 63) / 64, (g.my - 2 + 3) / 4, g.mz - 2);
+    TOSCA_LAUNCH("k_vjp_fp", k_vjp_fp, cg, cb, 0, s, A, g);
+    TOSCA_LAUNCH("k_vjp_face_i", k_vjp_face<0>, dim3((g.mx - 1 + 255) / 256, g.my - 2, g.mz - 2), dim3(256), 0, s, A, g);
+    TOSCA_LAUNCH("k_vjp_face_j", k_vjp_face<1>, dim3((g.my - 1 + 255) / 256, g.mx - 2, g.mz - 2), dim3(256), 0, s, A, g);
+    TOSCA_LAUNCH("k_vjp_face_k", k_vjp_face<2>, dim3((g.mz - 1 + 255) / 256, g.mx - 2, g.my - 2), dim3(256), 0, s, A, g);
+    TOSCA_LAUNCH("k_vjp_theta", k_vjp_theta, cg, cb, 0, s, A, g);
+    if (les) {
+        NutBCv B;
+        for (int q = 0; q < 6; q++) B.bc[q] = p->bc_nut[q];
+        const int mm[3] = {g.mx, g.my, g.mz};
+        for (int axis = 0; axis < 3; axis++) {
+            const int t1 = (axis == 0) ? 1 : 0, t2 = (axis == 2) ? 1 : 2;
+            TOSCA_LAUNCH("k_vjp_nut_bc", k_vjp_nut_bc, dim3((mm[t1] + 127) / 128, mm[t2] - 2, 2), dim3(128), 0, s, A, B,
+                         g, axis);
+        }
+        TOSCA_LAUNCH("k_vjp_nut", k_vjp_nut, cg, cb, 0, s, A, g);
+    }
+}
+
+}  // namespace tosca

@@ -110,6 +110,16 @@ class RhsContext:
                                                       _p32(theta), _ptr(rhs), float(scale), _ptr(cs), _p32(nut),
                                                       _ptr(btheta), float(buoy_scale)))
 
+    def rhs_vjp(self, mesh, params, ucont, ucat, theta, cs, rbar, scale, buoy_scale, ucont_bar, ucat_bar, theta_bar,
+                cs_bar):
+        """J^T rbar of R = FormU(ucont, ucat, UpdateNut(ucat, cs)) + Buoyancy(theta) into the four
+        *_bar arrays (overwritten)."""
+        self._stream()
+        m, p = mesh_struct(mesh), abi.make_params(params)
+        self._check(self.lib.tosca_rhs_vjp(self.h, C.byref(m), C.byref(p), _ptr(ucont), _ptr(ucat), _ptr(theta),
+                                           _ptr(cs), _ptr(rbar), float(scale), float(buoy_scale), _ptr(ucont_bar),
+                                           _ptr(ucat_bar), _ptr(theta_bar), _ptr(cs_bar)))
+
     # ---- instrumentation -----------------------------------------------------------------------
     def enable_timing(self, on: bool = True):
         self._check(self.lib.tosca_ctx_enable_timing(self.h, int(on)))
