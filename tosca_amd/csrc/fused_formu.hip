@@ -323,6 +323,9 @@ __global__ void __launch_bounds__(64 * NR) k_fused_formu(MainArgs<T> A, Geo g, i
     const bool seamI = colJ && i == g.mx - 1 && iper;       // ghost m-1 carries fp of cell 1
     const bool seamJ = colI && j == g.my - 1 && jper;
     const bool out = cell && lane <= TI && r <= TJ;
+    // faces this thread evaluates: the halo row 0 only supplies the j face of row 1 and the halo
+    // lane 0 only the i face of lane 1; every other thread needs its three faces for fp
+    const bool needF1 = r >= 1, needF2 = lane >= 1, needF3 = r >= 1 && lane >= 1;
     const bool ownI = (i == g.mx - 2) && !iper, ownJ = (j == g.my - 2) && !jper;
     // clamped column for addressing
     const int ia = min(i, g.mx - 1), ja = min(j, g.my - 1);
@@ -381,7 +384,8 @@ __global__ void __launch_bounds__(64 * NR) k_fused_formu(MainArgs<T> A, Geo g, i
         T buf[PRE];
         for (int k = kb - 2; k <= kb + 1; k++) { plane_load(k, buf); plane_store(k, buf); }
         __syncthreads();
-        F3p = kface(kb - 1);
+        if (needF3) F3p = kface(kb - 1);
+        else F3p = FluxT<T>{Vec3<T>{0, 0, 0}, Vec3<T>{0, 0, 0}};
         plane_load(kb + 2, buf);
         __syncthreads();
         plane_store(kb + 2, buf);
@@ -401,28 +405,37 @@ __global__ void __launch_bounds__(64 * NR) k_fused_formu(MainArgs<T> A, Geo g, i
         if (doPre) plane_load(t + 3, pre);
         const bool ghostK = (t == g.mz - 1);
         const long q = (long)min(t, g.mz - 2) * SK;           // faces of plane t (clamped at the ghost plane)
-        const FaceIn<T> in1 = face_in<0>(P, q, 1), in2 = face_in<1>(P, q, SJ), in3 = face_in<2>(P, q, SK);
+        FaceIn<T> in1{}, in2{}, in3{};
+        if (needF1) in1 = face_in<0>(P, q, 1);
+        if (needF2) in2 = face_in<1>(P, q, SJ);
+        if (needF3) in3 = face_in<2>(P, q, SK);
         const T nut0 = P.nut[q];
         // ---- face fluxes ---------------------------------------------------------------------------
         const Vec3<T> z3{0, 0, 0};
         FluxT<T> F1{z3, z3}, F2{z3, z3}, F3{z3, z3};
         if (!ghostK) {
-            F1 = face_pre<0, NR, T>(A, in1, nut0, ring, rrow, rcol, t);
-            F2 = face_pre<1, NR, T>(A, in2, nut0, ring, rrow, rcol, t);
-            if (t == g.mz - 2) F3 = ldflux<T>(pF3 + (long)nK * 6);
-            else F3 = face_pre<2, NR, T>(A, in3, nut0, ring, rrow, rcol, t);
-            if (bndI) F1 = ldflux<T>(pF1 + (long)t * g.my * 6);
-            if (bndJ) F2 = ldflux<T>(pF2 + (long)t * g.mx * 6);
+            if (needF1) F1 = bndI ? ldflux<T>(pF1 + (long)t * g.my * 6) : face_pre<0, NR, T>(A, in1, nut0, ring, rrow, rcol, t);
+            if (needF2) F2 = bndJ ? ldflux<T>(pF2 + (long)t * g.mx * 6) : face_pre<1, NR, T>(A, in2, nut0, ring, rrow, rcol, t);
+            if (needF3) {
+                if (t == g.mz - 2) F3 = ldflux<T>(pF3 + (long)nK * 6);
+                else F3 = face_pre<2, NR, T>(A, in3, nut0, ring, rrow, rcol, t);
+            }
         }
-        // ---- loads of the Rhs / buoyancy step (cells of plane t-1), in flight across the exchange
+        // ---- loads of the Rhs / buoyancy step (cells of plane t-1), in flight across the exchange;
+        // output cells only
         const long qp = (long)(t - 1) * SK;
-        const V3 csA = ld3(pcsi + 3 * qp, 0), csB = ld3(pcsi + 3 * qp, 1);
-        const V3 etA = ld3(peta + 3 * qp, 0), etB = ld3(peta + 3 * qp, SJ);
-        const V3 zeA = ld3(pzet + 3 * qp, 0), zeB = ld3(pzet + 3 * qp, SK);
+        const bool rhsStep = out && t > kb;
+        V3 csA{0, 0, 0}, csB{0, 0, 0}, etA{0, 0, 0}, etB{0, 0, 0}, zeA{0, 0, 0}, zeB{0, 0, 0};
         const long ro = 3 * (long)(t - 1) * SG;
-        const double r0 = prhs[ro], r1 = prhs[ro + 1], r2 = prhs[ro + 2];
+        double r0 = 0, r1 = 0, r2 = 0;
         double th = 0, thI = 0, thJ = 0, thK = 0;
-        if (buoy) { th = ptheta[qp]; thI = ptheta[qp + 1]; thJ = ptheta[qp + SJ]; thK = ptheta[qp + SK]; }
+        if (rhsStep) {
+            csA = ld3(pcsi + 3 * qp, 0); csB = ld3(pcsi + 3 * qp, 1);
+            etA = ld3(peta + 3 * qp, 0); etB = ld3(peta + 3 * qp, SJ);
+            zeA = ld3(pzet + 3 * qp, 0); zeB = ld3(pzet + 3 * qp, SK);
+            r0 = prhs[ro]; r1 = prhs[ro + 1]; r2 = prhs[ro + 2];
+            if (buoy) { th = ptheta[qp]; thI = ptheta[qp + 1]; thJ = ptheta[qp + SJ]; thK = ptheta[qp + SK]; }
+        }
         const Vec3<T> F1dm = shfl_up3(F1.d), F1vm = shfl_up3(F1.v);
         sF2[0][r][lane] = F2.d.x; sF2[1][r][lane] = F2.d.y; sF2[2][r][lane] = F2.d.z;
         sF2[3][r][lane] = F2.v.x; sF2[4][r][lane] = F2.v.y; sF2[5][r][lane] = F2.v.z;
@@ -450,7 +463,7 @@ __global__ void __launch_bounds__(64 * NR) k_fused_formu(MainArgs<T> A, Geo g, i
         V3 fJ{sFp[buf ^ 1][0][rp][lane], sFp[buf ^ 1][1][rp][lane], sFp[buf ^ 1][2][rp][lane]};
         if (ownI) fpI = fpPrev;
         if (ownJ) fJ = fpPrev;
-        if (t > kb && out) {
+        if (rhsStep) {
             const V3 fK = fp;
             double v1 = csA.x * fpPrev.x + csA.y * fpPrev.y + csA.z * fpPrev.z;
             double v2 = csB.x * fpI.x + csB.y * fpI.y + csB.z * fpI.z;
