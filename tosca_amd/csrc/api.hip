@@ -549,7 +549,8 @@ int tosca_rhs_vjp(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, cons
         (rc = check_dev(c, ucat_bar, v, "ucat_bar")) || (rc = check_dev(c, theta_bar, n * sizeof(double), "theta_bar")) ||
         (rc = check_dev(c, cs_bar, n * sizeof(double), "cs_bar")))
         return rc;
-    if ((rc = ensure_ws(c, ws_need(c)))) return rc;
+    // nu_t, fpbar, the nu_eff cotangents and the ucat-cotangent packets of the faces and cells
+    if ((rc = ensure_ws(c, std::max(ws_need(c), 52 * n + 64)))) return rc;
     tosca::vjp_rhs(g, m, p, ucont, ucat, theta, cs, rbar, scale, buoy_scale, ucont_bar, ucat_bar, theta_bar, cs_bar,
                    c->ws, c->stream);
     return launched(c, "rhs_vjp");

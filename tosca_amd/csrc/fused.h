@@ -60,7 +60,7 @@ template <class T>
 void nut_bcs(T *nut, const tosca_params *p, const Geo &g, hipStream_t s);
 
 // Adjoint of R = FormU(ucont, ucat, UpdateNut(ucat, cs)) + Buoyancy(theta): accumulates the
-// cotangents of the four inputs (overwritten) from rbar; ws holds >= 5 * nloc doubles.
+// cotangents of the four inputs (overwritten) from rbar; ws holds >= 52 * nloc doubles.
 void vjp_rhs(const Geo &g, const tosca_mesh *mesh, const tosca_params *p, const double *ucont, const double *ucat,
              const double *theta, const double *cs, const double *rbar, double scale, double buoy_scale,
              double *ucont_bar, double *ucat_bar, double *theta_bar, double *cs_bar, double *ws, hipStream_t s);

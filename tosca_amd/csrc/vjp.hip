@@ -11,13 +11,14 @@
 //                 cell 1 on a periodic axis, the copied cell m-2 otherwise)
 //   k_vjp_face    cell divergence (ueqn_solvers.c:871-993) -> face-flux cotangent, then the
 //                 convective flux (central / QUICK, inline.h:4314-4324, 5089-5109) -> ucont_bar and
-//                 ucat_bar, the viscous flux (ueqn_solvers.c:184-216...) -> its nine gradient
-//                 cotangents -> ucat_bar through the Compute_du_i/j/k stencils (inline.h:2748-2931),
-//                 and nu_eff -> nu_t_bar.  Scatter by atomic adds.
-//   k_vjp_theta   Buoyancy (ueqn_sources.c:2812-2870) -> theta_bar
-//   k_vjp_nut_bc  UpdateNutBCs (boundary.c:2812-2917) transposed: ghost nu_t_bar -> its source cell
-//   k_vjp_nut     nu_t = Cs Delta^2 |S| (les.c:2510-2514) -> Cs_bar, and |S| -> ucat_bar through
-//                 Compute_du_dxyz / Compute_du_center (inline.h:3344-3418, 3635-3663)
+//                 a packet of ucat cotangents, the viscous flux (ueqn_solvers.c:184-216...) -> the
+//                 cotangents of its nine derivatives (Compute_du_i/j/k, inline.h:2748-2931) into the
+//                 same packet, and the nu_eff cotangent.  Only the boundary faces scatter (atomics).
+//   k_vjp_nut     nu_t = Cs Delta^2 |S| (les.c:2510-2514), with the UpdateNutBCs ghosts
+//                 (boundary.c:2812-2917) transposed -> Cs_bar, and a packet for the |S| path
+//                 (Compute_du_dxyz / Compute_du_center, inline.h:3344-3418, 3635-3663)
+//   k_vjp_gather  per owned point: ucat_bar from the packets of the faces and cells whose regular
+//                 stencils reach it, and theta_bar from Buoyancy (ueqn_sources.c:2812-2870)
 // The forward intermediates the adjoint needs (nu_t, the face gradients, the interpolated face
 // velocities) are recomputed from the inputs instead of being stored by the forward pass.
 #include "fused.h"
@@ -108,9 +109,13 @@ struct VjpArgs {
     const double *tauI, *tauJ;
     double *fpbar, *nutbar;
     double *vbar, *ubar, *tbar, *csbar;
+    double *pk[3];      // per face direction: 12 SoA planes of n doubles (cv, gn, ga, gb)
+    double *nueb[3];    // per face direction: cotangent of nu_eff at each face (n doubles)
+    double *spk;        // per interior cell: 9 SoA planes (cotangents of the du_center derivatives)
+    long n;
     double scale, bscale, tRef, nu;
     int quick, les;
-    int bc_u[6];
+    int bc_u[6], bc_nut[6];
 };
 
 __device__ __forceinline__ V3 cellS(const VjpArgs &A, int q, int c) {
@@ -163,21 +168,33 @@ __global__ void __launch_bounds__(256) k_vjp_fp(VjpArgs A, Geo g) {
     st3(A.fpbar, c, fb);
 }
 
-// face-flux cotangent of the faces of direction DIR, and everything it feeds
+// ------------------------------------------------------------------------------------------------
+// Faces: the flux cotangent, ucont_bar, the nu_eff cotangent, and the ucat cotangent packets.
+// Interior faces (1 <= f <= m-3 along the face direction) store a packet -- the convective
+// cotangent cv = -V Fbar and the cotangents of the normal and two transverse derivatives -- that
+// k_vjp_gather later folds into ucat_bar with the regular stencils; the two boundary faces per line
+// (the only ones with the one-sided and clipped stencils) scatter into ucat_bar directly.
+// Thread x runs along i for every face direction (coalesced).
+// ------------------------------------------------------------------------------------------------
 template <int DIR>
 __global__ void __launch_bounds__(256) k_vjp_face(VjpArgs A, Geo g) {
-    // thread -> (f along DIR, transverse a, b); faces f = 0 .. m-2, transverse interior
     const int m[3] = {g.mx, g.my, g.mz};
-    const int ta = (DIR == 0) ? 1 : 0, tb = (DIR == 2) ? 1 : 2;
-    const int f = blockIdx.x * blockDim.x + threadIdx.x, a = blockIdx.y + 1, b = blockIdx.z + 1;
-    if (f > m[DIR] - 2 || a > m[ta] - 2 || b > m[tb] - 2) return;
+    int x[3];
+    x[0] = blockIdx.x * blockDim.x + threadIdx.x;
+    x[1] = blockIdx.y;
+    x[2] = blockIdx.z;
+    // faces f = 0 .. m-2 along DIR (thread index as is), transverse indices 1 .. m-2 (index + 1)
+    for (int d = 0; d < 3; d++) {
+        if (d != DIR) x[d] += 1;
+        if (x[d] > m[d] - 2) return;
+    }
+    const int f = x[DIR];
     const bool per = g.per[DIR] != TOSCA_PERIODIC_NONE;
     if (per && f == 0) return;                     // face 0 of a periodic axis is face m-2
-    int x[3];
-    x[DIR] = f; x[ta] = a; x[tb] = b;
     const int i = x[0], j = x[1], k = x[2];
     const int c = g.L(k, j, i);
     const int up = (DIR == 0) ? g.L(k, j, i + 1) : (DIR == 1) ? g.L(k, j + 1, i) : g.L(k + 1, j, i);
+    const bool interior = f >= 1 && f <= m[DIR] - 3;
     // cotangent of the face flux: + fp of the left cell, - fp of the right cell (interior cells)
     V3 Fb{0, 0, 0};
     if (f >= 1) Fb = add3v(Fb, ld3(A.fpbar, c));
@@ -187,31 +204,28 @@ __global__ void __launch_bounds__(256) k_vjp_face(VjpArgs A, Geo g) {
         t[DIR] = 1;
         Fb = add3v(Fb, sc3(-1.0, ld3(A.fpbar, g.L(t[2], t[1], t[0]))));
     }
-    if (Fb.x == 0.0 && Fb.y == 0.0 && Fb.z == 0.0) return;
     const double *u = A.ucat;
     // ---- convective: div = -V * u_face ---------------------------------------------------------
-    {
-        const double V = A.ucont[3 * c + DIR];
-        const V3 Vfb = sc3(-V, Fb);
-        if (A.quick) {
-            int L, R;
-            face2cell4(g.per[DIR], f, m[DIR], L, R);
-            int xl[3] = {i, j, k}, xr[3] = {i, j, k};
-            xl[DIR] = L; xr[DIR] = R;
-            const int cLL = g.L(xl[2], xl[1], xl[0]), cRR = g.L(xr[2], xr[1], xr[0]);
-            const V3 f0 = ld3(u, cLL), f1 = ld3(u, c), f2 = ld3(u, up), f3 = ld3(u, cRR);
-            const V3 uf = quick(f0, f1, f2, f3, V);
-            A.vbar[3 * c + DIR] = -dot3(Fb, uf);
+    const double V = A.ucont[3 * c + DIR];
+    const V3 cv = sc3(-V, Fb);
+    if (A.quick) {
+        int L, R;
+        face2cell4(g.per[DIR], f, m[DIR], L, R);
+        int xl[3] = {i, j, k}, xr[3] = {i, j, k};
+        xl[DIR] = L; xr[DIR] = R;
+        const int cLL = g.L(xl[2], xl[1], xl[0]), cRR = g.L(xr[2], xr[1], xr[0]);
+        const V3 uf = quick(ld3(u, cLL), ld3(u, c), ld3(u, up), ld3(u, cRR), V);
+        A.vbar[3 * c + DIR] = -dot3(Fb, uf);
+        if (!interior) {
             // d/dfU = 1/2 + 2/8, d/dfD = 1/2 - 1/8, d/dfUU = -1/8
-            if (V > 0) { aadd3(A.ubar, c, sc3(0.75, Vfb)); aadd3(A.ubar, up, sc3(0.375, Vfb)); aadd3(A.ubar, cLL, sc3(-0.125, Vfb)); }
-            else { aadd3(A.ubar, up, sc3(0.75, Vfb)); aadd3(A.ubar, c, sc3(0.375, Vfb)); aadd3(A.ubar, cRR, sc3(-0.125, Vfb)); }
-        } else {
-            const V3 a0 = ld3(u, c), a1 = ld3(u, up);
-            const V3 uf{0.5 * (a0.x + a1.x), 0.5 * (a0.y + a1.y), 0.5 * (a0.z + a1.z)};
-            A.vbar[3 * c + DIR] = -dot3(Fb, uf);
-            aadd3(A.ubar, c, sc3(0.5, Vfb));
-            aadd3(A.ubar, up, sc3(0.5, Vfb));
+            if (V > 0) { aadd3(A.ubar, c, sc3(0.75, cv)); aadd3(A.ubar, up, sc3(0.375, cv)); aadd3(A.ubar, cLL, sc3(-0.125, cv)); }
+            else { aadd3(A.ubar, up, sc3(0.75, cv)); aadd3(A.ubar, c, sc3(0.375, cv)); aadd3(A.ubar, cRR, sc3(-0.125, cv)); }
         }
+    } else {
+        const V3 a0 = ld3(u, c), a1 = ld3(u, up);
+        const V3 uf{0.5 * (a0.x + a1.x), 0.5 * (a0.y + a1.y), 0.5 * (a0.z + a1.z)};
+        A.vbar[3 * c + DIR] = -dot3(Fb, uf);
+        if (!interior) { aadd3(A.ubar, c, sc3(0.5, cv)); aadd3(A.ubar, up, sc3(0.5, cv)); }
     }
     // ---- viscous ------------------------------------------------------------------------------------
     double nuEff = A.nu;
@@ -233,102 +247,200 @@ __global__ void __launch_bounds__(256) k_vjp_face(VjpArgs A, Geo g) {
     const double *fap = (DIR == 0) ? A.iaj : (DIR == 1) ? A.jaj : A.kaj;
     const V3 fc = ld3(fcp, c), fe = ld3(fep, c), fz = ld3(fzp, c);
     const double ajc = fap[c];
+    double nb = 0.0;
     if (dep) {
         const Grad gr = (DIR == 0) ? du_i(g, u, i, j, k) : (DIR == 1) ? du_j(g, u, i, j, k) : du_k(g, u, i, j, k);
-        const V3 unit = visc_flux(gr, fc, fe, fz, DIR, ajc, 1.0);
-        const double nb = 0.5 * dot3(Fb, unit);
-        aadd(A.nutbar + c, nb);
-        aadd(A.nutbar + up, nb);
+        nb = dot3(Fb, visc_flux(gr, fc, fe, fz, DIR, ajc, 1.0));
     }
-    if (nuEff != 0.0) {
-        // visc_m = a nu (g_X1 d_X u_m + (M_X . d_X u) n_m) summed over X = csi, eta, zet
-        const V3 n = (DIR == 0) ? fc : (DIR == 1) ? fe : fz;
-        const double g11 = dot3(fc, n), g21 = dot3(fe, n), g31 = dot3(fz, n);
-        const double an = ajc * nuEff;
-        const V3 Fa = sc3(an, Fb);
-        auto gx = [&](double gX1, V3 MX) {
-            const double d = dot3(MX, Fa);
-            return V3{gX1 * Fa.x + d * n.x, gX1 * Fa.y + d * n.y, gX1 * Fa.z + d * n.z};
-        };
-        const V3 gc = gx(g11, fc), ge = gx(g21, fe), gz = gx(g31, fz);
+    A.nueb[DIR][c] = nb;
+    // visc_m = a nu (g_X1 d_X u_m + (M_X . d_X u) n_m) summed over X = csi, eta, zet
+    const V3 nrm = (DIR == 0) ? fc : (DIR == 1) ? fe : fz;
+    const double g11 = dot3(fc, nrm), g21 = dot3(fe, nrm), g31 = dot3(fz, nrm);
+    const V3 Fa = sc3(ajc * nuEff, Fb);
+    auto gx = [&](double gX1, V3 MX) {
+        const double d = dot3(MX, Fa);
+        return V3{gX1 * Fa.x + d * nrm.x, gX1 * Fa.y + d * nrm.y, gX1 * Fa.z + d * nrm.z};
+    };
+    const V3 gc = gx(g11, fc), ge = gx(g21, fe), gz = gx(g31, fz);
+    if (interior) {
+        // packet: cv, normal, transverse a, transverse b (a/b: the lower / higher other axis)
+        const V3 gn = (DIR == 0) ? gc : (DIR == 1) ? ge : gz;
+        const V3 ga = (DIR == 0) ? ge : gc;
+        const V3 gb = (DIR == 2) ? ge : gz;
+        const V3 pv[4] = {cv, gn, ga, gb};
+        double *pk = A.pk[DIR];
+        for (int w = 0; w < 4; w++) {
+            pk[(3 * w + 0) * A.n + c] = pv[w].x;
+            pk[(3 * w + 1) * A.n + c] = pv[w].y;
+            pk[(3 * w + 2) * A.n + c] = pv[w].z;
+        }
+    } else if (nuEff != 0.0) {
         if (DIR == 0) du_i_adj(g, A.ubar, i, j, k, gc, ge, gz);
         else if (DIR == 1) du_j_adj(g, A.ubar, i, j, k, gc, ge, gz);
         else du_k_adj(g, A.ubar, i, j, k, gc, ge, gz);
     }
 }
 
-// theta_bar: the buoyancy source on the three + faces of every interior cell
-__global__ void __launch_bounds__(256) k_vjp_theta(VjpArgs A, Geo g) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y * blockDim.y + threadIdx.y + 1,
-              k = blockIdx.z + 1;
-    if (i > g.mx - 2 || j > g.my - 2 || k > g.mz - 2) return;
-    const int c = g.L(k, j, i), gi = 3 * g.G(k, j, i);
-    const int nb[3] = {g.L(k, j, i + 1), g.L(k, j + 1, i), g.L(k + 1, j, i)};
-    const double gz[3] = {A.icsi[3 * c + 2], A.jeta[3 * c + 2], A.kzet[3 * c + 2]};
-    for (int q = 0; q < 3; q++) {
-        const double w = -A.bscale * (-9.81 * gz[q]) * 0.5 / A.tRef * A.rbar[gi + q];
-        aadd(A.tbar + c, w);
-        aadd(A.tbar + nb[q], w);
-    }
-}
-
-// nu_t ghosts of UpdateNutBCs, transposed: a ghost's cotangent moves to the interior cell it copies
-__global__ void k_vjp_nut_bc(VjpArgs A, NutBCv B, Geo g, int axis) {
-    const int m[3] = {g.mx, g.my, g.mz};
-    const int t1 = (axis == 0) ? 1 : 0, t2 = (axis == 2) ? 1 : 2;
-    const int a1 = blockIdx.x * blockDim.x + threadIdx.x + 1, a2 = blockIdx.y + 1, side = blockIdx.z;
-    if (a1 >= m[t1] - 1 || a2 >= m[t2] - 1) return;
-    const int patch = 2 * axis + side, n = m[axis];
-    int idx[3], src[3];
-    idx[t1] = src[t1] = a1;
-    idx[t2] = src[t2] = a2;
-    idx[axis] = side ? n - 1 : 0;
-    const int bc = B.bc[patch];
-    if (bc == TOSCA_BCNUT_ZEROGRADIENT) src[axis] = side ? n - 2 : 1;
-    else if (bc == TOSCA_BCNUT_PERIODIC && g.per[axis] != TOSCA_PERIODIC_NONE) src[axis] = side ? 1 : n - 2;
-    else return;
-    const double v = A.nutbar[g.L(idx[2], idx[1], idx[0])];
-    if (v != 0.0) aadd(A.nutbar + g.L(src[2], src[1], src[0]), v);
-}
-
-// Cs_bar and the |S| path of nu_t = Cs Delta^2 |S| at every interior cell
+// ------------------------------------------------------------------------------------------------
+// Interior cells: nu_t_bar (gathered from the nu_eff cotangents of the cell's six faces, plus the
+// ghosts UpdateNutBCs fills from this cell), Cs_bar, and the packet of the |S| path.
+// ------------------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_vjp_nut(VjpArgs A, Geo g) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y * blockDim.y + threadIdx.y + 1,
               k = blockIdx.z + 1;
     if (i > g.mx - 2 || j > g.my - 2 || k > g.mz - 2) return;
     const int c = g.L(k, j, i);
-    const double nb = A.nutbar[c];
-    if (nb == 0.0) return;
+    const int x[3] = {i, j, k}, m[3] = {g.mx, g.my, g.mz};
+    double nb = 0.0;
+    for (int q = 0; q < 3; q++) {
+        int t[3] = {i, j, k};
+        nb += 0.5 * A.nueb[q][c];                                      // face q of the cell
+        t[q] = x[q] - 1;
+        if (!(x[q] == 1 && g.per[q] != TOSCA_PERIODIC_NONE))          // periodic face 0 is not evaluated
+            nb += 0.5 * A.nueb[q][g.L(t[2], t[1], t[0])];
+        // ghosts filled from this cell: cotangent of nu_t there is 1/2 nu_eff_bar of the boundary face
+        const int bl = A.bc_nut[2 * q], br = A.bc_nut[2 * q + 1];
+        const bool pq = g.per[q] != TOSCA_PERIODIC_NONE;
+        t[q] = 0;                                                      // face 0 (its left cell is ghost 0)
+        const int f0 = g.L(t[2], t[1], t[0]);
+        t[q] = m[q] - 2;                                               // face m-2 (right cell ghost m-1)
+        const int fm = g.L(t[2], t[1], t[0]);
+        if (bl == TOSCA_BCNUT_ZEROGRADIENT && x[q] == 1) nb += 0.5 * A.nueb[q][f0];
+        if (bl == TOSCA_BCNUT_PERIODIC && pq && x[q] == m[q] - 2) nb += 0.5 * A.nueb[q][f0];
+        if (br == TOSCA_BCNUT_ZEROGRADIENT && x[q] == m[q] - 2) nb += 0.5 * A.nueb[q][fm];
+        if (br == TOSCA_BCNUT_PERIODIC && pq && x[q] == 1) nb += 0.5 * A.nueb[q][fm];
+    }
     const V3 cc = ld3(A.csi, c), ee = ld3(A.eta, c), zz = ld3(A.zet, c);
     const double a = A.aj[c];
     const Cart d = du_dxyz(cc, ee, zz, a, du_center(g, A.ucat, i, j, k));
     const double S = strain_mag(d);
     const double filter = pow(1. / a, 1. / 3.), d2 = filter * filter;
-    A.csbar[c] += nb * d2 * S;
-    if (S == 0.0) return;                          // |S| is not differentiable at 0: subgradient 0
-    // d|S| / dD_mn = 2 S_mn / |S|, S = sym(D)
-    const double sb = nb * A.cs[c] * d2 * 2.0 / S;
-    const double Sxx = d.du_dx, Sxy = 0.5 * (d.du_dy + d.dv_dx), Sxz = 0.5 * (d.du_dz + d.dw_dx);
-    const double Syy = d.dv_dy, Syz = 0.5 * (d.dv_dz + d.dw_dy), Szz = d.dw_dz;
-    // Dbar rows (u, v, w) x columns (x, y, z)
-    const V3 Du{sb * Sxx, sb * Sxy, sb * Sxz}, Dv{sb * Sxy, sb * Syy, sb * Syz}, Dw{sb * Sxz, sb * Syz, sb * Szz};
-    // D_m. = aj (d_c u_m csi + d_e u_m eta + d_z u_m zet)  ->  d_X u_m bar = aj (Dbar_m . M_X)
-    const V3 gc{a * dot3(Du, cc), a * dot3(Dv, cc), a * dot3(Dw, cc)};
-    const V3 ge{a * dot3(Du, ee), a * dot3(Dv, ee), a * dot3(Dw, ee)};
-    const V3 gz{a * dot3(Du, zz), a * dot3(Dv, zz), a * dot3(Dw, zz)};
-    du_center_adj(g, A.ubar, i, j, k, gc, ge, gz);
+    A.csbar[c] = nb * d2 * S;
+    V3 gc{0, 0, 0}, ge{0, 0, 0}, gz{0, 0, 0};
+    if (S != 0.0 && nb != 0.0) {                   // |S| is not differentiable at 0: subgradient 0
+        // d|S| / dD_mn = 2 S_mn / |S|, S = sym(D)
+        const double sb = nb * A.cs[c] * d2 * 2.0 / S;
+        const double Sxx = d.du_dx, Sxy = 0.5 * (d.du_dy + d.dv_dx), Sxz = 0.5 * (d.du_dz + d.dw_dx);
+        const double Syy = d.dv_dy, Syz = 0.5 * (d.dv_dz + d.dw_dy), Szz = d.dw_dz;
+        const V3 Du{sb * Sxx, sb * Sxy, sb * Sxz}, Dv{sb * Sxy, sb * Syy, sb * Syz}, Dw{sb * Sxz, sb * Syz, sb * Szz};
+        // D_m. = aj (d_c u_m csi + d_e u_m eta + d_z u_m zet)  ->  d_X u_m bar = aj (Dbar_m . M_X)
+        gc = V3{a * dot3(Du, cc), a * dot3(Dv, cc), a * dot3(Dw, cc)};
+        ge = V3{a * dot3(Du, ee), a * dot3(Dv, ee), a * dot3(Dw, ee)};
+        gz = V3{a * dot3(Du, zz), a * dot3(Dv, zz), a * dot3(Dw, zz)};
+    }
+    const V3 pv[3] = {gc, ge, gz};
+    for (int w = 0; w < 3; w++) {
+        A.spk[(3 * w + 0) * A.n + c] = pv[w].x;
+        A.spk[(3 * w + 1) * A.n + c] = pv[w].y;
+        A.spk[(3 * w + 2) * A.n + c] = pv[w].z;
+    }
+}
+
+namespace {
+__device__ __forceinline__ V3 pk3(const double *pk, long n, int w, int c) {
+    return V3{pk[(3 * w + 0) * n + c], pk[(3 * w + 1) * n + c], pk[(3 * w + 2) * n + c]};
+}
+}  // namespace
+
+// ------------------------------------------------------------------------------------------------
+// ucat_bar and theta_bar of every owned point (boundary ghosts included): the regular stencils of
+// the interior faces and interior cells, transposed.  Adds to the boundary-face scatter.
+// ------------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_vjp_gather(VjpArgs A, Geo g) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+    if (i > g.mx - 1) return;
+    const int x[3] = {i, j, k}, m[3] = {g.mx, g.my, g.mz};
+    const int c = g.L(k, j, i);
+    auto Lof = [&](const int *t) { return g.L(t[2], t[1], t[0]); };
+    V3 acc = ld3(A.ubar, c);
+    double tb = 0.0;
+    for (int q = 0; q < 3; q++) {
+        const int a = (q == 0) ? 1 : 0, b = (q == 2) ? 1 : 2;          // transverse axes, lower first
+        const double *pk = A.pk[q];
+        // an interior face of direction q at index point t
+        auto faceOK = [&](const int *t) {
+            return t[q] >= 1 && t[q] <= m[q] - 3 && t[a] >= 1 && t[a] <= m[a] - 2 && t[b] >= 1 && t[b] <= m[b] - 2;
+        };
+        int t[3];
+        // normal derivative and convective interpolation: x is the left cell of face x, the right
+        // cell of face x - q; QUICK also reaches x from faces x + q (V > 0) and x - 2q (V <= 0)
+        for (int s = -2; s <= 1; s++) {
+            t[0] = i; t[1] = j; t[2] = k;
+            t[q] = x[q] + s;
+            if (!faceOK(t)) continue;
+            const int fc = Lof(t);
+            const V3 cv = pk3(pk, A.n, 0, fc);
+            if (s == 0 || s == -1) {
+                const V3 gn = pk3(pk, A.n, 1, fc);
+                acc = add3v(acc, sc3(s == 0 ? -1.0 : 1.0, gn));
+            }
+            double w;
+            if (!A.quick) w = (s == 0 || s == -1) ? 0.5 : 0.0;
+            else {
+                const bool pos = A.ucont[3 * fc + q] > 0;
+                if (s == 0) w = pos ? 0.75 : 0.375;
+                else if (s == -1) w = pos ? 0.375 : 0.75;
+                else if (s == 1) w = pos ? -0.125 : 0.0;
+                else w = pos ? 0.0 : -0.125;
+            }
+            if (w != 0.0) acc = add3v(acc, sc3(w, cv));
+        }
+        // transverse four-point averages: faces x - e and x - e - q add, x + e and x + e - q subtract
+        for (int e = 0; e < 2; e++) {
+            const int ax = e == 0 ? a : b;
+            for (int sgn = -1; sgn <= 1; sgn += 2) {
+                for (int sq = 0; sq <= 1; sq++) {
+                    t[0] = i; t[1] = j; t[2] = k;
+                    t[ax] = x[ax] + sgn;
+                    t[q] = x[q] - sq;
+                    if (!faceOK(t)) continue;
+                    const V3 gt = pk3(pk, A.n, 2 + e, Lof(t));
+                    acc = add3v(acc, sc3(-0.25 * sgn, gt));
+                }
+            }
+        }
+        // |S| path: du_center of the interior cells x - q, x, x + q along q
+        if (A.les) {
+            const bool np = g.per[q] == TOSCA_PERIODIC_NONE;
+            auto cellOK = [&](const int *t) {
+                return t[0] >= 1 && t[0] <= m[0] - 2 && t[1] >= 1 && t[1] <= m[1] - 2 && t[2] >= 1 && t[2] <= m[2] - 2;
+            };
+            for (int s = -1; s <= 1; s++) {
+                t[0] = i; t[1] = j; t[2] = k;
+                t[q] = x[q] + s;
+                if (!cellOK(t)) continue;
+                const int cq = t[q];
+                double w = 0.0;
+                if (s == 1) w = (np && cq == m[q] - 2) ? -1.0 : (np && cq == 1) ? 0.0 : -0.5;
+                else if (s == -1) w = (np && cq == m[q] - 2) ? 0.0 : (np && cq == 1) ? 1.0 : 0.5;
+                else w = (np && cq == m[q] - 2) ? 1.0 : (np && cq == 1) ? -1.0 : 0.0;
+                if (w != 0.0) acc = add3v(acc, sc3(w, pk3(A.spk, A.n, q, Lof(t))));
+            }
+        }
+        // buoyancy: the + face q of this cell and of the cell below along q
+        for (int s = -1; s <= 0; s++) {
+            t[0] = i; t[1] = j; t[2] = k;
+            t[q] = x[q] + s;
+            if (t[0] < 1 || t[0] > m[0] - 2 || t[1] < 1 || t[1] > m[1] - 2 || t[2] < 1 || t[2] > m[2] - 2) continue;
+            const int cb = Lof(t);
+            const double *face = (q == 0) ? A.icsi : (q == 1) ? A.jeta : A.kzet;
+            tb += -A.bscale * (-9.81 * face[3 * cb + 2]) * 0.5 / A.tRef * A.rbar[3 * g.G(t[2], t[1], t[0]) + q];
+        }
+    }
+    st3(A.ubar, c, acc);
+    A.tbar[c] = tb;
 }
 
 void vjp_rhs(const Geo &g, const tosca_mesh *mesh, const tosca_params *p, const double *ucont, const double *ucat,
              const double *theta, const double *cs, const double *rbar, double scale, double buoy_scale,
              double *ucont_bar, double *ucat_bar, double *theta_bar, double *cs_bar, double *ws, hipStream_t s) {
     const long n = g.nloc();
-    double *nut = ws, *nutbar = ws + n, *fpbar = ws + 2 * n;
+    double *nut = ws, *fpbar = ws + n, *nueb = ws + 4 * n, *spk = ws + 7 * n, *pk = ws + 16 * n;   // 52 n in all
     launch_fill(ucont_bar, 3 * n, 0.0, s);
     launch_fill(ucat_bar, 3 * n, 0.0, s);
     launch_fill(theta_bar, n, 0.0, s);
     launch_fill(cs_bar, n, 0.0, s);
-    launch_fill(nutbar, n, 0.0, s);
+    launch_fill(nueb, 3 * n, 0.0, s);
     const bool les = p->les_active != 0;
     if (les) {   // recompute nu_t as the forward step does
         fused_nut<double>(g, mesh, ucat, cs, nut, nullptr, s);
@@ -343,29 +455,23 @@ void vjp_rhs(const Geo &g, const tosca_mesh *mesh, const tosca_params *p, const 
     A.kcsi = mesh->kcsi; A.keta = mesh->keta; A.kzet = mesh->kzet;
     A.aj = mesh->aj; A.iaj = mesh->iaj; A.jaj = mesh->jaj; A.kaj = mesh->kaj;
     A.tauI = p->tau_wall_i; A.tauJ = p->tau_wall_j;
-    A.fpbar = fpbar; A.nutbar = nutbar;
+    A.fpbar = fpbar; A.nutbar = nullptr;
     A.vbar = ucont_bar; A.ubar = ucat_bar; A.tbar = theta_bar; A.csbar = cs_bar;
+    for (int q = 0; q < 3; q++) { A.pk[q] = pk + 12 * q * n; A.nueb[q] = nueb + q * n; }
+    A.spk = spk;
+    A.n = n;
     A.scale = scale; A.bscale = buoy_scale; A.tRef = p->t_ref; A.nu = p->nu;
     A.quick = p->div_scheme == TOSCA_DIV_QUICK;
     A.les = les;
-    for (int q = 0; q < 6; q++) A.bc_u[q] = p->bc_u[q];
+    for (int q = 0; q < 6; q++) { A.bc_u[q] = p->bc_u[q]; A.bc_nut[q] = p->bc_nut[q]; }
     const dim3 cb(64, 4), cg((g.mx - 2 + 63) / 64, (g.my - 2 + 3) / 4, g.mz - 2);
     TOSCA_LAUNCH("k_vjp_fp", k_vjp_fp, cg, cb, 0, s, A, g);
+    // faces: x along i (0 .. mx-2 for i faces, 1 .. mx-2 otherwise), y / z the other two axes
     TOSCA_LAUNCH("k_vjp_face_i", k_vjp_face<0>, dim3((g.mx - 1 + 255) / 256, g.my - 2, g.mz - 2), dim3(256), 0, s, A, g);
-    TOSCA_LAUNCH("k_vjp_face_j", k_vjp_face<1>, dim3((g.my - 1 + 255) / 256, g.mx - 2, g.mz - 2), dim3(256), 0, s, A, g);
-    TOSCA_LAUNCH("k_vjp_face_k", k_vjp_face<2>, dim3((g.mz - 1 + 255) / 256, g.mx - 2, g.my - 2), dim3(256), 0, s, A, g);
-    TOSCA_LAUNCH("k_vjp_theta", k_vjp_theta, cg, cb, 0, s, A, g);
-    if (les) {
-        NutBCv B;
-        for (int q = 0; q < 6; q++) B.bc[q] = p->bc_nut[q];
-        const int mm[3] = {g.mx, g.my, g.mz};
-        for (int axis = 0; axis < 3; axis++) {
-            const int t1 = (axis == 0) ? 1 : 0, t2 = (axis == 2) ? 1 : 2;
-            TOSCA_LAUNCH("k_vjp_nut_bc", k_vjp_nut_bc, dim3((mm[t1] + 127) / 128, mm[t2] - 2, 2), dim3(128), 0, s, A, B,
-                         g, axis);
-        }
-        TOSCA_LAUNCH("k_vjp_nut", k_vjp_nut, cg, cb, 0, s, A, g);
-    }
+    TOSCA_LAUNCH("k_vjp_face_j", k_vjp_face<1>, dim3((g.mx - 2 + 255) / 256, g.my - 1, g.mz - 2), dim3(256), 0, s, A, g);
+    TOSCA_LAUNCH("k_vjp_face_k", k_vjp_face<2>, dim3((g.mx - 2 + 255) / 256, g.my - 2, g.mz - 1), dim3(256), 0, s, A, g);
+    if (les) TOSCA_LAUNCH("k_vjp_nut", k_vjp_nut, cg, cb, 0, s, A, g);
+    TOSCA_LAUNCH("k_vjp_gather", k_vjp_gather, dim3((g.mx + 255) / 256, g.my, g.mz), dim3(256), 0, s, A, g);
 }
 
 }  // namespace tosca
