@@ -246,14 +246,28 @@ __device__ __forceinline__ void face2cell4(int ptype, int f, int m, int &L, int 
     }
 }
 
-// Germano contraction (les.c:715-799): num = L_ba M_aq G_bq, denom = M_nm M_nl G_ml.
-__device__ __forceinline__ void germano(const double L[3][3], const double Sh[3][3], const double SSh[3][3], double filter,
-                        double test_filter, V3 c0, V3 e0, V3 z0, double &num_out, double &den_out) {
-    // pow(x, 2.) of the reference is x * x (exactly rounded); the inverse uses one reciprocal of det
+// The Germano contraction of les.c:715-799 in three stages, so that M and G can be formed by
+// different threads of a workgroup: M from the filtered strain terms, G from the cell metrics,
+// then num = L_ba M_aq G_bq and den = M_nm M_nl G_ml.  pow(x, 2.) of the reference is x * x (exactly rounded); the
+// inverse of the metric matrix uses one reciprocal of its determinant.
+__device__ __forceinline__ void germano_M(const double Sh[3][3], const double SSh[3][3], double filter,
+                                          double test_filter, V3 c0, V3 e0, V3 z0, double M[3][3]) {
     double S_hat = 0;
     for (int a = 0; a < 3; a++)
         for (int b = 0; b < 3; b++) S_hat += Sh[a][b] * Sh[a][b];
     S_hat = sqrt(2 * S_hat);
+    const double tf2 = test_filter * test_filter, f2 = filter * filter;
+    for (int a = 0; a < 3; a++) {
+        double Mc0 = (-tf2 * S_hat * Sh[a][0] + f2 * SSh[a][0]);
+        double Mc1 = (-tf2 * S_hat * Sh[a][1] + f2 * SSh[a][1]);
+        double Mc2 = (-tf2 * S_hat * Sh[a][2] + f2 * SSh[a][2]);
+        M[a][0] = Mc0 * c0.x + Mc1 * c0.y + Mc2 * c0.z;
+        M[a][1] = Mc0 * e0.x + Mc1 * e0.y + Mc2 * e0.z;
+        M[a][2] = Mc0 * z0.x + Mc1 * z0.y + Mc2 * z0.z;
+    }
+}
+
+__device__ __forceinline__ void germano_G(V3 c0, V3 e0, V3 z0, double G[3][3]) {
     const double a11 = c0.x, a12 = c0.y, a13 = c0.z, a21 = e0.x, a22 = e0.y, a23 = e0.z, a31 = z0.x, a32 = z0.y, a33 = z0.z;
     double det = a11 * (a33 * a22 - a32 * a23) - a21 * (a33 * a12 - a32 * a13) + a31 * (a23 * a12 - a22 * a13);
     const double rd = 1.0 / det;
@@ -264,23 +278,16 @@ __device__ __forceinline__ void germano(const double L[3][3], const double Sh[3]
     double xcsi = gc[0][0], xeta = gc[0][1], xzet = gc[0][2];
     double ycsi = gc[1][0], yeta = gc[1][1], yzet = gc[1][2];
     double zcsi = gc[2][0], zeta = gc[2][1], zzet = gc[2][2];
-    double G[3][3];
     G[0][0] = xcsi * xcsi + ycsi * ycsi + zcsi * zcsi;
     G[1][1] = xeta * xeta + yeta * yeta + zeta * zeta;
     G[2][2] = xzet * xzet + yzet * yzet + zzet * zzet;
     G[0][1] = G[1][0] = xeta * xcsi + yeta * ycsi + zeta * zcsi;
     G[0][2] = G[2][0] = xzet * xcsi + yzet * ycsi + zzet * zcsi;
     G[1][2] = G[2][1] = xeta * xzet + yeta * yzet + zeta * zzet;
-    double tf2 = test_filter * test_filter, f2 = filter * filter;
-    double M[3][3];
-    for (int a = 0; a < 3; a++) {
-        double Mc0 = (-tf2 * S_hat * Sh[a][0] + f2 * SSh[a][0]);
-        double Mc1 = (-tf2 * S_hat * Sh[a][1] + f2 * SSh[a][1]);
-        double Mc2 = (-tf2 * S_hat * Sh[a][2] + f2 * SSh[a][2]);
-        M[a][0] = Mc0 * c0.x + Mc1 * c0.y + Mc2 * c0.z;
-        M[a][1] = Mc0 * e0.x + Mc1 * e0.y + Mc2 * e0.z;
-        M[a][2] = Mc0 * z0.x + Mc1 * z0.y + Mc2 * z0.z;
-    }
+}
+
+__device__ __forceinline__ void germano_contract(const double L[3][3], const double M[3][3], const double G[3][3],
+                                                 double &num_out, double &den_out) {
     double num = 0, den = 0;
     for (int q = 0; q < 3; q++)
         for (int a = 0; a < 3; a++)
@@ -290,6 +297,14 @@ __device__ __forceinline__ void germano(const double L[3][3], const double Sh[3]
             for (int l = 0; l < 3; l++) den += M[n][m] * M[n][l] * G[m][l];
     num_out = num;
     den_out = den;
+}
+
+__device__ __forceinline__ void germano(const double L[3][3], const double Sh[3][3], const double SSh[3][3], double filter,
+                        double test_filter, V3 c0, V3 e0, V3 z0, double &num_out, double &den_out) {
+    double M[3][3], G[3][3];
+    germano_M(Sh, SSh, filter, test_filter, c0, e0, z0, M);
+    germano_G(c0, e0, z0, G);
+    germano_contract(L, M, G, num_out, den_out);
 }
 
 __device__ __forceinline__ double pmax(double a, double b) { return (a < b) ? b : a; }

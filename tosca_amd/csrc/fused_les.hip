@@ -25,8 +25,21 @@ constexpr int NQ = 28;   // 27 Germano quantities + the binomial-weighted volume
 constexpr int DSM_NR = 8;
 constexpr int DSM_TI = 62, DSM_TJ = DSM_NR - 2;
 
-template <class T> __device__ __forceinline__ T shup(T x) { return __shfl_up(x, 1, 64); }
-template <class T> __device__ __forceinline__ T shdn(T x) { return __shfl_down(x, 1, 64); }
+// neighbour values across the lanes of a wave: shup gives lane l the value of lane l-1, shdn that of
+// lane l+1.  Lane 0 (shup) and lane 63 (shdn) have no neighbour and receive 0; both are halo
+// lanes whose own filtered values are never used, and their ucat neighbours are loaded.  The DPP
+// wavefront shifts (wave_shr:1, wave_shl:1) are VALU register moves; __shfl_up/down go through
+// the LDS crossbar (ds_bpermute) and cost about a fifth of this kernel's time.
+__device__ __forceinline__ int dpp_shr(int x) { return __builtin_amdgcn_mov_dpp(x, 0x138, 0xf, 0xf, true); }
+__device__ __forceinline__ int dpp_shl(int x) { return __builtin_amdgcn_mov_dpp(x, 0x130, 0xf, 0xf, true); }
+__device__ __forceinline__ double shup(double x) {
+    return __hiloint2double(dpp_shr(__double2hiint(x)), dpp_shr(__double2loint(x)));
+}
+__device__ __forceinline__ double shdn(double x) {
+    return __hiloint2double(dpp_shl(__double2hiint(x)), dpp_shl(__double2loint(x)));
+}
+__device__ __forceinline__ float shup(float x) { return __int_as_float(dpp_shr(__float_as_int(x))); }
+__device__ __forceinline__ float shdn(float x) { return __int_as_float(dpp_shl(__float_as_int(x))); }
 template <class T> __device__ __forceinline__ Vec3<T> shup3(Vec3<T> a) { return Vec3<T>{shup(a.x), shup(a.y), shup(a.z)}; }
 template <class T> __device__ __forceinline__ Vec3<T> shdn3(Vec3<T> a) { return Vec3<T>{shdn(a.x), shdn(a.y), shdn(a.z)}; }
 
