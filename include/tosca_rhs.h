@@ -67,9 +67,15 @@ enum { TOSCA_PERIODIC_NONE = 0, TOSCA_PERIODIC_COPY = 1, TOSCA_PERIODIC_WRAP = 2
 /* convective schemes accepted by tosca_formu (ueqn->centralDiv / ueqn->quickDiv, src/ueqn.c:81-110) */
 enum { TOSCA_DIV_CENTRAL = 0, TOSCA_DIV_QUICK = 1 };
 
-/* boundaryU patch kinds that change FormU's viscous flux (ueqn_solvers.c:336-356,548-568,758-768).
- * Every other velocity BC (noSlip, fixedValue, zeroGradient, inletFunction, ...) is GENERIC.     */
-enum { TOSCA_BCU_GENERIC = 0, TOSCA_BCU_SLIP = 1, TOSCA_BCU_WALLFUNCTION = 2, TOSCA_BCU_PERIODIC = 3 };
+/* boundaryU patch kinds (mesh->boundaryU.iLeft ... kRight).  SLIP and WALLFUNCTION change FormU's
+ * viscous flux (ueqn_solvers.c:336-356,548-568,758-768); NOSLIP, SLIP, WALLFUNCTION and
+ * ZEROGRADIENT select the face resets of tosca_explicit_rhs_u (resetNoPenetrationFluxes,
+ * inline.h:5262-5355; resetNonResolvedCellFaces, inline.h:1674-1838).  Every other velocity BC
+ * (fixedValue, inletFunction, ...) is GENERIC.                                                    */
+enum {
+    TOSCA_BCU_GENERIC = 0, TOSCA_BCU_SLIP = 1, TOSCA_BCU_WALLFUNCTION = 2, TOSCA_BCU_PERIODIC = 3,
+    TOSCA_BCU_NOSLIP = 4, TOSCA_BCU_ZEROGRADIENT = 5
+};
 
 /* boundaryNut patch kinds applied by UpdateNutBCs (src/boundary.c:2812-2917) */
 enum { TOSCA_BCNUT_NONE = 0, TOSCA_BCNUT_ZEROGRADIENT = 1, TOSCA_BCNUT_FIXEDVALUE = 2, TOSCA_BCNUT_PERIODIC = 3 };
@@ -189,6 +195,28 @@ int tosca_rhs_vjp(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_params *pa
                   const double *ucont, const double *ucat, const double *theta, const double *cs,
                   const double *rbar, double scale, double buoy_scale,
                   double *ucont_bar, double *ucat_bar, double *theta_bar, double *cs_bar);
+
+/* ExplicitRhsU (src/solvers/ueqn_solvers.c:1276-1407), the immediate caller of FormU, with the
+ * terms this library covers, in the reference's order:
+ *   resetNoPenetrationFluxes (src/include/inline.h:5262-5355) on ucont: the normal flux at noSlip,
+ *     velocityWallFunction and slip patches set to 0, at zeroGradient patches clipped to outflow;
+ *   resetFacePeriodicFluxesVector (inline.h:5357-5451) on ucont;
+ *   contravariantToCartesian (src/ueqn.c:528-635): ucat at the interior cells from ucont;
+ *   resetCellPeriodicFluxes (inline.h:5453-5606) on ucat;
+ *   rhs = 0; rhs -= dp; rhs += 3/2 btheta - 1/2 btheta_o (ab2 != 0, clock->it > itStart) or
+ *     rhs += btheta (ab2 == 0); rhs += farm; FormU(rhs, 1.0, form_mode) -- VecAXPY order;
+ *   resetNonResolvedCellFaces (inline.h:1674-1838) on rhs, with ucont_o.
+ * ucont: lUcont (local AoS, in/out); ucat: lUcat (local AoS, out: interior cells and periodic
+ * boundary cells/ghosts; other ghost cells keep their values); nut: lNu_t (read when les_active);
+ * dp, btheta, btheta_o, farm, ucont_o, rhs: global AoS (dP, bTheta, bTheta_o, sourceFarmCont,
+ * Ucont_o, Rhs).  btheta == NULL: temperature equation off; farm == NULL: no wind farm.
+ * The reference's other source terms (Coriolis, canopy, driving controller, damping layers,
+ * pressure-gradient forcings, hyperviscosity, the pTilde buoyancy form) are not part of this
+ * path; a caller that uses them adds them to rhs before resetNonResolvedCellFaces would. */
+int tosca_explicit_rhs_u(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_params *params,
+                         double *ucont, double *ucat, const double *nut, const double *dp,
+                         const double *btheta, const double *btheta_o, int ab2, const double *farm,
+                         const double *ucont_o, double *rhs, int form_mode);
 
 #ifdef __cplusplus
 }

@@ -45,6 +45,7 @@ def load() -> C.CDLL:
         "oracle_buoyancy": (I, [In, M, Pa, P, D, P, P]),
         "oracle_rhs_les_step": (I, [In, M, Pa, P, P, P, P, D, P, P, P, D]),
         "oracle_rhs_jvp": (I, [In, M, Pa, P, P, P, P, P, P, P, P, D, D, P, P]),
+        "oracle_explicit_rhs_u": (I, [In, M, Pa, P, P, P, P, P, P, I, P, P, P, I]),
     }
     for n, (r, a) in sigs.items():
         f = getattr(lib, n)
@@ -138,3 +139,14 @@ def rhs_forward(info, mesh, params, ucont, ucat, theta, cs, scale=1.0, buoy_scal
     bt = np.zeros(shape)
     buoyancy(info, mesh, params, theta, buoy_scale, bt, None)
     return rhs + bt
+
+
+def explicit_rhs_u(info, mesh, params, ucont, ucat, nut, dp, btheta, btheta_o, ab2, farm, ucont_o, rhs,
+                   form_mode=0):
+    """ExplicitRhsU restated (oracle/tosca_oracle.cpp): ucont and ucat are modified in place, rhs
+    is overwritten.  btheta / farm None: the term is off."""
+    lib = load()
+    inf, m, p = abi.make_info(info), _mesh(mesh), abi.make_params(params)
+    _check(lib.oracle_explicit_rhs_u(C.byref(inf), C.byref(m), C.byref(p), _p(ucont), _p(ucat), _p(nut), _p(dp),
+                                     _p(btheta), _p(btheta_o), int(ab2), _p(farm), _p(ucont_o), _p(rhs),
+                                     int(form_mode)))

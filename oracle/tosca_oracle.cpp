@@ -571,6 +571,11 @@ int ORACLE_API(oracle_formu)(const tosca_dmda_info *info, const ORACLE_MESH *m, 
     if (rc) return rc;
     if (formMode < 0 || formMode > 2) { g_err = "bad formMode"; return TOSCA_ERR_ARG; }
     if (p->div_scheme != TOSCA_DIV_CENTRAL && p->div_scheme != TOSCA_DIV_QUICK) { g_err = "scheme"; return TOSCA_ERR_UNSUPPORTED; }
+    if (formMode != 1 && (((p->bc_u[0] == TOSCA_BCU_WALLFUNCTION || p->bc_u[1] == TOSCA_BCU_WALLFUNCTION) && !p->tau_wall_i) ||
+                          ((p->bc_u[2] == TOSCA_BCU_WALLFUNCTION || p->bc_u[3] == TOSCA_BCU_WALLFUNCTION) && !p->tau_wall_j))) {
+        g_err = "wall-function patch without tau_wall";
+        return TOSCA_ERR_ARG;
+    }
     const int64_t mx = l.mx, my = l.my, mz = l.mz, n = l.nloc();
     const V3 *ucont = (const V3 *)ucont_, *ucat = (const V3 *)ucat_;
     const V3 *csi = (const V3 *)m->csi, *eta = (const V3 *)m->eta, *zet = (const V3 *)m->zet;
@@ -1050,6 +1055,138 @@ int oracle_rhs_les_step(const tosca_dmda_info *info, const tosca_mesh *m, const 
     rc = oracle_buoyancy(info, m, p, theta, buoy_scale, btheta, nullptr);
     if (rc) return rc;
     return oracle_formu(info, m, p, ucont, ucat, nut, rhs, scale, 0);
+}
+#endif  // ORACLE_TANGENT
+
+#ifndef ORACLE_TANGENT
+// ================================================================================================
+// ExplicitRhsU (src/solvers/ueqn_solvers.c:1276-1407), the terms of the momentum right-hand side
+// this path covers.  On one rank a PETSc global vector is the owned part of its local vector, so
+// the "globalToLocal" resets below act on the local arrays and the GlobalToLocal scatter that
+// follows each of them is the periodic ghost fill of local_to_local.
+// ================================================================================================
+
+// resetNoPenetrationFluxes (src/include/inline.h:5262-5355): loops k < mz-1, j < my-1, i < mx-1
+static void reset_no_penetration(const Lay &l, const tosca_params *p, V3 *ucont) {
+    const int64_t mx = l.mx, my = l.my, mz = l.mz;
+    const int *bc = p->bc_u;   // iLeft iRight jLeft jRight kLeft kRight
+    auto zeroes = [](int b) { return b == TOSCA_BCU_NOSLIP || b == TOSCA_BCU_WALLFUNCTION || b == TOSCA_BCU_SLIP; };
+    for (int64_t k = 0; k < mz - 1; k++)
+        for (int64_t j = 0; j < my - 1; j++)
+            for (int64_t i = 0; i < mx - 1; i++) {
+                V3 &u = ucont[l.L(k, j, i)];
+                // noSlip, velocityWallFunction, slip: no flux through the patch (:5290-5311)
+                if (i == 0 && zeroes(bc[0])) u.x = 0.0;
+                if (i == mx - 2 && zeroes(bc[1])) u.x = 0.0;
+                if (j == 0 && zeroes(bc[2])) u.y = 0.0;
+                if (j == my - 2 && zeroes(bc[3])) u.y = 0.0;
+                if (k == 0 && zeroes(bc[4])) u.z = 0.0;
+                if (k == mz - 2 && zeroes(bc[5])) u.z = 0.0;
+                // zeroGradient: reverse (inflowing) flux removed (:5313-5336)
+                if (bc[0] == TOSCA_BCU_ZEROGRADIENT && i == 0 && u.x > 0.0) u.x = 0.0;
+                if (bc[1] == TOSCA_BCU_ZEROGRADIENT && i == mx - 2 && u.x < 0.0) u.x = 0.0;
+                if (bc[2] == TOSCA_BCU_ZEROGRADIENT && j == 0 && u.y > 0.0) u.y = 0.0;
+                if (bc[3] == TOSCA_BCU_ZEROGRADIENT && j == my - 2 && u.y < 0.0) u.y = 0.0;
+                if (bc[4] == TOSCA_BCU_ZEROGRADIENT && k == 0 && u.z > 0.0) u.z = 0.0;
+                if (bc[5] == TOSCA_BCU_ZEROGRADIENT && k == mz - 2 && u.z < 0.0) u.z = 0.0;
+            }
+    local_to_local(l, ucont);     // DMGlobalToLocal (:5343-5344)
+}
+
+// contravariantToCartesian (src/ueqn.c:528-635): interior cells, Cramer's rule on the cell basis
+static void contravariant_to_cartesian(const Lay &l, const tosca_mesh *m, const V3 *lucont, V3 *ucat) {
+    const V3 *csi = (const V3 *)m->csi, *eta = (const V3 *)m->eta, *zet = (const V3 *)m->zet;
+    for (int64_t k = 1; k < l.mz - 1; k++)
+        for (int64_t j = 1; j < l.my - 1; j++)
+            for (int64_t i = 1; i < l.mx - 1; i++) {
+                const int64_t c = l.L(k, j, i);
+                double mat[3][3] = {{csi[c].x, csi[c].y, csi[c].z}, {eta[c].x, eta[c].y, eta[c].z},
+                                    {zet[c].x, zet[c].y, zet[c].z}};
+                double q[3];
+                q[0] = 0.5 * (lucont[l.L(k, j, i - 1)].x + lucont[c].x);     // central (inline.h:4307-4312)
+                q[1] = 0.5 * (lucont[l.L(k, j - 1, i)].y + lucont[c].y);
+                q[2] = 0.5 * (lucont[l.L(k - 1, j, i)].z + lucont[c].z);
+                const double det = mat[0][0] * (mat[1][1] * mat[2][2] - mat[1][2] * mat[2][1]) -
+                                   mat[0][1] * (mat[1][0] * mat[2][2] - mat[1][2] * mat[2][0]) +
+                                   mat[0][2] * (mat[1][0] * mat[2][1] - mat[1][1] * mat[2][0]);
+                const double det0 = q[0] * (mat[1][1] * mat[2][2] - mat[1][2] * mat[2][1]) -
+                                    q[1] * (mat[0][1] * mat[2][2] - mat[0][2] * mat[2][1]) +
+                                    q[2] * (mat[0][1] * mat[1][2] - mat[0][2] * mat[1][1]);
+                const double det1 = -q[0] * (mat[1][0] * mat[2][2] - mat[1][2] * mat[2][0]) +
+                                    q[1] * (mat[0][0] * mat[2][2] - mat[0][2] * mat[2][0]) -
+                                    q[2] * (mat[0][0] * mat[1][2] - mat[0][2] * mat[1][0]);
+                const double det2 = q[0] * (mat[1][0] * mat[2][1] - mat[1][1] * mat[2][0]) -
+                                    q[1] * (mat[0][0] * mat[2][1] - mat[0][1] * mat[2][0]) +
+                                    q[2] * (mat[0][0] * mat[1][1] - mat[0][1] * mat[1][0]);
+                ucat[c] = V3{det0 / det, det1 / det, det2 / det};
+            }
+    local_to_local(l, ucat);      // DMGlobalToLocal (:621-622)
+}
+
+// resetNonResolvedCellFaces (src/include/inline.h:1674-1838) on a global vector (no IBM / overset)
+static void reset_nonresolved(const Lay &l, const tosca_params *p, V3 *v, const V3 *ucont_o) {
+    const int64_t mx = l.mx, my = l.my, mz = l.mz;
+    const int *bc = p->bc_u;
+    const bool iper = l.per[0] != TOSCA_PERIODIC_NONE, jper = l.per[1] != TOSCA_PERIODIC_NONE,
+               kper = l.per[2] != TOSCA_PERIODIC_NONE;
+    for (int64_t k = 0; k < mz; k++)
+        for (int64_t j = 0; j < my; j++)
+            for (int64_t i = 0; i < mx; i++) {
+                const int64_t g = l.G(k, j, i);
+                const V3 uo = ucont_o[g];
+                if (i == 0) { if (!(bc[0] == TOSCA_BCU_ZEROGRADIENT || uo.x < 0)) v[g].x = 0.0; }
+                else if (i == mx - 1) v[g].x = 0.0;
+                else if (i == mx - 2 && !iper) { if (!(bc[1] == TOSCA_BCU_ZEROGRADIENT || uo.x > 0)) v[g].x = 0.0; }
+                if (j == 0) { if (!(bc[2] == TOSCA_BCU_ZEROGRADIENT || uo.y < 0)) v[g].y = 0.0; }
+                else if (j == my - 1) v[g].y = 0.0;
+                else if (j == my - 2 && !jper) { if (!(bc[3] == TOSCA_BCU_ZEROGRADIENT || uo.y > 0)) v[g].y = 0.0; }
+                if (k == 0) { if (!(bc[4] == TOSCA_BCU_ZEROGRADIENT || uo.z < 0)) v[g].z = 0.0; }
+                else if (k == mz - 1) v[g].z = 0.0;
+                else if (k == mz - 2 && !kper) { if (!(bc[5] == TOSCA_BCU_ZEROGRADIENT || uo.z > 0)) v[g].z = 0.0; }
+            }
+}
+
+// VecAXPY(y, a, x): y = y + a x on every owned point
+static void axpy(int64_t n, double a, const V3 *x, V3 *y) {
+    for (int64_t g = 0; g < n; g++) {
+        y[g].x = y[g].x + a * x[g].x;
+        y[g].y = y[g].y + a * x[g].y;
+        y[g].z = y[g].z + a * x[g].z;
+    }
+}
+
+int oracle_explicit_rhs_u(const tosca_dmda_info *info, const tosca_mesh *m, const tosca_params *p,
+                          double *ucont_, double *ucat_, const double *nut, const double *dp,
+                          const double *btheta, const double *btheta_o, int ab2, const double *farm,
+                          const double *ucont_o, double *rhs_, int formMode) {
+    Lay l;
+    int rc = check_layout(info, p, l);
+    if (rc) return rc;
+    if (!ucont_ || !ucat_ || !dp || !ucont_o || !rhs_ || (btheta && ab2 && !btheta_o)) {
+        g_err = "explicit_rhs_u: missing array";
+        return TOSCA_ERR_ARG;
+    }
+    V3 *ucont = (V3 *)ucont_, *ucat = (V3 *)ucat_, *rhs = (V3 *)rhs_;
+    const int64_t ng = l.mx * l.my * l.mz;
+    reset_no_penetration(l, p, ucont);                        // :1281
+    reset_face_periodic(l, ucont);                            // :1284
+    contravariant_to_cartesian(l, m, ucont, ucat);            // :1287
+    reset_cell_periodic(l, ucat);                             // :1301
+    for (int64_t g = 0; g < ng; g++) rhs[g] = V3{0, 0, 0};   // :1304
+    axpy(ng, -1.0, (const V3 *)dp, rhs);                      // :1307
+    if (btheta) {                                             // :1328-1336
+        if (ab2) {
+            axpy(ng, 3.0 / 2.0, (const V3 *)btheta, rhs);
+            axpy(ng, -1.0 / 2.0, (const V3 *)btheta_o, rhs);
+        } else {
+            axpy(ng, 1.0, (const V3 *)btheta, rhs);
+        }
+    }
+    if (farm) axpy(ng, 1.0, (const V3 *)farm, rhs);          // :1341-1344
+    rc = oracle_formu(info, m, p, ucont_, ucat_, nut, rhs_, 1.0, formMode);   // :1347
+    if (rc) return rc;
+    reset_nonresolved(l, p, rhs, (const V3 *)ucont_o);       // :1403
+    return TOSCA_OK;
 }
 #endif  // ORACLE_TANGENT
 

@@ -34,6 +34,7 @@ SOURCES = {
     "fused_formu.hip": [],
     "fused_les.hip": [],
     "vjp.hip": [],
+    "explicit_rhs.hip": ["-ffp-contract=off"],
 }
 
 

@@ -32,7 +32,7 @@ G = 3  # DMDA ghost width (src/mesh.c:42)
 
 PERIODIC_NONE, PERIODIC_COPY, PERIODIC_WRAP = 0, 1, 2
 DIV_CENTRAL, DIV_QUICK = 0, 1
-BCU_GENERIC, BCU_SLIP, BCU_WALLFUNCTION, BCU_PERIODIC = 0, 1, 2, 3
+BCU_GENERIC, BCU_SLIP, BCU_WALLFUNCTION, BCU_PERIODIC, BCU_NOSLIP, BCU_ZEROGRADIENT = 0, 1, 2, 3, 4, 5
 BCNUT_NONE, BCNUT_ZEROGRADIENT, BCNUT_FIXEDVALUE, BCNUT_PERIODIC = 0, 1, 2, 3
 LES_SMAGORINSKY, LES_DSM, LES_DPASD = 0, 2, 5
 
@@ -567,7 +567,8 @@ def default_params(periodic, bcs_u, bcs_nut, nut_values=(0.0,) * 6, div_scheme=D
                    les_model=LES_DSM, nu=1.5e-5, t_ref=300.0, max_cs=0.5, les_active=1):
     bcu = []
     for b in bcs_u:
-        bcu.append({BC_SLIP: BCU_SLIP, BC_PERIODIC: BCU_PERIODIC}.get(b, BCU_GENERIC))
+        bcu.append({BC_SLIP: BCU_SLIP, BC_PERIODIC: BCU_PERIODIC, BC_NOSLIP: BCU_NOSLIP,
+                    BC_ZEROGRAD: BCU_ZEROGRADIENT}.get(b, BCU_GENERIC))
     bcn = []
     for b in bcs_nut:
         bcn.append({BC_PERIODIC: BCNUT_PERIODIC, BC_ZEROGRAD: BCNUT_ZEROGRADIENT,

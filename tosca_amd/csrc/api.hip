@@ -93,7 +93,7 @@ int check_params(tosca_ctx *c, const tosca_params *p) {
         if (p->periodic[a] < TOSCA_PERIODIC_NONE || p->periodic[a] > TOSCA_PERIODIC_WRAP)
             return set_err(c, TOSCA_ERR_ARG, "periodic[%d] = %d is not a periodic type", a, p->periodic[a]);
     for (int q = 0; q < 6; q++) {
-        if (p->bc_u[q] < TOSCA_BCU_GENERIC || p->bc_u[q] > TOSCA_BCU_PERIODIC)
+        if (p->bc_u[q] < TOSCA_BCU_GENERIC || p->bc_u[q] > TOSCA_BCU_ZEROGRADIENT)
             return set_err(c, TOSCA_ERR_ARG, "bc_u[%d] = %d is not a TOSCA_BCU_* kind", q, p->bc_u[q]);
         if (p->bc_nut[q] < TOSCA_BCNUT_NONE || p->bc_nut[q] > TOSCA_BCNUT_PERIODIC)
             return set_err(c, TOSCA_ERR_ARG, "bc_nut[%d] = %d is not a TOSCA_BCNUT_* kind", q, p->bc_nut[q]);
@@ -392,6 +392,57 @@ int tosca_formu(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const 
         tosca::fused_formu(g, A, c->ws + 3 * (size_t)g.nloc(), c->stream);
     }
     return launched(c, "FormU");
+}
+
+int tosca_explicit_rhs_u(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, double *ucont, double *ucat,
+                         const double *nut, const double *dp, const double *btheta, const double *btheta_o, int ab2,
+                         const double *farm, const double *ucont_o, double *rhs, int form_mode) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if ((rc = check_params(c, p))) return rc;
+    if (form_mode < 0 || form_mode > 2) return set_err(c, TOSCA_ERR_ARG, "form_mode = %d (0, 1 or 2)", form_mode);
+    if (p->div_scheme != TOSCA_DIV_CENTRAL && p->div_scheme != TOSCA_DIV_QUICK)
+        return set_err(c, TOSCA_ERR_UNSUPPORTED, "div_scheme = %d: only central (0) and QUICK (1) are supported", p->div_scheme);
+    set_geo_per(c, p);
+    const Geo &g = c->geo;
+    const size_t n = (size_t)g.nloc(), vloc = 3 * n * sizeof(double), vglob = 3 * (size_t)g.mx * g.my * g.mz * sizeof(double);
+    if ((rc = check_mesh(c, m, true, true, true))) return rc;
+    if ((rc = check_dev(c, ucont, vloc, "ucont")) || (rc = check_dev(c, ucat, vloc, "ucat")) ||
+        (rc = check_dev(c, dp, vglob, "dp")) || (rc = check_dev(c, ucont_o, vglob, "ucont_o")) ||
+        (rc = check_dev(c, rhs, vglob, "rhs")))
+        return rc;
+    if (btheta) {
+        if ((rc = check_dev(c, btheta, vglob, "btheta"))) return rc;
+        if (ab2 && (rc = check_dev(c, btheta_o, vglob, "btheta_o"))) return rc;
+    }
+    if (farm && (rc = check_dev(c, farm, vglob, "farm"))) return rc;
+    const bool visc = form_mode != 1;
+    if (p->les_active && visc) {
+        if ((rc = check_les_model(c, p))) return rc;
+        if ((rc = check_dev(c, nut, n * sizeof(double), "nut"))) return rc;
+    }
+    if (visc) {
+        if (p->bc_u[0] == TOSCA_BCU_WALLFUNCTION || p->bc_u[1] == TOSCA_BCU_WALLFUNCTION)
+            if ((rc = check_dev(c, p->tau_wall_i, 3 * (size_t)g.mz * g.my * sizeof(double), "tau_wall_i"))) return rc;
+        if (p->bc_u[2] == TOSCA_BCU_WALLFUNCTION || p->bc_u[3] == TOSCA_BCU_WALLFUNCTION)
+            if ((rc = check_dev(c, p->tau_wall_j, 3 * (size_t)g.mz * g.mx * sizeof(double), "tau_wall_j"))) return rc;
+    }
+    if ((rc = ensure_ws(c, ws_need(c)))) return rc;
+    hipStream_t s = c->stream;
+    tosca::launch_no_penetration(ucont, p, g, s);                    // ueqn_solvers.c:1281
+    tosca::launch_reset_face_periodic(ucont, 3, g, s);               // :1284
+    tosca::launch_contra_to_cart(ucont, m, ucat, g, s);              // :1287
+    tosca::launch_reset_cell_periodic(ucat, 3, g, s);                // :1301
+    tosca::launch_rhs_init(rhs, dp, btheta, ab2 ? btheta_o : nullptr, btheta ? ab2 : 0, farm, g, s);   // :1304-1344
+    const double *nu_t = (p->les_active && visc) ? nut : nullptr;   // :1347
+    if (c->ref_path) {
+        tosca::ref_formu(g, m, p, ucont, ucat, nu_t, rhs, 1.0, form_mode, c->ws, s);
+    } else {
+        tosca::FormUArgs A = formu_args(m, p, ucont, ucat, nu_t, rhs, 1.0, form_mode);
+        tosca::fused_formu(g, A, c->ws + 3 * (size_t)g.nloc(), s);
+    }
+    tosca::launch_non_resolved(rhs, ucont_o, p, g, s);               // :1403
+    return launched(c, "ExplicitRhsU");
 }
 
 int tosca_update_cs(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const double *ucat, double *cs) {

@@ -307,6 +307,30 @@ __device__ __forceinline__ void germano(const double L[3][3], const double Sh[3]
     germano_contract(L, M, G, num_out, den_out);
 }
 
+// Neighbour values across the lanes of a 64-wide wave: shup gives lane l the value of lane l-1,
+// shdn that of lane l+1.  Lane 0 (shup) and lane 63 (shdn) have no neighbour and receive 0; the
+// tiled kernels use both lanes as halo lanes whose own results are discarded.  The DPP wavefront
+// shifts (wave_shr:1, wave_shl:1) are VALU register moves; __shfl_up/down would go through the
+// LDS crossbar (ds_bpermute) with its latency.
+__device__ __forceinline__ int dpp_shr(int x) { return __builtin_amdgcn_mov_dpp(x, 0x138, 0xf, 0xf, true); }
+__device__ __forceinline__ int dpp_shl(int x) { return __builtin_amdgcn_mov_dpp(x, 0x130, 0xf, 0xf, true); }
+__device__ __forceinline__ double shup(double x) {
+    return __hiloint2double(dpp_shr(__double2hiint(x)), dpp_shr(__double2loint(x)));
+}
+__device__ __forceinline__ double shdn(double x) {
+    return __hiloint2double(dpp_shl(__double2hiint(x)), dpp_shl(__double2loint(x)));
+}
+__device__ __forceinline__ float shup(float x) { return __int_as_float(dpp_shr(__float_as_int(x))); }
+__device__ __forceinline__ float shdn(float x) { return __int_as_float(dpp_shl(__float_as_int(x))); }
+template <class T> __device__ __forceinline__ Vec3<T> shup3(Vec3<T> a) { return Vec3<T>{shup(a.x), shup(a.y), shup(a.z)}; }
+template <class T> __device__ __forceinline__ Vec3<T> shdn3(Vec3<T> a) { return Vec3<T>{shdn(a.x), shdn(a.y), shdn(a.z)}; }
+
+// The filter widths Delta = (1/aj)^(1/3) and Delta~ = (8 sum)^(1/3) of les.c:657-660, 2510 are
+// written pow(x, 1./3.) in the reference.  The fused kernels take cbrt: the two differ by a few
+// ulp (pow's exponent 1./3. is itself rounded), and on gfx950 the FP64 pow expands to about six
+// times the instructions of cbrt.
+__device__ __forceinline__ double cube_root(double x) { return cbrt(x); }
+
 __device__ __forceinline__ double pmax(double a, double b) { return (a < b) ? b : a; }
 __device__ __forceinline__ double pmin(double a, double b) { return (a < b) ? a : b; }
 

@@ -94,10 +94,6 @@ struct Ring {
     }
 };
 
-template <class T> __device__ __forceinline__ T shfl_up1(T x) { return __shfl_up(x, 1, 64); }
-template <class T> __device__ __forceinline__ T shfl_dn1(T x) { return __shfl_down(x, 1, 64); }
-template <class T> __device__ __forceinline__ Vec3<T> shfl_up3(Vec3<T> a) { return Vec3<T>{shfl_up1(a.x), shfl_up1(a.y), shfl_up1(a.z)}; }
-template <class T> __device__ __forceinline__ Vec3<T> shfl_dn3(Vec3<T> a) { return Vec3<T>{shfl_dn1(a.x), shfl_dn1(a.y), shfl_dn1(a.z)}; }
 
 // fp differences in the reference's order: ((((a - b) + c) - d) + e) - f, div and visc apart
 __device__ __forceinline__ double fpsum(double d1, double d1m, double d2, double d2m, double d3, double d3m,
@@ -436,7 +432,7 @@ __global__ void __launch_bounds__(64 * NR) k_fused_formu(MainArgs<T> A, Geo g, i
             r0 = prhs[ro]; r1 = prhs[ro + 1]; r2 = prhs[ro + 2];
             if (buoy) { th = ptheta[qp]; thI = ptheta[qp + 1]; thJ = ptheta[qp + SJ]; thK = ptheta[qp + SK]; }
         }
-        const Vec3<T> F1dm = shfl_up3(F1.d), F1vm = shfl_up3(F1.v);
+        const Vec3<T> F1dm = shup3(F1.d), F1vm = shup3(F1.v);
         sF2[0][r][lane] = F2.d.x; sF2[1][r][lane] = F2.d.y; sF2[2][r][lane] = F2.d.z;
         sF2[3][r][lane] = F2.v.x; sF2[4][r][lane] = F2.v.y; sF2[5][r][lane] = F2.v.z;
         __syncthreads();
@@ -458,7 +454,7 @@ __global__ void __launch_bounds__(64 * NR) k_fused_formu(MainArgs<T> A, Geo g, i
             fp = V3{qq[0], qq[1], qq[2]};
         }
         // Rhs of the previous plane (t-1): fp(t-1) own / i+1 (shuffle) / j+1 (LDS), fp(t) own
-        V3 fpI = shfl_dn3(fpPrev);
+        V3 fpI = shdn3(fpPrev);
         const int rp = r + 1 < NR ? r + 1 : r;
         V3 fJ{sFp[buf ^ 1][0][rp][lane], sFp[buf ^ 1][1][rp][lane], sFp[buf ^ 1][2][rp][lane]};
         if (ownI) fpI = fpPrev;

@@ -25,23 +25,6 @@ constexpr int NQ = 28;   // 27 Germano quantities + the binomial-weighted volume
 constexpr int DSM_NR = 8;
 constexpr int DSM_TI = 62, DSM_TJ = DSM_NR - 2;
 
-// neighbour values across the lanes of a wave: shup gives lane l the value of lane l-1, shdn that of
-// lane l+1.  Lane 0 (shup) and lane 63 (shdn) have no neighbour and receive 0; both are halo
-// lanes whose own filtered values are never used, and their ucat neighbours are loaded.  The DPP
-// wavefront shifts (wave_shr:1, wave_shl:1) are VALU register moves; __shfl_up/down go through
-// the LDS crossbar (ds_bpermute) and cost about a fifth of this kernel's time.
-__device__ __forceinline__ int dpp_shr(int x) { return __builtin_amdgcn_mov_dpp(x, 0x138, 0xf, 0xf, true); }
-__device__ __forceinline__ int dpp_shl(int x) { return __builtin_amdgcn_mov_dpp(x, 0x130, 0xf, 0xf, true); }
-__device__ __forceinline__ double shup(double x) {
-    return __hiloint2double(dpp_shr(__double2hiint(x)), dpp_shr(__double2loint(x)));
-}
-__device__ __forceinline__ double shdn(double x) {
-    return __hiloint2double(dpp_shl(__double2hiint(x)), dpp_shl(__double2loint(x)));
-}
-__device__ __forceinline__ float shup(float x) { return __int_as_float(dpp_shr(__float_as_int(x))); }
-__device__ __forceinline__ float shdn(float x) { return __int_as_float(dpp_shl(__float_as_int(x))); }
-template <class T> __device__ __forceinline__ Vec3<T> shup3(Vec3<T> a) { return Vec3<T>{shup(a.x), shup(a.y), shup(a.z)}; }
-template <class T> __device__ __forceinline__ Vec3<T> shdn3(Vec3<T> a) { return Vec3<T>{shdn(a.x), shdn(a.y), shdn(a.z)}; }
 
 // sum of the 1-D Simpson weights (1, 4, 1) over the interior points c-1, c, c+1 of an axis of size m
 __device__ __forceinline__ double simpson_mask_sum(int c, int m) {
@@ -167,8 +150,8 @@ __global__ void __launch_bounds__(64 * DSM_NR) k_dsm_lm(Geo g, const T *ucat, co
             Sh[1][1] = F[18]; Sh[1][2] = Sh[2][1] = F[19]; Sh[2][2] = F[20];
             SSh[0][0] = F[21]; SSh[0][1] = SSh[1][0] = F[22]; SSh[0][2] = SSh[2][0] = F[23];
             SSh[1][1] = F[24]; SSh[1][2] = SSh[2][1] = F[25]; SSh[2][2] = F[26];
-            const double filter = pow(1. / aj[c], 1. / 3.);
-            const double test_filter = pow(8.0 * sH[27][r][lane], 1. / 3.);
+            const double filter = cube_root(1. / aj[c]);
+            const double test_filter = cube_root(8.0 * sH[27][r][lane]);
             double nm, dn;
             germano(L, Sh, SSh, filter, test_filter, ld3(csi, c), ld3(eta, c), ld3(zet, c), nm, dn);
             LM[c] = nm;
@@ -254,7 +237,7 @@ __global__ void __launch_bounds__(BX * BY) k_dsm_cs_nut(Geo g, const double *LM,
         const double cs = pmin(pmax(C, 0.0), maxCs);
         Cs[c] = cs;
         if (nut) {
-            const double filter = pow(1. / aj[c], 1. / 3.);
+            const double filter = cube_root(1. / aj[c]);
             nut[c] = T(cs * (filter * filter) * Sabs[c]);
         }
     }
@@ -270,8 +253,8 @@ __global__ void __launch_bounds__(256) k_nut_fused(Geo g, const T *ucat, const d
     const V3 cc = ld3(csi, c), ee = ld3(eta, c), zz = ld3(zet, c);
     const double a = aj[c];
     const double S = strain_mag(du_dxyz(cvt3<T>(cc), cvt3<T>(ee), cvt3<T>(zz), T(a), du_center(g, ucat, i, j, k)));
-    const double filter = pow(1. / a, 1. / 3.);
-    const double nt = Cs[c] * pow(filter, 2.0) * S;
+    const double filter = cube_root(1. / a);
+    const double nt = Cs[c] * (filter * filter) * S;      // pow(x, 2.) of les.c:2510 is x * x, exactly rounded
     nut[c] = T(nt);
     if (ksgs) {
         const double V = 1.0 / a;
@@ -279,7 +262,7 @@ __global__ void __launch_bounds__(256) k_nut_fused(Geo g, const T *ucat, const d
         const double dy = V / sqrt(cc.x * cc.x + cc.y * cc.y + cc.z * cc.z);
         const double dz = V / sqrt(ee.x * ee.x + ee.y * ee.y + ee.z * ee.z);
         const double d2 = 3.0 / (1.0 / (dx * dx) + 1.0 / (dy * dy) + 1.0 / (dz * dz));
-        ksgs[c] = T(pow(nt, 2.0) / (pow(0.094, 2.0) * d2));
+        ksgs[c] = T((nt * nt) / ((0.094 * 0.094) * d2));
     }
 }
 

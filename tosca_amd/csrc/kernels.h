@@ -9,6 +9,7 @@ namespace tosca {
 // ghost handling (ref_kernels.hip)
 template <class T> void launch_wrap(T *a, int ncomp, const Geo &g, hipStream_t s);
 void launch_reset_cell_periodic(double *a, int ncomp, const Geo &g, hipStream_t s);
+void launch_reset_face_periodic(double *a, int ncomp, const Geo &g, hipStream_t s);
 void launch_fill(double *a, long n, double v, hipStream_t s);
 template <class T> void launch_nut_bcs(T *nut, const tosca_params *p, const Geo &g, hipStream_t s);
 
@@ -21,6 +22,13 @@ void ref_update_nut(const Geo &g, const tosca_mesh *m, const tosca_params *p, co
                     double *nut, double *ksgs, hipStream_t s);
 void ref_buoyancy(const Geo &g, const tosca_mesh *m, const tosca_params *p, const double *theta, double scale,
                   double *btheta, double *gcont, hipStream_t s);
+
+// ExplicitRhsU steps around FormU (explicit_rhs.hip)
+void launch_no_penetration(double *ucont, const tosca_params *p, const Geo &g, hipStream_t s);
+void launch_contra_to_cart(const double *ucont, const tosca_mesh *m, double *ucat, const Geo &g, hipStream_t s);
+void launch_rhs_init(double *rhs, const double *dp, const double *bt, const double *bto, int ab2, const double *farm,
+                     const Geo &g, hipStream_t s);
+void launch_non_resolved(double *rhs, const double *ucont_o, const tosca_params *p, const Geo &g, hipStream_t s);
 
 // mesh validation (api.hip): counts of offending points, written to `out` (device, 4 ints)
 void launch_mesh_check(const Geo &g, const tosca_mesh *m, int *out, hipStream_t s);

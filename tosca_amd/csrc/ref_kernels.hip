@@ -8,7 +8,8 @@
 //                                                      k_plane_partial/k_plane_final/k_dpasd_cs
 //   UpdateNut   src/les.c:2119-2578 + boundary.c:2630-2929 -> k_nut, k_nut_bc
 //   Buoyancy    src/sources/ueqn_sources.c:2726-2880 -> k_buoyancy
-//   ghost scatters (PETSc DMLocalToLocal) and resetCellPeriodicFluxes -> k_wrap_axis, k_reset_axis
+//   ghost scatters (PETSc DMLocalToLocal), resetCellPeriodicFluxes and
+//   resetFacePeriodicFluxesVector -> k_wrap_axis, k_reset_axis
 #include "kernels.h"
 
 namespace tosca {
@@ -498,6 +499,19 @@ void launch_reset_cell_periodic(double *a, int ncomp, const Geo &g, hipStream_t 
         if (g.per[axis] == TOSCA_PERIODIC_NONE) continue;
         int t1 = (axis == 0) ? 1 : 0, t2 = (axis == 2) ? 1 : 2;
         TOSCA_LAUNCH("k_reset_axis", k_reset_axis, dim3((m[t1] + 127) / 128, m[t2], 2), dim3(128), 0, s, a, ncomp, g, axis);
+    }
+    launch_wrap(a, ncomp, g, s);
+}
+
+// resetFacePeriodicFluxesVector (src/include/inline.h:5357-5451): on each periodic axis the
+// points of index 0 take the values at m-2 (type 1; type 2 reads ghost -2, which holds m-2),
+// then the ghost scatter; the side-0 launch of k_reset_axis
+void launch_reset_face_periodic(double *a, int ncomp, const Geo &g, hipStream_t s) {
+    const int m[3] = {g.mx, g.my, g.mz};
+    for (int axis = 0; axis < 3; axis++) {
+        if (g.per[axis] == TOSCA_PERIODIC_NONE) continue;
+        int t1 = (axis == 0) ? 1 : 0, t2 = (axis == 2) ? 1 : 2;
+        TOSCA_LAUNCH("k_reset_axis", k_reset_axis, dim3((m[t1] + 127) / 128, m[t2], 1), dim3(128), 0, s, a, ncomp, g, axis);
     }
     launch_wrap(a, ncomp, g, s);
 }

@@ -314,7 +314,7 @@ __global__ void __launch_bounds__(256) k_vjp_nut(VjpArgs A, Geo g) {
     const double a = A.aj[c];
     const Cart d = du_dxyz(cc, ee, zz, a, du_center(g, A.ucat, i, j, k));
     const double S = strain_mag(d);
-    const double filter = pow(1. / a, 1. / 3.), d2 = filter * filter;
+    const double filter = cube_root(1. / a), d2 = filter * filter;
     A.csbar[c] = nb * d2 * S;
     V3 gc{0, 0, 0}, ge{0, 0, 0}, gz{0, 0, 0};
     if (S != 0.0 && nb != 0.0) {                   // |S| is not differentiable at 0: subgradient 0

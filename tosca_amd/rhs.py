@@ -120,6 +120,18 @@ class RhsContext:
                                            _ptr(cs), _ptr(rbar), float(scale), float(buoy_scale), _ptr(ucont_bar),
                                            _ptr(ucat_bar), _ptr(theta_bar), _ptr(cs_bar)))
 
+    def ExplicitRhsU(self, mesh, params, ucont, ucat, nut, dp, btheta, btheta_o, ab2, farm, ucont_o, rhs,
+                     form_mode: int = 0):
+        """ExplicitRhsU(ueqn, formMode) (src/solvers/ueqn_solvers.c:1276-1407): no-penetration and
+        periodic resets of ucont (in place), ucat from ucont (in place), then rhs (overwritten) =
+        -dp + bTheta terms + farm + FormU, with the non-resolved faces reset.  btheta / farm None:
+        the term is off; ab2: the second-order bTheta extrapolation (clock->it > itStart)."""
+        self._stream()
+        m, p = mesh_struct(mesh), abi.make_params(params)
+        self._check(self.lib.tosca_explicit_rhs_u(self.h, C.byref(m), C.byref(p), _ptr(ucont), _ptr(ucat), _ptr(nut),
+                                                  _ptr(dp), _ptr(btheta), _ptr(btheta_o), int(ab2), _ptr(farm),
+                                                  _ptr(ucont_o), _ptr(rhs), int(form_mode)))
+
     # ---- instrumentation -----------------------------------------------------------------------
     def enable_timing(self, on: bool = True):
         self._check(self.lib.tosca_ctx_enable_timing(self.h, int(on)))
