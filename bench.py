@@ -188,6 +188,52 @@ def vjp_leg(torch, ctx, mesh, case, ucont, ucat, theta, cs, lay, args, step_ms):
             "forward": "UpdateNut + FormU + Buoyancy (FP64) with the step's Cs"}
 
 
+# ExplicitRhsU algorithmic bytes per cell: contravariant-to-Cartesian (ucont 24 + cell metrics 72
+# in, ucat 24 out), Rhs assembly (dP, bTheta, bTheta_o, farm 96 in, Rhs 24 out), FormU without
+# Buoyancy (ucont 24, ucat 24, nu_t 8, face metrics 216, face Jacobians 24, cell metrics 72, Rhs 48
+# read + write); the boundary-patch resets touch planes only
+EXPLICIT_BYTES = (24 + 72 + 24) + (96 + 24) + (24 + 24 + 8 + 216 + 24 + 72 + 48)   # = 656
+
+
+def explicit_leg(torch, ctx, mesh, case, ucont, ucat, nut, lay, args):
+    """ExplicitRhsU (tosca_explicit_rhs_u) with every source term on: the caller of the path.
+    It resets ucont and recomputes ucat in place, so it runs on copies of the step's fields."""
+    dev = "cuda"
+    gl = (lay.mz, lay.my, lay.mx, 3)
+    g = torch.Generator(device=dev).manual_seed(11)
+    dp, bt, bto, farm, uo = (torch.randn(gl, dtype=torch.float64, device=dev, generator=g) for _ in range(5))
+    V, U = ucont.clone(), ucat.clone()
+    rhs = torch.empty(gl, dtype=torch.float64, device=dev)
+
+    def run():
+        ctx.ExplicitRhsU(mesh, case.params, V, U, nut, dp, bt, bto, 1, farm, uo, rhs, 0)
+
+    for _ in range(args.warmup):
+        run()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        run()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    ctx.enable_timing(True)
+    ctx.reset_timing()
+    for _ in range(args.steps):
+        run()
+    tim = ctx.timings()
+    ctx.enable_timing(False)
+    value = lay.ncells / (ms * 1e-3)
+    gbs = EXPLICIT_BYTES * value / 1e9
+    print(f"# explicit_rhs_u: {ms:.3f} ms, {value:.3e} cells/s, {gbs:.0f} GB/s algorithmic", file=sys.stderr, flush=True)
+    del dp, bt, bto, farm, uo, V, U, rhs
+    return {"ms": ms, "value": value, "unit": "cell-updates/s", "algorithmic_bytes_per_cell": EXPLICIT_BYTES,
+            "achieved": gbs, "peak": HBM_PEAK_GBS, "frac": gbs / HBM_PEAK_GBS,
+            "kernels_ms": {k: v[0] / v[1] for k, v in tim.items()},
+            "terms": "-dP + 3/2 bTheta - 1/2 bTheta_o + farm + FormU(formMode 0), synthetic sources"}
+
+
 def workload_name(cfg: int, case) -> str:
     lay = case.lay
     n = f"{lay.mz - 2}x{lay.mx - 2}x{lay.my - 2}"
@@ -340,6 +386,7 @@ def main():
     }
     line["mixed"] = mixed_leg(torch, ctx, mesh, case, ucont, ucat, theta, lay, args, ms)
     line["vjp"] = vjp_leg(torch, ctx, mesh, case, ucont, ucat, theta, cs, lay, args, ms)
+    line["explicit_rhs_u"] = explicit_leg(torch, ctx, mesh, case, ucont, ucat, nut, lay, args)
     if not args.no_cpu:
         spec = C.CONFIGS[args.config]
         line["cpu_baseline"] = cpu_baseline((spec["builder"], spec["kwargs"]), args.cpu_seconds)

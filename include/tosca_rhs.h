@@ -196,7 +196,8 @@ int tosca_rhs_vjp(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_params *pa
                   const double *rbar, double scale, double buoy_scale,
                   double *ucont_bar, double *ucat_bar, double *theta_bar, double *cs_bar);
 
-/* ExplicitRhsU (src/solvers/ueqn_solvers.c:1276-1407), the immediate caller of FormU, with the
+/* ExplicitRhsU (src/solvers/ueqn_solvers.h:24; body src/solvers/ueqn_solvers.c:1276-1407), the
+ * immediate caller of FormU, with the
  * terms this library covers, in the reference's order:
  *   resetNoPenetrationFluxes (src/include/inline.h:5262-5355) on ucont: the normal flux at noSlip,
  *     velocityWallFunction and slip patches set to 0, at zeroGradient patches clipped to outflow;
