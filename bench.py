@@ -89,9 +89,15 @@ def copy_peak_gbs(torch, nbytes: int = 1 << 31, reps: int = 10) -> float:
     return 2 * nbytes / (ms * 1e-3) / 1e9
 
 
+F32_METRICS = ("csi", "eta", "zet", "icsi", "ieta", "izet", "jcsi", "jeta", "jzet", "kcsi", "keta", "kzet",
+               "iaj", "jaj", "kaj")
+
+
 def mixed_leg(torch, ctx, mesh, case, ucont, ucat, theta, lay, args, fp64_ms):
-    """The same step with FP32 flow fields (tosca_rhs_les_step_mixed): throughput, speed-up over the
-    FP64 step, and per-field errors against the FP64 step on the original FP64 inputs."""
+    """The same step with FP32 flow fields: tosca_rhs_les_step_mixed (FP64 metrics) and
+    tosca_rhs_les_step_mixed_m32 (FormU on FP32 metric copies, made once before the timing, as the
+    mesh is static).  Throughput, speed-up over the FP64 step, and per-field errors of one step
+    against the FP64 step on the original FP64 inputs."""
     dev = "cuda"
     V32, u32, t32 = ucont.float().contiguous(), ucat.float().contiguous(), theta.float().contiguous()
     gs, gl = lay.gshape, (lay.mz, lay.my, lay.mx, 3)
@@ -99,48 +105,59 @@ def mixed_leg(torch, ctx, mesh, case, ucont, ucat, theta, lay, args, fp64_ms):
     bt = torch.zeros(gl, dtype=torch.float64, device=dev)
     cs = torch.zeros(gs, dtype=torch.float64, device=dev)
     nut = torch.zeros(gs, dtype=torch.float32, device=dev)
-
-    def step():
-        ctx.rhs_les_step_mixed(mesh, case.params, V32, u32, t32, rhs, 1.0, cs, nut, bt, 1.0)
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(args.steps):
-        step()
-    e1.record()
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / args.steps
-    ctx.enable_timing(True)
-    ctx.reset_timing()
-    for _ in range(args.steps):
-        step()
-    tim = ctx.timings()
-    ctx.enable_timing(False)
-    kms = {k: v[0] / v[1] for k, v in tim.items()}
-    # errors of one step from rhs = 0, against the FP64 step on the FP64 inputs
     rhs64 = torch.zeros(gl, dtype=torch.float64, device=dev)
     bt64 = torch.zeros_like(rhs64)
     cs64 = torch.zeros(gs, dtype=torch.float64, device=dev)
     nut64 = torch.zeros(gs, dtype=torch.float64, device=dev)
     ctx.rhs_les_step(mesh, case.params, ucont, ucat, theta, rhs64, 1.0, cs64, nut64, bt64, 1.0)
-    rhs.zero_()
-    step()
-    torch.cuda.synchronize()
-    err = {}
-    for k, a, b in (("rhs", rhs, rhs64), ("cs", cs, cs64), ("nut", nut.double(), nut64), ("btheta", bt, bt64)):
-        d = (a - b).double()
-        nb = float(torch.linalg.vector_norm(b))
-        err[k] = {"rel_l2": float(torch.linalg.vector_norm(d)) / max(nb, 1e-300),
-                  "max_rel": float(d.abs().max()) / max(float(b.abs().max()), 1e-300),
-                  "rms_rel": float(d.pow(2).mean().sqrt()) / max(float(b.pow(2).mean().sqrt()), 1e-300)}
+
+    def measure(step):
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(args.steps):
+            step()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / args.steps
+        ctx.enable_timing(True)
+        ctx.reset_timing()
+        for _ in range(args.steps):
+            step()
+        tim = ctx.timings()
+        ctx.enable_timing(False)
+        rhs.zero_()
+        step()
+        torch.cuda.synchronize()
+        err = {}
+        for k, a, b in (("rhs", rhs, rhs64), ("cs", cs, cs64), ("nut", nut.double(), nut64), ("btheta", bt, bt64)):
+            d = (a - b).double()
+            nb = float(torch.linalg.vector_norm(b))
+            err[k] = {"rel_l2": float(torch.linalg.vector_norm(d)) / max(nb, 1e-300),
+                      "max_rel": float(d.abs().max()) / max(float(b.abs().max()), 1e-300),
+                      "rms_rel": float(d.pow(2).mean().sqrt()) / max(float(b.pow(2).mean().sqrt()), 1e-300)}
+        return ms, {k: v[0] / v[1] for k, v in tim.items()}, err
+
+    out = {}
+    ms, kms, err = measure(lambda: ctx.rhs_les_step_mixed(mesh, case.params, V32, u32, t32, rhs, 1.0, cs, nut, bt, 1.0))
     value = lay.ncells / (ms * 1e-3)
     print(f"# mixed: {ms:.3f} ms/step, {value:.3e} cells/s, x{fp64_ms / ms:.2f} vs FP64; rel L2 errors "
           + ", ".join(f"{k} {v['rel_l2']:.1e}" for k, v in err.items()), file=sys.stderr, flush=True)
-    return {"value": value, "unit": "cell-updates/s", "ms_per_step": ms, "speedup_vs_fp64": fp64_ms / ms,
-            "dtype": "f32 flow fields / f64 metrics and reductions", "errors_vs_fp64": err, "kernels_ms": kms}
+    out.update({"value": value, "unit": "cell-updates/s", "ms_per_step": ms, "speedup_vs_fp64": fp64_ms / ms,
+                "dtype": "f32 flow fields / f64 metrics and reductions", "errors_vs_fp64": err, "kernels_ms": kms})
+    mesh32 = {k: mesh[k].float().contiguous() for k in F32_METRICS}
+    ms, kms, err = measure(lambda: ctx.rhs_les_step_mixed_m32(mesh, mesh32, case.params, V32, u32, t32, rhs, 1.0, cs,
+                                                              nut, bt, 1.0))
+    value = lay.ncells / (ms * 1e-3)
+    print(f"# mixed, FP32 metric copies: {ms:.3f} ms/step, {value:.3e} cells/s, x{fp64_ms / ms:.2f} vs FP64; rel L2 "
+          "errors " + ", ".join(f"{k} {v['rel_l2']:.1e}" for k, v in err.items()), file=sys.stderr, flush=True)
+    out["fp32_metrics"] = {"value": value, "unit": "cell-updates/s", "ms_per_step": ms, "speedup_vs_fp64": fp64_ms / ms,
+                           "dtype": "f32 flow fields, f32 FormU metric copies / f64 LES metrics and reductions",
+                           "errors_vs_fp64": err, "kernels_ms": kms}
+    del mesh32
+    return out
 
 
 def vjp_leg(torch, ctx, mesh, case, ucont, ucat, theta, cs, lay, args, step_ms):

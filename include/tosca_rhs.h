@@ -181,6 +181,27 @@ int tosca_rhs_les_step_mixed(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca
                              double *rhs, double scale, double *cs, float *nut,
                              double *btheta, double buoy_scale);
 
+/* FP32 copies of the metrics FormU reads, for tosca_rhs_les_step_mixed_m32 (no reference
+ * counterpart).  Same layout as the FP64 fields of tosca_mesh (local AoS vectors, local scalars).
+ * The mesh is static in the reference (SetMeshMetrics runs once), so the copies are made once. */
+typedef struct {
+    const float *csi, *eta, *zet;
+    const float *icsi, *ieta, *izet;
+    const float *jcsi, *jeta, *jzet;
+    const float *kcsi, *keta, *kzet;
+    const float *iaj, *jaj, *kaj;
+} tosca_mesh_f32;
+
+/* tosca_rhs_les_step_mixed with FormU reading the FP32 metric copies of mesh32 instead of the FP64
+ * metrics (no reference counterpart).  FormU's flux arithmetic already runs in FP32 on FP32-rounded
+ * face metrics; here the face dot into rhs and the buoyancy source also see FP32-rounded metrics
+ * and Jacobians (arithmetic still FP64).  UpdateCs / UpdateNut read the FP64 mesh.  FormU's
+ * metric bytes halve (312 -> 156 B/cell). */
+int tosca_rhs_les_step_mixed_m32(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_mesh_f32 *mesh32,
+                                 const tosca_params *params, const float *ucont, const float *ucat,
+                                 const float *theta, double *rhs, double scale, double *cs, float *nut,
+                                 double *btheta, double buoy_scale);
+
 /* Adjoint (vector-Jacobian product) of the right-hand side with respect to velocity, temperature
  * and Cs (no reference counterpart: the reference has no adjoint).  The function is
  *   R(ucont, ucat, theta, cs) = FormU(ucont, ucat, nu_t, scale, form_mode 0) + Buoyancy(theta, buoy_scale),

@@ -18,7 +18,7 @@ HEADER = ROOT / "include" / "tosca_rhs.h"
 def declared_functions():
     text = HEADER.read_text()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(tosca_[a-z_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(tosca_[a-z0-9_]+)\s*\(", text)))
 
 
 def test_header_declares_expected_api():

@@ -96,3 +96,55 @@ def test_mixed_rejects_unsupported_model(rhs_mod):
     with pytest.raises(ToscaError):
         ctx.rhs_les_step_mixed(mesh, case.params, f(case.ucont), f(case.ucat), f(case.theta), rhs, 1.0, cs, nut,
                                rhs.clone(), 1.0)
+
+
+F32_FIELDS = ("csi", "eta", "zet", "icsi", "ieta", "izet", "jcsi", "jeta", "jzet", "kcsi", "keta", "kzet",
+              "iaj", "jaj", "kaj")
+
+
+@pytest.mark.parametrize("name,builder", CASES, ids=[c[0] for c in CASES])
+def test_mixed_step_fp32_metrics(rhs_mod, name, builder):
+    """tosca_rhs_les_step_mixed_m32: FormU (and the fused Buoyancy) read FP32 metric copies.
+    Arithmetic reference: the oracle's UpdateCs / UpdateNut on the FP64 mesh and FP32-rounded
+    velocity, its Buoyancy and FormU on the FP32-rounded metrics; bar 1e-5 as for the mixed step."""
+    case = builder()
+    lay = case.lay
+    print(f"\nmixed step, FP32 metrics: {name} {lay}", flush=True)
+    nm = np_mesh(case.mesh)
+    nm_r = dict(nm)
+    for k in F32_FIELDS:
+        nm_r[k] = nm[k].astype(np.float32).astype(np.float64)
+    u64, V64, t64 = to_np(case.ucat), to_np(case.ucont), to_np(case.theta)
+    u, V, t = (a.astype(np.float32).astype(np.float64) for a in (u64, V64, t64))
+    info, prm = lay.info(), case.params
+    cs = np.zeros(lay.gshape); nut = np.zeros(lay.gshape)
+    O.update_cs(info, nm, prm, u, cs)
+    O.update_nut(info, nm, prm, u, cs, nut)
+    bt = np.zeros((lay.mz, lay.my, lay.mx, 3))
+    O.buoyancy(info, nm_r, prm, t, 1.0, bt)
+    rhs = np.zeros((lay.mz, lay.my, lay.mx, 3))
+    O.formu(info, nm_r, prm, V, u, nut, rhs, 1.0, 0)
+    ref_arith = dict(rhs=rhs, cs=cs, nut=nut, btheta=bt)
+    ref_total = _oracle(case, V64, u64, t64)
+
+    ctx = rhs_mod.RhsContext(info)
+    mesh = {k: to_dev(v) for k, v in nm.items()}
+    mesh32 = {k: torch.as_tensor(nm[k].astype(np.float32)).to("cuda") for k in F32_FIELDS}
+    dev32 = lambda a: torch.as_tensor(a.astype(np.float32)).to("cuda")
+    rhs_g = torch.zeros((lay.mz, lay.my, lay.mx, 3), dtype=torch.float64, device="cuda")
+    bt_g = torch.zeros_like(rhs_g)
+    cs_g = torch.zeros(lay.gshape, dtype=torch.float64, device="cuda")
+    nut_g = torch.zeros(lay.gshape, dtype=torch.float32, device="cuda")
+    ctx.rhs_les_step_mixed_m32(mesh, mesh32, prm, dev32(V64), dev32(u64), dev32(t64), rhs_g, 1.0, cs_g, nut_g, bt_g,
+                               1.0)
+    torch.cuda.synchronize()
+    got = dict(rhs=to_np(rhs_g), cs=to_np(cs_g), nut=to_np(nut_g).astype(np.float64), btheta=to_np(bt_g))
+    worst = 0.0
+    for k in ("rhs", "cs", "nut", "btheta"):
+        ea, et = relerr(got[k], ref_arith[k]), relerr(got[k], ref_total[k])
+        print(f"  {k:7s} arithmetic relL2={ea:.2e} max={maxrel(got[k], ref_arith[k]):.2e} "
+              f"rms={_rms_rel(got[k], ref_arith[k]):.2e} | total relL2={et:.2e} "
+              f"max={maxrel(got[k], ref_total[k]):.2e}", flush=True)
+        worst = max(worst, ea)
+        assert et <= TOTAL_TOL, (k, et)
+    assert worst <= ARITH_TOL

@@ -34,6 +34,15 @@ class Mesh(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in MESH_FIELDS]
 
 
+# FP32 metric copies read by FormU on the mixed path (tosca_mesh_f32)
+MESH_F32_FIELDS = ("csi", "eta", "zet", "icsi", "ieta", "izet", "jcsi", "jeta", "jzet", "kcsi", "keta", "kzet",
+                   "iaj", "jaj", "kaj")
+
+
+class MeshF32(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in MESH_F32_FIELDS]
+
+
 class Params(C.Structure):
     _fields_ = [
         ("periodic", C.c_int * 3),
@@ -72,6 +81,8 @@ SIGNATURES = {
     "tosca_rhs_les_step": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P, P, D, P, P, P, D]),
     "tosca_rhs_les_step_mixed": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P, P, D, P, P, P, D]),
     "tosca_rhs_vjp": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P, P, P, D, D, P, P, P, P]),
+    "tosca_rhs_les_step_mixed_m32": (I, [P, C.POINTER(Mesh), C.POINTER(MeshF32), C.POINTER(Params), P, P, P, P, D, P, P,
+                                         P, D]),
     "tosca_explicit_rhs_u": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P, P, P, P, I, P, P, P, I]),
 }
 
@@ -128,5 +139,12 @@ def make_params(d: Mapping) -> Params:
 def make_mesh(ptrs: Mapping[str, int]) -> Mesh:
     m = Mesh()
     for n in MESH_FIELDS:
+        setattr(m, n, ptrs.get(n) or None)
+    return m
+
+
+def make_mesh_f32(ptrs: Mapping[str, int]) -> MeshF32:
+    m = MeshF32()
+    for n in MESH_F32_FIELDS:
         setattr(m, n, ptrs.get(n) or None)
     return m

@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <iterator>
 #include <vector>
+#include <type_traits>
 
 #include "fused.h"
 
@@ -187,9 +188,10 @@ size_t ws_need(const tosca_ctx *c) {
 
 // The fused UpdateCs -> UpdateNut -> Buoyancy -> FormU sequence on the context's stream, for
 // double (FP64 path) or float (mixed path) flow fields; arguments are validated by the caller.
-template <class T>
+template <class T, class MT = double>
 void queue_step(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const T *ucont, const T *ucat,
-                const T *theta, double *rhs, double scale, double *cs, T *nut, double *btheta, double buoy_scale) {
+                const T *theta, double *rhs, double scale, double *cs, T *nut, double *btheta, double buoy_scale,
+                const tosca_mesh_f32 *m32 = nullptr) {
     const Geo &g = c->geo;
     hipStream_t s = c->stream;
     // UpdateCs (+ the interior of UpdateNut)
@@ -211,13 +213,21 @@ void queue_step(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const 
     tosca::zero_outside_interior(nut, g, s);
     tosca::nut_bcs(nut, p, g, s);
     // Buoyancy + FormU
-    tosca::FormUArgsT<T> A{};
+    tosca::FormUArgsT<T, MT> A{};
     A.ucont = ucont; A.ucat = ucat; A.nut = nut;
-    A.csi = m->csi; A.eta = m->eta; A.zet = m->zet;
-    A.icsi = m->icsi; A.ieta = m->ieta; A.izet = m->izet;
-    A.jcsi = m->jcsi; A.jeta = m->jeta; A.jzet = m->jzet;
-    A.kcsi = m->kcsi; A.keta = m->keta; A.kzet = m->kzet;
-    A.iaj = m->iaj; A.jaj = m->jaj; A.kaj = m->kaj;
+    if constexpr (std::is_same<MT, float>::value) {
+        A.csi = m32->csi; A.eta = m32->eta; A.zet = m32->zet;
+        A.icsi = m32->icsi; A.ieta = m32->ieta; A.izet = m32->izet;
+        A.jcsi = m32->jcsi; A.jeta = m32->jeta; A.jzet = m32->jzet;
+        A.kcsi = m32->kcsi; A.keta = m32->keta; A.kzet = m32->kzet;
+        A.iaj = m32->iaj; A.jaj = m32->jaj; A.kaj = m32->kaj;
+    } else {
+        A.csi = m->csi; A.eta = m->eta; A.zet = m->zet;
+        A.icsi = m->icsi; A.ieta = m->ieta; A.izet = m->izet;
+        A.jcsi = m->jcsi; A.jeta = m->jeta; A.jzet = m->jzet;
+        A.kcsi = m->kcsi; A.keta = m->keta; A.kzet = m->kzet;
+        A.iaj = m->iaj; A.jaj = m->jaj; A.kaj = m->kaj;
+    }
     A.tauI = p->tau_wall_i; A.tauJ = p->tau_wall_j;
     A.rhs = rhs; A.scale = scale; A.mode = 0;
     A.quick = p->div_scheme == TOSCA_DIV_QUICK;
@@ -228,7 +238,37 @@ void queue_step(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const 
     A.btheta = btheta;
     A.bscale = buoy_scale;
     A.tRef = p->t_ref;
-    tosca::fused_formu<T>(g, A, c->ws + 3 * (size_t)g.nloc(), s);
+    tosca::fused_formu<T, MT>(g, A, c->ws + 3 * (size_t)g.nloc(), s);
+}
+
+// validation shared by the two mixed-precision steps (FP32 flow fields)
+int check_mixed(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const float *ucont, const float *ucat,
+                const float *theta, double *rhs, double *cs, float *nut, double *btheta) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if ((rc = check_params(c, p))) return rc;
+    if (!p->les_active) return set_err(c, TOSCA_ERR_ARG, "tosca_rhs_les_step_mixed needs les_active = 1");
+    if (p->les_model != TOSCA_LES_SMAGORINSKY && p->les_model != TOSCA_LES_DSM)
+        return set_err(c, TOSCA_ERR_UNSUPPORTED, "tosca_rhs_les_step_mixed: LES model %d (supported: Smagorinsky 0, "
+                                                 "dynamic Smagorinsky 2)", p->les_model);
+    if (p->div_scheme != TOSCA_DIV_CENTRAL && p->div_scheme != TOSCA_DIV_QUICK)
+        return set_err(c, TOSCA_ERR_UNSUPPORTED, "div_scheme = %d: only central (0) and QUICK (1) are supported", p->div_scheme);
+    if (!(p->t_ref != 0.0)) return set_err(c, TOSCA_ERR_ARG, "t_ref must be non-zero");
+    set_geo_per(c, p);
+    const Geo &g = c->geo;
+    const size_t n = (size_t)g.nloc(), v = 3 * n * sizeof(float), ng = 3 * (size_t)g.mx * g.my * g.mz * sizeof(double);
+    if ((rc = check_mesh(c, m, true, true, true))) return rc;
+    if ((rc = check_dev(c, ucont, v, "ucont")) || (rc = check_dev(c, ucat, v, "ucat")) ||
+        (rc = check_dev(c, theta, n * sizeof(float), "theta")) || (rc = check_dev(c, rhs, ng, "rhs")) ||
+        (rc = check_dev(c, cs, n * sizeof(double), "cs")) || (rc = check_dev(c, nut, n * sizeof(float), "nut")) ||
+        (rc = check_dev(c, btheta, ng, "btheta")))
+        return rc;
+    if (p->bc_u[0] == TOSCA_BCU_WALLFUNCTION || p->bc_u[1] == TOSCA_BCU_WALLFUNCTION)
+        if ((rc = check_dev(c, p->tau_wall_i, 3 * (size_t)g.mz * g.my * sizeof(double), "tau_wall_i"))) return rc;
+    if (p->bc_u[2] == TOSCA_BCU_WALLFUNCTION || p->bc_u[3] == TOSCA_BCU_WALLFUNCTION)
+        if ((rc = check_dev(c, p->tau_wall_j, 3 * (size_t)g.mz * g.mx * sizeof(double), "tau_wall_j"))) return rc;
+    if ((rc = ensure_ws(c, ws_need(c)))) return rc;
+    return TOSCA_OK;
 }
 
 }  // namespace
@@ -550,32 +590,30 @@ int tosca_rhs_les_step(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p,
 int tosca_rhs_les_step_mixed(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const float *ucont,
                              const float *ucat, const float *theta, double *rhs, double scale, double *cs, float *nut,
                              double *btheta, double buoy_scale) {
-    int rc = check_ctx(c);
+    int rc = check_mixed(c, m, p, ucont, ucat, theta, rhs, cs, nut, btheta);
     if (rc) return rc;
-    if ((rc = check_params(c, p))) return rc;
-    if (!p->les_active) return set_err(c, TOSCA_ERR_ARG, "tosca_rhs_les_step_mixed needs les_active = 1");
-    if (p->les_model != TOSCA_LES_SMAGORINSKY && p->les_model != TOSCA_LES_DSM)
-        return set_err(c, TOSCA_ERR_UNSUPPORTED, "tosca_rhs_les_step_mixed: LES model %d (supported: Smagorinsky 0, "
-                                                 "dynamic Smagorinsky 2)", p->les_model);
-    if (p->div_scheme != TOSCA_DIV_CENTRAL && p->div_scheme != TOSCA_DIV_QUICK)
-        return set_err(c, TOSCA_ERR_UNSUPPORTED, "div_scheme = %d: only central (0) and QUICK (1) are supported", p->div_scheme);
-    if (!(p->t_ref != 0.0)) return set_err(c, TOSCA_ERR_ARG, "t_ref must be non-zero");
-    set_geo_per(c, p);
-    const Geo &g = c->geo;
-    const size_t n = (size_t)g.nloc(), v = 3 * n * sizeof(float), ng = 3 * (size_t)g.mx * g.my * g.mz * sizeof(double);
-    if ((rc = check_mesh(c, m, true, true, true))) return rc;
-    if ((rc = check_dev(c, ucont, v, "ucont")) || (rc = check_dev(c, ucat, v, "ucat")) ||
-        (rc = check_dev(c, theta, n * sizeof(float), "theta")) || (rc = check_dev(c, rhs, ng, "rhs")) ||
-        (rc = check_dev(c, cs, n * sizeof(double), "cs")) || (rc = check_dev(c, nut, n * sizeof(float), "nut")) ||
-        (rc = check_dev(c, btheta, ng, "btheta")))
-        return rc;
-    if (p->bc_u[0] == TOSCA_BCU_WALLFUNCTION || p->bc_u[1] == TOSCA_BCU_WALLFUNCTION)
-        if ((rc = check_dev(c, p->tau_wall_i, 3 * (size_t)g.mz * g.my * sizeof(double), "tau_wall_i"))) return rc;
-    if (p->bc_u[2] == TOSCA_BCU_WALLFUNCTION || p->bc_u[3] == TOSCA_BCU_WALLFUNCTION)
-        if ((rc = check_dev(c, p->tau_wall_j, 3 * (size_t)g.mz * g.mx * sizeof(double), "tau_wall_j"))) return rc;
-    if ((rc = ensure_ws(c, ws_need(c)))) return rc;
     queue_step<float>(c, m, p, ucont, ucat, theta, rhs, scale, cs, nut, btheta, buoy_scale);
     return launched(c, "rhs_les_step_mixed");
+}
+
+int tosca_rhs_les_step_mixed_m32(tosca_ctx *c, const tosca_mesh *m, const tosca_mesh_f32 *m32, const tosca_params *p,
+                                 const float *ucont, const float *ucat, const float *theta, double *rhs, double scale,
+                                 double *cs, float *nut, double *btheta, double buoy_scale) {
+    int rc = check_mixed(c, m, p, ucont, ucat, theta, rhs, cs, nut, btheta);
+    if (rc) return rc;
+    if (!m32) return set_err(c, TOSCA_ERR_ARG, "mesh32 is NULL");
+    const size_t n = (size_t)c->geo.nloc(), v = 3 * n * sizeof(float), sc = n * sizeof(float);
+    const float *vv[12] = {m32->csi, m32->eta, m32->zet, m32->icsi, m32->ieta, m32->izet,
+                           m32->jcsi, m32->jeta, m32->jzet, m32->kcsi, m32->keta, m32->kzet};
+    const char *nm[12] = {"mesh32.csi", "mesh32.eta", "mesh32.zet", "mesh32.icsi", "mesh32.ieta", "mesh32.izet",
+                          "mesh32.jcsi", "mesh32.jeta", "mesh32.jzet", "mesh32.kcsi", "mesh32.keta", "mesh32.kzet"};
+    for (int q = 0; q < 12; q++)
+        if ((rc = check_dev(c, vv[q], v, nm[q]))) return rc;
+    if ((rc = check_dev(c, m32->iaj, sc, "mesh32.iaj")) || (rc = check_dev(c, m32->jaj, sc, "mesh32.jaj")) ||
+        (rc = check_dev(c, m32->kaj, sc, "mesh32.kaj")))
+        return rc;
+    queue_step<float, float>(c, m, p, ucont, ucat, theta, rhs, scale, cs, nut, btheta, buoy_scale, m32);
+    return launched(c, "rhs_les_step_mixed_m32");
 }
 
 int tosca_rhs_vjp(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const double *ucont, const double *ucat,

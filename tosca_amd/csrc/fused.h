@@ -10,12 +10,13 @@ namespace tosca {
 
 // T is the type of the flow fields (ucont, ucat, nut, theta): double on the FP64 path, float on
 // the mixed path.  Metrics, Jacobians, the wall stresses and the outputs are always double.
-template <class T>
+// flow type T; metric storage type MT (double, or the FP32 metric copies of the mixed path)
+template <class T, class MT = double>
 struct FormUArgsT {
     const T *ucont, *ucat, *nut;
-    const double *csi, *eta, *zet;
-    const double *icsi, *ieta, *izet, *jcsi, *jeta, *jzet, *kcsi, *keta, *kzet;
-    const double *iaj, *jaj, *kaj;
+    const MT *csi, *eta, *zet;
+    const MT *icsi, *ieta, *izet, *jcsi, *jeta, *jzet, *kcsi, *keta, *kzet;
+    const MT *iaj, *jaj, *kaj;
     const double *tauI, *tauJ;
     double *rhs;
     double scale;
@@ -31,8 +32,8 @@ using FormUArgs = FormUArgsT<double>;
 
 // FormU (+ Buoyancy when A.btheta != nullptr); ws holds formu_ws_doubles(g) doubles for the
 // boundary-face fluxes and periodic-seam fp.  Instantiated for double and float flow fields.
-template <class T>
-void fused_formu(const Geo &g, const FormUArgsT<T> &A, double *ws, hipStream_t s);
+template <class T, class MT = double>
+void fused_formu(const Geo &g, const FormUArgsT<T, MT> &A, double *ws, hipStream_t s);
 size_t formu_ws_doubles(const Geo &g);
 
 // Dynamic Smagorinsky: LM, MM and |S| of every interior cell into ws (3 local scalar fields),

@@ -28,8 +28,8 @@ template <class T> __device__ __forceinline__ Vec3<T> add3(Vec3<T> a, Vec3<T> b)
 
 // convective (div) and viscous (visc) flux of one face.  dir: 0 = i face (k,j,i)|(k,j,i+1),
 // 1 = j face, 2 = k face.  (i,j,k) is the face index after the periodic remap.
-template <int DIR, class T>
-__device__ __forceinline__ FluxT<T> face_flux(const FormUArgsT<T> &A, const Geo &g, int i, int j, int k) {
+template <int DIR, class T, class MT>
+__device__ __forceinline__ FluxT<T> face_flux(const FormUArgsT<T, MT> &A, const Geo &g, int i, int j, int k) {
     const int c = g.L(k, j, i);
     const T *u = A.ucat;
     FluxT<T> r;
@@ -70,18 +70,19 @@ __device__ __forceinline__ FluxT<T> face_flux(const FormUArgsT<T> &A, const Geo 
                 nuEff = T(A.nu) + nt;
             }
         }
-        const double *fc = (DIR == 0) ? A.icsi : (DIR == 1) ? A.jcsi : A.kcsi;
-        const double *fe = (DIR == 0) ? A.ieta : (DIR == 1) ? A.jeta : A.keta;
-        const double *fz = (DIR == 0) ? A.izet : (DIR == 1) ? A.jzet : A.kzet;
-        const double *fa = (DIR == 0) ? A.iaj : (DIR == 1) ? A.jaj : A.kaj;
+        const MT *fc = (DIR == 0) ? A.icsi : (DIR == 1) ? A.jcsi : A.kcsi;
+        const MT *fe = (DIR == 0) ? A.ieta : (DIR == 1) ? A.jeta : A.keta;
+        const MT *fz = (DIR == 0) ? A.izet : (DIR == 1) ? A.jzet : A.kzet;
+        const MT *fa = (DIR == 0) ? A.iaj : (DIR == 1) ? A.jaj : A.kaj;
         r.v = add3(base, visc_flux<T>(gr, cvt3<T>(ld3(fc, c)), cvt3<T>(ld3(fe, c)), cvt3<T>(ld3(fz, c)), DIR, T(fa[c]), nuEff));
     }
     return r;
 }
 
-__device__ __forceinline__ V3 ldnt3(const double *p, int c) {
-    return V3{__builtin_nontemporal_load(p + 3 * c), __builtin_nontemporal_load(p + 3 * c + 1),
-              __builtin_nontemporal_load(p + 3 * c + 2)};
+template <class MT>
+__device__ __forceinline__ V3 ldnt3(const MT *p, int c) {
+    return V3{double(__builtin_nontemporal_load(p + 3 * c)), double(__builtin_nontemporal_load(p + 3 * c + 1)),
+              double(__builtin_nontemporal_load(p + 3 * c + 2))};
 }
 
 // LDS ring of ucat planes: SoA [slot][component][row][col], rows j0-2 .. j0+NR, cols i0-2 .. i0+65
@@ -127,8 +128,8 @@ __host__ __device__ inline int tr_dims(const Geo &g, int d, int &n1, int &n2) {
     return n1 * n2;
 }
 
-template <int DIR, class T>
-__device__ __forceinline__ FluxT<T> face_general(const FormUArgsT<T> &A, const Geo &g, int f, int a, int b) {
+template <int DIR, class T, class MT>
+__device__ __forceinline__ FluxT<T> face_general(const FormUArgsT<T, MT> &A, const Geo &g, int f, int a, int b) {
     // face f along DIR at transverse (a, b) = (j, k) | (i, k) | (i, j); periodic face 0 is face m-2
     const int m = (DIR == 0) ? g.mx : (DIR == 1) ? g.my : g.mz;
     if (f == 0 && g.per[DIR] != TOSCA_PERIODIC_NONE) f = m - 2;
@@ -137,8 +138,8 @@ __device__ __forceinline__ FluxT<T> face_general(const FormUArgsT<T> &A, const G
     return face_flux<2>(A, g, a, b, f);
 }
 
-template <class T>
-__global__ void __launch_bounds__(128) k_formu_boundary(FormUArgsT<T> A, Geo g, BoundaryBufs B) {
+template <class T, class MT>
+__global__ void __launch_bounds__(128) k_formu_boundary(FormUArgsT<T, MT> A, Geo g, BoundaryBufs B) {
     const int d = blockIdx.z / 3, what = blockIdx.z % 3;      // what: 0 face 0, 1 face m-2, 2 seam fp
     int n1, n2;
     const int n = tr_dims(g, d, n1, n2);
@@ -172,12 +173,12 @@ __global__ void __launch_bounds__(128) k_formu_boundary(FormUArgsT<T> A, Geo g, 
 // ------------------------------------------------------------------------------------------------
 // main k-marching kernel: interior faces only; boundary faces and seams come from the buffers
 // ------------------------------------------------------------------------------------------------
-template <class T>
+template <class T, class MT>
 struct MainArgs {
     const T *ucont, *ucat, *nut;
-    const double *csi, *eta, *zet;
-    const double *icsi, *ieta, *izet, *jcsi, *jeta, *jzet, *kcsi, *keta, *kzet;
-    const double *iaj, *jaj, *kaj;
+    const MT *csi, *eta, *zet;
+    const MT *icsi, *ieta, *izet, *jcsi, *jeta, *jzet, *kcsi, *keta, *kzet;
+    const MT *iaj, *jaj, *kaj;
     double *rhs;
     const T *theta;
     double *btheta;
@@ -193,18 +194,18 @@ __device__ __forceinline__ FluxT<T> ldflux(const double *p) {
 }
 
 // per-thread column pointers: every array at the thread's (j, i) column, plane 0
-template <class T>
+template <class T, class MT>
 struct ColPtrs {
     const T *ucont, *nut;
-    const double *fc[3], *fe[3], *fz[3], *fa[3];
+    const MT *fc[3], *fe[3], *fz[3], *fa[3];
 };
 
 // metrics and fluxes of one interior face, loaded ahead of use
 template <class T>
 struct FaceIn { V3 fc, fe, fz; double fa; T V, nutR; };
 
-template <int DIR, class T>
-__device__ __forceinline__ FaceIn<T> face_in(const ColPtrs<T> &P, long q, int s) {
+template <int DIR, class T, class MT>
+__device__ __forceinline__ FaceIn<T> face_in(const ColPtrs<T, MT> &P, long q, int s) {
     FaceIn<T> f;
     f.fc = ldnt3(P.fc[DIR] + 3 * q, 0);
     f.fe = ldnt3(P.fe[DIR] + 3 * q, 0);
@@ -257,8 +258,8 @@ __device__ __forceinline__ Vec3<T> ring_conv(const Ring<NR, T> &R, int row, int 
 }
 
 // interior face flux from the LDS ring and preloaded face data (nut0: nu_t of the face's cell)
-template <int DIR, int NR, class T>
-__device__ __forceinline__ FluxT<T> face_pre(const MainArgs<T> &A, const FaceIn<T> &in, T nut0, const Ring<NR, T> &R,
+template <int DIR, int NR, class T, class MT>
+__device__ __forceinline__ FluxT<T> face_pre(const MainArgs<T, MT> &A, const FaceIn<T> &in, T nut0, const Ring<NR, T> &R,
                                              int row, int col, int t) {
     FluxT<T> r;
     r.d = Vec3<T>{0, 0, 0};
@@ -273,8 +274,8 @@ __device__ __forceinline__ FluxT<T> face_pre(const MainArgs<T> &A, const FaceIn<
 }
 
 // interior face flux from the LDS ring, loading its face data; q is the plane offset of the face's cell
-template <int DIR, int NR, class T>
-__device__ __forceinline__ FluxT<T> face_main(const MainArgs<T> &A, const ColPtrs<T> &P, const Ring<NR, T> &R, long q,
+template <int DIR, int NR, class T, class MT>
+__device__ __forceinline__ FluxT<T> face_main(const MainArgs<T, MT> &A, const ColPtrs<T, MT> &P, const Ring<NR, T> &R, long q,
                                               int row, int col, int t, int s) {
     FluxT<T> r;
     r.d = Vec3<T>{0, 0, 0};
@@ -289,8 +290,8 @@ __device__ __forceinline__ FluxT<T> face_main(const MainArgs<T> &A, const ColPtr
     return r;
 }
 
-template <int NR, class T>
-__global__ void __launch_bounds__(64 * NR) k_fused_formu(MainArgs<T> A, Geo g, int kchunk) {
+template <int NR, class T, class MT>
+__global__ void __launch_bounds__(64 * NR) k_fused_formu(MainArgs<T, MT> A, Geo g, int kchunk) {
     // Every thread evaluates its three faces and fp on every plane (threads outside the tile's
     // useful range work on clamped, in-bounds addresses and their results are discarded), so the
     // hot loop has no divergent branches; only the stores and the rare boundary-buffer loads are
@@ -327,13 +328,13 @@ __global__ void __launch_bounds__(64 * NR) k_fused_formu(MainArgs<T> A, Geo g, i
     const int ia = min(i, g.mx - 1), ja = min(j, g.my - 1);
     const int SJ = g.gxm, SK = g.gxm * g.gym;
     const int c0 = g.L(0, ja, ia);
-    ColPtrs<T> P;
+    ColPtrs<T, MT> P;
     P.ucont = A.ucont + 3 * (long)c0; P.nut = A.nut + c0;
     P.fc[0] = A.icsi + 3 * (long)c0; P.fe[0] = A.ieta + 3 * (long)c0; P.fz[0] = A.izet + 3 * (long)c0;
     P.fc[1] = A.jcsi + 3 * (long)c0; P.fe[1] = A.jeta + 3 * (long)c0; P.fz[1] = A.jzet + 3 * (long)c0;
     P.fc[2] = A.kcsi + 3 * (long)c0; P.fe[2] = A.keta + 3 * (long)c0; P.fz[2] = A.kzet + 3 * (long)c0;
     P.fa[0] = A.iaj + c0; P.fa[1] = A.jaj + c0; P.fa[2] = A.kaj + c0;
-    const double *pcsi = A.csi + 3 * (long)c0, *peta = A.eta + 3 * (long)c0, *pzet = A.zet + 3 * (long)c0;
+    const MT *pcsi = A.csi + 3 * (long)c0, *peta = A.eta + 3 * (long)c0, *pzet = A.zet + 3 * (long)c0;
     const T *ptheta = A.theta + c0;
     const long SG = (long)g.mx * g.my;
     double *prhs = A.rhs + 3 * (long)(ja * g.mx + ia);
@@ -426,9 +427,9 @@ __global__ void __launch_bounds__(64 * NR) k_fused_formu(MainArgs<T> A, Geo g, i
         double r0 = 0, r1 = 0, r2 = 0;
         double th = 0, thI = 0, thJ = 0, thK = 0;
         if (rhsStep) {
-            csA = ld3(pcsi + 3 * qp, 0); csB = ld3(pcsi + 3 * qp, 1);
-            etA = ld3(peta + 3 * qp, 0); etB = ld3(peta + 3 * qp, SJ);
-            zeA = ld3(pzet + 3 * qp, 0); zeB = ld3(pzet + 3 * qp, SK);
+            csA = cvt3<double>(ld3(pcsi + 3 * qp, 0)); csB = cvt3<double>(ld3(pcsi + 3 * qp, 1));
+            etA = cvt3<double>(ld3(peta + 3 * qp, 0)); etB = cvt3<double>(ld3(peta + 3 * qp, SJ));
+            zeA = cvt3<double>(ld3(pzet + 3 * qp, 0)); zeB = cvt3<double>(ld3(pzet + 3 * qp, SK));
             r0 = prhs[ro]; r1 = prhs[ro + 1]; r2 = prhs[ro + 2];
             if (buoy) { th = ptheta[qp]; thI = ptheta[qp + 1]; thJ = ptheta[qp + SJ]; thK = ptheta[qp + SK]; }
         }
@@ -493,8 +494,8 @@ __global__ void __launch_bounds__(64 * NR) k_fused_formu(MainArgs<T> A, Geo g, i
     }
 }
 
-template <class T>
-void fused_formu(const Geo &g, const FormUArgsT<T> &A, double *ws, hipStream_t s) {
+template <class T, class MT>
+void fused_formu(const Geo &g, const FormUArgsT<T, MT> &A, double *ws, hipStream_t s) {
     constexpr int NR = 8;
     const int TI = 62, TJ = NR - 2;
     // boundary faces and seams first
@@ -504,9 +505,9 @@ void fused_formu(const Geo &g, const FormUArgsT<T> &A, double *ws, hipStream_t s
     for (int d = 0; d < 3; d++) { const long n = tr_dims(g, d, n1, n2); B.bF[d] = ws + off; off += 2 * n * 6; }
     for (int d = 0; d < 3; d++) { const long n = tr_dims(g, d, n1, n2); B.seam[d] = ws + off; off += n * 3; }
     const int nmax = max(g.my * g.mz, max(g.mx * g.mz, g.mx * g.my));
-    TOSCA_LAUNCH(sizeof(T) == 8 ? "k_formu_boundary" : "k_formu_boundary_f32", k_formu_boundary<T>, dim3((nmax + 127) / 128, 1, 9), dim3(128), 0, s, A, g, B);
+    TOSCA_LAUNCH(sizeof(T) == 8 ? "k_formu_boundary" : sizeof(MT) == 8 ? "k_formu_boundary_f32" : "k_formu_boundary_f32m", (k_formu_boundary<T, MT>), dim3((nmax + 127) / 128, 1, 9), dim3(128), 0, s, A, g, B);
 
-    MainArgs<T> M;
+    MainArgs<T, MT> M;
     // the main kernel loads nu_t unconditionally; without LES (or without the viscous term) the
     // caller passes no nu_t, so any valid local array stands in (its values are never used)
     M.ucont = A.ucont; M.ucat = A.ucat; M.nut = A.nut ? A.nut : A.ucat;
@@ -527,13 +528,14 @@ void fused_formu(const Geo &g, const FormUArgsT<T> &A, double *ws, hipStream_t s
     const int kchunk = (nk + chunks - 1) / chunks;
     chunks = (nk + kchunk - 1) / kchunk;
     dim3 grid((g.mx - 2 + TI - 1) / TI, (g.my - 2 + TJ - 1) / TJ, chunks);
-    TOSCA_LAUNCH(sizeof(T) == 8 ? "k_fused_formu" : "k_fused_formu_f32", (k_fused_formu<NR, T>), grid, dim3(64, NR), 0, s, M, g, kchunk);
+    TOSCA_LAUNCH(sizeof(T) == 8 ? "k_fused_formu" : sizeof(MT) == 8 ? "k_fused_formu_f32" : "k_fused_formu_f32m", (k_fused_formu<NR, T, MT>), grid, dim3(64, NR), 0, s, M, g, kchunk);
     // the reference zeroes all of bTheta; the kernel writes the interior cells, this the rest
     if (A.btheta) zero_shell(A.btheta, 3, g.mx, g.my, g.mz, 0, 0, s);
 }
 
-template void fused_formu<double>(const Geo &, const FormUArgsT<double> &, double *, hipStream_t);
-template void fused_formu<float>(const Geo &, const FormUArgsT<float> &, double *, hipStream_t);
+template void fused_formu<double, double>(const Geo &, const FormUArgsT<double, double> &, double *, hipStream_t);
+template void fused_formu<float, double>(const Geo &, const FormUArgsT<float, double> &, double *, hipStream_t);
+template void fused_formu<float, float>(const Geo &, const FormUArgsT<float, float> &, double *, hipStream_t);
 
 size_t formu_ws_doubles(const Geo &g) {
     long tot = 0;

@@ -110,6 +110,17 @@ class RhsContext:
                                                       _p32(theta), _ptr(rhs), float(scale), _ptr(cs), _p32(nut),
                                                       _ptr(btheta), float(buoy_scale)))
 
+    def rhs_les_step_mixed_m32(self, mesh, mesh32, params, ucont, ucat, theta, rhs, scale, cs, nut, btheta,
+                               buoy_scale=1.0):
+        """Mixed-precision step whose FormU reads FP32 metric copies (mesh32: float32 tensors of the
+        metrics in abi.MESH_F32_FIELDS); the LES part reads the float64 mesh."""
+        self._stream()
+        m, p = mesh_struct(mesh), abi.make_params(params)
+        m32 = abi.make_mesh_f32({n: _p32(mesh32[n]) for n in abi.MESH_F32_FIELDS})
+        self._check(self.lib.tosca_rhs_les_step_mixed_m32(self.h, C.byref(m), C.byref(m32), C.byref(p), _p32(ucont),
+                                                          _p32(ucat), _p32(theta), _ptr(rhs), float(scale), _ptr(cs),
+                                                          _p32(nut), _ptr(btheta), float(buoy_scale)))
+
     def rhs_vjp(self, mesh, params, ucont, ucat, theta, cs, rbar, scale, buoy_scale, ucont_bar, ucat_bar, theta_bar,
                 cs_bar):
         """J^T rbar of R = FormU(ucont, ucat, UpdateNut(ucat, cs)) + Buoyancy(theta) into the four
