@@ -71,11 +71,17 @@ enum { TOSCA_DIV_CENTRAL = 0, TOSCA_DIV_QUICK = 1 };
  * viscous flux (ueqn_solvers.c:336-356,548-568,758-768); NOSLIP, SLIP, WALLFUNCTION and
  * ZEROGRADIENT select the face resets of tosca_explicit_rhs_u (resetNoPenetrationFluxes,
  * inline.h:5262-5355; resetNonResolvedCellFaces, inline.h:1674-1838).  Every other velocity BC
- * (fixedValue, inletFunction, ...) is GENERIC.                                                    */
+ * inletFunction and oversetInterpolate patches are GENERIC.  FIXEDVALUE patches take the values
+ * of tosca_update_cartesian_bcs; everywhere else FIXEDVALUE behaves as GENERIC.                   */
 enum {
     TOSCA_BCU_GENERIC = 0, TOSCA_BCU_SLIP = 1, TOSCA_BCU_WALLFUNCTION = 2, TOSCA_BCU_PERIODIC = 3,
-    TOSCA_BCU_NOSLIP = 4, TOSCA_BCU_ZEROGRADIENT = 5
+    TOSCA_BCU_NOSLIP = 4, TOSCA_BCU_ZEROGRADIENT = 5, TOSCA_BCU_FIXEDVALUE = 6
 };
+
+/* boundaryT patch kinds for tosca_update_temperature_bcs (src/boundary.c:2378-2555); a
+ * thetaWallFunction patch other than the Shumann surface-temperature model fills its ghosts as
+ * zeroGradient (boundary.c:2380-2415) and is passed as TOSCA_BCT_ZEROGRADIENT */
+enum { TOSCA_BCT_ZEROGRADIENT = 0, TOSCA_BCT_FIXEDVALUE = 1, TOSCA_BCT_FIXEDGRADIENT = 2, TOSCA_BCT_PERIODIC = 3 };
 
 /* boundaryNut patch kinds applied by UpdateNutBCs (src/boundary.c:2812-2917) */
 enum { TOSCA_BCNUT_NONE = 0, TOSCA_BCNUT_ZEROGRADIENT = 1, TOSCA_BCNUT_FIXEDVALUE = 2, TOSCA_BCNUT_PERIODIC = 3 };
@@ -239,6 +245,38 @@ int tosca_explicit_rhs_u(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_par
                          double *ucont, double *ucat, const double *nut, const double *dp,
                          const double *btheta, const double *btheta_o, int ab2, const double *farm,
                          const double *ucont_o, double *rhs, int form_mode);
+
+/* ---- boundary-condition ghost fills (the step before the path) ------------------------------ */
+
+/* UpdateContravariantBCs (src/boundary.c:614-822) on lUcont (local AoS, in place).  On every
+ * boundary patch's faces (loop range k < mz-1, j < my-1, i < mx-1): patches other than noSlip /
+ * slip / velocityWallFunction (PatchType 0, boundary.c:318-348) take the flux of the face-averaged
+ * ucat, no-penetration patches (PatchType 1) get zero flux, zeroGradient patches then lose an
+ * inflowing flux, and the left face of a periodic axis takes the right face's flux from before the
+ * call (the right face itself is re-interpolated, as in the reference).  Then the ghost scatter. */
+int tosca_update_contravariant_bcs(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_params *params,
+                                   double *ucont, const double *ucat);
+
+/* UpdateCartesianBCs (src/boundary.c:824-2050) on lUcat (local AoS, in place): the ghost cells of
+ * every patch next to the interior cells (edges and corners untouched) from
+ *   fixedValue   the patch value (u_fixed: host array of 6 x 3 doubles, iLval ... kRval; may be
+ *                NULL when no patch is TOSCA_BCU_FIXEDVALUE),
+ *   noSlip       -u of the boundary cell,
+ *   slip         the boundary cell's velocity mirrored in the patch (calculateNormal, inline.h:1231),
+ *   zeroGradient 2 u(1) - u(2) (linear extrapolation),
+ *   periodic     the opposite interior cell (type 1) or ghost (type 2),
+ * then the ghost scatter.  velocityWallFunction and inlet-function patches need the wall and
+ * inflow models and return TOSCA_ERR_UNSUPPORTED. */
+int tosca_update_cartesian_bcs(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_params *params, double *ucat,
+                               const double *u_fixed);
+
+/* UpdateTemperatureBCs (src/boundary.c:2052-2628) on lTmprt (local scalar, in place): ghosts of
+ * zeroGradient (t(1)), fixedValue (value), fixedGradient ((1/aj)/|area| value + t(1), the area of
+ * the boundary cell's own basis vector) and periodic patches, then the ghost scatter.  bc_t and
+ * t_value: host arrays of 6 (TOSCA_BCT_* kinds, values) in patch order.  Inlet functions and the
+ * Shumann wall model are not covered. */
+int tosca_update_temperature_bcs(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_params *params,
+                                 const int *bc_t, const double *t_value, double *theta);
 
 #ifdef __cplusplus
 }

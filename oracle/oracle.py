@@ -46,6 +46,9 @@ def load() -> C.CDLL:
         "oracle_rhs_les_step": (I, [In, M, Pa, P, P, P, P, D, P, P, P, D]),
         "oracle_rhs_jvp": (I, [In, M, Pa, P, P, P, P, P, P, P, P, D, D, P, P]),
         "oracle_explicit_rhs_u": (I, [In, M, Pa, P, P, P, P, P, P, I, P, P, P, I]),
+        "oracle_contravariant_bcs": (I, [In, M, Pa, P, P]),
+        "oracle_cartesian_bcs": (I, [In, M, Pa, P, P]),
+        "oracle_temperature_bcs": (I, [In, M, Pa, P, P, P]),
     }
     for n, (r, a) in sigs.items():
         f = getattr(lib, n)
@@ -150,3 +153,27 @@ def explicit_rhs_u(info, mesh, params, ucont, ucat, nut, dp, btheta, btheta_o, a
     _check(lib.oracle_explicit_rhs_u(C.byref(inf), C.byref(m), C.byref(p), _p(ucont), _p(ucat), _p(nut), _p(dp),
                                      _p(btheta), _p(btheta_o), int(ab2), _p(farm), _p(ucont_o), _p(rhs),
                                      int(form_mode)))
+
+
+def contravariant_bcs(info, mesh, params, ucont, ucat):
+    """UpdateContravariantBCs restated; ucont modified in place."""
+    lib = load()
+    inf, m, p = abi.make_info(info), _mesh(mesh), abi.make_params(params)
+    _check(lib.oracle_contravariant_bcs(C.byref(inf), C.byref(m), C.byref(p), _p(ucont), _p(ucat)))
+
+
+def cartesian_bcs(info, mesh, params, ucat, u_fixed=None):
+    """UpdateCartesianBCs subset restated; ucat modified in place; u_fixed (6, 3) or None."""
+    lib = load()
+    inf, m, p = abi.make_info(info), _mesh(mesh), abi.make_params(params)
+    uf = None if u_fixed is None else np.ascontiguousarray(u_fixed, dtype=np.float64).reshape(18)
+    _check(lib.oracle_cartesian_bcs(C.byref(inf), C.byref(m), C.byref(p), _p(ucat), _p(uf)))
+
+
+def temperature_bcs(info, mesh, params, bc_t, t_value, theta):
+    """UpdateTemperatureBCs subset restated; theta modified in place."""
+    lib = load()
+    inf, m, p = abi.make_info(info), _mesh(mesh), abi.make_params(params)
+    kinds = np.ascontiguousarray(bc_t, dtype=np.int32)
+    vals = np.ascontiguousarray(t_value, dtype=np.float64)
+    _check(lib.oracle_temperature_bcs(C.byref(inf), C.byref(m), C.byref(p), kinds.ctypes.data, _p(vals), _p(theta)))

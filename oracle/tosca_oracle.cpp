@@ -1191,6 +1191,244 @@ int oracle_explicit_rhs_u(const tosca_dmda_info *info, const tosca_mesh *m, cons
 #endif  // ORACLE_TANGENT
 
 #ifndef ORACLE_TANGENT
+// ================================================================================================
+// Boundary-condition ghost fills (src/boundary.c).  Each routine reads the local array as it was on
+// entry and writes a global working copy (the owned range), which the GlobalToLocal scatter then
+// copies back and wraps; on one rank the copy is the owned part of the local array.
+// ================================================================================================
+static void global_to_local(const Lay &l, const std::vector<V3> &g, V3 *loc) {
+    for (int64_t k = 0; k < l.mz; k++)
+        for (int64_t j = 0; j < l.my; j++)
+            for (int64_t i = 0; i < l.mx; i++) loc[l.L(k, j, i)] = g[l.L(k, j, i)];
+    local_to_local(l, loc);
+}
+
+// PatchType of checkBCsAndSetPatchTypes (src/boundary.c:318-348): 1 for noSlip, slip,
+// velocityWallFunction, 0 otherwise
+static int patch_type(int bc) {
+    return (bc == TOSCA_BCU_NOSLIP || bc == TOSCA_BCU_SLIP || bc == TOSCA_BCU_WALLFUNCTION) ? 1 : 0;
+}
+
+int oracle_contravariant_bcs(const tosca_dmda_info *info, const tosca_mesh *m, const tosca_params *p, double *ucont_,
+                             const double *ucat_) {
+    Lay l;
+    int rc = check_layout(info, p, l);
+    if (rc) return rc;
+    V3 *lucont = (V3 *)ucont_;
+    const V3 *lucat = (const V3 *)ucat_;
+    const V3 *icsi = (const V3 *)m->icsi, *jeta = (const V3 *)m->jeta, *kzet = (const V3 *)m->kzet;
+    const int *bc = p->bc_u;
+    const int64_t mx = l.mx, my = l.my, mz = l.mz;
+    std::vector<V3> ucont(lucont, lucont + l.nloc());
+    for (int64_t k = 0; k < mz - 1; k++)                     // src/boundary.c:646-807
+        for (int64_t j = 0; j < my - 1; j++)
+            for (int64_t i = 0; i < mx - 1; i++) {
+                const int64_t c = l.L(k, j, i);
+                if ((patch_type(bc[4]) == 0 && k == 0) || (patch_type(bc[5]) == 0 && k == mz - 2)) {
+                    const V3 a = lucat[l.L(k + 1, j, i)], b = lucat[c];
+                    ucont[c].z = 0.5 * (a.x + b.x) * kzet[c].x + 0.5 * (a.y + b.y) * kzet[c].y + 0.5 * (a.z + b.z) * kzet[c].z;
+                }
+                if ((patch_type(bc[2]) == 0 && j == 0) || (patch_type(bc[3]) == 0 && j == my - 2)) {
+                    const V3 a = lucat[l.L(k, j + 1, i)], b = lucat[c];
+                    ucont[c].y = 0.5 * (a.x + b.x) * jeta[c].x + 0.5 * (a.y + b.y) * jeta[c].y + 0.5 * (a.z + b.z) * jeta[c].z;
+                }
+                if ((patch_type(bc[0]) == 0 && i == 0) || (patch_type(bc[1]) == 0 && i == mx - 2)) {
+                    const V3 a = lucat[l.L(k, j, i + 1)], b = lucat[c];
+                    ucont[c].x = 0.5 * (a.x + b.x) * icsi[c].x + 0.5 * (a.y + b.y) * icsi[c].y + 0.5 * (a.z + b.z) * icsi[c].z;
+                }
+                if ((patch_type(bc[0]) == 1 && i == 0) || (patch_type(bc[1]) == 1 && i == mx - 2)) ucont[c].x = 0.0;
+                if ((patch_type(bc[2]) == 1 && j == 0) || (patch_type(bc[3]) == 1 && j == my - 2)) ucont[c].y = 0.0;
+                if ((patch_type(bc[4]) == 1 && k == 0) || (patch_type(bc[5]) == 1 && k == mz - 2)) ucont[c].z = 0.0;
+                if (bc[0] == TOSCA_BCU_ZEROGRADIENT && i == 0 && ucont[c].x > 0.0) ucont[c].x = 0.0;
+                if (bc[1] == TOSCA_BCU_ZEROGRADIENT && i == mx - 2 && ucont[c].x < 0.0) ucont[c].x = 0.0;
+                if (bc[2] == TOSCA_BCU_ZEROGRADIENT && j == 0 && ucont[c].y > 0.0) ucont[c].y = 0.0;
+                if (bc[3] == TOSCA_BCU_ZEROGRADIENT && j == my - 2 && ucont[c].y < 0.0) ucont[c].y = 0.0;
+                if (bc[4] == TOSCA_BCU_ZEROGRADIENT && k == 0 && ucont[c].z > 0.0) ucont[c].z = 0.0;
+                if (bc[5] == TOSCA_BCU_ZEROGRADIENT && k == mz - 2 && ucont[c].z < 0.0) ucont[c].z = 0.0;
+                if (bc[0] == TOSCA_BCU_PERIODIC && i == 0) {
+                    if (l.i_per()) ucont[c].x = lucont[l.L(k, j, mx - 2)].x;
+                    else if (l.ii_per()) ucont[c].x = lucont[l.L(k, j, -2)].x;
+                }
+                if (bc[2] == TOSCA_BCU_PERIODIC && j == 0) {
+                    if (l.j_per()) ucont[c].y = lucont[l.L(k, my - 2, i)].y;
+                    else if (l.jj_per()) ucont[c].y = lucont[l.L(k, -2, i)].y;
+                }
+                if (bc[4] == TOSCA_BCU_PERIODIC && k == 0) {
+                    if (l.k_per()) ucont[c].z = lucont[l.L(mz - 2, j, i)].z;
+                    else if (l.kk_per()) ucont[c].z = lucont[l.L(-2, j, i)].z;
+                }
+            }
+    global_to_local(l, ucont, lucont);                       // :816-817
+    return TOSCA_OK;
+}
+
+// calculateNormal (src/include/inline.h:1231-1262): unit covariant base vector of each direction
+static void unit_normals(V3 c, V3 e, V3 z, double ni[3], double nj[3], double nk[3]) {
+    const double g[3][3] = {{c.x, c.y, c.z}, {e.x, e.y, e.z}, {z.x, z.y, z.z}};
+    double G[3][3];
+    covariant(g, G);
+    double v[3][3] = {{G[0][0], G[1][0], G[2][0]}, {G[0][1], G[1][1], G[2][1]}, {G[0][2], G[1][2], G[2][2]}};
+    double *o[3] = {ni, nj, nk};
+    for (int a = 0; a < 3; a++) {
+        const double sum = sqrt(v[a][0] * v[a][0] + v[a][1] * v[a][1] + v[a][2] * v[a][2]);
+        o[a][0] = v[a][0] / sum; o[a][1] = v[a][1] / sum; o[a][2] = v[a][2] / sum;
+    }
+}
+
+int oracle_cartesian_bcs(const tosca_dmda_info *info, const tosca_mesh *m, const tosca_params *p, double *ucat_,
+                         const double *ufix) {
+    Lay l;
+    int rc = check_layout(info, p, l);
+    if (rc) return rc;
+    const int *bc = p->bc_u;
+    for (int q = 0; q < 6; q++) {
+        if (bc[q] == TOSCA_BCU_WALLFUNCTION || bc[q] == TOSCA_BCU_GENERIC) {
+            g_err = "cartesian BCs: wall-function and inlet-function patches are not covered";
+            return TOSCA_ERR_UNSUPPORTED;
+        }
+        if (bc[q] == TOSCA_BCU_FIXEDVALUE && !ufix) { g_err = "fixedValue patch without values"; return TOSCA_ERR_ARG; }
+    }
+    V3 *lucat = (V3 *)ucat_;
+    const V3 *csi = (const V3 *)m->csi, *eta = (const V3 *)m->eta, *zet = (const V3 *)m->zet;
+    const int64_t mx = l.mx, my = l.my, mz = l.mz;
+    std::vector<V3> ucat(lucat, lucat + l.nloc());
+    auto slip = [&](int64_t c, int dir, int64_t gdst) {     // boundary.c:1684-1830
+        double ni[3], nj[3], nk[3];
+        unit_normals(csi[c], eta[c], zet[c], ni, nj, nk);
+        const double *nn = dir == 0 ? ni : dir == 1 ? nj : nk;
+        const V3 n{nn[0], nn[1], nn[2]}, U = lucat[c];
+        const double d = U.x * n.x + U.y * n.y + U.z * n.z;
+        const V3 N2{2.0 * d * n.x, 2.0 * d * n.y, 2.0 * d * n.z};
+        ucat[gdst] = V3{U.x - N2.x, U.y - N2.y, U.z - N2.z};
+    };
+    auto fix = [&](int q, int64_t gdst) { ucat[gdst] = V3{ufix[3 * q], ufix[3 * q + 1], ufix[3 * q + 2]}; };
+    auto zg = [&](int64_t c, int64_t c2, int64_t gdst) {
+        const V3 a = lucat[c], b = lucat[c2];
+        ucat[gdst] = V3{2 * a.x - b.x, 2 * a.y - b.y, 2 * a.z - b.z};
+    };
+    auto neg = [&](int64_t c, int64_t gdst) { const V3 a = lucat[c]; ucat[gdst] = V3{-1.0 * a.x, -1.0 * a.y, -1.0 * a.z}; };
+    for (int64_t k = 1; k < mz - 1; k++)                     // src/boundary.c:919-2029
+        for (int64_t j = 1; j < my - 1; j++)
+            for (int64_t i = 1; i < mx - 1; i++) {
+                const int64_t c = l.L(k, j, i);
+                const int64_t gi0 = l.L(k, j, i - 1), gi1 = l.L(k, j, i + 1), gj0 = l.L(k, j - 1, i),
+                              gj1 = l.L(k, j + 1, i), gk0 = l.L(k - 1, j, i), gk1 = l.L(k + 1, j, i);
+                if (bc[0] == TOSCA_BCU_FIXEDVALUE && i == 1) fix(0, gi0);
+                if (bc[1] == TOSCA_BCU_FIXEDVALUE && i == mx - 2) fix(1, gi1);
+                if (bc[2] == TOSCA_BCU_FIXEDVALUE && j == 1) fix(2, gj0);
+                if (bc[3] == TOSCA_BCU_FIXEDVALUE && j == my - 2) fix(3, gj1);
+                if (bc[4] == TOSCA_BCU_FIXEDVALUE && k == 1) fix(4, gk0);
+                if (bc[5] == TOSCA_BCU_FIXEDVALUE && k == mz - 2) fix(5, gk1);
+                if (bc[0] == TOSCA_BCU_SLIP && i == 1) slip(c, 0, gi0);
+                if (bc[1] == TOSCA_BCU_SLIP && i == mx - 2) slip(c, 0, gi1);
+                if (bc[2] == TOSCA_BCU_SLIP && j == 1) slip(c, 1, gj0);
+                if (bc[3] == TOSCA_BCU_SLIP && j == my - 2) slip(c, 1, gj1);
+                if (bc[4] == TOSCA_BCU_SLIP && k == 1) slip(c, 2, gk0);
+                if (bc[5] == TOSCA_BCU_SLIP && k == mz - 2) slip(c, 2, gk1);
+                if (i == 1 && bc[0] == TOSCA_BCU_NOSLIP) neg(c, gi0);
+                if (i == mx - 2 && bc[1] == TOSCA_BCU_NOSLIP) neg(c, gi1);
+                if (j == 1 && bc[2] == TOSCA_BCU_NOSLIP) neg(c, gj0);
+                if (j == my - 2 && bc[3] == TOSCA_BCU_NOSLIP) neg(c, gj1);
+                if (k == 1 && bc[4] == TOSCA_BCU_NOSLIP) neg(c, gk0);
+                if (k == mz - 2 && bc[5] == TOSCA_BCU_NOSLIP) neg(c, gk1);
+                if (bc[0] == TOSCA_BCU_ZEROGRADIENT && i == 1) zg(c, gi1, gi0);
+                if (bc[1] == TOSCA_BCU_ZEROGRADIENT && i == mx - 2) zg(c, gi0, gi1);
+                if (bc[2] == TOSCA_BCU_ZEROGRADIENT && j == 1) zg(c, gj1, gj0);
+                if (bc[3] == TOSCA_BCU_ZEROGRADIENT && j == my - 2) zg(c, gj0, gj1);
+                if (bc[4] == TOSCA_BCU_ZEROGRADIENT && k == 1) zg(c, gk1, gk0);
+                if (bc[5] == TOSCA_BCU_ZEROGRADIENT && k == mz - 2) zg(c, gk0, gk1);
+                if (bc[0] == TOSCA_BCU_PERIODIC && i == 1) {
+                    if (l.i_per()) ucat[gi0] = lucat[l.L(k, j, mx - 2)];
+                    else if (l.ii_per()) ucat[gi0] = lucat[l.L(k, j, -2)];
+                }
+                if (bc[1] == TOSCA_BCU_PERIODIC && i == mx - 2) {
+                    if (l.i_per()) ucat[gi1] = lucat[l.L(k, j, 1)];
+                    else if (l.ii_per()) ucat[gi1] = lucat[l.L(k, j, mx + 1)];
+                }
+                if (bc[2] == TOSCA_BCU_PERIODIC && j == 1) {
+                    if (l.j_per()) ucat[gj0] = lucat[l.L(k, my - 2, i)];
+                    else if (l.jj_per()) ucat[gj0] = lucat[l.L(k, -2, i)];
+                }
+                if (bc[3] == TOSCA_BCU_PERIODIC && j == my - 2) {
+                    if (l.j_per()) ucat[gj1] = lucat[l.L(k, 1, i)];
+                    else if (l.jj_per()) ucat[gj1] = lucat[l.L(k, my + 1, i)];
+                }
+                if (bc[4] == TOSCA_BCU_PERIODIC && k == 1) {
+                    if (l.k_per()) ucat[gk0] = lucat[l.L(mz - 2, j, i)];
+                    else if (l.kk_per()) ucat[gk0] = lucat[l.L(-2, j, i)];
+                }
+                if (bc[5] == TOSCA_BCU_PERIODIC && k == mz - 2) {
+                    if (l.k_per()) ucat[gk1] = lucat[l.L(1, j, i)];
+                    else if (l.kk_per()) ucat[gk1] = lucat[l.L(mz + 1, j, i)];
+                }
+            }
+    global_to_local(l, ucat, lucat);                         // :2042-2043
+    return TOSCA_OK;
+}
+
+int oracle_temperature_bcs(const tosca_dmda_info *info, const tosca_mesh *m, const tosca_params *p, const int *bct,
+                           const double *tval, double *lt) {
+    Lay l;
+    int rc = check_layout(info, p, l);
+    if (rc) return rc;
+    for (int q = 0; q < 6; q++)
+        if (bct[q] < TOSCA_BCT_ZEROGRADIENT || bct[q] > TOSCA_BCT_PERIODIC) { g_err = "bad temperature BC kind"; return TOSCA_ERR_ARG; }
+    const V3 *csi = (const V3 *)m->csi, *eta = (const V3 *)m->eta, *zet = (const V3 *)m->zet;
+    const double *aj = m->aj;
+    const int64_t mx = l.mx, my = l.my, mz = l.mz;
+    std::vector<double> t(lt, lt + l.nloc());
+    auto area = [](V3 v) { return sqrt(v.x * v.x + v.y * v.y + v.z * v.z); };
+    for (int64_t k = 1; k < mz - 1; k++)                     // src/boundary.c:2173-2565
+        for (int64_t j = 1; j < my - 1; j++)
+            for (int64_t i = 1; i < mx - 1; i++) {
+                const int64_t c = l.L(k, j, i);
+                const int64_t g[6] = {l.L(k, j, i - 1), l.L(k, j, i + 1), l.L(k, j - 1, i), l.L(k, j + 1, i),
+                                      l.L(k - 1, j, i), l.L(k + 1, j, i)};
+                const bool on[6] = {i == 1, i == mx - 2, j == 1, j == my - 2, k == 1, k == mz - 2};
+                for (int q = 0; q < 6; q++)
+                    if (on[q] && bct[q] == TOSCA_BCT_ZEROGRADIENT) t[g[q]] = lt[c];
+                for (int q = 0; q < 6; q++)
+                    if (on[q] && bct[q] == TOSCA_BCT_FIXEDVALUE) t[g[q]] = tval[q];
+                for (int q = 0; q < 6; q++)
+                    if (on[q] && bct[q] == TOSCA_BCT_FIXEDGRADIENT) {
+                        const V3 b = (q < 2) ? csi[c] : (q < 4) ? eta[c] : zet[c];
+                        const double d = (1.0 / aj[c]) / area(b);
+                        t[g[q]] = d * tval[q] + lt[c];
+                    }
+                if (bct[0] == TOSCA_BCT_PERIODIC && on[0]) {
+                    if (l.i_per()) t[g[0]] = lt[l.L(k, j, mx - 2)];
+                    else if (l.ii_per()) t[g[0]] = lt[l.L(k, j, -2)];
+                }
+                if (bct[1] == TOSCA_BCT_PERIODIC && on[1]) {
+                    if (l.i_per()) t[g[1]] = lt[l.L(k, j, 1)];
+                    else if (l.ii_per()) t[g[1]] = lt[l.L(k, j, mx + 1)];
+                }
+                if (bct[2] == TOSCA_BCT_PERIODIC && on[2]) {
+                    if (l.j_per()) t[g[2]] = lt[l.L(k, my - 2, i)];
+                    else if (l.jj_per()) t[g[2]] = lt[l.L(k, -2, i)];
+                }
+                if (bct[3] == TOSCA_BCT_PERIODIC && on[3]) {
+                    if (l.j_per()) t[g[3]] = lt[l.L(k, 1, i)];
+                    else if (l.jj_per()) t[g[3]] = lt[l.L(k, my + 1, i)];
+                }
+                if (bct[4] == TOSCA_BCT_PERIODIC && on[4]) {
+                    if (l.k_per()) t[g[4]] = lt[l.L(mz - 2, j, i)];
+                    else if (l.kk_per()) t[g[4]] = lt[l.L(-2, j, i)];
+                }
+                if (bct[5] == TOSCA_BCT_PERIODIC && on[5]) {
+                    if (l.k_per()) t[g[5]] = lt[l.L(1, j, i)];
+                    else if (l.kk_per()) t[g[5]] = lt[l.L(mz + 1, j, i)];
+                }
+            }
+    for (int64_t k = 0; k < mz; k++)                          // DMGlobalToLocal (:2622-2623)
+        for (int64_t j = 0; j < my; j++)
+            for (int64_t i = 0; i < mx; i++) lt[l.L(k, j, i)] = t[l.L(k, j, i)];
+    local_to_local(l, lt);
+    return TOSCA_OK;
+}
+#endif  // ORACLE_TANGENT
+
+#ifndef ORACLE_TANGENT
 }  // extern "C"
 #else
 }  // namespace

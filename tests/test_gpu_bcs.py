@@ -1,0 +1,108 @@
+"""GPU parity of the boundary-condition ghost fills (tosca_update_contravariant_bcs,
+tosca_update_cartesian_bcs, tosca_update_temperature_bcs) against the CPU oracle's restatement.
+The whole local array is compared, so points a fill must leave alone are checked too.  The fills
+are copies, reflections and short expressions compiled without contraction: bar 1e-14 relative
+(they agree bit for bit in practice)."""
+import numpy as np
+import pytest
+import torch
+
+from tosca_amd import case as C
+from oracle import oracle as O
+from tests._util import np_mesh, to_np, to_dev, relerr, random_nodes
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-14
+
+
+@pytest.fixture(scope="module")
+def rhs_mod():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    from tosca_amd import rhs
+    return rhs
+
+
+def _case(periodic, seed, kinds):
+    rng = np.random.Generator(np.random.PCG64(700 + seed))
+    lay, mesh = C.mesh_metrics(random_nodes((7, 6, 5), seed), periodic)
+    bc_u = []
+    for q in range(6):
+        bc_u.append(C.BCU_PERIODIC if periodic[q // 2] else int(rng.choice(kinds)))
+    bcn = [C.BC_PERIODIC if periodic[a // 2] else C.BC_ZEROGRAD for a in range(6)]
+    params = C.default_params(periodic, [C.BC_PERIODIC if periodic[a // 2] else C.BC_FIXED for a in range(6)], bcn,
+                              div_scheme=C.DIV_CENTRAL, les_model=C.LES_DSM, nu=0.01)
+    return lay, np_mesh(mesh), dict(params, bc_u=tuple(bc_u)), rng
+
+
+PERS = [(0, 0, 0), (2, 0, 1), (1, 1, 1), (2, 2, 2), (0, 2, 0), (1, 0, 2)]
+
+
+@pytest.mark.parametrize("periodic", PERS)
+def test_contravariant_bcs(rhs_mod, periodic):
+    seed = PERS.index(periodic)
+    kinds = (C.BCU_NOSLIP, C.BCU_SLIP, C.BCU_WALLFUNCTION, C.BCU_ZEROGRADIENT, C.BCU_FIXEDVALUE, C.BCU_GENERIC)
+    lay, mesh, params, rng = _case(periodic, seed, kinds)
+    print(f"\nUpdateContravariantBCs periodic={periodic} bc_u={params['bc_u']}", flush=True)
+    V0, U = rng.standard_normal(lay.gshape + (3,)), rng.standard_normal(lay.gshape + (3,))
+    Vo = V0.copy()
+    O.contravariant_bcs(lay.info(), mesh, params, Vo, U)
+    ctx = rhs_mod.RhsContext(lay.info())
+    Vg = to_dev(V0)
+    ctx.UpdateContravariantBCs({k: to_dev(v) for k, v in mesh.items()}, params, Vg, to_dev(U))
+    torch.cuda.synchronize()
+    e = relerr(to_np(Vg), Vo)
+    print(f"  ucont relL2={e:.2e} identical={np.array_equal(to_np(Vg), Vo)}", flush=True)
+    assert e <= TOL
+    ctx.close()
+
+
+@pytest.mark.parametrize("periodic", PERS)
+def test_cartesian_bcs(rhs_mod, periodic):
+    seed = 10 + PERS.index(periodic)
+    kinds = (C.BCU_NOSLIP, C.BCU_SLIP, C.BCU_ZEROGRADIENT, C.BCU_FIXEDVALUE)
+    lay, mesh, params, rng = _case(periodic, seed, kinds)
+    print(f"\nUpdateCartesianBCs periodic={periodic} bc_u={params['bc_u']}", flush=True)
+    U0, ufix = rng.standard_normal(lay.gshape + (3,)), rng.standard_normal((6, 3))
+    Uo = U0.copy()
+    O.cartesian_bcs(lay.info(), mesh, params, Uo, ufix)
+    ctx = rhs_mod.RhsContext(lay.info())
+    Ug = to_dev(U0)
+    ctx.UpdateCartesianBCs({k: to_dev(v) for k, v in mesh.items()}, params, Ug, ufix)
+    torch.cuda.synchronize()
+    e = relerr(to_np(Ug), Uo)
+    print(f"  ucat relL2={e:.2e} identical={np.array_equal(to_np(Ug), Uo)}", flush=True)
+    assert e <= TOL
+    ctx.close()
+
+
+@pytest.mark.parametrize("periodic", PERS)
+def test_temperature_bcs(rhs_mod, periodic):
+    seed = 20 + PERS.index(periodic)
+    lay, mesh, params, rng = _case(periodic, seed, (C.BCU_ZEROGRADIENT,))
+    bct = [C.BCT_PERIODIC if periodic[q // 2] else
+           int(rng.choice((C.BCT_ZEROGRADIENT, C.BCT_FIXEDVALUE, C.BCT_FIXEDGRADIENT))) for q in range(6)]
+    vals = rng.standard_normal(6)
+    print(f"\nUpdateTemperatureBCs periodic={periodic} bc_t={bct}", flush=True)
+    t0 = 300.0 + rng.standard_normal(lay.gshape)
+    to = t0.copy()
+    O.temperature_bcs(lay.info(), mesh, params, bct, vals, to)
+    ctx = rhs_mod.RhsContext(lay.info())
+    tg = to_dev(t0)
+    ctx.UpdateTemperatureBCs({k: to_dev(v) for k, v in mesh.items()}, params, bct, vals, tg)
+    torch.cuda.synchronize()
+    e = relerr(to_np(tg), to)
+    print(f"  theta relL2={e:.2e} identical={np.array_equal(to_np(tg), to)}", flush=True)
+    assert e <= TOL
+    ctx.close()
+
+
+def test_cartesian_bcs_rejects_wall_function(rhs_mod):
+    lay, mesh, params, rng = _case((0, 0, 0), 40, (C.BCU_ZEROGRADIENT,))
+    params = dict(params, bc_u=(C.BCU_WALLFUNCTION,) + tuple(params["bc_u"][1:]))
+    ctx = rhs_mod.RhsContext(lay.info())
+    from tosca_amd.abi import ToscaError
+    with pytest.raises(ToscaError):
+        ctx.UpdateCartesianBCs({k: to_dev(v) for k, v in mesh.items()}, params, to_dev(rng.standard_normal(lay.gshape + (3,))))
+    ctx.close()

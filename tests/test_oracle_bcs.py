@@ -1,0 +1,261 @@
+"""Pins of the oracle's boundary-condition fills (UpdateContravariantBCs, UpdateCartesianBCs,
+UpdateTemperatureBCs subsets) from facts that do not depend on its code:
+
+  - each patch rule, written as a numpy expression on the patch's ghost plane (interior transverse
+    range for the cell fills, the loop range k < mz-1, j < my-1, i < mx-1 for the face fill), and
+    every other point unchanged;
+  - slip: the ghost velocity is the mirror image of the boundary cell's velocity in the patch, and on
+    an affine lattice the patch normal is the unit lattice vector of the axis;
+  - fixedGradient on a box lattice: (t_ghost - t_cell) / h = g, h the cell width across the patch;
+  - interpolated boundary fluxes of a uniform velocity on an affine lattice equal S . U.
+"""
+import numpy as np
+import pytest
+
+from tosca_amd import case as C
+from oracle import oracle as O
+from tests._util import np_mesh, random_nodes
+from tests.test_oracle_pins import AFFINE, affine_nodes, affine_geometry
+
+NOSLIP, ZG, SLIP, WALL, GEN, PER, FIX = (C.BCU_NOSLIP, C.BCU_ZEROGRADIENT, C.BCU_SLIP, C.BCU_WALLFUNCTION,
+                                         C.BCU_GENERIC, C.BCU_PERIODIC, C.BCU_FIXEDVALUE)
+AXIS = {0: 2, 1: 1, 2: 0}          # computational direction -> array axis of [k][j][i]
+
+
+def _setup(nodes, periodic, bc_u, seed):
+    lay, mesh = C.mesh_metrics(nodes, periodic)
+    bcn = [C.BC_PERIODIC if periodic[a // 2] else C.BC_ZEROGRAD for a in range(6)]
+    params = C.default_params(periodic, [C.BC_PERIODIC if periodic[a // 2] else C.BC_FIXED for a in range(6)], bcn,
+                              div_scheme=C.DIV_CENTRAL, les_model=C.LES_DSM, nu=0.01)
+    params = dict(params, bc_u=tuple(bc_u))
+    rng = np.random.Generator(np.random.PCG64(seed))
+    return lay, np_mesh(mesh), params, rng
+
+
+def _plane(lay, a, idx):
+    """index tuple of the plane idx along direction a (local array), interior transverse range"""
+    m = (lay.mx, lay.my, lay.mz)
+    sl = [None, None, None]
+    for d in range(3):
+        sl[AXIS[d]] = slice(4, 3 + m[d] - 1)
+    sl[AXIS[a]] = 3 + idx
+    return tuple(sl)
+
+
+# ------------------------------------------------------------------------------------------------
+# UpdateCartesianBCs
+# ------------------------------------------------------------------------------------------------
+def test_cartesian_bcs_patch_rules():
+    bc = [NOSLIP, ZG, SLIP, FIX, ZG, NOSLIP]
+    lay, mesh, params, rng = _setup(random_nodes((6, 7, 5), 11), (0, 0, 0), bc, 11)
+    u0 = rng.standard_normal(lay.gshape + (3,))
+    ufix = rng.standard_normal((6, 3))
+    u = u0.copy()
+    O.cartesian_bcs(lay.info(), mesh, params, u, ufix)
+    m = (lay.mx, lay.my, lay.mz)
+    want = u0.copy()
+    for q in range(6):
+        a, side = q // 2, q % 2
+        g_idx, c1, c2 = (0, 1, 2) if side == 0 else (m[a] - 1, m[a] - 2, m[a] - 3)
+        sg = _plane(lay, a, g_idx)
+        sc, sc2 = _plane(lay, a, c1), _plane(lay, a, c2)
+        if bc[q] == NOSLIP:
+            want[sg] = -u0[sc]
+        elif bc[q] == ZG:
+            want[sg] = 2 * u0[sc] - u0[sc2]
+        elif bc[q] == FIX:
+            want[sg] = ufix[q]
+        elif bc[q] == SLIP:
+            M = np.stack([mesh[n][sc] for n in ("csi", "eta", "zet")], axis=-2)      # rows csi, eta, zet
+            n = np.linalg.inv(M)[..., :, a]
+            n = n / np.linalg.norm(n, axis=-1, keepdims=True)
+            uc = u0[sc]
+            mir = uc - 2.0 * (uc * n).sum(-1, keepdims=True) * n
+            got = u[sg]
+            assert np.abs(got - mir).max() <= 1e-13 * np.abs(uc).max()
+            # a mirror image: same magnitude, normal component reversed, tangential kept
+            assert np.allclose(np.linalg.norm(got, axis=-1), np.linalg.norm(uc, axis=-1), rtol=1e-13, atol=0)
+            assert np.allclose((got * n).sum(-1), -(uc * n).sum(-1), rtol=0, atol=1e-13)
+            want[sg] = got
+    assert np.array_equal(u, want)
+
+
+@pytest.mark.parametrize("geom", ["box", "sheared"])
+def test_slip_normal_is_the_unit_lattice_vector(geom):
+    vecs = AFFINE[geom]
+    bc = [SLIP] * 6
+    lay, mesh, params, rng = _setup(affine_nodes((5, 6, 4), vecs), (0, 0, 0), bc, 12)
+    u0 = rng.standard_normal(lay.gshape + (3,))
+    u = u0.copy()
+    O.cartesian_bcs(lay.info(), mesh, params, u)
+    m = (lay.mx, lay.my, lay.mz)
+    for a in range(3):
+        e = vecs[a] / np.linalg.norm(vecs[a])
+        sg, sc = _plane(lay, a, 0), _plane(lay, a, 1)
+        uc = u0[sc]
+        want = uc - 2.0 * (uc @ e)[..., None] * e
+        assert np.abs(u[sg] - want).max() <= 1e-13
+
+
+@pytest.mark.parametrize("periodic", [(1, 0, 2), (2, 1, 0), (2, 2, 1)])
+def test_cartesian_bcs_periodic(periodic):
+    bc = []
+    for a in range(3):
+        bc += [PER, PER] if periodic[a] else [NOSLIP, ZG]
+    lay, mesh, params, rng = _setup(random_nodes((6, 7, 5), 13), periodic, bc, 13)
+    u0 = rng.standard_normal(lay.gshape + (3,))
+    # type-2 ghosts consistent with the owned values, as a scattered local array holds them
+    from tosca_amd.case import local_to_local
+    import torch
+    u0 = local_to_local(torch.as_tensor(u0), lay, periodic).numpy().copy()
+    u = u0.copy()
+    O.cartesian_bcs(lay.info(), mesh, params, u)
+    m = (lay.mx, lay.my, lay.mz)
+    for a in range(3):
+        if not periodic[a]:
+            continue
+        assert np.array_equal(u[_plane(lay, a, 0)], u0[_plane(lay, a, m[a] - 2)]), a
+        assert np.array_equal(u[_plane(lay, a, m[a] - 1)], u0[_plane(lay, a, 1)]), a
+
+
+def test_cartesian_bcs_rejects_wall_and_inlet_patches():
+    lay, mesh, params, rng = _setup(random_nodes((5, 5, 5), 14), (0, 0, 0), [WALL, ZG, ZG, ZG, ZG, ZG], 14)
+    u = rng.standard_normal(lay.gshape + (3,))
+    from tosca_amd.abi import ToscaError
+    with pytest.raises(ToscaError):
+        O.cartesian_bcs(lay.info(), mesh, params, u)
+    with pytest.raises(ToscaError):
+        O.cartesian_bcs(lay.info(), mesh, dict(params, bc_u=(GEN, ZG, ZG, ZG, ZG, ZG)), u)
+
+
+# ------------------------------------------------------------------------------------------------
+# UpdateContravariantBCs
+# ------------------------------------------------------------------------------------------------
+def _face_plane(lay, a, f):
+    """face index f along direction a, transverse range [0, m-1) (the reference's loop range)"""
+    m = (lay.mx, lay.my, lay.mz)
+    sl = [None, None, None]
+    for d in range(3):
+        sl[AXIS[d]] = slice(3, 3 + m[d] - 1)
+    sl[AXIS[a]] = 3 + f
+    return tuple(sl)
+
+
+def test_contravariant_bcs_patch_rules():
+    bc = [FIX, NOSLIP, ZG, SLIP, ZG, WALL]
+    lay, mesh, params, rng = _setup(random_nodes((6, 7, 5), 15), (0, 0, 0), bc, 15)
+    V0 = rng.standard_normal(lay.gshape + (3,))
+    U = rng.standard_normal(lay.gshape + (3,))
+    V = V0.copy()
+    O.contravariant_bcs(lay.info(), mesh, params, V, U)
+    m = (lay.mx, lay.my, lay.mz)
+    want = V0.copy()
+    area = {0: "icsi", 1: "jeta", 2: "kzet"}
+    for q in range(6):
+        a, side = q // 2, q % 2
+        f = 0 if side == 0 else m[a] - 2
+        sf = _face_plane(lay, a, f)
+        nb = list(sf)
+        nb[AXIS[a]] = sf[AXIS[a]] + 1
+        if bc[q] in (NOSLIP, SLIP, WALL):
+            want[sf + (a,)] = 0.0
+        else:
+            v = ((0.5 * (U[tuple(nb)] + U[sf])) * mesh[area[a]][sf]).sum(-1)
+            if bc[q] == ZG:
+                v = np.minimum(v, 0.0) if side == 0 else np.maximum(v, 0.0)
+            want[sf + (a,)] = v
+    assert np.abs(V - want).max() <= 1e-14 * np.abs(want).max()
+    # nothing but those faces changes
+    diff = V != V0
+    for q in range(6):
+        a, side = q // 2, q % 2
+        diff[_face_plane(lay, a, 0 if side == 0 else m[a] - 2) + (a,)] = False
+    assert not diff.any()
+
+
+@pytest.mark.parametrize("geom", ["box", "sheared"])
+def test_contravariant_bcs_uniform_flow(geom):
+    """interpolated boundary fluxes of a uniform velocity on an affine lattice are S . U"""
+    vecs = AFFINE[geom]
+    lay, mesh, params, rng = _setup(affine_nodes((5, 6, 4), vecs), (0, 0, 0), [FIX] * 6, 16)
+    U0 = np.array([0.7, -0.3, 0.45])
+    U = np.broadcast_to(U0, lay.gshape + (3,)).copy()
+    V = np.zeros(lay.gshape + (3,))
+    O.contravariant_bcs(lay.info(), mesh, params, V, U)
+    S, _, _ = affine_geometry(vecs)
+    m = (lay.mx, lay.my, lay.mz)
+    for a in range(3):
+        for f in (0, m[a] - 2):
+            # faces with interior transverse indices (the builder's face metrics of the boundary
+            # rows across the patch are zero)
+            sl = list(_plane(lay, a, f))
+            got = V[tuple(sl) + (a,)]
+            assert np.abs(got - S[a] @ U0).max() <= 1e-14 * abs(S[a] @ U0) + 1e-15
+
+
+@pytest.mark.parametrize("ptype", [C.PERIODIC_COPY, C.PERIODIC_WRAP])
+def test_contravariant_bcs_periodic_faces(ptype):
+    periodic = (ptype, 0, 0)
+    bc = [PER, PER, NOSLIP, NOSLIP, ZG, ZG]
+    lay, mesh, params, rng = _setup(random_nodes((6, 7, 5), 17), periodic, bc, 17)
+    import torch
+    from tosca_amd.case import local_to_local
+    V0 = local_to_local(torch.as_tensor(rng.standard_normal(lay.gshape + (3,))), lay, periodic).numpy().copy()
+    U = rng.standard_normal(lay.gshape + (3,))
+    V = V0.copy()
+    O.contravariant_bcs(lay.info(), mesh, params, V, U)
+    mx = lay.mx
+    s0, sR = _face_plane(lay, 0, 0), _face_plane(lay, 0, mx - 2)
+    # left face: the right face's flux from before the call; right face: re-interpolated from ucat
+    assert np.array_equal(V[s0 + (0,)], V0[sR + (0,)])
+    nb = list(sR)
+    nb[2] = sR[2] + 1
+    v = ((0.5 * (U[tuple(nb)] + U[sR])) * mesh["icsi"][sR]).sum(-1)
+    assert np.abs(V[sR + (0,)] - v).max() <= 1e-14 * np.abs(v).max()
+
+
+# ------------------------------------------------------------------------------------------------
+# UpdateTemperatureBCs
+# ------------------------------------------------------------------------------------------------
+def test_temperature_bcs_patch_rules():
+    T_ZG, T_FV, T_FG, T_PER = C.BCT_ZEROGRADIENT, C.BCT_FIXEDVALUE, C.BCT_FIXEDGRADIENT, C.BCT_PERIODIC
+    bct = [T_ZG, T_FV, T_FG, T_FG, T_FV, T_ZG]
+    lay, mesh, params, rng = _setup(affine_nodes((6, 7, 5), AFFINE["box"]), (0, 0, 0), [ZG] * 6, 18)
+    vals = rng.standard_normal(6)
+    t0 = 300.0 + rng.standard_normal(lay.gshape)
+    t = t0.copy()
+    O.temperature_bcs(lay.info(), mesh, params, bct, vals, t)
+    m = (lay.mx, lay.my, lay.mz)
+    want = t0.copy()
+    width = [np.linalg.norm(v) for v in AFFINE["box"]]       # cell width across an i, j, k patch
+    for q in range(6):
+        a, side = q // 2, q % 2
+        sg, sc = _plane(lay, a, 0 if side == 0 else m[a] - 1), _plane(lay, a, 1 if side == 0 else m[a] - 2)
+        if bct[q] == T_ZG:
+            want[sg] = t0[sc]
+        elif bct[q] == T_FV:
+            want[sg] = vals[q]
+        elif bct[q] == T_FG:
+            # box lattice: (1/aj) / |area| is the cell width across the patch
+            assert np.allclose((t[sg] - t0[sc]) / width[a], vals[q], rtol=1e-13, atol=1e-12)
+            want[sg] = t[sg]
+    assert np.array_equal(t, want)
+
+
+@pytest.mark.parametrize("periodic", [(1, 0, 2), (2, 1, 0)])
+def test_temperature_bcs_periodic(periodic):
+    bct = []
+    for a in range(3):
+        bct += [C.BCT_PERIODIC] * 2 if periodic[a] else [C.BCT_ZEROGRADIENT] * 2
+    bc = [PER if periodic[q // 2] else ZG for q in range(6)]
+    lay, mesh, params, rng = _setup(random_nodes((6, 7, 5), 19), periodic, bc, 19)
+    import torch
+    from tosca_amd.case import local_to_local
+    t0 = local_to_local(torch.as_tensor(rng.standard_normal(lay.gshape)), lay, periodic).numpy().copy()
+    t = t0.copy()
+    O.temperature_bcs(lay.info(), mesh, params, bct, np.zeros(6), t)
+    m = (lay.mx, lay.my, lay.mz)
+    for a in range(3):
+        if periodic[a]:
+            assert np.array_equal(t[_plane(lay, a, 0)], t0[_plane(lay, a, m[a] - 2)])
+            assert np.array_equal(t[_plane(lay, a, m[a] - 1)], t0[_plane(lay, a, 1)])

@@ -84,6 +84,9 @@ SIGNATURES = {
     "tosca_rhs_les_step_mixed_m32": (I, [P, C.POINTER(Mesh), C.POINTER(MeshF32), C.POINTER(Params), P, P, P, P, D, P, P,
                                          P, D]),
     "tosca_explicit_rhs_u": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P, P, P, P, I, P, P, P, I]),
+    "tosca_update_contravariant_bcs": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P]),
+    "tosca_update_cartesian_bcs": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P]),
+    "tosca_update_temperature_bcs": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P]),
 }
 
 _lib: Optional[C.CDLL] = None

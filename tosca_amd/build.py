@@ -35,6 +35,7 @@ SOURCES = {
     "fused_les.hip": [],
     "vjp.hip": [],
     "explicit_rhs.hip": ["-ffp-contract=off"],
+    "bc_fill.hip": ["-ffp-contract=off"],
 }
 
 

@@ -32,7 +32,8 @@ G = 3  # DMDA ghost width (src/mesh.c:42)
 
 PERIODIC_NONE, PERIODIC_COPY, PERIODIC_WRAP = 0, 1, 2
 DIV_CENTRAL, DIV_QUICK = 0, 1
-BCU_GENERIC, BCU_SLIP, BCU_WALLFUNCTION, BCU_PERIODIC, BCU_NOSLIP, BCU_ZEROGRADIENT = 0, 1, 2, 3, 4, 5
+BCU_GENERIC, BCU_SLIP, BCU_WALLFUNCTION, BCU_PERIODIC, BCU_NOSLIP, BCU_ZEROGRADIENT, BCU_FIXEDVALUE = 0, 1, 2, 3, 4, 5, 6
+BCT_ZEROGRADIENT, BCT_FIXEDVALUE, BCT_FIXEDGRADIENT, BCT_PERIODIC = 0, 1, 2, 3
 BCNUT_NONE, BCNUT_ZEROGRADIENT, BCNUT_FIXEDVALUE, BCNUT_PERIODIC = 0, 1, 2, 3
 LES_SMAGORINSKY, LES_DSM, LES_DPASD = 0, 2, 5
 
@@ -568,7 +569,7 @@ def default_params(periodic, bcs_u, bcs_nut, nut_values=(0.0,) * 6, div_scheme=D
     bcu = []
     for b in bcs_u:
         bcu.append({BC_SLIP: BCU_SLIP, BC_PERIODIC: BCU_PERIODIC, BC_NOSLIP: BCU_NOSLIP,
-                    BC_ZEROGRAD: BCU_ZEROGRADIENT}.get(b, BCU_GENERIC))
+                    BC_ZEROGRAD: BCU_ZEROGRADIENT, BC_FIXED: BCU_FIXEDVALUE}.get(b, BCU_GENERIC))
     bcn = []
     for b in bcs_nut:
         bcn.append({BC_PERIODIC: BCNUT_PERIODIC, BC_ZEROGRAD: BCNUT_ZEROGRADIENT,

@@ -94,7 +94,7 @@ int check_params(tosca_ctx *c, const tosca_params *p) {
         if (p->periodic[a] < TOSCA_PERIODIC_NONE || p->periodic[a] > TOSCA_PERIODIC_WRAP)
             return set_err(c, TOSCA_ERR_ARG, "periodic[%d] = %d is not a periodic type", a, p->periodic[a]);
     for (int q = 0; q < 6; q++) {
-        if (p->bc_u[q] < TOSCA_BCU_GENERIC || p->bc_u[q] > TOSCA_BCU_ZEROGRADIENT)
+        if (p->bc_u[q] < TOSCA_BCU_GENERIC || p->bc_u[q] > TOSCA_BCU_FIXEDVALUE)
             return set_err(c, TOSCA_ERR_ARG, "bc_u[%d] = %d is not a TOSCA_BCU_* kind", q, p->bc_u[q]);
         if (p->bc_nut[q] < TOSCA_BCNUT_NONE || p->bc_nut[q] > TOSCA_BCNUT_PERIODIC)
             return set_err(c, TOSCA_ERR_ARG, "bc_nut[%d] = %d is not a TOSCA_BCNUT_* kind", q, p->bc_nut[q]);
@@ -483,6 +483,57 @@ int tosca_explicit_rhs_u(tosca_ctx *c, const tosca_mesh *m, const tosca_params *
     }
     tosca::launch_non_resolved(rhs, ucont_o, p, g, s);               // :1403
     return launched(c, "ExplicitRhsU");
+}
+
+int tosca_update_contravariant_bcs(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, double *ucont,
+                                   const double *ucat) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if ((rc = check_params(c, p))) return rc;
+    set_geo_per(c, p);
+    const size_t v = 3 * (size_t)c->geo.nloc() * sizeof(double);
+    if (!m) return set_err(c, TOSCA_ERR_ARG, "mesh is NULL");
+    if ((rc = check_dev(c, m->icsi, v, "mesh.icsi")) || (rc = check_dev(c, m->jeta, v, "mesh.jeta")) ||
+        (rc = check_dev(c, m->kzet, v, "mesh.kzet")) || (rc = check_dev(c, ucont, v, "ucont")) ||
+        (rc = check_dev(c, ucat, v, "ucat")))
+        return rc;
+    tosca::launch_contravariant_bcs(ucont, ucat, m, p, c->geo, c->stream);
+    return launched(c, "UpdateContravariantBCs");
+}
+
+int tosca_update_cartesian_bcs(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, double *ucat,
+                               const double *u_fixed) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if ((rc = check_params(c, p))) return rc;
+    for (int q = 0; q < 6; q++) {
+        if (p->bc_u[q] == TOSCA_BCU_WALLFUNCTION || p->bc_u[q] == TOSCA_BCU_GENERIC)
+            return set_err(c, TOSCA_ERR_UNSUPPORTED, "bc_u[%d] = %d: wall-function and inlet-function velocity patches need "
+                                                     "the wall and inflow models, which are not part of this library", q, p->bc_u[q]);
+        if (p->bc_u[q] == TOSCA_BCU_FIXEDVALUE && !u_fixed)
+            return set_err(c, TOSCA_ERR_ARG, "bc_u[%d] is fixedValue but u_fixed is NULL", q);
+    }
+    set_geo_per(c, p);
+    if ((rc = check_mesh(c, m, false, true, false))) return rc;
+    if ((rc = check_dev(c, ucat, 3 * (size_t)c->geo.nloc() * sizeof(double), "ucat"))) return rc;
+    tosca::launch_cartesian_bcs(ucat, u_fixed, m, p, c->geo, c->stream);
+    return launched(c, "UpdateCartesianBCs");
+}
+
+int tosca_update_temperature_bcs(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const int *bc_t,
+                                 const double *t_value, double *theta) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if ((rc = check_params(c, p))) return rc;
+    if (!bc_t || !t_value) return set_err(c, TOSCA_ERR_ARG, "bc_t / t_value is NULL");
+    for (int q = 0; q < 6; q++)
+        if (bc_t[q] < TOSCA_BCT_ZEROGRADIENT || bc_t[q] > TOSCA_BCT_PERIODIC)
+            return set_err(c, TOSCA_ERR_ARG, "bc_t[%d] = %d is not a TOSCA_BCT_* kind", q, bc_t[q]);
+    set_geo_per(c, p);
+    if ((rc = check_mesh(c, m, false, true, false))) return rc;
+    if ((rc = check_dev(c, theta, (size_t)c->geo.nloc() * sizeof(double), "theta"))) return rc;
+    tosca::launch_temperature_bcs(theta, bc_t, t_value, m, c->geo, c->stream);
+    return launched(c, "UpdateTemperatureBCs");
 }
 
 int tosca_update_cs(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const double *ucat, double *cs) {

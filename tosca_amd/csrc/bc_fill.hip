@@ -1,0 +1,220 @@
+// bc_fill.hip -- boundary-condition ghost fills of the flow fields on gfx950 (src/boundary.c).
+// Compiled with FP contraction off: the fills are copies, reflections and short expressions,
+// evaluated as the reference writes them.
+//
+//   UpdateContravariantBCs  boundary.c:614-822    -> k_contra_periodic_copy, k_contra_bcs
+//   UpdateCartesianBCs      boundary.c:824-2050   -> k_cart_bcs (fixedValue, slip, noSlip,
+//                                                     zeroGradient, periodic patches)
+//   UpdateTemperatureBCs    boundary.c:2052-2628  -> k_temp_bcs (zeroGradient, fixedValue,
+//                                                     fixedGradient, periodic patches)
+// The reference reads the local array as it was on entry and writes a global copy; every kernel
+// here writes only locations that no thread of the same launch reads (ghost planes of the cell
+// fills; boundary faces of the flux fill, whose one cross-face read -- the left periodic face
+// copying the right face -- runs in a launch of its own before the right face is rewritten).
+// Each fill ends with the ghost scatter of the periodic-wrap axes.
+#include "kernels.h"
+
+namespace tosca {
+
+namespace {
+
+struct PatchSet {
+    int bc[6];          // patch kinds, order iLeft iRight jLeft jRight kLeft kRight
+    double val[18];     // fixedValue vectors (velocity) or values (temperature, first 6)
+};
+
+__device__ __forceinline__ void across(int a, int &t1, int &t2) {
+    t1 = (a == 0) ? 1 : 0;
+    t2 = (a == 2) ? 1 : 2;
+}
+
+__device__ __forceinline__ int patch_type(int bc) {     // checkBCsAndSetPatchTypes, boundary.c:318-348
+    return (bc == TOSCA_BCU_NOSLIP || bc == TOSCA_BCU_SLIP || bc == TOSCA_BCU_WALLFUNCTION) ? 1 : 0;
+}
+
+// left face of a periodic axis: the right face's flux from before the call (boundary.c:760-806);
+// type 1 reads face m-2, type 2 the ghost face -2
+__global__ void k_contra_periodic_copy(double *ucont, Geo g, PatchSet P) {
+    const int a = blockIdx.z;
+    if (P.bc[2 * a] != TOSCA_BCU_PERIODIC || g.per[a] == TOSCA_PERIODIC_NONE) return;
+    const int m[3] = {g.mx, g.my, g.mz};
+    int t1, t2;
+    across(a, t1, t2);
+    const int x1 = blockIdx.x * blockDim.x + threadIdx.x, x2 = blockIdx.y;
+    if (x1 >= m[t1] - 1 || x2 >= m[t2] - 1) return;
+    int d[3], s[3];
+    d[a] = 0;
+    s[a] = (g.per[a] == TOSCA_PERIODIC_COPY) ? m[a] - 2 : -2;
+    d[t1] = s[t1] = x1;
+    d[t2] = s[t2] = x2;
+    ucont[3 * (long)g.L(d[2], d[1], d[0]) + a] = ucont[3 * (long)g.L(s[2], s[1], s[0]) + a];
+}
+
+// every other boundary face: PatchType 0 -> flux of the face-averaged ucat (boundary.c:652-688),
+// PatchType 1 -> 0 (:690-713), zeroGradient -> inflow removed (:715-739)
+__global__ void k_contra_bcs(double *ucont, const double *ucat, const double *icsi, const double *jeta,
+                             const double *kzet, Geo g, PatchSet P) {
+    const int q = blockIdx.z, a = q >> 1, side = q & 1;
+    const int b = P.bc[q];
+    if (side == 0 && b == TOSCA_BCU_PERIODIC && g.per[a] != TOSCA_PERIODIC_NONE) return;   // copied above
+    const int m[3] = {g.mx, g.my, g.mz};
+    int t1, t2;
+    across(a, t1, t2);
+    const int x1 = blockIdx.x * blockDim.x + threadIdx.x, x2 = blockIdx.y;
+    if (x1 >= m[t1] - 1 || x2 >= m[t2] - 1) return;
+    int c[3], n[3];
+    c[a] = side ? m[a] - 2 : 0;
+    c[t1] = x1;
+    c[t2] = x2;
+    n[0] = c[0]; n[1] = c[1]; n[2] = c[2];
+    n[a] += 1;
+    const long ci = g.L(c[2], c[1], c[0]);
+    double v = ucont[3 * ci + a];
+    if (patch_type(b) == 0) {
+        const double *S = (a == 0) ? icsi : (a == 1) ? jeta : kzet;
+        const V3 A = ld3(ucat, g.L(n[2], n[1], n[0])), B = ld3(ucat, ci), s = ld3(S, ci);
+        v = 0.5 * (A.x + B.x) * s.x + 0.5 * (A.y + B.y) * s.y + 0.5 * (A.z + B.z) * s.z;
+    } else {
+        v = 0.0;
+    }
+    if (b == TOSCA_BCU_ZEROGRADIENT) {
+        if (side == 0 && v > 0.0) v = 0.0;
+        if (side == 1 && v < 0.0) v = 0.0;
+    }
+    ucont[3 * ci + a] = v;
+}
+
+// unit covariant base vector of direction a at a cell (calculateNormal, inline.h:1231-1262)
+__device__ V3 unit_normal(V3 cs, V3 et, V3 ze, int a) {
+    const double a11 = cs.x, a12 = cs.y, a13 = cs.z, a21 = et.x, a22 = et.y, a23 = et.z, a31 = ze.x, a32 = ze.y, a33 = ze.z;
+    const double det = a11 * (a33 * a22 - a32 * a23) - a21 * (a33 * a12 - a32 * a13) + a31 * (a23 * a12 - a22 * a13);
+    double x, y, z;     // column a of the inverse (Calculate_Covariant_metrics, inline.h:1210-1229)
+    if (a == 0) {
+        x = (a33 * a22 - a32 * a23) / det; y = -(a33 * a21 - a31 * a23) / det; z = (a32 * a21 - a31 * a22) / det;
+    } else if (a == 1) {
+        x = -(a33 * a12 - a32 * a13) / det; y = (a33 * a11 - a31 * a13) / det; z = -(a32 * a11 - a31 * a12) / det;
+    } else {
+        x = (a23 * a12 - a22 * a13) / det; y = -(a23 * a11 - a21 * a13) / det; z = (a22 * a11 - a21 * a12) / det;
+    }
+    const double sum = sqrt(x * x + y * y + z * z);
+    return V3{x / sum, y / sum, z / sum};
+}
+
+// ghost cell of a patch next to an interior boundary cell (transverse indices interior)
+__global__ void k_cart_bcs(double *ucat, const double *csi, const double *eta, const double *zet, Geo g, PatchSet P) {
+    const int q = blockIdx.z, a = q >> 1, side = q & 1;
+    const int m[3] = {g.mx, g.my, g.mz};
+    int t1, t2;
+    across(a, t1, t2);
+    const int x1 = blockIdx.x * blockDim.x + threadIdx.x + 1, x2 = blockIdx.y + 1;
+    if (x1 > m[t1] - 2 || x2 > m[t2] - 2) return;
+    int c[3], gh[3], c2[3];
+    c[t1] = gh[t1] = c2[t1] = x1;
+    c[t2] = gh[t2] = c2[t2] = x2;
+    c[a] = side ? m[a] - 2 : 1;
+    gh[a] = side ? m[a] - 1 : 0;
+    c2[a] = side ? m[a] - 3 : 2;
+    const long ci = g.L(c[2], c[1], c[0]), gi = g.L(gh[2], gh[1], gh[0]);
+    const int b = P.bc[q];
+    V3 out;
+    if (b == TOSCA_BCU_FIXEDVALUE) {                                     // boundary.c:1631-1666
+        out = V3{P.val[3 * q], P.val[3 * q + 1], P.val[3 * q + 2]};
+    } else if (b == TOSCA_BCU_SLIP) {                                    // :1670-1830
+        const V3 n = unit_normal(ld3(csi, ci), ld3(eta, ci), ld3(zet, ci), a), U = ld3(ucat, ci);
+        const double d = U.x * n.x + U.y * n.y + U.z * n.z;
+        const V3 N2{2.0 * d * n.x, 2.0 * d * n.y, 2.0 * d * n.z};
+        out = V3{U.x - N2.x, U.y - N2.y, U.z - N2.z};
+    } else if (b == TOSCA_BCU_NOSLIP) {                                  // :1832-1867
+        const V3 U = ld3(ucat, ci);
+        out = V3{-1.0 * U.x, -1.0 * U.y, -1.0 * U.z};
+    } else if (b == TOSCA_BCU_ZEROGRADIENT) {                            // :1869-1928
+        const V3 U = ld3(ucat, ci), U2 = ld3(ucat, g.L(c2[2], c2[1], c2[0]));
+        out = V3{2 * U.x - U2.x, 2 * U.y - U2.y, 2 * U.z - U2.z};
+    } else if (b == TOSCA_BCU_PERIODIC && g.per[a] != TOSCA_PERIODIC_NONE) {   // :1930-1990
+        int s[3] = {c[0], c[1], c[2]};
+        if (g.per[a] == TOSCA_PERIODIC_COPY) s[a] = side ? 1 : m[a] - 2;
+        else s[a] = side ? m[a] + 1 : -2;
+        out = ld3(ucat, g.L(s[2], s[1], s[0]));
+    } else {
+        return;
+    }
+    ucat[3 * gi] = out.x;
+    ucat[3 * gi + 1] = out.y;
+    ucat[3 * gi + 2] = out.z;
+}
+
+__global__ void k_temp_bcs(double *t, const double *csi, const double *eta, const double *zet, const double *aj, Geo g,
+                           PatchSet P) {
+    const int q = blockIdx.z, a = q >> 1, side = q & 1;
+    const int m[3] = {g.mx, g.my, g.mz};
+    int t1, t2;
+    across(a, t1, t2);
+    const int x1 = blockIdx.x * blockDim.x + threadIdx.x + 1, x2 = blockIdx.y + 1;
+    if (x1 > m[t1] - 2 || x2 > m[t2] - 2) return;
+    int c[3], gh[3];
+    c[t1] = gh[t1] = x1;
+    c[t2] = gh[t2] = x2;
+    c[a] = side ? m[a] - 2 : 1;
+    gh[a] = side ? m[a] - 1 : 0;
+    const long ci = g.L(c[2], c[1], c[0]), gi = g.L(gh[2], gh[1], gh[0]);
+    const int b = P.bc[q];
+    double out;
+    if (b == TOSCA_BCT_ZEROGRADIENT) {                                   // boundary.c:2379-2426
+        out = t[ci];
+    } else if (b == TOSCA_BCT_FIXEDVALUE) {                              // :2429-2458
+        out = P.val[q];
+    } else if (b == TOSCA_BCT_FIXEDGRADIENT) {                           // :2461-2555
+        const V3 s = ld3((a == 0) ? csi : (a == 1) ? eta : zet, ci);
+        const double area = sqrt(s.x * s.x + s.y * s.y + s.z * s.z);
+        const double d = (1.0 / aj[ci]) / area;
+        out = d * P.val[q] + t[ci];
+    } else if (b == TOSCA_BCT_PERIODIC && g.per[a] != TOSCA_PERIODIC_NONE) {   // :2557-2596
+        int s[3] = {c[0], c[1], c[2]};
+        if (g.per[a] == TOSCA_PERIODIC_COPY) s[a] = side ? 1 : m[a] - 2;
+        else s[a] = side ? m[a] + 1 : -2;
+        out = t[g.L(s[2], s[1], s[0])];
+    } else {
+        return;
+    }
+    t[gi] = out;
+}
+
+int maxdim(const Geo &g) { return std::max(g.mx, std::max(g.my, g.mz)); }
+
+PatchSet patches(const int *bc, const double *vals, int nvals) {
+    PatchSet P{};
+    for (int q = 0; q < 6; q++) P.bc[q] = bc[q];
+    if (vals)
+        for (int q = 0; q < nvals; q++) P.val[q] = vals[q];
+    return P;
+}
+
+}  // namespace
+
+void launch_contravariant_bcs(double *ucont, const double *ucat, const tosca_mesh *m, const tosca_params *p,
+                              const Geo &g, hipStream_t s) {
+    const int n = maxdim(g);
+    const PatchSet P = patches(p->bc_u, nullptr, 0);
+    TOSCA_LAUNCH("k_contra_periodic_copy", k_contra_periodic_copy, dim3((n + 127) / 128, n, 3), dim3(128), 0, s, ucont, g, P);
+    TOSCA_LAUNCH("k_contra_bcs", k_contra_bcs, dim3((n + 127) / 128, n, 6), dim3(128), 0, s, ucont, ucat, m->icsi,
+                 m->jeta, m->kzet, g, P);
+    launch_wrap(ucont, 3, g, s);
+}
+
+void launch_cartesian_bcs(double *ucat, const double *u_fixed, const tosca_mesh *m, const tosca_params *p,
+                          const Geo &g, hipStream_t s) {
+    const int n = maxdim(g);
+    const PatchSet P = patches(p->bc_u, u_fixed, 18);
+    TOSCA_LAUNCH("k_cart_bcs", k_cart_bcs, dim3((n + 127) / 128, n, 6), dim3(128), 0, s, ucat, m->csi, m->eta, m->zet, g, P);
+    launch_wrap(ucat, 3, g, s);
+}
+
+void launch_temperature_bcs(double *t, const int *bc_t, const double *t_value, const tosca_mesh *m, const Geo &g,
+                            hipStream_t s) {
+    const int n = maxdim(g);
+    const PatchSet P = patches(bc_t, t_value, 6);
+    TOSCA_LAUNCH("k_temp_bcs", k_temp_bcs, dim3((n + 127) / 128, n, 6), dim3(128), 0, s, t, m->csi, m->eta, m->zet, m->aj, g, P);
+    launch_wrap(t, 1, g, s);
+}
+
+}  // namespace tosca

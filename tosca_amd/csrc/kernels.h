@@ -30,6 +30,14 @@ void launch_rhs_init(double *rhs, const double *dp, const double *bt, const doub
                      const Geo &g, hipStream_t s);
 void launch_non_resolved(double *rhs, const double *ucont_o, const tosca_params *p, const Geo &g, hipStream_t s);
 
+// boundary-condition ghost fills (bc_fill.hip)
+void launch_contravariant_bcs(double *ucont, const double *ucat, const tosca_mesh *m, const tosca_params *p,
+                              const Geo &g, hipStream_t s);
+void launch_cartesian_bcs(double *ucat, const double *u_fixed, const tosca_mesh *m, const tosca_params *p,
+                          const Geo &g, hipStream_t s);
+void launch_temperature_bcs(double *t, const int *bc_t, const double *t_value, const tosca_mesh *m, const Geo &g,
+                            hipStream_t s);
+
 // mesh validation (api.hip): counts of offending points, written to `out` (device, 4 ints)
 void launch_mesh_check(const Geo &g, const tosca_mesh *m, int *out, hipStream_t s);
 

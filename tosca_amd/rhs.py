@@ -143,6 +143,32 @@ class RhsContext:
                                                   _ptr(dp), _ptr(btheta), _ptr(btheta_o), int(ab2), _ptr(farm),
                                                   _ptr(ucont_o), _ptr(rhs), int(form_mode)))
 
+    # ---- boundary-condition ghost fills (src/boundary.c) ---------------------------------------
+    def UpdateContravariantBCs(self, mesh, params, ucont, ucat):
+        """UpdateContravariantBCs(ueqn): boundary-face fluxes of ucont from ucat, in place."""
+        self._stream()
+        m, p = mesh_struct(mesh), abi.make_params(params)
+        self._check(self.lib.tosca_update_contravariant_bcs(self.h, C.byref(m), C.byref(p), _ptr(ucont), _ptr(ucat)))
+
+    def UpdateCartesianBCs(self, mesh, params, ucat, u_fixed=None):
+        """UpdateCartesianBCs(ueqn): ghost cells of ucat, in place; u_fixed: 6 x 3 fixedValue vectors."""
+        self._stream()
+        m, p = mesh_struct(mesh), abi.make_params(params)
+        uf = None
+        if u_fixed is not None:
+            uf = (C.c_double * 18)(*[float(x) for row in u_fixed for x in row])
+        self._check(self.lib.tosca_update_cartesian_bcs(self.h, C.byref(m), C.byref(p), _ptr(ucat),
+                                                        C.cast(uf, C.c_void_p) if uf is not None else None))
+
+    def UpdateTemperatureBCs(self, mesh, params, bc_t, t_value, theta):
+        """UpdateTemperatureBCs(teqn): ghost cells of theta, in place; bc_t: 6 TOSCA_BCT_* kinds."""
+        self._stream()
+        m, p = mesh_struct(mesh), abi.make_params(params)
+        kinds = (C.c_int * 6)(*[int(b) for b in bc_t])
+        vals = (C.c_double * 6)(*[float(v) for v in t_value])
+        self._check(self.lib.tosca_update_temperature_bcs(self.h, C.byref(m), C.byref(p), C.cast(kinds, C.c_void_p),
+                                                          C.cast(vals, C.c_void_p), _ptr(theta)))
+
     # ---- instrumentation -----------------------------------------------------------------------
     def enable_timing(self, on: bool = True):
         self._check(self.lib.tosca_ctx_enable_timing(self.h, int(on)))
