@@ -1,0 +1,213 @@
+"""Checks at the full 512x512x192 ABL shard (50.3 M cells), through properties that do not depend on
+the size (the CPU oracle would need minutes per call here; its parity is established at smaller
+sizes in the other GPU tests):
+
+  - the fused kernels against the reference-structured kernels, two independently structured GPU
+    implementations each checked against the oracle at small sizes: <= 1e-12 per output;
+  - FormU's form modes: convective (1) + viscous (2) = both (0), to round-off;
+  - free-stream preservation of FormU on the shard mesh (uniform velocity, consistent fluxes);
+  - the mixed-precision steps against the FP64 step: <= 1e-5 per output field;
+  - the adjoint against central finite differences of the forward in a random direction;
+  - ExplicitRhsU fused against reference-structured.
+"""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from tosca_amd import case as C
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def shard():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    from tosca_amd import rhs
+    print("\nbuilding the 512x512x192 shard on the GPU", flush=True)
+    case = C.shard_case(device="cuda")
+    mesh = {k: v.contiguous() for k, v in case.mesh.items() if k != "cent"}
+    yield rhs, case, mesh
+    del case, mesh
+    torch.cuda.empty_cache()
+
+
+def _ctx(rhs, lay, path):
+    old = os.environ.get("TOSCA_PATH")
+    os.environ["TOSCA_PATH"] = path
+    try:
+        return rhs.RhsContext(lay.info())
+    finally:
+        if old is None:
+            os.environ.pop("TOSCA_PATH", None)
+        else:
+            os.environ["TOSCA_PATH"] = old
+
+
+def _rel(a, b):
+    d = float(torch.linalg.vector_norm((a - b).double()))
+    n = float(torch.linalg.vector_norm(b.double()))
+    return d / n if n > 0 else d
+
+
+def _outs(lay, dt=torch.float64):
+    gl = (lay.mz, lay.my, lay.mx, 3)
+    return dict(rhs=torch.zeros(gl, dtype=torch.float64, device="cuda"),
+                btheta=torch.zeros(gl, dtype=torch.float64, device="cuda"),
+                cs=torch.zeros(lay.gshape, dtype=torch.float64, device="cuda"),
+                nut=torch.zeros(lay.gshape, dtype=dt, device="cuda"))
+
+
+def test_fused_vs_reference_structured_step(shard):
+    rhs, case, mesh = shard
+    lay = case.lay
+    outs = {}
+    for path in ("fused", "ref"):
+        ctx = _ctx(rhs, lay, path)
+        o = _outs(lay)
+        ctx.rhs_les_step(mesh, case.params, case.ucont, case.ucat, case.theta, o["rhs"], 1.0, o["cs"], o["nut"],
+                         o["btheta"], 1.0)
+        torch.cuda.synchronize()
+        outs[path] = o
+        ctx.close()
+    for k in ("rhs", "cs", "nut", "btheta"):
+        e = _rel(outs["fused"][k], outs["ref"][k])
+        print(f"  {k:7s} fused vs reference-structured relL2={e:.2e}", flush=True)
+        assert e <= 1e-12, k
+
+
+def test_formmode_split(shard):
+    rhs, case, mesh = shard
+    lay = case.lay
+    ctx = _ctx(rhs, lay, "fused")
+    o = _outs(lay)
+    ctx.UpdateCs(mesh, case.params, case.ucat, o["cs"])
+    ctx.UpdateNut(mesh, case.params, case.ucat, o["cs"], o["nut"])
+    r = {}
+    for mode in (0, 1, 2):
+        r[mode] = torch.zeros_like(o["rhs"])
+        ctx.FormU(mesh, case.params, case.ucont, case.ucat, o["nut"], r[mode], 1.0, mode)
+    torch.cuda.synchronize()
+    e = _rel(r[1] + r[2], r[0])
+    print(f"  formMode 1 + 2 vs 0 relL2={e:.2e}", flush=True)
+    assert e <= 1e-13
+    ctx.close()
+
+
+def test_free_stream_preservation(shard):
+    """uniform velocity with the face fluxes S . U of the same velocity: FormU vanishes (the
+    viscous part exactly, the convective part to the round-off of the metric identities)"""
+    rhs, case, mesh = shard
+    lay = case.lay
+    U0 = torch.tensor([7.0, -2.0, 0.5], dtype=torch.float64, device="cuda")
+    ucat = U0.expand(lay.gshape + (3,)).contiguous()
+    ucont = torch.stack([(mesh["icsi"] * U0).sum(-1), (mesh["jeta"] * U0).sum(-1), (mesh["kzet"] * U0).sum(-1)],
+                        dim=-1).contiguous()
+    nut = torch.full(lay.gshape, 0.3, dtype=torch.float64, device="cuda")
+    ctx = _ctx(rhs, lay, "fused")
+    r = {}
+    for mode in (1, 2):
+        r[mode] = torch.zeros((lay.mz, lay.my, lay.mx, 3), dtype=torch.float64, device="cuda")
+        ctx.FormU(mesh, case.params, ucont, ucat, nut, r[mode], 1.0, mode)
+    torch.cuda.synchronize()
+    # scale of one face flux: |V| |U|
+    scale = float(ucont.abs().max()) * float(U0.abs().max())
+    conv = float(r[1][1:-1, 1:-1, 1:-1].abs().max()) / scale
+    visc = float(r[2][1:-1, 1:-1, 1:-1].abs().max())
+    print(f"  free stream: convective max/scale={conv:.2e}, viscous max={visc:.2e}", flush=True)
+    assert conv <= 1e-12
+    assert visc == 0.0
+    ctx.close()
+
+
+def test_mixed_against_fp64(shard):
+    rhs, case, mesh = shard
+    lay = case.lay
+    ctx = _ctx(rhs, lay, "fused")
+    ref = _outs(lay)
+    ctx.rhs_les_step(mesh, case.params, case.ucont, case.ucat, case.theta, ref["rhs"], 1.0, ref["cs"], ref["nut"],
+                     ref["btheta"], 1.0)
+    V32, u32, t32 = case.ucont.float().contiguous(), case.ucat.float().contiguous(), case.theta.float().contiguous()
+    mesh32 = {k: mesh[k].float().contiguous() for k in ("csi", "eta", "zet", "icsi", "ieta", "izet", "jcsi", "jeta",
+                                                         "jzet", "kcsi", "keta", "kzet", "iaj", "jaj", "kaj")}
+    for variant in ("fp64 metrics", "fp32 metric copies"):
+        o = _outs(lay, torch.float32)
+        if variant == "fp64 metrics":
+            ctx.rhs_les_step_mixed(mesh, case.params, V32, u32, t32, o["rhs"], 1.0, o["cs"], o["nut"], o["btheta"], 1.0)
+        else:
+            ctx.rhs_les_step_mixed_m32(mesh, mesh32, case.params, V32, u32, t32, o["rhs"], 1.0, o["cs"], o["nut"],
+                                       o["btheta"], 1.0)
+        torch.cuda.synchronize()
+        for k in ("rhs", "cs", "nut", "btheta"):
+            e = _rel(o[k].double(), ref[k])
+            print(f"  mixed ({variant}) {k:7s} relL2={e:.2e}", flush=True)
+            assert e <= 1e-5, (variant, k)
+    del mesh32
+    ctx.close()
+
+
+def test_vjp_against_finite_differences(shard):
+    """<rbar, (R(x + eps d) - R(x - eps d)) / (2 eps)> = <J^T rbar, d> for a random direction d in
+    ucat (through FormU and nu_t) and cs"""
+    rhs, case, mesh = shard
+    lay = case.lay
+    ctx = _ctx(rhs, lay, "fused")
+    g = torch.Generator(device="cuda").manual_seed(21)
+    cs = torch.zeros(lay.gshape, dtype=torch.float64, device="cuda")
+    ctx.UpdateCs(mesh, case.params, case.ucat, cs)
+    gl = (lay.mz, lay.my, lay.mx, 3)
+    rbar = torch.randn(gl, dtype=torch.float64, device="cuda", generator=g)
+    du = torch.randn(lay.gshape + (3,), dtype=torch.float64, device="cuda", generator=g)
+    dc = torch.randn(lay.gshape, dtype=torch.float64, device="cuda", generator=g) * 1e-3
+    nut = torch.zeros(lay.gshape, dtype=torch.float64, device="cuda")
+
+    def forward(u, c):
+        r = torch.zeros(gl, dtype=torch.float64, device="cuda")
+        bt = torch.zeros(gl, dtype=torch.float64, device="cuda")
+        ctx.UpdateNut(mesh, case.params, u, c, nut)
+        ctx.FormU(mesh, case.params, case.ucont, u, nut, r, 1.0, 0)
+        ctx.Buoyancy(mesh, case.params, case.theta, 1.0, bt)
+        return r + bt
+
+    eps = 1e-6
+    Rp = forward((case.ucat + eps * du).contiguous(), (cs + eps * dc).contiguous())
+    Rm = forward((case.ucat - eps * du).contiguous(), (cs - eps * dc).contiguous())
+    fd = float((rbar * (Rp - Rm)).sum()) / (2 * eps)
+    del Rp, Rm
+    bars = [torch.empty_like(t) for t in (case.ucont, case.ucat, case.theta, cs)]
+    ctx.rhs_vjp(mesh, case.params, case.ucont, case.ucat, case.theta, cs, rbar, 1.0, 1.0, *bars)
+    torch.cuda.synchronize()
+    adj = float((bars[1] * du).sum()) + float((bars[3] * dc).sum())
+    e = abs(adj - fd) / abs(fd)
+    print(f"  <J^T rbar, d> = {adj:.10e}, finite difference {fd:.10e}, rel {e:.2e}", flush=True)
+    assert e <= 1e-6
+    ctx.close()
+
+
+def test_explicit_rhs_u_fused_vs_reference_structured(shard):
+    rhs, case, mesh = shard
+    lay = case.lay
+    gl = (lay.mz, lay.my, lay.mx, 3)
+    g = torch.Generator(device="cuda").manual_seed(31)
+    dp, bt, bto, farm, uo = (torch.randn(gl, dtype=torch.float64, device="cuda", generator=g) for _ in range(5))
+    nut = torch.zeros(lay.gshape, dtype=torch.float64, device="cuda")
+    cs = torch.zeros(lay.gshape, dtype=torch.float64, device="cuda")
+    res = {}
+    for path in ("fused", "ref"):
+        ctx = _ctx(rhs, lay, path)
+        if path == "fused":
+            ctx.UpdateCs(mesh, case.params, case.ucat, cs)
+            ctx.UpdateNut(mesh, case.params, case.ucat, cs, nut)
+        V, U = case.ucont.clone(), case.ucat.clone()
+        r = torch.empty(gl, dtype=torch.float64, device="cuda")
+        ctx.ExplicitRhsU(mesh, case.params, V, U, nut, dp, bt, bto, 1, farm, uo, r, 0)
+        torch.cuda.synchronize()
+        res[path] = (V, U, r)
+        ctx.close()
+    assert torch.equal(res["fused"][0], res["ref"][0])
+    assert torch.equal(res["fused"][1], res["ref"][1])
+    e = _rel(res["fused"][2], res["ref"][2])
+    print(f"  ExplicitRhsU rhs fused vs reference-structured relL2={e:.2e}", flush=True)
+    assert e <= 1e-12
