@@ -251,6 +251,53 @@ def explicit_leg(torch, ctx, mesh, case, ucont, ucat, nut, lay, args):
             "terms": "-dP + 3/2 bTheta - 1/2 bTheta_o + farm + FormU(formMode 0), synthetic sources"}
 
 
+# FormT, per interior cell: theta 8, ucont 24, nine face vectors 216, three face Jacobians 24,
+# aj 8, the closure's diffusivity (k_t or nu_t) 8, rhs written 8
+FORMT_BYTES = 8 + 24 + 216 + 24 + 8 + 8 + 8   # = 296
+
+
+def formt_leg(torch, ctx, mesh, case, ucont, theta, nut, lay, args):
+    """FormT (tosca_formt) on the step's fields, with the two closures it distinguishes under LES:
+    the subgrid diffusivity k_t (here synthetic, 3 nu_t) and nu_t over the stability-corrected
+    turbulent Prandtl number."""
+    dev = "cuda"
+    kt = (3.0 * nut).contiguous()
+    rhs = torch.empty((lay.mz, lay.my, lay.mx), dtype=torch.float64, device=dev)
+    tp = dict(pr=0.7, wall_t=(0, 0, 0, 0), q_wall_i=None, q_wall_j=None)
+    out = {}
+    for name, model in (("k_t", 2), ("prandtl", 0)):
+        params = dict(case.params, les_model=model)
+
+        def run():
+            ctx.FormT(mesh, params, tp, ucont, theta, nut, kt, rhs, 1.0, 0)
+
+        for _ in range(args.warmup):
+            run()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(args.steps):
+            run()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / args.steps
+        ctx.enable_timing(True)
+        ctx.reset_timing()
+        for _ in range(args.steps):
+            run()
+        tim = ctx.timings()
+        ctx.enable_timing(False)
+        value = lay.ncells / (ms * 1e-3)
+        gbs = FORMT_BYTES * value / 1e9
+        print(f"# formt ({name}): {ms:.3f} ms, {value:.3e} cells/s, {gbs:.0f} GB/s algorithmic", file=sys.stderr,
+              flush=True)
+        out[name] = {"ms": ms, "value": value, "unit": "cell-updates/s", "achieved": gbs, "peak": HBM_PEAK_GBS,
+                     "frac": gbs / HBM_PEAK_GBS, "kernels_ms": {k: v[0] / v[1] for k, v in tim.items()}}
+    out["algorithmic_bytes_per_cell"] = FORMT_BYTES
+    del kt, rhs
+    return out
+
+
 def workload_name(cfg: int, case) -> str:
     lay = case.lay
     n = f"{lay.mz - 2}x{lay.mx - 2}x{lay.my - 2}"
@@ -404,6 +451,7 @@ def main():
     line["mixed"] = mixed_leg(torch, ctx, mesh, case, ucont, ucat, theta, lay, args, ms)
     line["vjp"] = vjp_leg(torch, ctx, mesh, case, ucont, ucat, theta, cs, lay, args, ms)
     line["explicit_rhs_u"] = explicit_leg(torch, ctx, mesh, case, ucont, ucat, nut, lay, args)
+    line["formt"] = formt_leg(torch, ctx, mesh, case, ucont, theta, nut, lay, args)
     if not args.no_cpu:
         spec = C.CONFIGS[args.config]
         line["cpu_baseline"] = cpu_baseline((spec["builder"], spec["kwargs"]), args.cpu_seconds)

@@ -246,6 +246,36 @@ int tosca_explicit_rhs_u(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_par
                          const double *btheta, const double *btheta_o, int ab2, const double *farm,
                          const double *ucont_o, double *rhs, int form_mode);
 
+/* ---- temperature equation (the sibling of FormU) ----------------------------------------------- */
+
+/* Inputs of FormT beyond tosca_params (teqn_ / constants_ members). */
+typedef struct {
+    double pr;              /* constants->Pr                                                         */
+    int    wall_t[4];       /* thetaWallFunction on iLeft, iRight, jLeft, jRight (boundaryT)         */
+    /* wall heat flux of the thermal wall models, read where wall_t is set and LES is active:
+     * three planes x|y|z, each [zm][ym] (i patches: teqn->iRWM->qWall, for both i patches as the
+     * reference reads it, teqn_solvers.c:212-229) or [zm][xm] (j patches: jLWM->qWall, :337-354);
+     * device pointers, may be NULL where no patch needs them                                     */
+    const double *q_wall_i;
+    const double *q_wall_j;
+} tosca_teqn_params;
+
+/* FormT (src/solvers/teqn_solvers.c:6-592): rhs (global scalar) is OVERWRITTEN at the interior
+ * cells with scale * (divergence budget [form_mode != 2] + diffusion budget [form_mode != 1]) * aj.
+ * Face fluxes: convection -ucont * weno3 of theta on the four-cell stencil (inline.h:5147-5179,
+ * getFace2Cell4Stencil*); diffusion (g . grad theta) * face Jacobian * kappa_eff with
+ *   LES off:  nu / Pr;
+ *   LES on (the molecular part is dropped, as in the reference):
+ *     AMD, dynamicSmagorinsky, dynamicLASI, dynamicLASD, dynamicPASD: 0.5 (kt_L + kt_R) (les->lk_t);
+ *     smagorinsky, vreman: nu_t / Prt with the stability-corrected Prt (teqn_solvers.c:190-200);
+ *     thetaWallFunction faces: the wall heat flux, kappa 0.
+ * Then the periodic reset of the face fluxes (resetFacePeriodicFluxesVector).  kt is read for the
+ * dynamic models only (may be NULL otherwise); nut whenever LES is active.  Stability-based and
+ * structural models (bardinaVreman / bardinaAMD) return TOSCA_ERR_UNSUPPORTED. */
+int tosca_formt(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_params *params, const tosca_teqn_params *tparams,
+                const double *ucont, const double *theta, const double *nut, const double *kt, double *rhs,
+                double scale, int form_mode);
+
 /* ---- boundary-condition ghost fills (the step before the path) ------------------------------ */
 
 /* UpdateContravariantBCs (src/boundary.c:614-822) on lUcont (local AoS, in place).  On every

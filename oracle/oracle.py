@@ -49,6 +49,7 @@ def load() -> C.CDLL:
         "oracle_contravariant_bcs": (I, [In, M, Pa, P, P]),
         "oracle_cartesian_bcs": (I, [In, M, Pa, P, P]),
         "oracle_temperature_bcs": (I, [In, M, Pa, P, P, P]),
+        "oracle_formt": (I, [In, M, Pa, C.POINTER(abi.TeqnParams), P, P, P, P, P, D, I]),
     }
     for n, (r, a) in sigs.items():
         f = getattr(lib, n)
@@ -177,3 +178,13 @@ def temperature_bcs(info, mesh, params, bc_t, t_value, theta):
     kinds = np.ascontiguousarray(bc_t, dtype=np.int32)
     vals = np.ascontiguousarray(t_value, dtype=np.float64)
     _check(lib.oracle_temperature_bcs(C.byref(inf), C.byref(m), C.byref(p), kinds.ctypes.data, _p(vals), _p(theta)))
+
+
+def formt(info, mesh, params, tparams, ucont, theta, nut, kt, rhs, scale=1.0, form_mode=0):
+    """FormT restated; rhs (global scalar, shape (mz, my, mx)) is overwritten at interior cells.
+    tparams: dict(pr=..., wall_t=(4 flags), q_wall_i=array or None, q_wall_j=array or None)."""
+    lib = load()
+    inf, m, p = abi.make_info(info), _mesh(mesh), abi.make_params(params)
+    tp = abi.make_teqn_params({k: (_p(v) if k.startswith("q_wall") and v is not None else v) for k, v in tparams.items()})
+    _check(lib.oracle_formt(C.byref(inf), C.byref(m), C.byref(p), C.byref(tp), _p(ucont), _p(theta), _p(nut), _p(kt),
+                            _p(rhs), float(scale), int(form_mode)))

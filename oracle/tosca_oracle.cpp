@@ -1192,6 +1192,187 @@ int oracle_explicit_rhs_u(const tosca_dmda_info *info, const tosca_mesh *m, cons
 
 #ifndef ORACLE_TANGENT
 // ================================================================================================
+// FormT (src/solvers/teqn_solvers.c:6-592), nvert = meshTag = 0
+// ================================================================================================
+
+// Compute_dscalar_i / _j / _k (src/include/inline.h:3210-3342): one normal difference and two
+// four-point transverse averages, one-sided next to non-periodic boundary edges
+static void dscalar(const Lay &l, const double *K, int dir, int64_t i, int64_t j, int64_t k, double *dc, double *de,
+                    double *dz) {
+    const bool iN = !l.i_per() && !l.ii_per(), jN = !l.j_per() && !l.jj_per(), kN = !l.k_per() && !l.kk_per();
+    auto T = [&](int64_t kk, int64_t jj, int64_t ii) { return K[l.L(kk, jj, ii)]; };
+    const int64_t mx = l.mx, my = l.my, mz = l.mz;
+    if (dir == 0) {
+        *dc = T(k, j, i + 1) - T(k, j, i);
+        if (j == my - 2 && (i == 0 || i == mx - 2) && (jN || iN)) *de = (T(k, j, i + 1) + T(k, j, i) - T(k, j - 1, i + 1) - T(k, j - 1, i)) * 0.5;
+        else if (j == 1 && (i == 0 || i == mx - 2) && (jN || iN)) *de = (T(k, j + 1, i + 1) + T(k, j + 1, i) - T(k, j, i + 1) - T(k, j, i)) * 0.5;
+        else *de = (T(k, j + 1, i + 1) + T(k, j + 1, i) - T(k, j - 1, i + 1) - T(k, j - 1, i)) * 0.25;
+        if (k == mz - 2 && (i == 0 || i == mx - 2) && (kN || iN)) *dz = (T(k, j, i + 1) + T(k, j, i) - T(k - 1, j, i + 1) - T(k - 1, j, i)) * 0.5;
+        else if (k == 1 && (i == 0 || i == mx - 2) && (kN || iN)) *dz = (T(k + 1, j, i + 1) + T(k + 1, j, i) - T(k, j, i + 1) - T(k, j, i)) * 0.5;
+        else *dz = (T(k + 1, j, i + 1) + T(k + 1, j, i) - T(k - 1, j, i + 1) - T(k - 1, j, i)) * 0.25;
+    } else if (dir == 1) {
+        if (i == mx - 2 && (j == 0 || j == my - 2) && (iN || jN)) *dc = (T(k, j + 1, i) + T(k, j, i) - T(k, j + 1, i - 1) - T(k, j, i - 1)) * 0.5;
+        else if (i == 1 && (j == 0 || j == my - 2) && (iN || jN)) *dc = (T(k, j + 1, i + 1) + T(k, j, i + 1) - T(k, j + 1, i) - T(k, j, i)) * 0.5;
+        else *dc = (T(k, j + 1, i + 1) + T(k, j, i + 1) - T(k, j + 1, i - 1) - T(k, j, i - 1)) * 0.25;
+        *de = T(k, j + 1, i) - T(k, j, i);
+        if (k == mz - 2 && (j == 0 || j == my - 2) && (kN || jN)) *dz = (T(k, j + 1, i) + T(k, j, i) - T(k - 1, j + 1, i) - T(k - 1, j, i)) * 0.5;
+        else if (k == 1 && (j == 0 || j == my - 2) && (kN || jN)) *dz = (T(k + 1, j + 1, i) + T(k + 1, j, i) - T(k, j + 1, i) - T(k, j, i)) * 0.5;
+        else *dz = (T(k + 1, j + 1, i) + T(k + 1, j, i) - T(k - 1, j + 1, i) - T(k - 1, j, i)) * 0.25;
+    } else {
+        if (i == mx - 2 && (k == 0 || k == mz - 2) && (iN || kN)) *dc = (T(k + 1, j, i) + T(k, j, i) - T(k + 1, j, i - 1) - T(k, j, i - 1)) * 0.5;
+        else if (i == 1 && (k == 0 || k == mz - 2) && (iN || kN)) *dc = (T(k + 1, j, i + 1) + T(k, j, i + 1) - T(k + 1, j, i) - T(k, j, i)) * 0.5;
+        else *dc = (T(k + 1, j, i + 1) + T(k, j, i + 1) - T(k + 1, j, i - 1) - T(k, j, i - 1)) * 0.25;
+        if (j == my - 2 && (k == 0 || k == mz - 2) && (jN || kN)) *de = (T(k + 1, j, i) + T(k, j, i) - T(k + 1, j - 1, i) - T(k, j - 1, i)) * 0.5;
+        else if (j == 1 && (k == 0 || k == mz - 2) && (jN || kN)) *de = (T(k + 1, j + 1, i) + T(k, j + 1, i) - T(k + 1, j, i) - T(k, j, i)) * 0.5;
+        else *de = (T(k + 1, j + 1, i) + T(k, j + 1, i) - T(k + 1, j - 1, i) - T(k, j - 1, i)) * 0.25;
+        *dz = T(k + 1, j, i) - T(k, j, i);
+    }
+}
+
+// weno3 (src/include/inline.h:5147-5179); pow(x, 2.) is the exactly rounded square
+static double weno3(double f0, double f1, double f2, double f3, double wavespeed) {
+    double fL, fC, fR;
+    if (wavespeed > 0) { fL = f0; fC = f1; fR = f2; }
+    else { fL = f3; fC = f2; fR = f1; }
+    const double d0 = 2. / 3., d1 = 1. / 3.;
+    const double eps = 1.e-6;
+    const double beta0 = (fC - fR) * (fC - fR);
+    const double beta1 = (fL - fC) * (fL - fC);
+    const double alpha0 = d0 / ((eps + beta0) * (eps + beta0));
+    const double alpha1 = d1 / ((eps + beta1) * (eps + beta1));
+    const double sumalpha = alpha0 + alpha1;
+    const double w0 = alpha0 / sumalpha, w1 = alpha1 / sumalpha;
+    const double u0 = (fC * 0.5 + fR * 0.5);
+    const double u1 = (-fL * 0.5 + fC * 1.5);
+    return w0 * u0 + w1 * u1;
+}
+
+// models whose FormT diffusivity is les->lk_t (teqn_solvers.c:180-189)
+static bool kt_model(int m) {
+    return m == TOSCA_LES_AMD || m == TOSCA_LES_DSM || m == TOSCA_LES_DLASI || m == TOSCA_LES_DLASD ||
+           m == TOSCA_LES_DPASD || m == TOSCA_LES_BAMD;
+}
+
+int oracle_formt(const tosca_dmda_info *info, const tosca_mesh *m, const tosca_params *p, const tosca_teqn_params *tp,
+                 const double *ucont_, const double *tmprt, const double *nut, const double *kt, double *rhs,
+                 double scale, int formMode) {
+    Lay l;
+    int rc = check_layout(info, p, l);
+    if (rc) return rc;
+    if (!tp || formMode < 0 || formMode > 2) { g_err = "formt: bad arguments"; return TOSCA_ERR_ARG; }
+    const bool les = p->les_active != 0;
+    if (les && (p->les_model == TOSCA_LES_STABILITY || p->les_model == TOSCA_LES_BV || p->les_model == TOSCA_LES_BAMD)) {
+        g_err = "formt: stability-based and structural LES models are not covered";
+        return TOSCA_ERR_UNSUPPORTED;
+    }
+    const int64_t mx = l.mx, my = l.my, mz = l.mz, n = l.nloc();
+    const V3 *ucont = (const V3 *)ucont_;
+    const V3 *icsi = (const V3 *)m->icsi, *ieta = (const V3 *)m->ieta, *izet = (const V3 *)m->izet;
+    const V3 *jcsi = (const V3 *)m->jcsi, *jeta = (const V3 *)m->jeta, *jzet = (const V3 *)m->jzet;
+    const V3 *kcsi = (const V3 *)m->kcsi, *keta = (const V3 *)m->keta, *kzet = (const V3 *)m->kzet;
+    const double *iaj = m->iaj, *jaj = m->jaj, *kaj = m->kaj, *aj = m->aj;
+    const double Pr = tp->pr, tRef = p->t_ref;
+    const bool ktm = les && kt_model(p->les_model);
+    std::vector<V3> div(n, V3{0, 0, 0}), visc(n, V3{0, 0, 0});
+    // kappa_eff of one face (teqn_solvers.c:168-240 and the j / k copies); ajc the face Jacobian
+    auto kappa = [&](int64_t c, int64_t cn, double ajc, double dtde) {
+        double nu = p->nu;
+        if (!les) return nu / Pr;
+        nu = 0;
+        const double nutf = 0.5 * (nut[c] + nut[cn]);
+        if (ktm) {
+            const double diff = 0.5 * (kt[c] + kt[cn]);
+            return (nu / Pr) + diff;
+        }
+        const double gradTdotG = dtde * (-9.81);
+        double l_, delta = pow(1. / ajc, 1. / 3.);
+        if (gradTdotG < 0.) l_ = pmin(delta, 7.6 * nutf / delta * sqrt(tRef / fabs(gradTdotG)));
+        else l_ = delta;
+        const double Prt = 1.0 / (1.0 + (2.0 * l_ / delta));
+        return (nu / Pr) + (nutf / Prt);
+    };
+    for (int64_t k = 0; k < mz - 1; k++)
+        for (int64_t j = 0; j < my - 1; j++)
+            for (int64_t i = 0; i < mx - 1; i++) {
+                const int64_t c = l.L(k, j, i);
+                double dtdc, dtde, dtdz;
+                if (j != 0 && k != 0) {                      // i face (:122-246)
+                    const double ajc = iaj[c];
+                    const V3 cs = icsi[c], et = ieta[c], ze = izet[c];
+                    const double g11 = cs.x * cs.x + cs.y * cs.y + cs.z * cs.z;
+                    const double g21 = et.x * cs.x + et.y * cs.y + et.z * cs.z;
+                    const double g31 = ze.x * cs.x + ze.y * cs.y + ze.z * cs.z;
+                    dscalar(l, tmprt, 0, i, j, k, &dtdc, &dtde, &dtdz);
+                    int64_t iL, iR;
+                    face2cell4(l.per[0], i, mx, &iL, &iR);
+                    div[c].x = -ucont[c].x * weno3(tmprt[l.L(k, j, iL)], tmprt[c], tmprt[l.L(k, j, i + 1)],
+                                                   tmprt[l.L(k, j, iR)], ucont[c].x);
+                    double kap = kappa(c, l.L(k, j, i + 1), ajc, dtde);
+                    if (les && ((tp->wall_t[0] && i == 0) || (tp->wall_t[1] && i == mx - 2))) {
+                        const double signQ = (i == 0) ? -1.0 : 1.0;
+                        const double *q = tp->q_wall_i;
+                        const int64_t pl = mz * my, o = k * my + j;
+                        visc[c].x = signQ * (q[o] * cs.x + q[pl + o] * cs.y + q[2 * pl + o] * cs.z);
+                        kap = 0.0;
+                    }
+                    visc[c].x += (g11 * dtdc + g21 * dtde + g31 * dtdz) * ajc * kap;
+                }
+                if (i != 0 && k != 0) {                      // j face (:248-370)
+                    const double ajc = jaj[c];
+                    const V3 cs = jcsi[c], et = jeta[c], ze = jzet[c];
+                    const double g11 = cs.x * et.x + cs.y * et.y + cs.z * et.z;
+                    const double g21 = et.x * et.x + et.y * et.y + et.z * et.z;
+                    const double g31 = ze.x * et.x + ze.y * et.y + ze.z * et.z;
+                    dscalar(l, tmprt, 1, i, j, k, &dtdc, &dtde, &dtdz);
+                    int64_t jL, jR;
+                    face2cell4(l.per[1], j, my, &jL, &jR);
+                    div[c].y = -ucont[c].y * weno3(tmprt[l.L(k, jL, i)], tmprt[c], tmprt[l.L(k, j + 1, i)],
+                                                   tmprt[l.L(k, jR, i)], ucont[c].y);
+                    double kap = kappa(c, l.L(k, j + 1, i), ajc, dtde);
+                    if (les && ((tp->wall_t[2] && j == 0) || (tp->wall_t[3] && j == my - 2))) {
+                        const double signQ = (j == 0) ? -1.0 : 1.0;
+                        const double *q = tp->q_wall_j;
+                        const int64_t pl = mz * mx, o = k * mx + i;
+                        visc[c].y = signQ * (q[o] * et.x + q[pl + o] * et.y + q[2 * pl + o] * et.z);
+                        kap = 0.0;
+                    }
+                    visc[c].y += (g11 * dtdc + g21 * dtde + g31 * dtdz) * ajc * kap;
+                }
+                if (i != 0 && j != 0) {                      // k face (:372-478)
+                    const double ajc = kaj[c];
+                    const V3 cs = kcsi[c], et = keta[c], ze = kzet[c];
+                    const double g11 = cs.x * ze.x + cs.y * ze.y + cs.z * ze.z;
+                    const double g21 = et.x * ze.x + et.y * ze.y + et.z * ze.z;
+                    const double g31 = ze.x * ze.x + ze.y * ze.y + ze.z * ze.z;
+                    dscalar(l, tmprt, 2, i, j, k, &dtdc, &dtde, &dtdz);
+                    int64_t kL, kR;
+                    face2cell4(l.per[2], k, mz, &kL, &kR);
+                    div[c].z = -ucont[c].z * weno3(tmprt[l.L(kL, j, i)], tmprt[c], tmprt[l.L(k + 1, j, i)],
+                                                   tmprt[l.L(kR, j, i)], ucont[c].z);
+                    const double kap = kappa(c, l.L(k + 1, j, i), ajc, dtde);
+                    visc[c].z += (g11 * dtdc + g21 * dtde + g31 * dtdz) * ajc * kap;
+                }
+            }
+    local_to_local(l, div.data());                            // :487-490
+    local_to_local(l, visc.data());
+    reset_face_periodic(l, div.data());                       // :502-503
+    reset_face_periodic(l, visc.data());
+    for (int64_t k = 1; k < mz - 1; k++)                      // :511-537
+        for (int64_t j = 1; j < my - 1; j++)
+            for (int64_t i = 1; i < mx - 1; i++) {
+                const int64_t c = l.L(k, j, i);
+                const double divBudget = div[c].x - div[l.L(k, j, i - 1)].x + div[c].y - div[l.L(k, j - 1, i)].y +
+                                         div[c].z - div[l.L(k - 1, j, i)].z;
+                const double viscBudget = visc[c].x - visc[l.L(k, j, i - 1)].x + visc[c].y - visc[l.L(k, j - 1, i)].y +
+                                          visc[c].z - visc[l.L(k - 1, j, i)].z;
+                rhs[l.G(k, j, i)] = scale * ((formMode != 2 ? divBudget : 0.0) + (formMode != 1 ? viscBudget : 0.0)) * aj[c];
+            }
+    return TOSCA_OK;
+}
+#endif  // ORACLE_TANGENT
+
+#ifndef ORACLE_TANGENT
+// ================================================================================================
 // Boundary-condition ghost fills (src/boundary.c).  Each routine reads the local array as it was on
 // entry and writes a global working copy (the owned range), which the GlobalToLocal scatter then
 // copies back and wraps; on one rank the copy is the owned part of the local array.

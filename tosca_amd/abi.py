@@ -62,6 +62,23 @@ class Params(C.Structure):
     ]
 
 
+class TeqnParams(C.Structure):
+    _fields_ = [("pr", C.c_double), ("wall_t", C.c_int * 4), ("q_wall_i", C.c_void_p), ("q_wall_j", C.c_void_p)]
+
+
+def make_teqn_params(d: Mapping) -> TeqnParams:
+    """tosca_teqn_params from a dict: pr, wall_t (4 flags), q_wall_i / q_wall_j (device or host
+    pointers as ints, or None)."""
+    t = TeqnParams()
+    t.pr = float(d.get("pr", 0.7))
+    w = tuple(d.get("wall_t", (0, 0, 0, 0)))
+    for q in range(4):
+        t.wall_t[q] = int(w[q])
+    t.q_wall_i = d.get("q_wall_i") or None
+    t.q_wall_j = d.get("q_wall_j") or None
+    return t
+
+
 SIGNATURES = {
     "tosca_version": (C.c_char_p, []),
     "tosca_last_error": (C.c_char_p, [P]),
@@ -87,6 +104,7 @@ SIGNATURES = {
     "tosca_update_contravariant_bcs": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P]),
     "tosca_update_cartesian_bcs": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P]),
     "tosca_update_temperature_bcs": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P]),
+    "tosca_formt": (I, [P, C.POINTER(Mesh), C.POINTER(Params), C.POINTER(TeqnParams), P, P, P, P, P, D, I]),
 }
 
 _lib: Optional[C.CDLL] = None

@@ -36,6 +36,7 @@ SOURCES = {
     "vjp.hip": [],
     "explicit_rhs.hip": ["-ffp-contract=off"],
     "bc_fill.hip": ["-ffp-contract=off"],
+    "formt.hip": ["-ffp-contract=off"],
 }
 
 

@@ -36,6 +36,7 @@ BCU_GENERIC, BCU_SLIP, BCU_WALLFUNCTION, BCU_PERIODIC, BCU_NOSLIP, BCU_ZEROGRADI
 BCT_ZEROGRADIENT, BCT_FIXEDVALUE, BCT_FIXEDGRADIENT, BCT_PERIODIC = 0, 1, 2, 3
 BCNUT_NONE, BCNUT_ZEROGRADIENT, BCNUT_FIXEDVALUE, BCNUT_PERIODIC = 0, 1, 2, 3
 LES_SMAGORINSKY, LES_DSM, LES_DPASD = 0, 2, 5
+LES_STABILITY, LES_DLASI, LES_DLASD, LES_AMD, LES_VREMAN, LES_BV, LES_BAMD = 1, 3, 4, 6, 7, 8, 9
 
 MESH_VECTORS = ("csi", "eta", "zet", "icsi", "ieta", "izet", "jcsi", "jeta", "jzet",
                 "kcsi", "keta", "kzet")

@@ -485,6 +485,49 @@ int tosca_explicit_rhs_u(tosca_ctx *c, const tosca_mesh *m, const tosca_params *
     return launched(c, "ExplicitRhsU");
 }
 
+int tosca_formt(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const tosca_teqn_params *tp,
+                const double *ucont, const double *theta, const double *nut, const double *kt, double *rhs,
+                double scale, int form_mode) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if ((rc = check_params(c, p))) return rc;
+    if (!tp) return set_err(c, TOSCA_ERR_ARG, "teqn params is NULL");
+    if (form_mode < 0 || form_mode > 2) return set_err(c, TOSCA_ERR_ARG, "form_mode = %d is not 0, 1 or 2", form_mode);
+    if (!(tp->pr > 0.0)) return set_err(c, TOSCA_ERR_ARG, "Pr = %g must be positive", tp->pr);
+    const bool les = p->les_active != 0;
+    if (les) {
+        if (p->les_model < 0 || p->les_model > TOSCA_LES_BAMD)
+            return set_err(c, TOSCA_ERR_ARG, "les_model = %d is not an LES model", p->les_model);
+        if (p->les_model == TOSCA_LES_STABILITY)
+            return set_err(c, TOSCA_ERR_UNSUPPORTED, "the stability-based LES model is not supported");
+        if (p->les_model == TOSCA_LES_BV || p->les_model == TOSCA_LES_BAMD)
+            return set_err(c, TOSCA_ERR_UNSUPPORTED, "structural LES models (bardinaVreman/bardinaAMD) are not supported");
+    }
+    set_geo_per(c, p);
+    const Geo &g = c->geo;
+    const size_t n = (size_t)g.nloc(), ng = (size_t)g.mx * g.my * g.mz * sizeof(double);
+    if ((rc = check_mesh(c, m, true, true, true))) return rc;
+    if ((rc = check_dev(c, ucont, 3 * n * sizeof(double), "ucont")) ||
+        (rc = check_dev(c, theta, n * sizeof(double), "theta")) || (rc = check_dev(c, rhs, ng, "rhs")))
+        return rc;
+    if (les) {
+        const bool ktm = p->les_model == TOSCA_LES_AMD || p->les_model == TOSCA_LES_DSM ||
+                         p->les_model == TOSCA_LES_DLASI || p->les_model == TOSCA_LES_DLASD ||
+                         p->les_model == TOSCA_LES_DPASD;
+        if ((rc = check_dev(c, nut, n * sizeof(double), "nut"))) return rc;
+        if (ktm && (rc = check_dev(c, kt, n * sizeof(double), "kt"))) return rc;
+        if ((tp->wall_t[0] || tp->wall_t[1]) &&
+            (rc = check_dev(c, tp->q_wall_i, 3 * (size_t)g.mz * g.my * sizeof(double), "q_wall_i")))
+            return rc;
+        if ((tp->wall_t[2] || tp->wall_t[3]) &&
+            (rc = check_dev(c, tp->q_wall_j, 3 * (size_t)g.mz * g.mx * sizeof(double), "q_wall_j")))
+            return rc;
+    }
+    if ((rc = ensure_ws(c, ws_need(c)))) return rc;
+    tosca::ref_formt(g, m, p, tp, ucont, theta, nut, kt, rhs, scale, form_mode, c->ws, c->stream);
+    return launched(c, "FormT");
+}
+
 int tosca_update_contravariant_bcs(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, double *ucont,
                                    const double *ucat) {
     int rc = check_ctx(c);

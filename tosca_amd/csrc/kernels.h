@@ -38,6 +38,11 @@ void launch_cartesian_bcs(double *ucat, const double *u_fixed, const tosca_mesh 
 void launch_temperature_bcs(double *t, const int *bc_t, const double *t_value, const tosca_mesh *m, const Geo &g,
                             hipStream_t s);
 
+// FormT (formt.hip): reference-structured face and cell kernels; ws holds 6 local doubles per point
+void ref_formt(const Geo &g, const tosca_mesh *m, const tosca_params *p, const tosca_teqn_params *tp,
+               const double *ucont, const double *theta, const double *nut, const double *kt, double *rhs,
+               double scale, int mode, double *ws, hipStream_t s);
+
 // mesh validation (api.hip): counts of offending points, written to `out` (device, 4 ints)
 void launch_mesh_check(const Geo &g, const tosca_mesh *m, int *out, hipStream_t s);
 

@@ -170,6 +170,16 @@ class RhsContext:
                                                           C.cast(vals, C.c_void_p), _ptr(theta)))
 
     # ---- instrumentation -----------------------------------------------------------------------
+    def FormT(self, mesh, params, tparams, ucont, theta, nut, kt, rhs, scale: float = 1.0, form_mode: int = 0):
+        """FormT (src/solvers/teqn_solvers.c:6-592): rhs (global scalar) at interior cells.
+        tparams: dict(pr=..., wall_t=(iLeft, iRight, jLeft, jRight) thetaWallFunction flags,
+        q_wall_i=tensor (3, mz, my) or None, q_wall_j=tensor (3, mz, mx) or None)."""
+        m, p = mesh_struct(mesh), abi.make_params(params)
+        tp = abi.make_teqn_params(dict(tparams, q_wall_i=_ptr(tparams.get("q_wall_i")),
+                                       q_wall_j=_ptr(tparams.get("q_wall_j"))))
+        self._check(self.lib.tosca_formt(self.h, C.byref(m), C.byref(p), C.byref(tp), _ptr(ucont), _ptr(theta),
+                                         _ptr(nut), _ptr(kt), _ptr(rhs), float(scale), int(form_mode)))
+
     def enable_timing(self, on: bool = True):
         self._check(self.lib.tosca_ctx_enable_timing(self.h, int(on)))
 
