@@ -244,11 +244,26 @@ def explicit_leg(torch, ctx, mesh, case, ucont, ucat, nut, lay, args):
     value = lay.ncells / (ms * 1e-3)
     gbs = EXPLICIT_BYTES * value / 1e9
     print(f"# explicit_rhs_u: {ms:.3f} ms, {value:.3e} cells/s, {gbs:.0f} GB/s algorithmic", file=sys.stderr, flush=True)
-    del dp, bt, bto, farm, uo, V, U, rhs
+    # the four-stage Runge-Kutta predictor built on it (UeqnRK4), one call per step
+    utmp = torch.empty(gl, dtype=torch.float64, device=dev)
+    for _ in range(max(1, args.warmup // 2)):
+        ctx.UeqnRK4(mesh, case.params, V, U, nut, dp, bt, bto, 1, farm, uo, 1e-3, rhs, utmp, 0)
+    torch.cuda.synchronize()
+    e0.record()
+    n_rk = max(1, args.steps // 4)
+    for _ in range(n_rk):
+        ctx.UeqnRK4(mesh, case.params, V, U, nut, dp, bt, bto, 1, farm, uo, 1e-3, rhs, utmp, 0)
+    e1.record()
+    torch.cuda.synchronize()
+    rk_ms = e0.elapsed_time(e1) / n_rk
+    print(f"# ueqn_rk4: {rk_ms:.3f} ms per predictor step ({rk_ms / ms:.2f} x ExplicitRhsU)", file=sys.stderr,
+          flush=True)
+    del dp, bt, bto, farm, uo, V, U, rhs, utmp
     return {"ms": ms, "value": value, "unit": "cell-updates/s", "algorithmic_bytes_per_cell": EXPLICIT_BYTES,
             "achieved": gbs, "peak": HBM_PEAK_GBS, "frac": gbs / HBM_PEAK_GBS,
             "kernels_ms": {k: v[0] / v[1] for k, v in tim.items()},
-            "terms": "-dP + 3/2 bTheta - 1/2 bTheta_o + farm + FormU(formMode 0), synthetic sources"}
+            "terms": "-dP + 3/2 bTheta - 1/2 bTheta_o + farm + FormU(formMode 0), synthetic sources",
+            "ueqn_rk4_ms": rk_ms}
 
 
 # FormT, per interior cell: theta 8, ucont 24, nine face vectors 216, three face Jacobians 24,

@@ -246,6 +246,37 @@ int tosca_explicit_rhs_u(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_par
                          const double *btheta, const double *btheta_o, int ab2, const double *farm,
                          const double *ucont_o, double *rhs, int form_mode);
 
+/* Replaces UeqnFE (src/solvers/ueqn_solvers.c:1411-1441), the forward-Euler momentum predictor:
+ * rhs = ExplicitRhsU(form_mode) as tosca_explicit_rhs_u; rhs = dt rhs; rhs += ucont_o;
+ * ucont (owned range) = rhs, then the periodic ghost wrap of DMGlobalToLocal.  On return rhs
+ * holds the new Ucont, as the reference's Rhs does.  Arguments as tosca_explicit_rhs_u. */
+int tosca_ueqn_fe(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_params *params,
+                  double *ucont, double *ucat, const double *nut, const double *dp,
+                  const double *btheta, const double *btheta_o, int ab2, const double *farm,
+                  const double *ucont_o, double dt, double *rhs, int form_mode);
+
+/* Replaces UeqnRK4 (src/solvers/ueqn_solvers.c:1446-1504), the classical four-stage Runge-Kutta
+ * predictor with b = (1/6, 1/3, 1/3, 1/6) and a = (0, 1/2, 1/2, 1): utmp = ucont_o; per stage i,
+ * for i > 0 ucont = a_i dt rhs + ucont_o (VecWAXPY, owned range + ghost wrap), then
+ * rhs = ExplicitRhsU and utmp += b_i dt rhs; finally ucont = utmp.  The sources (dp, btheta,
+ * farm) and nut are held fixed over the stages, as in the reference.  utmp: global AoS work
+ * vector (Ueqn->Utmp), not aliasing rhs or ucont_o. */
+int tosca_ueqn_rk4(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_params *params,
+                   double *ucont, double *ucat, const double *nut, const double *dp,
+                   const double *btheta, const double *btheta_o, int ab2, const double *farm,
+                   const double *ucont_o, double dt, double *rhs, double *utmp, int form_mode);
+
+/* Replaces IMEXMatVec (src/solvers/ueqn_solvers.c:1739-1801), the matrix-free operator of the
+ * implicit (Crank-Nicolson / IMEX Runge-Kutta) momentum solve: av = v - coef Visc(v), where
+ * coef = dt scale (scale 1/2 for the CN steps of ABCN, 1 for RK3WCN / RK3SOCN, chosen by the
+ * caller) and Visc is FormU(formMode 2) of v after the ExplicitRhsU resets and
+ * contravariantToCartesian, followed by resetNonResolvedCellFaces with ucont_o.
+ * v, ucont_o, av: global AoS; ucont, ucat: local work arrays (lUcont, lUcat) left as the
+ * reference leaves them. */
+int tosca_imex_matvec(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_params *params,
+                      const double *v, double *ucont, double *ucat, const double *nut,
+                      const double *ucont_o, double coef, double *av);
+
 /* ---- temperature equation (the sibling of FormU) ----------------------------------------------- */
 
 /* Inputs of FormT beyond tosca_params (teqn_ / constants_ members). */

@@ -50,6 +50,9 @@ def load() -> C.CDLL:
         "oracle_cartesian_bcs": (I, [In, M, Pa, P, P]),
         "oracle_temperature_bcs": (I, [In, M, Pa, P, P, P]),
         "oracle_formt": (I, [In, M, Pa, C.POINTER(abi.TeqnParams), P, P, P, P, P, D, I]),
+        "oracle_ueqn_fe": (I, [In, M, Pa, P, P, P, P, P, P, I, P, P, D, P, I]),
+        "oracle_ueqn_rk4": (I, [In, M, Pa, P, P, P, P, P, P, I, P, P, D, P, P, I]),
+        "oracle_imex_matvec": (I, [In, M, Pa, P, P, P, P, P, D, P]),
     }
     for n, (r, a) in sigs.items():
         f = getattr(lib, n)
@@ -188,3 +191,29 @@ def formt(info, mesh, params, tparams, ucont, theta, nut, kt, rhs, scale=1.0, fo
     tp = abi.make_teqn_params({k: (_p(v) if k.startswith("q_wall") and v is not None else v) for k, v in tparams.items()})
     _check(lib.oracle_formt(C.byref(inf), C.byref(m), C.byref(p), C.byref(tp), _p(ucont), _p(theta), _p(nut), _p(kt),
                             _p(rhs), float(scale), int(form_mode)))
+
+
+def ueqn_fe(info, mesh, params, ucont, ucat, nut, dp, btheta, btheta_o, ab2, farm, ucont_o, dt, rhs, form_mode=0):
+    """UeqnFE restated: ucont (local) gets ucont_o + dt ExplicitRhsU, rhs holds the same values."""
+    lib = load()
+    inf, m, p = abi.make_info(info), _mesh(mesh), abi.make_params(params)
+    _check(lib.oracle_ueqn_fe(C.byref(inf), C.byref(m), C.byref(p), _p(ucont), _p(ucat), _p(nut), _p(dp), _p(btheta),
+                              _p(btheta_o), int(ab2), _p(farm), _p(ucont_o), float(dt), _p(rhs), int(form_mode)))
+
+
+def ueqn_rk4(info, mesh, params, ucont, ucat, nut, dp, btheta, btheta_o, ab2, farm, ucont_o, dt, rhs, utmp,
+             form_mode=0):
+    """UeqnRK4 restated: ucont (local) gets the four-stage update, utmp (global) is work."""
+    lib = load()
+    inf, m, p = abi.make_info(info), _mesh(mesh), abi.make_params(params)
+    _check(lib.oracle_ueqn_rk4(C.byref(inf), C.byref(m), C.byref(p), _p(ucont), _p(ucat), _p(nut), _p(dp), _p(btheta),
+                               _p(btheta_o), int(ab2), _p(farm), _p(ucont_o), float(dt), _p(rhs), _p(utmp),
+                               int(form_mode)))
+
+
+def imex_matvec(info, mesh, params, v, ucont, ucat, nut, ucont_o, coef, av):
+    """IMEXMatVec restated: av = v - coef Visc(v) (global); ucont and ucat (local) are work."""
+    lib = load()
+    inf, m, p = abi.make_info(info), _mesh(mesh), abi.make_params(params)
+    _check(lib.oracle_imex_matvec(C.byref(inf), C.byref(m), C.byref(p), _p(v), _p(ucont), _p(ucat), _p(nut),
+                                  _p(ucont_o), float(coef), _p(av)))

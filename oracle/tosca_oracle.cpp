@@ -1188,6 +1188,103 @@ int oracle_explicit_rhs_u(const tosca_dmda_info *info, const tosca_mesh *m, cons
     reset_nonresolved(l, p, rhs, (const V3 *)ucont_o);       // :1403
     return TOSCA_OK;
 }
+
+// ================================================================================================
+// Explicit momentum predictors and the IMEX operator around ExplicitRhsU
+// (src/solvers/ueqn_solvers.c:1411-1504, 1739-1801)
+// ================================================================================================
+
+// DMGlobalToLocal of a global vector on one rank: the owned range copied, then the type-2 wrap
+static void global_vec_to_local(const Lay &l, const V3 *g, V3 *loc) {
+    for (int64_t k = 0; k < l.mz; k++)
+        for (int64_t j = 0; j < l.my; j++)
+            for (int64_t i = 0; i < l.mx; i++) loc[l.L(k, j, i)] = g[l.G(k, j, i)];
+    local_to_local(l, loc);
+}
+
+// UeqnFE (:1411-1441): Rhs = ExplicitRhsU; VecScale(Rhs, dt); VecAXPY(Rhs, 1, Ucont_o);
+// Ucont = Rhs; GlobalToLocal.  rhs holds the new Ucont on return, as the reference's Rhs does.
+int oracle_ueqn_fe(const tosca_dmda_info *info, const tosca_mesh *m, const tosca_params *p, double *ucont_,
+                   double *ucat_, const double *nut, const double *dp, const double *btheta, const double *btheta_o,
+                   int ab2, const double *farm, const double *ucont_o, double dt, double *rhs_, int formMode) {
+    int rc = oracle_explicit_rhs_u(info, m, p, ucont_, ucat_, nut, dp, btheta, btheta_o, ab2, farm, ucont_o, rhs_,
+                                   formMode);
+    if (rc) return rc;
+    Lay l;
+    check_layout(info, p, l);
+    V3 *rhs = (V3 *)rhs_;
+    const int64_t ng = l.mx * l.my * l.mz;
+    for (int64_t g = 0; g < ng; g++) {
+        rhs[g].x = dt * rhs[g].x;
+        rhs[g].y = dt * rhs[g].y;
+        rhs[g].z = dt * rhs[g].z;
+    }
+    axpy(ng, 1.0, (const V3 *)ucont_o, rhs);
+    global_vec_to_local(l, rhs, (V3 *)ucont_);
+    return TOSCA_OK;
+}
+
+// UeqnRK4 (:1446-1504): classical four-stage Runge-Kutta on ExplicitRhsU with the sources held
+// fixed; stage i > 0 starts from Ucont = a_i dt Rhs + Ucont_o (VecWAXPY), and
+// Utmp = Ucont_o + sum_i b_i dt K_i is accumulated with VecAXPY.  utmp is a global work vector.
+int oracle_ueqn_rk4(const tosca_dmda_info *info, const tosca_mesh *m, const tosca_params *p, double *ucont_,
+                    double *ucat_, const double *nut, const double *dp, const double *btheta, const double *btheta_o,
+                    int ab2, const double *farm, const double *ucont_o_, double dt, double *rhs_, double *utmp_,
+                    int formMode) {
+    Lay l;
+    int rc = check_layout(info, p, l);
+    if (rc) return rc;
+    if (!utmp_) { g_err = "ueqn_rk4: missing utmp"; return TOSCA_ERR_ARG; }
+    const double b[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+    const double a[4] = {0.0, 0.5, 0.5, 1.0};
+    const int64_t ng = l.mx * l.my * l.mz;
+    V3 *rhs = (V3 *)rhs_, *utmp = (V3 *)utmp_;
+    const V3 *uo = (const V3 *)ucont_o_;
+    for (int64_t g = 0; g < ng; g++) utmp[g] = uo[g];
+    std::vector<V3> w(ng);
+    for (int i = 0; i < 4; i++) {
+        if (i != 0) {
+            const double al = a[i] * dt;
+            for (int64_t g = 0; g < ng; g++) {
+                w[g].x = al * rhs[g].x + uo[g].x;
+                w[g].y = al * rhs[g].y + uo[g].y;
+                w[g].z = al * rhs[g].z + uo[g].z;
+            }
+            global_vec_to_local(l, w.data(), (V3 *)ucont_);
+        }
+        rc = oracle_explicit_rhs_u(info, m, p, ucont_, ucat_, nut, dp, btheta, btheta_o, ab2, farm, ucont_o_, rhs_,
+                                   formMode);
+        if (rc) return rc;
+        axpy(ng, dt * b[i], rhs, utmp);
+    }
+    global_vec_to_local(l, utmp, (V3 *)ucont_);
+    return TOSCA_OK;
+}
+
+// IMEXMatVec (:1739-1801): Ucont = v and its ghosts; the ExplicitRhsU resets and
+// contravariantToCartesian; Av = 0; FormU(Av, -coef, 2) with coef = dt scale;
+// resetNonResolvedCellFaces(Av); VecAXPY(Av, 1, v)
+int oracle_imex_matvec(const tosca_dmda_info *info, const tosca_mesh *m, const tosca_params *p, const double *v_,
+                       double *ucont_, double *ucat_, const double *nut, const double *ucont_o, double coef,
+                       double *av_) {
+    Lay l;
+    int rc = check_layout(info, p, l);
+    if (rc) return rc;
+    if (!v_ || !ucont_ || !ucat_ || !ucont_o || !av_) { g_err = "imex_matvec: missing array"; return TOSCA_ERR_ARG; }
+    const int64_t ng = l.mx * l.my * l.mz;
+    V3 *ucont = (V3 *)ucont_, *ucat = (V3 *)ucat_, *av = (V3 *)av_;
+    global_vec_to_local(l, (const V3 *)v_, ucont);            // :1763-1767
+    reset_no_penetration(l, p, ucont);                        // :1770
+    reset_face_periodic(l, ucont);                            // :1773
+    contravariant_to_cartesian(l, m, ucont, ucat);            // :1777
+    reset_cell_periodic(l, ucat);                             // :1790
+    for (int64_t g = 0; g < ng; g++) av[g] = V3{0, 0, 0};    // :1793
+    rc = oracle_formu(info, m, p, ucont_, ucat_, nut, av_, -1.0 * coef, 2);   // :1794
+    if (rc) return rc;
+    reset_nonresolved(l, p, av, (const V3 *)ucont_o);         // :1795
+    axpy(ng, 1.0, (const V3 *)v_, av);                        // :1796
+    return TOSCA_OK;
+}
 #endif  // ORACLE_TANGENT
 
 #ifndef ORACLE_TANGENT

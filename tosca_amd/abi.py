@@ -105,6 +105,9 @@ SIGNATURES = {
     "tosca_update_cartesian_bcs": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P]),
     "tosca_update_temperature_bcs": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P]),
     "tosca_formt": (I, [P, C.POINTER(Mesh), C.POINTER(Params), C.POINTER(TeqnParams), P, P, P, P, P, D, I]),
+    "tosca_ueqn_fe": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P, P, P, P, I, P, P, D, P, I]),
+    "tosca_ueqn_rk4": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P, P, P, P, I, P, P, D, P, P, I]),
+    "tosca_imex_matvec": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P, P, P, D, P]),
 }
 
 _lib: Optional[C.CDLL] = None

@@ -271,6 +271,71 @@ int check_mixed(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const 
     return TOSCA_OK;
 }
 
+int check_wall_tau(tosca_ctx *c, const tosca_params *p) {
+    const Geo &g = c->geo;
+    int rc;
+    if (p->bc_u[0] == TOSCA_BCU_WALLFUNCTION || p->bc_u[1] == TOSCA_BCU_WALLFUNCTION)
+        if ((rc = check_dev(c, p->tau_wall_i, 3 * (size_t)g.mz * g.my * sizeof(double), "tau_wall_i"))) return rc;
+    if (p->bc_u[2] == TOSCA_BCU_WALLFUNCTION || p->bc_u[3] == TOSCA_BCU_WALLFUNCTION)
+        if ((rc = check_dev(c, p->tau_wall_j, 3 * (size_t)g.mz * g.mx * sizeof(double), "tau_wall_j"))) return rc;
+    return TOSCA_OK;
+}
+
+// ExplicitRhsU's arguments (ueqn_solvers.c:1276-1407)
+int check_explicit(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, double *ucont, double *ucat,
+                   const double *nut, const double *dp, const double *btheta, const double *btheta_o, int ab2,
+                   const double *farm, const double *ucont_o, double *rhs, int form_mode) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if ((rc = check_params(c, p))) return rc;
+    if (form_mode < 0 || form_mode > 2) return set_err(c, TOSCA_ERR_ARG, "form_mode = %d (0, 1 or 2)", form_mode);
+    if (p->div_scheme != TOSCA_DIV_CENTRAL && p->div_scheme != TOSCA_DIV_QUICK)
+        return set_err(c, TOSCA_ERR_UNSUPPORTED, "div_scheme = %d: only central (0) and QUICK (1) are supported", p->div_scheme);
+    set_geo_per(c, p);
+    const Geo &g = c->geo;
+    const size_t n = (size_t)g.nloc(), vloc = 3 * n * sizeof(double), vglob = 3 * (size_t)g.mx * g.my * g.mz * sizeof(double);
+    if ((rc = check_mesh(c, m, true, true, true))) return rc;
+    if ((rc = check_dev(c, ucont, vloc, "ucont")) || (rc = check_dev(c, ucat, vloc, "ucat")) ||
+        (rc = check_dev(c, dp, vglob, "dp")) || (rc = check_dev(c, ucont_o, vglob, "ucont_o")) ||
+        (rc = check_dev(c, rhs, vglob, "rhs")))
+        return rc;
+    if (btheta) {
+        if ((rc = check_dev(c, btheta, vglob, "btheta"))) return rc;
+        if (ab2 && (rc = check_dev(c, btheta_o, vglob, "btheta_o"))) return rc;
+    }
+    if (farm && (rc = check_dev(c, farm, vglob, "farm"))) return rc;
+    if (rhs == ucont_o || rhs == dp) return set_err(c, TOSCA_ERR_ARG, "rhs must not alias ucont_o or dp");
+    const bool visc = form_mode != 1;
+    if (p->les_active && visc) {
+        if ((rc = check_les_model(c, p))) return rc;
+        if ((rc = check_dev(c, nut, n * sizeof(double), "nut"))) return rc;
+    }
+    if (visc && (rc = check_wall_tau(c, p))) return rc;
+    return ensure_ws(c, ws_need(c));
+}
+
+// ExplicitRhsU's kernels in the reference's order, on validated arguments
+void queue_explicit(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, double *ucont, double *ucat,
+                    const double *nut, const double *dp, const double *btheta, const double *btheta_o, int ab2,
+                    const double *farm, const double *ucont_o, double *rhs, int form_mode) {
+    const Geo &g = c->geo;
+    hipStream_t s = c->stream;
+    const bool visc = form_mode != 1;
+    tosca::launch_no_penetration(ucont, p, g, s);                    // ueqn_solvers.c:1281
+    tosca::launch_reset_face_periodic(ucont, 3, g, s);               // :1284
+    tosca::launch_contra_to_cart(ucont, m, ucat, g, s);              // :1287
+    tosca::launch_reset_cell_periodic(ucat, 3, g, s);                // :1301
+    tosca::launch_rhs_init(rhs, dp, btheta, ab2 ? btheta_o : nullptr, btheta ? ab2 : 0, farm, g, s);   // :1304-1344
+    const double *nu_t = (p->les_active && visc) ? nut : nullptr;   // :1347
+    if (c->ref_path) {
+        tosca::ref_formu(g, m, p, ucont, ucat, nu_t, rhs, 1.0, form_mode, c->ws, s);
+    } else {
+        tosca::FormUArgs A = formu_args(m, p, ucont, ucat, nu_t, rhs, 1.0, form_mode);
+        tosca::fused_formu(g, A, c->ws + 3 * (size_t)g.nloc(), s);
+    }
+    tosca::launch_non_resolved(rhs, ucont_o, p, g, s);               // :1403
+}
+
 }  // namespace
 
 extern "C" {
@@ -437,52 +502,91 @@ int tosca_formu(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const 
 int tosca_explicit_rhs_u(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, double *ucont, double *ucat,
                          const double *nut, const double *dp, const double *btheta, const double *btheta_o, int ab2,
                          const double *farm, const double *ucont_o, double *rhs, int form_mode) {
+    int rc = check_explicit(c, m, p, ucont, ucat, nut, dp, btheta, btheta_o, ab2, farm, ucont_o, rhs, form_mode);
+    if (rc) return rc;
+    queue_explicit(c, m, p, ucont, ucat, nut, dp, btheta, btheta_o, ab2, farm, ucont_o, rhs, form_mode);
+    return launched(c, "ExplicitRhsU");
+}
+
+int tosca_ueqn_fe(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, double *ucont, double *ucat,
+                  const double *nut, const double *dp, const double *btheta, const double *btheta_o, int ab2,
+                  const double *farm, const double *ucont_o, double dt, double *rhs, int form_mode) {
+    int rc = check_explicit(c, m, p, ucont, ucat, nut, dp, btheta, btheta_o, ab2, farm, ucont_o, rhs, form_mode);
+    if (rc) return rc;
+    queue_explicit(c, m, p, ucont, ucat, nut, dp, btheta, btheta_o, ab2, farm, ucont_o, rhs, form_mode);  // :1424
+    tosca::launch_fe_update(rhs, ucont_o, dt, ucont, c->geo, c->stream);    // :1427-1433
+    tosca::launch_wrap(ucont, 3, c->geo, c->stream);                        // :1436-1437
+    return launched(c, "UeqnFE");
+}
+
+int tosca_ueqn_rk4(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, double *ucont, double *ucat,
+                   const double *nut, const double *dp, const double *btheta, const double *btheta_o, int ab2,
+                   const double *farm, const double *ucont_o, double dt, double *rhs, double *utmp, int form_mode) {
+    int rc = check_explicit(c, m, p, ucont, ucat, nut, dp, btheta, btheta_o, ab2, farm, ucont_o, rhs, form_mode);
+    if (rc) return rc;
+    const Geo &g = c->geo;
+    const size_t vglob = 3 * (size_t)g.mx * g.my * g.mz * sizeof(double);
+    if ((rc = check_dev(c, utmp, vglob, "utmp"))) return rc;
+    if (utmp == rhs || utmp == ucont_o) return set_err(c, TOSCA_ERR_ARG, "utmp must not alias rhs or ucont_o");
+    const double b[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};   // :1458-1466
+    const double a[4] = {0.0, 0.5, 0.5, 1.0};
+    hipStream_t s = c->stream;
+    if (hipMemcpyAsync(utmp, ucont_o, vglob, hipMemcpyDeviceToDevice, s) != hipSuccess)   // :1471
+        return set_err(c, TOSCA_ERR_DEVICE, "UeqnRK4: copy of ucont_o failed");
+    for (int i = 0; i < 4; i++) {
+        if (i != 0) {                                                        // :1480-1486
+            tosca::launch_waxpy_local(ucont, a[i] * dt, rhs, ucont_o, g, s);
+            tosca::launch_wrap(ucont, 3, g, s);
+        }
+        queue_explicit(c, m, p, ucont, ucat, nut, dp, btheta, btheta_o, ab2, farm, ucont_o, rhs, form_mode);   // :1489
+        tosca::launch_axpy(utmp, dt * b[i], rhs, 3L * g.mx * g.my * g.mz, s);   // :1492
+    }
+    tosca::launch_global_to_local(ucont, utmp, g, s);                      // :1495-1497
+    tosca::launch_wrap(ucont, 3, g, s);
+    return launched(c, "UeqnRK4");
+}
+
+int tosca_imex_matvec(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const double *v, double *ucont,
+                      double *ucat, const double *nut, const double *ucont_o, double coef, double *av) {
     int rc = check_ctx(c);
     if (rc) return rc;
     if ((rc = check_params(c, p))) return rc;
-    if (form_mode < 0 || form_mode > 2) return set_err(c, TOSCA_ERR_ARG, "form_mode = %d (0, 1 or 2)", form_mode);
     if (p->div_scheme != TOSCA_DIV_CENTRAL && p->div_scheme != TOSCA_DIV_QUICK)
         return set_err(c, TOSCA_ERR_UNSUPPORTED, "div_scheme = %d: only central (0) and QUICK (1) are supported", p->div_scheme);
     set_geo_per(c, p);
     const Geo &g = c->geo;
     const size_t n = (size_t)g.nloc(), vloc = 3 * n * sizeof(double), vglob = 3 * (size_t)g.mx * g.my * g.mz * sizeof(double);
     if ((rc = check_mesh(c, m, true, true, true))) return rc;
-    if ((rc = check_dev(c, ucont, vloc, "ucont")) || (rc = check_dev(c, ucat, vloc, "ucat")) ||
-        (rc = check_dev(c, dp, vglob, "dp")) || (rc = check_dev(c, ucont_o, vglob, "ucont_o")) ||
-        (rc = check_dev(c, rhs, vglob, "rhs")))
+    if ((rc = check_dev(c, v, vglob, "v")) || (rc = check_dev(c, ucont, vloc, "ucont")) ||
+        (rc = check_dev(c, ucat, vloc, "ucat")) || (rc = check_dev(c, ucont_o, vglob, "ucont_o")) ||
+        (rc = check_dev(c, av, vglob, "av")))
         return rc;
-    if (btheta) {
-        if ((rc = check_dev(c, btheta, vglob, "btheta"))) return rc;
-        if (ab2 && (rc = check_dev(c, btheta_o, vglob, "btheta_o"))) return rc;
-    }
-    if (farm && (rc = check_dev(c, farm, vglob, "farm"))) return rc;
-    const bool visc = form_mode != 1;
-    if (p->les_active && visc) {
+    if (av == v) return set_err(c, TOSCA_ERR_ARG, "av must not alias v");
+    if (p->les_active) {
         if ((rc = check_les_model(c, p))) return rc;
         if ((rc = check_dev(c, nut, n * sizeof(double), "nut"))) return rc;
     }
-    if (visc) {
-        if (p->bc_u[0] == TOSCA_BCU_WALLFUNCTION || p->bc_u[1] == TOSCA_BCU_WALLFUNCTION)
-            if ((rc = check_dev(c, p->tau_wall_i, 3 * (size_t)g.mz * g.my * sizeof(double), "tau_wall_i"))) return rc;
-        if (p->bc_u[2] == TOSCA_BCU_WALLFUNCTION || p->bc_u[3] == TOSCA_BCU_WALLFUNCTION)
-            if ((rc = check_dev(c, p->tau_wall_j, 3 * (size_t)g.mz * g.mx * sizeof(double), "tau_wall_j"))) return rc;
-    }
+    if ((rc = check_wall_tau(c, p))) return rc;
     if ((rc = ensure_ws(c, ws_need(c)))) return rc;
     hipStream_t s = c->stream;
-    tosca::launch_no_penetration(ucont, p, g, s);                    // ueqn_solvers.c:1281
-    tosca::launch_reset_face_periodic(ucont, 3, g, s);               // :1284
-    tosca::launch_contra_to_cart(ucont, m, ucat, g, s);              // :1287
-    tosca::launch_reset_cell_periodic(ucat, 3, g, s);                // :1301
-    tosca::launch_rhs_init(rhs, dp, btheta, ab2 ? btheta_o : nullptr, btheta ? ab2 : 0, farm, g, s);   // :1304-1344
-    const double *nu_t = (p->les_active && visc) ? nut : nullptr;   // :1347
-    if (c->ref_path) {
-        tosca::ref_formu(g, m, p, ucont, ucat, nu_t, rhs, 1.0, form_mode, c->ws, s);
+    const long ng3 = 3L * g.mx * g.my * g.mz;
+    tosca::launch_global_to_local(ucont, v, g, s);                         // :1763-1767
+    tosca::launch_wrap(ucont, 3, g, s);
+    tosca::launch_no_penetration(ucont, p, g, s);                          // :1770
+    tosca::launch_reset_face_periodic(ucont, 3, g, s);                     // :1773
+    tosca::launch_contra_to_cart(ucont, m, ucat, g, s);                    // :1777
+    tosca::launch_reset_cell_periodic(ucat, 3, g, s);                      // :1790
+    tosca::launch_fill(av, ng3, 0.0, s);                                   // :1793
+    const double *nu_t = p->les_active ? nut : nullptr;
+    if (c->ref_path) {                                                     // :1794
+        tosca::ref_formu(g, m, p, ucont, ucat, nu_t, av, -1.0 * coef, 2, c->ws, s);
     } else {
-        tosca::FormUArgs A = formu_args(m, p, ucont, ucat, nu_t, rhs, 1.0, form_mode);
+        tosca::FormUArgs A = formu_args(m, p, ucont, ucat, nu_t, av, -1.0 * coef, 2);
         tosca::fused_formu(g, A, c->ws + 3 * (size_t)g.nloc(), s);
     }
-    tosca::launch_non_resolved(rhs, ucont_o, p, g, s);               // :1403
-    return launched(c, "ExplicitRhsU");
+    tosca::launch_non_resolved(av, ucont_o, p, g, s);                      // :1795
+    tosca::launch_axpy(av, 1.0, v, ng3, s);                                // :1796
+    return launched(c, "IMEXMatVec");
 }
 
 int tosca_formt(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const tosca_teqn_params *tp,

@@ -6,6 +6,8 @@
 //   contravariantToCartesian   src/ueqn.c:528-635               -> k_contra_to_cart
 //   Rhs = -dP + bTheta terms + farm (VecSet / VecAXPY order)     -> k_rhs_init
 //   resetNonResolvedCellFaces  src/include/inline.h:1674-1838   -> k_non_resolved
+// and the vector updates of the explicit predictors UeqnFE / UeqnRK4 and of IMEXMatVec
+// (ueqn_solvers.c:1411-1504, 1739-1801): k_fe_update, k_waxpy_local, k_axpy, k_global_to_local.
 // The periodic resets of ucont and ucat are the reference-structured ghost kernels
 // (launch_reset_face_periodic, launch_reset_cell_periodic).
 #include "kernels.h"
@@ -125,6 +127,46 @@ PatchKinds kinds(const tosca_params *p) {
 
 int maxdim(const Geo &g) { return std::max(g.mx, std::max(g.my, g.mz)); }
 
+// local index of owned point q (global order [k][j][i])
+__device__ __forceinline__ long owned_local(const Geo &g, long q) {
+    const int i = (int)(q % g.mx);
+    const long r = q / g.mx;
+    const int j = (int)(r % g.my), k = (int)(r / g.my);
+    return g.L(k, j, i);
+}
+
+// UeqnFE (ueqn_solvers.c:1427-1433): VecScale(Rhs, dt); VecAXPY(Rhs, 1, Ucont_o); VecCopy(Rhs, Ucont),
+// the copy going into the owned range of the local array; one thread per double
+__global__ void k_fe_update(double *rhs, const double *uo, double dt, double *ucont, Geo g, long n) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    double r = dt * rhs[t];
+    r = r + 1.0 * uo[t];
+    rhs[t] = r;
+    ucont[3 * owned_local(g, t / 3) + t % 3] = r;
+}
+
+// VecWAXPY(Ucont, al, Rhs, Ucont_o) into the owned range of the local array (ueqn_solvers.c:1484)
+__global__ void k_waxpy_local(double *ucont, double al, const double *rhs, const double *uo, Geo g, long n) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    ucont[3 * owned_local(g, t / 3) + t % 3] = al * rhs[t] + uo[t];
+}
+
+// VecAXPY(y, a, x): y = y + a x
+__global__ void k_axpy(double *y, double a, const double *x, long n) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    y[t] = y[t] + a * x[t];
+}
+
+// the owned-range copy of DMGlobalToLocal for a vector field
+__global__ void k_global_to_local(double *loc, const double *glob, Geo g, long n) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    loc[3 * owned_local(g, t / 3) + t % 3] = glob[t];
+}
+
 }  // namespace
 
 void launch_no_penetration(double *ucont, const tosca_params *p, const Geo &g, hipStream_t s) {
@@ -141,6 +183,25 @@ void launch_rhs_init(double *rhs, const double *dp, const double *bt, const doub
                      const Geo &g, hipStream_t s) {
     const long n = 3L * g.mx * g.my * g.mz;
     TOSCA_LAUNCH("k_rhs_init", k_rhs_init, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, rhs, dp, bt, bto, ab2, farm, n);
+}
+
+void launch_fe_update(double *rhs, const double *ucont_o, double dt, double *ucont, const Geo &g, hipStream_t s) {
+    const long n = 3L * g.mx * g.my * g.mz;
+    TOSCA_LAUNCH("k_fe_update", k_fe_update, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, rhs, ucont_o, dt, ucont, g, n);
+}
+
+void launch_waxpy_local(double *ucont, double al, const double *rhs, const double *ucont_o, const Geo &g, hipStream_t s) {
+    const long n = 3L * g.mx * g.my * g.mz;
+    TOSCA_LAUNCH("k_waxpy_local", k_waxpy_local, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ucont, al, rhs, ucont_o, g, n);
+}
+
+void launch_axpy(double *y, double a, const double *x, long n, hipStream_t s) {
+    TOSCA_LAUNCH("k_axpy", k_axpy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, y, a, x, n);
+}
+
+void launch_global_to_local(double *loc, const double *glob, const Geo &g, hipStream_t s) {
+    const long n = 3L * g.mx * g.my * g.mz;
+    TOSCA_LAUNCH("k_global_to_local", k_global_to_local, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, loc, glob, g, n);
 }
 
 void launch_non_resolved(double *rhs, const double *ucont_o, const tosca_params *p, const Geo &g, hipStream_t s) {

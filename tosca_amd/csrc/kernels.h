@@ -29,6 +29,10 @@ void launch_contra_to_cart(const double *ucont, const tosca_mesh *m, double *uca
 void launch_rhs_init(double *rhs, const double *dp, const double *bt, const double *bto, int ab2, const double *farm,
                      const Geo &g, hipStream_t s);
 void launch_non_resolved(double *rhs, const double *ucont_o, const tosca_params *p, const Geo &g, hipStream_t s);
+void launch_fe_update(double *rhs, const double *ucont_o, double dt, double *ucont, const Geo &g, hipStream_t s);
+void launch_waxpy_local(double *ucont, double al, const double *rhs, const double *ucont_o, const Geo &g, hipStream_t s);
+void launch_axpy(double *y, double a, const double *x, long n, hipStream_t s);
+void launch_global_to_local(double *loc, const double *glob, const Geo &g, hipStream_t s);
 
 // boundary-condition ghost fills (bc_fill.hip)
 void launch_contravariant_bcs(double *ucont, const double *ucat, const tosca_mesh *m, const tosca_params *p,

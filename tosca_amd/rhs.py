@@ -170,6 +170,28 @@ class RhsContext:
                                                           C.cast(vals, C.c_void_p), _ptr(theta)))
 
     # ---- instrumentation -----------------------------------------------------------------------
+    def UeqnFE(self, mesh, params, ucont, ucat, nut, dp, btheta, btheta_o, ab2, farm, ucont_o, dt, rhs,
+               form_mode: int = 0):
+        """UeqnFE (src/solvers/ueqn_solvers.c:1411-1441): ucont <- ucont_o + dt ExplicitRhsU."""
+        m, p = mesh_struct(mesh), abi.make_params(params)
+        self._check(self.lib.tosca_ueqn_fe(self.h, C.byref(m), C.byref(p), _ptr(ucont), _ptr(ucat), _ptr(nut),
+                                           _ptr(dp), _ptr(btheta), _ptr(btheta_o), int(ab2), _ptr(farm),
+                                           _ptr(ucont_o), float(dt), _ptr(rhs), int(form_mode)))
+
+    def UeqnRK4(self, mesh, params, ucont, ucat, nut, dp, btheta, btheta_o, ab2, farm, ucont_o, dt, rhs, utmp,
+                form_mode: int = 0):
+        """UeqnRK4 (src/solvers/ueqn_solvers.c:1446-1504): four ExplicitRhsU stages."""
+        m, p = mesh_struct(mesh), abi.make_params(params)
+        self._check(self.lib.tosca_ueqn_rk4(self.h, C.byref(m), C.byref(p), _ptr(ucont), _ptr(ucat), _ptr(nut),
+                                            _ptr(dp), _ptr(btheta), _ptr(btheta_o), int(ab2), _ptr(farm),
+                                            _ptr(ucont_o), float(dt), _ptr(rhs), _ptr(utmp), int(form_mode)))
+
+    def IMEXMatVec(self, mesh, params, v, ucont, ucat, nut, ucont_o, coef, av):
+        """IMEXMatVec (src/solvers/ueqn_solvers.c:1739-1801): av = v - coef Visc(v)."""
+        m, p = mesh_struct(mesh), abi.make_params(params)
+        self._check(self.lib.tosca_imex_matvec(self.h, C.byref(m), C.byref(p), _ptr(v), _ptr(ucont), _ptr(ucat),
+                                               _ptr(nut), _ptr(ucont_o), float(coef), _ptr(av)))
+
     def FormT(self, mesh, params, tparams, ucont, theta, nut, kt, rhs, scale: float = 1.0, form_mode: int = 0):
         """FormT (src/solvers/teqn_solvers.c:6-592): rhs (global scalar) at interior cells.
         tparams: dict(pr=..., wall_t=(iLeft, iRight, jLeft, jRight) thetaWallFunction flags,
