@@ -269,6 +269,7 @@ def explicit_leg(torch, ctx, mesh, case, ucont, ucat, nut, lay, args):
 # FormT, per interior cell: theta 8, ucont 24, nine face vectors 216, three face Jacobians 24,
 # aj 8, the closure's diffusivity (k_t or nu_t) 8, rhs written 8
 FORMT_BYTES = 8 + 24 + 216 + 24 + 8 + 8 + 8   # = 296
+GRADP_BYTES = 8 + 216 + 24 + 24                # = 272
 
 
 def formt_leg(torch, ctx, mesh, case, ucont, theta, nut, lay, args):
@@ -309,7 +310,23 @@ def formt_leg(torch, ctx, mesh, case, ucont, theta, nut, lay, args):
         out[name] = {"ms": ms, "value": value, "unit": "cell-updates/s", "achieved": gbs, "peak": HBM_PEAK_GBS,
                      "frac": gbs / HBM_PEAK_GBS, "kernels_ms": {k: v[0] / v[1] for k, v in tim.items()}}
     out["algorithmic_bytes_per_cell"] = FORMT_BYTES
-    del kt, rhs
+    # GradP on the same mesh, theta standing in for the pressure: p 8, nine face vectors 216, three
+    # face Jacobians 24, dP written 24 B/cell
+    dp = torch.empty((lay.mz, lay.my, lay.mx, 3), dtype=torch.float64, device=dev)
+    for _ in range(args.warmup):
+        ctx.GradP(mesh, case.params, theta, dp)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(args.steps):
+        ctx.GradP(mesh, case.params, theta, dp)
+    e1.record()
+    torch.cuda.synchronize()
+    gp_ms = e0.elapsed_time(e1) / args.steps
+    gp_gbs = GRADP_BYTES * lay.ncells / (gp_ms * 1e-3) / 1e9
+    print(f"# grad_p: {gp_ms:.3f} ms, {gp_gbs:.0f} GB/s algorithmic", file=sys.stderr, flush=True)
+    out["grad_p"] = {"ms": gp_ms, "algorithmic_bytes_per_cell": GRADP_BYTES, "achieved": gp_gbs,
+                     "frac": gp_gbs / HBM_PEAK_GBS}
+    del kt, rhs, dp
     return out
 
 

@@ -246,6 +246,14 @@ int tosca_explicit_rhs_u(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_par
                          const double *btheta, const double *btheta_o, int ab2, const double *farm,
                          const double *ucont_o, double *rhs, int form_mode);
 
+/* Replaces GradP (src/peqn.c:3437-3773): the contravariant pressure-gradient fluxes
+ * dP^q = (g^q1 dp/dxi + g^q2 dp/deta + g^q3 dp/dzeta) aj_q at the faces 1..m-2 of the interior
+ * cells, with the reference's one-sided transverse differences on the corner face rows and its
+ * type-2 periodic neighbour (ghost m+1) on face m-2; the right boundary face of a non-periodic
+ * axis and every other point of dp are 0.  p: local pressure lP (ghosts as the caller's
+ * boundary conditions left them); dp: global AoS (ueqn->dP), overwritten. */
+int tosca_grad_p(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_params *params, const double *p, double *dp);
+
 /* Replaces UeqnFE (src/solvers/ueqn_solvers.c:1411-1441), the forward-Euler momentum predictor:
  * rhs = ExplicitRhsU(form_mode) as tosca_explicit_rhs_u; rhs = dt rhs; rhs += ucont_o;
  * ucont (owned range) = rhs, then the periodic ghost wrap of DMGlobalToLocal.  On return rhs

@@ -53,6 +53,7 @@ def load() -> C.CDLL:
         "oracle_ueqn_fe": (I, [In, M, Pa, P, P, P, P, P, P, I, P, P, D, P, I]),
         "oracle_ueqn_rk4": (I, [In, M, Pa, P, P, P, P, P, P, I, P, P, D, P, P, I]),
         "oracle_imex_matvec": (I, [In, M, Pa, P, P, P, P, P, D, P]),
+        "oracle_grad_p": (I, [In, M, Pa, P, P]),
     }
     for n, (r, a) in sigs.items():
         f = getattr(lib, n)
@@ -217,3 +218,10 @@ def imex_matvec(info, mesh, params, v, ucont, ucat, nut, ucont_o, coef, av):
     inf, m, p = abi.make_info(info), _mesh(mesh), abi.make_params(params)
     _check(lib.oracle_imex_matvec(C.byref(inf), C.byref(m), C.byref(p), _p(v), _p(ucont), _p(ucat), _p(nut),
                                   _p(ucont_o), float(coef), _p(av)))
+
+
+def grad_p(info, mesh, params, p, dp):
+    """GradP restated: dp (global vector) from the local pressure p."""
+    lib = load()
+    inf, m, pa = abi.make_info(info), _mesh(mesh), abi.make_params(params)
+    _check(lib.oracle_grad_p(C.byref(inf), C.byref(m), C.byref(pa), _p(p), _p(dp)))

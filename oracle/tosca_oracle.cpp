@@ -1289,6 +1289,70 @@ int oracle_imex_matvec(const tosca_dmda_info *info, const tosca_mesh *m, const t
 
 #ifndef ORACLE_TANGENT
 // ================================================================================================
+// GradP (src/peqn.c:3437-3773), meshTag = 0: the contravariant pressure-gradient fluxes
+// ================================================================================================
+int oracle_grad_p(const tosca_dmda_info *info, const tosca_mesh *m, const tosca_params *p_, const double *P,
+                  double *dp_) {
+    Lay l;
+    int rc = check_layout(info, p_, l);
+    if (rc) return rc;
+    if (!P || !dp_) { g_err = "grad_p: missing array"; return TOSCA_ERR_ARG; }
+    const int64_t mx = l.mx, my = l.my, mz = l.mz;
+    const V3 *icsi = (const V3 *)m->icsi, *ieta = (const V3 *)m->ieta, *izet = (const V3 *)m->izet;
+    const V3 *jcsi = (const V3 *)m->jcsi, *jeta = (const V3 *)m->jeta, *jzet = (const V3 *)m->jzet;
+    const V3 *kcsi = (const V3 *)m->kcsi, *keta = (const V3 *)m->keta, *kzet = (const V3 *)m->kzet;
+    V3 *dp = (V3 *)dp_;
+    auto p = [&](int64_t k, int64_t j, int64_t i) { return P[l.L(k, j, i)]; };
+    auto dot = [](const V3 &a, const V3 &b) { return a.x * b.x + a.y * b.y + a.z * b.z; };
+    const bool iper = l.i_per() || l.ii_per(), jper = l.j_per() || l.jj_per(), kper = l.k_per() || l.kk_per();
+    for (int64_t g = 0; g < mx * my * mz; g++) dp[g] = V3{0, 0, 0};          // :3468
+    for (int64_t k = 1; k < mz - 1; k++)
+        for (int64_t j = 1; j < my - 1; j++)
+            for (int64_t i = 1; i < mx - 1; i++) {
+                const int64_t c = l.L(k, j, i);
+                // g_ab: face basis b dotted with the face's own normal area vector (:3492-3500)
+                const double g11_i = dot(icsi[c], icsi[c]), g12_i = dot(ieta[c], icsi[c]), g13_i = dot(izet[c], icsi[c]);
+                const double g21_j = dot(jcsi[c], jeta[c]), g22_j = dot(jeta[c], jeta[c]), g23_j = dot(jzet[c], jeta[c]);
+                const double g31_k = dot(kcsi[c], kzet[c]), g32_k = dot(keta[c], kzet[c]), g33_k = dot(kzet[c], kzet[c]);
+                double dpdc, dpde, dpdz;
+                // i face (:3502-3566); the one-sided corner rows sit at i == mx-2 whatever the periodicity
+                dpdc = (i == mx - 2 && l.ii_per()) ? p(k, j, mx + 1) - p(k, j, i) : p(k, j, i + 1) - p(k, j, i);
+                if (j == my - 2 && i == mx - 2) dpde = (p(k, j, i) - p(k, j - 1, i) + p(k, j, i + 1) - p(k, j - 1, i + 1)) * 0.5;
+                else if (j == 1 && i == mx - 2) dpde = (p(k, j + 1, i) - p(k, j, i) + p(k, j + 1, i + 1) - p(k, j, i + 1)) * 0.5;
+                else dpde = (p(k, j + 1, i) - p(k, j - 1, i) + p(k, j + 1, i + 1) - p(k, j - 1, i + 1)) * 0.25;
+                if (k == mz - 2 && i == mx - 2) dpdz = (p(k, j, i) - p(k - 1, j, i) + p(k, j, i + 1) - p(k - 1, j, i + 1)) * 0.5;
+                else if (k == 1 && i == mx - 2) dpdz = (p(k + 1, j, i) - p(k, j, i) + p(k + 1, j, i + 1) - p(k, j, i + 1)) * 0.5;
+                else dpdz = (p(k + 1, j, i) - p(k - 1, j, i) + p(k + 1, j, i + 1) - p(k - 1, j, i + 1)) * 0.25;
+                dp[l.G(k, j, i)].x = (dpdc * g11_i + dpde * g12_i + dpdz * g13_i) * m->iaj[c];
+                // j face (:3568-3651)
+                if (i == mx - 2 && j == my - 2) dpdc = (p(k, j, i) - p(k, j, i - 1) + p(k, j + 1, i) - p(k, j + 1, i - 1)) * 0.5;
+                else if (i == 1 && j == my - 2) dpdc = (p(k, j, i + 1) - p(k, j, i) + p(k, j + 1, i + 1) - p(k, j + 1, i)) * 0.5;
+                else dpdc = (p(k, j, i + 1) - p(k, j, i - 1) + p(k, j + 1, i + 1) - p(k, j + 1, i - 1)) * 0.25;
+                dpde = (j == my - 2 && l.jj_per()) ? p(k, my + 1, i) - p(k, j, i) : p(k, j + 1, i) - p(k, j, i);
+                if (k == mz - 2 && j == my - 2) dpdz = (p(k, j, i) - p(k - 1, j, i) + p(k, j + 1, i) - p(k - 1, j + 1, i)) * 0.5;
+                else if (k == 1 && j == my - 2) dpdz = (p(k + 1, j, i) - p(k, j, i) + p(k + 1, j + 1, i) - p(k, j + 1, i)) * 0.5;
+                else dpdz = (p(k + 1, j, i) - p(k - 1, j, i) + p(k + 1, j + 1, i) - p(k - 1, j + 1, i)) * 0.25;
+                dp[l.G(k, j, i)].y = (dpdc * g21_j + dpde * g22_j + dpdz * g23_j) * m->jaj[c];
+                // k face (:3653-3734)
+                if (i == mx - 2 && k == mz - 2) dpdc = (p(k, j, i) - p(k, j, i - 1) + p(k + 1, j, i) - p(k + 1, j, i - 1)) * 0.5;
+                else if (i == 1 && k == mz - 2) dpdc = (p(k, j, i + 1) - p(k, j, i) + p(k + 1, j, i + 1) - p(k + 1, j, i)) * 0.5;
+                else dpdc = (p(k, j, i + 1) - p(k, j, i - 1) + p(k + 1, j, i + 1) - p(k + 1, j, i - 1)) * 0.25;
+                if (j == my - 2 && k == mz - 2) dpde = (p(k, j, i) - p(k, j - 1, i) + p(k + 1, j, i) - p(k + 1, j - 1, i)) * 0.5;
+                else if (j == 1 && k == mz - 2) dpde = (p(k, j + 1, i) - p(k, j, i) + p(k + 1, j + 1, i) - p(k + 1, j, i)) * 0.5;
+                else dpde = (p(k, j + 1, i) - p(k, j - 1, i) + p(k + 1, j + 1, i) - p(k + 1, j - 1, i)) * 0.25;
+                dpdz = (k == mz - 2 && l.kk_per()) ? p(mz + 1, j, i) - p(k, j, i) : (p(k + 1, j, i) - p(k, j, i));
+                dp[l.G(k, j, i)].z = (dpdc * g31_k + dpde * g32_k + dpdz * g33_k) * m->kaj[c];
+                // faces whose contravariant velocity is not solved (:3736-3757)
+                if (!iper && i == mx - 2) dp[l.G(k, j, i)].x = 0;
+                if (!jper && j == my - 2) dp[l.G(k, j, i)].y = 0;
+                if (!kper && k == mz - 2) dp[l.G(k, j, i)].z = 0;
+            }
+    return TOSCA_OK;
+}
+#endif  // ORACLE_TANGENT
+
+#ifndef ORACLE_TANGENT
+// ================================================================================================
 // FormT (src/solvers/teqn_solvers.c:6-592), nvert = meshTag = 0
 // ================================================================================================
 

@@ -108,6 +108,7 @@ SIGNATURES = {
     "tosca_ueqn_fe": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P, P, P, P, I, P, P, D, P, I]),
     "tosca_ueqn_rk4": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P, P, P, P, I, P, P, D, P, P, I]),
     "tosca_imex_matvec": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P, P, P, D, P]),
+    "tosca_grad_p": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P]),
 }
 
 _lib: Optional[C.CDLL] = None

@@ -37,6 +37,7 @@ SOURCES = {
     "explicit_rhs.hip": ["-ffp-contract=off"],
     "bc_fill.hip": ["-ffp-contract=off"],
     "formt.hip": ["-ffp-contract=off"],
+    "gradp.hip": ["-ffp-contract=off"],
 }
 
 

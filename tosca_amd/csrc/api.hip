@@ -589,6 +589,21 @@ int tosca_imex_matvec(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, 
     return launched(c, "IMEXMatVec");
 }
 
+int tosca_grad_p(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const double *pres, double *dp) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if ((rc = check_params(c, p))) return rc;
+    set_geo_per(c, p);
+    const Geo &g = c->geo;
+    const size_t n = (size_t)g.nloc();
+    if ((rc = check_mesh(c, m, true, false, true))) return rc;
+    if ((rc = check_dev(c, pres, n * sizeof(double), "p")) ||
+        (rc = check_dev(c, dp, 3 * (size_t)g.mx * g.my * g.mz * sizeof(double), "dp")))
+        return rc;
+    tosca::launch_grad_p(pres, m, dp, g, c->stream);
+    return launched(c, "GradP");
+}
+
 int tosca_formt(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const tosca_teqn_params *tp,
                 const double *ucont, const double *theta, const double *nut, const double *kt, double *rhs,
                 double scale, int form_mode) {

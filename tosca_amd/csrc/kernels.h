@@ -42,6 +42,9 @@ void launch_cartesian_bcs(double *ucat, const double *u_fixed, const tosca_mesh 
 void launch_temperature_bcs(double *t, const int *bc_t, const double *t_value, const tosca_mesh *m, const Geo &g,
                             hipStream_t s);
 
+// GradP (gradp.hip): dp (global vector) from the local pressure p
+void launch_grad_p(const double *p, const tosca_mesh *m, double *dp, const Geo &g, hipStream_t s);
+
 // FormT (formt.hip): reference-structured face and cell kernels; ws holds 6 local doubles per point
 void ref_formt(const Geo &g, const tosca_mesh *m, const tosca_params *p, const tosca_teqn_params *tp,
                const double *ucont, const double *theta, const double *nut, const double *kt, double *rhs,

@@ -170,6 +170,11 @@ class RhsContext:
                                                           C.cast(vals, C.c_void_p), _ptr(theta)))
 
     # ---- instrumentation -----------------------------------------------------------------------
+    def GradP(self, mesh, params, p, dp):
+        """GradP (src/peqn.c:3437-3773): dp (global vector) from the local pressure p."""
+        m, pa = mesh_struct(mesh), abi.make_params(params)
+        self._check(self.lib.tosca_grad_p(self.h, C.byref(m), C.byref(pa), _ptr(p), _ptr(dp)))
+
     def UeqnFE(self, mesh, params, ucont, ucat, nut, dp, btheta, btheta_o, ab2, farm, ucont_o, dt, rhs,
                form_mode: int = 0):
         """UeqnFE (src/solvers/ueqn_solvers.c:1411-1441): ucont <- ucont_o + dt ExplicitRhsU."""
