@@ -82,7 +82,7 @@ def test_formt_form_modes(rhs_mod, mode):
     assert _compare(rhs_mod, lay, mesh, params, f, tp, -0.75, mode, f"mode {mode}, scale -0.75") <= TOL
 
 
-@pytest.mark.parametrize("periodic", [(2, 0, 2), (0, 0, 0)])
+@pytest.mark.parametrize("periodic", [(2, 0, 2), (0, 0, 0), (1, 2, 1)])
 def test_formt_larger_mesh(rhs_mod, periodic):
     """several workgroups along every axis"""
     lay, mesh, params, f, tp = _case((70, 37, 23), periodic, 5, 1, C.LES_SMAGORINSKY, walls=True)

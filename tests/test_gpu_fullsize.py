@@ -211,3 +211,4 @@ def test_explicit_rhs_u_fused_vs_reference_structured(shard):
     e = _rel(res["fused"][2], res["ref"][2])
     print(f"  ExplicitRhsU rhs fused vs reference-structured relL2={e:.2e}", flush=True)
     assert e <= 1e-12
+

@@ -146,6 +146,78 @@ __device__ __forceinline__ double kappa_eff(const FormTArgs &A, long c, long cn,
     return (nu / A.pr) + (nut / Prt);
 }
 
+// The advective (d) and diffusive (v) flux of one face, as the reference's face loop evaluates it
+// (teqn_solvers.c:127-246 / 250-370 / 374-478).  Face (k, j, i) of direction DIR lies between
+// cell (k, j, i) and its + neighbour along DIR.
+__device__ __forceinline__ void flux_i(const FormTArgs &A, const Geo &g, int i, int j, int k, double &d, double &v) {
+    const double *T = A.t;
+    const long c = g.L(k, j, i);
+    const double ajc = A.iaj[c];
+    const V3 cs = ld3(A.icsi, c), et = ld3(A.ieta, c), ze = ld3(A.izet, c);
+    const double g11 = cs.x * cs.x + cs.y * cs.y + cs.z * cs.z;
+    const double g21 = et.x * cs.x + et.y * cs.y + et.z * cs.z;
+    const double g31 = ze.x * cs.x + ze.y * cs.y + ze.z * cs.z;
+    double dtdc, dtde, dtdz;
+    dscalar_i(g, T, i, j, k, dtdc, dtde, dtdz);
+    int iL, iR;
+    face_stencil(g.per[0], i, g.mx, iL, iR);
+    const double u = A.ucont[3 * c];
+    d = -u * weno3(T[g.L(k, j, iL)], T[c], T[g.L(k, j, i + 1)], T[g.L(k, j, iR)], u);
+    double kap = kappa_eff(A, c, g.L(k, j, i + 1), ajc, dtde);
+    v = 0.0;
+    if (A.les && ((A.wall[0] && i == 0) || (A.wall[1] && i == g.mx - 2))) {
+        const double signQ = (i == 0) ? -1.0 : 1.0;
+        const long pl = (long)g.mz * g.my, o = (long)k * g.my + j;
+        v = signQ * (A.qI[o] * cs.x + A.qI[pl + o] * cs.y + A.qI[2 * pl + o] * cs.z);
+        kap = 0.0;
+    }
+    v += (g11 * dtdc + g21 * dtde + g31 * dtdz) * ajc * kap;
+}
+
+__device__ __forceinline__ void flux_j(const FormTArgs &A, const Geo &g, int i, int j, int k, double &d, double &v) {
+    const double *T = A.t;
+    const long c = g.L(k, j, i);
+    const double ajc = A.jaj[c];
+    const V3 cs = ld3(A.jcsi, c), et = ld3(A.jeta, c), ze = ld3(A.jzet, c);
+    const double g11 = cs.x * et.x + cs.y * et.y + cs.z * et.z;
+    const double g21 = et.x * et.x + et.y * et.y + et.z * et.z;
+    const double g31 = ze.x * et.x + ze.y * et.y + ze.z * et.z;
+    double dtdc, dtde, dtdz;
+    dscalar_j(g, T, i, j, k, dtdc, dtde, dtdz);
+    int jL, jR;
+    face_stencil(g.per[1], j, g.my, jL, jR);
+    const double u = A.ucont[3 * c + 1];
+    d = -u * weno3(T[g.L(k, jL, i)], T[c], T[g.L(k, j + 1, i)], T[g.L(k, jR, i)], u);
+    double kap = kappa_eff(A, c, g.L(k, j + 1, i), ajc, dtde);
+    v = 0.0;
+    if (A.les && ((A.wall[2] && j == 0) || (A.wall[3] && j == g.my - 2))) {
+        const double signQ = (j == 0) ? -1.0 : 1.0;
+        const long pl = (long)g.mz * g.mx, o = (long)k * g.mx + i;
+        v = signQ * (A.qJ[o] * et.x + A.qJ[pl + o] * et.y + A.qJ[2 * pl + o] * et.z);
+        kap = 0.0;
+    }
+    v += (g11 * dtdc + g21 * dtde + g31 * dtdz) * ajc * kap;
+}
+
+__device__ __forceinline__ void flux_k(const FormTArgs &A, const Geo &g, int i, int j, int k, double &d, double &v) {
+    const double *T = A.t;
+    const long c = g.L(k, j, i);
+    const double ajc = A.kaj[c];
+    const V3 cs = ld3(A.kcsi, c), et = ld3(A.keta, c), ze = ld3(A.kzet, c);
+    const double g11 = cs.x * ze.x + cs.y * ze.y + cs.z * ze.z;
+    const double g21 = et.x * ze.x + et.y * ze.y + et.z * ze.z;
+    const double g31 = ze.x * ze.x + ze.y * ze.y + ze.z * ze.z;
+    double dtdc, dtde, dtdz;
+    dscalar_k(g, T, i, j, k, dtdc, dtde, dtdz);
+    int kL, kR;
+    face_stencil(g.per[2], k, g.mz, kL, kR);
+    const double u = A.ucont[3 * c + 2];
+    d = -u * weno3(T[g.L(kL, j, i)], T[c], T[g.L(k + 1, j, i)], T[g.L(kR, j, i)], u);
+    const double kap = kappa_eff(A, c, g.L(k + 1, j, i), ajc, dtde);
+    v = 0.0;
+    v += (g11 * dtdc + g21 * dtde + g31 * dtdz) * ajc * kap;
+}
+
 // One thread per local (ghosted) point; points off the reference's face loop get zero fluxes.
 __global__ void k_formt_faces(FormTArgs A, Geo g) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x - GW, j = blockIdx.y * blockDim.y + threadIdx.y - GW,
@@ -155,62 +227,9 @@ __global__ void k_formt_faces(FormTArgs A, Geo g) {
     V3 dv{0.0, 0.0, 0.0}, vs{0.0, 0.0, 0.0};
     const bool in = i >= 0 && j >= 0 && k >= 0 && i < g.mx - 1 && j < g.my - 1 && k < g.mz - 1;
     if (in) {
-        const double *T = A.t;
-        double dtdc, dtde, dtdz;
-        if (j != 0 && k != 0) {                      // i face
-            const double ajc = A.iaj[c];
-            const V3 cs = ld3(A.icsi, c), et = ld3(A.ieta, c), ze = ld3(A.izet, c);
-            const double g11 = cs.x * cs.x + cs.y * cs.y + cs.z * cs.z;
-            const double g21 = et.x * cs.x + et.y * cs.y + et.z * cs.z;
-            const double g31 = ze.x * cs.x + ze.y * cs.y + ze.z * cs.z;
-            dscalar_i(g, T, i, j, k, dtdc, dtde, dtdz);
-            int iL, iR;
-            face_stencil(g.per[0], i, g.mx, iL, iR);
-            const double u = A.ucont[3 * c];
-            dv.x = -u * weno3(T[g.L(k, j, iL)], T[c], T[g.L(k, j, i + 1)], T[g.L(k, j, iR)], u);
-            double kap = kappa_eff(A, c, g.L(k, j, i + 1), ajc, dtde);
-            if (A.les && ((A.wall[0] && i == 0) || (A.wall[1] && i == g.mx - 2))) {
-                const double signQ = (i == 0) ? -1.0 : 1.0;
-                const long pl = (long)g.mz * g.my, o = (long)k * g.my + j;
-                vs.x = signQ * (A.qI[o] * cs.x + A.qI[pl + o] * cs.y + A.qI[2 * pl + o] * cs.z);
-                kap = 0.0;
-            }
-            vs.x += (g11 * dtdc + g21 * dtde + g31 * dtdz) * ajc * kap;
-        }
-        if (i != 0 && k != 0) {                      // j face
-            const double ajc = A.jaj[c];
-            const V3 cs = ld3(A.jcsi, c), et = ld3(A.jeta, c), ze = ld3(A.jzet, c);
-            const double g11 = cs.x * et.x + cs.y * et.y + cs.z * et.z;
-            const double g21 = et.x * et.x + et.y * et.y + et.z * et.z;
-            const double g31 = ze.x * et.x + ze.y * et.y + ze.z * et.z;
-            dscalar_j(g, T, i, j, k, dtdc, dtde, dtdz);
-            int jL, jR;
-            face_stencil(g.per[1], j, g.my, jL, jR);
-            const double u = A.ucont[3 * c + 1];
-            dv.y = -u * weno3(T[g.L(k, jL, i)], T[c], T[g.L(k, j + 1, i)], T[g.L(k, jR, i)], u);
-            double kap = kappa_eff(A, c, g.L(k, j + 1, i), ajc, dtde);
-            if (A.les && ((A.wall[2] && j == 0) || (A.wall[3] && j == g.my - 2))) {
-                const double signQ = (j == 0) ? -1.0 : 1.0;
-                const long pl = (long)g.mz * g.mx, o = (long)k * g.mx + i;
-                vs.y = signQ * (A.qJ[o] * et.x + A.qJ[pl + o] * et.y + A.qJ[2 * pl + o] * et.z);
-                kap = 0.0;
-            }
-            vs.y += (g11 * dtdc + g21 * dtde + g31 * dtdz) * ajc * kap;
-        }
-        if (i != 0 && j != 0) {                      // k face
-            const double ajc = A.kaj[c];
-            const V3 cs = ld3(A.kcsi, c), et = ld3(A.keta, c), ze = ld3(A.kzet, c);
-            const double g11 = cs.x * ze.x + cs.y * ze.y + cs.z * ze.z;
-            const double g21 = et.x * ze.x + et.y * ze.y + et.z * ze.z;
-            const double g31 = ze.x * ze.x + ze.y * ze.y + ze.z * ze.z;
-            dscalar_k(g, T, i, j, k, dtdc, dtde, dtdz);
-            int kL, kR;
-            face_stencil(g.per[2], k, g.mz, kL, kR);
-            const double u = A.ucont[3 * c + 2];
-            dv.z = -u * weno3(T[g.L(kL, j, i)], T[c], T[g.L(k + 1, j, i)], T[g.L(kR, j, i)], u);
-            const double kap = kappa_eff(A, c, g.L(k + 1, j, i), ajc, dtde);
-            vs.z += (g11 * dtdc + g21 * dtde + g31 * dtdz) * ajc * kap;
-        }
+        if (j != 0 && k != 0) flux_i(A, g, i, j, k, dv.x, vs.x);
+        if (i != 0 && k != 0) flux_j(A, g, i, j, k, dv.y, vs.y);
+        if (i != 0 && j != 0) flux_k(A, g, i, j, k, dv.z, vs.z);
     }
     st3(A.div, c, dv);
     st3(A.visc, c, vs);
@@ -237,10 +256,9 @@ bool kt_model(int m) {
 
 }  // namespace
 
-void ref_formt(const Geo &g, const tosca_mesh *m, const tosca_params *p, const tosca_teqn_params *tp,
-               const double *ucont, const double *theta, const double *nut, const double *kt, double *rhs,
-               double scale, int mode, double *ws, hipStream_t s) {
-    const long n = g.nloc();
+namespace {
+FormTArgs formt_args(const Geo &g, const tosca_mesh *m, const tosca_params *p, const tosca_teqn_params *tp,
+                     const double *ucont, const double *theta, const double *nut, const double *kt) {
     FormTArgs A;
     A.ucont = ucont; A.t = theta; A.nut = nut; A.kt = kt;
     A.icsi = m->icsi; A.ieta = m->ieta; A.izet = m->izet;
@@ -248,11 +266,21 @@ void ref_formt(const Geo &g, const tosca_mesh *m, const tosca_params *p, const t
     A.kcsi = m->kcsi; A.keta = m->keta; A.kzet = m->kzet;
     A.iaj = m->iaj; A.jaj = m->jaj; A.kaj = m->kaj;
     A.qI = tp->q_wall_i; A.qJ = tp->q_wall_j;
-    A.div = ws; A.visc = ws + 3 * n;
+    A.div = nullptr; A.visc = nullptr;
     A.nu = p->nu; A.pr = tp->pr; A.tref = p->t_ref;
     A.les = p->les_active != 0;
     A.ktm = A.les && kt_model(p->les_model);
     for (int q = 0; q < 4; q++) A.wall[q] = tp->wall_t[q];
+    return A;
+}
+}  // namespace
+
+void ref_formt(const Geo &g, const tosca_mesh *m, const tosca_params *p, const tosca_teqn_params *tp,
+               const double *ucont, const double *theta, const double *nut, const double *kt, double *rhs,
+               double scale, int mode, double *ws, hipStream_t s) {
+    const long n = g.nloc();
+    FormTArgs A = formt_args(g, m, p, tp, ucont, theta, nut, kt);
+    A.div = ws; A.visc = ws + 3 * n;
     const dim3 blk(64, 4, 1);
     TOSCA_LAUNCH("k_formt_faces", k_formt_faces, dim3((g.gxm + 63) / 64, (g.gym + 3) / 4, g.gzm), blk, 0, s, A, g);
     launch_wrap(A.div, 3, g, s);
