@@ -246,6 +246,48 @@ int tosca_explicit_rhs_u(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_par
                          const double *btheta, const double *btheta_o, int ab2, const double *farm,
                          const double *ucont_o, double *rhs, int form_mode);
 
+/* Settings of the IMEX momentum solve, kspIMEX (src/ueqn.c:217-282): GMRES with restart
+ * `restart` (ueqn->gmresRestart, 30), no preconditioner, nonzero initial guess, converged when the
+ * GMRES residual estimate is <= max(rtol * ||r0||, atol) (relTolU 1e-30, absTolU 1e-5), at most
+ * max_it iterations (1000). */
+typedef struct {
+    int    restart;
+    int    max_it;
+    double rtol;
+    double atol;
+} tosca_krylov_params;
+
+/* Outcome of a solve: total GMRES iterations, the last residual estimate, and reason > 0
+ * converged, -3 iteration limit, -4 NaN residual, -5 breakdown, -9 null rotation. */
+typedef struct {
+    int    iterations;
+    double residual;
+    int    reason;
+} tosca_krylov_result;
+
+/* Replaces KSPSolve(ueqn->kspIMEX, b, x) of the IMEX schemes (ueqn_solvers.c:1564, 1641, 1716)
+ * with IMEXMatVec as the operator: restarted GMRES on the device (PETSc 3.15's KSPGMRES algorithm:
+ * classical Gram-Schmidt, Givens rotations, no preconditioner).  b, x: global AoS, x the initial
+ * guess on entry and the solution on return; ucont, ucat: the operator's local work arrays;
+ * coef = dt scale as for tosca_imex_matvec.  The Krylov basis, (restart + 2) global vectors, lives
+ * in the context and is allocated on first use. */
+int tosca_imex_solve(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_params *params,
+                     const double *b, double *x, double *ucont, double *ucat, const double *nut,
+                     const double *ucont_o, double coef, const tosca_krylov_params *kp,
+                     tosca_krylov_result *kr);
+
+/* Replaces UeqnRK3CN_W (scheme 0, "RK3WCN", src/solvers/ueqn_solvers.c:1508-1583) and
+ * UeqnRK3CN_SO (scheme 1, "RK3SOCN", :1586-1661): three IMEX Runge-Kutta stages, each with
+ * ExplicitRhsU(formMode 1) into rhs, the viscous FormU(formMode 2) with its non-resolved faces
+ * reset, the stage's right-hand side bU and explicit-Euler initial guess assembled with the
+ * reference's VecAXPY sequence, and tosca_imex_solve with coef = dt gamma_i.  Arguments as
+ * tosca_ueqn_rk4; kr reports the last stage's solve.  Work vectors live in the context. */
+int tosca_ueqn_rk3cn(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_params *params, int scheme,
+                     double *ucont, double *ucat, const double *nut, const double *dp,
+                     const double *btheta, const double *btheta_o, int ab2, const double *farm,
+                     const double *ucont_o, double dt, double *rhs, const tosca_krylov_params *kp,
+                     tosca_krylov_result *kr);
+
 /* Replaces GradP (src/peqn.c:3437-3773): the contravariant pressure-gradient fluxes
  * dP^q = (g^q1 dp/dxi + g^q2 dp/deta + g^q3 dp/dzeta) aj_q at the faces 1..m-2 of the interior
  * cells, with the reference's one-sided transverse differences on the corner face rows and its

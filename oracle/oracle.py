@@ -54,6 +54,9 @@ def load() -> C.CDLL:
         "oracle_ueqn_rk4": (I, [In, M, Pa, P, P, P, P, P, P, I, P, P, D, P, P, I]),
         "oracle_imex_matvec": (I, [In, M, Pa, P, P, P, P, P, D, P]),
         "oracle_grad_p": (I, [In, M, Pa, P, P]),
+        "oracle_gmres_dense": (I, [C.c_int64, P, P, P, I, D, D, I, P, P, P]),
+        "oracle_imex_solve": (I, [In, M, Pa, P, P, P, P, P, P, D, I, D, D, I, P, P, P]),
+        "oracle_ueqn_rk3cn": (I, [In, M, Pa, I, P, P, P, P, P, P, I, P, P, D, P, I, D, D, I, P, P, P]),
     }
     for n, (r, a) in sigs.items():
         f = getattr(lib, n)
@@ -225,3 +228,45 @@ def grad_p(info, mesh, params, p, dp):
     lib = load()
     inf, m, pa = abi.make_info(info), _mesh(mesh), abi.make_params(params)
     _check(lib.oracle_grad_p(C.byref(inf), C.byref(m), C.byref(pa), _p(p), _p(dp)))
+
+
+def _kout():
+    return C.c_int(0), C.c_double(0.0), C.c_int(0)
+
+
+def gmres_dense(A, b, x, restart=30, rtol=1e-30, atol=1e-5, maxit=1000):
+    """GMRES(restart) restated from PETSc on a dense matrix; x (initial guess) is updated in place.
+    Returns (iterations, residual estimate, reason: > 0 converged)."""
+    import numpy as np
+    lib = load()
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    its, res, conv = _kout()
+    _check(lib.oracle_gmres_dense(int(len(b)), _p(A), _p(b), _p(x), int(restart), float(rtol), float(atol), int(maxit),
+                                  C.byref(its), C.byref(res), C.byref(conv)))
+    return its.value, res.value, conv.value
+
+
+def imex_solve(info, mesh, params, b, x, ucont, ucat, nut, ucont_o, coef, restart=30, rtol=1e-30, atol=1e-5,
+               maxit=1000):
+    """KSPSolve(kspIMEX, b, x) restated: x (global) updated in place; returns (its, res, reason)."""
+    lib = load()
+    inf, m, p = abi.make_info(info), _mesh(mesh), abi.make_params(params)
+    its, res, conv = _kout()
+    _check(lib.oracle_imex_solve(C.byref(inf), C.byref(m), C.byref(p), _p(b), _p(x), _p(ucont), _p(ucat), _p(nut),
+                                 _p(ucont_o), float(coef), int(restart), float(rtol), float(atol), int(maxit),
+                                 C.byref(its), C.byref(res), C.byref(conv)))
+    return its.value, res.value, conv.value
+
+
+def ueqn_rk3cn(info, mesh, params, scheme, ucont, ucat, nut, dp, btheta, btheta_o, ab2, farm, ucont_o, dt, rhs,
+               restart=30, rtol=1e-30, atol=1e-5, maxit=1000):
+    """UeqnRK3CN_W (scheme 0) / UeqnRK3CN_SO (scheme 1) restated; returns the last stage's
+    (its, res, reason)."""
+    lib = load()
+    inf, m, p = abi.make_info(info), _mesh(mesh), abi.make_params(params)
+    its, res, conv = _kout()
+    _check(lib.oracle_ueqn_rk3cn(C.byref(inf), C.byref(m), C.byref(p), int(scheme), _p(ucont), _p(ucat), _p(nut),
+                                 _p(dp), _p(btheta), _p(btheta_o), int(ab2), _p(farm), _p(ucont_o), float(dt),
+                                 _p(rhs), int(restart), float(rtol), float(atol), int(maxit), C.byref(its),
+                                 C.byref(res), C.byref(conv)))
+    return its.value, res.value, conv.value

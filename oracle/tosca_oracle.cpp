@@ -27,6 +27,7 @@
 #include <string>
 #include <vector>
 #include <algorithm>
+#include <functional>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -1283,6 +1284,204 @@ int oracle_imex_matvec(const tosca_dmda_info *info, const tosca_mesh *m, const t
     if (rc) return rc;
     reset_nonresolved(l, p, av, (const V3 *)ucont_o);         // :1795
     axpy(ng, 1.0, (const V3 *)v_, av);                        // :1796
+    return TOSCA_OK;
+}
+
+// ================================================================================================
+// GMRES(m) as PETSc 3.15 runs it for kspIMEX (src/ueqn.c:261-282: KSPGMRES, PCNONE, nonzero
+// initial guess, KSPSetTolerances(rtol, atol, 1e30, max_it)).  PETSc is a dependency absent from
+// the reference tree; this restates its published algorithm (src/ksp/ksp/impls/gmres/gmres.c:
+// KSPSolve_GMRES, KSPGMRESCycle, KSPGMRESUpdateHessenberg, KSPGMRESBuildSoln, with the default
+// classical Gram-Schmidt orthogonalisation without refinement; KSPConvergedDefault).  The relative
+// test is taken against the initial residual norm; at the reference's rtol = 1e-30 only atol acts.
+// ================================================================================================
+static double vdot(int64_t n, const double *a, const double *b) {
+    double s = 0.0;
+    for (int64_t i = 0; i < n; i++) s += a[i] * b[i];
+    return s;
+}
+
+static int gmres(int64_t n, const std::function<int(const double *, double *)> &A, const double *b, double *x,
+                 int restart, double rtol, double atol, int maxit, int *its_out, double *res_out, int *conv) {
+    const int m = restart;
+    std::vector<double> V((size_t)(m + 1) * n), tmp(n);
+    std::vector<double> HH((size_t)(m + 2) * (m + 1), 0.0), cc(m + 1), ss(m + 1), grs(m + 2), nrs(m + 1);
+    auto H = [&](int a, int c) -> double & { return HH[(size_t)c * (m + 2) + a]; };
+    int its = 0, reason = 0;
+    double ttol = 0.0, res = 0.0;
+    bool first = true;
+    while (!reason) {
+        // KSPInitialResidual: r = b - A x into V_0, then VecNormalize
+        int rc = A(x, tmp.data());
+        if (rc) return rc;
+        double *v0 = V.data();
+        for (int64_t i = 0; i < n; i++) v0[i] = b[i] + (-1.0) * tmp[i];
+        res = std::sqrt(vdot(n, v0, v0));
+        if (res > 0.0) {
+            const double sc = 1.0 / res;
+            for (int64_t i = 0; i < n; i++) v0[i] = v0[i] * sc;
+        }
+        grs[0] = res;
+        if (first) { ttol = std::max(rtol * res, atol); first = false; }
+        if (std::isnan(res)) { reason = -4; break; }
+        if (res <= ttol) reason = 1;
+        int it = 0;
+        bool hapend = false;
+        while (!reason && it < m && its < maxit) {
+            double *w = V.data() + (size_t)(it + 1) * n;
+            rc = A(V.data() + (size_t)it * n, w);
+            if (rc) return rc;
+            // classical Gram-Schmidt: all dots first (VecMDot), then one update (VecMAXPY)
+            std::vector<double> h(it + 1);
+            for (int j = 0; j <= it; j++) h[j] = vdot(n, w, V.data() + (size_t)j * n);
+            for (int j = 0; j <= it; j++) {
+                const double a = -h[j];
+                const double *vj = V.data() + (size_t)j * n;
+                for (int64_t i = 0; i < n; i++) w[i] = w[i] + a * vj[i];
+                H(j, it) = h[j];
+            }
+            const double tt = std::sqrt(vdot(n, w, w));
+            H(it + 1, it) = tt;
+            double hapbnd = std::fabs(tt / grs[it]);
+            if (hapbnd > 1e-30) hapbnd = 1e-30;
+            hapend = tt < hapbnd;
+            if (!hapend) {
+                const double sc = 1.0 / tt;
+                for (int64_t i = 0; i < n; i++) w[i] = w[i] * sc;
+            }
+            // KSPGMRESUpdateHessenberg: previous rotations on the new column, then a new one
+            for (int j = 1; j <= it; j++) {
+                const double t0 = H(j - 1, it);
+                H(j - 1, it) = cc[j - 1] * t0 + ss[j - 1] * H(j, it);
+                H(j, it) = cc[j - 1] * H(j, it) - ss[j - 1] * t0;
+            }
+            if (!hapend) {
+                const double t1 = std::sqrt(H(it, it) * H(it, it) + H(it + 1, it) * H(it + 1, it));
+                if (t1 == 0.0) { reason = -9; break; }
+                cc[it] = H(it, it) / t1;
+                ss[it] = H(it + 1, it) / t1;
+                grs[it + 1] = -(ss[it] * grs[it]);
+                grs[it] = cc[it] * grs[it];
+                H(it, it) = cc[it] * H(it, it) + ss[it] * H(it + 1, it);
+                res = std::fabs(grs[it + 1]);
+            } else {
+                res = 0.0;
+            }
+            it++;
+            its++;
+            if (std::isnan(res)) reason = -4;
+            else if (res <= ttol) reason = 1;
+            if (hapend) { if (!reason) reason = -5; break; }
+        }
+        // KSPGMRESBuildSoln over columns 0 .. it-1
+        const int last = it - 1;
+        if (last >= 0) {
+            if (H(last, last) != 0.0) nrs[last] = grs[last] / H(last, last);
+            else nrs[last] = 0.0;
+            for (int ii = 1; ii <= last; ii++) {
+                const int k = last - ii;
+                double tt = grs[k];
+                for (int j = k + 1; j <= last; j++) tt = tt - H(k, j) * nrs[j];
+                nrs[k] = tt / H(k, k);
+            }
+            for (int64_t i = 0; i < n; i++) tmp[i] = 0.0;
+            for (int j = 0; j <= last; j++) {
+                const double *vj = V.data() + (size_t)j * n;
+                for (int64_t i = 0; i < n; i++) tmp[i] = tmp[i] + nrs[j] * vj[i];
+            }
+            for (int64_t i = 0; i < n; i++) x[i] = x[i] + 1.0 * tmp[i];
+        }
+        if (its >= maxit && !reason) reason = -3;
+    }
+    if (its_out) *its_out = its;
+    if (res_out) *res_out = res;
+    if (conv) *conv = reason;
+    return TOSCA_OK;
+}
+
+// GMRES on a dense row-major n x n matrix (test infrastructure for the solver itself)
+int oracle_gmres_dense(int64_t n, const double *Am, const double *b, double *x, int restart, double rtol, double atol,
+                       int maxit, int *its, double *res, int *conv) {
+    auto A = [&](const double *v, double *out) {
+        for (int64_t r = 0; r < n; r++) {
+            double s = 0.0;
+            for (int64_t c = 0; c < n; c++) s += Am[r * n + c] * v[c];
+            out[r] = s;
+        }
+        return 0;
+    };
+    return gmres(n, A, b, x, restart, rtol, atol, maxit, its, res, conv);
+}
+
+// KSPSolve(kspIMEX, b, x) with IMEXMatVec as the operator; ucont, ucat are its work arrays
+int oracle_imex_solve(const tosca_dmda_info *info, const tosca_mesh *m, const tosca_params *p, const double *b,
+                      double *x, double *ucont, double *ucat, const double *nut, const double *ucont_o, double coef,
+                      int restart, double rtol, double atol, int maxit, int *its, double *res, int *conv) {
+    Lay l;
+    int rc = check_layout(info, p, l);
+    if (rc) return rc;
+    const int64_t n = 3 * l.mx * l.my * l.mz;
+    auto A = [&](const double *v, double *out) {
+        return oracle_imex_matvec(info, m, p, v, ucont, ucat, nut, ucont_o, coef, out);
+    };
+    return gmres(n, A, b, x, restart, rtol, atol, maxit, its, res, conv);
+}
+
+// UeqnRK3CN_W (scheme 0, ueqn_solvers.c:1508-1583) and UeqnRK3CN_SO (scheme 1, :1586-1661): three
+// IMEX stages, convection and the other explicit terms from ExplicitRhsU(formMode 1), diffusion
+// Crank-Nicolson-like through the IMEX solve with coef = dt gamma_i.  its / res / conv report the
+// last stage's solve, as the reference prints them.
+int oracle_ueqn_rk3cn(const tosca_dmda_info *info, const tosca_mesh *m, const tosca_params *p, int scheme,
+                      double *ucont_, double *ucat_, const double *nut, const double *dp, const double *btheta,
+                      const double *btheta_o, int ab2, const double *farm, const double *ucont_o_, double dt,
+                      double *rhs_, int restart, double rtol, double atol, int maxit, int *its, double *res,
+                      int *conv) {
+    Lay l;
+    int rc = check_layout(info, p, l);
+    if (rc) return rc;
+    if (scheme != 0 && scheme != 1) { g_err = "ueqn_rk3cn: scheme must be 0 (Wray) or 1 (Shu-Osher)"; return TOSCA_ERR_ARG; }
+    const int64_t ng = l.mx * l.my * l.mz;
+    V3 *rhs = (V3 *)rhs_;
+    const V3 *uo = (const V3 *)ucont_o_;
+    std::vector<V3> rhs_o(ng, V3{0, 0, 0}), visc(ng), bU(ng), utmp(ng), ucg(ng);
+    const double wb[3] = {8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0}, wg[3] = {4.0 / 15.0, 1.0 / 15.0, 1.0 / 6.0},
+                 wd[3] = {0.0, 17.0 / 60.0, 5.0 / 12.0};
+    const double sa[3] = {1.0, 3.0 / 4.0, 1.0 / 3.0}, sb[3] = {1.0, 1.0 / 4.0, 2.0 / 3.0},
+                 sg[3] = {1.0 / 2.0, 1.0 / 8.0, 1.0 / 3.0};
+    for (int i = 0; i < 3; i++) {
+        rc = oracle_explicit_rhs_u(info, m, p, ucont_, ucat_, nut, dp, btheta, btheta_o, ab2, farm, ucont_o_, rhs_, 1);
+        if (rc) return rc;
+        for (int64_t g = 0; g < ng; g++) visc[g] = V3{0, 0, 0};
+        rc = oracle_formu(info, m, p, ucont_, ucat_, nut, (double *)visc.data(), 1.0, 2);
+        if (rc) return rc;
+        reset_nonresolved(l, p, visc.data(), uo);
+        for (int64_t k = 0; k < l.mz; k++)             // the global Ucont: owned range of lUcont
+            for (int64_t j = 0; j < l.my; j++)
+                for (int64_t ii = 0; ii < l.mx; ii++) ucg[l.G(k, j, ii)] = ((const V3 *)ucont_)[l.L(k, j, ii)];
+        double gam;
+        if (scheme == 0) {
+            bU = ucg;                                               // :1539
+            axpy(ng, dt * wb[i], rhs, bU.data());                  // :1542
+            axpy(ng, -1.0 * dt * wd[i], rhs_o.data(), bU.data());  // :1545
+            axpy(ng, dt * wg[i], visc.data(), bU.data());          // :1548
+            gam = wg[i];
+        } else {
+            for (int64_t g = 0; g < ng; g++) {                      // :1616-1617 VecCopy + VecScale
+                bU[g].x = sa[i] * uo[g].x; bU[g].y = sa[i] * uo[g].y; bU[g].z = sa[i] * uo[g].z;
+            }
+            if (i > 0) axpy(ng, 1.0 - sa[i], ucg.data(), bU.data());   // :1620-1624
+            axpy(ng, dt * sb[i], rhs, bU.data());                   // :1627
+            axpy(ng, dt * sg[i], visc.data(), bU.data());           // :1630
+            gam = sg[i];
+        }
+        utmp = bU;                                                  // initial guess
+        axpy(ng, dt * gam, visc.data(), utmp.data());
+        rc = oracle_imex_solve(info, m, p, (const double *)bU.data(), (double *)utmp.data(), ucont_, ucat_, nut,
+                               ucont_o_, dt * gam, restart, rtol, atol, maxit, its, res, conv);
+        if (rc) return rc;
+        global_vec_to_local(l, utmp.data(), (V3 *)ucont_);
+        if (scheme == 0) rhs_o.assign(rhs, rhs + ng);
+    }
     return TOSCA_OK;
 }
 #endif  // ORACLE_TANGENT

@@ -79,6 +79,21 @@ def make_teqn_params(d: Mapping) -> TeqnParams:
     return t
 
 
+class KrylovParams(C.Structure):
+    _fields_ = [("restart", C.c_int), ("max_it", C.c_int), ("rtol", C.c_double), ("atol", C.c_double)]
+
+
+class KrylovResult(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("residual", C.c_double), ("reason", C.c_int)]
+
+
+def make_krylov_params(restart=30, max_it=1000, rtol=1e-30, atol=1e-5) -> KrylovParams:
+    """kspIMEX settings; the defaults are the reference's (src/ueqn.c:217-282)"""
+    k = KrylovParams()
+    k.restart, k.max_it, k.rtol, k.atol = int(restart), int(max_it), float(rtol), float(atol)
+    return k
+
+
 SIGNATURES = {
     "tosca_version": (C.c_char_p, []),
     "tosca_last_error": (C.c_char_p, [P]),
@@ -109,6 +124,10 @@ SIGNATURES = {
     "tosca_ueqn_rk4": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P, P, P, P, I, P, P, D, P, P, I]),
     "tosca_imex_matvec": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P, P, P, D, P]),
     "tosca_grad_p": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P]),
+    "tosca_imex_solve": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P, P, P, P, D, C.POINTER(KrylovParams),
+                             C.POINTER(KrylovResult)]),
+    "tosca_ueqn_rk3cn": (I, [P, C.POINTER(Mesh), C.POINTER(Params), I, P, P, P, P, P, P, I, P, P, D, P,
+                             C.POINTER(KrylovParams), C.POINTER(KrylovResult)]),
 }
 
 _lib: Optional[C.CDLL] = None

@@ -38,6 +38,7 @@ SOURCES = {
     "bc_fill.hip": ["-ffp-contract=off"],
     "formt.hip": ["-ffp-contract=off"],
     "gradp.hip": ["-ffp-contract=off"],
+    "krylov.hip": ["-ffp-contract=off"],
 }
 
 

@@ -6,6 +6,7 @@
 // a GPU memory fault.  Unsupported reference features (IBM, overset, structural LES models,
 // schemes other than central/QUICK) are rejected with TOSCA_ERR_UNSUPPORTED.
 #include <cstdarg>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -29,6 +30,8 @@ struct tosca_ctx {
     size_t ws_elems = 0;
     int *chk = nullptr;
     size_t chk_elems = 0;
+    double *kws = nullptr;       // Krylov basis and the IMEX drivers' work vectors
+    size_t kws_elems = 0;
     tosca::KernelTimer timer;
     bool ref_path = false;   // TOSCA_PATH=ref at creation: reference-structured kernels only
 };
@@ -336,6 +339,181 @@ void queue_explicit(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, do
     tosca::launch_non_resolved(rhs, ucont_o, p, g, s);               // :1403
 }
 
+// IMEXMatVec's kernels (ueqn_solvers.c:1739-1801) on validated arguments: av = v - coef Visc(v)
+void queue_imex(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const double *v, double *ucont,
+                double *ucat, const double *nut, const double *ucont_o, double coef, double *av) {
+    const Geo &g = c->geo;
+    hipStream_t s = c->stream;
+    const long ng3 = 3L * g.mx * g.my * g.mz;
+    tosca::launch_global_to_local(ucont, v, g, s);                         // :1763-1767
+    tosca::launch_wrap(ucont, 3, g, s);
+    tosca::launch_no_penetration(ucont, p, g, s);                          // :1770
+    tosca::launch_reset_face_periodic(ucont, 3, g, s);                     // :1773
+    tosca::launch_contra_to_cart(ucont, m, ucat, g, s);                    // :1777
+    tosca::launch_reset_cell_periodic(ucat, 3, g, s);                      // :1790
+    tosca::launch_fill(av, ng3, 0.0, s);                                   // :1793
+    const double *nu_t = p->les_active ? nut : nullptr;
+    if (c->ref_path) {                                                     // :1794
+        tosca::ref_formu(g, m, p, ucont, ucat, nu_t, av, -1.0 * coef, 2, c->ws, s);
+    } else {
+        tosca::FormUArgs A = formu_args(m, p, ucont, ucat, nu_t, av, -1.0 * coef, 2);
+        tosca::fused_formu(g, A, c->ws + 3 * (size_t)g.nloc(), s);
+    }
+    tosca::launch_non_resolved(av, ucont_o, p, g, s);                      // :1795
+    tosca::launch_axpy(av, 1.0, v, ng3, s);                                // :1796
+}
+
+int check_imex(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const double *ucont, const double *ucat,
+               const double *nut, const double *ucont_o) {
+    int rc;
+    if ((rc = check_params(c, p))) return rc;
+    if (p->div_scheme != TOSCA_DIV_CENTRAL && p->div_scheme != TOSCA_DIV_QUICK)
+        return set_err(c, TOSCA_ERR_UNSUPPORTED, "div_scheme = %d: only central (0) and QUICK (1) are supported", p->div_scheme);
+    set_geo_per(c, p);
+    const Geo &g = c->geo;
+    const size_t n = (size_t)g.nloc(), vloc = 3 * n * sizeof(double), vglob = 3 * (size_t)g.mx * g.my * g.mz * sizeof(double);
+    if ((rc = check_mesh(c, m, true, true, true))) return rc;
+    if ((rc = check_dev(c, ucont, vloc, "ucont")) || (rc = check_dev(c, ucat, vloc, "ucat")) ||
+        (rc = check_dev(c, ucont_o, vglob, "ucont_o")))
+        return rc;
+    if (p->les_active) {
+        if ((rc = check_les_model(c, p))) return rc;
+        if ((rc = check_dev(c, nut, n * sizeof(double), "nut"))) return rc;
+    }
+    if ((rc = check_wall_tau(c, p))) return rc;
+    return ensure_ws(c, ws_need(c));
+}
+
+int check_krylov(tosca_ctx *c, const tosca_krylov_params *kp, tosca_krylov_result *kr) {
+    if (!kp || !kr) return set_err(c, TOSCA_ERR_ARG, "krylov params / result is NULL");
+    if (kp->restart < 1 || kp->restart > 200) return set_err(c, TOSCA_ERR_ARG, "restart = %d (1 .. 200)", kp->restart);
+    if (kp->max_it < 0) return set_err(c, TOSCA_ERR_ARG, "max_it = %d", kp->max_it);
+    if (!(kp->atol >= 0.0) || !(kp->rtol >= 0.0)) return set_err(c, TOSCA_ERR_ARG, "tolerances must be >= 0");
+    return TOSCA_OK;
+}
+
+// Krylov workspace: basis (restart + 1) n, tmp n, the drivers' extra vectors, mdot partials,
+// device scalars
+struct KrylovWs { double *V, *tmp, *extra, *partial, *dsc; };
+
+int ensure_kws(tosca_ctx *c, int restart, int nextra, KrylovWs &K) {
+    const size_t n = 3 * (size_t)c->geo.mx * c->geo.my * c->geo.mz;
+    const size_t np = tosca::mdot_partial_doubles(restart + 1), nsc = 2 * (size_t)(restart + 2);
+    const size_t need = (size_t)(restart + 2 + nextra) * n + np + nsc;
+    if (c->kws_elems < need) {
+        if (c->kws) { hipStreamSynchronize(c->stream); hipFree(c->kws); c->kws = nullptr; c->kws_elems = 0; }
+        if (hipMalloc(&c->kws, need * sizeof(double)) != hipSuccess) {
+            (void)hipGetLastError();
+            return set_err(c, TOSCA_ERR_DEVICE, "Krylov workspace allocation of %zu bytes failed", need * sizeof(double));
+        }
+        c->kws_elems = need;
+    }
+    K.V = c->kws;
+    K.tmp = K.V + (size_t)(restart + 1) * n;
+    K.extra = K.tmp + n;
+    K.partial = K.extra + (size_t)nextra * n;
+    K.dsc = K.partial + np;
+    return TOSCA_OK;
+}
+
+// device values -> host, after everything queued so far
+int fetch(tosca_ctx *c, const double *d, double *h, int cnt) {
+    if (hipMemcpyAsync(h, d, cnt * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+        return set_err(c, TOSCA_ERR_DEVICE, "GMRES: device-to-host copy failed: %s", hipGetErrorString(hipGetLastError()));
+    return TOSCA_OK;
+}
+
+// Restarted GMRES as PETSc 3.15 runs it for kspIMEX (src/ksp/ksp/impls/gmres/gmres.c:
+// KSPSolve_GMRES, KSPGMRESCycle with classical Gram-Schmidt, KSPGMRESUpdateHessenberg,
+// KSPGMRESBuildSoln; KSPConvergedDefault with the relative test against the initial residual).
+// Vector work on the device, the (restart + 1) x restart Hessenberg system on the host.
+template <class Op>
+int gmres_device(tosca_ctx *c, long n, Op A, const double *b, double *x, const tosca_krylov_params *kp,
+                 tosca_krylov_result *kr, const KrylovWs &K) {
+    hipStream_t s = c->stream;
+    const int m = kp->restart;
+    std::vector<double> HH((size_t)(m + 2) * (m + 1), 0.0), cc(m + 1), ss(m + 1), grs(m + 2), nrs(m + 1), h(m + 1);
+    auto H = [&](int a, int col) -> double & { return HH[(size_t)col * (m + 2) + a]; };
+    auto V = [&](int j) { return K.V + (size_t)j * n; };
+    int its = 0, reason = 0, rc;
+    double ttol = 0.0, res = 0.0, sq = 0.0;
+    bool first = true;
+    while (!reason) {
+        A(x, K.tmp);                                                        // KSPInitialResidual
+        tosca::launch_residual(V(0), b, K.tmp, n, s);
+        tosca::launch_mdot(V(0), n, 1, V(0), n, K.partial, K.dsc, s);
+        if ((rc = fetch(c, K.dsc, &sq, 1))) return rc;
+        res = std::sqrt(sq);
+        if (res > 0.0) tosca::launch_scale_to(V(0), V(0), 1.0 / res, n, s);
+        grs[0] = res;
+        if (first) { ttol = std::max(kp->rtol * res, kp->atol); first = false; }
+        if (std::isnan(res)) { reason = -4; break; }
+        if (res <= ttol) reason = 1;
+        int it = 0;
+        bool hapend = false;
+        while (!reason && it < m && its < kp->max_it) {
+            double *w = V(it + 1);
+            A(V(it), w);
+            tosca::launch_mdot(K.V, n, it + 1, w, n, K.partial, K.dsc, s);  // h_j = v_j . w
+            if ((rc = fetch(c, K.dsc, h.data(), it + 1))) return rc;
+            tosca::launch_maxpy(w, K.V, n, it + 1, K.dsc, -1.0, n, s);    // w -= sum h_j v_j
+            for (int j = 0; j <= it; j++) H(j, it) = h[j];
+            tosca::launch_mdot(w, n, 1, w, n, K.partial, K.dsc, s);
+            if ((rc = fetch(c, K.dsc, &sq, 1))) return rc;
+            const double tt = std::sqrt(sq);
+            H(it + 1, it) = tt;
+            double hapbnd = std::fabs(tt / grs[it]);
+            if (hapbnd > 1e-30) hapbnd = 1e-30;
+            hapend = tt < hapbnd;
+            if (!hapend) tosca::launch_scale_to(w, w, 1.0 / tt, n, s);
+            for (int j = 1; j <= it; j++) {                                 // previous rotations
+                const double t0 = H(j - 1, it);
+                H(j - 1, it) = cc[j - 1] * t0 + ss[j - 1] * H(j, it);
+                H(j, it) = cc[j - 1] * H(j, it) - ss[j - 1] * t0;
+            }
+            if (!hapend) {
+                const double t1 = std::sqrt(H(it, it) * H(it, it) + H(it + 1, it) * H(it + 1, it));
+                if (t1 == 0.0) { reason = -9; break; }
+                cc[it] = H(it, it) / t1;
+                ss[it] = H(it + 1, it) / t1;
+                grs[it + 1] = -(ss[it] * grs[it]);
+                grs[it] = cc[it] * grs[it];
+                H(it, it) = cc[it] * H(it, it) + ss[it] * H(it + 1, it);
+                res = std::fabs(grs[it + 1]);
+            } else {
+                res = 0.0;
+            }
+            it++;
+            its++;
+            if (std::isnan(res)) reason = -4;
+            else if (res <= ttol) reason = 1;
+            if (hapend) { if (!reason) reason = -5; break; }
+        }
+        const int last = it - 1;                                            // KSPGMRESBuildSoln
+        if (last >= 0) {
+            nrs[last] = (H(last, last) != 0.0) ? grs[last] / H(last, last) : 0.0;
+            for (int ii = 1; ii <= last; ii++) {
+                const int k = last - ii;
+                double tt = grs[k];
+                for (int j = k + 1; j <= last; j++) tt = tt - H(k, j) * nrs[j];
+                nrs[k] = tt / H(k, k);
+            }
+            if (hipMemcpyAsync(K.dsc, nrs.data(), (last + 1) * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess)
+                return set_err(c, TOSCA_ERR_DEVICE, "GMRES: host-to-device copy failed");
+            tosca::launch_fill(K.tmp, n, 0.0, s);
+            tosca::launch_maxpy(K.tmp, K.V, n, last + 1, K.dsc, 1.0, n, s);
+            tosca::launch_axpy(x, 1.0, K.tmp, n, s);
+            if (hipStreamSynchronize(s) != hipSuccess) return set_err(c, TOSCA_ERR_DEVICE, "GMRES: stream error");
+        }
+        if (its >= kp->max_it && !reason) reason = -3;
+    }
+    kr->iterations = its;
+    kr->residual = res;
+    kr->reason = reason;
+    return TOSCA_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -382,7 +560,8 @@ int tosca_ctx_destroy(tosca_ctx *c) {
     hipSetDevice(c->device);
     c->timer.release();
     if (tosca::g_timer == &c->timer) tosca::g_timer = nullptr;
-    if (c->ws || c->chk) hipStreamSynchronize(c->stream);
+    if (c->ws || c->chk || c->kws) hipStreamSynchronize(c->stream);
+    if (c->kws) hipFree(c->kws);
     if (c->ws) hipFree(c->ws);
     if (c->chk) hipFree(c->chk);
     delete c;
@@ -550,43 +729,93 @@ int tosca_imex_matvec(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, 
                       double *ucat, const double *nut, const double *ucont_o, double coef, double *av) {
     int rc = check_ctx(c);
     if (rc) return rc;
-    if ((rc = check_params(c, p))) return rc;
-    if (p->div_scheme != TOSCA_DIV_CENTRAL && p->div_scheme != TOSCA_DIV_QUICK)
-        return set_err(c, TOSCA_ERR_UNSUPPORTED, "div_scheme = %d: only central (0) and QUICK (1) are supported", p->div_scheme);
-    set_geo_per(c, p);
+    if ((rc = check_imex(c, m, p, ucont, ucat, nut, ucont_o))) return rc;
     const Geo &g = c->geo;
-    const size_t n = (size_t)g.nloc(), vloc = 3 * n * sizeof(double), vglob = 3 * (size_t)g.mx * g.my * g.mz * sizeof(double);
-    if ((rc = check_mesh(c, m, true, true, true))) return rc;
-    if ((rc = check_dev(c, v, vglob, "v")) || (rc = check_dev(c, ucont, vloc, "ucont")) ||
-        (rc = check_dev(c, ucat, vloc, "ucat")) || (rc = check_dev(c, ucont_o, vglob, "ucont_o")) ||
-        (rc = check_dev(c, av, vglob, "av")))
-        return rc;
+    const size_t vglob = 3 * (size_t)g.mx * g.my * g.mz * sizeof(double);
+    if ((rc = check_dev(c, v, vglob, "v")) || (rc = check_dev(c, av, vglob, "av"))) return rc;
     if (av == v) return set_err(c, TOSCA_ERR_ARG, "av must not alias v");
-    if (p->les_active) {
-        if ((rc = check_les_model(c, p))) return rc;
-        if ((rc = check_dev(c, nut, n * sizeof(double), "nut"))) return rc;
-    }
-    if ((rc = check_wall_tau(c, p))) return rc;
-    if ((rc = ensure_ws(c, ws_need(c)))) return rc;
-    hipStream_t s = c->stream;
-    const long ng3 = 3L * g.mx * g.my * g.mz;
-    tosca::launch_global_to_local(ucont, v, g, s);                         // :1763-1767
-    tosca::launch_wrap(ucont, 3, g, s);
-    tosca::launch_no_penetration(ucont, p, g, s);                          // :1770
-    tosca::launch_reset_face_periodic(ucont, 3, g, s);                     // :1773
-    tosca::launch_contra_to_cart(ucont, m, ucat, g, s);                    // :1777
-    tosca::launch_reset_cell_periodic(ucat, 3, g, s);                      // :1790
-    tosca::launch_fill(av, ng3, 0.0, s);                                   // :1793
-    const double *nu_t = p->les_active ? nut : nullptr;
-    if (c->ref_path) {                                                     // :1794
-        tosca::ref_formu(g, m, p, ucont, ucat, nu_t, av, -1.0 * coef, 2, c->ws, s);
-    } else {
-        tosca::FormUArgs A = formu_args(m, p, ucont, ucat, nu_t, av, -1.0 * coef, 2);
-        tosca::fused_formu(g, A, c->ws + 3 * (size_t)g.nloc(), s);
-    }
-    tosca::launch_non_resolved(av, ucont_o, p, g, s);                      // :1795
-    tosca::launch_axpy(av, 1.0, v, ng3, s);                                // :1796
+    queue_imex(c, m, p, v, ucont, ucat, nut, ucont_o, coef, av);
     return launched(c, "IMEXMatVec");
+}
+
+int tosca_imex_solve(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const double *b, double *x,
+                     double *ucont, double *ucat, const double *nut, const double *ucont_o, double coef,
+                     const tosca_krylov_params *kp, tosca_krylov_result *kr) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if ((rc = check_krylov(c, kp, kr))) return rc;
+    if ((rc = check_imex(c, m, p, ucont, ucat, nut, ucont_o))) return rc;
+    const Geo &g = c->geo;
+    const long n = 3L * g.mx * g.my * g.mz;
+    if ((rc = check_dev(c, b, n * sizeof(double), "b")) || (rc = check_dev(c, x, n * sizeof(double), "x"))) return rc;
+    if (b == x) return set_err(c, TOSCA_ERR_ARG, "x must not alias b");
+    KrylovWs K;
+    if ((rc = ensure_kws(c, kp->restart, 0, K))) return rc;
+    auto A = [&](const double *v, double *out) { queue_imex(c, m, p, v, ucont, ucat, nut, ucont_o, coef, out); };
+    if ((rc = gmres_device(c, n, A, b, x, kp, kr, K))) return rc;
+    return launched(c, "KSPSolve(IMEX)");
+}
+
+int tosca_ueqn_rk3cn(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, int scheme, double *ucont,
+                     double *ucat, const double *nut, const double *dp, const double *btheta, const double *btheta_o,
+                     int ab2, const double *farm, const double *ucont_o, double dt, double *rhs,
+                     const tosca_krylov_params *kp, tosca_krylov_result *kr) {
+    int rc = check_explicit(c, m, p, ucont, ucat, nut, dp, btheta, btheta_o, ab2, farm, ucont_o, rhs, 1);
+    if (rc) return rc;
+    if (scheme != 0 && scheme != 1) return set_err(c, TOSCA_ERR_ARG, "scheme = %d (0 RK3WCN, 1 RK3SOCN)", scheme);
+    if ((rc = check_krylov(c, kp, kr))) return rc;
+    if ((rc = check_imex(c, m, p, ucont, ucat, nut, ucont_o))) return rc;
+    const Geo &g = c->geo;
+    hipStream_t s = c->stream;
+    const long n = 3L * g.mx * g.my * g.mz;
+    KrylovWs K;
+    if ((rc = ensure_kws(c, kp->restart, 5, K))) return rc;
+    double *rhs_o = K.extra, *visc = K.extra + n, *bU = K.extra + 2 * n, *utmp = K.extra + 3 * n, *ucg = K.extra + 4 * n;
+    const double wb[3] = {8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0}, wg[3] = {4.0 / 15.0, 1.0 / 15.0, 1.0 / 6.0},
+                 wd[3] = {0.0, 17.0 / 60.0, 5.0 / 12.0};                   // ueqn_solvers.c:1518-1520
+    const double sa[3] = {1.0, 3.0 / 4.0, 1.0 / 3.0}, sb[3] = {1.0, 1.0 / 4.0, 2.0 / 3.0},
+                 sg[3] = {1.0 / 2.0, 1.0 / 8.0, 1.0 / 3.0};                 // :1596-1598
+    const size_t vb = n * sizeof(double);
+    const double *nu_t = p->les_active ? nut : nullptr;
+    tosca::launch_fill(rhs_o, n, 0.0, s);                                    // :1526
+    for (int i = 0; i < 3; i++) {
+        queue_explicit(c, m, p, ucont, ucat, nut, dp, btheta, btheta_o, ab2, farm, ucont_o, rhs, 1);
+        tosca::launch_fill(visc, n, 0.0, s);                                 // FormU(RhsVisc, 1, 2)
+        if (c->ref_path) {
+            tosca::ref_formu(g, m, p, ucont, ucat, nu_t, visc, 1.0, 2, c->ws, s);
+        } else {
+            tosca::FormUArgs A = formu_args(m, p, ucont, ucat, nu_t, visc, 1.0, 2);
+            tosca::fused_formu(g, A, c->ws + 3 * (size_t)g.nloc(), s);
+        }
+        tosca::launch_non_resolved(visc, ucont_o, p, g, s);
+        tosca::launch_local_to_global(ucg, ucont, g, s);                     // Ucont
+        double gam;
+        if (scheme == 0) {
+            if (hipMemcpyAsync(bU, ucg, vb, hipMemcpyDeviceToDevice, s) != hipSuccess)
+                return set_err(c, TOSCA_ERR_DEVICE, "RK3WCN: copy failed");
+            tosca::launch_axpy(bU, dt * wb[i], rhs, n, s);
+            tosca::launch_axpy(bU, -1.0 * dt * wd[i], rhs_o, n, s);
+            tosca::launch_axpy(bU, dt * wg[i], visc, n, s);
+            gam = wg[i];
+        } else {
+            tosca::launch_scale_to(bU, ucont_o, sa[i], n, s);
+            if (i > 0) tosca::launch_axpy(bU, 1.0 - sa[i], ucg, n, s);
+            tosca::launch_axpy(bU, dt * sb[i], rhs, n, s);
+            tosca::launch_axpy(bU, dt * sg[i], visc, n, s);
+            gam = sg[i];
+        }
+        if (hipMemcpyAsync(utmp, bU, vb, hipMemcpyDeviceToDevice, s) != hipSuccess)   // explicit-Euler guess
+            return set_err(c, TOSCA_ERR_DEVICE, "RK3CN: copy failed");
+        tosca::launch_axpy(utmp, dt * gam, visc, n, s);
+        const double coef = dt * gam;
+        auto A = [&](const double *v, double *out) { queue_imex(c, m, p, v, ucont, ucat, nut, ucont_o, coef, out); };
+        if ((rc = gmres_device(c, n, A, bU, utmp, kp, kr, K))) return rc;
+        tosca::launch_global_to_local(ucont, utmp, g, s);
+        tosca::launch_wrap(ucont, 3, g, s);
+        if (scheme == 0 && hipMemcpyAsync(rhs_o, rhs, vb, hipMemcpyDeviceToDevice, s) != hipSuccess)
+            return set_err(c, TOSCA_ERR_DEVICE, "RK3WCN: copy failed");
+    }
+    return launched(c, scheme == 0 ? "UeqnRK3CN_W" : "UeqnRK3CN_SO");
 }
 
 int tosca_grad_p(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const double *pres, double *dp) {

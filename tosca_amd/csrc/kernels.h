@@ -34,6 +34,15 @@ void launch_waxpy_local(double *ucont, double al, const double *rhs, const doubl
 void launch_axpy(double *y, double a, const double *x, long n, hipStream_t s);
 void launch_global_to_local(double *loc, const double *glob, const Geo &g, hipStream_t s);
 
+// Krylov vector kernels (krylov.hip)
+void launch_mdot(const double *V, long stride, int nv, const double *w, long n, double *partial, double *out,
+                 hipStream_t s);
+size_t mdot_partial_doubles(int nvmax);
+void launch_maxpy(double *w, const double *V, long stride, int nv, const double *h, double sign, long n, hipStream_t s);
+void launch_scale_to(double *y, const double *x, double a, long n, hipStream_t s);
+void launch_residual(double *r, const double *b, const double *t, long n, hipStream_t s);
+void launch_local_to_global(double *glob, const double *loc, const Geo &g, hipStream_t s);
+
 // boundary-condition ghost fills (bc_fill.hip)
 void launch_contravariant_bcs(double *ucont, const double *ucat, const tosca_mesh *m, const tosca_params *p,
                               const Geo &g, hipStream_t s);

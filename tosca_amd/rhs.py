@@ -197,6 +197,27 @@ class RhsContext:
         self._check(self.lib.tosca_imex_matvec(self.h, C.byref(m), C.byref(p), _ptr(v), _ptr(ucont), _ptr(ucat),
                                                _ptr(nut), _ptr(ucont_o), float(coef), _ptr(av)))
 
+    def IMEXSolve(self, mesh, params, b, x, ucont, ucat, nut, ucont_o, coef, **krylov):
+        """KSPSolve(kspIMEX, b, x) with IMEXMatVec: restarted GMRES on the device; x is updated in
+        place.  krylov: restart, max_it, rtol, atol (the reference's defaults otherwise).
+        Returns (iterations, residual, reason)."""
+        m, p = mesh_struct(mesh), abi.make_params(params)
+        kp, kr = abi.make_krylov_params(**krylov), abi.KrylovResult()
+        self._check(self.lib.tosca_imex_solve(self.h, C.byref(m), C.byref(p), _ptr(b), _ptr(x), _ptr(ucont), _ptr(ucat),
+                                              _ptr(nut), _ptr(ucont_o), float(coef), C.byref(kp), C.byref(kr)))
+        return kr.iterations, kr.residual, kr.reason
+
+    def UeqnRK3CN(self, mesh, params, scheme, ucont, ucat, nut, dp, btheta, btheta_o, ab2, farm, ucont_o, dt, rhs,
+                  **krylov):
+        """UeqnRK3CN_W (scheme 0) / UeqnRK3CN_SO (scheme 1, src/solvers/ueqn_solvers.c:1508-1661);
+        returns the last stage's (iterations, residual, reason)."""
+        m, p = mesh_struct(mesh), abi.make_params(params)
+        kp, kr = abi.make_krylov_params(**krylov), abi.KrylovResult()
+        self._check(self.lib.tosca_ueqn_rk3cn(self.h, C.byref(m), C.byref(p), int(scheme), _ptr(ucont), _ptr(ucat),
+                                              _ptr(nut), _ptr(dp), _ptr(btheta), _ptr(btheta_o), int(ab2), _ptr(farm),
+                                              _ptr(ucont_o), float(dt), _ptr(rhs), C.byref(kp), C.byref(kr)))
+        return kr.iterations, kr.residual, kr.reason
+
     def FormT(self, mesh, params, tparams, ucont, theta, nut, kt, rhs, scale: float = 1.0, form_mode: int = 0):
         """FormT (src/solvers/teqn_solvers.c:6-592): rhs (global scalar) at interior cells.
         tparams: dict(pr=..., wall_t=(iLeft, iRight, jLeft, jRight) thetaWallFunction flags,
