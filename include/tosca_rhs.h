@@ -288,6 +288,19 @@ int tosca_ueqn_rk3cn(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_params 
                      const double *ucont_o, double dt, double *rhs, const tosca_krylov_params *kp,
                      tosca_krylov_result *kr);
 
+/* Replaces UeqnABCN (src/solvers/ueqn_solvers.c:1665-1737): Adams-Bashforth 2 convection and
+ * Crank-Nicolson diffusion in one IMEX solve.  ab2 is clock->it > clock->itStart: it selects the
+ * AB2 bTheta terms of ExplicitRhsU, the AB2 convection with conv_o and the Crank-Nicolson half
+ * (coef = dt / 2); on the first step (ab2 == 0) the viscous term is fully implicit (coef = dt) and
+ * the reference adds Rhs to bU = Rhs a second time (:1694-1695), which is reproduced.
+ * conv_o: RhsConv_o (global AoS), read and replaced by this step's convective term.  Other
+ * arguments as tosca_ueqn_rk3cn. */
+int tosca_ueqn_abcn(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_params *params,
+                    double *ucont, double *ucat, const double *nut, const double *dp,
+                    const double *btheta, const double *btheta_o, int ab2, const double *farm,
+                    const double *ucont_o, double dt, double *rhs, double *conv_o,
+                    const tosca_krylov_params *kp, tosca_krylov_result *kr);
+
 /* Replaces GradP (src/peqn.c:3437-3773): the contravariant pressure-gradient fluxes
  * dP^q = (g^q1 dp/dxi + g^q2 dp/deta + g^q3 dp/dzeta) aj_q at the faces 1..m-2 of the interior
  * cells, with the reference's one-sided transverse differences on the corner face rows and its

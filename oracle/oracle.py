@@ -57,6 +57,7 @@ def load() -> C.CDLL:
         "oracle_gmres_dense": (I, [C.c_int64, P, P, P, I, D, D, I, P, P, P]),
         "oracle_imex_solve": (I, [In, M, Pa, P, P, P, P, P, P, D, I, D, D, I, P, P, P]),
         "oracle_ueqn_rk3cn": (I, [In, M, Pa, I, P, P, P, P, P, P, I, P, P, D, P, I, D, D, I, P, P, P]),
+        "oracle_ueqn_abcn": (I, [In, M, Pa, P, P, P, P, P, P, I, P, P, D, P, P, I, D, D, I, P, P, P]),
     }
     for n, (r, a) in sigs.items():
         f = getattr(lib, n)
@@ -269,4 +270,17 @@ def ueqn_rk3cn(info, mesh, params, scheme, ucont, ucat, nut, dp, btheta, btheta_
                                  _p(dp), _p(btheta), _p(btheta_o), int(ab2), _p(farm), _p(ucont_o), float(dt),
                                  _p(rhs), int(restart), float(rtol), float(atol), int(maxit), C.byref(its),
                                  C.byref(res), C.byref(conv)))
+    return its.value, res.value, conv.value
+
+
+def ueqn_abcn(info, mesh, params, ucont, ucat, nut, dp, btheta, btheta_o, ab2, farm, ucont_o, dt, rhs, conv_o,
+              restart=30, rtol=1e-30, atol=1e-5, maxit=1000):
+    """UeqnABCN restated; conv_o (RhsConv_o, global) is read and replaced; returns (its, res, reason)."""
+    lib = load()
+    inf, m, p = abi.make_info(info), _mesh(mesh), abi.make_params(params)
+    its, res, conv = _kout()
+    _check(lib.oracle_ueqn_abcn(C.byref(inf), C.byref(m), C.byref(p), _p(ucont), _p(ucat), _p(nut), _p(dp),
+                                _p(btheta), _p(btheta_o), int(ab2), _p(farm), _p(ucont_o), float(dt), _p(rhs),
+                                _p(conv_o), int(restart), float(rtol), float(atol), int(maxit), C.byref(its),
+                                C.byref(res), C.byref(conv)))
     return its.value, res.value, conv.value

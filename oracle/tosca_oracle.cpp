@@ -1484,6 +1484,54 @@ int oracle_ueqn_rk3cn(const tosca_dmda_info *info, const tosca_mesh *m, const to
     }
     return TOSCA_OK;
 }
+
+// UeqnABCN (ueqn_solvers.c:1665-1737): Adams-Bashforth convection, Crank-Nicolson diffusion.
+// `ab2` is clock->it > clock->itStart, for the bTheta terms of ExplicitRhsU and for the scheme;
+// conv_o (RhsConv_o, global) carries the convective term from the previous step and is replaced by
+// this step's.  On the first step the reference adds Rhs to bU = Rhs a second time (:1694-1695);
+// that is reproduced.
+int oracle_ueqn_abcn(const tosca_dmda_info *info, const tosca_mesh *m, const tosca_params *p, double *ucont_,
+                     double *ucat_, const double *nut, const double *dp, const double *btheta, const double *btheta_o,
+                     int ab2, const double *farm, const double *ucont_o_, double dt, double *rhs_, double *conv_o_,
+                     int restart, double rtol, double atol, int maxit, int *its, double *res, int *conv) {
+    Lay l;
+    int rc = check_layout(info, p, l);
+    if (rc) return rc;
+    if (!conv_o_) { g_err = "ueqn_abcn: missing conv_o"; return TOSCA_ERR_ARG; }
+    const int64_t ng = l.mx * l.my * l.mz;
+    V3 *rhs = (V3 *)rhs_, *conv_o = (V3 *)conv_o_;
+    const V3 *uo = (const V3 *)ucont_o_;
+    std::vector<V3> visc(ng, V3{0, 0, 0}), cnv(ng, V3{0, 0, 0}), bU(ng), utmp(ng);
+    rc = oracle_explicit_rhs_u(info, m, p, ucont_, ucat_, nut, dp, btheta, btheta_o, ab2, farm, ucont_o_, rhs_, 1);
+    if (rc) return rc;                                                   // :1675
+    if ((rc = oracle_formu(info, m, p, ucont_, ucat_, nut, (double *)visc.data(), 1.0, 2))) return rc;   // :1678
+    if ((rc = oracle_formu(info, m, p, ucont_, ucat_, nut, (double *)cnv.data(), 1.0, 1))) return rc;    // :1679
+    reset_nonresolved(l, p, visc.data(), uo);                           // :1680-1681
+    reset_nonresolved(l, p, cnv.data(), uo);
+    bU.assign(rhs, rhs + ng);                                           // :1684
+    if (ab2) {
+        axpy(ng, 0.5, cnv.data(), bU.data());                           // :1690
+        axpy(ng, -0.5, conv_o, bU.data());                              // :1691
+    } else {
+        axpy(ng, 1.0, rhs, bU.data());                                  // :1695
+    }
+    if (ab2) axpy(ng, 0.5, visc.data(), bU.data());                     // :1699-1700
+    for (int64_t g = 0; g < ng; g++) {                                  // :1703 VecScale
+        bU[g].x = dt * bU[g].x; bU[g].y = dt * bU[g].y; bU[g].z = dt * bU[g].z;
+    }
+    axpy(ng, 1.0, uo, bU.data());                                       // :1706
+    for (int64_t g = 0; g < ng; g++) utmp[g] = V3{0, 0, 0};            // :1709-1713
+    if (ab2) axpy(ng, 0.5 * dt, visc.data(), utmp.data());
+    else axpy(ng, dt, visc.data(), utmp.data());
+    axpy(ng, 1.0, bU.data(), utmp.data());
+    const double coef = dt * (ab2 ? 0.5 : 1.0);                         // IMEXMatVec's scale (:1751-1759)
+    rc = oracle_imex_solve(info, m, p, (const double *)bU.data(), (double *)utmp.data(), ucont_, ucat_, nut,
+                           ucont_o_, coef, restart, rtol, atol, maxit, its, res, conv);
+    if (rc) return rc;
+    global_vec_to_local(l, utmp.data(), (V3 *)ucont_);                  // :1723-1725
+    for (int64_t g = 0; g < ng; g++) conv_o[g] = cnv[g];                // :1730
+    return TOSCA_OK;
+}
 #endif  // ORACLE_TANGENT
 
 #ifndef ORACLE_TANGENT

@@ -1,5 +1,5 @@
 """GPU parity of the IMEX solve (tosca_imex_solve: restarted GMRES on the device with IMEXMatVec as
-the operator) and of the IMEX Runge-Kutta predictors (tosca_ueqn_rk3cn, RK3WCN and RK3SOCN)
+the operator) and of the IMEX predictors (tosca_ueqn_rk3cn: RK3WCN and RK3SOCN; tosca_ueqn_abcn)
 against the CPU oracle's restatements, on both kernel paths.
 
 The GPU sums its dot products in another (fixed) association than the CPU, so the Krylov iterates
@@ -122,6 +122,38 @@ def test_rk3cn(rhs_mod, periodic, atol, scheme):
     assert r_g > 0 and r_o > 0
     assert its_g == its_o
     assert e <= 1e-9 and er <= 1e-9
+    ctx.close()
+
+
+@pytest.mark.parametrize("periodic", [(0, 0, 0), (2, 0, 1), (1, 1, 1)])
+def test_abcn_two_steps(rhs_mod, periodic):
+    """UeqnABCN on a first step (ab2 = 0) and a second one (ab2 = 1) that uses the first's
+    RhsConv_o"""
+    seed = 70 + PERS.index(periodic)
+    lay, nm, f, params, rng, glob, tau_i, tau_j = _setup(periodic, seed)
+    print(f"\nABCN periodic={periodic} bc_u={params['bc_u']}", flush=True)
+    src = {k: 0.1 * rng.standard_normal(glob) for k in ("dp", "bt", "bto", "farm")}
+    uo = C.owned(f["ucont"], lay).numpy().copy()
+    nut = to_np(f["nut"])
+    dt = 0.05
+    po = dict(params, tau_wall_i=tau_i.ctypes.data, tau_wall_j=tau_j.ctypes.data)
+    uc_o, ua_o, rhs_o, cv_o = to_np(f["ucont"]), to_np(f["ucat"]), np.zeros(glob), np.zeros(glob)
+    ctx = rhs_mod.RhsContext(lay.info())
+    dm = {k: to_dev(v) for k, v in nm.items()}
+    ti, tj = to_dev(tau_i), to_dev(tau_j)
+    pg = dict(params, tau_wall_i=ti.data_ptr(), tau_wall_j=tj.data_ptr())
+    uc_g, ua_g, rhs_g, cv_g = (to_dev(to_np(f["ucont"])), to_dev(to_np(f["ucat"])), to_dev(np.zeros(glob)),
+                               to_dev(np.zeros(glob)))
+    for ab2 in (0, 1):
+        its_o, _, r_o = O.ueqn_abcn(lay.info(), nm, po, uc_o, ua_o, nut, src["dp"], src["bt"], src["bto"], ab2,
+                                    src["farm"], uo, dt, rhs_o, cv_o)
+        its_g, _, r_g = ctx.UeqnABCN(dm, pg, uc_g, ua_g, to_dev(nut), to_dev(src["dp"]), to_dev(src["bt"]),
+                                     to_dev(src["bto"]), ab2, to_dev(src["farm"]), to_dev(uo), dt, rhs_g, cv_g)
+        torch.cuda.synchronize()
+        e, ec = relerr(to_np(uc_g), uc_o), relerr(to_np(cv_g), cv_o)
+        print(f"  ab2={ab2}: its gpu {its_g} oracle {its_o}; ucont relL2 {e:.2e}; RhsConv relL2 {ec:.2e}", flush=True)
+        assert r_g > 0 and r_o > 0 and its_g == its_o
+        assert e <= 1e-9 and ec <= 1e-9
     ctx.close()
 
 

@@ -1,6 +1,6 @@
 """Pins of the oracle's GMRES restatement (PETSc 3.15 KSPGMRES as kspIMEX runs it) and of the IMEX
-Runge-Kutta drivers built on it (oracle_gmres_dense, oracle_imex_solve, oracle_ueqn_rk3cn;
-src/ueqn.c:217-282, src/solvers/ueqn_solvers.c:1508-1661), from facts outside their code:
+predictors built on it (oracle_gmres_dense, oracle_imex_solve, oracle_ueqn_rk3cn, oracle_ueqn_abcn;
+src/ueqn.c:217-282, src/solvers/ueqn_solvers.c:1508-1737), from facts outside their code:
 
   - Krylov theory: on a diagonalisable matrix with k distinct eigenvalues, full GMRES reaches the
     exact solution in at most k iterations; with b an eigenvector it stops after one (happy
@@ -12,7 +12,8 @@ src/ueqn.c:217-282, src/solvers/ueqn_solvers.c:1508-1661), from facts outside th
     IMEX operator;
   - both IMEX Runge-Kutta schemes are consistent: on a periodic free stream with uniform sources
     (no convection or diffusion), three stages advance U0 by exactly dt times the source, which
-    requires the stage weights to sum to one.
+    requires the stage weights to sum to one; ABCN likewise, by 2 dt times it on a first step
+    (the reference's doubled explicit term).
 """
 import numpy as np
 import pytest
@@ -144,3 +145,34 @@ def test_rk3cn_consistency_on_free_stream(scheme, periodic):
     got = V[G:G + lay.mz, G:G + lay.my, G:G + lay.mx]
     sl = (slice(1, lay.mz - 1), slice(1, lay.my - 1), slice(1, lay.mx - 1))
     assert np.abs(got[sl] - want[sl]).max() <= 1e-12 * np.abs(want).max()
+
+
+@pytest.mark.parametrize("ab2", [0, 1])
+def test_abcn_on_free_stream(ab2):
+    """ABCN on a periodic free stream with a uniform source S.f: no convection, no diffusion, so
+    one step gives U0 + dt f, except on the first step (ab2 == 0), where the reference adds the
+    explicit right-hand side twice (ueqn_solvers.c:1694-1695): U0 + 2 dt f"""
+    periodic = (2, 1, 2)
+    lay, mesh_t = C.mesh_metrics(affine_nodes((6, 7, 5), AFFINE["sheared"]), periodic)
+    mesh = np_mesh(mesh_t)
+    bcs = [C.BC_PERIODIC] * 6
+    params = C.default_params(periodic, bcs, bcs, div_scheme=C.DIV_CENTRAL, nu=0.05, les_active=0)
+    U0, f = np.array([0.6, 1.1, -0.4]), np.array([-0.2, 0.3, 0.5])
+    glob = (lay.mz, lay.my, lay.mx, 3)
+    u0 = _free_stream(lay, mesh, U0)
+    loc = torch.zeros(lay.gshape + (3,), dtype=torch.float64)
+    C.owned(loc, lay)[...] = torch.as_tensor(u0)
+    V = C.local_to_local(loc, lay, periodic).numpy().copy()
+    dp = -_free_stream(lay, mesh, f)
+    dt = 0.2
+    conv_o = np.zeros(glob)
+    rhs = np.zeros(glob)
+    bt = np.zeros(glob) if ab2 else None
+    its, res, reason = O.ueqn_abcn(lay.info(), mesh, params, V, np.zeros(lay.gshape + (3,)), np.zeros(lay.gshape), dp,
+                                   bt, bt, ab2, None, u0, dt, rhs, conv_o, atol=1e-12)
+    assert reason > 0
+    want = _free_stream(lay, mesh, U0 + (1 if ab2 else 2) * dt * f)
+    got = V[G:G + lay.mz, G:G + lay.my, G:G + lay.mx]
+    sl = (slice(1, lay.mz - 1), slice(1, lay.my - 1), slice(1, lay.mx - 1))
+    assert np.abs(got[sl] - want[sl]).max() <= 1e-12 * np.abs(want).max()
+    assert np.abs(conv_o[sl]).max() <= 1e-12 * np.abs(u0).max()     # the stored convection of a free stream

@@ -128,6 +128,8 @@ SIGNATURES = {
                              C.POINTER(KrylovResult)]),
     "tosca_ueqn_rk3cn": (I, [P, C.POINTER(Mesh), C.POINTER(Params), I, P, P, P, P, P, P, I, P, P, D, P,
                              C.POINTER(KrylovParams), C.POINTER(KrylovResult)]),
+    "tosca_ueqn_abcn": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P, P, P, P, I, P, P, D, P, P,
+                            C.POINTER(KrylovParams), C.POINTER(KrylovResult)]),
 }
 
 _lib: Optional[C.CDLL] = None

@@ -818,6 +818,60 @@ int tosca_ueqn_rk3cn(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, i
     return launched(c, scheme == 0 ? "UeqnRK3CN_W" : "UeqnRK3CN_SO");
 }
 
+int tosca_ueqn_abcn(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, double *ucont, double *ucat,
+                    const double *nut, const double *dp, const double *btheta, const double *btheta_o, int ab2,
+                    const double *farm, const double *ucont_o, double dt, double *rhs, double *conv_o,
+                    const tosca_krylov_params *kp, tosca_krylov_result *kr) {
+    int rc = check_explicit(c, m, p, ucont, ucat, nut, dp, btheta, btheta_o, ab2, farm, ucont_o, rhs, 1);
+    if (rc) return rc;
+    if ((rc = check_krylov(c, kp, kr))) return rc;
+    if ((rc = check_imex(c, m, p, ucont, ucat, nut, ucont_o))) return rc;
+    const Geo &g = c->geo;
+    hipStream_t s = c->stream;
+    const long n = 3L * g.mx * g.my * g.mz;
+    if ((rc = check_dev(c, conv_o, n * sizeof(double), "conv_o"))) return rc;
+    if (conv_o == rhs || conv_o == ucont_o) return set_err(c, TOSCA_ERR_ARG, "conv_o must not alias rhs or ucont_o");
+    KrylovWs K;
+    if ((rc = ensure_kws(c, kp->restart, 4, K))) return rc;
+    double *visc = K.extra, *cnv = K.extra + n, *bU = K.extra + 2 * n, *utmp = K.extra + 3 * n;
+    const double *nu_t = p->les_active ? nut : nullptr;
+    queue_explicit(c, m, p, ucont, ucat, nut, dp, btheta, btheta_o, ab2, farm, ucont_o, rhs, 1);   // :1675
+    for (int mode = 2; mode >= 1; mode--) {                                  // :1678-1681
+        double *out = mode == 2 ? visc : cnv;
+        tosca::launch_fill(out, n, 0.0, s);
+        if (c->ref_path) {
+            tosca::ref_formu(g, m, p, ucont, ucat, nu_t, out, 1.0, mode, c->ws, s);
+        } else {
+            tosca::FormUArgs A = formu_args(m, p, ucont, ucat, nu_t, out, 1.0, mode);
+            tosca::fused_formu(g, A, c->ws + 3 * (size_t)g.nloc(), s);
+        }
+        tosca::launch_non_resolved(out, ucont_o, p, g, s);
+    }
+    const size_t vb = n * sizeof(double);
+    if (hipMemcpyAsync(bU, rhs, vb, hipMemcpyDeviceToDevice, s) != hipSuccess)   // :1684
+        return set_err(c, TOSCA_ERR_DEVICE, "ABCN: copy failed");
+    if (ab2) {
+        tosca::launch_axpy(bU, 0.5, cnv, n, s);                               // :1690-1691
+        tosca::launch_axpy(bU, -0.5, conv_o, n, s);
+        tosca::launch_axpy(bU, 0.5, visc, n, s);                              // :1699-1700
+    } else {
+        tosca::launch_axpy(bU, 1.0, rhs, n, s);                               // :1695
+    }
+    tosca::launch_scale_to(bU, bU, dt, n, s);                                 // :1703
+    tosca::launch_axpy(bU, 1.0, ucont_o, n, s);                               // :1706
+    tosca::launch_fill(utmp, n, 0.0, s);                                      // :1709-1713
+    tosca::launch_axpy(utmp, ab2 ? 0.5 * dt : dt, visc, n, s);
+    tosca::launch_axpy(utmp, 1.0, bU, n, s);
+    const double coef = dt * (ab2 ? 0.5 : 1.0);
+    auto A = [&](const double *v, double *out) { queue_imex(c, m, p, v, ucont, ucat, nut, ucont_o, coef, out); };
+    if ((rc = gmres_device(c, n, A, bU, utmp, kp, kr, K))) return rc;
+    tosca::launch_global_to_local(ucont, utmp, g, s);                         // :1723-1725
+    tosca::launch_wrap(ucont, 3, g, s);
+    if (hipMemcpyAsync(conv_o, cnv, vb, hipMemcpyDeviceToDevice, s) != hipSuccess)   // :1730
+        return set_err(c, TOSCA_ERR_DEVICE, "ABCN: copy failed");
+    return launched(c, "UeqnABCN");
+}
+
 int tosca_grad_p(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const double *pres, double *dp) {
     int rc = check_ctx(c);
     if (rc) return rc;

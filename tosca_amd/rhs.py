@@ -218,6 +218,18 @@ class RhsContext:
                                               _ptr(ucont_o), float(dt), _ptr(rhs), C.byref(kp), C.byref(kr)))
         return kr.iterations, kr.residual, kr.reason
 
+    def UeqnABCN(self, mesh, params, ucont, ucat, nut, dp, btheta, btheta_o, ab2, farm, ucont_o, dt, rhs, conv_o,
+                 **krylov):
+        """UeqnABCN (src/solvers/ueqn_solvers.c:1665-1737); conv_o carries RhsConv_o between steps.
+        Returns (iterations, residual, reason) of the solve."""
+        m, p = mesh_struct(mesh), abi.make_params(params)
+        kp, kr = abi.make_krylov_params(**krylov), abi.KrylovResult()
+        self._check(self.lib.tosca_ueqn_abcn(self.h, C.byref(m), C.byref(p), _ptr(ucont), _ptr(ucat), _ptr(nut),
+                                             _ptr(dp), _ptr(btheta), _ptr(btheta_o), int(ab2), _ptr(farm),
+                                             _ptr(ucont_o), float(dt), _ptr(rhs), _ptr(conv_o), C.byref(kp),
+                                             C.byref(kr)))
+        return kr.iterations, kr.residual, kr.reason
+
     def FormT(self, mesh, params, tparams, ucont, theta, nut, kt, rhs, scale: float = 1.0, form_mode: int = 0):
         """FormT (src/solvers/teqn_solvers.c:6-592): rhs (global scalar) at interior cells.
         tparams: dict(pr=..., wall_t=(iLeft, iRight, jLeft, jRight) thetaWallFunction flags,
