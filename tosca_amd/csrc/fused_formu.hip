@@ -3,7 +3,7 @@
 // The reference evaluates FormU in three sweeps over HBM (face fluxes into six workspace vectors,
 // their divergence into lFp, the face-dot into Rhs; src/solvers/ueqn_solvers.c:173-1035) plus a
 // separate Buoyancy sweep (src/sources/ueqn_sources.c:2726-2880).  Here one workgroup owns a
-// 62 x (NR-2) tile of (i, j) cells and marches along k; per plane it
+// 62 x (NR-1) tile of (i, j) cells and marches along k; per plane it
 //   1. computes the i-, j- and k-face fluxes (convective + viscous) of its columns,
 //   2. takes the i-face flux of the left neighbour from the lane below (cross-lane shuffle: a
 //      wave is one 64-wide row of i), the j-face flux of the row below from LDS and the k-face
@@ -183,6 +183,7 @@ struct MainArgs {
     const T *theta;
     double *btheta;
     const double *bF1, *bF2, *bF3, *sI, *sJ, *sK;
+    double *sv1, *sv2;     // j-tile seams: the two halves of the last output row's j-face dot
     double scale, nu, bscale, tRef;
     int mode, quick, les;
 };
@@ -296,7 +297,7 @@ __global__ void __launch_bounds__(64 * NR) k_fused_formu(MainArgs<T, MT> A, Geo 
     // useful range work on clamped, in-bounds addresses and their results are discarded), so the
     // hot loop has no divergent branches; only the stores and the rare boundary-buffer loads are
     // predicated.
-    constexpr int TI = 62, TJ = NR - 2;
+    constexpr int TI = 62, TJ = NR - 1;
     __shared__ T sF2[6][NR][64];          // j-face fluxes of the plane (div xyz, visc xyz)
     __shared__ double sFp[2][3][NR][64];  // fp of the previous plane (double-buffered)
     __shared__ Ring<NR, T> ring;          // ucat planes t-1 .. t+2
@@ -320,12 +321,19 @@ __global__ void __launch_bounds__(64 * NR) k_fused_formu(MainArgs<T, MT> A, Geo 
     const bool seamI = colJ && i == g.mx - 1 && iper;       // ghost m-1 carries fp of cell 1
     const bool seamJ = colI && j == g.my - 1 && jper;
     const bool out = cell && lane <= TI && r <= TJ;
+    // The last output row has no row above it in the tile: the j-face dot of its Rhs is split at
+    // the seam.  It keeps the half of its own cell (v1) in sv1 and the next tile's first output
+    // row keeps the half of its cell (the same v2 this row would form) in sv2; k_formu_jseam adds
+    // both.  The domain's last interior row completes its dot here (own fp, or the periodic seam).
+    const bool seamRow = r == NR - 1 && j < g.my - 2;
+    const bool seamNext = r == 1 && bj > 0;
     // faces this thread evaluates: the halo row 0 only supplies the j face of row 1 and the halo
     // lane 0 only the i face of lane 1; every other thread needs its three faces for fp
     const bool needF1 = r >= 1, needF2 = lane >= 1, needF3 = r >= 1 && lane >= 1;
     const bool ownI = (i == g.mx - 2) && !iper, ownJ = (j == g.my - 2) && !jper;
     // clamped column for addressing
     const int ia = min(i, g.mx - 1), ja = min(j, g.my - 1);
+    const long svo = ((long)bj * g.mz) * g.mx + ia, svn = ((long)(bj - 1) * g.mz) * g.mx + ia;
     const int SJ = g.gxm, SK = g.gxm * g.gym;
     const int c0 = g.L(0, ja, ia);
     ColPtrs<T, MT> P;
@@ -460,6 +468,10 @@ __global__ void __launch_bounds__(64 * NR) k_fused_formu(MainArgs<T, MT> A, Geo 
         V3 fJ{sFp[buf ^ 1][0][rp][lane], sFp[buf ^ 1][1][rp][lane], sFp[buf ^ 1][2][rp][lane]};
         if (ownI) fpI = fpPrev;
         if (ownJ) fJ = fpPrev;
+        else if (r == NR - 1 && jper && j == g.my - 2) {
+            const double *qq = pSJ + (long)(t - 1) * g.mx * 3;
+            fJ = V3{qq[0], qq[1], qq[2]};
+        }
         if (rhsStep) {
             const V3 fK = fp;
             double v1 = csA.x * fpPrev.x + csA.y * fpPrev.y + csA.z * fpPrev.z;
@@ -468,11 +480,13 @@ __global__ void __launch_bounds__(64 * NR) k_fused_formu(MainArgs<T, MT> A, Geo 
             v1 = etA.x * fpPrev.x + etA.y * fpPrev.y + etA.z * fpPrev.z;
             v2 = etB.x * fJ.x + etB.y * fJ.y + etB.z * fJ.z;
             const double ry = A.scale * fa1p * (0.5 * (v1 + v2));
+            if (seamRow) A.sv1[svo + (long)(t - 1) * g.mx] = v1;
+            if (seamNext) A.sv2[svn + (long)(t - 1) * g.mx] = v1;
             v1 = zeA.x * fpPrev.x + zeA.y * fpPrev.y + zeA.z * fpPrev.z;
             v2 = zeB.x * fK.x + zeB.y * fK.y + zeB.z * fK.z;
             const double rz = A.scale * fa2p * (0.5 * (v1 + v2));
             prhs[ro] = r0 + rx;
-            prhs[ro + 1] = r1 + ry;
+            if (!seamRow) prhs[ro + 1] = r1 + ry;
             prhs[ro + 2] = r2 + rz;
             if (buoy) {
                 // Boussinesq source on the three + faces of the cell (ueqn_sources.c:2812-2870):
@@ -494,16 +508,34 @@ __global__ void __launch_bounds__(64 * NR) k_fused_formu(MainArgs<T, MT> A, Geo 
     }
 }
 
+// Rhs.y of the j-tile seam rows: rhs += scale aj_j (0.5 (v1 + v2)) with the two halves the main
+// kernel kept (the same expression, so the same value, as the other rows' in-kernel update)
+template <class MT>
+__global__ void k_formu_jseam(double *rhs, const double *sv1, const double *sv2, const MT *jaj, double scale, Geo g,
+                              int tj) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y, sidx = blockIdx.z;
+    const int jA = (sidx + 1) * tj;                 // last output row of j tile sidx
+    if (i < 1 || i > g.mx - 2 || k < 1 || k > g.mz - 2 || jA >= g.my - 2) return;
+    const long o = ((long)sidx * g.mz + k) * g.mx + i;
+    const double fa = jaj[g.L(k, jA, i)];
+    double *pr = rhs + 3 * (long)g.G(k, jA, i) + 1;
+    *pr = *pr + scale * fa * (0.5 * (sv1[o] + sv2[o]));
+}
+
+constexpr int FORMU_NR = 8;      // rows of the main kernel's workgroup (one wave each)
+
 template <class T, class MT>
 void fused_formu(const Geo &g, const FormUArgsT<T, MT> &A, double *ws, hipStream_t s) {
-    constexpr int NR = 8;
-    const int TI = 62, TJ = NR - 2;
+    constexpr int NR = FORMU_NR;
+    const int TI = 62, TJ = NR - 1;
     // boundary faces and seams first
     BoundaryBufs B;
     long off = 0;
     int n1, n2;
     for (int d = 0; d < 3; d++) { const long n = tr_dims(g, d, n1, n2); B.bF[d] = ws + off; off += 2 * n * 6; }
     for (int d = 0; d < 3; d++) { const long n = tr_dims(g, d, n1, n2); B.seam[d] = ws + off; off += n * 3; }
+    const int nsj = (g.my - 2 + TJ - 1) / TJ;                 // j tiles; the last one has no seam
+    double *sv1 = ws + off, *sv2 = ws + off + (long)nsj * g.mz * g.mx;
     const int nmax = max(g.my * g.mz, max(g.mx * g.mz, g.mx * g.my));
     TOSCA_LAUNCH(sizeof(T) == 8 ? "k_formu_boundary" : sizeof(MT) == 8 ? "k_formu_boundary_f32" : "k_formu_boundary_f32m", (k_formu_boundary<T, MT>), dim3((nmax + 127) / 128, 1, 9), dim3(128), 0, s, A, g, B);
 
@@ -518,6 +550,7 @@ void fused_formu(const Geo &g, const FormUArgsT<T, MT> &A, double *ws, hipStream
     M.theta = A.btheta ? A.theta : A.ucat;          // any valid local array when Buoyancy is off
     M.btheta = A.btheta ? A.btheta : A.rhs;
     M.bF1 = B.bF[0]; M.bF2 = B.bF[1]; M.bF3 = B.bF[2]; M.sI = B.seam[0]; M.sJ = B.seam[1]; M.sK = B.seam[2];
+    M.sv1 = sv1; M.sv2 = sv2;
     M.scale = A.scale; M.nu = A.nu; M.bscale = A.bscale; M.tRef = A.btheta ? A.tRef : 1.0;
     M.mode = A.mode; M.quick = A.quick; M.les = A.les;
 
@@ -529,6 +562,9 @@ void fused_formu(const Geo &g, const FormUArgsT<T, MT> &A, double *ws, hipStream
     chunks = (nk + kchunk - 1) / kchunk;
     dim3 grid((g.mx - 2 + TI - 1) / TI, (g.my - 2 + TJ - 1) / TJ, chunks);
     TOSCA_LAUNCH(sizeof(T) == 8 ? "k_fused_formu" : sizeof(MT) == 8 ? "k_fused_formu_f32" : "k_fused_formu_f32m", (k_fused_formu<NR, T, MT>), grid, dim3(64, NR), 0, s, M, g, kchunk);
+    if (nsj > 1)
+        TOSCA_LAUNCH("k_formu_jseam", k_formu_jseam<MT>, dim3((g.mx + 127) / 128, g.mz, nsj - 1), dim3(128), 0, s,
+                     A.rhs, sv1, sv2, A.jaj, A.scale, g, TJ);
     // the reference zeroes all of bTheta; the kernel writes the interior cells, this the rest
     if (A.btheta) zero_shell(A.btheta, 3, g.mx, g.my, g.mz, 0, 0, s);
 }
@@ -541,6 +577,9 @@ size_t formu_ws_doubles(const Geo &g) {
     long tot = 0;
     int n1, n2;
     for (int d = 0; d < 3; d++) tot += (long)tr_dims(g, d, n1, n2) * (12 + 3);
+    const int TJ = FORMU_NR - 1;                              // j tiles of the main kernel
+    const int nsj = (g.my - 2 + TJ - 1) / TJ;
+    tot += 2L * nsj * g.mz * g.mx;
     return (size_t)tot;
 }
 
