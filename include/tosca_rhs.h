@@ -175,6 +175,18 @@ int tosca_rhs_les_step(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_param
                        double *rhs, double scale, double *cs, double *nut,
                        double *btheta, double buoy_scale);
 
+/* The same step with host-resident flow fields (the staged residency mode): ucont, ucat, theta
+ * (local), rhs (global, accumulated), cs, nut (local) and btheta (global) are host arrays of the
+ * same layouts; the mesh and params arrays stay on the device.  The context copies the inputs
+ * into device buffers it owns (allocated on first use), runs tosca_rhs_les_step and copies the
+ * outputs back before returning (synchronous).  Pinned host memory (hipHostMalloc) moves at the
+ * link's full rate; pageable memory works but is slower.  Host sizes cannot be checked: they must
+ * match the layout.  times_ms (may be NULL) receives {copy ms (both directions), step ms}. */
+int tosca_rhs_les_step_host(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_params *params,
+                            const double *ucont, const double *ucat, const double *theta, double *rhs,
+                            double scale, double *cs, double *nut, double *btheta, double buoy_scale,
+                            double *times_ms);
+
 /* Mixed-precision form of tosca_rhs_les_step (no reference counterpart: a new precision mode of
  * the same unit of work).  The flow fields are FP32 (ucont, ucat, theta in; nut out); the metrics
  * and Jacobians stay FP64 (tosca_mesh), as do rhs (accumulated), cs and btheta.  Arithmetic:

@@ -124,6 +124,7 @@ SIGNATURES = {
     "tosca_ueqn_rk4": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P, P, P, P, I, P, P, D, P, P, I]),
     "tosca_imex_matvec": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P, P, P, D, P]),
     "tosca_grad_p": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P]),
+    "tosca_rhs_les_step_host": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P, P, D, P, P, P, D, P]),
     "tosca_imex_solve": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P, P, P, P, D, C.POINTER(KrylovParams),
                              C.POINTER(KrylovResult)]),
     "tosca_ueqn_rk3cn": (I, [P, C.POINTER(Mesh), C.POINTER(Params), I, P, P, P, P, P, P, I, P, P, D, P,

@@ -32,6 +32,8 @@ struct tosca_ctx {
     size_t chk_elems = 0;
     double *kws = nullptr;       // Krylov basis and the IMEX drivers' work vectors
     size_t kws_elems = 0;
+    double *hs = nullptr;        // device staging of the host-resident step
+    size_t hs_elems = 0;
     tosca::KernelTimer timer;
     bool ref_path = false;   // TOSCA_PATH=ref at creation: reference-structured kernels only
 };
@@ -560,8 +562,9 @@ int tosca_ctx_destroy(tosca_ctx *c) {
     hipSetDevice(c->device);
     c->timer.release();
     if (tosca::g_timer == &c->timer) tosca::g_timer = nullptr;
-    if (c->ws || c->chk || c->kws) hipStreamSynchronize(c->stream);
+    if (c->ws || c->chk || c->kws || c->hs) hipStreamSynchronize(c->stream);
     if (c->kws) hipFree(c->kws);
+    if (c->hs) hipFree(c->hs);
     if (c->ws) hipFree(c->ws);
     if (c->chk) hipFree(c->chk);
     delete c;
@@ -1081,6 +1084,67 @@ int tosca_rhs_les_step(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p,
     if ((rc = ensure_ws(c, ws_need(c)))) return rc;
     queue_step<double>(c, m, p, ucont, ucat, theta, rhs, scale, cs, nut, btheta, buoy_scale);
     return launched(c, "rhs_les_step");
+}
+
+int tosca_rhs_les_step_host(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const double *ucont,
+                            const double *ucat, const double *theta, double *rhs, double scale, double *cs,
+                            double *nut, double *btheta, double buoy_scale, double *times_ms) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!ucont || !ucat || !theta || !rhs || !cs || !nut || !btheta)
+        return set_err(c, TOSCA_ERR_ARG, "host arrays must not be NULL");
+    if ((rc = check_params(c, p))) return rc;
+    set_geo_per(c, p);
+    const Geo &g = c->geo;
+    const size_t n = (size_t)g.nloc(), ng = 3 * (size_t)g.mx * g.my * g.mz;
+    const size_t need = 3 * n + 3 * n + n + ng + n + n + ng;
+    if (c->hs_elems < need) {
+        if (c->hs) { hipStreamSynchronize(c->stream); hipFree(c->hs); c->hs = nullptr; c->hs_elems = 0; }
+        if (hipMalloc(&c->hs, need * sizeof(double)) != hipSuccess) {
+            (void)hipGetLastError();
+            return set_err(c, TOSCA_ERR_DEVICE, "staging allocation of %zu bytes failed", need * sizeof(double));
+        }
+        c->hs_elems = need;
+    }
+    double *dV = c->hs, *dU = dV + 3 * n, *dT = dU + 3 * n, *dR = dT + n, *dC = dR + ng, *dN = dC + n, *dB = dN + n;
+    hipStream_t s = c->stream;
+    hipEvent_t e[4];
+    for (int q = 0; q < 4; q++) hipEventCreate(&e[q]);
+    hipEventRecord(e[0], s);
+    hipError_t he = hipSuccess;
+    auto cp = [&](void *dst, const void *src, size_t bytes, hipMemcpyKind kind) {
+        if (he == hipSuccess) he = hipMemcpyAsync(dst, src, bytes, kind, s);
+    };
+    cp(dV, ucont, 3 * n * sizeof(double), hipMemcpyHostToDevice);
+    cp(dU, ucat, 3 * n * sizeof(double), hipMemcpyHostToDevice);
+    cp(dT, theta, n * sizeof(double), hipMemcpyHostToDevice);
+    cp(dR, rhs, ng * sizeof(double), hipMemcpyHostToDevice);             // rhs accumulates
+    cp(dN, nut, n * sizeof(double), hipMemcpyHostToDevice);              // ghosts the step leaves alone
+    cp(dC, cs, n * sizeof(double), hipMemcpyHostToDevice);
+    hipEventRecord(e[1], s);
+    if (he == hipSuccess) rc = tosca_rhs_les_step(c, m, p, dV, dU, dT, dR, scale, dC, dN, dB, buoy_scale);
+    hipEventRecord(e[2], s);
+    if (he == hipSuccess && rc == TOSCA_OK) {
+        cp(rhs, dR, ng * sizeof(double), hipMemcpyDeviceToHost);
+        cp(cs, dC, n * sizeof(double), hipMemcpyDeviceToHost);
+        cp(nut, dN, n * sizeof(double), hipMemcpyDeviceToHost);
+        cp(btheta, dB, ng * sizeof(double), hipMemcpyDeviceToHost);
+    }
+    hipEventRecord(e[3], s);
+    const hipError_t se = hipStreamSynchronize(s);
+    if (times_ms && rc == TOSCA_OK && he == hipSuccess && se == hipSuccess) {
+        float a = 0, b = 0, d = 0;
+        hipEventElapsedTime(&a, e[0], e[1]);
+        hipEventElapsedTime(&b, e[1], e[2]);
+        hipEventElapsedTime(&d, e[2], e[3]);
+        times_ms[0] = a + d;
+        times_ms[1] = b;
+    }
+    for (int q = 0; q < 4; q++) hipEventDestroy(e[q]);
+    if (rc) return rc;
+    if (he != hipSuccess || se != hipSuccess)
+        return set_err(c, TOSCA_ERR_DEVICE, "host staging failed: %s", hipGetErrorString(he != hipSuccess ? he : se));
+    return TOSCA_OK;
 }
 
 int tosca_rhs_les_step_mixed(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const float *ucont,

@@ -230,6 +230,20 @@ class RhsContext:
                                              C.byref(kr)))
         return kr.iterations, kr.residual, kr.reason
 
+    def rhs_les_step_host(self, mesh, params, ucont, ucat, theta, rhs, scale, cs, nut, btheta, buoy_scale=1.0):
+        """The step with host-resident flow fields (contiguous float64 CPU tensors, preferably pinned);
+        the mesh stays on the device.  Returns (copy ms, step ms)."""
+        def hp(t):
+            if t.dtype != torch.float64 or t.is_cuda or not t.is_contiguous():
+                raise ValueError("host arrays must be contiguous float64 CPU tensors")
+            return t.data_ptr()
+        m, p = mesh_struct(mesh), abi.make_params(params)
+        times = (C.c_double * 2)()
+        self._check(self.lib.tosca_rhs_les_step_host(self.h, C.byref(m), C.byref(p), hp(ucont), hp(ucat), hp(theta),
+                                                     hp(rhs), float(scale), hp(cs), hp(nut), hp(btheta),
+                                                     float(buoy_scale), C.cast(times, C.c_void_p)))
+        return times[0], times[1]
+
     def FormT(self, mesh, params, tparams, ucont, theta, nut, kt, rhs, scale: float = 1.0, form_mode: int = 0):
         """FormT (src/solvers/teqn_solvers.c:6-592): rhs (global scalar) at interior cells.
         tparams: dict(pr=..., wall_t=(iLeft, iRight, jLeft, jRight) thetaWallFunction flags,
