@@ -430,8 +430,8 @@ def cpu_baseline(cfg_case_kwargs, seconds: float):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
-              f"(the multi-GPU sharding is a design in DESIGN.md, not a measured path)", file=sys.stderr)
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              f"(the multi-GPU sharding is designed, not built or measured)", file=sys.stderr)
         sys.exit(2)
     import torch
     from tosca_amd import case as C

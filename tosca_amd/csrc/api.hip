@@ -30,7 +30,7 @@ struct tosca_ctx {
     size_t ws_elems = 0;
     int *chk = nullptr;
     size_t chk_elems = 0;
-    double *kws = nullptr;       // Krylov basis and the IMEX drivers' work vectors
+    double *kws = nullptr;       // Krylov basis and the IMEX predictors' work vectors
     size_t kws_elems = 0;
     double *hs = nullptr;        // device staging of the host-resident step
     size_t hs_elems = 0;
@@ -394,7 +394,7 @@ int check_krylov(tosca_ctx *c, const tosca_krylov_params *kp, tosca_krylov_resul
     return TOSCA_OK;
 }
 
-// Krylov workspace: basis (restart + 1) n, tmp n, the drivers' extra vectors, mdot partials,
+// Krylov workspace: basis (restart + 1) n, tmp n, the predictors' extra vectors, mdot partials,
 // device scalars
 struct KrylovWs { double *V, *tmp, *extra, *partial, *dsc; };
 
