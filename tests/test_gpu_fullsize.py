@@ -8,7 +8,8 @@ sizes in the other GPU tests):
   - free-stream preservation of FormU on the shard mesh (uniform velocity, consistent fluxes);
   - the mixed-precision steps against the FP64 step: <= 1e-5 per output field;
   - the adjoint against central finite differences of the forward in a random direction;
-  - ExplicitRhsU fused against reference-structured.
+  - ExplicitRhsU and the RK4 predictor fused against reference-structured;
+  - GradP exact for a pressure linear in space.
 """
 import os
 
@@ -212,3 +213,54 @@ def test_explicit_rhs_u_fused_vs_reference_structured(shard):
     print(f"  ExplicitRhsU rhs fused vs reference-structured relL2={e:.2e}", flush=True)
     assert e <= 1e-12
 
+
+
+def test_rk4_fused_vs_reference_structured(shard):
+    """UeqnRK4 at full size (153 M doubles per global vector) on both paths: the predictor's vector
+    kernels are identical, the FormU inside differs by round-off only"""
+    rhs, case, mesh = shard
+    lay = case.lay
+    gl = (lay.mz, lay.my, lay.mx, 3)
+    g = torch.Generator(device="cuda").manual_seed(41)
+    dp = 0.01 * torch.randn(gl, dtype=torch.float64, device="cuda", generator=g)
+    uo = case.ucont[3:3 + lay.mz, 3:3 + lay.my, 3:3 + lay.mx].contiguous()
+    nut = torch.zeros(lay.gshape, dtype=torch.float64, device="cuda")
+    cs = torch.zeros(lay.gshape, dtype=torch.float64, device="cuda")
+    res = {}
+    for path in ("fused", "ref"):
+        ctx = _ctx(rhs, lay, path)
+        if path == "fused":
+            ctx.UpdateCs(mesh, case.params, case.ucat, cs)
+            ctx.UpdateNut(mesh, case.params, case.ucat, cs, nut)
+        V, U = case.ucont.clone(), case.ucat.clone()
+        r = torch.empty(gl, dtype=torch.float64, device="cuda")
+        ut = torch.empty(gl, dtype=torch.float64, device="cuda")
+        ctx.UeqnRK4(mesh, case.params, V, U, nut, dp, None, None, 0, None, uo, 0.05, r, ut, 0)
+        torch.cuda.synchronize()
+        res[path] = V
+        ctx.close()
+    e = _rel(res["fused"], res["ref"])
+    print(f"  UeqnRK4 ucont fused vs reference-structured relL2={e:.2e}", flush=True)
+    assert e <= 1e-12
+
+
+def test_grad_p_linear_pressure(shard):
+    """GradP of a pressure linear in space on the shard's stretched Cartesian mesh: S^q . grad p at
+    every face between interior cells (the finite differences and face Jacobians are exact for it)"""
+    rhs, case, mesh = shard
+    lay = case.lay
+    b = torch.tensor([0.003, -0.002, 0.0045], dtype=torch.float64, device="cuda")
+    p = (case.mesh["cent"] * b).sum(-1).contiguous()
+    dp = torch.empty((lay.mz, lay.my, lay.mx, 3), dtype=torch.float64, device="cuda")
+    ctx = _ctx(rhs, lay, "fused")
+    ctx.GradP(mesh, case.params, p, dp)
+    torch.cuda.synchronize()
+    own = lambda a: a[3:3 + lay.mz, 3:3 + lay.my, 3:3 + lay.mx]
+    worst = 0.0
+    for q, name in enumerate(("icsi", "jeta", "kzet")):
+        want = (own(mesh[name]) * b).sum(-1)
+        d = (dp[2:-2, 2:-2, 2:-2, q] - want[2:-2, 2:-2, 2:-2]).abs().max().item()
+        worst = max(worst, d / want.abs().max().item())
+    print(f"  GradP linear pressure: max relative deviation {worst:.2e}", flush=True)
+    assert worst <= 1e-11
+    ctx.close()
