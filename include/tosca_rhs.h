@@ -313,6 +313,26 @@ int tosca_ueqn_abcn(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_params *
                     const double *ucont_o, double dt, double *rhs, double *conv_o,
                     const tosca_krylov_params *kp, tosca_krylov_result *kr);
 
+/* The constant right-hand side of the Crank-Nicolson momentum solve, built by UeqnSNES before
+ * SNESSolve (src/solvers/ueqn_solvers.c:1099-1156, the terms of this path): bU = -dp, + bTheta
+ * terms (3/2 btheta - 1/2 btheta_o when ab2, btheta otherwise; btheta NULL: off), + 0.5 rhs_o
+ * when ab2 (the explicit Crank-Nicolson half), + farm (NULL: off).  All global AoS; ab2 is
+ * clock->it > clock->itStart. */
+int tosca_cn_rhs(tosca_ctx *ctx, const tosca_params *params, const double *dp, const double *btheta,
+                 const double *btheta_o, int ab2, const double *rhs_o, const double *farm, double *bU);
+
+/* Replaces SNESFuncEval (src/solvers/ueqn_solvers.c:1185-1272), the residual of the Crank-Nicolson
+ * momentum solve that UeqnSNES hands to SNESSolve, and with it every function evaluation of the
+ * matrix-free Jacobian (MatMFFD):
+ *   F(u) = dt (bU + s FormU(u, formMode 0)), non-resolved faces reset, - u + ucont_o,
+ * with s = 1 on the first step (ab2 == 0, backward Euler) and 1/2 after (Crank-Nicolson), FormU
+ * evaluated after the ExplicitRhsU resets and contravariantToCartesian of u.  u, bU, ucont_o, F:
+ * global AoS; ucont, ucat: the local work arrays lUcont, lUcat (overwritten, as in the
+ * reference); nut: lNu_t.  A PETSc caller registers it with SNESSetFunction on device Vecs. */
+int tosca_snes_func_eval(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_params *params,
+                         const double *u, double *ucont, double *ucat, const double *nut,
+                         const double *bU, const double *ucont_o, double dt, int ab2, double *F);
+
 /* Replaces GradP (src/peqn.c:3437-3773): the contravariant pressure-gradient fluxes
  * dP^q = (g^q1 dp/dxi + g^q2 dp/deta + g^q3 dp/dzeta) aj_q at the faces 1..m-2 of the interior
  * cells, with the reference's one-sided transverse differences on the corner face rows and its

@@ -58,6 +58,8 @@ def load() -> C.CDLL:
         "oracle_imex_solve": (I, [In, M, Pa, P, P, P, P, P, P, D, I, D, D, I, P, P, P]),
         "oracle_ueqn_rk3cn": (I, [In, M, Pa, I, P, P, P, P, P, P, I, P, P, D, P, I, D, D, I, P, P, P]),
         "oracle_ueqn_abcn": (I, [In, M, Pa, P, P, P, P, P, P, I, P, P, D, P, P, I, D, D, I, P, P, P]),
+        "oracle_cn_rhs": (I, [In, Pa, P, P, P, I, P, P, P]),
+        "oracle_snes_func_eval": (I, [In, M, Pa, P, P, P, P, P, P, D, I, P]),
     }
     for n, (r, a) in sigs.items():
         f = getattr(lib, n)
@@ -284,3 +286,19 @@ def ueqn_abcn(info, mesh, params, ucont, ucat, nut, dp, btheta, btheta_o, ab2, f
                                 _p(conv_o), int(restart), float(rtol), float(atol), int(maxit), C.byref(its),
                                 C.byref(res), C.byref(conv)))
     return its.value, res.value, conv.value
+
+
+def cn_rhs(info, params, dp, btheta, btheta_o, ab2, rhs_o, farm, bU):
+    """UeqnSNES's constant right-hand side restated; bU (global) is overwritten."""
+    lib = load()
+    inf, p = abi.make_info(info), abi.make_params(params)
+    _check(lib.oracle_cn_rhs(C.byref(inf), C.byref(p), _p(dp), _p(btheta), _p(btheta_o), int(ab2), _p(rhs_o),
+                             _p(farm), _p(bU)))
+
+
+def snes_func_eval(info, mesh, params, u, ucont, ucat, nut, bU, ucont_o, dt, ab2, F):
+    """SNESFuncEval restated: F (global) = the Crank-Nicolson residual at u (global)."""
+    lib = load()
+    inf, m, p = abi.make_info(info), _mesh(mesh), abi.make_params(params)
+    _check(lib.oracle_snes_func_eval(C.byref(inf), C.byref(m), C.byref(p), _p(u), _p(ucont), _p(ucat), _p(nut),
+                                     _p(bU), _p(ucont_o), float(dt), int(ab2), _p(F)))

@@ -1288,6 +1288,66 @@ int oracle_imex_matvec(const tosca_dmda_info *info, const tosca_mesh *m, const t
 }
 
 // ================================================================================================
+// The Crank-Nicolson momentum solve's residual (UeqnSNES / SNESFuncEval, ueqn_solvers.c:1087-1272)
+// ================================================================================================
+
+// UeqnSNES's constant right-hand side (:1099-1150, the terms of this path): bU = -dP, the bTheta
+// terms (AB2 when ab2), + 0.5 Rhs_o when ab2 (Crank-Nicolson half), + farm
+int oracle_cn_rhs(const tosca_dmda_info *info, const tosca_params *p, const double *dp, const double *btheta,
+                  const double *btheta_o, int ab2, const double *rhs_o, const double *farm, double *bU_) {
+    Lay l;
+    int rc = check_layout(info, p, l);
+    if (rc) return rc;
+    if (!dp || !bU_ || (btheta && ab2 && !btheta_o) || (ab2 && !rhs_o)) { g_err = "cn_rhs: missing array"; return TOSCA_ERR_ARG; }
+    const int64_t ng = l.mx * l.my * l.mz;
+    V3 *bU = (V3 *)bU_;
+    const V3 *d = (const V3 *)dp;
+    for (int64_t g = 0; g < ng; g++) {                                  // :1102-1103 VecCopy + VecScale(-1)
+        bU[g].x = -1.0 * d[g].x; bU[g].y = -1.0 * d[g].y; bU[g].z = -1.0 * d[g].z;
+    }
+    if (btheta) {                                                       // :1125-1135
+        if (ab2) {
+            axpy(ng, 3.0 / 2.0, (const V3 *)btheta, bU);
+            axpy(ng, -1.0 / 2.0, (const V3 *)btheta_o, bU);
+        } else {
+            axpy(ng, 1.0, (const V3 *)btheta, bU);
+        }
+    }
+    if (ab2) axpy(ng, 0.5, (const V3 *)rhs_o, bU);                      // :1151-1152
+    if (farm) axpy(ng, 1.0, (const V3 *)farm, bU);                      // :1155-1156
+    return TOSCA_OK;
+}
+
+// SNESFuncEval (:1189-1272): F(U) = dt (bU + s FormU(U)) with the non-resolved faces reset, - U +
+// Ucont_o; s = 1 on the first step (backward Euler), 0.5 after (Crank-Nicolson).  u is the SNES
+// iterate (global); ucont, ucat are lUcont, lUcat (overwritten, as the reference overwrites them).
+int oracle_snes_func_eval(const tosca_dmda_info *info, const tosca_mesh *m, const tosca_params *p, const double *u_,
+                          double *ucont_, double *ucat_, const double *nut, const double *bU, const double *ucont_o,
+                          double dt, int ab2, double *F_) {
+    Lay l;
+    int rc = check_layout(info, p, l);
+    if (rc) return rc;
+    if (!u_ || !ucont_ || !ucat_ || !bU || !ucont_o || !F_) { g_err = "snes_func_eval: missing array"; return TOSCA_ERR_ARG; }
+    const int64_t ng = l.mx * l.my * l.mz;
+    V3 *ucont = (V3 *)ucont_, *ucat = (V3 *)ucat_, *F = (V3 *)F_;
+    global_vec_to_local(l, (const V3 *)u_, ucont);                       // :1195-1199
+    reset_no_penetration(l, p, ucont);                                   // :1202
+    reset_face_periodic(l, ucont);                                       // :1205
+    contravariant_to_cartesian(l, m, ucont, ucat);                       // :1208
+    reset_cell_periodic(l, ucat);                                        // :1221
+    for (int64_t g = 0; g < ng; g++) F[g] = ((const V3 *)bU)[g];        // :1224
+    const double scale = ab2 ? 0.5 : 1.0;                                // :1227
+    if ((rc = oracle_formu(info, m, p, ucont_, ucat_, nut, F_, scale, 0))) return rc;   // :1230
+    for (int64_t g = 0; g < ng; g++) {                                   // :1264 VecScale(Rhs, dt)
+        F[g].x = dt * F[g].x; F[g].y = dt * F[g].y; F[g].z = dt * F[g].z;
+    }
+    reset_nonresolved(l, p, F, (const V3 *)ucont_o);                     // :1267
+    axpy(ng, -1.0, (const V3 *)u_, F);                                  // :1270
+    axpy(ng, 1.0, (const V3 *)ucont_o, F);                              // :1271
+    return TOSCA_OK;
+}
+
+// ================================================================================================
 // GMRES(m) as PETSc 3.15 runs it for kspIMEX (src/ueqn.c:261-282: KSPGMRES, PCNONE, nonzero
 // initial guess, KSPSetTolerances(rtol, atol, 1e30, max_it)).  PETSc is a dependency absent from
 // the reference tree; this restates its published algorithm (src/ksp/ksp/impls/gmres/gmres.c:

@@ -124,6 +124,8 @@ SIGNATURES = {
     "tosca_ueqn_rk4": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P, P, P, P, I, P, P, D, P, P, I]),
     "tosca_imex_matvec": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P, P, P, D, P]),
     "tosca_grad_p": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P]),
+    "tosca_cn_rhs": (I, [P, C.POINTER(Params), P, P, P, I, P, P, P]),
+    "tosca_snes_func_eval": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P, P, P, P, D, I, P]),
     "tosca_rhs_les_step_host": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P, P, D, P, P, P, D, P]),
     "tosca_imex_solve": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P, P, P, P, D, C.POINTER(KrylovParams),
                              C.POINTER(KrylovResult)]),

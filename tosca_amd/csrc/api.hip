@@ -875,6 +875,74 @@ int tosca_ueqn_abcn(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, do
     return launched(c, "UeqnABCN");
 }
 
+int tosca_cn_rhs(tosca_ctx *c, const tosca_params *p, const double *dp, const double *btheta,
+                 const double *btheta_o, int ab2, const double *rhs_o, const double *farm, double *bU) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if ((rc = check_params(c, p))) return rc;
+    set_geo_per(c, p);
+    const Geo &g = c->geo;
+    const long n = 3L * g.mx * g.my * g.mz;
+    const size_t vb = n * sizeof(double);
+    if ((rc = check_dev(c, dp, vb, "dp")) || (rc = check_dev(c, bU, vb, "bU"))) return rc;
+    if (btheta) {
+        if ((rc = check_dev(c, btheta, vb, "btheta"))) return rc;
+        if (ab2 && (rc = check_dev(c, btheta_o, vb, "btheta_o"))) return rc;
+    }
+    if (ab2 && (rc = check_dev(c, rhs_o, vb, "rhs_o"))) return rc;
+    if (farm && (rc = check_dev(c, farm, vb, "farm"))) return rc;
+    if (bU == dp || bU == btheta || bU == btheta_o || bU == rhs_o || bU == farm)
+        return set_err(c, TOSCA_ERR_ARG, "bU must not alias an input");
+    hipStream_t s = c->stream;
+    tosca::launch_scale_to(bU, dp, -1.0, n, s);                              // ueqn_solvers.c:1102-1103
+    if (btheta) {                                                             // :1125-1135
+        if (ab2) {
+            tosca::launch_axpy(bU, 3.0 / 2.0, btheta, n, s);
+            tosca::launch_axpy(bU, -1.0 / 2.0, btheta_o, n, s);
+        } else {
+            tosca::launch_axpy(bU, 1.0, btheta, n, s);
+        }
+    }
+    if (ab2) tosca::launch_axpy(bU, 0.5, rhs_o, n, s);                        // :1151-1152
+    if (farm) tosca::launch_axpy(bU, 1.0, farm, n, s);                        // :1155-1156
+    return launched(c, "UeqnSNES constant RHS");
+}
+
+int tosca_snes_func_eval(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const double *u, double *ucont,
+                         double *ucat, const double *nut, const double *bU, const double *ucont_o, double dt, int ab2,
+                         double *F) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if ((rc = check_imex(c, m, p, ucont, ucat, nut, ucont_o))) return rc;
+    const Geo &g = c->geo;
+    const long n = 3L * g.mx * g.my * g.mz;
+    const size_t vb = n * sizeof(double);
+    if ((rc = check_dev(c, u, vb, "u")) || (rc = check_dev(c, bU, vb, "bU")) || (rc = check_dev(c, F, vb, "F"))) return rc;
+    if (F == u || F == bU || F == ucont_o) return set_err(c, TOSCA_ERR_ARG, "F must not alias an input");
+    hipStream_t s = c->stream;
+    tosca::launch_global_to_local(ucont, u, g, s);                           // :1195-1199
+    tosca::launch_wrap(ucont, 3, g, s);
+    tosca::launch_no_penetration(ucont, p, g, s);                            // :1202
+    tosca::launch_reset_face_periodic(ucont, 3, g, s);                       // :1205
+    tosca::launch_contra_to_cart(ucont, m, ucat, g, s);                      // :1208
+    tosca::launch_reset_cell_periodic(ucat, 3, g, s);                        // :1221
+    if (hipMemcpyAsync(F, bU, vb, hipMemcpyDeviceToDevice, s) != hipSuccess)  // :1224
+        return set_err(c, TOSCA_ERR_DEVICE, "SNESFuncEval: copy failed");
+    const double scale = ab2 ? 0.5 : 1.0;                                     // :1227
+    const double *nu_t = p->les_active ? nut : nullptr;
+    if (c->ref_path) {                                                        // :1230
+        tosca::ref_formu(g, m, p, ucont, ucat, nu_t, F, scale, 0, c->ws, s);
+    } else {
+        tosca::FormUArgs A = formu_args(m, p, ucont, ucat, nu_t, F, scale, 0);
+        tosca::fused_formu(g, A, c->ws + 3 * (size_t)g.nloc(), s);
+    }
+    tosca::launch_scale_to(F, F, dt, n, s);                                   // :1264
+    tosca::launch_non_resolved(F, ucont_o, p, g, s);                          // :1267
+    tosca::launch_axpy(F, -1.0, u, n, s);                                     // :1270
+    tosca::launch_axpy(F, 1.0, ucont_o, n, s);                                // :1271
+    return launched(c, "SNESFuncEval");
+}
+
 int tosca_grad_p(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const double *pres, double *dp) {
     int rc = check_ctx(c);
     if (rc) return rc;

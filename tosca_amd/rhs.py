@@ -244,6 +244,18 @@ class RhsContext:
                                                      float(buoy_scale), C.cast(times, C.c_void_p)))
         return times[0], times[1]
 
+    def CNRhs(self, params, dp, btheta, btheta_o, ab2, rhs_o, farm, bU):
+        """UeqnSNES's constant right-hand side bU (src/solvers/ueqn_solvers.c:1099-1156)."""
+        p = abi.make_params(params)
+        self._check(self.lib.tosca_cn_rhs(self.h, C.byref(p), _ptr(dp), _ptr(btheta), _ptr(btheta_o), int(ab2),
+                                          _ptr(rhs_o), _ptr(farm), _ptr(bU)))
+
+    def SNESFuncEval(self, mesh, params, u, ucont, ucat, nut, bU, ucont_o, dt, ab2, F):
+        """SNESFuncEval (src/solvers/ueqn_solvers.c:1185-1272): F = the Crank-Nicolson residual at u."""
+        m, p = mesh_struct(mesh), abi.make_params(params)
+        self._check(self.lib.tosca_snes_func_eval(self.h, C.byref(m), C.byref(p), _ptr(u), _ptr(ucont), _ptr(ucat),
+                                                  _ptr(nut), _ptr(bU), _ptr(ucont_o), float(dt), int(ab2), _ptr(F)))
+
     def FormT(self, mesh, params, tparams, ucont, theta, nut, kt, rhs, scale: float = 1.0, form_mode: int = 0):
         """FormT (src/solvers/teqn_solvers.c:6-592): rhs (global scalar) at interior cells.
         tparams: dict(pr=..., wall_t=(iLeft, iRight, jLeft, jRight) thetaWallFunction flags,
