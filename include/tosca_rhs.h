@@ -113,6 +113,12 @@ typedef struct {
      * the reference (ueqn_solvers.c:336-345, 548-556).                                      */
     const double *tau_wall_i;
     const double *tau_wall_j;
+    /* velocity wall model of the iLeft, iRight, jLeft, jRight patches where they are
+     * TOSCA_BCU_WALLFUNCTION (boundaryU.iLWF ... jRWF, boundary.c:4700-4980): -1 Cabot,
+     * -3 Shumann, 0 unset.  Read only by tosca_update_cartesian_bcs, which fills the ghost
+     * cells of Shumann j patches (boundary.c:1153-1224) and leaves those of Shumann i patches
+     * as they are, as the reference does; Cabot patches are rejected.                       */
+    int wall_u[4];
 } tosca_params;
 
 /* ---- context ------------------------------------------------------------------------------ */
@@ -421,8 +427,12 @@ int tosca_update_contravariant_bcs(tosca_ctx *ctx, const tosca_mesh *mesh, const
  *   slip         the boundary cell's velocity mirrored in the patch (calculateNormal, inline.h:1231),
  *   zeroGradient 2 u(1) - u(2) (linear extrapolation),
  *   periodic     the opposite interior cell (type 1) or ghost (type 2),
- * then the ghost scatter.  velocityWallFunction and inlet-function patches need the wall and
- * inflow models and return TOSCA_ERR_UNSUPPORTED. */
+ *   velocityWallFunction with the Shumann model (params->wall_u = -3) on a j patch
+ *                (boundary.c:1153-1224): 2 Ub_par - Uc_par - Ub_nor, the parts parallel and
+ *                normal to the patch (calculateNormal's j vector) of the boundary cell b and the
+ *                next interior cell c; on an i patch its ghosts are left as they are,
+ * then the ghost scatter.  Inlet-function patches, Cabot wall functions (wall_u = -1) and
+ * wall-function k patches need the inflow and wall models and return TOSCA_ERR_UNSUPPORTED. */
 int tosca_update_cartesian_bcs(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_params *params, double *ucat,
                                const double *u_fixed);
 

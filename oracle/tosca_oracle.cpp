@@ -1931,8 +1931,8 @@ int oracle_cartesian_bcs(const tosca_dmda_info *info, const tosca_mesh *m, const
     if (rc) return rc;
     const int *bc = p->bc_u;
     for (int q = 0; q < 6; q++) {
-        if (bc[q] == TOSCA_BCU_WALLFUNCTION || bc[q] == TOSCA_BCU_GENERIC) {
-            g_err = "cartesian BCs: wall-function and inlet-function patches are not covered";
+        if (bc[q] == TOSCA_BCU_GENERIC || (bc[q] == TOSCA_BCU_WALLFUNCTION && (q >= 4 || p->wall_u[q] != -3))) {
+            g_err = "cartesian BCs: inlet-function patches and wall functions other than Shumann are not covered";
             return TOSCA_ERR_UNSUPPORTED;
         }
         if (bc[q] == TOSCA_BCU_FIXEDVALUE && !ufix) { g_err = "fixedValue patch without values"; return TOSCA_ERR_ARG; }
@@ -1956,12 +1956,28 @@ int oracle_cartesian_bcs(const tosca_dmda_info *info, const tosca_mesh *m, const
         ucat[gdst] = V3{2 * a.x - b.x, 2 * a.y - b.y, 2 * a.z - b.z};
     };
     auto neg = [&](int64_t c, int64_t gdst) { const V3 a = lucat[c]; ucat[gdst] = V3{-1.0 * a.x, -1.0 * a.y, -1.0 * a.z}; };
+    // Shumann wall function on a j patch (boundary.c:1153-1224): the wall-parallel part of the
+    // boundary cell b is extrapolated linearly through the ghost from the next cell c, and the
+    // wall-normal part of b is reflected; the slopes dudn0/dudn1 the reference also forms are
+    // not used
+    auto shumann = [&](int64_t c, int64_t cn, int64_t gdst) {
+        double ni[3], nj[3], nk[3];
+        unit_normals(csi[c], eta[c], zet[c], ni, nj, nk);
+        const V3 n{nj[0], nj[1], nj[2]}, Ub = lucat[c], Uc = lucat[cn];
+        const double db = Ub.x * n.x + Ub.y * n.y + Ub.z * n.z, dc = Uc.x * n.x + Uc.y * n.y + Uc.z * n.z;
+        const V3 bN{db * n.x, db * n.y, db * n.z}, cN{dc * n.x, dc * n.y, dc * n.z};
+        const V3 bP{Ub.x - bN.x, Ub.y - bN.y, Ub.z - bN.z}, cP{Uc.x - cN.x, Uc.y - cN.y, Uc.z - cN.z};
+        V3 gh{2. * bP.x - cP.x, 2. * bP.y - cP.y, 2. * bP.z - cP.z};
+        ucat[gdst] = V3{gh.x - bN.x, gh.y - bN.y, gh.z - bN.z};
+    };
     for (int64_t k = 1; k < mz - 1; k++)                     // src/boundary.c:919-2029
         for (int64_t j = 1; j < my - 1; j++)
             for (int64_t i = 1; i < mx - 1; i++) {
                 const int64_t c = l.L(k, j, i);
                 const int64_t gi0 = l.L(k, j, i - 1), gi1 = l.L(k, j, i + 1), gj0 = l.L(k, j - 1, i),
                               gj1 = l.L(k, j + 1, i), gk0 = l.L(k - 1, j, i), gk1 = l.L(k + 1, j, i);
+                if (bc[2] == TOSCA_BCU_WALLFUNCTION && j == 1) shumann(c, gj1, gj0);
+                if (bc[3] == TOSCA_BCU_WALLFUNCTION && j == my - 2) shumann(c, gj0, gj1);
                 if (bc[0] == TOSCA_BCU_FIXEDVALUE && i == 1) fix(0, gi0);
                 if (bc[1] == TOSCA_BCU_FIXEDVALUE && i == mx - 2) fix(1, gi1);
                 if (bc[2] == TOSCA_BCU_FIXEDVALUE && j == 1) fix(2, gj0);

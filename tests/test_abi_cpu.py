@@ -36,22 +36,22 @@ def test_library_exports_every_declared_symbol():
         assert getattr(lib, n) is not None
 
 
-def test_struct_sizes_match_header():
+def test_struct_sizes_match_header(tmp_path):
     # tosca_dmda_info: 15 int64; tosca_mesh: 18 pointers; tosca_params: see header field order
     assert C.sizeof(abi.DmdaInfo) == 15 * 8
     assert C.sizeof(abi.Mesh) == 18 * 8
-    assert C.sizeof(abi.Params) == 64 + 48 + 24 + 24 + 16
+    assert C.sizeof(abi.Params) == 64 + 48 + 24 + 24 + 16 + 16
     src = r'''
 #include <stdio.h>
 #include <stddef.h>
 #include "tosca_rhs.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(tosca_dmda_info), sizeof(tosca_mesh), sizeof(tosca_params),
-         offsetof(tosca_params, nut_value), offsetof(tosca_params, nu), offsetof(tosca_params, tau_wall_i));
+  printf("%zu %zu %zu %zu %zu %zu %zu\n", sizeof(tosca_dmda_info), sizeof(tosca_mesh), sizeof(tosca_params),
+         offsetof(tosca_params, nut_value), offsetof(tosca_params, nu), offsetof(tosca_params, tau_wall_i),
+         offsetof(tosca_params, wall_u));
   return 0; }
 '''
-    d = ROOT / "_work" / "abi_probe"
-    d.mkdir(parents=True, exist_ok=True)
+    d = tmp_path
     (d / "p.c").write_text(src)
     subprocess.run(["gcc", "-I", str(ROOT / "include"), str(d / "p.c"), "-o", str(d / "p")], check=True)
     vals = list(map(int, subprocess.run([str(d / "p")], capture_output=True, text=True, check=True).stdout.split()))
@@ -61,6 +61,7 @@ int main(void) {
     assert vals[3] == abi.Params.nut_value.offset
     assert vals[4] == abi.Params.nu.offset
     assert vals[5] == abi.Params.tau_wall_i.offset
+    assert vals[6] == abi.Params.wall_u.offset
 
 
 def _info(mx=10, my=8, mz=6, **kw):

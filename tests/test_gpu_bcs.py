@@ -59,6 +59,33 @@ def test_contravariant_bcs(rhs_mod, periodic):
 
 
 @pytest.mark.parametrize("periodic", PERS)
+def test_cartesian_bcs_shumann(rhs_mod, periodic):
+    """velocityWallFunction patches with the Shumann model on every non-periodic i and j patch
+    (the i patches' ghosts stay as they are), the k patches from the other kinds"""
+    seed = 30 + PERS.index(periodic)
+    lay, mesh, params, rng = _case(periodic, seed, (C.BCU_NOSLIP, C.BCU_SLIP, C.BCU_ZEROGRADIENT, C.BCU_FIXEDVALUE))
+    bc_u = tuple(C.BCU_WALLFUNCTION if q < 4 and not periodic[q // 2] else params["bc_u"][q] for q in range(6))
+    params = dict(params, bc_u=bc_u, wall_u=(-3, -3, -3, -3))
+    print(f"\nUpdateCartesianBCs Shumann periodic={periodic} bc_u={params['bc_u']}", flush=True)
+    U0, ufix = rng.standard_normal(lay.gshape + (3,)), rng.standard_normal((6, 3))
+    Uo = U0.copy()
+    O.cartesian_bcs(lay.info(), mesh, params, Uo, ufix)
+    ctx = rhs_mod.RhsContext(lay.info())
+    Ug = to_dev(U0)
+    ctx.UpdateCartesianBCs({k: to_dev(v) for k, v in mesh.items()}, params, Ug, ufix)
+    torch.cuda.synchronize()
+    e = relerr(to_np(Ug), Uo)
+    print(f"  ucat relL2={e:.2e} identical={np.array_equal(to_np(Ug), Uo)}", flush=True)
+    assert e <= TOL
+    from tosca_amd.abi import ToscaError
+    if not periodic[1]:
+        with pytest.raises(ToscaError):
+            ctx.UpdateCartesianBCs({k: to_dev(v) for k, v in mesh.items()}, dict(params, wall_u=(-3, -3, -1, -3)), Ug,
+                                   ufix)
+    ctx.close()
+
+
+@pytest.mark.parametrize("periodic", PERS)
 def test_cartesian_bcs(rhs_mod, periodic):
     seed = 10 + PERS.index(periodic)
     kinds = (C.BCU_NOSLIP, C.BCU_SLIP, C.BCU_ZEROGRADIENT, C.BCU_FIXEDVALUE)

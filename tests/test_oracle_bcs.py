@@ -128,6 +128,77 @@ def test_cartesian_bcs_rejects_wall_and_inlet_patches():
         O.cartesian_bcs(lay.info(), mesh, dict(params, bc_u=(GEN, ZG, ZG, ZG, ZG, ZG)), u)
 
 
+def test_cartesian_bcs_rejects_cabot_and_k_wall_functions():
+    lay, mesh, params, rng = _setup(random_nodes((5, 5, 5), 15), (0, 0, 0), [ZG, ZG, WALL, ZG, ZG, ZG], 15)
+    u = rng.standard_normal(lay.gshape + (3,))
+    from tosca_amd.abi import ToscaError
+    with pytest.raises(ToscaError):
+        O.cartesian_bcs(lay.info(), mesh, dict(params, wall_u=(0, 0, -1, 0)), u)
+    with pytest.raises(ToscaError):
+        O.cartesian_bcs(lay.info(), mesh, dict(params, bc_u=(ZG, ZG, ZG, ZG, WALL, ZG), wall_u=(-3,) * 4), u)
+
+
+def test_shumann_patch_rule():
+    """Shumann wall functions (wall_u = -3) on all four i and j patches: on the j patches the ghost
+    is 2 Ub_par - Uc_par - Ub_nor, with the patch normal taken as the normalised j column of the
+    inverse of the (csi, eta, zet) matrix (numpy.linalg.inv); the i patches' ghosts are untouched"""
+    bc = [WALL, WALL, WALL, WALL, ZG, NOSLIP]
+    lay, mesh, params, rng = _setup(random_nodes((6, 7, 5), 16), (0, 0, 0), bc, 16)
+    params = dict(params, wall_u=(-3, -3, -3, -3))
+    u0 = rng.standard_normal(lay.gshape + (3,))
+    u = u0.copy()
+    O.cartesian_bcs(lay.info(), mesh, params, u)
+    m = (lay.mx, lay.my, lay.mz)
+    want = u0.copy()
+    for q in range(2, 6):
+        a, side = q // 2, q % 2
+        g_idx, c1, c2 = (0, 1, 2) if side == 0 else (m[a] - 1, m[a] - 2, m[a] - 3)
+        sg, sc, sc2 = _plane(lay, a, g_idx), _plane(lay, a, c1), _plane(lay, a, c2)
+        if bc[q] == NOSLIP:
+            want[sg] = -u0[sc]
+        elif bc[q] == ZG:
+            want[sg] = 2 * u0[sc] - u0[sc2]
+        else:
+            M = np.stack([mesh[n][sc] for n in ("csi", "eta", "zet")], axis=-2)
+            n = np.linalg.inv(M)[..., :, 1]
+            n = n / np.linalg.norm(n, axis=-1, keepdims=True)
+            nor = lambda v: (v * n).sum(-1, keepdims=True) * n
+            ub, uc = u0[sc], u0[sc2]
+            ref = 2 * (ub - nor(ub)) - (uc - nor(uc)) - nor(ub)
+            got = u[sg]
+            assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max()
+            want[sg] = got
+    assert np.array_equal(u, want)
+
+
+@pytest.mark.parametrize("geom", ["box", "sheared"])
+def test_shumann_no_penetration_and_preserved_gradient(geom):
+    """on an affine lattice calculateNormal's j vector is the unit lattice vector e of the j axis
+    (as for slip).  With a velocity whose part orthogonal to e is linear in the cell index j and
+    whose part along e is arbitrary, the Shumann ghost continues the orthogonal profile linearly
+    to the ghost cell (the gradient at the wall equals the interior one) and makes the part along
+    e averaged across the wall face zero (no penetration)"""
+    vecs = AFFINE[geom]
+    lay, mesh, params, rng = _setup(affine_nodes((5, 6, 4), vecs), (0, 0, 0), [ZG, ZG, WALL, WALL, ZG, ZG], 17)
+    params = dict(params, wall_u=(0, 0, -3, -3))
+    m = (lay.mx, lay.my, lay.mz)
+    e = vecs[1] / np.linalg.norm(vecs[1])
+    par = np.cross(e, [0.3, 0.1, 1.0])
+    par = par / np.linalg.norm(par)
+    jj = np.arange(lay.my + 6)[None, :, None] - 3.0       # cell index j of the local array
+    w = rng.standard_normal(lay.gshape)
+    u0 = (0.7 + 0.25 * jj)[..., None] * par + w[..., None] * e
+    u = u0.copy()
+    O.cartesian_bcs(lay.info(), mesh, params, u)
+    for side in (0, 1):
+        g, b = (0, 1) if side == 0 else (m[1] - 1, m[1] - 2)
+        sg, sb = _plane(lay, 1, g), _plane(lay, 1, b)
+        ug = u[sg]
+        assert np.abs(ug @ par - (0.7 + 0.25 * g)).max() <= 1e-13
+        assert np.abs((ug + u0[sb]) @ e).max() <= 1e-13 * np.abs(w).max()
+        assert np.abs(ug - (ug @ par)[..., None] * par - (ug @ e)[..., None] * e).max() <= 1e-13
+
+
 # ------------------------------------------------------------------------------------------------
 # UpdateContravariantBCs
 # ------------------------------------------------------------------------------------------------

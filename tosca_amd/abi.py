@@ -59,6 +59,7 @@ class Params(C.Structure):
         ("t_ref", C.c_double),
         ("tau_wall_i", C.c_void_p),
         ("tau_wall_j", C.c_void_p),
+        ("wall_u", C.c_int * 4),
     ]
 
 
@@ -182,6 +183,7 @@ def make_params(d: Mapping) -> Params:
     p.t_ref = float(d["t_ref"])
     p.tau_wall_i = d.get("tau_wall_i") or None
     p.tau_wall_j = d.get("tau_wall_j") or None
+    p.wall_u[:] = [int(v) for v in d.get("wall_u", (0, 0, 0, 0))]
     return p
 
 

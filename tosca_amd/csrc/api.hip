@@ -1023,9 +1023,13 @@ int tosca_update_cartesian_bcs(tosca_ctx *c, const tosca_mesh *m, const tosca_pa
     if (rc) return rc;
     if ((rc = check_params(c, p))) return rc;
     for (int q = 0; q < 6; q++) {
-        if (p->bc_u[q] == TOSCA_BCU_WALLFUNCTION || p->bc_u[q] == TOSCA_BCU_GENERIC)
-            return set_err(c, TOSCA_ERR_UNSUPPORTED, "bc_u[%d] = %d: wall-function and inlet-function velocity patches need "
-                                                     "the wall and inflow models, which are not part of this library", q, p->bc_u[q]);
+        if (p->bc_u[q] == TOSCA_BCU_GENERIC)
+            return set_err(c, TOSCA_ERR_UNSUPPORTED, "bc_u[%d]: inlet-function velocity patches need the inflow models, "
+                                                     "which are not part of this library", q);
+        if (p->bc_u[q] == TOSCA_BCU_WALLFUNCTION && (q >= 4 || p->wall_u[q] != -3))
+            return set_err(c, TOSCA_ERR_UNSUPPORTED, "bc_u[%d] is a wall function with wall model %d: only the Shumann "
+                                                     "model (wall_u = -3) of an i or j patch is supported here; the Cabot "
+                                                     "model (-1) needs its iterative wall-shear solve", q, q < 4 ? p->wall_u[q] : 0);
         if (p->bc_u[q] == TOSCA_BCU_FIXEDVALUE && !u_fixed)
             return set_err(c, TOSCA_ERR_ARG, "bc_u[%d] is fixedValue but u_fixed is NULL", q);
     }

@@ -4,7 +4,8 @@
 //
 //   UpdateContravariantBCs  boundary.c:614-822    -> k_contra_periodic_copy, k_contra_bcs
 //   UpdateCartesianBCs      boundary.c:824-2050   -> k_cart_bcs (fixedValue, slip, noSlip,
-//                                                     zeroGradient, periodic patches)
+//                                                     zeroGradient, periodic patches, Shumann
+//                                                     wall-function j patches)
 //   UpdateTemperatureBCs    boundary.c:2052-2628  -> k_temp_bcs (zeroGradient, fixedValue,
 //                                                     fixedGradient, periodic patches)
 // The reference reads the local array as it was on entry and writes a global copy; every kernel
@@ -20,6 +21,7 @@ namespace {
 
 struct PatchSet {
     int bc[6];          // patch kinds, order iLeft iRight jLeft jRight kLeft kRight
+    int wall[6];        // velocity wall model of the i and j patches (-3 Shumann), 0 elsewhere
     double val[18];     // fixedValue vectors (velocity) or values (temperature, first 6)
 };
 
@@ -127,6 +129,15 @@ __global__ void k_cart_bcs(double *ucat, const double *csi, const double *eta, c
     } else if (b == TOSCA_BCU_NOSLIP) {                                  // :1832-1867
         const V3 U = ld3(ucat, ci);
         out = V3{-1.0 * U.x, -1.0 * U.y, -1.0 * U.z};
+    } else if (b == TOSCA_BCU_WALLFUNCTION && a == 1 && P.wall[q] == -3) {   // Shumann, :1153-1224
+        // wall-parallel part extrapolated linearly from the next cell, wall-normal part reflected
+        const V3 n = unit_normal(ld3(csi, ci), ld3(eta, ci), ld3(zet, ci), 1);
+        const V3 Ub = ld3(ucat, ci), Uc = ld3(ucat, g.L(c2[2], c2[1], c2[0]));
+        const double db = Ub.x * n.x + Ub.y * n.y + Ub.z * n.z, dc = Uc.x * n.x + Uc.y * n.y + Uc.z * n.z;
+        const V3 bN{db * n.x, db * n.y, db * n.z}, cN{dc * n.x, dc * n.y, dc * n.z};
+        const V3 bP{Ub.x - bN.x, Ub.y - bN.y, Ub.z - bN.z}, cP{Uc.x - cN.x, Uc.y - cN.y, Uc.z - cN.z};
+        const V3 gp{2. * bP.x - cP.x, 2. * bP.y - cP.y, 2. * bP.z - cP.z};
+        out = V3{gp.x - bN.x, gp.y - bN.y, gp.z - bN.z};
     } else if (b == TOSCA_BCU_ZEROGRADIENT) {                            // :1869-1928
         const V3 U = ld3(ucat, ci), U2 = ld3(ucat, g.L(c2[2], c2[1], c2[0]));
         out = V3{2 * U.x - U2.x, 2 * U.y - U2.y, 2 * U.z - U2.z};
@@ -204,7 +215,8 @@ void launch_contravariant_bcs(double *ucont, const double *ucat, const tosca_mes
 void launch_cartesian_bcs(double *ucat, const double *u_fixed, const tosca_mesh *m, const tosca_params *p,
                           const Geo &g, hipStream_t s) {
     const int n = maxdim(g);
-    const PatchSet P = patches(p->bc_u, u_fixed, 18);
+    PatchSet P = patches(p->bc_u, u_fixed, 18);
+    for (int q = 0; q < 4; q++) P.wall[q] = p->wall_u[q];
     TOSCA_LAUNCH("k_cart_bcs", k_cart_bcs, dim3((n + 127) / 128, n, 6), dim3(128), 0, s, ucat, m->csi, m->eta, m->zet, g, P);
     launch_wrap(ucat, 3, g, s);
 }
