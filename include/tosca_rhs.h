@@ -78,10 +78,16 @@ enum {
     TOSCA_BCU_NOSLIP = 4, TOSCA_BCU_ZEROGRADIENT = 5, TOSCA_BCU_FIXEDVALUE = 6
 };
 
-/* boundaryT patch kinds for tosca_update_temperature_bcs (src/boundary.c:2378-2555); a
- * thetaWallFunction patch other than the Shumann surface-temperature model fills its ghosts as
- * zeroGradient (boundary.c:2380-2415) and is passed as TOSCA_BCT_ZEROGRADIENT */
-enum { TOSCA_BCT_ZEROGRADIENT = 0, TOSCA_BCT_FIXEDVALUE = 1, TOSCA_BCT_FIXEDGRADIENT = 2, TOSCA_BCT_PERIODIC = 3 };
+/* boundaryT patch kinds for tosca_update_temperature_bcs (src/boundary.c:2378-2555).  A
+ * thetaWallFunction i or j patch fills its ghosts as zeroGradient (boundary.c:2380-2415) and is
+ * passed as TOSCA_BCT_ZEROGRADIENT, except the jLeft patch with the surface-temperature model
+ * (boundaryT.jLWF == -4) after the first iteration, whose ghost is 2 Ts - t(1) with Ts the
+ * surface temperature interpolated to the current time (boundary.c:2392-2410): that patch is
+ * passed as TOSCA_BCT_SURFACETEMP with Ts as its t_value. */
+enum {
+    TOSCA_BCT_ZEROGRADIENT = 0, TOSCA_BCT_FIXEDVALUE = 1, TOSCA_BCT_FIXEDGRADIENT = 2, TOSCA_BCT_PERIODIC = 3,
+    TOSCA_BCT_SURFACETEMP = 4
+};
 
 /* boundaryNut patch kinds applied by UpdateNutBCs (src/boundary.c:2812-2917) */
 enum { TOSCA_BCNUT_NONE = 0, TOSCA_BCNUT_ZEROGRADIENT = 1, TOSCA_BCNUT_FIXEDVALUE = 2, TOSCA_BCNUT_PERIODIC = 3 };
@@ -439,8 +445,9 @@ int tosca_update_cartesian_bcs(tosca_ctx *ctx, const tosca_mesh *mesh, const tos
 /* UpdateTemperatureBCs (src/boundary.c:2052-2628) on lTmprt (local scalar, in place): ghosts of
  * zeroGradient (t(1)), fixedValue (value), fixedGradient ((1/aj)/|area| value + t(1), the area of
  * the boundary cell's own basis vector) and periodic patches, then the ghost scatter.  bc_t and
- * t_value: host arrays of 6 (TOSCA_BCT_* kinds, values) in patch order.  Inlet functions and the
- * Shumann wall model are not covered. */
+ * t_value: host arrays of 6 (TOSCA_BCT_* kinds, values) in patch order.  The jLeft patch may be
+ * TOSCA_BCT_SURFACETEMP (2 Ts - t(1), Ts = t_value[2]; boundary.c:2392-2410); thetaWallFunction
+ * i and j patches are otherwise zeroGradient.  Inlet functions are not covered. */
 int tosca_update_temperature_bcs(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_params *params,
                                  const int *bc_t, const double *t_value, double *theta);
 

@@ -2036,8 +2036,10 @@ int oracle_temperature_bcs(const tosca_dmda_info *info, const tosca_mesh *m, con
     Lay l;
     int rc = check_layout(info, p, l);
     if (rc) return rc;
-    for (int q = 0; q < 6; q++)
-        if (bct[q] < TOSCA_BCT_ZEROGRADIENT || bct[q] > TOSCA_BCT_PERIODIC) { g_err = "bad temperature BC kind"; return TOSCA_ERR_ARG; }
+    for (int q = 0; q < 6; q++) {
+        if (bct[q] < TOSCA_BCT_ZEROGRADIENT || bct[q] > TOSCA_BCT_SURFACETEMP) { g_err = "bad temperature BC kind"; return TOSCA_ERR_ARG; }
+        if (bct[q] == TOSCA_BCT_SURFACETEMP && q != 2) { g_err = "surface temperature on a patch other than jLeft"; return TOSCA_ERR_ARG; }
+    }
     const V3 *csi = (const V3 *)m->csi, *eta = (const V3 *)m->eta, *zet = (const V3 *)m->zet;
     const double *aj = m->aj;
     const int64_t mx = l.mx, my = l.my, mz = l.mz;
@@ -2052,6 +2054,7 @@ int oracle_temperature_bcs(const tosca_dmda_info *info, const tosca_mesh *m, con
                 const bool on[6] = {i == 1, i == mx - 2, j == 1, j == my - 2, k == 1, k == mz - 2};
                 for (int q = 0; q < 6; q++)
                     if (on[q] && bct[q] == TOSCA_BCT_ZEROGRADIENT) t[g[q]] = lt[c];
+                if (on[2] && bct[2] == TOSCA_BCT_SURFACETEMP) t[g[2]] = 2.0 * tval[2] - lt[c];   // :2392-2410
                 for (int q = 0; q < 6; q++)
                     if (on[q] && bct[q] == TOSCA_BCT_FIXEDVALUE) t[g[q]] = tval[q];
                 for (int q = 0; q < 6; q++)

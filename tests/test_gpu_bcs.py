@@ -110,6 +110,8 @@ def test_temperature_bcs(rhs_mod, periodic):
     lay, mesh, params, rng = _case(periodic, seed, (C.BCU_ZEROGRADIENT,))
     bct = [C.BCT_PERIODIC if periodic[q // 2] else
            int(rng.choice((C.BCT_ZEROGRADIENT, C.BCT_FIXEDVALUE, C.BCT_FIXEDGRADIENT))) for q in range(6)]
+    if not periodic[1] and seed % 2 == 0:
+        bct[2] = C.BCT_SURFACETEMP           # jLeft with the surface-temperature model
     vals = rng.standard_normal(6)
     print(f"\nUpdateTemperatureBCs periodic={periodic} bc_t={bct}", flush=True)
     t0 = 300.0 + rng.standard_normal(lay.gshape)

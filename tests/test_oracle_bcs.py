@@ -313,6 +313,29 @@ def test_temperature_bcs_patch_rules():
     assert np.array_equal(t, want)
 
 
+def test_temperature_bcs_surface_temperature():
+    """jLeft with the surface-temperature model: the ghost makes the temperature interpolated to
+    the wall face (the mean of ghost and boundary cell) equal the surface temperature; other
+    patches reject the kind"""
+    bct = [C.BCT_ZEROGRADIENT, C.BCT_ZEROGRADIENT, C.BCT_SURFACETEMP, C.BCT_FIXEDVALUE, C.BCT_ZEROGRADIENT,
+           C.BCT_ZEROGRADIENT]
+    lay, mesh, params, rng = _setup(random_nodes((6, 7, 5), 19), (0, 0, 0), [ZG] * 6, 19)
+    vals = np.array([0.0, 0.0, 287.25, 301.0, 0.0, 0.0])
+    t0 = 300.0 + rng.standard_normal(lay.gshape)
+    t = t0.copy()
+    O.temperature_bcs(lay.info(), mesh, params, bct, vals, t)
+    sg, sc = _plane(lay, 1, 0), _plane(lay, 1, 1)
+    assert np.abs(0.5 * (t[sg] + t0[sc]) - vals[2]).max() <= 1e-12
+    m = (lay.mx, lay.my, lay.mz)
+    assert (t[_plane(lay, 1, m[1] - 1)] == vals[3]).all()
+    from tosca_amd.abi import ToscaError
+    for q in (0, 1, 3, 4, 5):
+        bad = list(bct)
+        bad[2], bad[q] = C.BCT_ZEROGRADIENT, C.BCT_SURFACETEMP
+        with pytest.raises(ToscaError):
+            O.temperature_bcs(lay.info(), mesh, params, bad, vals, t0.copy())
+
+
 @pytest.mark.parametrize("periodic", [(1, 0, 2), (2, 1, 0)])
 def test_temperature_bcs_periodic(periodic):
     bct = []

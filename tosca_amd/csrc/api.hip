@@ -1046,9 +1046,12 @@ int tosca_update_temperature_bcs(tosca_ctx *c, const tosca_mesh *m, const tosca_
     if (rc) return rc;
     if ((rc = check_params(c, p))) return rc;
     if (!bc_t || !t_value) return set_err(c, TOSCA_ERR_ARG, "bc_t / t_value is NULL");
-    for (int q = 0; q < 6; q++)
-        if (bc_t[q] < TOSCA_BCT_ZEROGRADIENT || bc_t[q] > TOSCA_BCT_PERIODIC)
+    for (int q = 0; q < 6; q++) {
+        if (bc_t[q] < TOSCA_BCT_ZEROGRADIENT || bc_t[q] > TOSCA_BCT_SURFACETEMP)
             return set_err(c, TOSCA_ERR_ARG, "bc_t[%d] = %d is not a TOSCA_BCT_* kind", q, bc_t[q]);
+        if (bc_t[q] == TOSCA_BCT_SURFACETEMP && q != 2)
+            return set_err(c, TOSCA_ERR_ARG, "bc_t[%d]: the surface-temperature ghost applies to the jLeft patch only", q);
+    }
     set_geo_per(c, p);
     if ((rc = check_mesh(c, m, false, true, false))) return rc;
     if ((rc = check_dev(c, theta, (size_t)c->geo.nloc() * sizeof(double), "theta"))) return rc;

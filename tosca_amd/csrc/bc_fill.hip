@@ -7,7 +7,8 @@
 //                                                     zeroGradient, periodic patches, Shumann
 //                                                     wall-function j patches)
 //   UpdateTemperatureBCs    boundary.c:2052-2628  -> k_temp_bcs (zeroGradient, fixedValue,
-//                                                     fixedGradient, periodic patches)
+//                                                     fixedGradient, periodic patches, surface
+//                                                     temperature on jLeft)
 // The reference reads the local array as it was on entry and writes a global copy; every kernel
 // here writes only locations that no thread of the same launch reads (ghost planes of the cell
 // fills; boundary faces of the flux fill, whose one cross-face read -- the left periodic face
@@ -172,6 +173,8 @@ __global__ void k_temp_bcs(double *t, const double *csi, const double *eta, cons
     double out;
     if (b == TOSCA_BCT_ZEROGRADIENT) {                                   // boundary.c:2379-2426
         out = t[ci];
+    } else if (b == TOSCA_BCT_SURFACETEMP) {                             // :2392-2410, jLeft only
+        out = 2.0 * P.val[q] - t[ci];
     } else if (b == TOSCA_BCT_FIXEDVALUE) {                              // :2429-2458
         out = P.val[q];
     } else if (b == TOSCA_BCT_FIXEDGRADIENT) {                           // :2461-2555
