@@ -122,9 +122,12 @@ typedef struct {
     /* velocity wall model of the iLeft, iRight, jLeft, jRight patches where they are
      * TOSCA_BCU_WALLFUNCTION (boundaryU.iLWF ... jRWF, boundary.c:4700-4980): -1 Cabot,
      * -3 Shumann, 0 unset.  Read only by tosca_update_cartesian_bcs, which fills the ghost
-     * cells of Shumann j patches (boundary.c:1153-1224) and leaves those of Shumann i patches
-     * as they are, as the reference does; Cabot patches are rejected.                       */
+     * cells of Cabot i and j patches (boundary.c:968-1150) and of Shumann j patches
+     * (:1153-1224), and leaves those of Shumann i patches as they are, as the reference does. */
     int wall_u[4];
+    /* kRough of the Cabot wall models of those patches (iLWM->wmCabot->roughness ...); a value
+     * above 1e-19 selects the rough-wall law utau_wf, otherwise the smooth-wall Cabot law    */
+    double wall_rough[4];
 } tosca_params;
 
 /* ---- context ------------------------------------------------------------------------------ */
@@ -437,10 +440,21 @@ int tosca_update_contravariant_bcs(tosca_ctx *ctx, const tosca_mesh *mesh, const
  *                (boundary.c:1153-1224): 2 Ub_par - Uc_par - Ub_nor, the parts parallel and
  *                normal to the patch (calculateNormal's j vector) of the boundary cell b and the
  *                next interior cell c; on an i patch its ghosts are left as they are,
- * then the ghost scatter.  Inlet-function patches, Cabot wall functions (wall_u = -1) and
- * wall-function k patches need the inflow and wall models and return TOSCA_ERR_UNSUPPORTED. */
+ *   velocityWallFunction with the Cabot model (wall_u = -1) on an i or j patch (boundary.c:
+ *                968-1150): u* of the boundary cell from the parallel speed |Ub_par| at the
+ *                wall distance sb = 0.5 / aj / |area| (uTauCabot, the Newton solve of the
+ *                integrated near-wall eddy-viscosity law, wallfunctions.c:280-543, or utau_wf
+ *                for rough walls, :568-585), written to ustar at the boundary cell; the ghost
+ *                gets the parallel speed max(|Ub_par| - 2 sb u*^2 / nu, -|Ub_par|) along Ub_par
+ *                minus Ub_nor (0 and u* = 0 where |Ub_par| < 1e-20).  The right patch of an
+ *                axis whose left patch is Cabot too takes the left patch's roughness, as in the
+ *                reference (boundary.c:978-979, 1072-1073),
+ * then the ghost scatter.  ustar: local scalar array (lUstar), device; may be NULL when no patch
+ * is Cabot.  Inlet-function patches and wall-function k patches need the inflow models or have
+ * no wall model and return TOSCA_ERR_UNSUPPORTED.  The first call with a smooth Cabot patch
+ * builds the context's copy of the reference's integration table (5e6 + 1 doubles). */
 int tosca_update_cartesian_bcs(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_params *params, double *ucat,
-                               const double *u_fixed);
+                               const double *u_fixed, double *ustar);
 
 /* UpdateTemperatureBCs (src/boundary.c:2052-2628) on lTmprt (local scalar, in place): ghosts of
  * zeroGradient (t(1)), fixedValue (value), fixedGradient ((1/aj)/|area| value + t(1), the area of

@@ -47,7 +47,9 @@ def load() -> C.CDLL:
         "oracle_rhs_jvp": (I, [In, M, Pa, P, P, P, P, P, P, P, P, D, D, P, P]),
         "oracle_explicit_rhs_u": (I, [In, M, Pa, P, P, P, P, P, P, I, P, P, P, I]),
         "oracle_contravariant_bcs": (I, [In, M, Pa, P, P]),
-        "oracle_cartesian_bcs": (I, [In, M, Pa, P, P]),
+        "oracle_cartesian_bcs": (I, [In, M, Pa, P, P, P]),
+        "oracle_cabot_utau": (D, [D, D, D]),
+        "oracle_cabot_table": (D, [C.c_int64]),
         "oracle_temperature_bcs": (I, [In, M, Pa, P, P, P]),
         "oracle_formt": (I, [In, M, Pa, C.POINTER(abi.TeqnParams), P, P, P, P, P, D, I]),
         "oracle_ueqn_fe": (I, [In, M, Pa, P, P, P, P, P, P, I, P, P, D, P, I]),
@@ -173,12 +175,23 @@ def contravariant_bcs(info, mesh, params, ucont, ucat):
     _check(lib.oracle_contravariant_bcs(C.byref(inf), C.byref(m), C.byref(p), _p(ucont), _p(ucat)))
 
 
-def cartesian_bcs(info, mesh, params, ucat, u_fixed=None):
-    """UpdateCartesianBCs subset restated; ucat modified in place; u_fixed (6, 3) or None."""
+def cartesian_bcs(info, mesh, params, ucat, u_fixed=None, ustar=None):
+    """UpdateCartesianBCs subset restated; ucat (and ustar, a local scalar array, where Cabot wall
+    functions write it) modified in place; u_fixed (6, 3) or None."""
     lib = load()
     inf, m, p = abi.make_info(info), _mesh(mesh), abi.make_params(params)
     uf = None if u_fixed is None else np.ascontiguousarray(u_fixed, dtype=np.float64).reshape(18)
-    _check(lib.oracle_cartesian_bcs(C.byref(inf), C.byref(m), C.byref(p), _p(ucat), _p(uf)))
+    _check(lib.oracle_cartesian_bcs(C.byref(inf), C.byref(m), C.byref(p), _p(ucat), _p(uf), _p(ustar)))
+
+
+def cabot_utau(nu, u, y):
+    """uTauCabot (wallfunctions.c:519-543): friction velocity of the smooth-wall Cabot law."""
+    return float(load().oracle_cabot_utau(float(nu), float(u), float(y)))
+
+
+def cabot_table(i):
+    """entry i of the Cabot integration table: the integral of 1/(1 + nu_t+) over y+ in [0, 2 i]."""
+    return float(load().oracle_cabot_table(int(i)))
 
 
 def temperature_bcs(info, mesh, params, bc_t, t_value, theta):

@@ -1924,19 +1924,115 @@ static void unit_normals(V3 c, V3 e, V3 z, double ni[3], double nj[3], double nk
     }
 }
 
+// ---- Cabot wall law (src/wallfunctions.c) ----------------------------------------------------
+// near_wall_eddy_viscosity (wallfunctions.c:280-284)
+static double nw_nut(double yp) { return 0.41 * yp * pow(1. - exp(-yp / 19.), 2.0); }
+
+// integration_buffer of pre_integrate (wallfunctions.c:296-360): entry i is the integral of
+// 1 / (1 + nu_t+) over y+ in [0, 2 i], summed interval by interval in index order.  The table is
+// grown on demand: an entry depends only on the ones before it, so a partial table holds the same
+// values as the reference's full one (n_yp = 5e6).
+static const int64_t CAB_NYP = 5000000;
+static std::vector<double> g_cabot{0.0};
+static double cabot_entry(int64_t i) {
+    if ((int64_t)g_cabot.size() <= i) {
+        for (int64_t e = (int64_t)g_cabot.size(); e <= i; e++) {
+            const int N = 24;
+            const double ya_p = (double)(e - 1) * 2, yb_p = (double)(e) * 2, dy_p = (yb_p - ya_p) / (double)N;
+            std::vector<double> E(N + 1);
+            double val = 0, ybegin_p = ya_p;
+            for (int k = 0; k <= N; k++) E[k] = 1. / (1. + nw_nut(ybegin_p + dy_p * k));
+            for (int k = 0; k < N; k++) {
+                const double F0 = E[k];
+                const double F1 = 1. / (1. + nw_nut(ybegin_p + dy_p * 1. / 3.));
+                const double F2 = 1. / (1. + nw_nut(ybegin_p + dy_p * 2. / 3.));
+                const double F3 = E[k + 1];
+                val += dy_p / 3. * (3 * F0 + 9 * F1 + 9 * F2 + 3 * F3) / 8.;
+                ybegin_p += dy_p;
+            }
+            g_cabot.push_back(g_cabot.back() + val);
+        }
+    }
+    return g_cabot[i];
+}
+
+// integrate_F (wallfunctions.c:364-420) for a smooth wall.  Past y+ = 1e7 the reference starts
+// from integration_buffer[n_yp] with its local n_yp = 0 (pre_integrate sets it on the process's
+// first call only), i.e. from 0.  Indices outside the table (y+ < 0, or y+ at its last point) are
+// clamped to it; the reference reads outside its buffer there.
+static double cabot_integral(double nu, double utau, double yb) {
+    double val = 0;
+    const double ya_plus = 0 * utau / nu, yb_plus = yb * utau / nu;
+    (void)ya_plus;
+    if (yb_plus <= 1e7) {
+        const double qb = yb_plus / (double)2;
+        int64_t ib = qb > 0.0 ? (int64_t)qb : 0;
+        ib = std::min<int64_t>(ib, CAB_NYP - 1);
+        const double lo = cabot_entry(ib), hi = cabot_entry(ib + 1);
+        const double int_a = 0;
+        const double int_b = (hi - lo) / (double)(2) * (yb_plus - (double)(ib) * 2) + lo;
+        return (int_b - int_a) / utau;
+    }
+    const int N = 4;
+    const double ya = 1e7, ydiff = yb_plus - ya, dy = ydiff / (double)N;
+    std::vector<double> E(N + 1);
+    double ybegin = ya;
+    for (int i = 0; i <= N; i++) E[i] = 1. / (1. + nw_nut(ybegin + dy * i));
+    for (int i = 0; i < N; i++) {
+        const double F0 = E[i], F1 = 1. / (1. + nw_nut(ybegin + dy * 1. / 3.)), F2 = 1. / (1. + nw_nut(ybegin + dy * 2. / 3.)),
+                     F3 = E[i + 1];
+        val += dy / 3. * (3 * F0 + 9 * F1 + 9 * F2 + 3 * F3) / 8.;
+        ybegin += dy;
+    }
+    return val / utau;
+}
+
+static double f_cabot(double nu, double u, double y, double utau) { return utau * utau * cabot_integral(nu, utau, y) - u; }
+
+// uTauCabot (wallfunctions.c:519-543)
+static double utau_cabot(double nu, double u, double y) {
+    double x = 0, x0 = 0.01;
+    const double eps = 1.e-7;
+    for (int i = 0; i < 30; i++) {
+        const double df = (f_cabot(nu, u, y, x0 + eps) - f_cabot(nu, u, y, x0 - eps)) / (2.0 * eps);
+        x = x0 - f_cabot(nu, u, y, x0) / df;
+        if (fabs(x0 - x) < 1.e-10) break;
+        x0 = x;
+    }
+    return x;
+}
+
+// utau_wf (wallfunctions.c:568-585)
+static double utau_rough(double nu, double ks, double sb, double ut) {
+    const double k_s = ks * 0.033, kappa = 0.41;
+    if (k_s > 1.e-13) return ut * kappa / log(sb / k_s);
+    const double A = 8.3, B = 1.0 / 7.0;
+    return pow(ut * pow(nu, B) / (A * pow(sb, B)), 1.0 / (1.0 + B));
+}
+
+extern "C" double oracle_cabot_utau(double nu, double u, double y) { return utau_cabot(nu, u, y); }
+extern "C" double oracle_cabot_table(int64_t i) { return cabot_entry(i); }
+
 int oracle_cartesian_bcs(const tosca_dmda_info *info, const tosca_mesh *m, const tosca_params *p, double *ucat_,
-                         const double *ufix) {
+                         const double *ufix, double *ustar) {
     Lay l;
     int rc = check_layout(info, p, l);
     if (rc) return rc;
     const int *bc = p->bc_u;
     for (int q = 0; q < 6; q++) {
-        if (bc[q] == TOSCA_BCU_GENERIC || (bc[q] == TOSCA_BCU_WALLFUNCTION && (q >= 4 || p->wall_u[q] != -3))) {
-            g_err = "cartesian BCs: inlet-function patches and wall functions other than Shumann are not covered";
+        if (bc[q] == TOSCA_BCU_GENERIC || (bc[q] == TOSCA_BCU_WALLFUNCTION && q >= 4)) {
+            g_err = "cartesian BCs: inlet-function patches and k wall functions are not covered";
             return TOSCA_ERR_UNSUPPORTED;
         }
+        if (bc[q] == TOSCA_BCU_WALLFUNCTION && p->wall_u[q] != -1 && p->wall_u[q] != -3) {
+            g_err = "cartesian BCs: wall function without a wall model";
+            return TOSCA_ERR_ARG;
+        }
+        if (bc[q] == TOSCA_BCU_WALLFUNCTION && p->wall_u[q] == -1 && !ustar) { g_err = "Cabot patch without ustar"; return TOSCA_ERR_ARG; }
         if (bc[q] == TOSCA_BCU_FIXEDVALUE && !ufix) { g_err = "fixedValue patch without values"; return TOSCA_ERR_ARG; }
     }
+    const int *wf = p->wall_u;
+    const double *aj = m->aj;
     V3 *lucat = (V3 *)ucat_;
     const V3 *csi = (const V3 *)m->csi, *eta = (const V3 *)m->eta, *zet = (const V3 *)m->zet;
     const int64_t mx = l.mx, my = l.my, mz = l.mz;
@@ -1960,6 +2056,37 @@ int oracle_cartesian_bcs(const tosca_dmda_info *info, const tosca_mesh *m, const
     // boundary cell b is extrapolated linearly through the ghost from the next cell c, and the
     // wall-normal part of b is reflected; the slopes dudn0/dudn1 the reference also forms are
     // not used
+    // Cabot wall function on an i (dir 0) or j (dir 1) patch (boundary.c:968-1150)
+    auto cabot = [&](int64_t c, int dir, int64_t gdst, double roughness) {
+        const V3 A = dir == 0 ? csi[c] : eta[c];
+        const double area = sqrt(A.x * A.x + A.y * A.y + A.z * A.z);
+        const double sb = 0.5 / aj[c] / area;
+        const V3 Ub = lucat[c];
+        double ni[3], nj[3], nk[3];
+        unit_normals(csi[c], eta[c], zet[c], ni, nj, nk);
+        const double *nn = dir == 0 ? ni : nj;
+        const V3 n{nn[0], nn[1], nn[2]};
+        const double d = Ub.x * n.x + Ub.y * n.y + Ub.z * n.z;
+        const V3 UbN{d * n.x, d * n.y, d * n.z};
+        const V3 UbP{Ub.x - UbN.x, Ub.y - UbN.y, Ub.z - UbN.z};
+        const double mag = sqrt(UbP.x * UbP.x + UbP.y * UbP.y + UbP.z * UbP.z);
+        const double nu = p->nu;
+        if (mag < 1.e-20) {           // u* is solved for and then reset (its value is not used)
+            ustar[c] = 0.0;
+            ucat[gdst] = V3{0.0, 0.0, 0.0};
+            return;
+        }
+        ustar[c] = roughness > 1.e-19 ? utau_rough(nu, roughness, sb, mag) : utau_cabot(nu, mag, sb);
+        const double tau_w = ustar[c] * ustar[c];
+        const double a1 = mag - 2.0 * sb * tau_w / nu, a2 = -1.0 * mag;
+        const double gm = (a1 < a2) ? a2 : a1;
+        const double vm = sqrt(UbP.x * UbP.x + UbP.y * UbP.y + UbP.z * UbP.z);
+        const V3 dir_{UbP.x / (vm + 1.e-12), UbP.y / (vm + 1.e-12), UbP.z / (vm + 1.e-12)};
+        const V3 G{gm * dir_.x, gm * dir_.y, gm * dir_.z};
+        ucat[gdst] = V3{G.x - UbN.x, G.y - UbN.y, G.z - UbN.z};
+    };
+    const double roughI = wf[0] == -1 ? p->wall_rough[0] : p->wall_rough[1];   // boundary.c:978-979
+    const double roughJ = wf[2] == -1 ? p->wall_rough[2] : p->wall_rough[3];   // :1072-1073
     auto shumann = [&](int64_t c, int64_t cn, int64_t gdst) {
         double ni[3], nj[3], nk[3];
         unit_normals(csi[c], eta[c], zet[c], ni, nj, nk);
@@ -1976,8 +2103,12 @@ int oracle_cartesian_bcs(const tosca_dmda_info *info, const tosca_mesh *m, const
                 const int64_t c = l.L(k, j, i);
                 const int64_t gi0 = l.L(k, j, i - 1), gi1 = l.L(k, j, i + 1), gj0 = l.L(k, j - 1, i),
                               gj1 = l.L(k, j + 1, i), gk0 = l.L(k - 1, j, i), gk1 = l.L(k + 1, j, i);
-                if (bc[2] == TOSCA_BCU_WALLFUNCTION && j == 1) shumann(c, gj1, gj0);
-                if (bc[3] == TOSCA_BCU_WALLFUNCTION && j == my - 2) shumann(c, gj0, gj1);
+                if (bc[0] == TOSCA_BCU_WALLFUNCTION && wf[0] == -1 && i == 1) cabot(c, 0, gi0, roughI);
+                else if (bc[1] == TOSCA_BCU_WALLFUNCTION && wf[1] == -1 && i == mx - 2) cabot(c, 0, gi1, roughI);
+                if (bc[2] == TOSCA_BCU_WALLFUNCTION && wf[2] == -1 && j == 1) cabot(c, 1, gj0, roughJ);
+                else if (bc[3] == TOSCA_BCU_WALLFUNCTION && wf[3] == -1 && j == my - 2) cabot(c, 1, gj1, roughJ);
+                if (bc[2] == TOSCA_BCU_WALLFUNCTION && wf[2] == -3 && j == 1) shumann(c, gj1, gj0);
+                if (bc[3] == TOSCA_BCU_WALLFUNCTION && wf[3] == -3 && j == my - 2) shumann(c, gj0, gj1);
                 if (bc[0] == TOSCA_BCU_FIXEDVALUE && i == 1) fix(0, gi0);
                 if (bc[1] == TOSCA_BCU_FIXEDVALUE && i == mx - 2) fix(1, gi1);
                 if (bc[2] == TOSCA_BCU_FIXEDVALUE && j == 1) fix(2, gj0);

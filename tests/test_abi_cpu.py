@@ -40,15 +40,15 @@ def test_struct_sizes_match_header(tmp_path):
     # tosca_dmda_info: 15 int64; tosca_mesh: 18 pointers; tosca_params: see header field order
     assert C.sizeof(abi.DmdaInfo) == 15 * 8
     assert C.sizeof(abi.Mesh) == 18 * 8
-    assert C.sizeof(abi.Params) == 64 + 48 + 24 + 24 + 16 + 16
+    assert C.sizeof(abi.Params) == 64 + 48 + 24 + 24 + 16 + 16 + 32
     src = r'''
 #include <stdio.h>
 #include <stddef.h>
 #include "tosca_rhs.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu %zu\n", sizeof(tosca_dmda_info), sizeof(tosca_mesh), sizeof(tosca_params),
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(tosca_dmda_info), sizeof(tosca_mesh), sizeof(tosca_params),
          offsetof(tosca_params, nut_value), offsetof(tosca_params, nu), offsetof(tosca_params, tau_wall_i),
-         offsetof(tosca_params, wall_u));
+         offsetof(tosca_params, wall_u), offsetof(tosca_params, wall_rough));
   return 0; }
 '''
     d = tmp_path
@@ -62,6 +62,7 @@ int main(void) {
     assert vals[4] == abi.Params.nu.offset
     assert vals[5] == abi.Params.tau_wall_i.offset
     assert vals[6] == abi.Params.wall_u.offset
+    assert vals[7] == abi.Params.wall_rough.offset
 
 
 def _info(mx=10, my=8, mz=6, **kw):

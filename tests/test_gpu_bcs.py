@@ -80,8 +80,44 @@ def test_cartesian_bcs_shumann(rhs_mod, periodic):
     from tosca_amd.abi import ToscaError
     if not periodic[1]:
         with pytest.raises(ToscaError):
-            ctx.UpdateCartesianBCs({k: to_dev(v) for k, v in mesh.items()}, dict(params, wall_u=(-3, -3, -1, -3)), Ug,
+            ctx.UpdateCartesianBCs({k: to_dev(v) for k, v in mesh.items()}, dict(params, wall_u=(-3, -3, 0, -3)), Ug,
                                    ufix)
+    ctx.close()
+
+
+@pytest.mark.parametrize("rough", ["smooth", "rough", "mixed"])
+@pytest.mark.parametrize("periodic", PERS)
+def test_cartesian_bcs_cabot(rhs_mod, periodic, rough):
+    """velocityWallFunction patches with the Cabot model on every non-periodic i and j patch
+    (smooth walls through the integration table and the Newton solve, rough walls through the log
+    law, or both), the k patches from the other kinds; ucat and ustar compared.  The smooth law's
+    Newton iteration stops at steps below 1e-10 and the table's transcendental functions may round
+    differently on the two sides: bar 1e-12."""
+    seed = 50 + PERS.index(periodic)
+    lay, mesh, params, rng = _case(periodic, seed, (C.BCU_NOSLIP, C.BCU_SLIP, C.BCU_ZEROGRADIENT, C.BCU_FIXEDVALUE))
+    bc_u = tuple(C.BCU_WALLFUNCTION if q < 4 and not periodic[q // 2] else params["bc_u"][q] for q in range(6))
+    ks = {"smooth": (0.0,) * 4, "rough": (0.05, 0.2, 0.01, 1e-12), "mixed": (0.0, 0.3, 0.05, 0.0)}[rough]
+    params = dict(params, bc_u=bc_u, wall_u=(-1, -1, -1, -1), wall_rough=ks, nu=2e-3)
+    print(f"\nUpdateCartesianBCs Cabot periodic={periodic} roughness={ks} bc_u={params['bc_u']}", flush=True)
+    U0, ufix = rng.standard_normal(lay.gshape + (3,)), rng.standard_normal((6, 3))
+    S0 = np.full(lay.gshape, -3.0)
+    Uo, So = U0.copy(), S0.copy()
+    O.cartesian_bcs(lay.info(), mesh, params, Uo, ufix, So)
+    ctx = rhs_mod.RhsContext(lay.info())
+    Ug, Sg = to_dev(U0), to_dev(S0)
+    dm = {k: to_dev(v) for k, v in mesh.items()}
+    for call in range(2):       # the second call reuses the context's table
+        ctx.UpdateCartesianBCs(dm, params, Ug, ufix, Sg)
+        torch.cuda.synchronize()
+        eu, es = relerr(to_np(Ug), Uo), relerr(to_np(Sg), So)
+        print(f"  call {call}: ucat relL2={eu:.2e} identical={np.array_equal(to_np(Ug), Uo)}; "
+              f"ustar relL2={es:.2e} identical={np.array_equal(to_np(Sg), So)}", flush=True)
+        assert eu <= 1e-12 and es <= 1e-12
+        Ug, Sg = to_dev(U0), to_dev(S0)
+    from tosca_amd.abi import ToscaError
+    if any(not periodic[a] for a in range(2)):
+        with pytest.raises(ToscaError):
+            ctx.UpdateCartesianBCs(dm, params, Ug, ufix)                # Cabot patches need ustar
     ctx.close()
 
 

@@ -128,14 +128,154 @@ def test_cartesian_bcs_rejects_wall_and_inlet_patches():
         O.cartesian_bcs(lay.info(), mesh, dict(params, bc_u=(GEN, ZG, ZG, ZG, ZG, ZG)), u)
 
 
-def test_cartesian_bcs_rejects_cabot_and_k_wall_functions():
+def test_cartesian_bcs_rejects_k_wall_functions_and_missing_ustar():
     lay, mesh, params, rng = _setup(random_nodes((5, 5, 5), 15), (0, 0, 0), [ZG, ZG, WALL, ZG, ZG, ZG], 15)
     u = rng.standard_normal(lay.gshape + (3,))
     from tosca_amd.abi import ToscaError
-    with pytest.raises(ToscaError):
+    with pytest.raises(ToscaError):      # Cabot writes ustar
         O.cartesian_bcs(lay.info(), mesh, dict(params, wall_u=(0, 0, -1, 0)), u)
+    with pytest.raises(ToscaError):      # a wall function without a wall model
+        O.cartesian_bcs(lay.info(), mesh, dict(params, wall_u=(0, 0, 0, 0)), u)
     with pytest.raises(ToscaError):
         O.cartesian_bcs(lay.info(), mesh, dict(params, bc_u=(ZG, ZG, ZG, ZG, WALL, ZG), wall_u=(-3,) * 4), u)
+
+
+# ------------------------------------------------------------------------------------------------
+# Cabot wall law
+# ------------------------------------------------------------------------------------------------
+def _inv_nut(y):
+    """1 / (1 + nu_t+) with the near-wall eddy viscosity kappa y+ (1 - exp(-y+/19))^2, kappa 0.41"""
+    return 1.0 / (1.0 + 0.41 * y * (1.0 - np.exp(-y / 19.0)) ** 2)
+
+
+@pytest.mark.parametrize("i", [1, 7, 60, 2500, 40000])
+def test_cabot_table_is_the_integral(i):
+    """table entry i = the integral of 1/(1 + nu_t+) over [0, 2 i] (scipy quad, per unit interval),
+    to the 2.5e-11 relative error of 24 Simpson 3/8 panels per interval"""
+    from scipy.integrate import quad
+    want = sum(quad(_inv_nut, 2.0 * e, 2.0 * e + 2.0, epsabs=1e-16, epsrel=1e-15)[0] for e in range(min(i, 50)))
+    if i > 50:
+        want += quad(_inv_nut, 100.0, 2.0 * i, limit=400, epsabs=1e-13, epsrel=1e-15)[0]
+    assert abs(O.cabot_table(i) - want) <= 3e-11 * want
+
+
+@pytest.mark.parametrize("nu,u,y", [(1.5e-5, 8.0, 5.0), (1e-3, 0.7, 0.05), (1e-2, 2.0, 0.3), (1.5e-5, 0.05, 1e-4),
+                                    (1e-4, 1.0, 2e-3)])
+def test_cabot_utau_solves_the_interpolated_law(nu, u, y):
+    """u* from uTauCabot solves u = u* I(y u*/nu), I the table (entries from scipy quad) interpolated
+    linearly at the even y+ points; the root is found independently with brentq"""
+    from scipy.integrate import quad
+    from scipy.optimize import brentq
+
+    def I(yp):
+        ib = int(yp / 2.0)
+        lo = quad(_inv_nut, 0.0, 2.0 * ib, limit=400, epsabs=1e-13, epsrel=1e-13)[0] if ib else 0.0
+        hi = lo + quad(_inv_nut, 2.0 * ib, 2.0 * ib + 2, epsabs=1e-15, epsrel=1e-15)[0]
+        return (hi - lo) / 2.0 * (yp - 2.0 * ib) + lo
+
+    us = O.cabot_utau(nu, u, y)
+    ref = brentq(lambda t: t * I(y * t / nu) - u, 1e-8, 50.0, xtol=1e-15, rtol=1e-14)
+    assert abs(us - ref) <= 1e-9 * ref
+    yp = y * us / nu
+    if yp < 0.5:              # viscous sublayer: u+ = y+
+        assert abs(us - np.sqrt(nu * u / y)) <= 1e-3 * us
+
+
+def _cabot_case(seed, wall_u, rough, nu=1e-3):
+    bc = [WALL if wall_u[q] else ZG for q in range(4)] + [ZG, NOSLIP]
+    lay, mesh, params, rng = _setup(affine_nodes((6, 7, 5), AFFINE["sheared"]), (0, 0, 0), bc, seed)
+    params = dict(params, wall_u=tuple(wall_u), wall_rough=tuple(rough), nu=nu)
+    u0 = rng.standard_normal(lay.gshape + (3,))
+    return lay, mesh, params, u0
+
+
+@pytest.mark.parametrize("rough", [0.0, 0.02])
+def test_cabot_ghost_carries_the_wall_stress(rough):
+    """on every Cabot patch: u* of the law for the boundary cell's parallel speed at
+    sb = 0.5 / aj / |area| is stored in ustar (on the cells shared by an i and a j patch, the j
+    patch's, which the reference computes last); the ghost's velocity difference to the boundary
+    cell, over the distance 2 sb, times nu, is the wall stress u*^2 along the parallel direction
+    where the clip at -|Ub_par| does not act, and the ghost's parallel speed is -|Ub_par| where it
+    does; the normal part is reflected.  The smooth law always clips: its u+ = U/u* never exceeds
+    y+ = sb u*/nu (the integrand 1/(1 + nu_t+) is at most 1), so the unclipped ghost speed
+    u* (u+ - 2 y+) would be below -|Ub_par|.  The rough case runs at a viscosity where it does not."""
+    nu = 1e-3 if rough == 0.0 else 2.0
+    lay, mesh, params, u0 = _cabot_case(31, (-1, -1, -1, -1), (rough,) * 4, nu)
+    u = u0.copy()
+    ustar = np.full(lay.gshape, -7.0)
+    O.cartesian_bcs(lay.info(), mesh, params, u, None, ustar)
+    m = (lay.mx, lay.my, lay.mz)
+    nu, nchecked = params["nu"], 0
+    vecs = AFFINE["sheared"]
+    for q in range(4):
+        a, side = q // 2, q % 2
+        sg, sc = _plane(lay, a, 0 if side == 0 else m[a] - 1), _plane(lay, a, 1 if side == 0 else m[a] - 2)
+        n = vecs[a] / np.linalg.norm(vecs[a])              # calculateNormal's vector on an affine lattice
+        area = np.linalg.norm(mesh[("csi", "eta")[a]][sc], axis=-1)
+        sb = 0.5 / mesh["aj"][sc] / area
+        ub = u0[sc]
+        unor = (ub @ n)[..., None] * n
+        upar = ub - unor
+        mag = np.linalg.norm(upar, axis=-1)
+        if rough == 0.0:
+            us = np.vectorize(lambda mg, s: O.cabot_utau(nu, mg, s))(mag, sb)
+        else:
+            us = 0.41 * mag / np.log(sb / (0.033 * rough))
+        stored = ustar[sc]
+        if a == 0:            # rows j = 1 and my-2 hold the j patches' u*
+            assert np.abs(stored[:, 1:-1] - us[:, 1:-1]).max() <= 1e-12 * us.max()
+            assert np.abs(stored[:, [0, -1]] - us[:, [0, -1]]).max() > 1e-6 * us.max()
+        else:
+            assert np.abs(stored - us).max() <= 1e-12 * us.max()
+        ug = u[sg]
+        assert np.abs((ug + ub) @ n).max() <= 1e-12
+        d = upar / mag[..., None]
+        dot = lambda x, y: (x * y).sum(-1)
+        stress = nu * dot(ub - ug, d) / (2 * sb)
+        free = mag - 2 * sb * us ** 2 / nu > -mag
+        clipped = -mag * mag / (mag + 1e-12)      # -|Ub_par| along nUnit's direction (its eps 1e-12)
+        if rough == 0.0:
+            assert not free.any() and (sb * us / nu >= mag / us).all()
+            assert np.abs(dot(ug, d) - clipped).max() <= 1e-14
+        else:
+            assert np.abs(stress - us ** 2)[free].max() <= 1e-9 * (us ** 2).max()
+            assert np.abs(dot(ug, d) - clipped)[~free].max(initial=0.0) <= 1e-14
+            nchecked += free.sum()
+        assert np.abs((ug - dot(ug, d)[..., None] * d + unor)).max() <= 1e-12
+    assert rough == 0.0 or nchecked > 20
+    written = ustar != -7.0             # only the boundary cells of the i and j patches
+    for q in range(4):
+        a, side = q // 2, q % 2
+        written[_plane(lay, a, 1 if side == 0 else m[a] - 2)] = False
+    assert not written.any()
+
+
+def test_cabot_rough_wall_laws_and_shared_roughness():
+    """rough walls: u* = kappa U / ln(sb / z0), z0 = 0.033 ks (log law; U the parallel speed), and
+    U / u* = 8.3 (sb u* / nu)^(1/7) for 0.033 ks <= 1e-13 (power law); the right patch of an axis
+    uses the left patch's roughness when both are Cabot"""
+    lay, mesh, params, u0 = _cabot_case(32, (-1, -1, -1, -3), (0.1, 0.5, 1e-12, 0.0))
+    u = u0.copy()
+    ustar = np.zeros(lay.gshape)
+    O.cartesian_bcs(lay.info(), mesh, params, u, None, ustar)
+    m = (lay.mx, lay.my, lay.mz)
+    vecs = AFFINE["sheared"]
+    nu = params["nu"]
+    for q, ks in ((0, 0.1), (1, 0.1), (2, 1e-12)):
+        a, side = q // 2, q % 2
+        sc = _plane(lay, a, 1 if side == 0 else m[a] - 2)
+        n = vecs[a] / np.linalg.norm(vecs[a])
+        area = np.linalg.norm(mesh[("csi", "eta")[a]][sc], axis=-1)
+        sb = 0.5 / mesh["aj"][sc] / area
+        ub = u0[sc]
+        mag = np.linalg.norm(ub - (ub @ n)[..., None] * n, axis=-1)
+        us = ustar[sc]
+        if a == 0:            # row j = 1 holds the jLeft Cabot patch's u*
+            us, mag, sb = us[:, 1:], mag[:, 1:], sb[:, 1:]
+        if 0.033 * ks > 1e-13:
+            assert np.abs(us / 0.41 * np.log(sb / (0.033 * ks)) - mag).max() <= 1e-13 * mag.max()
+        else:
+            assert np.abs(8.3 * us * (sb * us / nu) ** (1 / 7) - mag).max() <= 1e-12 * mag.max()
 
 
 def test_shumann_patch_rule():

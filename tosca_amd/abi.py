@@ -60,6 +60,7 @@ class Params(C.Structure):
         ("tau_wall_i", C.c_void_p),
         ("tau_wall_j", C.c_void_p),
         ("wall_u", C.c_int * 4),
+        ("wall_rough", C.c_double * 4),
     ]
 
 
@@ -118,7 +119,7 @@ SIGNATURES = {
                                          P, D]),
     "tosca_explicit_rhs_u": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P, P, P, P, I, P, P, P, I]),
     "tosca_update_contravariant_bcs": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P]),
-    "tosca_update_cartesian_bcs": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P]),
+    "tosca_update_cartesian_bcs": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P]),
     "tosca_update_temperature_bcs": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P]),
     "tosca_formt": (I, [P, C.POINTER(Mesh), C.POINTER(Params), C.POINTER(TeqnParams), P, P, P, P, P, D, I]),
     "tosca_ueqn_fe": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P, P, P, P, I, P, P, D, P, I]),
@@ -184,6 +185,7 @@ def make_params(d: Mapping) -> Params:
     p.tau_wall_i = d.get("tau_wall_i") or None
     p.tau_wall_j = d.get("tau_wall_j") or None
     p.wall_u[:] = [int(v) for v in d.get("wall_u", (0, 0, 0, 0))]
+    p.wall_rough[:] = [float(v) for v in d.get("wall_rough", (0.0, 0.0, 0.0, 0.0))]
     return p
 
 

@@ -34,6 +34,7 @@ struct tosca_ctx {
     size_t kws_elems = 0;
     double *hs = nullptr;        // device staging of the host-resident step
     size_t hs_elems = 0;
+    double *cabot = nullptr;     // the Cabot wall law's integration table, built on first use
     tosca::KernelTimer timer;
     bool ref_path = false;   // TOSCA_PATH=ref at creation: reference-structured kernels only
 };
@@ -565,6 +566,7 @@ int tosca_ctx_destroy(tosca_ctx *c) {
     if (c->ws || c->chk || c->kws || c->hs) hipStreamSynchronize(c->stream);
     if (c->kws) hipFree(c->kws);
     if (c->hs) hipFree(c->hs);
+    if (c->cabot) hipFree(c->cabot);
     if (c->ws) hipFree(c->ws);
     if (c->chk) hipFree(c->chk);
     delete c;
@@ -1018,25 +1020,50 @@ int tosca_update_contravariant_bcs(tosca_ctx *c, const tosca_mesh *m, const tosc
 }
 
 int tosca_update_cartesian_bcs(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, double *ucat,
-                               const double *u_fixed) {
+                               const double *u_fixed, double *ustar) {
     int rc = check_ctx(c);
     if (rc) return rc;
     if ((rc = check_params(c, p))) return rc;
+    bool cabot = false, smooth = false;
     for (int q = 0; q < 6; q++) {
         if (p->bc_u[q] == TOSCA_BCU_GENERIC)
             return set_err(c, TOSCA_ERR_UNSUPPORTED, "bc_u[%d]: inlet-function velocity patches need the inflow models, "
                                                      "which are not part of this library", q);
-        if (p->bc_u[q] == TOSCA_BCU_WALLFUNCTION && (q >= 4 || p->wall_u[q] != -3))
-            return set_err(c, TOSCA_ERR_UNSUPPORTED, "bc_u[%d] is a wall function with wall model %d: only the Shumann "
-                                                     "model (wall_u = -3) of an i or j patch is supported here; the Cabot "
-                                                     "model (-1) needs its iterative wall-shear solve", q, q < 4 ? p->wall_u[q] : 0);
+        if (p->bc_u[q] == TOSCA_BCU_WALLFUNCTION) {
+            if (q >= 4)
+                return set_err(c, TOSCA_ERR_UNSUPPORTED, "bc_u[%d]: k patches have no velocity wall model", q);
+            if (p->wall_u[q] != -1 && p->wall_u[q] != -3)
+                return set_err(c, TOSCA_ERR_ARG, "bc_u[%d] is a wall function but wall_u[%d] = %d is neither -1 (Cabot) "
+                                                 "nor -3 (Shumann)", q, q, p->wall_u[q]);
+            if (p->wall_u[q] == -1) {
+                cabot = true;
+                const int lead = (p->wall_u[q & ~1] == -1) ? (q & ~1) : q;     // the roughness this patch uses
+                if (!(p->wall_rough[lead] > 1.e-19)) smooth = true;
+            }
+        }
         if (p->bc_u[q] == TOSCA_BCU_FIXEDVALUE && !u_fixed)
             return set_err(c, TOSCA_ERR_ARG, "bc_u[%d] is fixedValue but u_fixed is NULL", q);
     }
     set_geo_per(c, p);
     if ((rc = check_mesh(c, m, false, true, false))) return rc;
-    if ((rc = check_dev(c, ucat, 3 * (size_t)c->geo.nloc() * sizeof(double), "ucat"))) return rc;
-    tosca::launch_cartesian_bcs(ucat, u_fixed, m, p, c->geo, c->stream);
+    const size_t n = (size_t)c->geo.nloc();
+    if ((rc = check_dev(c, ucat, 3 * n * sizeof(double), "ucat"))) return rc;
+    if (cabot) {
+        if (!(p->nu > 0.0)) return set_err(c, TOSCA_ERR_ARG, "Cabot wall function with nu = %g", p->nu);
+        if (!ustar) return set_err(c, TOSCA_ERR_ARG, "a Cabot wall-function patch writes ustar, which is NULL");
+        if ((rc = check_dev(c, m->aj, n * sizeof(double), "mesh.aj")) || (rc = check_dev(c, ustar, n * sizeof(double), "ustar")))
+            return rc;
+    }
+    if (smooth && !c->cabot) {
+        if (hipMalloc(&c->cabot, tosca::cabot_table_doubles() * sizeof(double)) != hipSuccess) {
+            (void)hipGetLastError();
+            c->cabot = nullptr;
+            return set_err(c, TOSCA_ERR_DEVICE, "Cabot table allocation of %zu bytes failed",
+                           tosca::cabot_table_doubles() * sizeof(double));
+        }
+        tosca::launch_cabot_table(c->cabot, c->stream);
+    }
+    tosca::launch_cartesian_bcs(ucat, u_fixed, ustar, c->cabot, m, p, c->geo, c->stream);
     return launched(c, "UpdateCartesianBCs");
 }
 

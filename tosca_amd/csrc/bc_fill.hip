@@ -4,8 +4,11 @@
 //
 //   UpdateContravariantBCs  boundary.c:614-822    -> k_contra_periodic_copy, k_contra_bcs
 //   UpdateCartesianBCs      boundary.c:824-2050   -> k_cart_bcs (fixedValue, slip, noSlip,
-//                                                     zeroGradient, periodic patches, Shumann
+//                                                     zeroGradient, periodic patches, Cabot
+//                                                     wall-function i and j patches, Shumann
 //                                                     wall-function j patches)
+//   Cabot wall law          wallfunctions.c:280-585 -> k_cabot_intervals, k_cabot_prefix (the
+//                                                     integration table), cabot_utau (Newton)
 //   UpdateTemperatureBCs    boundary.c:2052-2628  -> k_temp_bcs (zeroGradient, fixedValue,
 //                                                     fixedGradient, periodic patches, surface
 //                                                     temperature on jLeft)
@@ -22,9 +25,130 @@ namespace {
 
 struct PatchSet {
     int bc[6];          // patch kinds, order iLeft iRight jLeft jRight kLeft kRight
-    int wall[6];        // velocity wall model of the i and j patches (-3 Shumann), 0 elsewhere
+    int wall[6];        // velocity wall model of the i and j patches (-1 Cabot, -3 Shumann), 0 elsewhere
+    double rough[6];    // Cabot roughness each i and j patch uses
     double val[18];     // fixedValue vectors (velocity) or values (temperature, first 6)
 };
+
+// ---- Cabot wall law (wallfunctions.c) --------------------------------------------------------
+// near_wall_eddy_viscosity (wallfunctions.c:280-284): nu_t+ = kappa y+ (1 - exp(-y+/19))^2
+__device__ double near_wall_nut(double yp) { return 0.41 * yp * pow(1. - exp(-yp / 19.), 2.0); }
+
+// integral of 1 / (1 + nu_t+) over the y+ interval [2 (i - 1), 2 i], 24 Simpson 3/8 panels, as
+// pre_integrate forms it (wallfunctions.c:316-355): the panel ends from ya + dy k, the interior
+// points from a running panel start; written to tab[i] and summed in order by k_cabot_prefix
+__global__ void k_cabot_intervals(double *tab, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x + 1;
+    if (i > n) return;
+    const int N = 24;
+    const double ya = (double)(i - 1) * CABOT_INTERVAL, yb = (double)i * CABOT_INTERVAL, dy = (yb - ya) / (double)N;
+    double val = 0, y0 = ya;
+    double E0 = 1. / (1. + near_wall_nut(ya + dy * 0.0));
+    for (int k = 0; k < N; k++) {
+        const double E1 = 1. / (1. + near_wall_nut(ya + dy * (double)(k + 1)));
+        const double F1 = 1. / (1. + near_wall_nut(y0 + dy * 1. / 3.));
+        const double F2 = 1. / (1. + near_wall_nut(y0 + dy * 2. / 3.));
+        val += dy / 3. * (3 * E0 + 9 * F1 + 9 * F2 + 3 * E1) / 8.;
+        y0 += dy;
+        E0 = E1;
+    }
+    tab[i] = val;
+}
+
+// tab[0] = 0, tab[i] = tab[i-1] + interval i (integration_buffer, :313-356) as a chunked running
+// sum: thread c adds the intervals of chunk c in index order (k_cabot_chunk_sums), one thread adds
+// the chunk totals in order (k_cabot_chunk_offsets), and thread c walks its chunk again from that
+// offset (k_cabot_chunk_apply).  The association differs from one running sum only in where the
+// chunk totals are formed.
+constexpr long CABOT_CHUNK = 4096;
+__global__ void k_cabot_chunk_sums(const double *tab, long n, double *sums, long nc) {
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const long i0 = 1 + c * CABOT_CHUNK, i1 = min(n, i0 + CABOT_CHUNK - 1);
+    double s = 0.0;
+#pragma unroll 16
+    for (long i = i0; i <= i1; i++) s = s + tab[i];
+    sums[c] = s;
+}
+
+__global__ void k_cabot_chunk_offsets(double *sums, long nc) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double s = 0.0;
+    for (long c = 0; c < nc; c++) {
+        const double v = sums[c];
+        sums[c] = s;
+        s = s + v;
+    }
+}
+
+__global__ void k_cabot_chunk_apply(double *tab, long n, const double *sums, long nc) {
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    if (c == 0) tab[0] = 0.0;
+    const long i0 = 1 + c * CABOT_CHUNK, i1 = min(n, i0 + CABOT_CHUNK - 1);
+    double s = sums[c];
+#pragma unroll 16
+    for (long i = i0; i <= i1; i++) {
+        s = s + tab[i];
+        tab[i] = s;
+    }
+}
+
+// integrate_F (wallfunctions.c:364-420) divided out: returns the integral of 1 / (1 + nu_t+) from
+// 0 to y+ = yb utau / nu, over utau.  Below y+ = 1e7 it interpolates the table linearly; above, the
+// reference adds 4 panels from 1e7 to y+ to integration_buffer[n_yp], whose local n_yp is 0 on
+// every call but the process's first (pre_integrate sets it only once), so to 0.  Where the
+// reference would index outside its table (y+ < 0, or y+ at the table's last point) the index is
+// clamped to the table.
+__device__ double cabot_F(const double *tab, double nu, double utau, double yb) {
+    const double yp = yb * utau / nu;
+    if (yp <= (double)CABOT_MAX_YP) {
+        const double qb = yp / (double)CABOT_INTERVAL;
+        long ib = qb > 0.0 ? (long)qb : 0;              // (PetscInt) truncation; no conversion of y+ < 0
+        ib = ib > CABOT_NYP - 1 ? CABOT_NYP - 1 : ib;
+        const double int_b = (tab[ib + 1] - tab[ib]) / (double)CABOT_INTERVAL * (yp - (double)ib * CABOT_INTERVAL) + tab[ib];
+        return (int_b - 0.0) / utau;
+    }
+    const int N = 4;
+    const double ya = (double)CABOT_MAX_YP, dy = (yp - ya) / (double)N;
+    double val = 0.0, y0 = ya;
+    double E0 = 1. / (1. + near_wall_nut(ya + dy * 0.0));
+    for (int k = 0; k < N; k++) {
+        const double E1 = 1. / (1. + near_wall_nut(ya + dy * (double)(k + 1)));
+        const double F1 = 1. / (1. + near_wall_nut(y0 + dy * 1. / 3.));
+        const double F2 = 1. / (1. + near_wall_nut(y0 + dy * 2. / 3.));
+        val += dy / 3. * (3 * E0 + 9 * F1 + 9 * F2 + 3 * E1) / 8.;
+        y0 += dy;
+        E0 = E1;
+    }
+    return val / utau;
+}
+
+__device__ double cabot_f(const double *tab, double nu, double u, double y, double utau) {   // f_Cabot, :467-470
+    return utau * utau * cabot_F(tab, nu, utau, y) - u;
+}
+
+// uTauCabot (wallfunctions.c:519-543): Newton from 0.01 with a central-difference derivative
+// (step 1e-7), at most 30 steps, stopping when a step is below 1e-10
+__device__ double cabot_utau(const double *tab, double nu, double u, double y) {
+    double x = 0.0, x0 = 0.01;
+    const double eps = 1.e-7;
+    for (int it = 0; it < 30; it++) {
+        const double df = (cabot_f(tab, nu, u, y, x0 + eps) - cabot_f(tab, nu, u, y, x0 - eps)) / (2.0 * eps);
+        x = x0 - cabot_f(tab, nu, u, y, x0) / df;
+        if (fabs(x0 - x) < 1.e-10) break;
+        x0 = x;
+    }
+    return x;
+}
+
+// utau_wf (wallfunctions.c:568-585): log law over 0.033 ks, or a 1/7 power law for tiny ks
+__device__ double rough_utau(double nu, double ks, double sb, double ut) {
+    const double k_s = ks * 0.033, kappa = 0.41;
+    if (k_s > 1.e-13) return ut * kappa / log(sb / k_s);
+    const double A = 8.3, B = 1.0 / 7.0;
+    return pow(ut * pow(nu, B) / (A * pow(sb, B)), 1.0 / (1.0 + B));
+}
 
 __device__ __forceinline__ void across(int a, int &t1, int &t2) {
     t1 = (a == 0) ? 1 : 0;
@@ -104,7 +228,8 @@ __device__ V3 unit_normal(V3 cs, V3 et, V3 ze, int a) {
 }
 
 // ghost cell of a patch next to an interior boundary cell (transverse indices interior)
-__global__ void k_cart_bcs(double *ucat, const double *csi, const double *eta, const double *zet, Geo g, PatchSet P) {
+__global__ void k_cart_bcs(double *ucat, const double *csi, const double *eta, const double *zet, const double *aj,
+                           double *ustar, const double *tab, double nu, Geo g, PatchSet P) {
     const int q = blockIdx.z, a = q >> 1, side = q & 1;
     const int m[3] = {g.mx, g.my, g.mz};
     int t1, t2;
@@ -130,6 +255,32 @@ __global__ void k_cart_bcs(double *ucat, const double *csi, const double *eta, c
     } else if (b == TOSCA_BCU_NOSLIP) {                                  // :1832-1867
         const V3 U = ld3(ucat, ci);
         out = V3{-1.0 * U.x, -1.0 * U.y, -1.0 * U.z};
+    } else if (b == TOSCA_BCU_WALLFUNCTION && a <= 1 && P.wall[q] == -1) {   // Cabot, :968-1150
+        const V3 cs = ld3(csi, ci), et = ld3(eta, ci), ze = ld3(zet, ci), A = (a == 0) ? cs : et;
+        const double area = sqrt(A.x * A.x + A.y * A.y + A.z * A.z);
+        const double sb = 0.5 / aj[ci] / area;
+        const V3 Ub = ld3(ucat, ci), n = unit_normal(cs, et, ze, a);
+        const double d = Ub.x * n.x + Ub.y * n.y + Ub.z * n.z;
+        const V3 bN{d * n.x, d * n.y, d * n.z};
+        const V3 bP{Ub.x - bN.x, Ub.y - bN.y, Ub.z - bN.z};
+        const double mag = sqrt(bP.x * bP.x + bP.y * bP.y + bP.z * bP.z);
+        // a cell on an i and a j Cabot patch keeps the j patch's u*, which the reference computes
+        // after the i patch's; the ghost of each patch uses its own
+        const bool jwins = a == 0 && ((c[1] == 1 && P.bc[2] == TOSCA_BCU_WALLFUNCTION && P.wall[2] == -1) ||
+                                      (c[1] == m[1] - 2 && P.bc[3] == TOSCA_BCU_WALLFUNCTION && P.wall[3] == -1));
+        if (mag < 1.e-20) {                  // the reference solves for u* and then discards it
+            if (!jwins) ustar[ci] = 0.0;
+            out = V3{0.0, 0.0, 0.0};
+        } else {
+            const double us = P.rough[q] > 1.e-19 ? rough_utau(nu, P.rough[q], sb, mag) : cabot_utau(tab, nu, mag, sb);
+            if (!jwins) ustar[ci] = us;
+            const double tau_w = us * us;
+            const double a1 = mag - 2.0 * sb * tau_w / nu, a2 = -1.0 * mag;
+            const double gmag = a1 < a2 ? a2 : a1;                          // PetscMax
+            const double um = sqrt(bP.x * bP.x + bP.y * bP.y + bP.z * bP.z) + 1.e-12;   // nUnit
+            const V3 dir{bP.x / um, bP.y / um, bP.z / um};
+            out = V3{gmag * dir.x - bN.x, gmag * dir.y - bN.y, gmag * dir.z - bN.z};
+        }
     } else if (b == TOSCA_BCU_WALLFUNCTION && a == 1 && P.wall[q] == -3) {   // Shumann, :1153-1224
         // wall-parallel part extrapolated linearly from the next cell, wall-normal part reflected
         const V3 n = unit_normal(ld3(csi, ci), ld3(eta, ci), ld3(zet, ci), 1);
@@ -215,12 +366,30 @@ void launch_contravariant_bcs(double *ucont, const double *ucat, const tosca_mes
     launch_wrap(ucont, 3, g, s);
 }
 
-void launch_cartesian_bcs(double *ucat, const double *u_fixed, const tosca_mesh *m, const tosca_params *p,
-                          const Geo &g, hipStream_t s) {
+size_t cabot_table_doubles() { return (size_t)CABOT_NYP + 1 + (size_t)((CABOT_NYP + CABOT_CHUNK - 1) / CABOT_CHUNK); }
+
+void launch_cabot_table(double *tab, hipStream_t s) {
+    const long n = CABOT_NYP, nc = (n + CABOT_CHUNK - 1) / CABOT_CHUNK;
+    double *sums = tab + n + 1;
+    TOSCA_LAUNCH("k_cabot_intervals", k_cabot_intervals, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, tab, n);
+    TOSCA_LAUNCH("k_cabot_chunk_sums", k_cabot_chunk_sums, dim3((unsigned)((nc + 63) / 64)), dim3(64), 0, s, tab, n, sums, nc);
+    TOSCA_LAUNCH("k_cabot_chunk_offsets", k_cabot_chunk_offsets, dim3(1), dim3(64), 0, s, sums, nc);
+    TOSCA_LAUNCH("k_cabot_chunk_apply", k_cabot_chunk_apply, dim3((unsigned)((nc + 63) / 64)), dim3(64), 0, s, tab, n, sums, nc);
+}
+
+void launch_cartesian_bcs(double *ucat, const double *u_fixed, double *ustar, const double *cabot_tab,
+                          const tosca_mesh *m, const tosca_params *p, const Geo &g, hipStream_t s) {
     const int n = maxdim(g);
     PatchSet P = patches(p->bc_u, u_fixed, 18);
     for (int q = 0; q < 4; q++) P.wall[q] = p->wall_u[q];
-    TOSCA_LAUNCH("k_cart_bcs", k_cart_bcs, dim3((n + 127) / 128, n, 6), dim3(128), 0, s, ucat, m->csi, m->eta, m->zet, g, P);
+    // the roughness a Cabot patch uses: the right patch of an axis reads the left patch's model
+    // when that one is Cabot too (boundary.c:978-979, 1072-1073)
+    for (int a = 0; a < 2; a++) {
+        const double r = p->wall_u[2 * a] == -1 ? p->wall_rough[2 * a] : p->wall_rough[2 * a + 1];
+        P.rough[2 * a] = P.rough[2 * a + 1] = r;
+    }
+    TOSCA_LAUNCH("k_cart_bcs", k_cart_bcs, dim3((n + 127) / 128, n, 6), dim3(128), 0, s, ucat, m->csi, m->eta, m->zet,
+                 m->aj, ustar, cabot_tab, p->nu, g, P);
     launch_wrap(ucat, 3, g, s);
 }
 

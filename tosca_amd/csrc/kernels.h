@@ -46,7 +46,12 @@ void launch_local_to_global(double *glob, const double *loc, const Geo &g, hipSt
 // boundary-condition ghost fills (bc_fill.hip)
 void launch_contravariant_bcs(double *ucont, const double *ucat, const tosca_mesh *m, const tosca_params *p,
                               const Geo &g, hipStream_t s);
-void launch_cartesian_bcs(double *ucat, const double *u_fixed, const tosca_mesh *m, const tosca_params *p,
+// the Cabot wall law's integration table: y+ in [0, 1e7] at steps of 2 (wallfunctions.c:296-360)
+constexpr long CABOT_INTERVAL = 2, CABOT_MAX_YP = 10000000, CABOT_NYP = CABOT_MAX_YP / CABOT_INTERVAL;
+size_t cabot_table_doubles();       // the table (CABOT_NYP + 1) and the chunk totals of its prefix sum
+void launch_cabot_table(double *tab, hipStream_t s);
+void launch_cartesian_bcs(double *ucat, const double *u_fixed, double *ustar, const double *cabot_tab,
+                          const tosca_mesh *m, const tosca_params *p,
                           const Geo &g, hipStream_t s);
 void launch_temperature_bcs(double *t, const int *bc_t, const double *t_value, const tosca_mesh *m, const Geo &g,
                             hipStream_t s);
