@@ -270,14 +270,80 @@ def explicit_leg(torch, ctx, mesh, case, ucont, ucat, nut, lay, args):
     imex_ms = e0.elapsed_time(e1)
     print(f"# ueqn_rk3cn (RK3WCN): {imex_ms:.1f} ms per predictor step, last stage GMRES {its} iterations, "
           f"residual {res:.2e}", file=sys.stderr, flush=True)
-    del dp, bt, bto, farm, uo, V, U, rhs, utmp
+    # the Crank-Nicolson residual the matrix-free Newton solve evaluates (UeqnSNES's constant
+    # right-hand side once per step, SNESFuncEval per Newton and finite-difference evaluation)
+    bU, F = torch.empty(gl, dtype=torch.float64, device=dev), torch.empty(gl, dtype=torch.float64, device=dev)
+    ctx.CNRhs(case.params, dp, bt, bto, 1, rhs, farm, bU)
+    u = uo + 1e-3 * rhs
+
+    def cn_time(fn):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(args.steps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / args.steps
+
+    cn_ms = cn_time(lambda: ctx.CNRhs(case.params, dp, bt, bto, 1, rhs, farm, bU))
+    fe_ms = cn_time(lambda: ctx.SNESFuncEval(mesh, case.params, u, V, U, nut, bU, uo, 1e-3, 1, F))
+    print(f"# cn_rhs: {cn_ms:.3f} ms; snes_func_eval: {fe_ms:.3f} ms ({fe_ms / ms:.2f} x ExplicitRhsU)", file=sys.stderr,
+          flush=True)
+    del dp, bt, bto, farm, uo, V, U, rhs, utmp, bU, F, u
     return {"ms": ms, "value": value, "unit": "cell-updates/s", "algorithmic_bytes_per_cell": EXPLICIT_BYTES,
             "achieved": gbs, "peak": HBM_PEAK_GBS, "frac": gbs / HBM_PEAK_GBS,
             "kernels_ms": {k: v[0] / v[1] for k, v in tim.items()},
             "terms": "-dP + 3/2 bTheta - 1/2 bTheta_o + farm + FormU(formMode 0), synthetic sources",
             "ueqn_rk4_ms": rk_ms,
             "ueqn_rk3cn_w": {"ms": imex_ms, "dt": dt_imex, "last_stage_gmres_iterations": its, "residual": res,
-                             "reason": reason}}
+                             "reason": reason},
+            "cn_rhs_ms": cn_ms, "snes_func_eval_ms": fe_ms}
+
+
+def boundary_leg(torch, ctx, mesh, case, ucont, ucat, theta, lay, args):
+    """The ghost fills before the path on the shard (periodic in x and y, walls at the bottom and
+    top): UpdateContravariantBCs, UpdateCartesianBCs with the bottom patch as noSlip, as a Shumann
+    and as a Cabot wall function (smooth: the integration table is built in the warm-up; rough),
+    and UpdateTemperatureBCs.  They touch the boundary planes only, so they are latency-bound; the
+    times are what they add to a step."""
+    from tosca_amd import case as C
+    dev = "cuda"
+    V, U, T = ucont.clone(), ucat.clone(), theta.clone()
+    ustar = torch.zeros(lay.gshape, dtype=torch.float64, device=dev)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(args.steps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / args.steps
+
+    p = case.params
+    bc = list(p["bc_u"])
+    out = {"contravariant_ms": timed(lambda: ctx.UpdateContravariantBCs(mesh, p, V, U))}
+    out["cartesian_noslip_slip_ms"] = timed(lambda: ctx.UpdateCartesianBCs(mesh, p, U))
+    wf = tuple(C.BCU_WALLFUNCTION if q == 2 else bc[q] for q in range(6))
+    for name, extra in (("shumann", dict(wall_u=(0, 0, -3, 0))),
+                        ("cabot_smooth", dict(wall_u=(0, 0, -1, 0), wall_rough=(0.0,) * 4)),
+                        ("cabot_rough", dict(wall_u=(0, 0, -1, 0), wall_rough=(0.0, 0.0, 0.1, 0.0)))):
+        pw = dict(p, bc_u=wf, **extra)
+        out[f"cartesian_{name}_ms"] = timed(lambda: ctx.UpdateCartesianBCs(mesh, pw, U, None, ustar))
+    bct = [C.BCT_PERIODIC, C.BCT_PERIODIC, C.BCT_ZEROGRADIENT, C.BCT_FIXEDGRADIENT, C.BCT_PERIODIC, C.BCT_PERIODIC]
+    tval = [0.0, 0.0, 0.0, 0.003, 0.0, 0.0]
+    out["temperature_ms"] = timed(lambda: ctx.UpdateTemperatureBCs(mesh, p, bct, tval, T))
+    out["ghost_points"] = {"i": 2 * (lay.my - 2) * (lay.mz - 2), "j": 2 * (lay.mx - 2) * (lay.mz - 2),
+                           "k": 2 * (lay.mx - 2) * (lay.my - 2)}
+    print("# ghost fills: " + ", ".join(f"{k[:-3]} {v:.3f} ms" for k, v in out.items() if k.endswith("_ms")),
+          file=sys.stderr, flush=True)
+    del V, U, T, ustar
+    return out
 
 
 # FormT, per interior cell: theta 8, ucont 24, nine face vectors 216, three face Jacobians 24,
@@ -527,6 +593,7 @@ def main():
     line["vjp"] = vjp_leg(torch, ctx, mesh, case, ucont, ucat, theta, cs, lay, args, ms)
     line["explicit_rhs_u"] = explicit_leg(torch, ctx, mesh, case, ucont, ucat, nut, lay, args)
     line["formt"] = formt_leg(torch, ctx, mesh, case, ucont, theta, nut, lay, args)
+    line["ghost_fills"] = boundary_leg(torch, ctx, mesh, case, ucont, ucat, theta, lay, args)
     line["host_staged"] = host_leg(torch, ctx, mesh, case, ucont, ucat, theta, lay, args, ms)
     if not args.no_cpu:
         spec = C.CONFIGS[args.config]

@@ -895,18 +895,10 @@ int tosca_cn_rhs(tosca_ctx *c, const tosca_params *p, const double *dp, const do
     if (farm && (rc = check_dev(c, farm, vb, "farm"))) return rc;
     if (bU == dp || bU == btheta || bU == btheta_o || bU == rhs_o || bU == farm)
         return set_err(c, TOSCA_ERR_ARG, "bU must not alias an input");
-    hipStream_t s = c->stream;
-    tosca::launch_scale_to(bU, dp, -1.0, n, s);                              // ueqn_solvers.c:1102-1103
-    if (btheta) {                                                             // :1125-1135
-        if (ab2) {
-            tosca::launch_axpy(bU, 3.0 / 2.0, btheta, n, s);
-            tosca::launch_axpy(bU, -1.0 / 2.0, btheta_o, n, s);
-        } else {
-            tosca::launch_axpy(bU, 1.0, btheta, n, s);
-        }
-    }
-    if (ab2) tosca::launch_axpy(bU, 0.5, rhs_o, n, s);                        // :1151-1152
-    if (farm) tosca::launch_axpy(bU, 1.0, farm, n, s);                        // :1155-1156
+    // -dP (ueqn_solvers.c:1102-1103), buoyancy AB2 or first step (:1125-1135), 1/2 Rhs_o after the
+    // first step (:1151-1152), farm (:1155-1156), in one pass
+    tosca::launch_cn_rhs(bU, dp, btheta, (btheta && ab2) ? btheta_o : nullptr, ab2 ? rhs_o : nullptr, farm,
+                         ab2 ? 3.0 / 2.0 : 1.0, n, c->stream);
     return launched(c, "UeqnSNES constant RHS");
 }
 

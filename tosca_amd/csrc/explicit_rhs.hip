@@ -160,6 +160,22 @@ __global__ void k_axpy(double *y, double a, const double *x, long n) {
     y[t] = y[t] + a * x[t];
 }
 
+// UeqnSNES's constant right-hand side in one pass: bU = -dP (+ 3/2 bTheta - 1/2 bTheta_o, or
+// + bTheta) (+ 1/2 Rhs_o) (+ farm), each term added to the running value in the order of the
+// reference's VecScale / VecAXPY sequence (ueqn_solvers.c:1102-1156), so per element the result
+// equals the one of the separate passes; null pointers skip their terms
+__global__ void k_cn_rhs(double *bU, const double *dp, const double *bt, const double *bto, const double *rhs_o,
+                         const double *farm, double wbt, long n) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    double y = dp[t] * -1.0;
+    if (bt) y = y + wbt * bt[t];
+    if (bto) y = y + -1.0 / 2.0 * bto[t];
+    if (rhs_o) y = y + 0.5 * rhs_o[t];
+    if (farm) y = y + 1.0 * farm[t];
+    bU[t] = y;
+}
+
 // the owned-range copy of DMGlobalToLocal for a vector field
 __global__ void k_global_to_local(double *loc, const double *glob, Geo g, long n) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -197,6 +213,12 @@ void launch_waxpy_local(double *ucont, double al, const double *rhs, const doubl
 
 void launch_axpy(double *y, double a, const double *x, long n, hipStream_t s) {
     TOSCA_LAUNCH("k_axpy", k_axpy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, y, a, x, n);
+}
+
+void launch_cn_rhs(double *bU, const double *dp, const double *bt, const double *bto, const double *rhs_o,
+                   const double *farm, double wbt, long n, hipStream_t s) {
+    TOSCA_LAUNCH("k_cn_rhs", k_cn_rhs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, bU, dp, bt, bto, rhs_o, farm,
+                 wbt, n);
 }
 
 void launch_global_to_local(double *loc, const double *glob, const Geo &g, hipStream_t s) {
