@@ -930,10 +930,7 @@ int tosca_snes_func_eval(tosca_ctx *c, const tosca_mesh *m, const tosca_params *
         tosca::FormUArgs A = formu_args(m, p, ucont, ucat, nu_t, F, scale, 0);
         tosca::fused_formu(g, A, c->ws + 3 * (size_t)g.nloc(), s);
     }
-    tosca::launch_scale_to(F, F, dt, n, s);                                   // :1264
-    tosca::launch_non_resolved(F, ucont_o, p, g, s);                          // :1267
-    tosca::launch_axpy(F, -1.0, u, n, s);                                     // :1270
-    tosca::launch_axpy(F, 1.0, ucont_o, n, s);                                // :1271
+    tosca::launch_snes_finish(F, u, ucont_o, dt, p, g, s);                    // :1264-1271
     return launched(c, "SNESFuncEval");
 }
 

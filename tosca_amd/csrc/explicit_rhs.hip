@@ -176,6 +176,30 @@ __global__ void k_cn_rhs(double *bU, const double *dp, const double *bt, const d
     bU[t] = y;
 }
 
+// The end of SNESFuncEval in one pass (ueqn_solvers.c:1264-1271): F = dt F, the non-resolved
+// face reset of k_non_resolved (same rule, evaluated per element), then F - u + ucont_o, in the
+// order of the reference's VecScale / reset / VecAXPY sequence
+__global__ void k_snes_finish(double *F, const double *u, const double *ucont_o, double dt, Geo g, PatchKinds P, long n) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int a = (int)(t % 3);
+    const long q = t / 3;
+    const int i = (int)(q % g.mx);
+    const long r = q / g.mx;
+    const int j = (int)(r % g.my), k = (int)(r / g.my);
+    const int idx = (a == 0) ? i : (a == 1) ? j : k, m = (a == 0) ? g.mx : (a == 1) ? g.my : g.mz;
+    double y = F[t] * dt;
+    const double uo = ucont_o[t];
+    bool zero = false;
+    if (idx == 0) zero = !(P.bc[2 * a] == TOSCA_BCU_ZEROGRADIENT || uo < 0);
+    else if (idx == m - 1) zero = true;
+    else if (idx == m - 2 && g.per[a] == TOSCA_PERIODIC_NONE) zero = !(P.bc[2 * a + 1] == TOSCA_BCU_ZEROGRADIENT || uo > 0);
+    if (zero) y = 0.0;
+    y = y + -1.0 * u[t];
+    y = y + 1.0 * uo;
+    F[t] = y;
+}
+
 // the owned-range copy of DMGlobalToLocal for a vector field
 __global__ void k_global_to_local(double *loc, const double *glob, Geo g, long n) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -224,6 +248,13 @@ void launch_cn_rhs(double *bU, const double *dp, const double *bt, const double 
 void launch_global_to_local(double *loc, const double *glob, const Geo &g, hipStream_t s) {
     const long n = 3L * g.mx * g.my * g.mz;
     TOSCA_LAUNCH("k_global_to_local", k_global_to_local, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, loc, glob, g, n);
+}
+
+void launch_snes_finish(double *F, const double *u, const double *ucont_o, double dt, const tosca_params *p,
+                        const Geo &g, hipStream_t s) {
+    const long n = 3L * g.mx * g.my * g.mz;
+    TOSCA_LAUNCH("k_snes_finish", k_snes_finish, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, F, u, ucont_o, dt, g,
+                 kinds(p), n);
 }
 
 void launch_non_resolved(double *rhs, const double *ucont_o, const tosca_params *p, const Geo &g, hipStream_t s) {
