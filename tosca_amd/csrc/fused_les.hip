@@ -183,63 +183,75 @@ __device__ __forceinline__ bool remap_axis(int &I, int centre, int m, int ptype,
 }
 }  // namespace
 
-// Box average over an LDS tile: BX x BY threads own BX x BY x BZ output cells (BZ planes each).
-// The weights 1/aj and the remapped LM/MM of the tile plus a one-cell halo are staged once per
-// point (one division per point instead of 27 per output cell); the per-point rules do not depend
-// on the centre cell, because the centre index j of les.c:1911 never equals my-1.
-constexpr int BX = 32, BY = 8, BZ = 4;
-constexpr int HX = BX + 2, HY = BY + 2, HZ = BZ + 2, HN = HX * HY * HZ;
+// Box average, marching along k: a workgroup of CX x CY threads owns a CX x CY column of output
+// cells over a chunk of k planes.  Per plane the weights 1/aj and the remapped LM w, MM w of the
+// tile plus a one-cell i/j halo are staged once per point in LDS (double-buffered); each thread
+// forms the 3 x 3 Simpson-weighted sums of its point in the plane and keeps those of the last
+// three planes in registers, so the k direction costs no reloads.  The products are the
+// reference's (the weights are powers of two, so (wR wQ wP) w L = wR ((wQ wP) w) L exactly); the
+// 27 terms are summed plane by plane.  The per-point rules do not depend on the centre cell,
+// because the centre index j of les.c:1911 never equals my-1.
+constexpr int CX = 64, CY = 8, HX = CX + 2, HY = CY + 2, HP = HX * HY;
 
 template <class T>
-__global__ void __launch_bounds__(BX * BY) k_dsm_cs_nut(Geo g, const double *LM, const double *MM, const double *Sabs,
-                                                        const double *aj, double maxCs, double *Cs, T *nut) {
-    __shared__ double W[HN], Lw[HN], Mw[HN];
-    const int tid = threadIdx.x + BX * threadIdx.y;
-    const int tile = xcd_tile();     // i fastest, then j, then k
-    const int i0 = 1 + (tile % gridDim.x) * BX, j0 = 1 + ((tile / gridDim.x) % gridDim.y) * BY,
-              k0 = 1 + (tile / (gridDim.x * gridDim.y)) * BZ;
-    for (int idx = tid; idx < HN; idx += BX * BY) {
-        const int ri = idx % HX, rj = (idx / HX) % HY, rk = idx / (HX * HY);
-        int I = i0 - 1 + ri, J = j0 - 1 + rj, K = k0 - 1 + rk;
-        double w = 0.0, l = 0.0, mm = 0.0;
-        if (I <= g.mx - 1 && J <= g.my - 1 && K <= g.mz - 1) {     // points past the owned range feed no output
-            w = 1. / aj[g.L(K, J, I)];
-            bool ghost = false;
-            bool ok = remap_axis(I, 1, g.mx, g.per[0], false, ghost);
-            ok = remap_axis(J, 1, g.my, g.per[1], true, ghost) && ok;
-            ok = remap_axis(K, 1, g.mz, g.per[2], false, ghost) && ok;
-            if (!ok) w = 0;
-            if (!ghost && w != 0) { const int c = g.L(K, J, I); l = LM[c]; mm = MM[c]; }
-        }
-        W[idx] = w;
-        Lw[idx] = l;
-        Mw[idx] = mm;
-    }
-    __syncthreads();
+__global__ void __launch_bounds__(CX * CY) k_dsm_cs_nut(Geo g, const double *LM, const double *MM, const double *Sabs,
+                                                        const double *aj, double maxCs, double *Cs, T *nut, int kchunk) {
+    __shared__ double W[2][HP], Lw[2][HP], Mw[2][HP];
+    const int tid = threadIdx.x + CX * threadIdx.y;
+    const int tile = xcd_tile();     // i fastest, then j, then the k chunk
+    const int bi = tile % gridDim.x, bj = (tile / gridDim.x) % gridDim.y, bk = tile / (gridDim.x * gridDim.y);
+    const int i0 = 1 + bi * CX, j0 = 1 + bj * CY;
+    const int kb = 1 + bk * kchunk, ke = min(kb + kchunk, g.mz - 1);     // output planes [kb, ke)
     const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
-    if (i > g.mx - 2 || j > g.my - 2) return;
-    for (int t = 0; t < BZ; t++) {
-        const int k = k0 + t;
-        if (k > g.mz - 2) return;
-        double wsum = 0, lsum = 0, msum = 0;
-        for (int R = 0; R < 3; R++)
-            for (int Q = 0; Q < 3; Q++)
-                for (int P = 0; P < 3; P++) {
-                    const int n = ((t + R) * HY + (threadIdx.y + Q)) * HX + (threadIdx.x + P);
-                    const double sw = (R == 1 ? 4. : 1.) * (Q == 1 ? 4. : 1.) * (P == 1 ? 4. : 1.);
-                    const double w = W[n];
-                    wsum += sw * w;
-                    lsum += sw * w * Lw[n];
-                    msum += sw * w * Mw[n];
-                }
-        const double C = 0.5 * (lsum / wsum) / ((msum / wsum) + 1e-10);
-        const int c = g.L(k, j, i);
-        const double cs = pmin(pmax(C, 0.0), maxCs);
-        Cs[c] = cs;
-        if (nut) {
-            const double filter = cube_root(1. / aj[c]);
-            nut[c] = T(cs * (filter * filter) * Sabs[c]);
+    const bool own = i <= g.mx - 2 && j <= g.my - 2;
+    double pw[2] = {0, 0}, pl[2] = {0, 0}, pm[2] = {0, 0};     // 2-D sums of planes K-2, K-1
+    int b = 0;
+    for (int K = kb - 1; K <= ke; K++) {
+        for (int idx = tid; idx < HP; idx += CX * CY) {
+            int I = i0 - 1 + idx % HX, J = j0 - 1 + idx / HX, KK = K;
+            double w = 0.0, l = 0.0, mm = 0.0;
+            if (I <= g.mx - 1 && J <= g.my - 1) {     // points past the owned range feed no output
+                w = 1. / aj[g.L(KK, J, I)];
+                bool ghost = false;
+                bool ok = remap_axis(I, 1, g.mx, g.per[0], false, ghost);
+                ok = remap_axis(J, 1, g.my, g.per[1], true, ghost) && ok;
+                ok = remap_axis(KK, 1, g.mz, g.per[2], false, ghost) && ok;
+                if (!ok) w = 0;
+                if (!ghost && w != 0) { const int c = g.L(KK, J, I); l = LM[c]; mm = MM[c]; }
+            }
+            W[b][idx] = w;
+            Lw[b][idx] = l;
+            Mw[b][idx] = mm;
         }
+        __syncthreads();
+        // 3 x 3 sums of this plane at (i, j), Q (j) outer, P (i) inner, as the reference's loops
+        double sw = 0, sl = 0, sm = 0;
+        for (int Q = 0; Q < 3; Q++)
+            for (int P = 0; P < 3; P++) {
+                const int n = (threadIdx.y + Q) * HX + (threadIdx.x + P);
+                const double wq = (Q == 1 ? 4. : 1.) * (P == 1 ? 4. : 1.);
+                const double w = W[b][n];
+                sw += wq * w;
+                sl += wq * w * Lw[b][n];
+                sm += wq * w * Mw[b][n];
+            }
+        const int k = K - 1;
+        if (own && k >= kb && k < ke) {
+            const double wsum = (pw[0] + 4. * pw[1]) + sw, lsum = (pl[0] + 4. * pl[1]) + sl,
+                         msum = (pm[0] + 4. * pm[1]) + sm;
+            const double C = 0.5 * (lsum / wsum) / ((msum / wsum) + 1e-10);
+            const int c = g.L(k, j, i);
+            const double cs = pmin(pmax(C, 0.0), maxCs);
+            Cs[c] = cs;
+            if (nut) {
+                const double filter = cube_root(1. / aj[c]);
+                nut[c] = T(cs * (filter * filter) * Sabs[c]);
+            }
+        }
+        pw[0] = pw[1]; pw[1] = sw;
+        pl[0] = pl[1]; pl[1] = sl;
+        pm[0] = pm[1]; pm[1] = sm;
+        b ^= 1;       // the next plane stages into the other buffer; the barrier above orders the reuse
     }
 }
 
@@ -312,8 +324,13 @@ void fused_dsm(const Geo &g, const tosca_mesh *m, const T *ucat, double maxCs, d
     dim3 grid((g.mx - 2 + DSM_TI - 1) / DSM_TI, (g.my - 2 + DSM_TJ - 1) / DSM_TJ, chunks);
     TOSCA_LAUNCH(sizeof(T) == 8 ? "k_dsm_lm" : "k_dsm_lm_f32", k_dsm_lm<T>, grid, dim3(64, DSM_NR), 0, s, g, ucat, m->csi, m->eta, m->zet, m->aj, LM, MM,
                  Sabs, kchunk);
-    TOSCA_LAUNCH(sizeof(T) == 8 ? "k_dsm_cs_nut" : "k_dsm_cs_nut_f32", k_dsm_cs_nut<T>, dim3((g.mx - 2 + BX - 1) / BX, (g.my - 2 + BY - 1) / BY, (g.mz - 2 + BZ - 1) / BZ),
-                 dim3(BX, BY), 0, s, g, LM, MM, Sabs, m->aj, maxCs, cs, nut);
+    const int ctiles = ((g.mx - 2 + CX - 1) / CX) * ((g.my - 2 + CY - 1) / CY);
+    int cchunks = max(1, min((2304 + ctiles - 1) / ctiles, nk / 8 > 0 ? nk / 8 : 1));
+    const int ckchunk = (nk + cchunks - 1) / cchunks;
+    cchunks = (nk + ckchunk - 1) / ckchunk;
+    TOSCA_LAUNCH(sizeof(T) == 8 ? "k_dsm_cs_nut" : "k_dsm_cs_nut_f32", k_dsm_cs_nut<T>,
+                 dim3((g.mx - 2 + CX - 1) / CX, (g.my - 2 + CY - 1) / CY, cchunks), dim3(CX, CY), 0, s, g, LM, MM, Sabs,
+                 m->aj, maxCs, cs, nut, ckchunk);
 }
 
 template <class T>
