@@ -130,18 +130,13 @@ __global__ void __launch_bounds__(64 * DSM_NR) k_dsm_lm(Geo g, const T *ucat, co
             B[n] = wK * f;
             q[n] = done;
         }
-        // the finished sums go to this thread's own LDS slots once every thread has read its j
-        // neighbours, so the contraction below reads them back instead of holding them in registers
-        __syncthreads();
-#pragma unroll
-        for (int n = 0; n < NQ; n++) sH[n][r][lane] = q[n];
         if (emit) {
             const int k = K - 1;
             const long c = g.L(k, j, i);
             const double rden = 1.0 / (dij * simpson_mask_sum(k, g.mz));
             double F[27];
 #pragma unroll
-            for (int n = 0; n < 27; n++) F[n] = sH[n][r][lane] * rden;
+            for (int n = 0; n < 27; n++) F[n] = q[n] * rden;
             double L[3][3], Sh[3][3], SSh[3][3];
             L[0][0] = F[6] - F[3] * F[0]; L[0][1] = F[7] - F[3] * F[1]; L[0][2] = F[8] - F[3] * F[2];
             L[1][0] = F[9] - F[4] * F[0]; L[1][1] = F[10] - F[4] * F[1]; L[1][2] = F[11] - F[4] * F[2];
@@ -151,7 +146,7 @@ __global__ void __launch_bounds__(64 * DSM_NR) k_dsm_lm(Geo g, const T *ucat, co
             SSh[0][0] = F[21]; SSh[0][1] = SSh[1][0] = F[22]; SSh[0][2] = SSh[2][0] = F[23];
             SSh[1][1] = F[24]; SSh[1][2] = SSh[2][1] = F[25]; SSh[2][2] = F[26];
             const double filter = cube_root(1. / aj[c]);
-            const double test_filter = cube_root(8.0 * sH[27][r][lane]);
+            const double test_filter = cube_root(8.0 * q[27]);
             double nm, dn;
             germano(L, Sh, SSh, filter, test_filter, ld3(csi, c), ld3(eta, c), ld3(zet, c), nm, dn);
             LM[c] = nm;
@@ -161,8 +156,8 @@ __global__ void __launch_bounds__(64 * DSM_NR) k_dsm_lm(Geo g, const T *ucat, co
         sabsPrev = sabs;
         uKm = uK;
         uK = uKp;
-        // no barrier needed here: sU of the next plane is written after every thread has passed the
-        // sH barrier above (so all sU reads are done), and sH after the next sU barrier
+        // the next plane writes sH only after its sU barrier, which every thread reaches after its j
+        // filter reads of this plane
     }
 }
 
