@@ -273,13 +273,21 @@ __global__ void __launch_bounds__(256) k_nut_fused(Geo g, const T *ucat, const d
     }
 }
 
-// zero an ni x nj x nk box of an array with row length nx and plane size nx*ny (ncomp values per point)
+// zero six disjoint boxes of an array with row length nx and plane size nx*ny (ncomp values per
+// point) in one launch: blockIdx.y selects the box, the blocks stride over its values
+struct ShellBoxes { int i0[6], j0[6], k0[6], ni[6], nj[6], nk[6]; };
+
 template <class T>
-__global__ void k_zero_box(T *a, int ncomp, int nx, int ny, int i0, int j0, int k0, int ni) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ni * ncomp) return;
-    const long p = ((long)(k0 + blockIdx.z) * ny + (j0 + blockIdx.y)) * nx + i0;
-    a[p * ncomp + t] = T(0);
+__global__ void k_zero_box(T *a, int ncomp, int nx, int ny, ShellBoxes B) {
+    const int b = blockIdx.y;
+    if (B.ni[b] <= 0 || B.nj[b] <= 0 || B.nk[b] <= 0) return;
+    const long row = (long)B.ni[b] * ncomp, tot = row * B.nj[b] * B.nk[b];
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += (long)gridDim.x * blockDim.x) {
+        const long e = t % row, r = t / row;
+        const int jj = (int)(r % B.nj[b]), kk = (int)(r / B.nj[b]);
+        const long p = ((long)(B.k0[b] + kk) * ny + (B.j0[b] + jj)) * nx + B.i0[b];
+        a[p * ncomp + e] = T(0);
+    }
 }
 
 template <class T>
@@ -287,17 +295,17 @@ void zero_shell(T *a, int ncomp, int nx, int ny, int nz, int lo, int hi, hipStre
     // boxes (in array coordinates) covering everything but [lo+1 .. n-hi-2] on every axis:
     // `lo` / `hi` extra layers outside the boundary cells on the low / high side
     const int w0 = lo + 1, w1 = hi + 1;            // shell thickness per side
-    auto box = [&](int i0, int j0, int k0, int ni, int nj, int nk) {
-        if (ni <= 0 || nj <= 0 || nk <= 0) return;
-        TOSCA_LAUNCH("k_zero_box", k_zero_box<T>, dim3((ni * ncomp + 255) / 256, nj, nk), dim3(256), 0, s, a, ncomp, nx, ny,
-                     i0, j0, k0, ni);
+    ShellBoxes B{};
+    auto box = [&](int q, int i0, int j0, int k0, int ni, int nj, int nk) {
+        B.i0[q] = i0; B.j0[q] = j0; B.k0[q] = k0; B.ni[q] = ni; B.nj[q] = nj; B.nk[q] = nk;
     };
-    box(0, 0, 0, nx, ny, w0);                                   // low k slab
-    box(0, 0, nz - w1, nx, ny, w1);                             // high k slab
-    box(0, 0, w0, nx, w0, nz - w0 - w1);                        // low j slab
-    box(0, ny - w1, w0, nx, w1, nz - w0 - w1);                  // high j slab
-    box(0, w0, w0, w0, ny - w0 - w1, nz - w0 - w1);             // low i slab
-    box(nx - w1, w0, w0, w1, ny - w0 - w1, nz - w0 - w1);       // high i slab
+    box(0, 0, 0, 0, nx, ny, w0);                                   // low k slab
+    box(1, 0, 0, nz - w1, nx, ny, w1);                             // high k slab
+    box(2, 0, 0, w0, nx, w0, nz - w0 - w1);                        // low j slab
+    box(3, 0, ny - w1, w0, nx, w1, nz - w0 - w1);                  // high j slab
+    box(4, 0, w0, w0, w0, ny - w0 - w1, nz - w0 - w1);             // low i slab
+    box(5, nx - w1, w0, w0, w1, ny - w0 - w1, nz - w0 - w1);       // high i slab
+    TOSCA_LAUNCH("k_zero_box", k_zero_box<T>, dim3(512, 6), dim3(256), 0, s, a, ncomp, nx, ny, B);
 }
 
 template <class T>
