@@ -1,6 +1,6 @@
 """GPU parity: the HIP library (through the C ABI) against the CPU oracle on identical inputs.
 
-Bar (north star): normwise relative L2 error <= 1e-12 per output field for the FP64 path.
+Bar: normwise relative L2 error <= 1e-12 per output field for the FP64 path.
 Output buffers start from the same random values on both sides, so entries a routine must not
 touch are checked as well.
 """
