@@ -12,7 +12,23 @@
 //   UpdateNut        src/les.c:2119-2578 (Smagorinsky / dynamic branch :2510-2535)
 //   UpdateNutBCs     src/boundary.c:2630-2929 (zeroGradient / fixedValue / periodic subset)
 //   Buoyancy         src/sources/ueqn_sources.c:2726-2880
-// and the helpers of src/include/inline.h they call (cited at each helper below).  PETSc's
+// and, for the routines next to that path:
+//   ExplicitRhsU, UeqnFE, UeqnRK4, IMEXMatVec, UeqnRK3CN_W/_SO, UeqnABCN, UeqnSNES's constant
+//                    right-hand side and SNESFuncEval   src/solvers/ueqn_solvers.c:1087-1838
+//   contravariantToCartesian   src/ueqn.c:528-635
+//   KSPGMRES as kspIMEX runs it (PETSc 3.15: classical Gram-Schmidt, Givens rotations)
+//   UpdateContravariantBCs / UpdateCartesianBCs / UpdateTemperatureBCs   src/boundary.c:614-2628
+//   the Cabot wall law   src/wallfunctions.c:280-585
+//   FormT            src/solvers/teqn_solvers.c:6-595
+//   GradP            src/peqn.c:3437-3773
+// and the helpers of src/include/inline.h they call (cited at each helper below).
+//
+// Every function is pinned by tests under tests/test_oracle_*.py against facts outside its code
+// (closed forms, invariants, library routines, quadrature, brute force).  Parity unpinned: the
+// one-sided boundary branches of Compute_du_i/j/k (inline.h:2762-2930), QUICK clipping at
+// non-periodic boundary faces (inline.h:2215-2227) and the cell-centred versus face metrics of
+// FormU's face dot, which no closed form separates; they are cross-checked only through the two
+// independently structured GPU kernel sets agreeing with this file.  PETSc's
 // single-rank DMLocalToLocal scatter is restated from PETSc's documented DMDA semantics:
 // a ghost point of a DM_BOUNDARY_PERIODIC axis holds the owned value at (index mod m); ghost
 // points of a DM_BOUNDARY_GHOSTED axis are left untouched.
