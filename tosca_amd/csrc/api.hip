@@ -718,15 +718,12 @@ int tosca_ueqn_rk4(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, dou
     if (hipMemcpyAsync(utmp, ucont_o, vglob, hipMemcpyDeviceToDevice, s) != hipSuccess)   // :1471
         return set_err(c, TOSCA_ERR_DEVICE, "UeqnRK4: copy of ucont_o failed");
     for (int i = 0; i < 4; i++) {
-        if (i != 0) {                                                        // :1480-1486
-            tosca::launch_waxpy_local(ucont, a[i] * dt, rhs, ucont_o, g, s);
-            tosca::launch_wrap(ucont, 3, g, s);
-        }
         queue_explicit(c, m, p, ucont, ucat, nut, dp, btheta, btheta_o, ab2, farm, ucont_o, rhs, form_mode);   // :1489
-        tosca::launch_axpy(utmp, dt * b[i], rhs, 3L * g.mx * g.my * g.mz, s);   // :1492
+        // utmp += dt b_i Rhs (:1492) and, in the same pass, the next stage's ucont = Ucont_o +
+        // a_{i+1} dt Rhs (:1480-1486) or, after the last stage, ucont = utmp (:1495-1497)
+        tosca::launch_rk_stage(utmp, dt * b[i], rhs, ucont, i < 3 ? a[i + 1] * dt : 0.0, ucont_o, i == 3, g, s);
+        tosca::launch_wrap(ucont, 3, g, s);
     }
-    tosca::launch_global_to_local(ucont, utmp, g, s);                      // :1495-1497
-    tosca::launch_wrap(ucont, 3, g, s);
     return launched(c, "UeqnRK4");
 }
 

@@ -153,6 +153,20 @@ __global__ void k_waxpy_local(double *ucont, double al, const double *rhs, const
     ucont[3 * owned_local(g, t / 3) + t % 3] = al * rhs[t] + uo[t];
 }
 
+// One pass over a Runge-Kutta stage's Rhs (ueqn_solvers.c:1480-1497): the accumulator update
+// utmp = utmp + a Rhs (VecAXPY) and the owned range of the next stage's local ucont, either
+// al Rhs + Ucont_o (VecWAXPY, use_acc == 0) or the updated accumulator (the final copy,
+// use_acc == 1); each value is formed by the same expression as in the separate passes
+__global__ void k_rk_stage(double *utmp, double a, const double *rhs, double *ucont, double al, const double *uo,
+                           int use_acc, Geo g, long n) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const double r = rhs[t];
+    const double acc = utmp[t] + a * r;
+    utmp[t] = acc;
+    ucont[3 * owned_local(g, t / 3) + t % 3] = use_acc ? acc : al * r + uo[t];
+}
+
 // VecAXPY(y, a, x): y = y + a x
 __global__ void k_axpy(double *y, double a, const double *x, long n) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -233,6 +247,13 @@ void launch_fe_update(double *rhs, const double *ucont_o, double dt, double *uco
 void launch_waxpy_local(double *ucont, double al, const double *rhs, const double *ucont_o, const Geo &g, hipStream_t s) {
     const long n = 3L * g.mx * g.my * g.mz;
     TOSCA_LAUNCH("k_waxpy_local", k_waxpy_local, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ucont, al, rhs, ucont_o, g, n);
+}
+
+void launch_rk_stage(double *utmp, double a, const double *rhs, double *ucont, double al, const double *ucont_o,
+                     int use_acc, const Geo &g, hipStream_t s) {
+    const long n = 3L * g.mx * g.my * g.mz;
+    TOSCA_LAUNCH("k_rk_stage", k_rk_stage, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, utmp, a, rhs, ucont, al,
+                 ucont_o, use_acc, g, n);
 }
 
 void launch_axpy(double *y, double a, const double *x, long n, hipStream_t s) {

@@ -32,6 +32,8 @@ void launch_non_resolved(double *rhs, const double *ucont_o, const tosca_params 
 void launch_fe_update(double *rhs, const double *ucont_o, double dt, double *ucont, const Geo &g, hipStream_t s);
 void launch_waxpy_local(double *ucont, double al, const double *rhs, const double *ucont_o, const Geo &g, hipStream_t s);
 void launch_axpy(double *y, double a, const double *x, long n, hipStream_t s);
+void launch_rk_stage(double *utmp, double a, const double *rhs, double *ucont, double al, const double *ucont_o,
+                     int use_acc, const Geo &g, hipStream_t s);
 void launch_snes_finish(double *F, const double *u, const double *ucont_o, double dt, const tosca_params *p,
                         const Geo &g, hipStream_t s);
 void launch_cn_rhs(double *bU, const double *dp, const double *bt, const double *bto, const double *rhs_o,
