@@ -428,6 +428,20 @@ int tosca_formt(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_params *para
 int tosca_update_contravariant_bcs(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_params *params,
                                    double *ucont, const double *ucat);
 
+/* The kLeft inletFunction profile of tosca_update_cartesian_bcs (mesh->inletF.kLeft, read in
+ * src/inflow.c:40-120; applied at boundary.c:1228-1300): the analytic velocity profiles. */
+enum { TOSCA_INLET_LOGLAW = 2, TOSCA_INLET_SINUSOIDAL = 6 };
+typedef struct {
+    int type;                        /* typeU: TOSCA_INLET_LOGLAW or TOSCA_INLET_SINUSOIDAL          */
+    const double *cent;              /* mesh->lCent: cell centres, local AoS, device                */
+    double grnd_level;               /* mesh->grndLevel (log law: h = cent.z - grnd_level)          */
+    double ymin, ly;                 /* mesh->bounds.ymin, mesh->bounds.Ly (sinusoidal)             */
+    double u_tau, roughness, h_inv;  /* uTau, roughness, hInv (log law)                             */
+    double amplitude, periods;       /* sinusoidal amplitude and number of periods across Ly        */
+    double udir[3];                  /* Udir, the unit flow direction                               */
+    double uref[3];                  /* Uref (sinusoidal: its magnitude scales the profile)         */
+} tosca_inlet;
+
 /* UpdateCartesianBCs (src/boundary.c:824-2050) on lUcat (local AoS, in place): the ghost cells of
  * every patch next to the interior cells (edges and corners untouched) from
  *   fixedValue   the patch value (u_fixed: host array of 6 x 3 doubles, iLval ... kRval; may be
@@ -449,12 +463,18 @@ int tosca_update_contravariant_bcs(tosca_ctx *ctx, const tosca_mesh *mesh, const
  *                minus Ub_nor (0 and u* = 0 where |Ub_par| < 1e-20).  The right patch of an
  *                axis whose left patch is Cabot too takes the left patch's roughness, as in the
  *                reference (boundary.c:978-979, 1072-1073),
+ *   inletFunction on kLeft (bc_u[4] = TOSCA_BCU_GENERIC with inlet != NULL, boundary.c:1228-1300):
+ *                the log law max(uTau/0.4 ln(h/roughness), 1e-5) Udir below hInv and its value at
+ *                hInv above, h = cent.z - grnd_level; or the sinusoidal profile
+ *                (1 + amplitude cos(2 pi periods (cent.y - ymin)/Ly)) |Uref| Udir,
  * then the ghost scatter.  ustar: local scalar array (lUstar), device; may be NULL when no patch
- * is Cabot.  Inlet-function patches and wall-function k patches need the inflow models or have
- * no wall model and return TOSCA_ERR_UNSUPPORTED.  The first call with a smooth Cabot patch
- * builds the context's copy of the reference's integration table (5e6 + 1 doubles). */
+ * is Cabot.  inlet: may be NULL when kLeft is not an inlet function.  The other inlet-function
+ * profiles (random power law, mapped inflow databases, Nieuwstadt), inlet functions on other
+ * patches and wall-function k patches return TOSCA_ERR_UNSUPPORTED.  The first call with a
+ * smooth Cabot patch builds the context's copy of the reference's integration table (5e6 + 1
+ * doubles). */
 int tosca_update_cartesian_bcs(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_params *params, double *ucat,
-                               const double *u_fixed, double *ustar);
+                               const double *u_fixed, double *ustar, const tosca_inlet *inlet);
 
 /* UpdateTemperatureBCs (src/boundary.c:2052-2628) on lTmprt (local scalar, in place): ghosts of
  * zeroGradient (t(1)), fixedValue (value), fixedGradient ((1/aj)/|area| value + t(1), the area of

@@ -47,7 +47,7 @@ def load() -> C.CDLL:
         "oracle_rhs_jvp": (I, [In, M, Pa, P, P, P, P, P, P, P, P, D, D, P, P]),
         "oracle_explicit_rhs_u": (I, [In, M, Pa, P, P, P, P, P, P, I, P, P, P, I]),
         "oracle_contravariant_bcs": (I, [In, M, Pa, P, P]),
-        "oracle_cartesian_bcs": (I, [In, M, Pa, P, P, P]),
+        "oracle_cartesian_bcs": (I, [In, M, Pa, P, P, P, P]),
         "oracle_cabot_utau": (D, [D, D, D]),
         "oracle_cabot_table": (D, [C.c_int64]),
         "oracle_temperature_bcs": (I, [In, M, Pa, P, P, P]),
@@ -175,13 +175,19 @@ def contravariant_bcs(info, mesh, params, ucont, ucat):
     _check(lib.oracle_contravariant_bcs(C.byref(inf), C.byref(m), C.byref(p), _p(ucont), _p(ucat)))
 
 
-def cartesian_bcs(info, mesh, params, ucat, u_fixed=None, ustar=None):
+def cartesian_bcs(info, mesh, params, ucat, u_fixed=None, ustar=None, inlet=None):
     """UpdateCartesianBCs subset restated; ucat (and ustar, a local scalar array, where Cabot wall
-    functions write it) modified in place; u_fixed (6, 3) or None."""
+    functions write it) modified in place; u_fixed (6, 3) or None; inlet: dict of the kLeft
+    inlet-function profile (abi.make_inlet keys, cent a numpy array of the local cell centres)."""
     lib = load()
     inf, m, p = abi.make_info(info), _mesh(mesh), abi.make_params(params)
     uf = None if u_fixed is None else np.ascontiguousarray(u_fixed, dtype=np.float64).reshape(18)
-    _check(lib.oracle_cartesian_bcs(C.byref(inf), C.byref(m), C.byref(p), _p(ucat), _p(uf), _p(ustar)))
+    il = None
+    if inlet is not None:
+        cent = np.ascontiguousarray(inlet["cent"], dtype=np.float64)
+        il = abi.make_inlet(dict(inlet, cent=cent.ctypes.data))
+    _check(lib.oracle_cartesian_bcs(C.byref(inf), C.byref(m), C.byref(p), _p(ucat), _p(uf), _p(ustar),
+                                    C.cast(C.pointer(il), C.c_void_p) if il is not None else None))
 
 
 def cabot_utau(nu, u, y):

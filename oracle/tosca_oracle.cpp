@@ -2030,14 +2030,14 @@ extern "C" double oracle_cabot_utau(double nu, double u, double y) { return utau
 extern "C" double oracle_cabot_table(int64_t i) { return cabot_entry(i); }
 
 int oracle_cartesian_bcs(const tosca_dmda_info *info, const tosca_mesh *m, const tosca_params *p, double *ucat_,
-                         const double *ufix, double *ustar) {
+                         const double *ufix, double *ustar, const tosca_inlet *inlet) {
     Lay l;
     int rc = check_layout(info, p, l);
     if (rc) return rc;
     const int *bc = p->bc_u;
     for (int q = 0; q < 6; q++) {
-        if (bc[q] == TOSCA_BCU_GENERIC || (bc[q] == TOSCA_BCU_WALLFUNCTION && q >= 4)) {
-            g_err = "cartesian BCs: inlet-function patches and k wall functions are not covered";
+        if ((bc[q] == TOSCA_BCU_GENERIC && !(q == 4 && inlet)) || (bc[q] == TOSCA_BCU_WALLFUNCTION && q >= 4)) {
+            g_err = "cartesian BCs: inlet functions other than on kLeft and k wall functions are not covered";
             return TOSCA_ERR_UNSUPPORTED;
         }
         if (bc[q] == TOSCA_BCU_WALLFUNCTION && p->wall_u[q] != -1 && p->wall_u[q] != -3) {
@@ -2049,6 +2049,8 @@ int oracle_cartesian_bcs(const tosca_dmda_info *info, const tosca_mesh *m, const
     }
     const int *wf = p->wall_u;
     const double *aj = m->aj;
+    const bool inletK = inlet && bc[4] == TOSCA_BCU_GENERIC;
+    if (inletK && inlet->type != 2 && inlet->type != 6) { g_err = "inlet function type not covered"; return TOSCA_ERR_UNSUPPORTED; }
     V3 *lucat = (V3 *)ucat_;
     const V3 *csi = (const V3 *)m->csi, *eta = (const V3 *)m->eta, *zet = (const V3 *)m->zet;
     const int64_t mx = l.mx, my = l.my, mz = l.mz;
@@ -2119,6 +2121,25 @@ int oracle_cartesian_bcs(const tosca_dmda_info *info, const tosca_mesh *m, const
                 const int64_t c = l.L(k, j, i);
                 const int64_t gi0 = l.L(k, j, i - 1), gi1 = l.L(k, j, i + 1), gj0 = l.L(k, j - 1, i),
                               gj1 = l.L(k, j + 1, i), gk0 = l.L(k - 1, j, i), gk1 = l.L(k + 1, j, i);
+                if (k == 1 && inletK) {                          // boundary.c:1228-1300
+                    const V3 cc = ((const V3 *)inlet->cent)[c];
+                    double uMag = 0;
+                    if (inlet->type == 2) {                      // log-law profile
+                        const double h = cc.z - inlet->grnd_level, vkConstant = 0.4;
+                        if (h <= inlet->h_inv) {
+                            const double v = (inlet->u_tau / vkConstant) * std::log(h / inlet->roughness);
+                            uMag = (v < 1e-5) ? 1e-5 : v;
+                        } else {
+                            uMag = (inlet->u_tau / vkConstant) * std::log(inlet->h_inv / inlet->roughness);
+                        }
+                    } else {                                     // sinusoidal inflow
+                        const double fraction = (cc.y - inlet->ymin) / inlet->ly;
+                        const double *u = inlet->uref;
+                        uMag = (1.0 + inlet->amplitude * std::cos(inlet->periods * 2.0 * M_PI * fraction)) *
+                               sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+                    }
+                    ucat[gk0] = V3{uMag * inlet->udir[0], uMag * inlet->udir[1], uMag * inlet->udir[2]};
+                }
                 if (bc[0] == TOSCA_BCU_WALLFUNCTION && wf[0] == -1 && i == 1) cabot(c, 0, gi0, roughI);
                 else if (bc[1] == TOSCA_BCU_WALLFUNCTION && wf[1] == -1 && i == mx - 2) cabot(c, 0, gi1, roughI);
                 if (bc[2] == TOSCA_BCU_WALLFUNCTION && wf[2] == -1 && j == 1) cabot(c, 1, gj0, roughJ);

@@ -171,3 +171,44 @@ def test_cartesian_bcs_rejects_wall_function(rhs_mod):
     with pytest.raises(ToscaError):
         ctx.UpdateCartesianBCs({k: to_dev(v) for k, v in mesh.items()}, params, to_dev(rng.standard_normal(lay.gshape + (3,))))
     ctx.close()
+
+
+@pytest.mark.parametrize("kind", ["loglaw", "sinusoidal"])
+def test_cartesian_bcs_inlet(rhs_mod, kind):
+    """kLeft inletFunction profiles (log law, sinusoidal) on a random curvilinear mesh with the
+    other patches of the other kinds; the cell centres come from the mesh builder.  log and cos
+    may round differently on the two sides: bar 1e-14."""
+    rng = np.random.Generator(np.random.PCG64(61))
+    lay, mesh_t = C.mesh_metrics(random_nodes((7, 6, 5), 61), (0, 0, 0))
+    cent = to_np(mesh_t["cent"])
+    mesh = np_mesh(mesh_t)
+    bcn = [C.BC_ZEROGRAD] * 6
+    params = C.default_params((0, 0, 0), [C.BC_FIXED] * 6, bcn, div_scheme=C.DIV_CENTRAL, les_model=C.LES_DSM, nu=0.01)
+    params = dict(params, bc_u=(C.BCU_ZEROGRADIENT, C.BCU_SLIP, C.BCU_NOSLIP, C.BCU_SLIP, C.BCU_GENERIC,
+                                C.BCU_ZEROGRADIENT))
+    own = cent[4:-4, 4:-4, 4:-4]           # interior cells (the builder leaves boundary-cell centres at 0)
+    zc = own[..., 2]
+    if kind == "loglaw":
+        inlet = dict(type=C.INLET_LOGLAW, grnd_level=float(zc.min()) - 0.05, u_tau=0.4, roughness=0.03,
+                     h_inv=float(np.median(zc)), udir=(0.6, 0.8, 0.0))
+    else:
+        yc = own[..., 1]
+        inlet = dict(type=C.INLET_SINUSOIDAL, ymin=float(yc.min()), ly=float(np.ptp(yc)), amplitude=0.25, periods=1.5,
+                     udir=(0.0, 0.6, 0.8), uref=(1.0, 2.0, 2.0))
+    print(f"\nUpdateCartesianBCs inlet {kind} bc_u={params['bc_u']}", flush=True)
+    U0 = rng.standard_normal(lay.gshape + (3,))
+    Uo = U0.copy()
+    O.cartesian_bcs(lay.info(), mesh, params, Uo, None, None, dict(inlet, cent=cent))
+    ctx = rhs_mod.RhsContext(lay.info())
+    Ug = to_dev(U0)
+    ctx.UpdateCartesianBCs({k: to_dev(v) for k, v in mesh.items()}, params, Ug, None, None,
+                           dict(inlet, cent=to_dev(cent)))
+    torch.cuda.synchronize()
+    e = relerr(to_np(Ug), Uo)
+    print(f"  ucat relL2={e:.2e} identical={np.array_equal(to_np(Ug), Uo)}", flush=True)
+    assert e <= TOL
+    from tosca_amd.abi import ToscaError
+    with pytest.raises(ToscaError):
+        ctx.UpdateCartesianBCs({k: to_dev(v) for k, v in mesh.items()}, params, Ug, None, None,
+                               dict(inlet, type=3, cent=to_dev(cent)))
+    ctx.close()

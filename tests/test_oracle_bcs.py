@@ -141,6 +141,70 @@ def test_cartesian_bcs_rejects_k_wall_functions_and_missing_ustar():
 
 
 # ------------------------------------------------------------------------------------------------
+# kLeft inlet functions
+# ------------------------------------------------------------------------------------------------
+def _inlet_box(seed):
+    """stretched box: x along k (the inflow patch is kLeft), y along i, z along j"""
+    x = np.linspace(0.0, 60.0, 6)
+    y = np.linspace(-20.0, 20.0, 9)                      # 8 cells across y
+    z = np.cumsum(np.r_[0.0, 0.3 * 1.6 ** np.arange(7)])
+    lay, mesh_t = C.mesh_metrics(C.cartesian_nodes(x, y, z, "cpu"), (0, 0, 0))
+    cent = mesh_t["cent"].numpy().copy()
+    bcn = [C.BC_ZEROGRAD] * 6
+    params = C.default_params((0, 0, 0), [C.BC_FIXED] * 6, bcn, div_scheme=C.DIV_CENTRAL, les_model=C.LES_DSM, nu=0.01)
+    params = dict(params, bc_u=(ZG, ZG, NOSLIP, SLIP, GEN, ZG))
+    rng = np.random.Generator(np.random.PCG64(seed))
+    return lay, np_mesh(mesh_t), params, cent, rng
+
+
+def test_inlet_log_law():
+    """log law max(uTau/0.4 ln(h/z0), 1e-5) along Udir below hInv, its value at hInv above, with
+    h the height of the boundary cell's centre above the ground; nothing but ghost planes changes"""
+    lay, mesh, params, cent, rng = _inlet_box(41)
+    udir = np.array([0.8, 0.6, 0.0])
+    inlet = dict(type=C.INLET_LOGLAW, cent=cent, grnd_level=0.05, u_tau=0.45, roughness=0.2, h_inv=8.0, udir=udir)
+    u0 = rng.standard_normal(lay.gshape + (3,))
+    u = u0.copy()
+    O.cartesian_bcs(lay.info(), mesh, params, u, None, None, inlet)
+    sg, sc = _plane(lay, 2, 0), _plane(lay, 2, 1)
+    h = cent[sc][..., 2] - 0.05
+    mag = np.where(h <= 8.0, np.maximum(0.45 / 0.4 * np.log(h / 0.2), 1e-5), 0.45 / 0.4 * np.log(8.0 / 0.2))
+    assert (h < 0.2).any() and (h > 8.0).any()            # the clip and the cap both act
+    assert np.abs(u[sg] - mag[..., None] * udir).max() <= 1e-14 * mag.max()
+    above = u[sg][h > 8.0]
+    assert np.array_equal(above, np.broadcast_to(above[0], above.shape))
+    changed = u != u0                  # only ghost planes change (the other patches fill theirs)
+    m = (lay.mx, lay.my, lay.mz)
+    for q in range(6):
+        changed[_plane(lay, q // 2, 0 if q % 2 == 0 else m[q // 2] - 1)] = False
+    assert not changed.any()
+
+
+def test_inlet_sinusoidal_mean():
+    """sinusoidal profile: over whole periods across Ly on the uniform y cells the mean speed is
+    |Uref|, the extremes are (1 +- amplitude cos(pi/4)) |Uref| at the cell-centre phases, along Udir"""
+    lay, mesh, params, cent, rng = _inlet_box(42)
+    uref = np.array([3.0, 0.0, 4.0])
+    udir = uref / np.linalg.norm(uref)
+    inlet = dict(type=C.INLET_SINUSOIDAL, cent=cent, ymin=-20.0, ly=40.0, amplitude=0.3, periods=2.0, udir=udir,
+                 uref=uref)
+    u = rng.standard_normal(lay.gshape + (3,))
+    O.cartesian_bcs(lay.info(), mesh, params, u, None, None, inlet)
+    g = u[_plane(lay, 2, 0)]                               # [j][i]
+    mag = g @ udir
+    assert np.abs(g - mag[..., None] * udir).max() <= 1e-14
+    assert np.abs(mag.mean(axis=1) - 5.0).max() <= 1e-13
+    # 4 cells per period: their centres sit at the phases pi/4 + k pi/2
+    c45 = np.sqrt(0.5)
+    assert abs(mag.max() - 5.0 * (1 + 0.3 * c45)) <= 1e-12 and abs(mag.min() - 5.0 * (1 - 0.3 * c45)) <= 1e-12
+    from tosca_amd.abi import ToscaError
+    with pytest.raises(ToscaError):                        # the random power law is not covered
+        O.cartesian_bcs(lay.info(), mesh, params, u, None, None, dict(inlet, type=1))
+    with pytest.raises(ToscaError):                        # inlet functions on kLeft only
+        O.cartesian_bcs(lay.info(), mesh, dict(params, bc_u=(GEN, ZG, NOSLIP, SLIP, GEN, ZG)), u, None, None, inlet)
+
+
+# ------------------------------------------------------------------------------------------------
 # Cabot wall law
 # ------------------------------------------------------------------------------------------------
 def _inv_nut(y):

@@ -64,6 +64,25 @@ class Params(C.Structure):
     ]
 
 
+class Inlet(C.Structure):
+    _fields_ = [("type", C.c_int), ("cent", C.c_void_p), ("grnd_level", C.c_double), ("ymin", C.c_double),
+                ("ly", C.c_double), ("u_tau", C.c_double), ("roughness", C.c_double), ("h_inv", C.c_double),
+                ("amplitude", C.c_double), ("periods", C.c_double), ("udir", C.c_double * 3), ("uref", C.c_double * 3)]
+
+
+def make_inlet(d: Mapping) -> "Inlet":
+    """tosca_inlet from a dict: type, cent (device or host pointer as int), grnd_level, ymin, ly,
+    u_tau, roughness, h_inv, amplitude, periods, udir, uref."""
+    x = Inlet()
+    x.type = int(d["type"])
+    x.cent = d.get("cent") or None
+    for n in ("grnd_level", "ymin", "ly", "u_tau", "roughness", "h_inv", "amplitude", "periods"):
+        setattr(x, n, float(d.get(n, 0.0)))
+    x.udir[:] = [float(v) for v in d.get("udir", (1.0, 0.0, 0.0))]
+    x.uref[:] = [float(v) for v in d.get("uref", (0.0, 0.0, 0.0))]
+    return x
+
+
 class TeqnParams(C.Structure):
     _fields_ = [("pr", C.c_double), ("wall_t", C.c_int * 4), ("q_wall_i", C.c_void_p), ("q_wall_j", C.c_void_p)]
 
@@ -119,7 +138,7 @@ SIGNATURES = {
                                          P, D]),
     "tosca_explicit_rhs_u": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P, P, P, P, I, P, P, P, I]),
     "tosca_update_contravariant_bcs": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P]),
-    "tosca_update_cartesian_bcs": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P]),
+    "tosca_update_cartesian_bcs": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P, P]),
     "tosca_update_temperature_bcs": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P]),
     "tosca_formt": (I, [P, C.POINTER(Mesh), C.POINTER(Params), C.POINTER(TeqnParams), P, P, P, P, P, D, I]),
     "tosca_ueqn_fe": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P, P, P, P, I, P, P, D, P, I]),

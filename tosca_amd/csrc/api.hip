@@ -1006,15 +1006,15 @@ int tosca_update_contravariant_bcs(tosca_ctx *c, const tosca_mesh *m, const tosc
 }
 
 int tosca_update_cartesian_bcs(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, double *ucat,
-                               const double *u_fixed, double *ustar) {
+                               const double *u_fixed, double *ustar, const tosca_inlet *inlet) {
     int rc = check_ctx(c);
     if (rc) return rc;
     if ((rc = check_params(c, p))) return rc;
     bool cabot = false, smooth = false;
     for (int q = 0; q < 6; q++) {
-        if (p->bc_u[q] == TOSCA_BCU_GENERIC)
-            return set_err(c, TOSCA_ERR_UNSUPPORTED, "bc_u[%d]: inlet-function velocity patches need the inflow models, "
-                                                     "which are not part of this library", q);
+        if (p->bc_u[q] == TOSCA_BCU_GENERIC && !(q == 4 && inlet))
+            return set_err(c, TOSCA_ERR_UNSUPPORTED, "bc_u[%d]: inlet-function velocity patches are covered on kLeft only, "
+                                                     "with their profile in inlet", q);
         if (p->bc_u[q] == TOSCA_BCU_WALLFUNCTION) {
             if (q >= 4)
                 return set_err(c, TOSCA_ERR_UNSUPPORTED, "bc_u[%d]: k patches have no velocity wall model", q);
@@ -1040,6 +1040,12 @@ int tosca_update_cartesian_bcs(tosca_ctx *c, const tosca_mesh *m, const tosca_pa
         if ((rc = check_dev(c, m->aj, n * sizeof(double), "mesh.aj")) || (rc = check_dev(c, ustar, n * sizeof(double), "ustar")))
             return rc;
     }
+    if (inlet && p->bc_u[4] == TOSCA_BCU_GENERIC) {
+        if (inlet->type != TOSCA_INLET_LOGLAW && inlet->type != TOSCA_INLET_SINUSOIDAL)
+            return set_err(c, TOSCA_ERR_UNSUPPORTED, "inlet function type %d: only the log law (2) and the sinusoidal "
+                                                     "profile (6) are covered", inlet->type);
+        if ((rc = check_dev(c, inlet->cent, 3 * n * sizeof(double), "inlet.cent"))) return rc;
+    }
     if (smooth && !c->cabot) {
         if (hipMalloc(&c->cabot, tosca::cabot_table_doubles() * sizeof(double)) != hipSuccess) {
             (void)hipGetLastError();
@@ -1049,7 +1055,8 @@ int tosca_update_cartesian_bcs(tosca_ctx *c, const tosca_mesh *m, const tosca_pa
         }
         tosca::launch_cabot_table(c->cabot, c->stream);
     }
-    tosca::launch_cartesian_bcs(ucat, u_fixed, ustar, c->cabot, m, p, c->geo, c->stream);
+    tosca::launch_cartesian_bcs(ucat, u_fixed, ustar, c->cabot, (inlet && p->bc_u[4] == TOSCA_BCU_GENERIC) ? inlet : nullptr,
+                                m, p, c->geo, c->stream);
     return launched(c, "UpdateCartesianBCs");
 }
 

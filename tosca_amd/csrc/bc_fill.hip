@@ -227,9 +227,16 @@ __device__ V3 unit_normal(V3 cs, V3 et, V3 ze, int a) {
     return V3{x / sum, y / sum, z / sum};
 }
 
+// the kLeft inlet-function profile (type 0: none)
+struct InletArgs {
+    int type;
+    const double *cent;
+    double grnd, ymin, ly, u_tau, rough, h_inv, amp, periods, udir[3], uref_mag;
+};
+
 // ghost cell of a patch next to an interior boundary cell (transverse indices interior)
 __global__ void k_cart_bcs(double *ucat, const double *csi, const double *eta, const double *zet, const double *aj,
-                           double *ustar, const double *tab, double nu, Geo g, PatchSet P) {
+                           double *ustar, const double *tab, double nu, Geo g, PatchSet P, InletArgs In) {
     const int q = blockIdx.z, a = q >> 1, side = q & 1;
     const int m[3] = {g.mx, g.my, g.mz};
     int t1, t2;
@@ -290,6 +297,22 @@ __global__ void k_cart_bcs(double *ucat, const double *csi, const double *eta, c
         const V3 bP{Ub.x - bN.x, Ub.y - bN.y, Ub.z - bN.z}, cP{Uc.x - cN.x, Uc.y - cN.y, Uc.z - cN.z};
         const V3 gp{2. * bP.x - cP.x, 2. * bP.y - cP.y, 2. * bP.z - cP.z};
         out = V3{gp.x - bN.x, gp.y - bN.y, gp.z - bN.z};
+    } else if (b == TOSCA_BCU_GENERIC && q == 4 && In.type != 0) {      // inletFunction, :1228-1300
+        const V3 cc = ld3(In.cent, ci);
+        double uMag;
+        if (In.type == TOSCA_INLET_LOGLAW) {                             // :1254-1286
+            const double h = cc.z - In.grnd, vk = 0.4;
+            if (h <= In.h_inv) {
+                const double v = (In.u_tau / vk) * log(h / In.rough);
+                uMag = (v < 1e-5) ? 1e-5 : v;                            // PetscMax
+            } else {
+                uMag = (In.u_tau / vk) * log(In.h_inv / In.rough);
+            }
+        } else {                                                         // sinusoidal, :1294-1299
+            const double fraction = (cc.y - In.ymin) / In.ly;
+            uMag = (1.0 + In.amp * cos(In.periods * 2.0 * 3.14159265358979323846 * fraction)) * In.uref_mag;
+        }
+        out = V3{uMag * In.udir[0], uMag * In.udir[1], uMag * In.udir[2]};
     } else if (b == TOSCA_BCU_ZEROGRADIENT) {                            // :1869-1928
         const V3 U = ld3(ucat, ci), U2 = ld3(ucat, g.L(c2[2], c2[1], c2[0]));
         out = V3{2 * U.x - U2.x, 2 * U.y - U2.y, 2 * U.z - U2.z};
@@ -378,7 +401,17 @@ void launch_cabot_table(double *tab, hipStream_t s) {
 }
 
 void launch_cartesian_bcs(double *ucat, const double *u_fixed, double *ustar, const double *cabot_tab,
-                          const tosca_mesh *m, const tosca_params *p, const Geo &g, hipStream_t s) {
+                          const tosca_inlet *inlet, const tosca_mesh *m, const tosca_params *p, const Geo &g,
+                          hipStream_t s) {
+    InletArgs In{};
+    if (inlet) {
+        In.type = inlet->type; In.cent = inlet->cent; In.grnd = inlet->grnd_level; In.ymin = inlet->ymin;
+        In.ly = inlet->ly; In.u_tau = inlet->u_tau; In.rough = inlet->roughness; In.h_inv = inlet->h_inv;
+        In.amp = inlet->amplitude; In.periods = inlet->periods;
+        for (int d = 0; d < 3; d++) In.udir[d] = inlet->udir[d];
+        const double *u = inlet->uref;
+        In.uref_mag = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);   // nMag(Uref)
+    }
     const int n = maxdim(g);
     PatchSet P = patches(p->bc_u, u_fixed, 18);
     for (int q = 0; q < 4; q++) P.wall[q] = p->wall_u[q];
@@ -389,7 +422,7 @@ void launch_cartesian_bcs(double *ucat, const double *u_fixed, double *ustar, co
         P.rough[2 * a] = P.rough[2 * a + 1] = r;
     }
     TOSCA_LAUNCH("k_cart_bcs", k_cart_bcs, dim3((n + 127) / 128, n, 6), dim3(128), 0, s, ucat, m->csi, m->eta, m->zet,
-                 m->aj, ustar, cabot_tab, p->nu, g, P);
+                 m->aj, ustar, cabot_tab, p->nu, g, P, In);
     launch_wrap(ucat, 3, g, s);
 }
 

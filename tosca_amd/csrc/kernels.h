@@ -57,7 +57,7 @@ constexpr long CABOT_INTERVAL = 2, CABOT_MAX_YP = 10000000, CABOT_NYP = CABOT_MA
 size_t cabot_table_doubles();       // the table (CABOT_NYP + 1) and the chunk totals of its prefix sum
 void launch_cabot_table(double *tab, hipStream_t s);
 void launch_cartesian_bcs(double *ucat, const double *u_fixed, double *ustar, const double *cabot_tab,
-                          const tosca_mesh *m, const tosca_params *p,
+                          const tosca_inlet *inlet, const tosca_mesh *m, const tosca_params *p,
                           const Geo &g, hipStream_t s);
 void launch_temperature_bcs(double *t, const int *bc_t, const double *t_value, const tosca_mesh *m, const Geo &g,
                             hipStream_t s);

@@ -150,17 +150,21 @@ class RhsContext:
         m, p = mesh_struct(mesh), abi.make_params(params)
         self._check(self.lib.tosca_update_contravariant_bcs(self.h, C.byref(m), C.byref(p), _ptr(ucont), _ptr(ucat)))
 
-    def UpdateCartesianBCs(self, mesh, params, ucat, u_fixed=None, ustar=None):
+    def UpdateCartesianBCs(self, mesh, params, ucat, u_fixed=None, ustar=None, inlet=None):
         """UpdateCartesianBCs(ueqn): ghost cells of ucat, in place; u_fixed: 6 x 3 fixedValue vectors;
-        ustar: local scalar array (lUstar) the Cabot wall-function patches write."""
+        ustar: local scalar array (lUstar) the Cabot wall-function patches write; inlet: dict of the
+        kLeft inlet-function profile (abi.make_inlet keys, cent a CUDA tensor)."""
         self._stream()
         m, p = mesh_struct(mesh), abi.make_params(params)
         uf = None
         if u_fixed is not None:
             uf = (C.c_double * 18)(*[float(x) for row in u_fixed for x in row])
+        il = None
+        if inlet is not None:
+            il = abi.make_inlet(dict(inlet, cent=_ptr(inlet.get("cent"))))
         self._check(self.lib.tosca_update_cartesian_bcs(self.h, C.byref(m), C.byref(p), _ptr(ucat),
                                                         C.cast(uf, C.c_void_p) if uf is not None else None,
-                                                        _ptr(ustar)))
+                                                        _ptr(ustar), C.cast(C.pointer(il), C.c_void_p) if il is not None else None))
 
     def UpdateTemperatureBCs(self, mesh, params, bc_t, t_value, theta):
         """UpdateTemperatureBCs(teqn): ghost cells of theta, in place; bc_t: 6 TOSCA_BCT_* kinds."""
