@@ -212,6 +212,7 @@ def _inv_nut(y):
     return 1.0 / (1.0 + 0.41 * y * (1.0 - np.exp(-y / 19.0)) ** 2)
 
 
+@pytest.mark.filterwarnings("ignore::scipy.integrate.IntegrationWarning")
 @pytest.mark.parametrize("i", [1, 7, 60, 2500, 40000])
 def test_cabot_table_is_the_integral(i):
     """table entry i = the integral of 1/(1 + nu_t+) over [0, 2 i] (scipy quad, per unit interval),
@@ -223,6 +224,7 @@ def test_cabot_table_is_the_integral(i):
     assert abs(O.cabot_table(i) - want) <= 3e-11 * want
 
 
+@pytest.mark.filterwarnings("ignore::scipy.integrate.IntegrationWarning")
 @pytest.mark.parametrize("nu,u,y", [(1.5e-5, 8.0, 5.0), (1e-3, 0.7, 0.05), (1e-2, 2.0, 0.3), (1.5e-5, 0.05, 1e-4),
                                     (1e-4, 1.0, 2e-3)])
 def test_cabot_utau_solves_the_interpolated_law(nu, u, y):
