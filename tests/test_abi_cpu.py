@@ -46,7 +46,7 @@ def test_struct_sizes_match_header(tmp_path):
 #include <stddef.h>
 #include "tosca_rhs.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(tosca_dmda_info), sizeof(tosca_mesh), sizeof(tosca_params),
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(tosca_inlet), offsetof(tosca_inlet, udir), sizeof(tosca_dmda_info), sizeof(tosca_mesh), sizeof(tosca_params),
          offsetof(tosca_params, nut_value), offsetof(tosca_params, nu), offsetof(tosca_params, tau_wall_i),
          offsetof(tosca_params, wall_u), offsetof(tosca_params, wall_rough));
   return 0; }
@@ -55,6 +55,8 @@ int main(void) {
     (d / "p.c").write_text(src)
     subprocess.run(["gcc", "-I", str(ROOT / "include"), str(d / "p.c"), "-o", str(d / "p")], check=True)
     vals = list(map(int, subprocess.run([str(d / "p")], capture_output=True, text=True, check=True).stdout.split()))
+    assert vals[0] == C.sizeof(abi.Inlet) and vals[1] == abi.Inlet.udir.offset
+    vals = vals[2:]
     assert vals[0] == C.sizeof(abi.DmdaInfo)
     assert vals[1] == C.sizeof(abi.Mesh)
     assert vals[2] == C.sizeof(abi.Params)
