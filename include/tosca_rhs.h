@@ -152,7 +152,12 @@ int tosca_ctx_timing_entry(tosca_ctx *ctx, int idx, char *name, int name_len, do
 
 /* Validates the static mesh once (replaces the checks of SetMeshMetrics, src/mesh.c:1306-1346,
  * and the IBM/overset flags): nvert and meshtag all zero, aj > 1e-20 and |csi|,|eta|,|zet| >= 1e-20
- * on non-corner cells.  Synchronous.  Returns TOSCA_ERR_MESH with a message on failure. */
+ * on the interior cells (NaN fails both).  Synchronous.  Returns TOSCA_ERR_MESH with a message on
+ * failure.  REQUIRED once per mesh and context: the kernels never read nvert / meshTag, so every
+ * entry point below that takes a mesh returns TOSCA_ERR_MESH unless a mesh with the same nvert,
+ * meshtag, aj, csi, eta and zet pointers passed this check on the same context (the context keeps
+ * the last 64 such meshes; a failed check removes the mesh).  The check does not see later writes
+ * into those arrays: the mesh is static in the reference (SetMeshMetrics runs once). */
 int tosca_mesh_check(tosca_ctx *ctx, const tosca_mesh *mesh);
 
 /* FormU (src/solvers/ueqn_solvers.c:6-1083).
@@ -284,8 +289,11 @@ typedef struct {
     double atol;
 } tosca_krylov_params;
 
-/* Outcome of a solve: total GMRES iterations, the last residual estimate, and reason > 0
- * converged, -3 iteration limit, -4 NaN residual, -5 breakdown, -9 null rotation. */
+/* Outcome of a solve: total GMRES iterations, the last residual estimate, and reason, a PETSc
+ * KSPConvergedReason value: 2 KSP_CONVERGED_RTOL, 3 KSP_CONVERGED_ATOL, -2 KSP_DIVERGED_NULL (zero
+ * Givens rotation), -3 KSP_DIVERGED_ITS, -4 KSP_DIVERGED_DTOL (residual >= 1e30 ||b||),
+ * -5 KSP_DIVERGED_BREAKDOWN, -9 KSP_DIVERGED_NANORINF.  The tolerance is max(rtol ||b||, atol), with
+ * ||r0|| in place of ||b|| when b = 0 (KSPConvergedDefault with a nonzero initial guess). */
 typedef struct {
     int    iterations;
     double residual;

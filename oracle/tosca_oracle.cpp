@@ -1368,8 +1368,11 @@ int oracle_snes_func_eval(const tosca_dmda_info *info, const tosca_mesh *m, cons
 // initial guess, KSPSetTolerances(rtol, atol, 1e30, max_it)).  PETSc is a dependency absent from
 // the reference tree; this restates its published algorithm (src/ksp/ksp/impls/gmres/gmres.c:
 // KSPSolve_GMRES, KSPGMRESCycle, KSPGMRESUpdateHessenberg, KSPGMRESBuildSoln, with the default
-// classical Gram-Schmidt orthogonalisation without refinement; KSPConvergedDefault).  The relative
-// test is taken against the initial residual norm; at the reference's rtol = 1e-30 only atol acts.
+// classical Gram-Schmidt orthogonalisation without refinement; KSPConvergedDefault).  kspIMEX has a
+// nonzero initial guess and no preconditioner (src/ueqn.c:264-276), so KSPConvergedDefault takes the
+// relative and divergence tests against ||b|| (||r0|| when b = 0): ttol = max(rtol ||b||, atol),
+// dtol 1e30 (ueqn.c:277).  Reasons are PETSc's KSPConvergedReason values: 2 rtol, 3 atol, -2 null
+// rotation (KSP_DIVERGED_NULL), -3 iteration limit, -4 dtol, -5 breakdown, -9 NaN or Inf.
 // ================================================================================================
 static double vdot(int64_t n, const double *a, const double *b) {
     double s = 0.0;
@@ -1384,8 +1387,16 @@ static int gmres(int64_t n, const std::function<int(const double *, double *)> &
     std::vector<double> HH((size_t)(m + 2) * (m + 1), 0.0), cc(m + 1), ss(m + 1), grs(m + 2), nrs(m + 1);
     auto H = [&](int a, int c) -> double & { return HH[(size_t)c * (m + 2) + a]; };
     int its = 0, reason = 0;
-    double ttol = 0.0, res = 0.0;
+    double ttol = 0.0, res = 0.0, rnorm0 = 0.0;
+    const double bnorm = std::sqrt(vdot(n, b, b)), divtol = 1e30;
     bool first = true;
+    // KSPConvergedDefault: NaN / Inf, then the tolerance (atol when below it, else rtol), then dtol
+    auto test = [&](double r) {
+        if (std::isnan(r) || std::isinf(r)) return -9;
+        if (r <= ttol) return r < atol ? 3 : 2;
+        if (r >= divtol * rnorm0) return -4;
+        return 0;
+    };
     while (!reason) {
         // KSPInitialResidual: r = b - A x into V_0, then VecNormalize
         int rc = A(x, tmp.data());
@@ -1398,9 +1409,13 @@ static int gmres(int64_t n, const std::function<int(const double *, double *)> &
             for (int64_t i = 0; i < n; i++) v0[i] = v0[i] * sc;
         }
         grs[0] = res;
-        if (first) { ttol = std::max(rtol * res, atol); first = false; }
-        if (std::isnan(res)) { reason = -4; break; }
-        if (res <= ttol) reason = 1;
+        if (first) {
+            rnorm0 = bnorm > 0.0 ? bnorm : res;           // nonzero guess: ||b||, or ||r0|| for b = 0
+            ttol = std::max(rtol * rnorm0, atol);
+            first = false;
+        }
+        reason = test(res);
+        if (reason < 0) break;
         int it = 0;
         bool hapend = false;
         while (!reason && it < m && its < maxit) {
@@ -1433,7 +1448,7 @@ static int gmres(int64_t n, const std::function<int(const double *, double *)> &
             }
             if (!hapend) {
                 const double t1 = std::sqrt(H(it, it) * H(it, it) + H(it + 1, it) * H(it + 1, it));
-                if (t1 == 0.0) { reason = -9; break; }
+                if (t1 == 0.0) { reason = -2; break; }
                 cc[it] = H(it, it) / t1;
                 ss[it] = H(it + 1, it) / t1;
                 grs[it + 1] = -(ss[it] * grs[it]);
@@ -1445,8 +1460,7 @@ static int gmres(int64_t n, const std::function<int(const double *, double *)> &
             }
             it++;
             its++;
-            if (std::isnan(res)) reason = -4;
-            else if (res <= ttol) reason = 1;
+            reason = test(res);
             if (hapend) { if (!reason) reason = -5; break; }
         }
         // KSPGMRESBuildSoln over columns 0 .. it-1

@@ -93,6 +93,41 @@ def test_imex_solve(rhs_mod, periodic, ucat0):
     ctx.close()
 
 
+@pytest.mark.parametrize("periodic", [(0, 0, 0), (2, 0, 1)])
+def test_imex_solve_rtol_governs(rhs_mod, periodic):
+    """A relative tolerance with atol = 0 and a guess far from the solution: PETSc's rule measures
+    rtol against ||b|| (nonzero initial guess), so the GPU and the oracle must stop at the same
+    iteration with reason KSP_CONVERGED_RTOL (2); the residual there is <= rtol ||b||."""
+    seed = 60 + PERS.index(periodic)
+    lay, nm, f, params, rng, glob, tau_i, tau_j = _setup(periodic, seed, (C.BCU_GENERIC, C.BCU_SLIP, C.BCU_NOSLIP))
+    f = dict(f, ucat=torch.zeros_like(f["ucat"]))
+    print(f"\nIMEX solve, rtol governs, periodic={periodic}", flush=True)
+    b = rng.standard_normal(glob)
+    x0 = 50.0 * rng.standard_normal(glob)
+    uo = C.owned(f["ucont"], lay).numpy().copy()
+    nut = to_np(f["nut"])
+    coef, rtol = 0.9, 1e-3
+    po = dict(params, tau_wall_i=tau_i.ctypes.data, tau_wall_j=tau_j.ctypes.data)
+    xo = x0.copy()
+    its_o, res_o, r_o = O.imex_solve(lay.info(), nm, po, b, xo, to_np(f["ucont"]), to_np(f["ucat"]), nut, uo, coef,
+                                     rtol=rtol, atol=0.0)
+    ctx = rhs_mod.RhsContext(lay.info())
+    dm = {k: to_dev(v) for k, v in nm.items()}
+    ti, tj = to_dev(tau_i), to_dev(tau_j)
+    pg = dict(params, tau_wall_i=ti.data_ptr(), tau_wall_j=tj.data_ptr())
+    xg = to_dev(x0)
+    its_g, res_g, r_g = ctx.IMEXSolve(dm, pg, to_dev(b), xg, to_dev(to_np(f["ucont"])), to_dev(to_np(f["ucat"])),
+                                      to_dev(nut), to_dev(uo), coef, rtol=rtol, atol=0.0)
+    torch.cuda.synchronize()
+    bn = float(np.linalg.norm(b.ravel()))
+    print(f"  its gpu {its_g} oracle {its_o}; reason gpu {r_g} oracle {r_o}; res gpu {res_g:.3e} "
+          f"rtol*||b|| {rtol * bn:.3e}", flush=True)
+    assert r_o == 2 and r_g == 2
+    assert its_g == its_o
+    assert res_g <= rtol * bn and res_o <= rtol * bn
+    ctx.close()
+
+
 @pytest.mark.parametrize("scheme", [0, 1])
 @pytest.mark.parametrize("atol", [1e-5, 1e-11])
 @pytest.mark.parametrize("periodic", [(0, 0, 0), (2, 0, 1), (1, 1, 1)])

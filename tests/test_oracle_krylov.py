@@ -88,6 +88,73 @@ def test_breakdowns_and_exact_guess():
     assert its == 0 and reason > 0
 
 
+def _residual_history(A, b, x0, kmax):
+    """GMRES residual estimates after 1..kmax iterations of one cycle (the iteration limit stops
+    each run; atol = rtol = 0 never stops it earlier)."""
+    out = []
+    for k in range(1, kmax + 1):
+        x = x0.copy()
+        its, res, reason = O.gmres_dense(A, b, x, restart=kmax + 5, rtol=0.0, atol=0.0, maxit=k)
+        assert its == k and reason == -3
+        out.append(res)
+    return out
+
+
+def test_relative_tolerance_is_taken_against_the_rhs_norm():
+    """kspIMEX has a nonzero initial guess (src/ueqn.c:276), so PETSc's KSPConvergedDefault measures
+    rtol against ||b||, not against ||r0||.  With a guess far from the solution (||r0|| = 300 ||b||)
+    the two rules stop at different iterations; the solve must stop at the first iteration whose
+    residual estimate is <= rtol ||b||, with reason KSP_CONVERGED_RTOL (2)."""
+    rng = np.random.Generator(np.random.PCG64(21))
+    n = 40
+    A = np.eye(n) + 0.5 * rng.standard_normal((n, n)) / np.sqrt(n)
+    b = rng.standard_normal(n)
+    x0 = 300.0 * np.linalg.norm(b) / np.sqrt(n) * rng.standard_normal(n)
+    hist = _residual_history(A, b, x0, 12)
+    bn, r0 = np.linalg.norm(b), np.linalg.norm(b - A @ x0)
+    assert r0 > 100 * bn
+    k = 6
+    rtol = hist[k - 1] / bn * (1 + 1e-9)                 # met first at iteration k by ||b||
+    assert hist[k - 2] > rtol * bn
+    k_r0 = next(i + 1 for i, r in enumerate(hist) if r <= rtol * r0)
+    assert k_r0 < k                                      # the ||r0|| rule would stop earlier
+    x = x0.copy()
+    its, res, reason = O.gmres_dense(A, b, x, restart=30, rtol=rtol, atol=0.0)
+    assert its == k and reason == 2
+    assert np.linalg.norm(b - A @ x) <= rtol * bn * (1 + 1e-6)
+
+
+def test_atol_reason_and_zero_rhs():
+    """Converging under atol reports KSP_CONVERGED_ATOL (3); with b = 0 and a nonzero guess the
+    relative test falls back to ||r0||."""
+    rng = np.random.Generator(np.random.PCG64(22))
+    n = 30
+    A = np.eye(n) + 0.4 * rng.standard_normal((n, n)) / np.sqrt(n)
+    b = rng.standard_normal(n)
+    x = np.zeros(n)
+    its, res, reason = O.gmres_dense(A, b, x, restart=30, rtol=1e-30, atol=1e-8)
+    assert reason == 3 and res < 1e-8
+    z = np.zeros(n)
+    x0 = rng.standard_normal(n)
+    hist = _residual_history(A, z, x0, 8)
+    r0 = np.linalg.norm(A @ x0)
+    k = 4
+    rtol = hist[k - 1] / r0 * (1 + 1e-9)
+    x = x0.copy()
+    its, res, reason = O.gmres_dense(A, z, x, restart=30, rtol=rtol, atol=0.0)
+    assert its == k and reason == 2
+
+
+def test_nan_residual_reason():
+    """A NaN residual is KSP_DIVERGED_NANORINF (-9)."""
+    n = 10
+    A = np.eye(n)
+    A[3, 3] = np.nan
+    x = np.zeros(n)
+    its, res, reason = O.gmres_dense(A, np.ones(n), x, restart=5, atol=1e-8)
+    assert reason == -9
+
+
 def _imex_case(periodic, seed):
     lay, mesh = C.mesh_metrics(random_nodes((7, 6, 5), seed), periodic)
     rng = np.random.Generator(np.random.PCG64(400 + seed))

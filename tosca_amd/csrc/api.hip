@@ -12,6 +12,7 @@
 #include <cstring>
 #include <string>
 #include <algorithm>
+#include <array>
 #include <iterator>
 #include <vector>
 #include <type_traits>
@@ -35,6 +36,9 @@ struct tosca_ctx {
     double *hs = nullptr;        // device staging of the host-resident step
     size_t hs_elems = 0;
     double *cabot = nullptr;     // the Cabot wall law's integration table, built on first use
+    // meshes tosca_mesh_check accepted (their nvert, meshtag, aj, csi, eta, zet pointers): the
+    // kernels do not read nvert / meshTag, so every entry point that takes a mesh requires one
+    std::vector<std::array<const void *, 6>> checked;
     tosca::KernelTimer timer;
     bool ref_path = false;   // TOSCA_PATH=ref at creation: reference-structured kernels only
 };
@@ -76,15 +80,30 @@ int check_dev(tosca_ctx *c, const void *p, size_t need, const char *name) {
         return set_err(c, TOSCA_ERR_ARG, "%s is not device memory", name);
     hipDeviceptr_t base = nullptr;
     size_t size = 0;
-    if (hipMemGetAddressRange(&base, &size, const_cast<void *>(p)) == hipSuccess && base) {
-        const char *b = reinterpret_cast<const char *>(base), *q = reinterpret_cast<const char *>(p);
-        if (q + need > b + size)
-            return set_err(c, TOSCA_ERR_ARG, "%s: allocation too small (%zu bytes needed past the pointer, %zu available)",
-                           name, need, (size_t)(b + size - q));
-    } else {
+    if (hipMemGetAddressRange(&base, &size, const_cast<void *>(p)) != hipSuccess || !base) {
         (void)hipGetLastError();
+        return set_err(c, TOSCA_ERR_ARG, "%s: the allocation holding it is unknown to the HIP runtime, so its size "
+                                         "cannot be checked", name);
     }
+    const char *b = reinterpret_cast<const char *>(base), *q = reinterpret_cast<const char *>(p);
+    if (q + need > b + size)
+        return set_err(c, TOSCA_ERR_ARG, "%s: allocation too small (%zu bytes needed past the pointer, %zu available)",
+                       name, need, (size_t)(b + size - q));
     return TOSCA_OK;
+}
+
+std::array<const void *, 6> mesh_key(const tosca_mesh *m) {
+    return {m->nvert, m->meshtag, m->aj, m->csi, m->eta, m->zet};
+}
+
+// the mesh must have passed tosca_mesh_check on this context
+int require_checked(tosca_ctx *c, const tosca_mesh *m) {
+    if (!m) return set_err(c, TOSCA_ERR_ARG, "mesh is NULL");
+    const auto key = mesh_key(m);
+    for (const auto &k : c->checked)
+        if (k == key) return TOSCA_OK;
+    return set_err(c, TOSCA_ERR_MESH, "the mesh has not passed tosca_mesh_check on this context (its nvert / meshTag "
+                                      "are validated there once, not read per call)");
 }
 
 int check_ctx(tosca_ctx *c) {
@@ -122,10 +141,11 @@ int check_les_model(tosca_ctx *c, const tosca_params *p) {
     }
 }
 
-int check_mesh(tosca_ctx *c, const tosca_mesh *m, bool face, bool cell, bool jac) {
+int check_mesh(tosca_ctx *c, const tosca_mesh *m, bool face, bool cell, bool jac, bool need_checked = true) {
     if (!m) return set_err(c, TOSCA_ERR_ARG, "mesh is NULL");
     const size_t n = (size_t)c->geo.nloc(), v = 3 * n * sizeof(double), s = n * sizeof(double);
     int rc;
+    if (need_checked && (rc = require_checked(c, m))) return rc;
     if (cell) {
         if ((rc = check_dev(c, m->csi, v, "mesh.csi")) || (rc = check_dev(c, m->eta, v, "mesh.eta")) ||
             (rc = check_dev(c, m->zet, v, "mesh.zet")) || (rc = check_dev(c, m->aj, s, "mesh.aj")))
@@ -429,7 +449,9 @@ int fetch(tosca_ctx *c, const double *d, double *h, int cnt) {
 
 // Restarted GMRES as PETSc 3.15 runs it for kspIMEX (src/ksp/ksp/impls/gmres/gmres.c:
 // KSPSolve_GMRES, KSPGMRESCycle with classical Gram-Schmidt, KSPGMRESUpdateHessenberg,
-// KSPGMRESBuildSoln; KSPConvergedDefault with the relative test against the initial residual).
+// KSPGMRESBuildSoln; KSPConvergedDefault).  kspIMEX has a nonzero initial guess and no
+// preconditioner (src/ueqn.c:264-276), so the relative and divergence tests are taken against ||b||
+// (||r0|| when b = 0), dtol 1e30 (ueqn.c:277); reasons are PETSc's KSPConvergedReason values.
 // Vector work on the device, the (restart + 1) x restart Hessenberg system on the host.
 template <class Op>
 int gmres_device(tosca_ctx *c, long n, Op A, const double *b, double *x, const tosca_krylov_params *kp,
@@ -440,8 +462,19 @@ int gmres_device(tosca_ctx *c, long n, Op A, const double *b, double *x, const t
     auto H = [&](int a, int col) -> double & { return HH[(size_t)col * (m + 2) + a]; };
     auto V = [&](int j) { return K.V + (size_t)j * n; };
     int its = 0, reason = 0, rc;
-    double ttol = 0.0, res = 0.0, sq = 0.0;
+    double ttol = 0.0, res = 0.0, sq = 0.0, rnorm0 = 0.0, bsq = 0.0;
+    const double divtol = 1e30;
+    tosca::launch_mdot(b, n, 1, b, n, K.partial, K.dsc, s);                // ||b||
+    if ((rc = fetch(c, K.dsc, &bsq, 1))) return rc;
+    const double bnorm = std::sqrt(bsq);
     bool first = true;
+    // KSPConvergedDefault: NaN / Inf (-9), the tolerance (3 below atol, else 2), dtol (-4)
+    auto test = [&](double r) {
+        if (std::isnan(r) || std::isinf(r)) return -9;
+        if (r <= ttol) return r < kp->atol ? 3 : 2;
+        if (r >= divtol * rnorm0) return -4;
+        return 0;
+    };
     while (!reason) {
         A(x, K.tmp);                                                        // KSPInitialResidual
         tosca::launch_residual(V(0), b, K.tmp, n, s);
@@ -450,9 +483,13 @@ int gmres_device(tosca_ctx *c, long n, Op A, const double *b, double *x, const t
         res = std::sqrt(sq);
         if (res > 0.0) tosca::launch_scale_to(V(0), V(0), 1.0 / res, n, s);
         grs[0] = res;
-        if (first) { ttol = std::max(kp->rtol * res, kp->atol); first = false; }
-        if (std::isnan(res)) { reason = -4; break; }
-        if (res <= ttol) reason = 1;
+        if (first) {
+            rnorm0 = bnorm > 0.0 ? bnorm : res;
+            ttol = std::max(kp->rtol * rnorm0, kp->atol);
+            first = false;
+        }
+        reason = test(res);
+        if (reason < 0) break;
         int it = 0;
         bool hapend = false;
         while (!reason && it < m && its < kp->max_it) {
@@ -477,7 +514,7 @@ int gmres_device(tosca_ctx *c, long n, Op A, const double *b, double *x, const t
             }
             if (!hapend) {
                 const double t1 = std::sqrt(H(it, it) * H(it, it) + H(it + 1, it) * H(it + 1, it));
-                if (t1 == 0.0) { reason = -9; break; }
+                if (t1 == 0.0) { reason = -2; break; }            // KSP_DIVERGED_NULL
                 cc[it] = H(it, it) / t1;
                 ss[it] = H(it + 1, it) / t1;
                 grs[it + 1] = -(ss[it] * grs[it]);
@@ -489,8 +526,7 @@ int gmres_device(tosca_ctx *c, long n, Op A, const double *b, double *x, const t
             }
             it++;
             its++;
-            if (std::isnan(res)) reason = -4;
-            else if (res <= ttol) reason = 1;
+            reason = test(res);
             if (hapend) { if (!reason) reason = -5; break; }
         }
         const int last = it - 1;                                            // KSPGMRESBuildSoln
@@ -623,9 +659,11 @@ int tosca_ctx_timing_entry(tosca_ctx *c, int idx, char *name, int name_len, doub
 int tosca_mesh_check(tosca_ctx *c, const tosca_mesh *m) {
     int rc = check_ctx(c);
     if (rc) return rc;
-    if ((rc = check_mesh(c, m, false, true, false))) return rc;
+    if ((rc = check_mesh(c, m, false, true, false, false))) return rc;
     const size_t s = (size_t)c->geo.nloc() * sizeof(double);
     if ((rc = check_dev(c, m->nvert, s, "mesh.nvert")) || (rc = check_dev(c, m->meshtag, s, "mesh.meshtag"))) return rc;
+    const auto key = mesh_key(m);
+    c->checked.erase(std::remove(c->checked.begin(), c->checked.end(), key), c->checked.end());
     const size_t need = 4 * (size_t)(c->geo.mz) + 8;
     if (c->chk_elems < need) {
         if (c->chk) hipFree(c->chk);
@@ -645,6 +683,8 @@ int tosca_mesh_check(tosca_ctx *c, const tosca_mesh *m) {
     if (cnt[1]) return set_err(c, TOSCA_ERR_MESH, "%ld points with meshTag != 0 (overset meshes are not supported)", cnt[1]);
     if (cnt[2]) return set_err(c, TOSCA_ERR_MESH, "%ld interior cells with negative or close-to-zero volume (aj < 1e-20)", cnt[2]);
     if (cnt[3]) return set_err(c, TOSCA_ERR_MESH, "%ld interior cells with a close-to-zero face area (|csi|,|eta|,|zet| < 1e-20)", cnt[3]);
+    if (c->checked.size() >= 64) c->checked.erase(c->checked.begin());
+    c->checked.push_back(key);
     return TOSCA_OK;
 }
 
@@ -996,7 +1036,7 @@ int tosca_update_contravariant_bcs(tosca_ctx *c, const tosca_mesh *m, const tosc
     if ((rc = check_params(c, p))) return rc;
     set_geo_per(c, p);
     const size_t v = 3 * (size_t)c->geo.nloc() * sizeof(double);
-    if (!m) return set_err(c, TOSCA_ERR_ARG, "mesh is NULL");
+    if ((rc = require_checked(c, m))) return rc;
     if ((rc = check_dev(c, m->icsi, v, "mesh.icsi")) || (rc = check_dev(c, m->jeta, v, "mesh.jeta")) ||
         (rc = check_dev(c, m->kzet, v, "mesh.kzet")) || (rc = check_dev(c, ucont, v, "ucont")) ||
         (rc = check_dev(c, ucat, v, "ucat")))
@@ -1133,7 +1173,7 @@ int tosca_buoyancy(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, con
     set_geo_per(c, p);
     const Geo &g = c->geo;
     const size_t n = (size_t)g.nloc(), ng = 3 * (size_t)g.mx * g.my * g.mz * sizeof(double);
-    if (!m) return set_err(c, TOSCA_ERR_ARG, "mesh is NULL");
+    if ((rc = require_checked(c, m))) return rc;
     if ((rc = check_dev(c, m->icsi, 3 * n * sizeof(double), "mesh.icsi")) ||
         (rc = check_dev(c, m->jeta, 3 * n * sizeof(double), "mesh.jeta")) ||
         (rc = check_dev(c, m->kzet, 3 * n * sizeof(double), "mesh.kzet")) ||

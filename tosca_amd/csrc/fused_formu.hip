@@ -414,6 +414,9 @@ __global__ void __launch_bounds__(64 * NR) k_fused_formu(MainArgs<T, MT> A, Geo 
         if (needF1) in1 = face_in<0>(P, q, 1);
         if (needF2) in2 = face_in<1>(P, q, SJ);
         if (needF3) in3 = face_in<2>(P, q, SK);
+        // nu_t is loaded whether or not the viscous LES term uses it (the loop has no divergent
+        // branches): A.nut must never be null; the launcher substitutes a valid local array when
+        // the caller passes none (a null nu_t here faulted the GPU once, in formMode 1)
         const T nut0 = P.nut[q];
         // ---- face fluxes ---------------------------------------------------------------------------
         const Vec3<T> z3{0, 0, 0};
@@ -439,6 +442,8 @@ __global__ void __launch_bounds__(64 * NR) k_fused_formu(MainArgs<T, MT> A, Geo 
             etA = cvt3<double>(ld3(peta + 3 * qp, 0)); etB = cvt3<double>(ld3(peta + 3 * qp, SJ));
             zeA = cvt3<double>(ld3(pzet + 3 * qp, 0)); zeB = cvt3<double>(ld3(pzet + 3 * qp, SK));
             r0 = prhs[ro]; r1 = prhs[ro + 1]; r2 = prhs[ro + 2];
+            // theta, like nu_t, must be a valid local array even when Buoyancy is off (the launcher
+            // substitutes one); only the buoy flag decides whether it is read
             if (buoy) { th = ptheta[qp]; thI = ptheta[qp + 1]; thJ = ptheta[qp + SJ]; thK = ptheta[qp + SK]; }
         }
         const Vec3<T> F1dm = shup3(F1.d), F1vm = shup3(F1.v);

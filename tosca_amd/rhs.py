@@ -37,6 +37,9 @@ def mesh_struct(mesh: Mapping[str, torch.Tensor]) -> abi.Mesh:
     return abi.make_mesh({n: _ptr(mesh[n]) for n in abi.MESH_FIELDS if n in mesh})
 
 
+_CHECK_KEY = ("nvert", "meshtag", "aj", "csi", "eta", "zet")
+
+
 class RhsContext:
     """One tosca_ctx (workspace + stream) for one DMDA layout on one GPU."""
 
@@ -49,6 +52,7 @@ class RhsContext:
             raise abi.ToscaError(rc, self.lib.tosca_last_error(None).decode())
         self.h = h
         self.device = device
+        self._checked = set()
 
     def close(self):
         if getattr(self, "h", None):
@@ -65,6 +69,14 @@ class RhsContext:
         if rc != abi.TOSCA_OK:
             raise abi.ToscaError(rc, self.lib.tosca_last_error(self.h).decode())
 
+    def _mesh(self, mesh) -> abi.Mesh:
+        """The mesh struct; a mesh not yet seen by this context is validated first, once, as
+        SetMeshMetrics validates it once in the reference (src/mesh.c:1306-1346)."""
+        key = tuple(_ptr(mesh[n]) if n in mesh else None for n in _CHECK_KEY)
+        if key not in self._checked and None not in key:
+            self.mesh_check(mesh)
+        return mesh_struct(mesh)
+
     def _stream(self):
         self._check(self.lib.tosca_ctx_set_stream(self.h, C.c_void_p(torch.cuda.current_stream().cuda_stream)))
 
@@ -74,30 +86,30 @@ class RhsContext:
     # ---- the four reference operators ---------------------------------------------------------
     def FormU(self, mesh, params, ucont, ucat, nut, rhs, scale: float = 1.0, form_mode: int = 0):
         self._stream()
-        m, p = mesh_struct(mesh), abi.make_params(params)
+        m, p = self._mesh(mesh), abi.make_params(params)
         self._check(self.lib.tosca_formu(self.h, C.byref(m), C.byref(p), _ptr(ucont), _ptr(ucat), _ptr(nut),
                                          _ptr(rhs), float(scale), int(form_mode)))
 
     def UpdateCs(self, mesh, params, ucat, cs):
         self._stream()
-        m, p = mesh_struct(mesh), abi.make_params(params)
+        m, p = self._mesh(mesh), abi.make_params(params)
         self._check(self.lib.tosca_update_cs(self.h, C.byref(m), C.byref(p), _ptr(ucat), _ptr(cs)))
 
     def UpdateNut(self, mesh, params, ucat, cs, nut, ksgs=None):
         self._stream()
-        m, p = mesh_struct(mesh), abi.make_params(params)
+        m, p = self._mesh(mesh), abi.make_params(params)
         self._check(self.lib.tosca_update_nut(self.h, C.byref(m), C.byref(p), _ptr(ucat), _ptr(cs), _ptr(nut),
                                               _ptr(ksgs)))
 
     def Buoyancy(self, mesh, params, theta, scale, btheta, gcont=None):
         self._stream()
-        m, p = mesh_struct(mesh), abi.make_params(params)
+        m, p = self._mesh(mesh), abi.make_params(params)
         self._check(self.lib.tosca_buoyancy(self.h, C.byref(m), C.byref(p), _ptr(theta), float(scale),
                                             _ptr(btheta), _ptr(gcont)))
 
     def rhs_les_step(self, mesh, params, ucont, ucat, theta, rhs, scale, cs, nut, btheta, buoy_scale=1.0):
         self._stream()
-        m, p = mesh_struct(mesh), abi.make_params(params)
+        m, p = self._mesh(mesh), abi.make_params(params)
         self._check(self.lib.tosca_rhs_les_step(self.h, C.byref(m), C.byref(p), _ptr(ucont), _ptr(ucat), _ptr(theta),
                                                 _ptr(rhs), float(scale), _ptr(cs), _ptr(nut), _ptr(btheta),
                                                 float(buoy_scale)))
@@ -105,7 +117,7 @@ class RhsContext:
     def rhs_les_step_mixed(self, mesh, params, ucont, ucat, theta, rhs, scale, cs, nut, btheta, buoy_scale=1.0):
         """Mixed-precision step: ucont, ucat, theta, nut float32; mesh, rhs, cs, btheta float64."""
         self._stream()
-        m, p = mesh_struct(mesh), abi.make_params(params)
+        m, p = self._mesh(mesh), abi.make_params(params)
         self._check(self.lib.tosca_rhs_les_step_mixed(self.h, C.byref(m), C.byref(p), _p32(ucont), _p32(ucat),
                                                       _p32(theta), _ptr(rhs), float(scale), _ptr(cs), _p32(nut),
                                                       _ptr(btheta), float(buoy_scale)))
@@ -115,7 +127,7 @@ class RhsContext:
         """Mixed-precision step whose FormU reads FP32 metric copies (mesh32: float32 tensors of the
         metrics in abi.MESH_F32_FIELDS); the LES part reads the float64 mesh."""
         self._stream()
-        m, p = mesh_struct(mesh), abi.make_params(params)
+        m, p = self._mesh(mesh), abi.make_params(params)
         m32 = abi.make_mesh_f32({n: _p32(mesh32[n]) for n in abi.MESH_F32_FIELDS})
         self._check(self.lib.tosca_rhs_les_step_mixed_m32(self.h, C.byref(m), C.byref(m32), C.byref(p), _p32(ucont),
                                                           _p32(ucat), _p32(theta), _ptr(rhs), float(scale), _ptr(cs),
@@ -126,7 +138,7 @@ class RhsContext:
         """J^T rbar of R = FormU(ucont, ucat, UpdateNut(ucat, cs)) + Buoyancy(theta) into the four
         *_bar arrays (overwritten)."""
         self._stream()
-        m, p = mesh_struct(mesh), abi.make_params(params)
+        m, p = self._mesh(mesh), abi.make_params(params)
         self._check(self.lib.tosca_rhs_vjp(self.h, C.byref(m), C.byref(p), _ptr(ucont), _ptr(ucat), _ptr(theta),
                                            _ptr(cs), _ptr(rbar), float(scale), float(buoy_scale), _ptr(ucont_bar),
                                            _ptr(ucat_bar), _ptr(theta_bar), _ptr(cs_bar)))
@@ -138,7 +150,7 @@ class RhsContext:
         -dp + bTheta terms + farm + FormU, with the non-resolved faces reset.  btheta / farm None:
         the term is off; ab2: the second-order bTheta extrapolation (clock->it > itStart)."""
         self._stream()
-        m, p = mesh_struct(mesh), abi.make_params(params)
+        m, p = self._mesh(mesh), abi.make_params(params)
         self._check(self.lib.tosca_explicit_rhs_u(self.h, C.byref(m), C.byref(p), _ptr(ucont), _ptr(ucat), _ptr(nut),
                                                   _ptr(dp), _ptr(btheta), _ptr(btheta_o), int(ab2), _ptr(farm),
                                                   _ptr(ucont_o), _ptr(rhs), int(form_mode)))
@@ -147,7 +159,7 @@ class RhsContext:
     def UpdateContravariantBCs(self, mesh, params, ucont, ucat):
         """UpdateContravariantBCs(ueqn): boundary-face fluxes of ucont from ucat, in place."""
         self._stream()
-        m, p = mesh_struct(mesh), abi.make_params(params)
+        m, p = self._mesh(mesh), abi.make_params(params)
         self._check(self.lib.tosca_update_contravariant_bcs(self.h, C.byref(m), C.byref(p), _ptr(ucont), _ptr(ucat)))
 
     def UpdateCartesianBCs(self, mesh, params, ucat, u_fixed=None, ustar=None, inlet=None):
@@ -155,7 +167,7 @@ class RhsContext:
         ustar: local scalar array (lUstar) the Cabot wall-function patches write; inlet: dict of the
         kLeft inlet-function profile (abi.make_inlet keys, cent a CUDA tensor)."""
         self._stream()
-        m, p = mesh_struct(mesh), abi.make_params(params)
+        m, p = self._mesh(mesh), abi.make_params(params)
         uf = None
         if u_fixed is not None:
             uf = (C.c_double * 18)(*[float(x) for row in u_fixed for x in row])
@@ -169,7 +181,7 @@ class RhsContext:
     def UpdateTemperatureBCs(self, mesh, params, bc_t, t_value, theta):
         """UpdateTemperatureBCs(teqn): ghost cells of theta, in place; bc_t: 6 TOSCA_BCT_* kinds."""
         self._stream()
-        m, p = mesh_struct(mesh), abi.make_params(params)
+        m, p = self._mesh(mesh), abi.make_params(params)
         kinds = (C.c_int * 6)(*[int(b) for b in bc_t])
         vals = (C.c_double * 6)(*[float(v) for v in t_value])
         self._check(self.lib.tosca_update_temperature_bcs(self.h, C.byref(m), C.byref(p), C.cast(kinds, C.c_void_p),
@@ -178,13 +190,13 @@ class RhsContext:
     # ---- instrumentation -----------------------------------------------------------------------
     def GradP(self, mesh, params, p, dp):
         """GradP (src/peqn.c:3437-3773): dp (global vector) from the local pressure p."""
-        m, pa = mesh_struct(mesh), abi.make_params(params)
+        m, pa = self._mesh(mesh), abi.make_params(params)
         self._check(self.lib.tosca_grad_p(self.h, C.byref(m), C.byref(pa), _ptr(p), _ptr(dp)))
 
     def UeqnFE(self, mesh, params, ucont, ucat, nut, dp, btheta, btheta_o, ab2, farm, ucont_o, dt, rhs,
                form_mode: int = 0):
         """UeqnFE (src/solvers/ueqn_solvers.c:1411-1441): ucont <- ucont_o + dt ExplicitRhsU."""
-        m, p = mesh_struct(mesh), abi.make_params(params)
+        m, p = self._mesh(mesh), abi.make_params(params)
         self._check(self.lib.tosca_ueqn_fe(self.h, C.byref(m), C.byref(p), _ptr(ucont), _ptr(ucat), _ptr(nut),
                                            _ptr(dp), _ptr(btheta), _ptr(btheta_o), int(ab2), _ptr(farm),
                                            _ptr(ucont_o), float(dt), _ptr(rhs), int(form_mode)))
@@ -192,14 +204,14 @@ class RhsContext:
     def UeqnRK4(self, mesh, params, ucont, ucat, nut, dp, btheta, btheta_o, ab2, farm, ucont_o, dt, rhs, utmp,
                 form_mode: int = 0):
         """UeqnRK4 (src/solvers/ueqn_solvers.c:1446-1504): four ExplicitRhsU stages."""
-        m, p = mesh_struct(mesh), abi.make_params(params)
+        m, p = self._mesh(mesh), abi.make_params(params)
         self._check(self.lib.tosca_ueqn_rk4(self.h, C.byref(m), C.byref(p), _ptr(ucont), _ptr(ucat), _ptr(nut),
                                             _ptr(dp), _ptr(btheta), _ptr(btheta_o), int(ab2), _ptr(farm),
                                             _ptr(ucont_o), float(dt), _ptr(rhs), _ptr(utmp), int(form_mode)))
 
     def IMEXMatVec(self, mesh, params, v, ucont, ucat, nut, ucont_o, coef, av):
         """IMEXMatVec (src/solvers/ueqn_solvers.c:1739-1801): av = v - coef Visc(v)."""
-        m, p = mesh_struct(mesh), abi.make_params(params)
+        m, p = self._mesh(mesh), abi.make_params(params)
         self._check(self.lib.tosca_imex_matvec(self.h, C.byref(m), C.byref(p), _ptr(v), _ptr(ucont), _ptr(ucat),
                                                _ptr(nut), _ptr(ucont_o), float(coef), _ptr(av)))
 
@@ -207,7 +219,7 @@ class RhsContext:
         """KSPSolve(kspIMEX, b, x) with IMEXMatVec: restarted GMRES on the device; x is updated in
         place.  krylov: restart, max_it, rtol, atol (the reference's defaults otherwise).
         Returns (iterations, residual, reason)."""
-        m, p = mesh_struct(mesh), abi.make_params(params)
+        m, p = self._mesh(mesh), abi.make_params(params)
         kp, kr = abi.make_krylov_params(**krylov), abi.KrylovResult()
         self._check(self.lib.tosca_imex_solve(self.h, C.byref(m), C.byref(p), _ptr(b), _ptr(x), _ptr(ucont), _ptr(ucat),
                                               _ptr(nut), _ptr(ucont_o), float(coef), C.byref(kp), C.byref(kr)))
@@ -217,7 +229,7 @@ class RhsContext:
                   **krylov):
         """UeqnRK3CN_W (scheme 0) / UeqnRK3CN_SO (scheme 1, src/solvers/ueqn_solvers.c:1508-1661);
         returns the last stage's (iterations, residual, reason)."""
-        m, p = mesh_struct(mesh), abi.make_params(params)
+        m, p = self._mesh(mesh), abi.make_params(params)
         kp, kr = abi.make_krylov_params(**krylov), abi.KrylovResult()
         self._check(self.lib.tosca_ueqn_rk3cn(self.h, C.byref(m), C.byref(p), int(scheme), _ptr(ucont), _ptr(ucat),
                                               _ptr(nut), _ptr(dp), _ptr(btheta), _ptr(btheta_o), int(ab2), _ptr(farm),
@@ -228,7 +240,7 @@ class RhsContext:
                  **krylov):
         """UeqnABCN (src/solvers/ueqn_solvers.c:1665-1737); conv_o carries RhsConv_o between steps.
         Returns (iterations, residual, reason) of the solve."""
-        m, p = mesh_struct(mesh), abi.make_params(params)
+        m, p = self._mesh(mesh), abi.make_params(params)
         kp, kr = abi.make_krylov_params(**krylov), abi.KrylovResult()
         self._check(self.lib.tosca_ueqn_abcn(self.h, C.byref(m), C.byref(p), _ptr(ucont), _ptr(ucat), _ptr(nut),
                                              _ptr(dp), _ptr(btheta), _ptr(btheta_o), int(ab2), _ptr(farm),
@@ -243,7 +255,7 @@ class RhsContext:
             if t.dtype != torch.float64 or t.is_cuda or not t.is_contiguous():
                 raise ValueError("host arrays must be contiguous float64 CPU tensors")
             return t.data_ptr()
-        m, p = mesh_struct(mesh), abi.make_params(params)
+        m, p = self._mesh(mesh), abi.make_params(params)
         times = (C.c_double * 2)()
         self._check(self.lib.tosca_rhs_les_step_host(self.h, C.byref(m), C.byref(p), hp(ucont), hp(ucat), hp(theta),
                                                      hp(rhs), float(scale), hp(cs), hp(nut), hp(btheta),
@@ -258,7 +270,7 @@ class RhsContext:
 
     def SNESFuncEval(self, mesh, params, u, ucont, ucat, nut, bU, ucont_o, dt, ab2, F):
         """SNESFuncEval (src/solvers/ueqn_solvers.c:1185-1272): F = the Crank-Nicolson residual at u."""
-        m, p = mesh_struct(mesh), abi.make_params(params)
+        m, p = self._mesh(mesh), abi.make_params(params)
         self._check(self.lib.tosca_snes_func_eval(self.h, C.byref(m), C.byref(p), _ptr(u), _ptr(ucont), _ptr(ucat),
                                                   _ptr(nut), _ptr(bU), _ptr(ucont_o), float(dt), int(ab2), _ptr(F)))
 
@@ -266,7 +278,7 @@ class RhsContext:
         """FormT (src/solvers/teqn_solvers.c:6-592): rhs (global scalar) at interior cells.
         tparams: dict(pr=..., wall_t=(iLeft, iRight, jLeft, jRight) thetaWallFunction flags,
         q_wall_i=tensor (3, mz, my) or None, q_wall_j=tensor (3, mz, mx) or None)."""
-        m, p = mesh_struct(mesh), abi.make_params(params)
+        m, p = self._mesh(mesh), abi.make_params(params)
         tp = abi.make_teqn_params(dict(tparams, q_wall_i=_ptr(tparams.get("q_wall_i")),
                                        q_wall_j=_ptr(tparams.get("q_wall_j"))))
         self._check(self.lib.tosca_formt(self.h, C.byref(m), C.byref(p), C.byref(tp), _ptr(ucont), _ptr(theta),
@@ -292,6 +304,9 @@ class RhsContext:
         return out
 
     def mesh_check(self, mesh):
+        """tosca_mesh_check: validates the mesh (nvert, meshTag, volumes, areas) and registers it with
+        the context; every operator taking this mesh requires it."""
         self._stream()
         m = mesh_struct(mesh)
         self._check(self.lib.tosca_mesh_check(self.h, C.byref(m)))
+        self._checked.add(tuple(_ptr(mesh[n]) for n in _CHECK_KEY))
