@@ -24,11 +24,12 @@
 // and the helpers of src/include/inline.h they call (cited at each helper below).
 //
 // Every function is pinned by tests under tests/test_oracle_*.py against facts outside its code
-// (closed forms, invariants, library routines, quadrature, brute force).  Parity unpinned: the
-// one-sided boundary branches of Compute_du_i/j/k (inline.h:2762-2930), QUICK clipping at
-// non-periodic boundary faces (inline.h:2215-2227) and the cell-centred versus face metrics of
-// FormU's face dot, which no closed form separates; they are cross-checked only through the two
-// independently structured GPU kernel sets agreeing with this file.  PETSc's
+// (closed forms, invariants, library routines, quadrature, brute force).  FormU's boundary
+// behaviour -- the one-sided corner rows of Compute_du_i/j/k (inline.h:2762-2930), QUICK clipping
+// at non-periodic boundary faces (inline.h:2215-2227), slip and wall-function faces, the fp ghost
+// copies and the cell-centred metric of the face dot -- is pinned on bounded lattices with a
+// linear velocity and NaN in every ghost point those rules never read
+// (tests/test_oracle_boundary_pins.py).  PETSc's
 // single-rank DMLocalToLocal scatter is restated from PETSc's documented DMDA semantics:
 // a ghost point of a DM_BOUNDARY_PERIODIC axis holds the owned value at (index mod m); ghost
 // points of a DM_BOUNDARY_GHOSTED axis are left untouched.
