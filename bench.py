@@ -2,7 +2,9 @@
 """Benchmark: momentum right-hand side + LES cell-updates/s on one MI355X.
 
 One step = one evaluation of UpdateCs -> UpdateNut -> Buoyancy -> FormU (formMode 0) over every
-interior cell of the workload (tosca_rhs_les_step), with all inputs resident in HBM.
+interior cell of the workload (tosca_rhs_les_step), with all inputs resident in HBM; on the
+512x512x192 shard also the wind farm's source (the VecAXPY of sourceFarmCont, a 4 x 4 actuator-disk
+farm, tosca_rhs_les_step_farm).
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--config 5] [--path fused|ref]
 
@@ -48,6 +50,9 @@ KERNEL_BYTES = {
 # Whole step: every ABI input read once, every output written once (rhs read + written);
 # nvert / meshTag are validated once per mesh (tosca_mesh_check) and not read per step.
 STEP_BYTES = (24 + 24 + 216 + 72 + 24 + 8 + 8 + 24) + (24 + 8 + 8 + 24)   # = 464
+# The wind-farm source of the shard workload (sourceFarmCont, a global vector) is read once more by
+# FormU's kernel and by the step: + 24 B/cell each.
+FARM_BYTES = 24
 
 
 def parse():
@@ -59,6 +64,7 @@ def parse():
     ap.add_argument("--path", default=os.environ.get("TOSCA_PATH", "default"))
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU-oracle baseline")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-farm", action="store_true", help="shard workload without its wind-farm source")
     return ap.parse_args()
 
 
@@ -93,7 +99,39 @@ F32_METRICS = ("csi", "eta", "zet", "icsi", "ieta", "izet", "jcsi", "jeta", "jze
                "iaj", "jaj", "kaj")
 
 
-def mixed_leg(torch, ctx, mesh, case, ucont, ucat, theta, lay, args, fp64_ms):
+def farm_leg(torch, ctx, mesh, case, cent, lay, args):
+    """The 4 x 4 actuator-disk farm of the shard: the Gaussian projection of the disks' force
+    (tosca_project_uadm) and its face fluxes (tosca_body_force_to_cont), made once per time step
+    in the reference (UpdateWindTurbines); the step then reads the face fluxes (sourceFarmCont)."""
+    from tosca_amd import farm as F
+    from tosca_amd.rhs import DeviceFarm
+    lengths = case.meta["lengths"]
+    dx = lengths[0] / (lay.mz - 2)
+    farm = F.shard_farm(dx=dx, lengths=(lengths[0], lengths[1]))
+    dfarm = DeviceFarm(farm)
+    scat = torch.zeros(lay.gshape + (3,), dtype=torch.float64, device="cuda")
+    scont = torch.zeros((lay.mz, lay.my, lay.mx, 3), dtype=torch.float64, device="cuda")
+    e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+    ctx.ProjectUADM(case.params, cent, dfarm, scat)
+    ctx.BodyForceToCont(mesh, case.params, scat, scont)
+    torch.cuda.synchronize()
+    e0.record()
+    ctx.ProjectUADM(case.params, cent, dfarm, scat)
+    e1.record()
+    ctx.BodyForceToCont(mesh, case.params, scat, scont)
+    e2.record()
+    torch.cuda.synchronize()
+    proj_ms, conv_ms = e0.elapsed_time(e1), e1.elapsed_time(e2)
+    print(f"# farm: {farm.n_rotors} rotors x {farm.n_points} points, projection {proj_ms:.3f} ms, "
+          f"face fluxes {conv_ms:.3f} ms", file=sys.stderr, flush=True)
+    del scat
+    info = {"rotors": farm.n_rotors, "points_per_rotor": farm.n_points, "model": "uniformADM, Ct 0.75, Uref 8 m/s",
+            "rotor_diameter_m": 126.0, "hub_height_m": 90.0, "spacing_m": 7 * 126.0, "epsilon_m": float(farm.eps[0]),
+            "projection_ms": proj_ms, "face_flux_ms": conv_ms}
+    return scont, info
+
+
+def mixed_leg(torch, ctx, mesh, case, ucont, ucat, theta, lay, args, fp64_ms, farm=None):
     """The same step with FP32 flow fields: tosca_rhs_les_step_mixed (FP64 metrics) and
     tosca_rhs_les_step_mixed_m32 (FormU on FP32 metric copies, made once before the timing, as the
     mesh is static).  Throughput, speed-up over the FP64 step, and per-field errors of one step
@@ -109,7 +147,18 @@ def mixed_leg(torch, ctx, mesh, case, ucont, ucat, theta, lay, args, fp64_ms):
     bt64 = torch.zeros_like(rhs64)
     cs64 = torch.zeros(gs, dtype=torch.float64, device=dev)
     nut64 = torch.zeros(gs, dtype=torch.float64, device=dev)
-    ctx.rhs_les_step(mesh, case.params, ucont, ucat, theta, rhs64, 1.0, cs64, nut64, bt64, 1.0)
+    if farm is None:
+        ctx.rhs_les_step(mesh, case.params, ucont, ucat, theta, rhs64, 1.0, cs64, nut64, bt64, 1.0)
+    else:
+        ctx.rhs_les_step_farm(mesh, case.params, ucont, ucat, theta, farm, rhs64, 1.0, cs64, nut64, bt64, 1.0)
+
+    def mixed(m32=None):
+        if farm is None and m32 is None:
+            return lambda: ctx.rhs_les_step_mixed(mesh, case.params, V32, u32, t32, rhs, 1.0, cs, nut, bt, 1.0)
+        if farm is None:
+            return lambda: ctx.rhs_les_step_mixed_m32(mesh, m32, case.params, V32, u32, t32, rhs, 1.0, cs, nut, bt, 1.0)
+        return lambda: ctx.rhs_les_step_mixed_farm(mesh, case.params, V32, u32, t32, farm, rhs, 1.0, cs, nut, bt, 1.0,
+                                                   mesh32=m32)
 
     def measure(step):
         for _ in range(args.warmup):
@@ -141,15 +190,14 @@ def mixed_leg(torch, ctx, mesh, case, ucont, ucat, theta, lay, args, fp64_ms):
         return ms, {k: v[0] / v[1] for k, v in tim.items()}, err
 
     out = {}
-    ms, kms, err = measure(lambda: ctx.rhs_les_step_mixed(mesh, case.params, V32, u32, t32, rhs, 1.0, cs, nut, bt, 1.0))
+    ms, kms, err = measure(mixed())
     value = lay.ncells / (ms * 1e-3)
     print(f"# mixed: {ms:.3f} ms/step, {value:.3e} cells/s, x{fp64_ms / ms:.2f} vs FP64; rel L2 errors "
           + ", ".join(f"{k} {v['rel_l2']:.1e}" for k, v in err.items()), file=sys.stderr, flush=True)
     out.update({"value": value, "unit": "cell-updates/s", "ms_per_step": ms, "speedup_vs_fp64": fp64_ms / ms,
                 "dtype": "f32 flow fields / f64 metrics and reductions", "errors_vs_fp64": err, "kernels_ms": kms})
     mesh32 = {k: mesh[k].float().contiguous() for k in F32_METRICS}
-    ms, kms, err = measure(lambda: ctx.rhs_les_step_mixed_m32(mesh, mesh32, case.params, V32, u32, t32, rhs, 1.0, cs,
-                                                              nut, bt, 1.0))
+    ms, kms, err = measure(mixed(mesh32))
     value = lay.ncells / (ms * 1e-3)
     print(f"# mixed, FP32 metric copies: {ms:.3f} ms/step, {value:.3e} cells/s, x{fp64_ms / ms:.2f} vs FP64; rel L2 "
           "errors " + ", ".join(f"{k} {v['rel_l2']:.1e}" for k, v in err.items()), file=sys.stderr, flush=True)
@@ -514,7 +562,7 @@ def main():
     lay = case.lay
     mesh = {k: v.contiguous() for k, v in case.mesh.items() if k != "cent"}
     ucont, ucat, theta = case.ucont.contiguous(), case.ucat.contiguous(), case.theta.contiguous()
-    del case.mesh["cent"]
+    cent = case.mesh.pop("cent").contiguous()
     gs, gl = lay.gshape, (lay.mz, lay.my, lay.mx, 3)
     rhs = torch.zeros(gl, dtype=torch.float64, device=dev)
     cs = torch.zeros(gs, dtype=torch.float64, device=dev)
@@ -526,9 +574,18 @@ def main():
 
     ctx = RhsContext(lay.info())
     ctx.mesh_check(mesh)
+    # the shard workload carries the wind farm: its face fluxes are made once, outside the timing,
+    # and the step adds them (VecAXPY of sourceFarmCont) in FormU's pass
+    farm, farm_info = None, None
+    if args.config == 5 and not args.no_farm:
+        farm, farm_info = farm_leg(torch, ctx, mesh, case, cent, lay, args)
+    del cent
 
     def step():
-        ctx.rhs_les_step(mesh, case.params, ucont, ucat, theta, rhs, 1.0, cs, nut, btheta, 1.0)
+        if farm is None:
+            ctx.rhs_les_step(mesh, case.params, ucont, ucat, theta, rhs, 1.0, cs, nut, btheta, 1.0)
+        else:
+            ctx.rhs_les_step_farm(mesh, case.params, ucont, ucat, theta, farm, rhs, 1.0, cs, nut, btheta, 1.0)
 
     for _ in range(args.warmup):
         step()
@@ -559,6 +616,9 @@ def main():
     dom = max(per, key=lambda k: per[k][0] * per[k][1])
     dom_ms = per[dom][0]
     dom_bytes = KERNEL_BYTES.get(dom)
+    if farm is not None and dom_bytes is not None and dom.startswith("k_fused_formu"):
+        dom_bytes += FARM_BYTES
+    step_bytes = STEP_BYTES + (FARM_BYTES if farm is not None else 0)
     achieved = dom_bytes * lay.ncells / (dom_ms * 1e-3) / 1e9 if dom_bytes else None
     traffic = None
     # HBM bytes per launch measured with rocprofv3 (FETCH_SIZE + WRITE_SIZE, calibrated on a copy)
@@ -573,7 +633,7 @@ def main():
             "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,
             "algorithmic_bytes_per_cell": dom_bytes, "cells_per_launch": lay.ncells, "avg_launch_ms": dom_ms,
             "copy_peak": copy_gbs, "frac_of_copy": (achieved / copy_gbs) if achieved else None}
-    step_gbs = STEP_BYTES * value / 1e9
+    step_gbs = step_bytes * value / 1e9
 
     line = {
         "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
@@ -582,14 +642,15 @@ def main():
         "config": {"workload": workload_name(args.config, case), "cells": lay.ncells,
                    "les_model": {0: "smagorinsky", 2: "dynamicSmagorinsky", 5: "dynamicPASD"}[case.params["les_model"]],
                    "div_scheme": {0: "central", 1: "quick"}[case.params["div_scheme"]],
-                   "path": os.environ.get("TOSCA_PATH", "default"), "parallelism": "single GPU"},
+                   "path": os.environ.get("TOSCA_PATH", "default"), "parallelism": "single GPU",
+                   "wind_farm": farm_info},
         "roofline": roof,
-        "step_roofline": {"bound": "hbm", "algorithmic_bytes_per_cell": STEP_BYTES, "achieved": step_gbs,
+        "step_roofline": {"bound": "hbm", "algorithmic_bytes_per_cell": step_bytes, "achieved": step_gbs,
                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": step_gbs / HBM_PEAK_GBS,
                           "frac_of_copy": step_gbs / copy_gbs},
         "kernels_ms": {k: v[0] for k, v in per.items()},
     }
-    line["mixed"] = mixed_leg(torch, ctx, mesh, case, ucont, ucat, theta, lay, args, ms)
+    line["mixed"] = mixed_leg(torch, ctx, mesh, case, ucont, ucat, theta, lay, args, ms, farm)
     line["vjp"] = vjp_leg(torch, ctx, mesh, case, ucont, ucat, theta, cs, lay, args, ms)
     line["explicit_rhs_u"] = explicit_leg(torch, ctx, mesh, case, ucont, ucat, nut, lay, args)
     line["formt"] = formt_leg(torch, ctx, mesh, case, ucont, theta, nut, lay, args)

@@ -493,6 +493,58 @@ int tosca_update_cartesian_bcs(tosca_ctx *ctx, const tosca_mesh *mesh, const tos
 int tosca_update_temperature_bcs(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_params *params,
                                  const int *bc_t, const double *t_value, double *theta);
 
+/* ---- wind-farm body force (the sourceFarmCont term of ExplicitRhsU) -------------------------- */
+
+/* Uniform actuator-disk rotors (the uniformADM turbine model; windTurbine members, src/include/
+ * turbines.h).  Device arrays of float64:
+ *   center [n_rotors][3]            rotor centres (wt->rotCenter)
+ *   sphere [n_rotors]               radius of the controlled-cell sphere, rTip + eps * nsigma
+ *                                   (turbines_parallel.c:57-80)
+ *   eps    [n_rotors]               Gaussian projection width (wt->eps, "epsilon")
+ *   points [n_rotors][n_points][3]  disk points (wt->uadm.points, meshUADM)
+ *   force  [n_rotors][n_points][3]  body force per unit density at each point (wt->uadm.B)
+ * nsigma is the projection cut-off in widths, sqrt(log(1/0.001)) in the reference
+ * (wt->prjNSigma, turbines_io.c:3469). */
+typedef struct {
+    int n_rotors;
+    int n_points;
+    double nsigma;
+    const double *center;
+    const double *sphere;
+    const double *eps;
+    const double *points;
+    const double *force;
+} tosca_uadm;
+
+/* projectBladeForce, uniformADM branch (src/turbines/turbines_force.c:717-932, 1179-1185): scat
+ * (farm->lsourceFarmCat, local AoS) is set to zero and, at every interior cell within a rotor's
+ * sphere (|cent - center| <= sphere), receives exp(-(r/eps)^2) / (eps^3 pi^1.5) * force for each
+ * of that rotor's points at a distance r < eps * nsigma, rotors and points in order; then the
+ * ghost scatter.  cent: mesh->lCent (local AoS).  The other turbine models (ADM, ALM, AFM) and the
+ * tower and nacelle forces are not covered. */
+int tosca_project_uadm(tosca_ctx *ctx, const tosca_params *params, const double *cent, const tosca_uadm *farm,
+                       double *scat);
+
+/* bodyForceCartesian2Contravariant (src/turbines/turbines_force.c:1741-1835): scont
+ * (farm->sourceFarmCont, global AoS) = 0, then at every interior cell the body force of scat
+ * averaged to the cell's +i, +j, +k faces and dotted with icsi, jeta, kzet. */
+int tosca_body_force_to_cont(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_params *params, const double *scat,
+                             double *scont);
+
+/* tosca_rhs_les_step with the wind-farm source: rhs += farm (VecAXPY(Rhs, 1, sourceFarmCont),
+ * src/solvers/ueqn_solvers.c:1340-1344, over every owned point) before the FormU accumulation,
+ * in one pass with it.  farm: global AoS (tosca_body_force_to_cont's scont). */
+int tosca_rhs_les_step_farm(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_params *params,
+                            const double *ucont, const double *ucat, const double *theta, const double *farm,
+                            double *rhs, double scale, double *cs, double *nut, double *btheta, double buoy_scale);
+
+/* tosca_rhs_les_step_mixed (mesh32 == NULL) or tosca_rhs_les_step_mixed_m32 (mesh32 != NULL) with
+ * the wind-farm source (farm: float64 global AoS, added in FP64 as in tosca_rhs_les_step_farm). */
+int tosca_rhs_les_step_mixed_farm(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_mesh_f32 *mesh32,
+                                  const tosca_params *params, const float *ucont, const float *ucat,
+                                  const float *theta, const double *farm, double *rhs, double scale, double *cs,
+                                  float *nut, double *btheta, double buoy_scale);
+
 #ifdef __cplusplus
 }
 #endif

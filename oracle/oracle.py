@@ -62,6 +62,9 @@ def load() -> C.CDLL:
         "oracle_ueqn_abcn": (I, [In, M, Pa, P, P, P, P, P, P, I, P, P, D, P, P, I, D, D, I, P, P, P]),
         "oracle_cn_rhs": (I, [In, Pa, P, P, P, I, P, P, P]),
         "oracle_snes_func_eval": (I, [In, M, Pa, P, P, P, P, P, P, D, I, P]),
+        "oracle_project_uadm": (I, [In, Pa, P, I, I, D, P, P, P, P, P, P]),
+        "oracle_body_force_to_cont": (I, [In, M, Pa, P, P]),
+        "oracle_rhs_les_step_farm": (I, [In, M, Pa, P, P, P, P, P, D, P, P, P, D]),
     }
     for n, (r, a) in sigs.items():
         f = getattr(lib, n)
@@ -321,3 +324,31 @@ def snes_func_eval(info, mesh, params, u, ucont, ucat, nut, bU, ucont_o, dt, ab2
     inf, m, p = abi.make_info(info), _mesh(mesh), abi.make_params(params)
     _check(lib.oracle_snes_func_eval(C.byref(inf), C.byref(m), C.byref(p), _p(u), _p(ucont), _p(ucat), _p(nut),
                                      _p(bU), _p(ucont_o), float(dt), int(ab2), _p(F)))
+
+
+def project_uadm(info, params, cent, farm, scat):
+    """projectBladeForce's uniformADM branch restated; scat (local AoS) is overwritten.  farm: a
+    tosca_amd.farm.UadmFarm (numpy arrays)."""
+    lib = load()
+    inf, p = abi.make_info(info), abi.make_params(params)
+    arr = {k: np.ascontiguousarray(getattr(farm, k), dtype=np.float64) for k in ("center", "sphere", "eps", "points", "force")}
+    from tosca_amd.farm import PRJ_NSIGMA
+    _check(lib.oracle_project_uadm(C.byref(inf), C.byref(p), _p(cent), int(farm.n_rotors), int(farm.n_points),
+                                   float(PRJ_NSIGMA), _p(arr["center"]), _p(arr["sphere"]), _p(arr["eps"]),
+                                   _p(arr["points"]), _p(arr["force"]), _p(scat)))
+
+
+def body_force_to_cont(info, mesh, params, scat, scont):
+    """bodyForceCartesian2Contravariant restated; scont (global AoS) is overwritten."""
+    lib = load()
+    inf, m, p = abi.make_info(info), _mesh(mesh), abi.make_params(params)
+    _check(lib.oracle_body_force_to_cont(C.byref(inf), C.byref(m), C.byref(p), _p(scat), _p(scont)))
+
+
+def rhs_les_step_farm(info, mesh, params, ucont, ucat, theta, farm, rhs, scale, cs, nut, btheta, buoy_scale=1.0):
+    """rhs += farm, then rhs_les_step."""
+    lib = load()
+    inf, m, p = abi.make_info(info), _mesh(mesh), abi.make_params(params)
+    _check(lib.oracle_rhs_les_step_farm(C.byref(inf), C.byref(m), C.byref(p), _p(ucont), _p(ucat), _p(theta),
+                                        _p(farm), _p(rhs), float(scale), _p(cs), _p(nut), _p(btheta),
+                                        float(buoy_scale)))

@@ -21,6 +21,8 @@
 //   the Cabot wall law   src/wallfunctions.c:280-585
 //   FormT            src/solvers/teqn_solvers.c:6-595
 //   GradP            src/peqn.c:3437-3773
+//   projectBladeForce (uniformADM branch) and bodyForceCartesian2Contravariant
+//                    src/turbines/turbines_force.c:717-932, 1741-1835
 // and the helpers of src/include/inline.h they call (cited at each helper below).
 //
 // Every function is pinned by tests under tests/test_oracle_*.py against facts outside its code
@@ -2276,6 +2278,91 @@ int oracle_temperature_bcs(const tosca_dmda_info *info, const tosca_mesh *m, con
             for (int64_t i = 0; i < mx; i++) lt[l.L(k, j, i)] = t[l.L(k, j, i)];
     local_to_local(l, lt);
     return TOSCA_OK;
+}
+
+// ================================================================================================
+// Wind-farm body force of uniform actuator disks
+// ================================================================================================
+
+// projectBladeForce, uniformADM branch (src/turbines/turbines_force.c:717-932): VecSet(lsourceFarmCat,
+// 0); for each rotor t, each disk point p, each controlled cell c of t (interior cells, k-j-i order,
+// with |cent - rotCenter| <= rTip + eps prjNSigma, turbines_parallel.c:57-121): r = |x_p - cent|,
+// and where r < eps prjNSigma, sCat += (exp(-(r/eps)^2) / (pow(eps, 3) pow(pi, 1.5))) B_p; then
+// DMLocalToLocal (:1184-1185).
+int oracle_project_uadm(const tosca_dmda_info *info, const tosca_params *p, const double *cent_, int nrot,
+                        int npts, double nsigma, const double *center, const double *sphere, const double *eps_,
+                        const double *points, const double *force, double *scat_) {
+    Lay l;
+    int rc = check_layout(info, p, l);
+    if (rc) return rc;
+    const V3 *cent = (const V3 *)cent_;
+    V3 *scat = (V3 *)scat_;
+    for (int64_t q = 0; q < l.nloc(); q++) scat[q] = V3{0, 0, 0};
+    for (int t = 0; t < nrot; t++) {
+        const double rPrj = eps_[t] * nsigma, eps = eps_[t];
+        std::vector<int64_t> cells;                                   // the sphere's controlled cells
+        for (int64_t k = 1; k < l.mz - 1; k++)
+            for (int64_t j = 1; j < l.my - 1; j++)
+                for (int64_t i = 1; i < l.mx - 1; i++) {
+                    const V3 c = cent[l.L(k, j, i)];
+                    const V3 d{c.x - center[3 * t], c.y - center[3 * t + 1], c.z - center[3 * t + 2]};
+                    if (sqrt(d.x * d.x + d.y * d.y + d.z * d.z) <= sphere[t]) cells.push_back(l.L(k, j, i));
+                }
+        for (int pp = 0; pp < npts; pp++) {
+            const double *xp = points + 3 * ((int64_t)t * npts + pp), *B = force + 3 * ((int64_t)t * npts + pp);
+            for (int64_t c : cells) {
+                const V3 r_c{xp[0] - cent[c].x, xp[1] - cent[c].y, xp[2] - cent[c].z};
+                const double r = sqrt(r_c.x * r_c.x + r_c.y * r_c.y + r_c.z * r_c.z);
+                if (r < rPrj) {
+                    const double pf = std::exp(-(r / eps) * (r / eps)) / (pow(eps, 3) * pow(M_PI, 1.5));
+                    scat[c].x += pf * B[0];
+                    scat[c].y += pf * B[1];
+                    scat[c].z += pf * B[2];
+                }
+            }
+        }
+    }
+    local_to_local(l, scat);
+    return TOSCA_OK;
+}
+
+// bodyForceCartesian2Contravariant (src/turbines/turbines_force.c:1741-1835): VecSet(sourceFarmCont,
+// 0); at the interior cells the Cartesian force averaged to the +i, +j, +k faces, dotted with
+// icsi, jeta, kzet.
+int oracle_body_force_to_cont(const tosca_dmda_info *info, const tosca_mesh *m, const tosca_params *p,
+                              const double *scat_, double *scont_) {
+    Lay l;
+    int rc = check_layout(info, p, l);
+    if (rc) return rc;
+    const V3 *s = (const V3 *)scat_, *icsi = (const V3 *)m->icsi, *jeta = (const V3 *)m->jeta,
+             *kzet = (const V3 *)m->kzet;
+    V3 *sc = (V3 *)scont_;
+    for (int64_t g = 0; g < l.mx * l.my * l.mz; g++) sc[g] = V3{0, 0, 0};
+    for (int64_t k = 1; k < l.mz - 1; k++)
+        for (int64_t j = 1; j < l.my - 1; j++)
+            for (int64_t i = 1; i < l.mx - 1; i++) {
+                const V3 a = s[l.L(k, j, i)], bi = s[l.L(k, j, i + 1)], bj = s[l.L(k, j + 1, i)], bk = s[l.L(k + 1, j, i)];
+                const V3 fI{0.5 * (a.x + bi.x), 0.5 * (a.y + bi.y), 0.5 * (a.z + bi.z)};
+                const V3 fJ{0.5 * (a.x + bj.x), 0.5 * (a.y + bj.y), 0.5 * (a.z + bj.z)};
+                const V3 fK{0.5 * (a.x + bk.x), 0.5 * (a.y + bk.y), 0.5 * (a.z + bk.z)};
+                const V3 n1 = icsi[l.L(k, j, i)], n2 = jeta[l.L(k, j, i)], n3 = kzet[l.L(k, j, i)];
+                V3 &o = sc[l.G(k, j, i)];
+                o.x = (fI.x * n1.x + fI.y * n1.y + fI.z * n1.z);
+                o.y = (fJ.x * n2.x + fJ.y * n2.y + fJ.z * n2.z);
+                o.z = (fK.x * n3.x + fK.y * n3.y + fK.z * n3.z);
+            }
+    return TOSCA_OK;
+}
+
+// tosca_rhs_les_step with the farm source: VecAXPY(Rhs, 1, sourceFarmCont), then the step
+int oracle_rhs_les_step_farm(const tosca_dmda_info *info, const tosca_mesh *m, const tosca_params *p,
+                             const double *ucont, const double *ucat, const double *theta, const double *farm,
+                             double *rhs, double scale, double *cs, double *nut, double *btheta, double buoy_scale) {
+    Lay l;
+    int rc = check_layout(info, p, l);
+    if (rc) return rc;
+    axpy(l.mx * l.my * l.mz, 1.0, (const V3 *)farm, (V3 *)rhs);
+    return oracle_rhs_les_step(info, m, p, ucont, ucat, theta, rhs, scale, cs, nut, btheta, buoy_scale);
 }
 #endif  // ORACLE_TANGENT
 

@@ -264,3 +264,50 @@ def test_grad_p_linear_pressure(shard):
     print(f"  GradP linear pressure: max relative deviation {worst:.2e}", flush=True)
     assert worst <= 1e-11
     ctx.close()
+
+
+def test_farm_fullsize(shard):
+    """The 4 x 4 actuator-disk farm on the shard: the projected force against the oracle, its
+    integral against the truncated Gaussian's mass times the rotors' total thrust (0.5 Ct Uref^2 A
+    each), and the step with the farm source, fused against reference-structured."""
+    import math
+    from scipy.special import erf
+    from oracle import oracle as O
+    from tosca_amd import farm as F
+    rhs, case, mesh = shard
+    lay = case.lay
+    farm = F.shard_farm()
+    ctx = _ctx(rhs, lay, "fused")
+    scat = torch.empty(lay.gshape + (3,), dtype=torch.float64, device="cuda")
+    scont = torch.empty((lay.mz, lay.my, lay.mx, 3), dtype=torch.float64, device="cuda")
+    ctx.ProjectUADM(case.params, case.mesh["cent"].contiguous(), rhs.DeviceFarm(farm), scat)
+    ctx.BodyForceToCont(mesh, case.params, scat, scont)
+    torch.cuda.synchronize()
+    vol = 1.0 / C.interior(mesh["aj"], lay)
+    total = (C.interior(scat, lay) * vol[..., None]).reshape(-1, 3).sum(0).cpu().numpy()
+    ns = F.PRJ_NSIGMA
+    mass = erf(ns) - 2.0 * ns * math.exp(-ns * ns) / math.sqrt(math.pi)
+    thrust = 16 * 0.5 * 0.75 * 8.0 ** 2 * math.pi * 63.0 ** 2 / (1.0 + 1e-12)
+    print(f"  farm: integral of the projected force {-total[0]:.6e}, expected {mass * thrust:.6e}", flush=True)
+    assert abs(-total[0] - mass * thrust) <= 2e-3 * mass * thrust
+    print("  farm: oracle projection on the host (a few seconds)", flush=True)
+    ref = np.empty(lay.gshape + (3,))
+    O.project_uadm(lay.info(), case.params, case.mesh["cent"].cpu().numpy(), farm, ref)
+    got = scat.cpu().numpy()
+    e = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+    print(f"  farm: scat relL2 vs oracle {e:.2e}", flush=True)
+    assert e <= 1e-13
+    del ref, got
+    res = {}
+    for path in ("fused", "ref"):
+        c2 = _ctx(rhs, lay, path)
+        o = _outs(lay)
+        c2.rhs_les_step_farm(mesh, case.params, case.ucont, case.ucat, case.theta, scont, o["rhs"], 1.0, o["cs"],
+                             o["nut"], o["btheta"], 1.0)
+        torch.cuda.synchronize()
+        res[path] = o["rhs"]
+        c2.close()
+    e = _rel(res["fused"], res["ref"])
+    print(f"  rhs_les_step_farm rhs fused vs reference-structured relL2={e:.2e}", flush=True)
+    assert e <= 1e-12
+    ctx.close()

@@ -115,6 +115,11 @@ def make_krylov_params(restart=30, max_it=1000, rtol=1e-30, atol=1e-5) -> Krylov
     return k
 
 
+class Uadm(C.Structure):
+    _fields_ = [("n_rotors", C.c_int), ("n_points", C.c_int), ("nsigma", C.c_double), ("center", C.c_void_p),
+                ("sphere", C.c_void_p), ("eps", C.c_void_p), ("points", C.c_void_p), ("force", C.c_void_p)]
+
+
 SIGNATURES = {
     "tosca_version": (C.c_char_p, []),
     "tosca_last_error": (C.c_char_p, [P]),
@@ -154,6 +159,11 @@ SIGNATURES = {
                              C.POINTER(KrylovParams), C.POINTER(KrylovResult)]),
     "tosca_ueqn_abcn": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P, P, P, P, I, P, P, D, P, P,
                             C.POINTER(KrylovParams), C.POINTER(KrylovResult)]),
+    "tosca_project_uadm": (I, [P, C.POINTER(Params), P, C.POINTER(Uadm), P]),
+    "tosca_body_force_to_cont": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P]),
+    "tosca_rhs_les_step_farm": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P, P, P, D, P, P, P, D]),
+    "tosca_rhs_les_step_mixed_farm": (I, [P, C.POINTER(Mesh), C.POINTER(MeshF32), C.POINTER(Params), P, P, P, P, P,
+                                          D, P, P, P, D]),
 }
 
 _lib: Optional[C.CDLL] = None

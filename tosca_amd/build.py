@@ -39,6 +39,7 @@ SOURCES = {
     "formt.hip": ["-ffp-contract=off"],
     "gradp.hip": ["-ffp-contract=off"],
     "krylov.hip": ["-ffp-contract=off"],
+    "farm.hip": ["-ffp-contract=off"],
 }
 
 

@@ -217,18 +217,19 @@ size_t ws_need(const tosca_ctx *c) {
 template <class T, class MT = double>
 void queue_step(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const T *ucont, const T *ucat,
                 const T *theta, double *rhs, double scale, double *cs, T *nut, double *btheta, double buoy_scale,
-                const tosca_mesh_f32 *m32 = nullptr) {
+                const tosca_mesh_f32 *m32 = nullptr, const double *farm = nullptr, double *ws = nullptr) {
     const Geo &g = c->geo;
     hipStream_t s = c->stream;
+    if (!ws) ws = c->ws;
     // UpdateCs (+ the interior of UpdateNut)
     if (p->les_model == TOSCA_LES_DSM) {
-        tosca::fused_dsm<T>(g, m, ucat, p->max_cs, cs, nut, c->ws, s);
+        tosca::fused_dsm<T>(g, m, ucat, p->max_cs, cs, nut, ws, s);
     } else if (p->les_model == TOSCA_LES_SMAGORINSKY) {
         tosca::launch_fill(cs, (long)g.nloc(), 0.0289, s);   // VecSet(lCs, std_cs), les.c:172-179
         tosca::fused_nut<T>(g, m, ucat, cs, nut, nullptr, s);
     } else {
         // plane-averaged dynamic model: reference-structured kernels (FP64 flow fields only)
-        tosca::ref_update_cs(g, m, p, (const double *)ucat, cs, c->ws, s);
+        tosca::ref_update_cs(g, m, p, (const double *)ucat, cs, ws, s);
         tosca::fused_nut<T>(g, m, ucat, cs, nut, nullptr, s);
     }
     if (p->les_model == TOSCA_LES_DSM) {
@@ -255,7 +256,7 @@ void queue_step(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const 
         A.iaj = m->iaj; A.jaj = m->jaj; A.kaj = m->kaj;
     }
     A.tauI = p->tau_wall_i; A.tauJ = p->tau_wall_j;
-    A.rhs = rhs; A.scale = scale; A.mode = 0;
+    A.rhs = rhs; A.farm = farm; A.scale = scale; A.mode = 0;
     A.quick = p->div_scheme == TOSCA_DIV_QUICK;
     A.les = 1;
     A.nu = p->nu;
@@ -264,7 +265,7 @@ void queue_step(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const 
     A.btheta = btheta;
     A.bscale = buoy_scale;
     A.tRef = p->t_ref;
-    tosca::fused_formu<T, MT>(g, A, c->ws + 3 * (size_t)g.nloc(), s);
+    tosca::fused_formu<T, MT>(g, A, ws + 3 * (size_t)g.nloc(), s);
 }
 
 // validation shared by the two mixed-precision steps (FP32 flow fields)
@@ -294,6 +295,23 @@ int check_mixed(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const 
     if (p->bc_u[2] == TOSCA_BCU_WALLFUNCTION || p->bc_u[3] == TOSCA_BCU_WALLFUNCTION)
         if ((rc = check_dev(c, p->tau_wall_j, 3 * (size_t)g.mz * g.mx * sizeof(double), "tau_wall_j"))) return rc;
     if ((rc = ensure_ws(c, ws_need(c)))) return rc;
+    return TOSCA_OK;
+}
+
+// the FP32 metric copies of the _m32 steps
+int check_mesh32(tosca_ctx *c, const tosca_mesh_f32 *m32) {
+    if (!m32) return set_err(c, TOSCA_ERR_ARG, "mesh32 is NULL");
+    int rc;
+    const size_t n = (size_t)c->geo.nloc(), v = 3 * n * sizeof(float), sc = n * sizeof(float);
+    const float *vv[12] = {m32->csi, m32->eta, m32->zet, m32->icsi, m32->ieta, m32->izet,
+                           m32->jcsi, m32->jeta, m32->jzet, m32->kcsi, m32->keta, m32->kzet};
+    const char *nm[12] = {"mesh32.csi", "mesh32.eta", "mesh32.zet", "mesh32.icsi", "mesh32.ieta", "mesh32.izet",
+                          "mesh32.jcsi", "mesh32.jeta", "mesh32.jzet", "mesh32.kcsi", "mesh32.keta", "mesh32.kzet"};
+    for (int q = 0; q < 12; q++)
+        if ((rc = check_dev(c, vv[q], v, nm[q]))) return rc;
+    if ((rc = check_dev(c, m32->iaj, sc, "mesh32.iaj")) || (rc = check_dev(c, m32->jaj, sc, "mesh32.jaj")) ||
+        (rc = check_dev(c, m32->kaj, sc, "mesh32.kaj")))
+        return rc;
     return TOSCA_OK;
 }
 
@@ -1185,14 +1203,24 @@ int tosca_buoyancy(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, con
     return launched(c, "Buoyancy");
 }
 
-int tosca_rhs_les_step(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const double *ucont,
-                       const double *ucat, const double *theta, double *rhs, double scale, double *cs, double *nut,
-                       double *btheta, double buoy_scale) {
+}  // extern "C"
+
+namespace {
+// tosca_rhs_les_step and tosca_rhs_les_step_farm (farm: the VecAXPY of sourceFarmCont, or null)
+int rhs_les_step_impl(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const double *ucont,
+                      const double *ucat, const double *theta, const double *farm, double *rhs, double scale,
+                      double *cs, double *nut, double *btheta, double buoy_scale) {
     int rc = check_ctx(c);
     if (rc) return rc;
     if ((rc = check_params(c, p))) return rc;
     if (!p->les_active) return set_err(c, TOSCA_ERR_ARG, "tosca_rhs_les_step needs les_active = 1");
+    const size_t ngv = 3 * (size_t)c->geo.mx * c->geo.my * c->geo.mz;
+    if (farm && (rc = check_dev(c, farm, ngv * sizeof(double), "farm"))) return rc;
     if (c->ref_path) {
+        if (farm) {
+            if ((rc = check_dev(c, rhs, ngv * sizeof(double), "rhs"))) return rc;
+            tosca::launch_axpy(rhs, 1.0, farm, (long)ngv, c->stream);
+        }
         if ((rc = tosca_update_cs(c, m, p, ucat, cs))) return rc;
         if ((rc = tosca_update_nut(c, m, p, ucat, cs, nut, nullptr))) return rc;
         if ((rc = tosca_buoyancy(c, m, p, theta, buoy_scale, btheta, nullptr))) return rc;
@@ -1217,8 +1245,24 @@ int tosca_rhs_les_step(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p,
     if (p->bc_u[2] == TOSCA_BCU_WALLFUNCTION || p->bc_u[3] == TOSCA_BCU_WALLFUNCTION)
         if ((rc = check_dev(c, p->tau_wall_j, 3 * (size_t)g.mz * g.mx * sizeof(double), "tau_wall_j"))) return rc;
     if ((rc = ensure_ws(c, ws_need(c)))) return rc;
-    queue_step<double>(c, m, p, ucont, ucat, theta, rhs, scale, cs, nut, btheta, buoy_scale);
+    queue_step<double>(c, m, p, ucont, ucat, theta, rhs, scale, cs, nut, btheta, buoy_scale, nullptr, farm);
     return launched(c, "rhs_les_step");
+}
+}  // namespace
+
+extern "C" {
+
+int tosca_rhs_les_step(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const double *ucont,
+                       const double *ucat, const double *theta, double *rhs, double scale, double *cs, double *nut,
+                       double *btheta, double buoy_scale) {
+    return rhs_les_step_impl(c, m, p, ucont, ucat, theta, nullptr, rhs, scale, cs, nut, btheta, buoy_scale);
+}
+
+int tosca_rhs_les_step_farm(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const double *ucont,
+                            const double *ucat, const double *theta, const double *farm, double *rhs, double scale,
+                            double *cs, double *nut, double *btheta, double buoy_scale) {
+    if (c && !farm) return set_err(c, TOSCA_ERR_ARG, "farm is NULL");
+    return rhs_les_step_impl(c, m, p, ucont, ucat, theta, farm, rhs, scale, cs, nut, btheta, buoy_scale);
 }
 
 int tosca_rhs_les_step_host(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const double *ucont,
@@ -1291,22 +1335,68 @@ int tosca_rhs_les_step_mixed(tosca_ctx *c, const tosca_mesh *m, const tosca_para
     return launched(c, "rhs_les_step_mixed");
 }
 
+int tosca_rhs_les_step_mixed_farm(tosca_ctx *c, const tosca_mesh *m, const tosca_mesh_f32 *m32, const tosca_params *p,
+                                  const float *ucont, const float *ucat, const float *theta, const double *farm,
+                                  double *rhs, double scale, double *cs, float *nut, double *btheta, double buoy_scale) {
+    int rc = check_mixed(c, m, p, ucont, ucat, theta, rhs, cs, nut, btheta);
+    if (rc) return rc;
+    if ((rc = check_dev(c, farm, 3 * (size_t)c->geo.mx * c->geo.my * c->geo.mz * sizeof(double), "farm"))) return rc;
+    if (m32) {
+        if ((rc = check_mesh32(c, m32))) return rc;
+        queue_step<float, float>(c, m, p, ucont, ucat, theta, rhs, scale, cs, nut, btheta, buoy_scale, m32, farm);
+    } else {
+        queue_step<float>(c, m, p, ucont, ucat, theta, rhs, scale, cs, nut, btheta, buoy_scale, nullptr, farm);
+    }
+    return launched(c, "rhs_les_step_mixed_farm");
+}
+
+int tosca_project_uadm(tosca_ctx *c, const tosca_params *p, const double *cent, const tosca_uadm *F, double *scat) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if ((rc = check_params(c, p))) return rc;
+    if (!F) return set_err(c, TOSCA_ERR_ARG, "farm is NULL");
+    if (F->n_rotors < 0 || (F->n_rotors > 0 && F->n_points < 1))
+        return set_err(c, TOSCA_ERR_ARG, "farm: n_rotors = %d, n_points = %d", F->n_rotors, F->n_points);
+    if (!(F->nsigma > 0.0)) return set_err(c, TOSCA_ERR_ARG, "farm: nsigma must be positive");
+    set_geo_per(c, p);
+    const Geo &g = c->geo;
+    const size_t n = (size_t)g.nloc();
+    if ((rc = check_dev(c, cent, 3 * n * sizeof(double), "cent")) || (rc = check_dev(c, scat, 3 * n * sizeof(double), "scat")))
+        return rc;
+    if (F->n_rotors > 0) {
+        const size_t nr = (size_t)F->n_rotors, np = nr * (size_t)F->n_points;
+        if ((rc = check_dev(c, F->center, 3 * nr * sizeof(double), "farm.center")) ||
+            (rc = check_dev(c, F->sphere, nr * sizeof(double), "farm.sphere")) ||
+            (rc = check_dev(c, F->eps, nr * sizeof(double), "farm.eps")) ||
+            (rc = check_dev(c, F->points, 3 * np * sizeof(double), "farm.points")) ||
+            (rc = check_dev(c, F->force, 3 * np * sizeof(double), "farm.force")))
+            return rc;
+    }
+    tosca::launch_uadm_project(cent, F, scat, g, c->stream);
+    return launched(c, "project_uadm");
+}
+
+int tosca_body_force_to_cont(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const double *scat,
+                             double *scont) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if ((rc = check_params(c, p))) return rc;
+    set_geo_per(c, p);
+    const Geo &g = c->geo;
+    if ((rc = check_mesh(c, m, true, false, false))) return rc;
+    if ((rc = check_dev(c, scat, 3 * (size_t)g.nloc() * sizeof(double), "scat")) ||
+        (rc = check_dev(c, scont, 3 * (size_t)g.mx * g.my * g.mz * sizeof(double), "scont")))
+        return rc;
+    tosca::launch_body_force_to_cont(scat, m, scont, g, c->stream);
+    return launched(c, "body_force_to_cont");
+}
+
 int tosca_rhs_les_step_mixed_m32(tosca_ctx *c, const tosca_mesh *m, const tosca_mesh_f32 *m32, const tosca_params *p,
                                  const float *ucont, const float *ucat, const float *theta, double *rhs, double scale,
                                  double *cs, float *nut, double *btheta, double buoy_scale) {
     int rc = check_mixed(c, m, p, ucont, ucat, theta, rhs, cs, nut, btheta);
     if (rc) return rc;
-    if (!m32) return set_err(c, TOSCA_ERR_ARG, "mesh32 is NULL");
-    const size_t n = (size_t)c->geo.nloc(), v = 3 * n * sizeof(float), sc = n * sizeof(float);
-    const float *vv[12] = {m32->csi, m32->eta, m32->zet, m32->icsi, m32->ieta, m32->izet,
-                           m32->jcsi, m32->jeta, m32->jzet, m32->kcsi, m32->keta, m32->kzet};
-    const char *nm[12] = {"mesh32.csi", "mesh32.eta", "mesh32.zet", "mesh32.icsi", "mesh32.ieta", "mesh32.izet",
-                          "mesh32.jcsi", "mesh32.jeta", "mesh32.jzet", "mesh32.kcsi", "mesh32.keta", "mesh32.kzet"};
-    for (int q = 0; q < 12; q++)
-        if ((rc = check_dev(c, vv[q], v, nm[q]))) return rc;
-    if ((rc = check_dev(c, m32->iaj, sc, "mesh32.iaj")) || (rc = check_dev(c, m32->jaj, sc, "mesh32.jaj")) ||
-        (rc = check_dev(c, m32->kaj, sc, "mesh32.kaj")))
-        return rc;
+    if ((rc = check_mesh32(c, m32))) return rc;
     queue_step<float, float>(c, m, p, ucont, ucat, theta, rhs, scale, cs, nut, btheta, buoy_scale, m32);
     return launched(c, "rhs_les_step_mixed_m32");
 }

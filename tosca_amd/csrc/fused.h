@@ -19,6 +19,7 @@ struct FormUArgsT {
     const MT *iaj, *jaj, *kaj;
     const double *tauI, *tauJ;
     double *rhs;
+    const double *farm;    // wind-farm source added to rhs before the accumulation (nullptr: none)
     double scale;
     int mode, quick, les;
     double nu;
@@ -51,6 +52,10 @@ void fused_nut(const Geo &g, const tosca_mesh *m, const T *ucat, const double *c
 // the interior being [1+lo, n-2-hi] on each axis in array coordinates
 template <class T>
 void zero_shell(T *a, int ncomp, int nx, int ny, int nz, int lo, int hi, hipStream_t s);
+
+// y += x (VecAXPY with a = 1) on the points of an nx*ny*nz vector array outside [1, n-2] on every
+// axis: the part of the wind-farm AXPY the fused FormU kernel does not cover
+void shell_add(double *y, const double *x, int nx, int ny, int nz, hipStream_t s);
 
 // zero every point of a local scalar array outside the interior cells
 template <class T>

@@ -180,6 +180,7 @@ struct MainArgs {
     const MT *icsi, *ieta, *izet, *jcsi, *jeta, *jzet, *kcsi, *keta, *kzet;
     const MT *iaj, *jaj, *kaj;
     double *rhs;
+    const double *farm;    // added to rhs at the output cells before the accumulation (nullptr: none)
     const T *theta;
     double *btheta;
     const double *bF1, *bF2, *bF3, *sI, *sJ, *sK;
@@ -442,6 +443,11 @@ __global__ void __launch_bounds__(64 * NR) k_fused_formu(MainArgs<T, MT> A, Geo 
             etA = cvt3<double>(ld3(peta + 3 * qp, 0)); etB = cvt3<double>(ld3(peta + 3 * qp, SJ));
             zeA = cvt3<double>(ld3(pzet + 3 * qp, 0)); zeB = cvt3<double>(ld3(pzet + 3 * qp, SK));
             r0 = prhs[ro]; r1 = prhs[ro + 1]; r2 = prhs[ro + 2];
+            if (A.farm) {
+                // VecAXPY(Rhs, 1, sourceFarmCont) before FormU's accumulation (ueqn_solvers.c:1341-1347)
+                const double *pf = A.farm + (prhs - A.rhs);
+                r0 = r0 + 1.0 * pf[ro]; r1 = r1 + 1.0 * pf[ro + 1]; r2 = r2 + 1.0 * pf[ro + 2];
+            }
             // theta, like nu_t, must be a valid local array even when Buoyancy is off (the launcher
             // substitutes one); only the buoy flag decides whether it is read
             if (buoy) { th = ptheta[qp]; thI = ptheta[qp + 1]; thJ = ptheta[qp + SJ]; thK = ptheta[qp + SK]; }
@@ -492,6 +498,7 @@ __global__ void __launch_bounds__(64 * NR) k_fused_formu(MainArgs<T, MT> A, Geo 
             const double rz = A.scale * fa2p * (0.5 * (v1 + v2));
             prhs[ro] = r0 + rx;
             if (!seamRow) prhs[ro + 1] = r1 + ry;
+            else if (A.farm) prhs[ro + 1] = r1;       // k_formu_jseam adds the dot to rhs + farm
             prhs[ro + 2] = r2 + rz;
             if (buoy) {
                 // Boussinesq source on the three + faces of the cell (ueqn_sources.c:2812-2870):
@@ -552,6 +559,7 @@ void fused_formu(const Geo &g, const FormUArgsT<T, MT> &A, double *ws, hipStream
     M.icsi = A.icsi; M.ieta = A.ieta; M.izet = A.izet; M.jcsi = A.jcsi; M.jeta = A.jeta; M.jzet = A.jzet;
     M.kcsi = A.kcsi; M.keta = A.keta; M.kzet = A.kzet; M.iaj = A.iaj; M.jaj = A.jaj; M.kaj = A.kaj;
     M.rhs = A.rhs;
+    M.farm = A.farm;
     M.theta = A.btheta ? A.theta : A.ucat;          // any valid local array when Buoyancy is off
     M.btheta = A.btheta ? A.btheta : A.rhs;
     M.bF1 = B.bF[0]; M.bF2 = B.bF[1]; M.bF3 = B.bF[2]; M.sI = B.seam[0]; M.sJ = B.seam[1]; M.sK = B.seam[2];
@@ -567,6 +575,8 @@ void fused_formu(const Geo &g, const FormUArgsT<T, MT> &A, double *ws, hipStream
     chunks = (nk + kchunk - 1) / kchunk;
     dim3 grid((g.mx - 2 + TI - 1) / TI, (g.my - 2 + TJ - 1) / TJ, chunks);
     TOSCA_LAUNCH(sizeof(T) == 8 ? "k_fused_formu" : sizeof(MT) == 8 ? "k_fused_formu_f32" : "k_fused_formu_f32m", (k_fused_formu<NR, T, MT>), grid, dim3(64, NR), 0, s, M, g, kchunk);
+    // the farm AXPY on the owned points outside the interior cells (the kernel covers the interior)
+    if (A.farm) shell_add(A.rhs, A.farm, g.mx, g.my, g.mz, s);
     if (nsj > 1)
         TOSCA_LAUNCH("k_formu_jseam", k_formu_jseam<MT>, dim3((g.mx + 127) / 128, g.mz, nsj - 1), dim3(128), 0, s,
                      A.rhs, sv1, sv2, A.jaj, A.scale, g, TJ);

@@ -308,6 +308,33 @@ void zero_shell(T *a, int ncomp, int nx, int ny, int nz, int lo, int hi, hipStre
     TOSCA_LAUNCH("k_zero_box", k_zero_box<T>, dim3(512, 6), dim3(256), 0, s, a, ncomp, nx, ny, B);
 }
 
+// y = y + 1 x over the same six boxes (three values per point)
+__global__ void k_add_box(double *y, const double *x, int nx, int ny, ShellBoxes B) {
+    const int b = blockIdx.y;
+    if (B.ni[b] <= 0 || B.nj[b] <= 0 || B.nk[b] <= 0) return;
+    const long row = (long)B.ni[b] * 3, tot = row * B.nj[b] * B.nk[b];
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += (long)gridDim.x * blockDim.x) {
+        const long e = t % row, r = t / row;
+        const int jj = (int)(r % B.nj[b]), kk = (int)(r / B.nj[b]);
+        const long p = ((long)(B.k0[b] + kk) * ny + (B.j0[b] + jj)) * nx + B.i0[b];
+        y[p * 3 + e] = y[p * 3 + e] + 1.0 * x[p * 3 + e];
+    }
+}
+
+void shell_add(double *y, const double *x, int nx, int ny, int nz, hipStream_t s) {
+    ShellBoxes B{};
+    auto box = [&](int q, int i0, int j0, int k0, int ni, int nj, int nk) {
+        B.i0[q] = i0; B.j0[q] = j0; B.k0[q] = k0; B.ni[q] = ni; B.nj[q] = nj; B.nk[q] = nk;
+    };
+    box(0, 0, 0, 0, nx, ny, 1);
+    box(1, 0, 0, nz - 1, nx, ny, 1);
+    box(2, 0, 0, 1, nx, 1, nz - 2);
+    box(3, 0, ny - 1, 1, nx, 1, nz - 2);
+    box(4, 0, 1, 1, 1, ny - 2, nz - 2);
+    box(5, nx - 1, 1, 1, 1, ny - 2, nz - 2);
+    TOSCA_LAUNCH("k_add_box", k_add_box, dim3(256, 6), dim3(256), 0, s, y, x, nx, ny, B);
+}
+
 template <class T>
 void zero_outside_interior(T *a, const Geo &g, hipStream_t s) {
     zero_shell(a, 1, g.gxm, g.gym, g.gzm, GW, GW, s);

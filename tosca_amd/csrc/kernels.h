@@ -70,6 +70,11 @@ void ref_formt(const Geo &g, const tosca_mesh *m, const tosca_params *p, const t
                const double *ucont, const double *theta, const double *nut, const double *kt, double *rhs,
                double scale, int mode, double *ws, hipStream_t s);
 
+// wind-farm body force (farm.hip): projectBladeForce's uniformADM branch into the local scat, and
+// bodyForceCartesian2Contravariant into the global scont
+void launch_uadm_project(const double *cent, const tosca_uadm *F, double *scat, const Geo &g, hipStream_t s);
+void launch_body_force_to_cont(const double *scat, const tosca_mesh *m, double *scont, const Geo &g, hipStream_t s);
+
 // mesh validation (api.hip): counts of offending points, written to `out` (device, 4 ints)
 void launch_mesh_check(const Geo &g, const tosca_mesh *m, int *out, hipStream_t s);
 

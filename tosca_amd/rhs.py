@@ -284,6 +284,42 @@ class RhsContext:
         self._check(self.lib.tosca_formt(self.h, C.byref(m), C.byref(p), C.byref(tp), _ptr(ucont), _ptr(theta),
                                          _ptr(nut), _ptr(kt), _ptr(rhs), float(scale), int(form_mode)))
 
+    # ---- wind-farm body force (src/turbines/turbines_force.c) ---------------------------------
+    def ProjectUADM(self, params, cent, farm: "DeviceFarm", scat):
+        """projectBladeForce, uniformADM branch (turbines_force.c:717-932): scat (local AoS) from the
+        rotors of farm, cent = mesh->lCent (local AoS)."""
+        self._stream()
+        p = abi.make_params(params)
+        self._check(self.lib.tosca_project_uadm(self.h, C.byref(p), _ptr(cent), C.byref(farm.struct), _ptr(scat)))
+
+    def BodyForceToCont(self, mesh, params, scat, scont):
+        """bodyForceCartesian2Contravariant (turbines_force.c:1741-1835): scont (global AoS)."""
+        self._stream()
+        m, p = self._mesh(mesh), abi.make_params(params)
+        self._check(self.lib.tosca_body_force_to_cont(self.h, C.byref(m), C.byref(p), _ptr(scat), _ptr(scont)))
+
+    def rhs_les_step_farm(self, mesh, params, ucont, ucat, theta, farm, rhs, scale, cs, nut, btheta,
+                          buoy_scale=1.0):
+        """rhs += farm (sourceFarmCont, global AoS), then rhs_les_step, in one pass."""
+        self._stream()
+        m, p = self._mesh(mesh), abi.make_params(params)
+        self._check(self.lib.tosca_rhs_les_step_farm(self.h, C.byref(m), C.byref(p), _ptr(ucont), _ptr(ucat),
+                                                     _ptr(theta), _ptr(farm), _ptr(rhs), float(scale), _ptr(cs),
+                                                     _ptr(nut), _ptr(btheta), float(buoy_scale)))
+
+    def rhs_les_step_mixed_farm(self, mesh, params, ucont, ucat, theta, farm, rhs, scale, cs, nut, btheta,
+                                buoy_scale=1.0, mesh32=None):
+        """rhs_les_step_mixed (or _m32 when mesh32, the float32 metric copies, is given) with the
+        float64 farm source added first."""
+        self._stream()
+        m, p = self._mesh(mesh), abi.make_params(params)
+        m32 = None
+        if mesh32 is not None:
+            m32 = C.byref(abi.make_mesh_f32({n: _p32(mesh32[n]) for n in abi.MESH_F32_FIELDS}))
+        self._check(self.lib.tosca_rhs_les_step_mixed_farm(self.h, C.byref(m), m32, C.byref(p), _p32(ucont),
+                                                           _p32(ucat), _p32(theta), _ptr(farm), _ptr(rhs), float(scale),
+                                                           _ptr(cs), _p32(nut), _ptr(btheta), float(buoy_scale)))
+
     def enable_timing(self, on: bool = True):
         self._check(self.lib.tosca_ctx_enable_timing(self.h, int(on)))
 
@@ -310,3 +346,18 @@ class RhsContext:
         m = mesh_struct(mesh)
         self._check(self.lib.tosca_mesh_check(self.h, C.byref(m)))
         self._checked.add(tuple(_ptr(mesh[n]) for n in _CHECK_KEY))
+
+
+class DeviceFarm:
+    """A tosca_amd.farm.UadmFarm on the device: float64 CUDA tensors and the tosca_uadm struct that
+    points at them (kept alive together)."""
+
+    def __init__(self, farm, device="cuda"):
+        from .farm import PRJ_NSIGMA
+        self.t = {k: torch.as_tensor(getattr(farm, k), dtype=torch.float64).contiguous().to(device)
+                  for k in ("center", "sphere", "eps", "points", "force")}
+        u = abi.Uadm()
+        u.n_rotors, u.n_points, u.nsigma = int(farm.n_rotors), int(farm.n_points), float(PRJ_NSIGMA)
+        for k, v in self.t.items():
+            setattr(u, k, v.data_ptr() if v.numel() else None)
+        self.struct = u
