@@ -54,6 +54,8 @@ def load() -> C.CDLL:
         "oracle_formt": (I, [In, M, Pa, C.POINTER(abi.TeqnParams), P, P, P, P, P, D, I]),
         "oracle_ueqn_fe": (I, [In, M, Pa, P, P, P, P, P, P, I, P, P, D, P, I]),
         "oracle_ueqn_rk4": (I, [In, M, Pa, P, P, P, P, P, P, I, P, P, D, P, P, I]),
+        "oracle_ueqn_rk3w": (I, [In, M, Pa, P, P, P, P, P, P, I, P, D, I, P, P, P, I]),
+        "oracle_rk3w_jvp": (I, [In, M, Pa, P, P, P, P, P, P, I, P, D, I, P, P, I, P, P]),
         "oracle_imex_matvec": (I, [In, M, Pa, P, P, P, P, P, D, P]),
         "oracle_grad_p": (I, [In, M, Pa, P, P]),
         "oracle_gmres_dense": (I, [C.c_int64, P, P, P, I, D, D, I, P, P, P]),
@@ -238,6 +240,32 @@ def ueqn_rk4(info, mesh, params, ucont, ucat, nut, dp, btheta, btheta_o, ab2, fa
     _check(lib.oracle_ueqn_rk4(C.byref(inf), C.byref(m), C.byref(p), _p(ucont), _p(ucat), _p(nut), _p(dp), _p(btheta),
                                _p(btheta_o), int(ab2), _p(farm), _p(ucont_o), float(dt), _p(rhs), _p(utmp),
                                int(form_mode)))
+
+
+def ueqn_rk3w(info, mesh, params, ucont, ucat, nut, dp, btheta, btheta_o, ab2, farm, dt, n_stages, ucont_o, rhs,
+              rhs_o, form_mode=0):
+    """The explicit Wray Runge-Kutta predictor restated: n_stages stages on ucont (local, in
+    place); ucont_o, rhs, rhs_o (global) are set as the reference's Ucont_o, Rhs, Rhs_o."""
+    lib = load()
+    inf, m, p = abi.make_info(info), _mesh(mesh), abi.make_params(params)
+    _check(lib.oracle_ueqn_rk3w(C.byref(inf), C.byref(m), C.byref(p), _p(ucont), _p(ucat), _p(nut), _p(dp),
+                                _p(btheta), _p(btheta_o), int(ab2), _p(farm), float(dt), int(n_stages), _p(ucont_o),
+                                _p(rhs), _p(rhs_o), int(form_mode)))
+
+
+def rk3w_jvp(info, mesh, params, ucont, ucat, nut, dp, btheta, btheta_o, ab2, farm, dt, n_stages, d_ucont=None,
+             d_nut=None, form_mode=0):
+    """The final Ucont (global) of ueqn_rk3w from the local ucont / ucat given, and its exact
+    tangent for the directions d_ucont (global, the initial Ucont) and d_nut (local nu_t).
+    Returns (ucont_final, d_ucont_final); the inputs are not modified."""
+    lib = load()
+    inf, m, p = abi.make_info(info), _mesh(mesh), abi.make_params(params)
+    shape = (info["mz"], info["my"], info["mx"], 3)
+    uf, duf = np.zeros(shape), np.zeros(shape)
+    _check(lib.oracle_rk3w_jvp(C.byref(inf), C.byref(m), C.byref(p), _p(ucont), _p(ucat), _p(nut), _p(dp),
+                               _p(btheta), _p(btheta_o), int(ab2), _p(farm), float(dt), int(n_stages), _p(d_ucont),
+                               _p(d_nut), int(form_mode), _p(uf), _p(duf)))
+    return uf, duf
 
 
 def imex_matvec(info, mesh, params, v, ucont, ucat, nut, ucont_o, coef, av):

@@ -1078,10 +1078,10 @@ int oracle_rhs_les_step(const tosca_dmda_info *info, const tosca_mesh *m, const 
 }
 #endif  // ORACLE_TANGENT
 
-#ifndef ORACLE_TANGENT
 // ================================================================================================
 // ExplicitRhsU (src/solvers/ueqn_solvers.c:1276-1407), the terms of the momentum right-hand side
-// this path covers.  On one rank a PETSc global vector is the owned part of its local vector, so
+// this path covers.  Written over Real like FormU, so the tangent build differentiates it (and
+// the explicit Wray Runge-Kutta predictor built on it, below).  On one rank a PETSc global vector is the owned part of its local vector, so
 // the "globalToLocal" resets below act on the local arrays and the GlobalToLocal scatter that
 // follows each of them is the periodic ghost fill of local_to_local.
 // ================================================================================================
@@ -1114,30 +1114,30 @@ static void reset_no_penetration(const Lay &l, const tosca_params *p, V3 *ucont)
 }
 
 // contravariantToCartesian (src/ueqn.c:528-635): interior cells, Cramer's rule on the cell basis
-static void contravariant_to_cartesian(const Lay &l, const tosca_mesh *m, const V3 *lucont, V3 *ucat) {
+static void contravariant_to_cartesian(const Lay &l, const ORACLE_MESH *m, const V3 *lucont, V3 *ucat) {
     const V3 *csi = (const V3 *)m->csi, *eta = (const V3 *)m->eta, *zet = (const V3 *)m->zet;
     for (int64_t k = 1; k < l.mz - 1; k++)
         for (int64_t j = 1; j < l.my - 1; j++)
             for (int64_t i = 1; i < l.mx - 1; i++) {
                 const int64_t c = l.L(k, j, i);
-                double mat[3][3] = {{csi[c].x, csi[c].y, csi[c].z}, {eta[c].x, eta[c].y, eta[c].z},
-                                    {zet[c].x, zet[c].y, zet[c].z}};
-                double q[3];
+                Real mat[3][3] = {{csi[c].x, csi[c].y, csi[c].z}, {eta[c].x, eta[c].y, eta[c].z},
+                                  {zet[c].x, zet[c].y, zet[c].z}};
+                Real q[3];
                 q[0] = 0.5 * (lucont[l.L(k, j, i - 1)].x + lucont[c].x);     // central (inline.h:4307-4312)
                 q[1] = 0.5 * (lucont[l.L(k, j - 1, i)].y + lucont[c].y);
                 q[2] = 0.5 * (lucont[l.L(k - 1, j, i)].z + lucont[c].z);
-                const double det = mat[0][0] * (mat[1][1] * mat[2][2] - mat[1][2] * mat[2][1]) -
-                                   mat[0][1] * (mat[1][0] * mat[2][2] - mat[1][2] * mat[2][0]) +
-                                   mat[0][2] * (mat[1][0] * mat[2][1] - mat[1][1] * mat[2][0]);
-                const double det0 = q[0] * (mat[1][1] * mat[2][2] - mat[1][2] * mat[2][1]) -
-                                    q[1] * (mat[0][1] * mat[2][2] - mat[0][2] * mat[2][1]) +
-                                    q[2] * (mat[0][1] * mat[1][2] - mat[0][2] * mat[1][1]);
-                const double det1 = -q[0] * (mat[1][0] * mat[2][2] - mat[1][2] * mat[2][0]) +
-                                    q[1] * (mat[0][0] * mat[2][2] - mat[0][2] * mat[2][0]) -
-                                    q[2] * (mat[0][0] * mat[1][2] - mat[0][2] * mat[1][0]);
-                const double det2 = q[0] * (mat[1][0] * mat[2][1] - mat[1][1] * mat[2][0]) -
-                                    q[1] * (mat[0][0] * mat[2][1] - mat[0][1] * mat[2][0]) +
-                                    q[2] * (mat[0][0] * mat[1][1] - mat[0][1] * mat[1][0]);
+                const Real det = mat[0][0] * (mat[1][1] * mat[2][2] - mat[1][2] * mat[2][1]) -
+                                 mat[0][1] * (mat[1][0] * mat[2][2] - mat[1][2] * mat[2][0]) +
+                                 mat[0][2] * (mat[1][0] * mat[2][1] - mat[1][1] * mat[2][0]);
+                const Real det0 = q[0] * (mat[1][1] * mat[2][2] - mat[1][2] * mat[2][1]) -
+                                  q[1] * (mat[0][1] * mat[2][2] - mat[0][2] * mat[2][1]) +
+                                  q[2] * (mat[0][1] * mat[1][2] - mat[0][2] * mat[1][1]);
+                const Real det1 = -q[0] * (mat[1][0] * mat[2][2] - mat[1][2] * mat[2][0]) +
+                                  q[1] * (mat[0][0] * mat[2][2] - mat[0][2] * mat[2][0]) -
+                                  q[2] * (mat[0][0] * mat[1][2] - mat[0][2] * mat[1][0]);
+                const Real det2 = q[0] * (mat[1][0] * mat[2][1] - mat[1][1] * mat[2][0]) -
+                                  q[1] * (mat[0][0] * mat[2][1] - mat[0][1] * mat[2][0]) +
+                                  q[2] * (mat[0][0] * mat[1][1] - mat[0][1] * mat[1][0]);
                 ucat[c] = V3{det0 / det, det1 / det, det2 / det};
             }
     local_to_local(l, ucat);      // DMGlobalToLocal (:621-622)
@@ -1175,10 +1175,10 @@ static void axpy(int64_t n, double a, const V3 *x, V3 *y) {
     }
 }
 
-int oracle_explicit_rhs_u(const tosca_dmda_info *info, const tosca_mesh *m, const tosca_params *p,
-                          double *ucont_, double *ucat_, const double *nut, const double *dp,
-                          const double *btheta, const double *btheta_o, int ab2, const double *farm,
-                          const double *ucont_o, double *rhs_, int formMode) {
+int ORACLE_API(oracle_explicit_rhs_u)(const tosca_dmda_info *info, const ORACLE_MESH *m, const tosca_params *p,
+                                      Real *ucont_, Real *ucat_, const Real *nut, const Real *dp,
+                                      const Real *btheta, const Real *btheta_o, int ab2, const Real *farm,
+                                      const Real *ucont_o, Real *rhs_, int formMode) {
     Lay l;
     int rc = check_layout(info, p, l);
     if (rc) return rc;
@@ -1203,7 +1203,7 @@ int oracle_explicit_rhs_u(const tosca_dmda_info *info, const tosca_mesh *m, cons
         }
     }
     if (farm) axpy(ng, 1.0, (const V3 *)farm, rhs);          // :1341-1344
-    rc = oracle_formu(info, m, p, ucont_, ucat_, nut, rhs_, 1.0, formMode);   // :1347
+    rc = ORACLE_API(oracle_formu)(info, m, p, ucont_, ucat_, nut, rhs_, 1.0, formMode);   // :1347
     if (rc) return rc;
     reset_nonresolved(l, p, rhs, (const V3 *)ucont_o);       // :1403
     return TOSCA_OK;
@@ -1221,6 +1221,59 @@ static void global_vec_to_local(const Lay &l, const V3 *g, V3 *loc) {
             for (int64_t i = 0; i < l.mx; i++) loc[l.L(k, j, i)] = g[l.G(k, j, i)];
     local_to_local(l, loc);
 }
+
+// the owned range of a local vector as a global vector
+static void local_vec_to_global(const Lay &l, const V3 *loc, V3 *g) {
+    for (int64_t k = 0; k < l.mz; k++)
+        for (int64_t j = 0; j < l.my; j++)
+            for (int64_t i = 0; i < l.mx; i++) g[l.G(k, j, i)] = loc[l.L(k, j, i)];
+}
+
+// Explicit Wray Runge-Kutta predictor: the three-stage low-storage scheme of UeqnRK3CN_W
+// (src/solvers/ueqn_solvers.c:1508-1583) with its coefficients beta = (8/15, 5/12, 3/4),
+// delta = (0, 17/60, 5/12) (:1518-1520) and the viscous term explicit as well, i.e. with the
+// whole right-hand side in the beta / delta terms (beta_i - delta_i = 2 gamma_i).  Stage i:
+//   Rhs = ExplicitRhsU(formMode) (its resets update Ucont in place);
+//   bU = Ucont; bU += dt beta_i Rhs; bU += -1.0 dt delta_i Rhs_o  (VecCopy / VecAXPY order, :1533-1541);
+//   Ucont = bU; GlobalToLocal; Rhs_o = Rhs (:1563-1569).
+// n_stages stages run back to back; every third stage begins a time step, where the time loop sets
+// Ucont_o = Ucont (the input of resetNonResolvedCellFaces) and the predictor sets Rhs_o = 0
+// (:1522).  The sources and nu_t are held fixed.  ucont_o, rhs, rhs_o: global (rhs: last Rhs).
+int ORACLE_API(oracle_ueqn_rk3w)(const tosca_dmda_info *info, const ORACLE_MESH *m, const tosca_params *p,
+                                 Real *ucont_, Real *ucat_, const Real *nut, const Real *dp, const Real *btheta,
+                                 const Real *btheta_o, int ab2, const Real *farm, double dt, int n_stages,
+                                 Real *ucont_o_, Real *rhs_, Real *rhs_o_, int formMode) {
+    Lay l;
+    int rc = check_layout(info, p, l);
+    if (rc) return rc;
+    if (!ucont_ || !ucat_ || !ucont_o_ || !rhs_ || !rhs_o_ || n_stages < 0) {
+        g_err = "ueqn_rk3w: missing array or negative stage count";
+        return TOSCA_ERR_ARG;
+    }
+    const double beta[3] = {8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};
+    const double delta[3] = {0.0, 17.0 / 60.0, 5.0 / 12.0};
+    const int64_t ng = l.mx * l.my * l.mz;
+    V3 *ucont = (V3 *)ucont_, *rhs = (V3 *)rhs_, *rhs_o = (V3 *)rhs_o_, *uo = (V3 *)ucont_o_;
+    std::vector<V3> bU(ng);
+    for (int s = 0; s < n_stages; s++) {
+        const int i = s % 3;
+        if (i == 0) {
+            local_vec_to_global(l, ucont, uo);
+            for (int64_t g = 0; g < ng; g++) rhs_o[g] = V3{0.0, 0.0, 0.0};
+        }
+        rc = ORACLE_API(oracle_explicit_rhs_u)(info, m, p, ucont_, ucat_, nut, dp, btheta, btheta_o, ab2, farm,
+                                               ucont_o_, rhs_, formMode);
+        if (rc) return rc;
+        local_vec_to_global(l, ucont, bU.data());
+        axpy(ng, dt * beta[i], rhs, bU.data());
+        axpy(ng, -1.0 * dt * delta[i], rhs_o, bU.data());
+        global_vec_to_local(l, bU.data(), ucont);
+        for (int64_t g = 0; g < ng; g++) rhs_o[g] = rhs[g];
+    }
+    return TOSCA_OK;
+}
+
+#ifndef ORACLE_TANGENT
 
 // UeqnFE (:1411-1441): Rhs = ExplicitRhsU; VecScale(Rhs, dt); VecAXPY(Rhs, 1, Ucont_o);
 // Ucont = Rhs; GlobalToLocal.  rhs holds the new Ucont on return, as the reference's Rhs does.
