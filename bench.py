@@ -13,6 +13,7 @@ Prints ONE JSON line (rank 0).  Only one GPU is supported: --gpus > 1 exits with
 from __future__ import annotations
 
 import argparse
+import subprocess
 import json
 import math
 import os
@@ -251,6 +252,60 @@ def vjp_leg(torch, ctx, mesh, case, ucont, ucat, theta, cs, lay, args, step_ms):
     return {"vjp_ms": vjp_ms, "forward_ms": fwd_ms, "vjp_over_forward": vjp_ms / fwd_ms,
             "vjp_over_step": vjp_ms / step_ms, "kernels_ms": {k: v[0] / v[1] for k, v in tim.items()},
             "forward": "UpdateNut + FormU + Buoyancy (FP64) with the step's Cs"}
+
+
+def rk3w_leg(torch, ctx, mesh, case, ucont, ucat, nut, farm, lay, args, n_stages=10):
+    """The checkpointed adjoint of the explicit Wray Runge-Kutta predictor over n_stages stages
+    (tosca_ueqn_rk3w_adjoint: forward sweep storing each stage's state, then the backward sweep)
+    against the forward predictor alone (tosca_ueqn_rk3w) over the same stages; the step's nu_t
+    held, -dP, bTheta and the farm (when the workload has it) as sources."""
+    dev = "cuda"
+    gl = (lay.mz, lay.my, lay.mx, 3)
+    g = torch.Generator(device=dev).manual_seed(13)
+    dp, bt = (0.01 * torch.randn(gl, dtype=torch.float64, device=dev, generator=g) for _ in range(2))
+    w = [torch.empty(gl, dtype=torch.float64, device=dev) for _ in range(3)]
+    V, U = ucont.clone(), ucat.clone()
+    xbar = torch.randn(gl, dtype=torch.float64, device=dev, generator=g)
+    ubar = torch.empty_like(xbar)
+    nbar = torch.empty_like(nut)
+    dt = 0.1
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def fwd():
+        V.copy_(ucont)
+        U.copy_(ucat)
+        ctx.UeqnRK3W(mesh, case.params, V, U, nut, dp, bt, None, 0, farm, dt, n_stages, *w)
+
+    def adj():
+        V.copy_(ucont)
+        U.copy_(ucat)
+        ubar.copy_(xbar)
+        ctx.UeqnRK3W_adjoint(mesh, case.params, V, U, nut, dp, bt, None, 0, farm, dt, n_stages, *w, ubar, nbar)
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    reps = max(1, args.steps // 5)
+    fwd_ms, adj_ms = timed(fwd, reps), timed(adj, reps)
+    ctx.enable_timing(True)
+    ctx.reset_timing()
+    adj()
+    tim = ctx.timings()
+    ctx.enable_timing(False)
+    print(f"# rk3w adjoint, {n_stages} stages: {adj_ms:.1f} ms (forward sweep + backward) vs forward {fwd_ms:.1f} ms "
+          f"(x{adj_ms / fwd_ms:.2f})", file=sys.stderr, flush=True)
+    del dp, bt, w, V, U, xbar, ubar, nbar
+    return {"n_stages": n_stages, "dt": dt, "forward_ms": fwd_ms, "adjoint_ms": adj_ms,
+            "adjoint_over_forward": adj_ms / fwd_ms, "kernels_ms": {k: v[0] for k, v in tim.items()},
+            "function": "final Ucont of the explicit Wray RK3 predictor as a function of the initial Ucont and "
+                        "nu_t; every stage's state checkpointed in HBM, both sweeps timed"}
 
 
 # ExplicitRhsU algorithmic bytes per cell: contravariant-to-Cartesian (ucont 24 + cell metrics 72
@@ -495,12 +550,65 @@ def workload_name(cfg: int, case) -> str:
     return f"{names[cfg]} {n} cells (x,y,z)"
 
 
+def _cpu_model() -> str:
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _native_oracle() -> str:
+    """Builds the oracle with -march=native on this host (for the timing only) and selects it;
+    returns a description of the build used."""
+    try:
+        subprocess.run(["make", "-s", "-C", str(ROOT / "oracle"), "native"], check=True, timeout=240,
+                       capture_output=True)
+        os.environ["TOSCA_ORACLE_LIB"] = str(ROOT / "oracle" / "liboracle_native.so")
+        return "g++ -O3 -march=native -ffp-contract=off -fopenmp, built on this host"
+    except Exception as e:      # no compiler or a failed build: the portable build is timed instead
+        return f"g++ -O3 -march=x86-64-v3 -ffp-contract=off -fopenmp (native build failed: {type(e).__name__})"
+
+
 def cpu_baseline(cfg_case_kwargs, seconds: float):
-    """Times the CPU oracle on a bounded sample of the same workload: the same case builder and
-    physics on a smaller grid, all host threads (OpenMP over k slabs) and one thread."""
+    """Times the CPU oracle (the reference-structured C++ restatement of the same passes) on the
+    host: the benchmark's workload on a bounded sample of the same case builder and physics (all
+    of this process's CPU share, OpenMP over k slabs, and one thread), plus one step each of the
+    64^3 Taylor-Green box and the 256x128x96 hill at their full sizes and of the stretched ABL box
+    at an eighth of its size."""
     import numpy as np
+    build = _native_oracle()
     from tosca_amd import case as C
     from oracle import oracle as O
+
+    ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    # the host's share for this GPU: OMP_NUM_THREADS where the environment sets it
+    ncores = max(1, min(ncpu, int(os.environ.get("OMP_NUM_THREADS", ncpu) or ncpu)))
+
+    def prepare(builder, kw):
+        case = builder(device="cpu", **kw)
+        lay = case.lay
+        mesh = {k: v.numpy().copy() for k, v in case.mesh.items() if k != "cent"}
+        f = (case.ucont.numpy().copy(), case.ucat.numpy().copy(), case.theta.numpy().copy())
+        out = dict(rhs=np.zeros((lay.mz, lay.my, lay.mx, 3)), cs=np.zeros(lay.gshape), nut=np.zeros(lay.gshape),
+                   btheta=np.zeros((lay.mz, lay.my, lay.mx, 3)))
+        return case, lay, mesh, f, out
+
+    def run(prep, threads, budget, max_reps=50):
+        case, lay, mesh, f, out = prep
+        O.set_threads(threads)
+        step = lambda: O.rhs_les_step(lay.info(), mesh, case.params, *f, out["rhs"], 1.0, out["cs"], out["nut"],
+                                      out["btheta"], 1.0)
+        step()                                                        # warm-up
+        t0, reps = time.perf_counter(), 0
+        while True:
+            step()
+            reps += 1
+            el = time.perf_counter() - t0
+            if el >= budget or reps >= max_reps:
+                return lay.ncells * reps / el, reps
 
     builder, kw = cfg_case_kwargs
     kw = dict(kw)
@@ -508,36 +616,27 @@ def cpu_baseline(cfg_case_kwargs, seconds: float):
         kw.update(n=48)
     else:
         kw.update(nx=96, ny=96, nz=48)
-    case = builder(device="cpu", **kw)
-    lay = case.lay
-    mesh = {k: v.numpy().copy() for k, v in case.mesh.items() if k != "cent"}
-    ucont, ucat, theta = case.ucont.numpy().copy(), case.ucat.numpy().copy(), case.theta.numpy().copy()
-    gs, gl = lay.gshape, (lay.mz, lay.my, lay.mx, 3)
-    out = dict(rhs=np.zeros(gl), cs=np.zeros(gs), nut=np.zeros(gs), btheta=np.zeros(gl))
-
-    def run(threads, budget):
-        O.set_threads(threads)
-        O.rhs_les_step(lay.info(), mesh, case.params, ucont, ucat, theta, out["rhs"], 1.0, out["cs"], out["nut"],
-                       out["btheta"], 1.0)                            # warm-up
-        t0, reps = time.perf_counter(), 0
-        while True:
-            O.rhs_les_step(lay.info(), mesh, case.params, ucont, ucat, theta, out["rhs"], 1.0, out["cs"],
-                           out["nut"], out["btheta"], 1.0)
-            reps += 1
-            el = time.perf_counter() - t0
-            if el >= budget or reps >= 50:
-                return lay.ncells * reps / el, reps
-
-    ncores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    ncores = max(1, min(ncores, 16))
-    v_all, r_all = run(ncores, 0.6 * seconds)
-    v_one, r_one = run(1, 0.4 * seconds)
+    prep = prepare(builder, kw)
+    lay = prep[1]
+    v_all, r_all = run(prep, ncores, 0.6 * seconds)
+    v_one, r_one = run(prep, 1, 0.4 * seconds)
+    sample = (f"CPU oracle (reference-structured C++ restatement) on the same case builder and physics at "
+              f"{lay.mz - 2}x{lay.mx - 2}x{lay.my - 2} = {lay.ncells} cells; {r_all} steps on {ncores} threads, "
+              f"{r_one} steps on 1 thread")
+    del prep
+    configs = {}
+    for name, cfg, extra in (("tgv 64^3", 1, {}), ("hill 256x128x96", 2, {}),
+                             ("abl 192x128x64 (an eighth of 384x256x128, 200 Fourier modes)", 3,
+                              dict(nx=192, ny=128, nz=64, n_modes=200))):
+        spec = C.CONFIGS[cfg]
+        p = prepare(spec["builder"], dict(spec["kwargs"], **extra))
+        v, reps = run(p, ncores, 0.0, max_reps=1)
+        configs[name] = {"value": v, "unit": "cell-updates/s", "cells": p[1].ncells, "threads": ncores}
+        del p
     return {
         "value": v_all, "unit": "cell-updates/s", "cores": ncores, "kind": "port",
-        "value_1core": v_one,
-        "sample": f"CPU oracle (reference-structured C++ restatement, g++ -O3 -ffp-contract=off, OpenMP over k) on "
-                  f"the same case builder and physics at {lay.mz - 2}x{lay.mx - 2}x{lay.my - 2} = {lay.ncells} cells; "
-                  f"{r_all} steps on {ncores} threads, {r_one} steps on 1 thread",
+        "value_1core": v_one, "sample": sample, "host": {"cpu_model": _cpu_model(), "nproc": ncpu}, "build": build,
+        "configs": configs,
     }
 
 
@@ -653,6 +752,7 @@ def main():
     line["mixed"] = mixed_leg(torch, ctx, mesh, case, ucont, ucat, theta, lay, args, ms, farm)
     line["vjp"] = vjp_leg(torch, ctx, mesh, case, ucont, ucat, theta, cs, lay, args, ms)
     line["explicit_rhs_u"] = explicit_leg(torch, ctx, mesh, case, ucont, ucat, nut, lay, args)
+    line["rk3w_adjoint"] = rk3w_leg(torch, ctx, mesh, case, ucont, ucat, nut, farm, lay, args)
     line["formt"] = formt_leg(torch, ctx, mesh, case, ucont, theta, nut, lay, args)
     line["ghost_fills"] = boundary_leg(torch, ctx, mesh, case, ucont, ucat, theta, lay, args)
     line["host_staged"] = host_leg(torch, ctx, mesh, case, ucont, ucat, theta, lay, args, ms)

@@ -384,6 +384,40 @@ int tosca_ueqn_rk4(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_params *p
                    const double *btheta, const double *btheta_o, int ab2, const double *farm,
                    const double *ucont_o, double dt, double *rhs, double *utmp, int form_mode);
 
+/* The explicit Wray Runge-Kutta momentum predictor: the three-stage low-storage scheme of
+ * UeqnRK3CN_W (src/solvers/ueqn_solvers.c:1508-1583) with its coefficients beta = (8/15, 5/12, 3/4),
+ * delta = (0, 17/60, 5/12) (:1518-1520) and the whole right-hand side explicit (the viscous term
+ * too: beta_i - delta_i = 2 gamma_i).  Stage i: rhs = ExplicitRhsU(form_mode) as
+ * tosca_explicit_rhs_u; Ucont = (Ucont + dt beta_i rhs) + (-1.0 dt delta_i) rhs_o, the reference's
+ * VecCopy / VecAXPY order (:1533-1541); Ucont into the owned range of ucont with the periodic ghost
+ * wrap; rhs_o = rhs (:1563-1569).  n_stages stages run back to back and every third one begins a
+ * time step, where ucont_o = Ucont (as the time loop sets Ucont_o before the predictor) and rhs_o = 0
+ * (:1522).  The sources (dp, btheta, farm) and nut are held fixed.  ucont, ucat: lUcont, lUcat (in
+ * place); ucont_o, rhs, rhs_o: global work vectors (on return: the last step's Ucont_o, the last
+ * stage's Rhs twice).  Other arguments as tosca_explicit_rhs_u. */
+int tosca_ueqn_rk3w(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_params *params,
+                    double *ucont, double *ucat, const double *nut, const double *dp,
+                    const double *btheta, const double *btheta_o, int ab2, const double *farm,
+                    double dt, int n_stages, double *ucont_o, double *rhs, double *rhs_o, int form_mode);
+
+/* Checkpointed adjoint of tosca_ueqn_rk3w (form_mode 0) over n_stages stages (no reference
+ * counterpart: the reference has no adjoint).  The function is the final Ucont (global) of the
+ * predictor as a function of the initial Ucont (the owned range of ucont) and of the local nu_t
+ * array; the sources, ucat's non-periodic boundary cells and the ghosts outside periodic axes are
+ * constants.  Runs the forward sweep (leaving ucont, ucat, ucont_o, rhs, rhs_o as tosca_ueqn_rk3w
+ * leaves them), storing every stage's starting state in the context (n_stages + 1 global vectors),
+ * then the backward sweep: per stage, in reverse order, the transposes of the stage update,
+ * resetNonResolvedCellFaces, FormU (tosca_rhs_vjp's kernels with nu_t an input), the periodic
+ * resets of ucat, contravariantToCartesian and the face resets.
+ *   ucont_bar: global, in: the cotangent of the final Ucont; out: that of the initial Ucont.
+ *   nut_bar: local, out: the cotangent of nu_t (read only when params->les_active).
+ * The zeroGradient inflow clip and QUICK's upwind switch are differentiated with the branch held. */
+int tosca_ueqn_rk3w_adjoint(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_params *params,
+                            double *ucont, double *ucat, const double *nut, const double *dp,
+                            const double *btheta, const double *btheta_o, int ab2, const double *farm,
+                            double dt, int n_stages, double *ucont_o, double *rhs, double *rhs_o,
+                            double *ucont_bar, double *nut_bar);
+
 /* Replaces IMEXMatVec (src/solvers/ueqn_solvers.c:1739-1801), the matrix-free operator of the
  * implicit (Crank-Nicolson / IMEX Runge-Kutta) momentum solve: av = v - coef Visc(v), where
  * coef = dt scale (scale 1/2 for the CN steps of ABCN, 1 for RK3WCN / RK3SOCN, chosen by the

@@ -7,6 +7,7 @@ include/tosca_rhs.h (host memory).
 from __future__ import annotations
 
 import ctypes as C
+import os
 import subprocess
 from pathlib import Path
 from typing import Mapping, Optional
@@ -16,7 +17,8 @@ import numpy as np
 from tosca_amd import abi
 
 HERE = Path(__file__).resolve().parent
-LIB_PATH = HERE / "liboracle.so"
+# TOSCA_ORACLE_LIB selects another build of the same source (the benchmark's -march=native build)
+LIB_PATH = Path(os.environ.get("TOSCA_ORACLE_LIB", str(HERE / "liboracle.so")))
 
 _lib = None
 

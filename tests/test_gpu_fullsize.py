@@ -311,3 +311,61 @@ def test_farm_fullsize(shard):
     print(f"  rhs_les_step_farm rhs fused vs reference-structured relL2={e:.2e}", flush=True)
     assert e <= 1e-12
     ctx.close()
+
+
+def test_rk3w_adjoint_finite_differences(shard):
+    """The checkpointed adjoint of 10 explicit Wray Runge-Kutta stages on the shard (DSM nu_t held, a
+    bTheta source on): <xbar, (F(u0 + eps d) - F(u0 - eps d)) / (2 eps)> = <J^T xbar, d> for xbar ~
+    N(0, 1) and a random direction d scaled to the fluxes, both on the resolved faces, eps = 1e-6"""
+    rhs, case, mesh = shard
+    lay = case.lay
+    ctx = _ctx(rhs, lay, "fused")
+    gl = (lay.mz, lay.my, lay.mx, 3)
+    g = torch.Generator(device="cuda").manual_seed(31)
+    cs = torch.zeros(lay.gshape, dtype=torch.float64, device="cuda")
+    nut = torch.zeros(lay.gshape, dtype=torch.float64, device="cuda")
+    ctx.UpdateCs(mesh, case.params, case.ucat, cs)
+    ctx.UpdateNut(mesh, case.params, case.ucat, cs, nut)
+    dp = torch.zeros(gl, dtype=torch.float64, device="cuda")
+    bt = torch.zeros(gl, dtype=torch.float64, device="cuda")
+    ctx.Buoyancy(mesh, case.params, case.theta, 1.0, bt)
+    del cs
+    dt, ns = 0.1, 10
+    w = [torch.empty(gl, dtype=torch.float64, device="cuda") for _ in range(3)]
+    u0 = C.owned(case.ucont, lay).contiguous()
+
+    def final(u):
+        uc = case.ucont.clone()
+        C.owned(uc, lay)[...] = u
+        C.local_to_local(uc, lay, case.periodic)
+        ua = case.ucat.clone()
+        ctx.UeqnRK3W(mesh, case.params, uc, ua, nut, dp, bt, None, 0, None, dt, ns, *w)
+        return C.owned(uc, lay).contiguous()
+
+    # the resolved faces of each flux component: interior cells across, and along the face direction
+    # faces 1 .. m-2 of a periodic axis, 1 .. m-3 of a walled one.  Face 0 of a periodic axis is a copy
+    # of face m-2, face m-1 is unused and a wall's normal flux is reset to zero with a sign test on
+    # Ucont_o (resetNonResolvedCellFaces), a switch a finite difference must not cross.
+    mask = torch.zeros(gl, dtype=torch.float64, device="cuda")
+    m = (lay.mx, lay.my, lay.mz)
+    for a in range(3):
+        sl = [slice(1, lay.mz - 1), slice(1, lay.my - 1), slice(1, lay.mx - 1)]
+        d_ = 2 - a
+        sl[d_] = slice(1, m[a] - 1 if case.periodic[a] else m[a] - 2)
+        mask[tuple(sl) + (a,)] = 1.0
+    xbar = torch.randn(gl, dtype=torch.float64, device="cuda", generator=g) * mask
+    d = torch.randn(gl, dtype=torch.float64, device="cuda", generator=g) * float(u0.square().mean().sqrt()) * mask
+    eps = 1e-6
+    fd = float((xbar * (final(u0 + eps * d) - final(u0 - eps * d))).sum()) / (2 * eps)
+    uc, ua = case.ucont.clone(), case.ucat.clone()
+    ubar = xbar.clone()
+    nbar = torch.zeros(lay.gshape, dtype=torch.float64, device="cuda")
+    ctx.UeqnRK3W_adjoint(mesh, case.params, uc, ua, nut, dp, bt, None, 0, None, dt, ns, *w, ubar, nbar)
+    torch.cuda.synchronize()
+    adj = float((ubar * d).sum())
+    e = abs(adj - fd) / abs(fd)
+    print(f"  RK3W adjoint, {ns} stages: <J^T xbar, d> = {adj:.10e}, finite difference {fd:.10e}, rel {e:.2e}",
+          flush=True)
+    assert e <= 1e-6
+    del uc, ua, ubar, nbar, w
+    ctx.close()

@@ -6,11 +6,13 @@ in-tree by tosca_amd/build.py and fails loudly when it is missing (there is no C
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 from typing import Dict, Mapping, Optional
 
 HERE = Path(__file__).resolve().parent
-LIB_PATH = HERE / "libtosca_rhs.so"
+# TOSCA_LIB selects another build of the library (the host-sanitizer build of the CPU tests)
+LIB_PATH = Path(os.environ.get("TOSCA_LIB", str(HERE / "libtosca_rhs.so")))
 
 TOSCA_OK, TOSCA_ERR_ARG, TOSCA_ERR_UNSUPPORTED, TOSCA_ERR_LAYOUT, TOSCA_ERR_DEVICE, TOSCA_ERR_MESH = range(6)
 ERROR_NAMES = {0: "OK", 1: "ERR_ARG", 2: "ERR_UNSUPPORTED", 3: "ERR_LAYOUT", 4: "ERR_DEVICE", 5: "ERR_MESH"}
@@ -142,6 +144,9 @@ SIGNATURES = {
     "tosca_rhs_les_step_mixed_m32": (I, [P, C.POINTER(Mesh), C.POINTER(MeshF32), C.POINTER(Params), P, P, P, P, D, P, P,
                                          P, D]),
     "tosca_explicit_rhs_u": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P, P, P, P, I, P, P, P, I]),
+    "tosca_ueqn_rk3w": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P, P, P, P, I, P, D, I, P, P, P, I]),
+    "tosca_ueqn_rk3w_adjoint": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P, P, P, P, I, P, D, I, P, P, P, P,
+                                    P]),
     "tosca_update_contravariant_bcs": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P]),
     "tosca_update_cartesian_bcs": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P, P]),
     "tosca_update_temperature_bcs": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P]),

@@ -40,6 +40,7 @@ SOURCES = {
     "gradp.hip": ["-ffp-contract=off"],
     "krylov.hip": ["-ffp-contract=off"],
     "farm.hip": ["-ffp-contract=off"],
+    "rk3w.hip": ["-ffp-contract=off"],
 }
 
 

@@ -36,6 +36,8 @@ struct tosca_ctx {
     double *hs = nullptr;        // device staging of the host-resident step
     size_t hs_elems = 0;
     double *cabot = nullptr;     // the Cabot wall law's integration table, built on first use
+    double *adj = nullptr;       // the predictor adjoint's stage checkpoints and work vectors
+    size_t adj_elems = 0;
     // meshes tosca_mesh_check accepted (their nvert, meshtag, aj, csi, eta, zet pointers): the
     // kernels do not read nvert / meshTag, so every entry point that takes a mesh requires one
     std::vector<std::array<const void *, 6>> checked;
@@ -617,8 +619,9 @@ int tosca_ctx_destroy(tosca_ctx *c) {
     hipSetDevice(c->device);
     c->timer.release();
     if (tosca::g_timer == &c->timer) tosca::g_timer = nullptr;
-    if (c->ws || c->chk || c->kws || c->hs) hipStreamSynchronize(c->stream);
+    if (c->ws || c->chk || c->kws || c->hs || c->adj) hipStreamSynchronize(c->stream);
     if (c->kws) hipFree(c->kws);
+    if (c->adj) hipFree(c->adj);
     if (c->hs) hipFree(c->hs);
     if (c->cabot) hipFree(c->cabot);
     if (c->ws) hipFree(c->ws);
@@ -783,6 +786,139 @@ int tosca_ueqn_rk4(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, dou
         tosca::launch_wrap(ucont, 3, g, s);
     }
     return launched(c, "UeqnRK4");
+}
+
+}  // extern "C"
+
+namespace {
+
+constexpr double kWrayBeta[3] = {8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};   // ueqn_solvers.c:1518
+constexpr double kWrayDelta[3] = {0.0, 17.0 / 60.0, 5.0 / 12.0};       // :1520
+
+// the explicit Wray Runge-Kutta stages on validated arguments; chk (nullable): the global state at
+// the start of each stage, before ExplicitRhsU's resets, is stored at chk + s * ng3, and the state
+// after the last stage at chk + n_stages * ng3
+void queue_rk3w(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, double *ucont, double *ucat,
+                const double *nut, const double *dp, const double *btheta, const double *btheta_o, int ab2,
+                const double *farm, double dt, int n_stages, double *ucont_o, double *rhs, double *rhs_o,
+                int form_mode, double *chk) {
+    const Geo &g = c->geo;
+    hipStream_t s = c->stream;
+    const long ng3 = 3L * g.mx * g.my * g.mz;
+    for (int st = 0; st < n_stages; st++) {
+        const int i = st % 3;
+        if (i == 0) {                                       // a time step begins
+            tosca::launch_local_to_global(ucont_o, ucont, g, s);       // Ucont_o = Ucont (time loop)
+            tosca::launch_fill(rhs_o, ng3, 0.0, s);                    // VecSet(Rhs_o, 0), :1522
+        }
+        if (chk) tosca::launch_local_to_global(chk + st * ng3, ucont, g, s);
+        queue_explicit(c, m, p, ucont, ucat, nut, dp, btheta, btheta_o, ab2, farm, ucont_o, rhs, form_mode);   // :1527
+        // Ucont = (Ucont + dt beta_i Rhs) + (-1.0 dt delta_i) Rhs_o; Rhs_o = Rhs (:1533-1541, 1563-1569)
+        tosca::launch_rk3w_update(ucont, rhs, rhs_o, dt * kWrayBeta[i], -1.0 * dt * kWrayDelta[i], g, s);
+        tosca::launch_wrap(ucont, 3, g, s);
+    }
+    if (chk) tosca::launch_local_to_global(chk + (long)n_stages * ng3, ucont, g, s);
+}
+
+int check_rk3w(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, double *ucont, double *ucat,
+               const double *nut, const double *dp, const double *btheta, const double *btheta_o, int ab2,
+               const double *farm, int n_stages, double *ucont_o, double *rhs, double *rhs_o, int form_mode) {
+    int rc = check_explicit(c, m, p, ucont, ucat, nut, dp, btheta, btheta_o, ab2, farm, ucont_o, rhs, form_mode);
+    if (rc) return rc;
+    if (n_stages < 0) return set_err(c, TOSCA_ERR_ARG, "n_stages = %d", n_stages);
+    const Geo &g = c->geo;
+    const size_t vglob = 3 * (size_t)g.mx * g.my * g.mz * sizeof(double);
+    if ((rc = check_dev(c, rhs_o, vglob, "rhs_o"))) return rc;
+    const void *v[6] = {ucont_o, rhs, rhs_o, dp, btheta, farm};
+    for (int a = 0; a < 3; a++)
+        for (int b = a + 1; b < 6; b++)
+            if (v[a] == v[b]) return set_err(c, TOSCA_ERR_ARG, "ucont_o, rhs and rhs_o must be distinct from each other "
+                                                               "and from the sources");
+    return TOSCA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tosca_ueqn_rk3w(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, double *ucont, double *ucat,
+                    const double *nut, const double *dp, const double *btheta, const double *btheta_o, int ab2,
+                    const double *farm, double dt, int n_stages, double *ucont_o, double *rhs, double *rhs_o,
+                    int form_mode) {
+    int rc = check_rk3w(c, m, p, ucont, ucat, nut, dp, btheta, btheta_o, ab2, farm, n_stages, ucont_o, rhs, rhs_o,
+                        form_mode);
+    if (rc) return rc;
+    queue_rk3w(c, m, p, ucont, ucat, nut, dp, btheta, btheta_o, ab2, farm, dt, n_stages, ucont_o, rhs, rhs_o, form_mode,
+               nullptr);
+    return launched(c, "UeqnRK3W");
+}
+
+int tosca_ueqn_rk3w_adjoint(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, double *ucont, double *ucat,
+                            const double *nut, const double *dp, const double *btheta, const double *btheta_o,
+                            int ab2, const double *farm, double dt, int n_stages, double *ucont_o, double *rhs,
+                            double *rhs_o, double *ucont_bar, double *nut_bar) {
+    int rc = check_rk3w(c, m, p, ucont, ucat, nut, dp, btheta, btheta_o, ab2, farm, n_stages, ucont_o, rhs, rhs_o, 0);
+    if (rc) return rc;
+    const Geo &g = c->geo;
+    const size_t n = (size_t)g.nloc(), ng3 = 3 * (size_t)g.mx * g.my * g.mz;
+    if ((rc = check_dev(c, ucont_bar, ng3 * sizeof(double), "ucont_bar"))) return rc;
+    const bool les = p->les_active != 0;
+    if (les && (rc = check_dev(c, nut_bar, n * sizeof(double), "nut_bar"))) return rc;
+    if (ucont_bar == rhs || ucont_bar == rhs_o || ucont_bar == ucont_o)
+        return set_err(c, TOSCA_ERR_ARG, "ucont_bar must not alias the work vectors");
+    // the FormU adjoint's workspace, then the checkpoints (n_stages + 1 states) and the work vectors
+    if ((rc = ensure_ws(c, std::max(ws_need(c), 52 * n + 64)))) return rc;
+    const size_t need = ((size_t)n_stages + 1) * ng3 + 2 * ng3 + 9 * n;
+    if (c->adj_elems < need) {
+        if (c->adj) { hipStreamSynchronize(c->stream); hipFree(c->adj); c->adj = nullptr; c->adj_elems = 0; }
+        if (hipMalloc(&c->adj, need * sizeof(double)) != hipSuccess) {
+            (void)hipGetLastError();
+            return set_err(c, TOSCA_ERR_DEVICE, "checkpoint store of %zu bytes failed", need * sizeof(double));
+        }
+        c->adj_elems = need;
+    }
+    hipStream_t s = c->stream;
+    double *chk = c->adj, *carry = chk + ((size_t)n_stages + 1) * ng3, *rbar = carry + ng3, *vbar = rbar + ng3,
+           *ubar = vbar + 3 * n, *qbar = ubar + 3 * n;
+    // forward sweep, storing each stage's starting state
+    queue_rk3w(c, m, p, ucont, ucat, nut, dp, btheta, btheta_o, ab2, farm, dt, n_stages, ucont_o, rhs, rhs_o, 0, chk);
+    // backward sweep: each stage's steps transposed, in reverse order
+    tosca::launch_fill(carry, (long)ng3, 0.0, s);
+    if (les) tosca::launch_fill(nut_bar, (long)n, 0.0, s);
+    for (int st = n_stages - 1; st >= 0; st--) {
+        const int i = st % 3;
+        const double *x = chk + (size_t)st * ng3, *uo = chk + (size_t)(st - i) * ng3;
+        // the stage's reset fluxes and Cartesian velocity, recomputed from its checkpoint
+        tosca::launch_global_to_local(ucont, x, g, s);
+        tosca::launch_wrap(ucont, 3, g, s);
+        tosca::launch_no_penetration(ucont, p, g, s);
+        tosca::launch_reset_face_periodic(ucont, 3, g, s);
+        tosca::launch_contra_to_cart(ucont, m, ucat, g, s);
+        tosca::launch_reset_cell_periodic(ucat, 3, g, s);
+        // cotangent of the stage's Rhs; resetNonResolvedCellFaces zeroes the same entries of it
+        tosca::launch_rk3w_adj_combine(ucont_bar, carry, rbar, dt * kWrayBeta[i], -1.0 * dt * kWrayDelta[i], g, s);
+        tosca::launch_non_resolved(rbar, uo, p, g, s);
+        // FormU (the sources are constant): cotangents of the local ucont, ucat and nu_t
+        tosca::vjp_formu_nut(g, m, p, ucont, ucat, nut, rbar, 1.0, vbar, ubar, les ? nut_bar : nullptr, c->ws, s);
+        // contravariantToCartesian and the periodic resets of ucat, transposed into the ucont cotangent
+        tosca::launch_reset_cell_periodic_adj(ubar, 3, g, s);
+        tosca::launch_c2c_adj(ubar, m, qbar, vbar, g, s);
+        // the stage update reads the reset state: its cotangent is the next state's plus ExplicitRhsU's
+        tosca::launch_add_owned(ucont_bar, vbar, g, s);
+        tosca::launch_face_resets_adj(ucont_bar, x, p, g, s);
+    }
+    // leave ucont and ucat as the forward sweep left them
+    if (n_stages > 0) {
+        tosca::launch_global_to_local(ucont, chk + (size_t)(n_stages - 1) * ng3, g, s);
+        tosca::launch_wrap(ucont, 3, g, s);
+        tosca::launch_no_penetration(ucont, p, g, s);
+        tosca::launch_reset_face_periodic(ucont, 3, g, s);
+        tosca::launch_contra_to_cart(ucont, m, ucat, g, s);
+        tosca::launch_reset_cell_periodic(ucat, 3, g, s);
+        tosca::launch_global_to_local(ucont, chk + (size_t)n_stages * ng3, g, s);
+        tosca::launch_wrap(ucont, 3, g, s);
+    }
+    return launched(c, "UeqnRK3W adjoint");
 }
 
 int tosca_imex_matvec(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const double *v, double *ucont,

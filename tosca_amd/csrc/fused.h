@@ -71,4 +71,10 @@ void vjp_rhs(const Geo &g, const tosca_mesh *mesh, const tosca_params *p, const 
              const double *theta, const double *cs, const double *rbar, double scale, double buoy_scale,
              double *ucont_bar, double *ucat_bar, double *theta_bar, double *cs_bar, double *ws, hipStream_t s);
 
+// Adjoint of FormU(ucont, ucat, nu_t; scale, formMode 0) with nu_t an input: ucont_bar and ucat_bar
+// overwritten, the nu_t cotangent added to nut_bar (LES active); ws holds >= 52 * nloc doubles.
+void vjp_formu_nut(const Geo &g, const tosca_mesh *mesh, const tosca_params *p, const double *ucont, const double *ucat,
+                   const double *nut, const double *rbar, double scale, double *ucont_bar, double *ucat_bar,
+                   double *nut_bar, double *ws, hipStream_t s);
+
 }  // namespace tosca

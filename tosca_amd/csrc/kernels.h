@@ -40,6 +40,16 @@ void launch_cn_rhs(double *bU, const double *dp, const double *bt, const double 
                    const double *farm, double wbt, long n, hipStream_t s);
 void launch_global_to_local(double *loc, const double *glob, const Geo &g, hipStream_t s);
 
+// explicit Wray Runge-Kutta predictor and its adjoint's transposed steps (rk3w.hip)
+void launch_rk3w_update(double *ucont, const double *rhs, double *rhs_o, double a, double b, const Geo &g,
+                        hipStream_t s);
+void launch_rk3w_adj_combine(const double *xbar, double *carry, double *rbar, double a, double b, const Geo &g,
+                             hipStream_t s);
+void launch_reset_cell_periodic_adj(double *a, int ncomp, const Geo &g, hipStream_t s);
+void launch_c2c_adj(const double *ubar, const tosca_mesh *m, double *qbar, double *vbar, const Geo &g, hipStream_t s);
+void launch_add_owned(double *xbar, const double *vbar, const Geo &g, hipStream_t s);
+void launch_face_resets_adj(double *xbar, const double *x, const tosca_params *p, const Geo &g, hipStream_t s);
+
 // Krylov vector kernels (krylov.hip)
 void launch_mdot(const double *V, long stride, int nv, const double *w, long n, double *partial, double *out,
                  hipStream_t s);

@@ -109,12 +109,14 @@ struct VjpArgs {
     const double *tauI, *tauJ;
     double *fpbar, *nutbar;
     double *vbar, *ubar, *tbar, *csbar;
+    double *nutbar_acc; // nu_t given as an input: its cotangent is accumulated here (local)
     double *pk[3];      // per face direction: 12 SoA planes of n doubles (cv, gn, ga, gb)
     double *nueb[3];    // per face direction: cotangent of nu_eff at each face (n doubles)
     double *spk;        // per interior cell: 9 SoA planes (cotangents of the du_center derivatives)
     long n;
     double scale, bscale, tRef, nu;
     int quick, les;
+    int nut_in;         // nu_t is an input array (no |S| path, no Cs)
     int bc_u[6], bc_nut[6];
 };
 
@@ -400,7 +402,7 @@ __global__ void __launch_bounds__(256) k_vjp_gather(VjpArgs A, Geo g) {
             }
         }
         // |S| path: du_center of the interior cells x - q, x, x + q along q
-        if (A.les) {
+        if (A.les && !A.nut_in) {
             const bool np = g.per[q] == TOSCA_PERIODIC_NONE;
             auto cellOK = [&](const int *t) {
                 return t[0] >= 1 && t[0] <= m[0] - 2 && t[1] >= 1 && t[1] <= m[1] - 2 && t[2] >= 1 && t[2] <= m[2] - 2;
@@ -418,6 +420,7 @@ __global__ void __launch_bounds__(256) k_vjp_gather(VjpArgs A, Geo g) {
             }
         }
         // buoyancy: the + face q of this cell and of the cell below along q
+        if (!A.tbar) continue;
         for (int s = -1; s <= 0; s++) {
             t[0] = i; t[1] = j; t[2] = k;
             t[q] = x[q] + s;
@@ -428,34 +431,53 @@ __global__ void __launch_bounds__(256) k_vjp_gather(VjpArgs A, Geo g) {
         }
     }
     st3(A.ubar, c, acc);
-    A.tbar[c] = tb;
+    if (A.tbar) A.tbar[c] = tb;
 }
 
-void vjp_rhs(const Geo &g, const tosca_mesh *mesh, const tosca_params *p, const double *ucont, const double *ucat,
-             const double *theta, const double *cs, const double *rbar, double scale, double buoy_scale,
-             double *ucont_bar, double *ucat_bar, double *theta_bar, double *cs_bar, double *ws, hipStream_t s) {
+// ------------------------------------------------------------------------------------------------
+// nu_t as an input array: its cotangent at every owned point, accumulated (+=).  The face of
+// direction q at index point c averages nu_t of cells c and c + e_q, so point c collects half the
+// nu_eff cotangent of faces c and c - e_q (zero where a face is not evaluated).
+// ------------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_vjp_nutbar(VjpArgs A, Geo g) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+    if (i > g.mx - 1) return;
+    const int c = g.L(k, j, i);
+    const double s = A.nueb[0][c] + A.nueb[0][g.L(k, j, i - 1)] + A.nueb[1][c] + A.nueb[1][g.L(k, j - 1, i)] +
+                     A.nueb[2][c] + A.nueb[2][g.L(k - 1, j, i)];
+    A.nutbar_acc[c] += 0.5 * s;
+}
+
+// The transposed stages on the stream.  cs != nullptr: nu_t = UpdateNut(ucat, cs) is recomputed
+// and differentiated (Cs_bar, |S| path); nut_in != nullptr: nu_t is an input and its cotangent is
+// added to nut_bar.  theta_bar == nullptr: no Buoyancy term.
+void vjp_rhs_impl(const Geo &g, const tosca_mesh *mesh, const tosca_params *p, const double *ucont,
+                  const double *ucat, const double *theta, const double *cs, const double *nut_in, const double *rbar,
+                  double scale, double buoy_scale, double *ucont_bar, double *ucat_bar, double *theta_bar,
+                  double *cs_bar, double *nut_bar, double *ws, hipStream_t s) {
     const long n = g.nloc();
     double *nut = ws, *fpbar = ws + n, *nueb = ws + 4 * n, *spk = ws + 7 * n, *pk = ws + 16 * n;   // 52 n in all
     launch_fill(ucont_bar, 3 * n, 0.0, s);
     launch_fill(ucat_bar, 3 * n, 0.0, s);
-    launch_fill(theta_bar, n, 0.0, s);
-    launch_fill(cs_bar, n, 0.0, s);
+    if (theta_bar) launch_fill(theta_bar, n, 0.0, s);
+    if (cs_bar) launch_fill(cs_bar, n, 0.0, s);
     launch_fill(nueb, 3 * n, 0.0, s);
     const bool les = p->les_active != 0;
-    if (les) {   // recompute nu_t as the forward step does
+    if (les && !nut_in) {   // recompute nu_t as the forward step does
         fused_nut<double>(g, mesh, ucat, cs, nut, nullptr, s);
         zero_outside_interior(nut, g, s);
         nut_bcs(nut, p, g, s);
     }
     VjpArgs A;
-    A.ucont = ucont; A.ucat = ucat; A.theta = theta; A.cs = cs; A.nut = les ? nut : ucat; A.rbar = rbar;
+    A.ucont = ucont; A.ucat = ucat; A.theta = theta; A.cs = cs; A.rbar = rbar;
+    A.nut = les ? (nut_in ? nut_in : nut) : ucat;
     A.csi = mesh->csi; A.eta = mesh->eta; A.zet = mesh->zet;
     A.icsi = mesh->icsi; A.ieta = mesh->ieta; A.izet = mesh->izet;
     A.jcsi = mesh->jcsi; A.jeta = mesh->jeta; A.jzet = mesh->jzet;
     A.kcsi = mesh->kcsi; A.keta = mesh->keta; A.kzet = mesh->kzet;
     A.aj = mesh->aj; A.iaj = mesh->iaj; A.jaj = mesh->jaj; A.kaj = mesh->kaj;
     A.tauI = p->tau_wall_i; A.tauJ = p->tau_wall_j;
-    A.fpbar = fpbar; A.nutbar = nullptr;
+    A.fpbar = fpbar; A.nutbar = nullptr; A.nutbar_acc = nut_bar;
     A.vbar = ucont_bar; A.ubar = ucat_bar; A.tbar = theta_bar; A.csbar = cs_bar;
     for (int q = 0; q < 3; q++) { A.pk[q] = pk + 12 * q * n; A.nueb[q] = nueb + q * n; }
     A.spk = spk;
@@ -463,6 +485,7 @@ void vjp_rhs(const Geo &g, const tosca_mesh *mesh, const tosca_params *p, const 
     A.scale = scale; A.bscale = buoy_scale; A.tRef = p->t_ref; A.nu = p->nu;
     A.quick = p->div_scheme == TOSCA_DIV_QUICK;
     A.les = les;
+    A.nut_in = nut_in != nullptr;
     for (int q = 0; q < 6; q++) { A.bc_u[q] = p->bc_u[q]; A.bc_nut[q] = p->bc_nut[q]; }
     const dim3 cb(64, 4), cg((g.mx - 2 + 63) / 64, (g.my - 2 + 3) / 4, g.mz - 2);
     TOSCA_LAUNCH("k_vjp_fp", k_vjp_fp, cg, cb, 0, s, A, g);
@@ -470,8 +493,25 @@ void vjp_rhs(const Geo &g, const tosca_mesh *mesh, const tosca_params *p, const 
     TOSCA_LAUNCH("k_vjp_face_i", k_vjp_face<0>, dim3((g.mx - 1 + 255) / 256, g.my - 2, g.mz - 2), dim3(256), 0, s, A, g);
     TOSCA_LAUNCH("k_vjp_face_j", k_vjp_face<1>, dim3((g.mx - 2 + 255) / 256, g.my - 1, g.mz - 2), dim3(256), 0, s, A, g);
     TOSCA_LAUNCH("k_vjp_face_k", k_vjp_face<2>, dim3((g.mx - 2 + 255) / 256, g.my - 2, g.mz - 1), dim3(256), 0, s, A, g);
-    if (les) TOSCA_LAUNCH("k_vjp_nut", k_vjp_nut, cg, cb, 0, s, A, g);
+    if (les && !nut_in) TOSCA_LAUNCH("k_vjp_nut", k_vjp_nut, cg, cb, 0, s, A, g);
+    if (les && nut_in) TOSCA_LAUNCH("k_vjp_nutbar", k_vjp_nutbar, dim3((g.mx + 255) / 256, g.my, g.mz), dim3(256), 0, s, A, g);
     TOSCA_LAUNCH("k_vjp_gather", k_vjp_gather, dim3((g.mx + 255) / 256, g.my, g.mz), dim3(256), 0, s, A, g);
+}
+
+void vjp_rhs(const Geo &g, const tosca_mesh *mesh, const tosca_params *p, const double *ucont, const double *ucat,
+             const double *theta, const double *cs, const double *rbar, double scale, double buoy_scale,
+             double *ucont_bar, double *ucat_bar, double *theta_bar, double *cs_bar, double *ws, hipStream_t s) {
+    vjp_rhs_impl(g, mesh, p, ucont, ucat, theta, cs, nullptr, rbar, scale, buoy_scale, ucont_bar, ucat_bar, theta_bar,
+                 cs_bar, nullptr, ws, s);
+}
+
+// FormU(formMode 0, scale) as a function of (ucont, ucat, nu_t): ucont_bar and ucat_bar
+// overwritten, the nu_t cotangent added to nut_bar (when LES is active)
+void vjp_formu_nut(const Geo &g, const tosca_mesh *mesh, const tosca_params *p, const double *ucont, const double *ucat,
+                   const double *nut, const double *rbar, double scale, double *ucont_bar, double *ucat_bar,
+                   double *nut_bar, double *ws, hipStream_t s) {
+    vjp_rhs_impl(g, mesh, p, ucont, ucat, nullptr, nullptr, p->les_active ? nut : nullptr, rbar, scale, 0.0, ucont_bar,
+                 ucat_bar, nullptr, nullptr, nut_bar, ws, s);
 }
 
 }  // namespace tosca

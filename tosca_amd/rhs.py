@@ -209,6 +209,29 @@ class RhsContext:
                                             _ptr(dp), _ptr(btheta), _ptr(btheta_o), int(ab2), _ptr(farm),
                                             _ptr(ucont_o), float(dt), _ptr(rhs), _ptr(utmp), int(form_mode)))
 
+    def UeqnRK3W(self, mesh, params, ucont, ucat, nut, dp, btheta, btheta_o, ab2, farm, dt, n_stages, ucont_o, rhs,
+                 rhs_o, form_mode: int = 0):
+        """The explicit Wray Runge-Kutta predictor (UeqnRK3CN_W's scheme, src/solvers/ueqn_solvers.c:
+        1508-1583, with the whole right-hand side explicit): n_stages ExplicitRhsU stages, a time
+        step every third; ucont_o, rhs, rhs_o are global work vectors."""
+        self._stream()
+        m, p = self._mesh(mesh), abi.make_params(params)
+        self._check(self.lib.tosca_ueqn_rk3w(self.h, C.byref(m), C.byref(p), _ptr(ucont), _ptr(ucat), _ptr(nut),
+                                             _ptr(dp), _ptr(btheta), _ptr(btheta_o), int(ab2), _ptr(farm), float(dt),
+                                             int(n_stages), _ptr(ucont_o), _ptr(rhs), _ptr(rhs_o), int(form_mode)))
+
+    def UeqnRK3W_adjoint(self, mesh, params, ucont, ucat, nut, dp, btheta, btheta_o, ab2, farm, dt, n_stages, ucont_o,
+                         rhs, rhs_o, ucont_bar, nut_bar):
+        """Checkpointed adjoint of UeqnRK3W (form mode 0): the forward sweep as UeqnRK3W, then
+        ucont_bar (global) goes from the cotangent of the final Ucont to that of the initial one,
+        and nut_bar (local) gets the cotangent of nu_t."""
+        self._stream()
+        m, p = self._mesh(mesh), abi.make_params(params)
+        self._check(self.lib.tosca_ueqn_rk3w_adjoint(self.h, C.byref(m), C.byref(p), _ptr(ucont), _ptr(ucat),
+                                                     _ptr(nut), _ptr(dp), _ptr(btheta), _ptr(btheta_o), int(ab2),
+                                                     _ptr(farm), float(dt), int(n_stages), _ptr(ucont_o), _ptr(rhs),
+                                                     _ptr(rhs_o), _ptr(ucont_bar), _ptr(nut_bar)))
+
     def IMEXMatVec(self, mesh, params, v, ucont, ucat, nut, ucont_o, coef, av):
         """IMEXMatVec (src/solvers/ueqn_solvers.c:1739-1801): av = v - coef Visc(v)."""
         m, p = self._mesh(mesh), abi.make_params(params)
