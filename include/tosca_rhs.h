@@ -168,6 +168,20 @@ int tosca_formu(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_params *para
                 const double *ucont, const double *ucat, const double *nut,
                 double *rhs, double scale, int form_mode);
 
+/* No counterpart in the reference.  tosca_formu for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight solver loops such as a Krylov or Newton iteration, hosts
+ * that forbid hipMalloc): same arguments, same kernels (always the fused ones), same results,
+ * but its scratch (boundary-face fluxes and tile seams) lives in a fixed device array of
+ * TOSCA_NOALLOC_SCRATCH_DOUBLES doubles (256 MiB) that every GPU the library runs on holds until
+ * the process ends, instead of the context's workspace.  It calls no allocating routine and does
+ * not use the context's kernel timer.  A layout whose FormU needs more scratch than that is
+ * rejected with TOSCA_ERR_LAYOUT (a 512x512x192 shard needs about 22 M doubles).  All calls on one
+ * GPU share the array: calls that overlap on one GPU (several streams or host threads) race. */
+#define TOSCA_NOALLOC_SCRATCH_DOUBLES ((size_t)32 * 1024 * 1024)
+int tosca_formu_noalloc(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_params *params,
+                        const double *ucont, const double *ucat, const double *nut,
+                        double *rhs, double scale, int form_mode);
+
 /* UpdateCs (src/les.c:165-2115): writes les->lCs (local scalar) for SMAGORINSKY (Cs = 0.0289
  * everywhere), DSM (test filter + Germano + 3x3x3 box average + clip) and DPASD (3- and 5-point
  * test filters + j-plane averages + clip).  Uses context workspace. */

@@ -135,6 +135,7 @@ SIGNATURES = {
     "tosca_ctx_timing_entry": (I, [P, I, C.c_char_p, I, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "tosca_mesh_check": (I, [P, C.POINTER(Mesh)]),
     "tosca_formu": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P, P, D, I]),
+    "tosca_formu_noalloc": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P, P, D, I]),
     "tosca_update_cs": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P]),
     "tosca_update_nut": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P, P]),
     "tosca_buoyancy": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, D, P, P]),

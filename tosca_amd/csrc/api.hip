@@ -16,6 +16,7 @@
 #include <iterator>
 #include <vector>
 #include <type_traits>
+#include <mutex>
 
 #include "fused.h"
 
@@ -43,11 +44,42 @@ struct tosca_ctx {
     std::vector<std::array<const void *, 6>> checked;
     tosca::KernelTimer timer;
     bool ref_path = false;   // TOSCA_PATH=ref at creation: reference-structured kernels only
+    double *scratch = nullptr;   // this device's copy of `scratch` (tosca_formu_noalloc), or null
+    std::string scratch_err;     // why the lookup of that copy failed
 };
 
 namespace tosca {
 thread_local KernelTimer *g_timer = nullptr;
 }
+
+// Scratch of tosca_formu_noalloc: FormU's boundary-face fluxes, periodic-seam fp and j-tile seam
+// halves, in a file-scope device array instead of the context's workspace.  Every GPU the library
+// runs on holds its own copy (TOSCA_NOALLOC_SCRATCH_DOUBLES doubles) until the process ends.  Its
+// host-side name is not a device address: host code uses only the per-device address that
+// hipGetSymbolAddress returned (g_scratch_dev), never `scratch` itself.
+static __device__ __align__(256) double scratch[TOSCA_NOALLOC_SCRATCH_DOUBLES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_dev[kMaxDevices] = {};   // device address of `scratch` per device, set once
+
+// Looks up this device's copy of `scratch` (the current device must be `device`); once per device.
+hipError_t scratch_setup(int device, double **out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    if (!g_scratch_dev[device]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) { (void)hipGetLastError(); return e; }
+        g_scratch_dev[device] = static_cast<double *>(ptr);
+    }
+    *out = g_scratch_dev[device];
+    return hipSuccess;
+}
+
+}  // namespace
 
 namespace {
 
@@ -610,6 +642,12 @@ int tosca_ctx_create(tosca_ctx **out, const tosca_dmda_info *in, int device) {
     c->info = *in;
     c->geo.mx = (int)in->mx; c->geo.my = (int)in->my; c->geo.mz = (int)in->mz;
     c->geo.gxm = (int)in->gxm; c->geo.gym = (int)in->gym; c->geo.gzm = (int)in->gzm;
+    // the allocation-free FormU's scratch on this device; a failed lookup disables only that entry point
+    const hipError_t se = scratch_setup(device, &c->scratch);
+    if (se != hipSuccess) {
+        c->scratch = nullptr;
+        c->scratch_err = std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(se);
+    }
     *out = c;
     return TOSCA_OK;
 }
@@ -742,6 +780,41 @@ int tosca_formu(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const 
         tosca::fused_formu(g, A, c->ws + 3 * (size_t)g.nloc(), c->stream);
     }
     return launched(c, "FormU");
+}
+
+int tosca_formu_noalloc(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const double *ucont,
+                        const double *ucat, const double *nut, double *rhs, double scale, int form_mode) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    tosca::g_timer = nullptr;      // the context's kernel timer creates events: not used here
+    if (!c->scratch) return set_err(c, TOSCA_ERR_DEVICE, "tosca_formu_noalloc: no scratch on device %d (%s)", c->device,
+                                    c->scratch_err.c_str());
+    if ((rc = check_params(c, p))) return rc;
+    if (form_mode < 0 || form_mode > 2) return set_err(c, TOSCA_ERR_ARG, "form_mode = %d (0, 1 or 2)", form_mode);
+    if (p->div_scheme != TOSCA_DIV_CENTRAL && p->div_scheme != TOSCA_DIV_QUICK)
+        return set_err(c, TOSCA_ERR_UNSUPPORTED, "div_scheme = %d: only central (0) and QUICK (1) are supported", p->div_scheme);
+    set_geo_per(c, p);
+    const Geo &g = c->geo;
+    const size_t need = tosca::formu_ws_doubles(g);
+    if (need > TOSCA_NOALLOC_SCRATCH_DOUBLES)
+        return set_err(c, TOSCA_ERR_LAYOUT, "tosca_formu_noalloc: this layout needs %zu doubles of scratch, more than the "
+                                            "%zu the library holds per device (use tosca_formu)", need,
+                       (size_t)TOSCA_NOALLOC_SCRATCH_DOUBLES);
+    const size_t n = (size_t)g.nloc();
+    if ((rc = check_mesh(c, m, true, true, true))) return rc;
+    if ((rc = check_dev(c, ucont, 3 * n * sizeof(double), "ucont")) || (rc = check_dev(c, ucat, 3 * n * sizeof(double), "ucat")))
+        return rc;
+    const bool visc = form_mode != 1;
+    if (p->les_active && visc) {
+        if ((rc = check_les_model(c, p))) return rc;
+        if ((rc = check_dev(c, nut, n * sizeof(double), "nut"))) return rc;
+        if ((rc = check_wall_tau(c, p))) return rc;
+    }
+    if ((rc = check_dev(c, rhs, 3 * (size_t)g.mx * g.my * g.mz * sizeof(double), "rhs"))) return rc;
+    // the fused kernels write every scratch element they read before reading it: nothing to clear
+    tosca::FormUArgs A = formu_args(m, p, ucont, ucat, (p->les_active && visc) ? nut : nullptr, rhs, scale, form_mode);
+    tosca::fused_formu(g, A, c->scratch, c->stream);
+    return launched(c, "FormU (allocation-free)");
 }
 
 int tosca_explicit_rhs_u(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, double *ucont, double *ucat,

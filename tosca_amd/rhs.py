@@ -90,6 +90,14 @@ class RhsContext:
         self._check(self.lib.tosca_formu(self.h, C.byref(m), C.byref(p), _ptr(ucont), _ptr(ucat), _ptr(nut),
                                          _ptr(rhs), float(scale), int(form_mode)))
 
+    def FormU_noalloc(self, mesh, params, ucont, ucat, nut, rhs, scale: float = 1.0, form_mode: int = 0):
+        """FormU through tosca_formu_noalloc: the same result as FormU, with the scratch in the
+        library's fixed per-device array instead of the context's workspace (no allocation)."""
+        self._stream()
+        m, p = self._mesh(mesh), abi.make_params(params)
+        self._check(self.lib.tosca_formu_noalloc(self.h, C.byref(m), C.byref(p), _ptr(ucont), _ptr(ucat), _ptr(nut),
+                                                 _ptr(rhs), float(scale), int(form_mode)))
+
     def UpdateCs(self, mesh, params, ucat, cs):
         self._stream()
         m, p = self._mesh(mesh), abi.make_params(params)
