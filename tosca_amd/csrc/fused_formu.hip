@@ -195,26 +195,38 @@ __device__ __forceinline__ FluxT<T> ldflux(const double *p) {
     return FluxT<T>{Vec3<T>{T(p[0]), T(p[1]), T(p[2])}, Vec3<T>{T(p[3]), T(p[4]), T(p[5])}};
 }
 
-// per-thread column pointers: every array at the thread's (j, i) column, plane 0
-template <class T, class MT>
-struct ColPtrs {
-    const T *ucont, *nut;
-    const MT *fc[3], *fe[3], *fz[3], *fa[3];
-};
+// Every array of the main kernel is addressed as its kernel-argument base (uniform, in scalar
+// registers) plus a 32-bit element index per lane, instead of a 64-bit pointer per array and lane:
+// the column's local index c0 plus the plane offset, made opaque once per plane (opaque_index) so
+// that the compiler cannot turn the per-array addresses back into loop-carried 64-bit pointers.
+__device__ __forceinline__ int opaque_index(int x) {
+    __asm__ volatile("" : "+v"(x));
+    return x;
+}
+
+template <int DIR, class T, class MT>
+__device__ __forceinline__ const MT *face_fc(const MainArgs<T, MT> &A) { return DIR == 0 ? A.icsi : DIR == 1 ? A.jcsi : A.kcsi; }
+template <int DIR, class T, class MT>
+__device__ __forceinline__ const MT *face_fe(const MainArgs<T, MT> &A) { return DIR == 0 ? A.ieta : DIR == 1 ? A.jeta : A.keta; }
+template <int DIR, class T, class MT>
+__device__ __forceinline__ const MT *face_fz(const MainArgs<T, MT> &A) { return DIR == 0 ? A.izet : DIR == 1 ? A.jzet : A.kzet; }
+template <int DIR, class T, class MT>
+__device__ __forceinline__ const MT *face_fa(const MainArgs<T, MT> &A) { return DIR == 0 ? A.iaj : DIR == 1 ? A.jaj : A.kaj; }
 
 // metrics and fluxes of one interior face, loaded ahead of use
 template <class T>
 struct FaceIn { V3 fc, fe, fz; double fa; T V, nutR; };
 
+// c: local index of the face's cell
 template <int DIR, class T, class MT>
-__device__ __forceinline__ FaceIn<T> face_in(const ColPtrs<T, MT> &P, long q, int s) {
+__device__ __forceinline__ FaceIn<T> face_in(const MainArgs<T, MT> &A, int c, int s) {
     FaceIn<T> f;
-    f.fc = ldnt3(P.fc[DIR] + 3 * q, 0);
-    f.fe = ldnt3(P.fe[DIR] + 3 * q, 0);
-    f.fz = ldnt3(P.fz[DIR] + 3 * q, 0);
-    f.fa = P.fa[DIR][q];
-    f.V = __builtin_nontemporal_load(P.ucont + 3 * q + DIR);
-    f.nutR = P.nut[q + s];
+    f.fc = ldnt3(face_fc<DIR>(A), c);
+    f.fe = ldnt3(face_fe<DIR>(A), c);
+    f.fz = ldnt3(face_fz<DIR>(A), c);
+    f.fa = face_fa<DIR>(A)[c];
+    f.V = __builtin_nontemporal_load(A.ucont + 3 * c + DIR);
+    f.nutR = A.nut[c + s];
     return f;
 }
 
@@ -275,19 +287,19 @@ __device__ __forceinline__ FluxT<T> face_pre(const MainArgs<T, MT> &A, const Fac
     return r;
 }
 
-// interior face flux from the LDS ring, loading its face data; q is the plane offset of the face's cell
+// interior face flux from the LDS ring, loading its face data; c is the local index of the face's cell
 template <int DIR, int NR, class T, class MT>
-__device__ __forceinline__ FluxT<T> face_main(const MainArgs<T, MT> &A, const ColPtrs<T, MT> &P, const Ring<NR, T> &R, long q,
-                                              int row, int col, int t, int s) {
+__device__ __forceinline__ FluxT<T> face_main(const MainArgs<T, MT> &A, const Ring<NR, T> &R, int c, int row, int col,
+                                              int t, int s) {
     FluxT<T> r;
     r.d = Vec3<T>{0, 0, 0};
     r.v = Vec3<T>{0, 0, 0};
-    if (A.mode != 2) r.d = ring_conv<DIR, NR, T>(R, row, col, t, A.quick, P.ucont[3 * q + DIR]);
+    if (A.mode != 2) r.d = ring_conv<DIR, NR, T>(R, row, col, t, A.quick, A.ucont[3 * c + DIR]);
     if (A.mode != 1) {
         T nuEff = T(A.nu);
-        if (A.les) nuEff = T(A.nu) + T(0.5) * (P.nut[q] + P.nut[q + s]);
-        r.v = ring_visc<DIR, NR, T>(R, row, col, t, ldnt3(P.fc[DIR] + 3 * q, 0), ldnt3(P.fe[DIR] + 3 * q, 0),
-                                    ldnt3(P.fz[DIR] + 3 * q, 0), P.fa[DIR][q], nuEff);
+        if (A.les) nuEff = T(A.nu) + T(0.5) * (A.nut[c] + A.nut[c + s]);
+        r.v = ring_visc<DIR, NR, T>(R, row, col, t, ldnt3(face_fc<DIR>(A), c), ldnt3(face_fe<DIR>(A), c),
+                                    ldnt3(face_fz<DIR>(A), c), face_fa<DIR>(A)[c], nuEff);
     }
     return r;
 }
@@ -334,26 +346,16 @@ __global__ void __launch_bounds__(64 * NR) k_fused_formu(MainArgs<T, MT> A, Geo 
     const bool ownI = (i == g.mx - 2) && !iper, ownJ = (j == g.my - 2) && !jper;
     // clamped column for addressing
     const int ia = min(i, g.mx - 1), ja = min(j, g.my - 1);
-    const long svo = ((long)bj * g.mz) * g.mx + ia, svn = ((long)(bj - 1) * g.mz) * g.mx + ia;
+    const int svo = (bj * g.mz) * g.mx + ia, svn = ((bj - 1) * g.mz) * g.mx + ia;
     const int SJ = g.gxm, SK = g.gxm * g.gym;
-    const int c0 = g.L(0, ja, ia);
-    ColPtrs<T, MT> P;
-    P.ucont = A.ucont + 3 * (long)c0; P.nut = A.nut + c0;
-    P.fc[0] = A.icsi + 3 * (long)c0; P.fe[0] = A.ieta + 3 * (long)c0; P.fz[0] = A.izet + 3 * (long)c0;
-    P.fc[1] = A.jcsi + 3 * (long)c0; P.fe[1] = A.jeta + 3 * (long)c0; P.fz[1] = A.jzet + 3 * (long)c0;
-    P.fc[2] = A.kcsi + 3 * (long)c0; P.fe[2] = A.keta + 3 * (long)c0; P.fz[2] = A.kzet + 3 * (long)c0;
-    P.fa[0] = A.iaj + c0; P.fa[1] = A.jaj + c0; P.fa[2] = A.kaj + c0;
-    const MT *pcsi = A.csi + 3 * (long)c0, *peta = A.eta + 3 * (long)c0, *pzet = A.zet + 3 * (long)c0;
-    const T *ptheta = A.theta + c0;
-    const long SG = (long)g.mx * g.my;
-    double *prhs = A.rhs + 3 * (long)(ja * g.mx + ia);
-    double *pbt = A.btheta + 3 * (long)(ja * g.mx + ia);
+    const int c0 = g.L(0, ja, ia);            // local index of the column at plane 0
+    const int SG = g.mx * g.my;
+    const int g0 = ja * g.mx + ia;            // global (owned-range) index of the column at plane 0
     const bool buoy = A.btheta != A.rhs;   // the launcher passes btheta == rhs when Buoyancy is off
     const int nI = g.my * g.mz, nJ = g.mx * g.mz, nK = g.mx * g.my;
-    const double *pF1 = A.bF1 + ((long)((i == 0) ? 0 : nI) + ja) * 6;       // + k * my * 6
-    const double *pF2 = A.bF2 + ((long)((j == 0) ? 0 : nJ) + ia) * 6;       // + k * mx * 6
-    const double *pF3 = A.bF3 + ((long)ja * g.mx + ia) * 6;                 // + (0 | nK) * 6
-    const double *pSI = A.sI + (long)ja * 3, *pSJ = A.sJ + (long)ia * 3, *pSK = A.sK + ((long)ja * g.mx + ia) * 3;
+    const int oF1 = ((i == 0) ? 0 : nI) + ja;    // boundary-face buffers, in fluxes of 6 doubles: + k * my
+    const int oF2 = ((j == 0) ? 0 : nJ) + ia;    // + k * mx
+    const int oF3 = ja * g.mx + ia;              // + (0 | nK)
 
     // ucat ring: rows j0-2 .., cols i0-2 ..
     const int ri0 = i0 - 2, rj0 = j0 - 2;
@@ -380,9 +382,9 @@ __global__ void __launch_bounds__(64 * NR) k_fused_formu(MainArgs<T, MT> A, Geo 
     };
     const int rrow = r + 1, rcol = lane + 1;
     auto kface = [&](int k) -> FluxT<T> {      // k face k of this column
-        if (k == 0) return ldflux<T>(pF3);
-        if (k == g.mz - 2) return ldflux<T>(pF3 + (long)nK * 6);
-        return face_main<2, NR, T>(A, P, ring, (long)k * SK, rrow, rcol, k, SK);
+        if (k == 0) return ldflux<T>(A.bF3 + 6 * oF3);
+        if (k == g.mz - 2) return ldflux<T>(A.bF3 + 6 * (oF3 + nK));
+        return face_main<2, NR, T>(A, ring, c0 + k * SK, rrow, rcol, k, SK);
     };
     // prime: k face kb-1 needs planes kb-2 .. kb+1, the first plane step kb-1 .. kb+2
     FluxT<T> F3p;
@@ -399,58 +401,68 @@ __global__ void __launch_bounds__(64 * NR) k_fused_formu(MainArgs<T, MT> A, Geo 
     }
     V3 fpPrev{0, 0, 0};
     int buf = 0;
+    // values of plane t the next plane step reuses instead of loading them again: the k+1 cell
+    // metrics zet and theta of the Rhs / buoyancy step, and nu_t of the k face's upper cell
+    V3 zeNext{0, 0, 0};
+    double thNext = 0;
+    T nutNext = T(0);
     // values of plane t-1 the Rhs step reuses: inverse face Jacobians and the z components of the
     // face bases (buoyancy), loaded with the face data of that plane
-    double fa0p = P.fa[0][(long)(kb - 1) * SK], fa1p = P.fa[1][(long)(kb - 1) * SK], fa2p = P.fa[2][(long)(kb - 1) * SK];
-    double gzIp = P.fc[0][3 * (long)(kb - 1) * SK + 2], gzJp = P.fe[1][3 * (long)(kb - 1) * SK + 2],
-           gzKp = P.fz[2][3 * (long)(kb - 1) * SK + 2];
+    const int cp0 = c0 + (kb - 1) * SK;
+    double fa0p = A.iaj[cp0], fa1p = A.jaj[cp0], fa2p = A.kaj[cp0];
+    double gzIp = A.icsi[3 * cp0 + 2], gzJp = A.jeta[3 * cp0 + 2], gzKp = A.kzet[3 * cp0 + 2];
     for (int t = kb; t <= ke; t++) {
         // ---- every global load of this plane step, issued together ------------------------------
         T pre[PRE];
         const bool doPre = t + 3 <= ke + 2;
         if (doPre) plane_load(t + 3, pre);
         const bool ghostK = (t == g.mz - 1);
-        const long q = (long)min(t, g.mz - 2) * SK;           // faces of plane t (clamped at the ghost plane)
+        // faces of plane t (clamped at the ghost plane): local index of the face's cell
+        const int cq = opaque_index(c0 + min(t, g.mz - 2) * SK);
         FaceIn<T> in1{}, in2{}, in3{};
-        if (needF1) in1 = face_in<0>(P, q, 1);
-        if (needF2) in2 = face_in<1>(P, q, SJ);
-        if (needF3) in3 = face_in<2>(P, q, SK);
+        if (needF1) in1 = face_in<0>(A, cq, 1);
+        if (needF2) in2 = face_in<1>(A, cq, SJ);
+        if (needF3) in3 = face_in<2>(A, cq, SK);
         // nu_t is loaded whether or not the viscous LES term uses it (the loop has no divergent
         // branches): A.nut must never be null; the launcher substitutes a valid local array when
         // the caller passes none (a null nu_t here faulted the GPU once, in formMode 1)
-        const T nut0 = P.nut[q];
+        const T nut0 = (needF3 && t > kb) ? nutNext : A.nut[cq];
         // ---- face fluxes ---------------------------------------------------------------------------
         const Vec3<T> z3{0, 0, 0};
         FluxT<T> F1{z3, z3}, F2{z3, z3}, F3{z3, z3};
         if (!ghostK) {
-            if (needF1) F1 = bndI ? ldflux<T>(pF1 + (long)t * g.my * 6) : face_pre<0, NR, T>(A, in1, nut0, ring, rrow, rcol, t);
-            if (needF2) F2 = bndJ ? ldflux<T>(pF2 + (long)t * g.mx * 6) : face_pre<1, NR, T>(A, in2, nut0, ring, rrow, rcol, t);
+            if (needF1) F1 = bndI ? ldflux<T>(A.bF1 + 6 * (oF1 + t * g.my)) : face_pre<0, NR, T>(A, in1, nut0, ring, rrow, rcol, t);
+            if (needF2) F2 = bndJ ? ldflux<T>(A.bF2 + 6 * (oF2 + t * g.mx)) : face_pre<1, NR, T>(A, in2, nut0, ring, rrow, rcol, t);
             if (needF3) {
-                if (t == g.mz - 2) F3 = ldflux<T>(pF3 + (long)nK * 6);
+                if (t == g.mz - 2) F3 = ldflux<T>(A.bF3 + 6 * (oF3 + nK));
                 else F3 = face_pre<2, NR, T>(A, in3, nut0, ring, rrow, rcol, t);
             }
         }
         // ---- loads of the Rhs / buoyancy step (cells of plane t-1), in flight across the exchange;
         // output cells only
-        const long qp = (long)(t - 1) * SK;
+        const int cpq = opaque_index(c0 + (t - 1) * SK);        // the cells of plane t-1
         const bool rhsStep = out && t > kb;
         V3 csA{0, 0, 0}, csB{0, 0, 0}, etA{0, 0, 0}, etB{0, 0, 0}, zeA{0, 0, 0}, zeB{0, 0, 0};
-        const long ro = 3 * (long)(t - 1) * SG;
+        const int ro = 3 * (g0 + (t - 1) * SG);                 // rhs, btheta, farm (owned range)
         double r0 = 0, r1 = 0, r2 = 0;
         double th = 0, thI = 0, thJ = 0, thK = 0;
+        if (out) {
+            // the upper cell's zet and theta: this step's k neighbours, the next step's own values
+            zeB = cvt3<double>(ld3(A.zet, cpq + SK));
+            if (buoy) thK = A.theta[cpq + SK];
+        }
         if (rhsStep) {
-            csA = cvt3<double>(ld3(pcsi + 3 * qp, 0)); csB = cvt3<double>(ld3(pcsi + 3 * qp, 1));
-            etA = cvt3<double>(ld3(peta + 3 * qp, 0)); etB = cvt3<double>(ld3(peta + 3 * qp, SJ));
-            zeA = cvt3<double>(ld3(pzet + 3 * qp, 0)); zeB = cvt3<double>(ld3(pzet + 3 * qp, SK));
-            r0 = prhs[ro]; r1 = prhs[ro + 1]; r2 = prhs[ro + 2];
+            csA = cvt3<double>(ld3(A.csi, cpq)); csB = cvt3<double>(ld3(A.csi, cpq + 1));
+            etA = cvt3<double>(ld3(A.eta, cpq)); etB = cvt3<double>(ld3(A.eta, cpq + SJ));
+            zeA = zeNext;
+            r0 = A.rhs[ro]; r1 = A.rhs[ro + 1]; r2 = A.rhs[ro + 2];
             if (A.farm) {
                 // VecAXPY(Rhs, 1, sourceFarmCont) before FormU's accumulation (ueqn_solvers.c:1341-1347)
-                const double *pf = A.farm + (prhs - A.rhs);
-                r0 = r0 + 1.0 * pf[ro]; r1 = r1 + 1.0 * pf[ro + 1]; r2 = r2 + 1.0 * pf[ro + 2];
+                r0 = r0 + 1.0 * A.farm[ro]; r1 = r1 + 1.0 * A.farm[ro + 1]; r2 = r2 + 1.0 * A.farm[ro + 2];
             }
             // theta, like nu_t, must be a valid local array even when Buoyancy is off (the launcher
             // substitutes one); only the buoy flag decides whether it is read
-            if (buoy) { th = ptheta[qp]; thI = ptheta[qp + 1]; thJ = ptheta[qp + SJ]; thK = ptheta[qp + SK]; }
+            if (buoy) { th = thNext; thI = A.theta[cpq + 1]; thJ = A.theta[cpq + SJ]; }
         }
         const Vec3<T> F1dm = shup3(F1.d), F1vm = shup3(F1.v);
         sF2[0][r][lane] = F2.d.x; sF2[1][r][lane] = F2.d.y; sF2[2][r][lane] = F2.d.z;
@@ -464,13 +476,13 @@ __global__ void __launch_bounds__(64 * NR) k_fused_formu(MainArgs<T, MT> A, Geo 
         fp.y = fpsum(F1.d.y, F1dm.y, F2.d.y, F2dm.y, F3.d.y, F3p.d.y, F1.v.y, F1vm.y, F2.v.y, F2vm.y, F3.v.y, F3p.v.y, A.mode);
         fp.z = fpsum(F1.d.z, F1dm.z, F2.d.z, F2dm.z, F3.d.z, F3p.d.z, F1.v.z, F1vm.z, F2.v.z, F2vm.z, F3.v.z, F3p.v.z, A.mode);
         if (ghostK) {
-            if (kper) fp = V3{pSK[0], pSK[1], pSK[2]};
+            if (kper) { const double *qq = A.sK + 3 * (ja * g.mx + ia); fp = V3{qq[0], qq[1], qq[2]}; }
             else fp = fpPrev;                        // non-periodic top: ghost fp copies cell m-2
         } else if (seamI) {
-            const double *qq = pSI + (long)t * g.my * 3;
+            const double *qq = A.sI + 3 * (ja + t * g.my);
             fp = V3{qq[0], qq[1], qq[2]};
         } else if (seamJ) {
-            const double *qq = pSJ + (long)t * g.mx * 3;
+            const double *qq = A.sJ + 3 * (ia + t * g.mx);
             fp = V3{qq[0], qq[1], qq[2]};
         }
         // Rhs of the previous plane (t-1): fp(t-1) own / i+1 (shuffle) / j+1 (LDS), fp(t) own
@@ -480,7 +492,7 @@ __global__ void __launch_bounds__(64 * NR) k_fused_formu(MainArgs<T, MT> A, Geo 
         if (ownI) fpI = fpPrev;
         if (ownJ) fJ = fpPrev;
         else if (r == NR - 1 && jper && j == g.my - 2) {
-            const double *qq = pSJ + (long)(t - 1) * g.mx * 3;
+            const double *qq = A.sJ + 3 * (ia + (t - 1) * g.mx);
             fJ = V3{qq[0], qq[1], qq[2]};
         }
         if (rhsStep) {
@@ -491,22 +503,22 @@ __global__ void __launch_bounds__(64 * NR) k_fused_formu(MainArgs<T, MT> A, Geo 
             v1 = etA.x * fpPrev.x + etA.y * fpPrev.y + etA.z * fpPrev.z;
             v2 = etB.x * fJ.x + etB.y * fJ.y + etB.z * fJ.z;
             const double ry = A.scale * fa1p * (0.5 * (v1 + v2));
-            if (seamRow) A.sv1[svo + (long)(t - 1) * g.mx] = v1;
-            if (seamNext) A.sv2[svn + (long)(t - 1) * g.mx] = v1;
+            if (seamRow) A.sv1[svo + (t - 1) * g.mx] = v1;
+            if (seamNext) A.sv2[svn + (t - 1) * g.mx] = v1;
             v1 = zeA.x * fpPrev.x + zeA.y * fpPrev.y + zeA.z * fpPrev.z;
             v2 = zeB.x * fK.x + zeB.y * fK.y + zeB.z * fK.z;
             const double rz = A.scale * fa2p * (0.5 * (v1 + v2));
-            prhs[ro] = r0 + rx;
-            if (!seamRow) prhs[ro + 1] = r1 + ry;
-            else if (A.farm) prhs[ro + 1] = r1;       // k_formu_jseam adds the dot to rhs + farm
-            prhs[ro + 2] = r2 + rz;
+            A.rhs[ro] = r0 + rx;
+            if (!seamRow) A.rhs[ro + 1] = r1 + ry;
+            else if (A.farm) A.rhs[ro + 1] = r1;      // k_formu_jseam adds the dot to rhs + farm
+            A.rhs[ro + 2] = r2 + rz;
             if (buoy) {
                 // Boussinesq source on the three + faces of the cell (ueqn_sources.c:2812-2870):
                 // g . S with g = (0, 0, -9.81) is -9.81 * S.z exactly
                 const double tI = 0.5 * (th + thI), tJ = 0.5 * (th + thJ), tK = 0.5 * (th + thK);
-                pbt[ro] = A.bscale * ((-9.81 * gzIp) * (A.tRef - tI) / A.tRef);
-                pbt[ro + 1] = A.bscale * ((-9.81 * gzJp) * (A.tRef - tJ) / A.tRef);
-                pbt[ro + 2] = A.bscale * ((-9.81 * gzKp) * (A.tRef - tK) / A.tRef);
+                A.btheta[ro] = A.bscale * ((-9.81 * gzIp) * (A.tRef - tI) / A.tRef);
+                A.btheta[ro + 1] = A.bscale * ((-9.81 * gzJp) * (A.tRef - tJ) / A.tRef);
+                A.btheta[ro + 2] = A.bscale * ((-9.81 * gzKp) * (A.tRef - tK) / A.tRef);
             }
         }
         sFp[buf][0][r][lane] = fp.x; sFp[buf][1][r][lane] = fp.y; sFp[buf][2][r][lane] = fp.z;
@@ -517,6 +529,9 @@ __global__ void __launch_bounds__(64 * NR) k_fused_formu(MainArgs<T, MT> A, Geo 
         F3p = F3;
         fa0p = in1.fa; fa1p = in2.fa; fa2p = in3.fa;
         gzIp = in1.fc.z; gzJp = in2.fe.z; gzKp = in3.fz.z;
+        zeNext = zeB;
+        thNext = thK;
+        nutNext = in3.nutR;
     }
 }
 

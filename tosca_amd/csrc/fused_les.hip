@@ -16,6 +16,8 @@
 //   k_dsm_cs_nut: 3x3x3 volume-weighted Simpson box average of LM/MM with the reference's
 //     periodic remap and boundary-ghost weights (les.c:1885-1950), clip to [0, maxCs], and
 //     nu_t = Cs Delta^2 |S| (les.c:2510-2514) -- Cs and nu_t written in one pass.
+#include <type_traits>
+
 #include "fused.h"
 
 namespace tosca {
@@ -32,17 +34,38 @@ __device__ __forceinline__ double simpson_mask_sum(int c, int m) {
 }
 }  // namespace
 
+// Type of the filtered quantities and their sums: FP64 with FP64 flow fields.  With FP32 flow
+// fields they are FP32, after the tile's reference velocity is taken out of both factors of the
+// Leonard term (below), so that L = <U u> - <U><u> does not lose its digits to cancellation.
+template <class T> struct DsmSum { using type = double; };
+template <> struct DsmSum<float> { using type = float; };
+
 template <class T>
 __global__ void __launch_bounds__(64 * DSM_NR) k_dsm_lm(Geo g, const T *ucat, const double *csi,
                                                         const double *eta, const double *zet, const double *aj,
                                                         double *LM, double *MM, double *Sabs, int kchunk) {
+    using S = typename DsmSum<T>::type;
+    constexpr bool shifted = !std::is_same<S, double>::value;
     __shared__ T sU[3][DSM_NR][64];            // ucat of the plane, for the j neighbours
-    __shared__ double sH[NQ][DSM_NR][64];      // i-filtered quantities of the plane
+    __shared__ S sH[NQ][DSM_NR][64];           // i-filtered quantities of the plane
     const int lane = threadIdx.x, r = threadIdx.y;
     const int tile = xcd_tile();                // j fastest, then i, then the k chunk
     const int bj = tile % gridDim.y, bi = (tile / gridDim.y) % gridDim.x, bk = tile / (gridDim.x * gridDim.y);
     const int i = bi * DSM_TI + lane, j = bj * DSM_TJ + r;     // this thread's point column
     const int kb = 1 + bk * kchunk, ke = min(kb + kchunk, g.mz - 1);   // output planes [kb, ke)
+    // L_ij = <U_i u_j> - <U_i><u_j> is unchanged when U_i and u_j are shifted by constants (the
+    // filter is normalised), so with FP32 sums every point of the tile's march takes the velocity and
+    // the contravariant velocity of one interior cell of the tile out of the two factors
+    Vec3<T> uref{T(0), T(0), T(0)}, Uref{T(0), T(0), T(0)};
+    if constexpr (shifted) {
+        const int ic = min(max(bi * DSM_TI + 32, 1), g.mx - 2), jc = min(max(bj * DSM_TJ + 3, 1), g.my - 2);
+        const int kc = min(kb, g.mz - 2);
+        const int cr = g.L(kc, jc, ic);
+        uref = ld3(ucat, cr);
+        const Vec3<T> cr3 = cvt3<T>(ld3(csi, cr)), er3 = cvt3<T>(ld3(eta, cr)), zr3 = cvt3<T>(ld3(zet, cr));
+        Uref = Vec3<T>{uref.x * cr3.x + uref.y * cr3.y + uref.z * cr3.z, uref.x * er3.x + uref.y * er3.y + uref.z * er3.z,
+                       uref.x * zr3.x + uref.y * zr3.y + uref.z * zr3.z};
+    }
     const bool mi = i >= 1 && i <= g.mx - 2, mj = j >= 1 && j <= g.my - 2;
     const bool outCol = lane >= 1 && lane <= DSM_TI && r >= 1 && r <= DSM_TJ && mi && mj;
     // addresses of columns past the array are clamped; those points are masked out
@@ -58,9 +81,9 @@ __global__ void __launch_bounds__(64 * DSM_NR) k_dsm_lm(Geo g, const T *ucat, co
     const double *pc = csi + 3 * c0, *pe = eta + 3 * c0, *pz = zet + 3 * c0, *pa = aj + c0;
 
     // partial sums along k: A for output K-1 (terms K-2, K-1 in), B for output K (term K-1 in)
-    double A[NQ], B[NQ];
+    S A[NQ], B[NQ];
 #pragma unroll
-    for (int n = 0; n < NQ; n++) { A[n] = 0.0; B[n] = 0.0; }
+    for (int n = 0; n < NQ; n++) { A[n] = S(0); B[n] = S(0); }
     double sabsPrev = 0.0;
 
     for (int K = kb - 1; K <= ke; K++) {
@@ -84,12 +107,12 @@ __global__ void __launch_bounds__(64 * DSM_NR) k_dsm_lm(Geo g, const T *ucat, co
         // ---- point quantities (les.c:466-611), zero outside the mask -------------------------------
         // base values b: u v w, U V W, Sxx Sxy Sxz Syy Syz Szz, |S|, 1/aj; the 28 filtered quantities
         // are formed from them one at a time, right before their i filter
-        double b[14];
+        S b[14];
 #pragma unroll
-        for (int n = 0; n < 14; n++) b[n] = 0.0;
+        for (int n = 0; n < 14; n++) b[n] = S(0);
         if (pt) {
             // velocity arithmetic in T (the strain, |S| and the contravariant velocity); the
-            // filtered quantities below are formed and summed in double
+            // filtered quantities below are formed and summed in S
             const Vec3<T> cT = cvt3<T>(cC), eT = cvt3<T>(eC), zT = cvt3<T>(zC);
             const CartT<T> d = du_dxyz(cT, eT, zT, T(aC), du_center_v(g, i, j, K, uK, uim, uip, ujm, ujp, uKm, uKp));
             const T h = T(0.5);
@@ -97,35 +120,39 @@ __global__ void __launch_bounds__(64 * DSM_NR) k_dsm_lm(Geo g, const T *ucat, co
             b[3] = uK.x * cT.x + uK.y * cT.y + uK.z * cT.z;
             b[4] = uK.x * eT.x + uK.y * eT.y + uK.z * eT.z;
             b[5] = uK.x * zT.x + uK.y * zT.y + uK.z * zT.z;
+            if constexpr (shifted) {
+                b[0] -= uref.x; b[1] -= uref.y; b[2] -= uref.z;
+                b[3] -= Uref.x; b[4] -= Uref.y; b[5] -= Uref.z;
+            }
             b[6] = h * (d.du_dx + d.du_dx); b[7] = h * (d.du_dy + d.dv_dx); b[8] = h * (d.du_dz + d.dw_dx);
             b[9] = h * (d.dv_dy + d.dv_dy); b[10] = h * (d.dv_dz + d.dw_dy); b[11] = h * (d.dw_dz + d.dw_dz);
             b[12] = strain_mag(d);
-            b[13] = 1. / aC;
+            b[13] = S(1. / aC);
         }
         const double sabs = b[12];
         // ---- filter along i (lanes), then j (LDS) ---------------------------------------------------
 #pragma unroll
         for (int n = 0; n < NQ; n++) {
-            double v;
+            S v;
             if (n < 6) v = b[n];
             else if (n < 15) v = b[3 + (n - 6) / 3] * b[(n - 6) % 3];      // U_a u_b
             else if (n < 21) v = b[6 + (n - 15)];                            // S_ij
             else if (n < 27) v = b[6 + (n - 21)] * b[12];                    // S_ij |S|
             else v = b[13];
-            const double lo = shup(v), hi = shdn(v);
-            sH[n][r][lane] = (n < 27) ? (lo + 4.0 * v + hi) : (0.25 * lo + 0.5 * v + 0.25 * hi);
+            const S lo = shup(v), hi = shdn(v);
+            sH[n][r][lane] = (n < 27) ? (lo + S(4.0) * v + hi) : (S(0.25) * lo + S(0.5) * v + S(0.25) * hi);
         }
         __syncthreads();
         const int rm = r > 0 ? r - 1 : r, rp = r < DSM_NR - 1 ? r + 1 : r;
         // ---- filter along k, and the finished output cell K-1 --------------------------------------
         const bool emit = outCol && K - 1 >= kb && K - 1 < ke;
-        double q[NQ];
+        S q[NQ];
 #pragma unroll
         for (int n = 0; n < NQ; n++) {
-            const double f = (n < 27) ? (sH[n][rm][lane] + 4.0 * sH[n][r][lane] + sH[n][rp][lane])
-                                      : (0.25 * sH[n][rm][lane] + 0.5 * sH[n][r][lane] + 0.25 * sH[n][rp][lane]);
-            const double wK = (n < 27) ? 1.0 : 0.25, wC = (n < 27) ? 4.0 : 0.5;
-            const double done = A[n] + wK * f;       // output K-1 complete
+            const S f = (n < 27) ? (sH[n][rm][lane] + S(4.0) * sH[n][r][lane] + sH[n][rp][lane])
+                                 : (S(0.25) * sH[n][rm][lane] + S(0.5) * sH[n][r][lane] + S(0.25) * sH[n][rp][lane]);
+            const S wK = (n < 27) ? S(1.0) : S(0.25), wC = (n < 27) ? S(4.0) : S(0.5);
+            const S done = A[n] + wK * f;            // output K-1 complete
             A[n] = B[n] + wC * f;
             B[n] = wK * f;
             q[n] = done;
@@ -136,7 +163,7 @@ __global__ void __launch_bounds__(64 * DSM_NR) k_dsm_lm(Geo g, const T *ucat, co
             const double rden = 1.0 / (dij * simpson_mask_sum(k, g.mz));
             double F[27];
 #pragma unroll
-            for (int n = 0; n < 27; n++) F[n] = q[n] * rden;
+            for (int n = 0; n < 27; n++) F[n] = double(q[n]) * rden;
             double L[3][3], Sh[3][3], SSh[3][3];
             L[0][0] = F[6] - F[3] * F[0]; L[0][1] = F[7] - F[3] * F[1]; L[0][2] = F[8] - F[3] * F[2];
             L[1][0] = F[9] - F[4] * F[0]; L[1][1] = F[10] - F[4] * F[1]; L[1][2] = F[11] - F[4] * F[2];
@@ -146,7 +173,7 @@ __global__ void __launch_bounds__(64 * DSM_NR) k_dsm_lm(Geo g, const T *ucat, co
             SSh[0][0] = F[21]; SSh[0][1] = SSh[1][0] = F[22]; SSh[0][2] = SSh[2][0] = F[23];
             SSh[1][1] = F[24]; SSh[1][2] = SSh[2][1] = F[25]; SSh[2][2] = F[26];
             const double filter = cube_root(1. / aj[c]);
-            const double test_filter = cube_root(8.0 * q[27]);
+            const double test_filter = cube_root(8.0 * double(q[27]));
             double nm, dn;
             germano(L, Sh, SSh, filter, test_filter, ld3(csi, c), ld3(eta, c), ld3(zet, c), nm, dn);
             LM[c] = nm;
