@@ -307,6 +307,15 @@ __device__ __forceinline__ void germano(const double L[3][3], const double Sh[3]
     germano_contract(L, M, G, num_out, den_out);
 }
 
+// An index the compiler must treat as unknown: kernels that march over planes form each plane's
+// element index from a per-lane base and a uniform offset and pass it through here once per plane,
+// so that the per-array addresses are recomputed from (kernel-argument base + index) at each use
+// instead of being strength-reduced into loop-carried 64-bit pointers, one register pair per array.
+__device__ __forceinline__ int opaque_index(int x) {
+    __asm__ volatile("" : "+v"(x));
+    return x;
+}
+
 // Neighbour values across the lanes of a 64-wide wave: shup gives lane l the value of lane l-1,
 // shdn that of lane l+1.  Lane 0 (shup) and lane 63 (shdn) have no neighbour and receive 0; the
 // tiled kernels use both lanes as halo lanes whose own results are discarded.  The DPP wavefront

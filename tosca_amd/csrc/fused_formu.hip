@@ -197,12 +197,7 @@ __device__ __forceinline__ FluxT<T> ldflux(const double *p) {
 
 // Every array of the main kernel is addressed as its kernel-argument base (uniform, in scalar
 // registers) plus a 32-bit element index per lane, instead of a 64-bit pointer per array and lane:
-// the column's local index c0 plus the plane offset, made opaque once per plane (opaque_index) so
-// that the compiler cannot turn the per-array addresses back into loop-carried 64-bit pointers.
-__device__ __forceinline__ int opaque_index(int x) {
-    __asm__ volatile("" : "+v"(x));
-    return x;
-}
+// the column's local index c0 plus the plane offset, made opaque once per plane (opaque_index).
 
 template <int DIR, class T, class MT>
 __device__ __forceinline__ const MT *face_fc(const MainArgs<T, MT> &A) { return DIR == 0 ? A.icsi : DIR == 1 ? A.jcsi : A.kcsi; }

@@ -73,12 +73,16 @@ __global__ void __launch_bounds__(64 * DSM_NR) k_dsm_lm(Geo g, const T *ucat, co
     const int ima = max(ia - 1, -GW), ipa = min(ia + 1, g.mx + 2), jma = max(ja - 1, -GW), jpa = min(ja + 1, g.my + 2);
     const double dij = mi && mj ? simpson_mask_sum(i, g.mx) * simpson_mask_sum(j, g.my) : 1.0;
     const int SK = g.gxm * g.gym;
-    const long c0 = g.L(0, ja, ia);
-    const T *pu = ucat + 3 * c0;
+    // every array is addressed as its kernel-argument base plus a 32-bit index (the column's local
+    // index at plane 0 plus the plane offset), not through 64-bit pointers kept per lane
+    const int c0 = g.L(0, ja, ia);
 
     // k-ring of ucat for this column: planes K-1, K, K+1 of the current point plane K
-    Vec3<T> uKm = ld3(pu, (kb - 2) * SK), uK = ld3(pu, (kb - 1) * SK);
-    const double *pc = csi + 3 * c0, *pe = eta + 3 * c0, *pz = zet + 3 * c0, *pa = aj + c0;
+    Vec3<T> uKm = ld3(ucat, c0 + (kb - 2) * SK), uK = ld3(ucat, c0 + (kb - 1) * SK);
+    // the metrics of the previous plane's point, the cell the finished output belongs to: carried in
+    // registers where the FP32 sums leave room for them, loaded again with FP64 sums
+    V3 cP{0, 0, 0}, eP{0, 0, 0}, zP{0, 0, 0};
+    double aP = 1.0;
 
     // partial sums along k: A for output K-1 (terms K-2, K-1 in), B for output K (term K-1 in)
     S A[NQ], B[NQ];
@@ -89,9 +93,10 @@ __global__ void __launch_bounds__(64 * DSM_NR) k_dsm_lm(Geo g, const T *ucat, co
     for (int K = kb - 1; K <= ke; K++) {
         // loads of this plane, issued before the exchange barrier (a register prefetch of the next
         // plane spills at this register budget and measured slower)
-        const Vec3<T> uKp = ld3(pu, (K + 1) * SK);
-        const V3 cC = ld3(pc, K * SK), eC = ld3(pe, K * SK), zC = ld3(pz, K * SK);
-        const double aC = pa[(long)K * SK];
+        const int cK = opaque_index(c0 + K * SK);
+        const Vec3<T> uKp = ld3(ucat, cK + SK);
+        const V3 cC = ld3(csi, cK), eC = ld3(eta, cK), zC = ld3(zet, cK);
+        const double aC = aj[cK];
         // ---- neighbours of the point (i, j, K) ----------------------------------------------------
         sU[0][r][lane] = uK.x; sU[1][r][lane] = uK.y; sU[2][r][lane] = uK.z;
         const bool pt = mi && mj && K >= 1 && K <= g.mz - 2;     // point inside the filter mask
@@ -172,15 +177,17 @@ __global__ void __launch_bounds__(64 * DSM_NR) k_dsm_lm(Geo g, const T *ucat, co
             Sh[1][1] = F[18]; Sh[1][2] = Sh[2][1] = F[19]; Sh[2][2] = F[20];
             SSh[0][0] = F[21]; SSh[0][1] = SSh[1][0] = F[22]; SSh[0][2] = SSh[2][0] = F[23];
             SSh[1][1] = F[24]; SSh[1][2] = SSh[2][1] = F[25]; SSh[2][2] = F[26];
-            const double filter = cube_root(1. / aj[c]);
+            if constexpr (!shifted) { cP = ld3(csi, c); eP = ld3(eta, c); zP = ld3(zet, c); aP = aj[c]; }
+            const double filter = cube_root(1. / aP);
             const double test_filter = cube_root(8.0 * double(q[27]));
             double nm, dn;
-            germano(L, Sh, SSh, filter, test_filter, ld3(csi, c), ld3(eta, c), ld3(zet, c), nm, dn);
+            germano(L, Sh, SSh, filter, test_filter, cP, eP, zP, nm, dn);
             LM[c] = nm;
             MM[c] = dn;
             Sabs[c] = sabsPrev;
         }
         sabsPrev = sabs;
+        if constexpr (shifted) { cP = cC; eP = eC; zP = zC; aP = aC; }
         uKm = uK;
         uK = uKp;
         // the next plane writes sH only after its sU barrier, which every thread reaches after its j

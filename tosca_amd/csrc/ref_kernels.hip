@@ -349,11 +349,18 @@ __global__ void k_box_clip(const double *LM, const double *MM, const double *aj,
 }
 
 // DPASD j-plane sums (les.c:1979-2027): block (jplane, chunk) sums LM, MM, QN, NN over a
-// contiguous chunk of the plane's (k, i) cells; partials are combined in a fixed order below so
-// the result is bitwise reproducible from run to run.
+// contiguous chunk of the plane's (k, i) cells.  Each thread sums its strided cells, each wave
+// reduces its 64 lanes with a butterfly of cross-lane exchanges (no LDS, no barrier), and the
+// four wave sums are added in wave order after one barrier; the chunk partials are combined in a
+// fixed order by k_dpasd_cs, so the plane means are bitwise reproducible from run to run.
 constexpr int PLANE_CHUNKS = 16;
-__global__ void k_plane_partial(const double *LM, const double *MM, const double *QN, const double *NN,
-                                double *part, Geo g) {
+constexpr int PLANE_THREADS = 256;
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__global__ void __launch_bounds__(PLANE_THREADS) k_plane_partial(const double *LM, const double *MM, const double *QN,
+                                                                 const double *NN, double *part, Geo g) {
     const int j = blockIdx.x + 1, chunk = blockIdx.y;
     const int nk = g.mz - 2, ni = g.mx - 2, n = nk * ni;
     const int per = (n + PLANE_CHUNKS - 1) / PLANE_CHUNKS;
@@ -364,16 +371,19 @@ __global__ void k_plane_partial(const double *LM, const double *MM, const double
         int c = g.L(k, j, i);
         s[0] += LM[c]; s[1] += MM[c]; s[2] += QN[c]; s[3] += NN[c];
     }
-    __shared__ double red[4][256];
-    for (int q = 0; q < 4; q++) red[q][threadIdx.x] = s[q];
-    __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w)
-            for (int q = 0; q < 4; q++) red[q][threadIdx.x] += red[q][threadIdx.x + w];
-        __syncthreads();
+    __shared__ double red[4][PLANE_THREADS / 64];
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    for (int q = 0; q < 4; q++) {
+        const double w = wave_sum(s[q]);
+        if (lane == 0) red[q][wave] = w;
     }
+    __syncthreads();
     if (threadIdx.x == 0)
-        for (int q = 0; q < 4; q++) part[((long)j * PLANE_CHUNKS + chunk) * 4 + q] = red[q][0];
+        for (int q = 0; q < 4; q++) {
+            double t = 0.0;
+            for (int w = 0; w < PLANE_THREADS / 64; w++) t += red[q][w];
+            part[((long)j * PLANE_CHUNKS + chunk) * 4 + q] = t;
+        }
 }
 
 __global__ void k_dpasd_cs(const double *part, double *Cs, double maxCs, Geo g) {
@@ -558,7 +568,8 @@ void ref_update_cs(const Geo &g, const tosca_mesh *m, const tosca_params *p, con
         TOSCA_LAUNCH("k_box_clip", k_box_clip, gi, blk2(), 0, s, LM, MM, m->aj, cs, p->max_cs, g);
     } else {
         TOSCA_LAUNCH("k_test_filter5", k_test_filter<5>, gi, blk2(), 0, s, ucat, m->csi, m->eta, m->zet, m->aj, S, QN, NN, g);
-        TOSCA_LAUNCH("k_plane_partial", k_plane_partial, dim3(g.my - 2, PLANE_CHUNKS), dim3(256), 0, s, LM, MM, QN, NN, part, g);
+        TOSCA_LAUNCH("k_plane_partial", k_plane_partial, dim3(g.my - 2, PLANE_CHUNKS), dim3(PLANE_THREADS), 0, s, LM, MM, QN, NN,
+                     part, g);
         TOSCA_LAUNCH("k_dpasd_cs", k_dpasd_cs, gi, blk2(), 0, s, part, cs, p->max_cs, g);
     }
     launch_wrap(cs, 1, g, s);
