@@ -47,6 +47,13 @@ KERNEL_BYTES = {
     "k_box_clip": 16 + 8 + 8,                             # LM, MM, aj -> Cs
     "k_nut": 24 + 72 + 8 + 8 + 8,                         # ucat, cell metrics, aj, Cs -> nut
     "k_buoyancy": 8 + 72 + 24,                            # theta, icsi/jeta/kzet -> bTheta
+    # plane-averaged dynamic model (DPASD), fused: the 5-point filter's 28 (i, j)-filtered sums
+    # pass through HBM once (written by k_dpasd_ij, read by k_dpasd_k)
+    "k_dpasd_ij": 24 + 72 + 8 + 28 * 8,                   # ucat, cell metrics, aj -> 28 sums
+    "k_dpasd_k": 28 * 8 + 2 * 8 + 72 + 8,                 # 28 sums, LM, MM, cell metrics, aj -> plane partials
+    "k_dpasd_cs_nut": 8 + 8 + 2 * 8,                      # |S|, aj -> Cs, nut
+    "k_dpasd_ij_f32": 12 + 72 + 8 + 28 * 4,
+    "k_dpasd_k_f32": 28 * 4 + 2 * 8 + 72 + 8,
 }
 # Whole step: every ABI input read once, every output written once (rhs read + written);
 # nvert / meshTag are validated once per mesh (tosca_mesh_check) and not read per step.
@@ -252,6 +259,100 @@ def vjp_leg(torch, ctx, mesh, case, ucont, ucat, theta, cs, lay, args, step_ms):
     return {"vjp_ms": vjp_ms, "forward_ms": fwd_ms, "vjp_over_forward": vjp_ms / fwd_ms,
             "vjp_over_step": vjp_ms / step_ms, "kernels_ms": {k: v[0] / v[1] for k, v in tim.items()},
             "forward": "UpdateNut + FormU + Buoyancy (FP64) with the step's Cs"}
+
+
+def dpasd_leg(torch, ctx, mesh, case, ucont, ucat, theta, lay, args, farm=None):
+    """The same step with the plane-averaged dynamic model (DPASD: the 3- and 5-point test filters
+    and the j-plane means) on the fused kernels, FP64 and mixed, with the per-kernel times and
+    algorithmic rates of its kernels; and UpdateCs alone on the reference-structured kernels (a
+    gathered 5 x 5 x 5 stencil per cell), timed over a few calls."""
+    dev = "cuda"
+    params = dict(case.params, les_model=5)
+    gs, gl = lay.gshape, (lay.mz, lay.my, lay.mx, 3)
+    rhs = torch.zeros(gl, dtype=torch.float64, device=dev)
+    bt = torch.zeros_like(rhs)
+    cs = torch.zeros(gs, dtype=torch.float64, device=dev)
+    nut = torch.zeros(gs, dtype=torch.float64, device=dev)
+    V32, u32, t32 = ucont.float().contiguous(), ucat.float().contiguous(), theta.float().contiguous()
+    cs32 = torch.zeros_like(cs)
+    nut32 = torch.zeros(gs, dtype=torch.float32, device=dev)
+
+    def fp64():
+        if farm is None:
+            ctx.rhs_les_step(mesh, params, ucont, ucat, theta, rhs, 1.0, cs, nut, bt, 1.0)
+        else:
+            ctx.rhs_les_step_farm(mesh, params, ucont, ucat, theta, farm, rhs, 1.0, cs, nut, bt, 1.0)
+
+    def mixed():
+        if farm is None:
+            ctx.rhs_les_step_mixed(mesh, params, V32, u32, t32, rhs, 1.0, cs32, nut32, bt, 1.0)
+        else:
+            ctx.rhs_les_step_mixed_farm(mesh, params, V32, u32, t32, farm, rhs, 1.0, cs32, nut32, bt, 1.0)
+
+    def timed(fn, reps):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        ctx.enable_timing(True)
+        ctx.reset_timing()
+        for _ in range(reps):
+            fn()
+        tim = ctx.timings()
+        ctx.enable_timing(False)
+        return ms, {k: v[0] / v[1] for k, v in tim.items()}
+
+    out = {}
+    for name, fn in (("fp64", fp64), ("mixed", mixed)):
+        ms, kms = timed(fn, args.steps)
+        rates = {k: KERNEL_BYTES[k] * lay.ncells / (v * 1e-3) / 1e9 for k, v in kms.items()
+                 if k.startswith("k_dpasd") and k in KERNEL_BYTES}
+        out[name] = {"ms_per_step": ms, "value": lay.ncells / (ms * 1e-3), "unit": "cell-updates/s",
+                     "kernels_ms": kms, "algorithmic_GBs": rates,
+                     "frac_of_hbm_peak": {k: v / HBM_PEAK_GBS for k, v in rates.items()}}
+        print(f"# dpasd {name}: {ms:.3f} ms/step, {lay.ncells / (ms * 1e-3):.3e} cells/s; "
+              + ", ".join(f"{k} {kms[k]:.3f} ms ({r:.0f} GB/s)" for k, r in rates.items()), file=sys.stderr, flush=True)
+    out["mixed"]["speedup_vs_fp64"] = out["fp64"]["ms_per_step"] / out["mixed"]["ms_per_step"]
+    d = (cs32 - cs)[3:-3, 3:-3, 3:-3]
+    out["mixed"]["cs_rel_l2_vs_fp64"] = float(torch.linalg.vector_norm(d)) / max(
+        float(torch.linalg.vector_norm(cs[3:-3, 3:-3, 3:-3])), 1e-300)
+    # UpdateCs alone, fused and on the reference-structured kernels
+    ms_f, _ = timed(lambda: ctx.UpdateCs(mesh, params, ucat, cs), 5)
+    old = os.environ.get("TOSCA_PATH")
+    os.environ["TOSCA_PATH"] = "ref"
+    from tosca_amd.rhs import RhsContext
+    rctx = RhsContext(lay.info())
+    if old is None:
+        os.environ.pop("TOSCA_PATH")
+    else:
+        os.environ["TOSCA_PATH"] = old
+    rctx.mesh_check(mesh)
+    cs_r = torch.zeros_like(cs)
+    rctx.UpdateCs(mesh, params, ucat, cs_r)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(2):
+        rctx.UpdateCs(mesh, params, ucat, cs_r)
+    e1.record()
+    torch.cuda.synchronize()
+    ms_r = e0.elapsed_time(e1) / 2
+    dr = (cs_r - cs)[3:-3, 3:-3, 3:-3]
+    rel = float(torch.linalg.vector_norm(dr)) / max(float(torch.linalg.vector_norm(cs_r[3:-3, 3:-3, 3:-3])), 1e-300)
+    rctx.close()
+    del rctx
+    out["update_cs"] = {"fused_ms": ms_f, "reference_structured_ms": ms_r, "speedup": ms_r / ms_f,
+                        "cs_rel_l2_fused_vs_reference_structured": rel}
+    print(f"# dpasd mixed x{out['mixed']['speedup_vs_fp64']:.2f} (Cs rel L2 {out['mixed']['cs_rel_l2_vs_fp64']:.1e}); "
+          f"UpdateCs fused {ms_f:.2f} ms vs reference-structured {ms_r:.1f} ms (Cs rel L2 {rel:.1e})",
+          file=sys.stderr, flush=True)
+    return out
 
 
 def rk3w_leg(torch, ctx, mesh, case, ucont, ucat, nut, farm, lay, args, n_stages=10):
@@ -750,6 +851,7 @@ def main():
         "kernels_ms": {k: v[0] for k, v in per.items()},
     }
     line["mixed"] = mixed_leg(torch, ctx, mesh, case, ucont, ucat, theta, lay, args, ms, farm)
+    line["dpasd"] = dpasd_leg(torch, ctx, mesh, case, ucont, ucat, theta, lay, args, farm)
     line["vjp"] = vjp_leg(torch, ctx, mesh, case, ucont, ucat, theta, cs, lay, args, ms)
     line["explicit_rhs_u"] = explicit_leg(torch, ctx, mesh, case, ucont, ucat, nut, lay, args)
     line["rk3w_adjoint"] = rk3w_leg(torch, ctx, mesh, case, ucont, ucat, nut, farm, lay, args)

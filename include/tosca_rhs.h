@@ -224,10 +224,11 @@ int tosca_rhs_les_step_host(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_
 /* Mixed-precision form of tosca_rhs_les_step (no reference counterpart: a new precision mode of
  * the same unit of work).  The flow fields are FP32 (ucont, ucat, theta in; nut out); the metrics
  * and Jacobians stay FP64 (tosca_mesh), as do rhs (accumulated), cs and btheta.  Arithmetic:
- * velocity, strain and face-flux arithmetic in FP32; the cell-face flux differences (fp), the
- * face dot into rhs, the buoyancy source, the test-filter sums, the Germano contraction, the box
- * average and its denominator in FP64.  LES models: SMAGORINSKY and DSM (others:
- * TOSCA_ERR_UNSUPPORTED). */
+ * velocity, strain and face-flux arithmetic in FP32; the test-filter sums in FP32 after a
+ * per-tile reference velocity is taken out of both factors of the Leonard term (which leaves it
+ * unchanged); the cell-face flux differences (fp), the face dot into rhs, the buoyancy source, the
+ * Germano contraction, the box average, the plane means and the coefficient in FP64.  LES models:
+ * SMAGORINSKY, DSM and DPASD (others: TOSCA_ERR_UNSUPPORTED). */
 int tosca_rhs_les_step_mixed(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_params *params,
                              const float *ucont, const float *ucat, const float *theta,
                              double *rhs, double scale, double *cs, float *nut,

@@ -49,6 +49,8 @@ CASES = [
     ("hill multitile", lambda: C.hill_case(nx=40, ny=130, nz=20, les_model=C.LES_DSM)),
     ("abl dsm", lambda: C.abl_case(nx=36, ny=70, nz=24, les_model=C.LES_DSM, n_modes=64)),
     ("shard dsm", lambda: C.shard_case(nx=64, ny=64, nz=32)),
+    ("hill dpasd", lambda: C.hill_case(nx=24, ny=12, nz=10, les_model=C.LES_DPASD)),
+    ("hill dpasd multitile", lambda: C.hill_case(nx=40, ny=130, nz=28, les_model=C.LES_DPASD)),
 ]
 
 
@@ -84,7 +86,7 @@ def test_mixed_step(rhs_mod, name, builder):
 
 
 def test_mixed_rejects_unsupported_model(rhs_mod):
-    case = C.hill_case(nx=12, ny=8, nz=8, les_model=C.LES_DPASD)
+    case = C.hill_case(nx=12, ny=8, nz=8, les_model=C.LES_AMD)
     lay = case.lay
     ctx = rhs_mod.RhsContext(lay.info())
     mesh = {k: to_dev(v) for k, v in np_mesh(case.mesh).items()}

@@ -172,9 +172,30 @@ def test_formu_random_multitile(rhs_mod, periodic, scheme):
     (C.tgv_case, dict(n=66, ptype=C.PERIODIC_COPY, les_model=C.LES_SMAGORINSKY, div_scheme=C.DIV_CENTRAL)),
     (C.hill_case, dict(nx=40, ny=130, nz=20, les_model=C.LES_DSM)),
     (C.abl_case, dict(nx=36, ny=70, nz=24, les_model=C.LES_DSM, n_modes=64)),
+    (C.hill_case, dict(nx=40, ny=130, nz=28, les_model=C.LES_DPASD)),
+    (C.abl_case, dict(nx=40, ny=70, nz=30, les_model=C.LES_DPASD, n_modes=64)),
 ])
 def test_step_multitile(rhs_mod, builder, kw):
     _step_parity(rhs_mod, builder(**kw), f"{builder.__name__} {kw}")
+
+
+# several i tiles, j tiles and k chunks of the plane-averaged model's two filter kernels, and
+# several partials per j plane
+@pytest.mark.parametrize("periodic", [(2, 0, 1), (0, 0, 0), (1, 1, 2)])
+def test_update_cs_dpasd_multitile(rhs_mod, periodic):
+    lay, mesh, f, params = random_case(n=(40, 130, 28), periodic=periodic, seed=17, les_model=C.LES_DPASD, max_cs=1e6,
+                                       amp=0.1)
+    print(f"\nUpdateCs DPASD multi-tile periodic={periodic} {lay}", flush=True)
+    out = out_arrays(lay)
+    nm = np_mesh(mesh)
+    ucat = to_np(f["ucat"])
+    cs_o = out["cs"].copy()
+    O.update_cs(lay.info(), nm, params, ucat, cs_o)
+    ctx = rhs_mod.RhsContext(lay.info())
+    cs_g = to_dev(out["cs"])
+    ctx.UpdateCs(_dev_mesh(mesh), params, to_dev(ucat), cs_g)
+    torch.cuda.synchronize()
+    assert _report("cs", to_np(cs_g), cs_o) <= TOL
 
 
 @pytest.mark.parametrize("model", [C.LES_DSM, C.LES_DPASD])

@@ -242,8 +242,8 @@ tosca::FormUArgs formu_args(const tosca_mesh *m, const tosca_params *p, const do
 size_t ws_need(const tosca_ctx *c) {
     // 22 local fields (FormU: 6 face-flux vectors + fp; UpdateCs: strain 10 + LM/MM/QN/NN)
     // plus the DPASD plane partials
-    return std::max(22 * (size_t)c->geo.nloc() + (size_t)c->geo.my * 16 * 4 + 64,
-                    3 * (size_t)c->geo.nloc() + tosca::formu_ws_doubles(c->geo) + 64);
+    return std::max({22 * (size_t)c->geo.nloc() + (size_t)c->geo.my * 16 * 4 + 64,
+                     3 * (size_t)c->geo.nloc() + tosca::formu_ws_doubles(c->geo) + 64, tosca::dpasd_ws_doubles(c->geo)});
 }
 
 // The fused UpdateCs -> UpdateNut -> Buoyancy -> FormU sequence on the context's stream, for
@@ -258,15 +258,13 @@ void queue_step(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const 
     // UpdateCs (+ the interior of UpdateNut)
     if (p->les_model == TOSCA_LES_DSM) {
         tosca::fused_dsm<T>(g, m, ucat, p->max_cs, cs, nut, ws, s);
-    } else if (p->les_model == TOSCA_LES_SMAGORINSKY) {
+    } else if (p->les_model == TOSCA_LES_DPASD) {
+        tosca::fused_dpasd<T>(g, m, ucat, p->max_cs, cs, nut, ws, s);
+    } else {
         tosca::launch_fill(cs, (long)g.nloc(), 0.0289, s);   // VecSet(lCs, std_cs), les.c:172-179
         tosca::fused_nut<T>(g, m, ucat, cs, nut, nullptr, s);
-    } else {
-        // plane-averaged dynamic model: reference-structured kernels (FP64 flow fields only)
-        tosca::ref_update_cs(g, m, p, (const double *)ucat, cs, ws, s);
-        tosca::fused_nut<T>(g, m, ucat, cs, nut, nullptr, s);
     }
-    if (p->les_model == TOSCA_LES_DSM) {
+    if (p->les_model == TOSCA_LES_DSM || p->les_model == TOSCA_LES_DPASD) {
         tosca::launch_wrap(cs, 1, g, s);
         tosca::launch_reset_cell_periodic(cs, 1, g, s);
     }
@@ -309,9 +307,9 @@ int check_mixed(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const 
     if (rc) return rc;
     if ((rc = check_params(c, p))) return rc;
     if (!p->les_active) return set_err(c, TOSCA_ERR_ARG, "tosca_rhs_les_step_mixed needs les_active = 1");
-    if (p->les_model != TOSCA_LES_SMAGORINSKY && p->les_model != TOSCA_LES_DSM)
+    if (p->les_model != TOSCA_LES_SMAGORINSKY && p->les_model != TOSCA_LES_DSM && p->les_model != TOSCA_LES_DPASD)
         return set_err(c, TOSCA_ERR_UNSUPPORTED, "tosca_rhs_les_step_mixed: LES model %d (supported: Smagorinsky 0, "
-                                                 "dynamic Smagorinsky 2)", p->les_model);
+                                                 "dynamic Smagorinsky 2, plane-averaged dynamic 5)", p->les_model);
     if (p->div_scheme != TOSCA_DIV_CENTRAL && p->div_scheme != TOSCA_DIV_QUICK)
         return set_err(c, TOSCA_ERR_UNSUPPORTED, "div_scheme = %d: only central (0) and QUICK (1) are supported", p->div_scheme);
     if (!(p->t_ref != 0.0)) return set_err(c, TOSCA_ERR_ARG, "t_ref must be non-zero");
@@ -1358,8 +1356,9 @@ int tosca_update_cs(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, co
         if ((rc = check_dev(c, ucat, 3 * n * sizeof(double), "ucat"))) return rc;
     }
     if ((rc = ensure_ws(c, ws_need(c)))) return rc;
-    if (!c->ref_path && p->les_model == TOSCA_LES_DSM) {
-        tosca::fused_dsm<double>(c->geo, m, ucat, p->max_cs, cs, nullptr, c->ws, c->stream);
+    if (!c->ref_path && (p->les_model == TOSCA_LES_DSM || p->les_model == TOSCA_LES_DPASD)) {
+        if (p->les_model == TOSCA_LES_DSM) tosca::fused_dsm<double>(c->geo, m, ucat, p->max_cs, cs, nullptr, c->ws, c->stream);
+        else tosca::fused_dpasd<double>(c->geo, m, ucat, p->max_cs, cs, nullptr, c->ws, c->stream);
         tosca::launch_wrap(cs, 1, c->geo, c->stream);
         tosca::launch_reset_cell_periodic(cs, 1, c->geo, c->stream);
     } else {

@@ -334,6 +334,13 @@ __device__ __forceinline__ float shdn(float x) { return __int_as_float(dpp_shl(_
 template <class T> __device__ __forceinline__ Vec3<T> shup3(Vec3<T> a) { return Vec3<T>{shup(a.x), shup(a.y), shup(a.z)}; }
 template <class T> __device__ __forceinline__ Vec3<T> shdn3(Vec3<T> a) { return Vec3<T>{shdn(a.x), shdn(a.y), shdn(a.z)}; }
 
+// Sum of v over the 64 lanes of a wave by a butterfly of cross-lane exchanges; every lane
+// receives the same sum (each step adds the same two operands in every lane pair).
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
 // The filter widths Delta = (1/aj)^(1/3) and Delta~ = (8 sum)^(1/3) of les.c:657-660, 2510 are
 // written pow(x, 1./3.) in the reference.  The fused kernels take cbrt: the two differ by a few
 // ulp (pow's exponent 1./3. is itself rounded), and on gfx950 the FP64 pow expands to about six

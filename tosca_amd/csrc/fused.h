@@ -1,7 +1,7 @@
 // fused.h -- argument blocks and launchers of the fused gfx950 path (fused_formu.hip,
-// fused_les.hip).  The fused path serves tosca_formu, tosca_update_cs (DSM),
-// tosca_update_nut and tosca_rhs_les_step; DPASD and the standalone Buoyancy keep the
-// reference-structured kernels.
+// fused_les.hip).  The fused path serves tosca_formu, tosca_update_cs (DSM and DPASD),
+// tosca_update_nut and tosca_rhs_les_step; the standalone Buoyancy keeps the
+// reference-structured kernel.
 #pragma once
 
 #include "kernels.h"
@@ -43,6 +43,14 @@ size_t formu_ws_doubles(const Geo &g);
 template <class T>
 void fused_dsm(const Geo &g, const tosca_mesh *m, const T *ucat, double maxCs, double *cs, T *nut, double *ws,
                hipStream_t s);
+
+// Plane-averaged dynamic model (DPASD): LM, MM, |S| (3-point filter) and the 5-point filter's
+// QN, NN summed over every j plane into Cs(j), written to cs (interior) with, when nut != nullptr,
+// nut = Cs Delta^2 |S| (interior).  ws holds dpasd_ws_doubles(g) doubles.  T: type of ucat and nut.
+template <class T>
+void fused_dpasd(const Geo &g, const tosca_mesh *m, const T *ucat, double maxCs, double *cs, T *nut, double *ws,
+                 hipStream_t s);
+size_t dpasd_ws_doubles(const Geo &g);
 
 // nut = Cs Delta^2 |S| at interior cells from a given Cs (Smagorinsky, or any stored Cs)
 template <class T>

@@ -34,6 +34,52 @@ __device__ __forceinline__ double simpson_mask_sum(int c, int m) {
 }
 }  // namespace
 
+// The base values of the test-filter quantities at one point (les.c:466-611): u v w, U V W (the
+// contravariant velocity u.csi, u.eta, u.zet), Sxx Sxy Sxz Syy Syz Szz, |S|, 1/aj.  The velocity
+// arithmetic (strain, |S|, U) is in the flow type T; the shift (uref, Uref) is subtracted from
+// u and U when the sums are FP32.
+template <class S, class T>
+__device__ __forceinline__ void filter_base(const Geo &g, int i, int j, int K, Vec3<T> uK, Vec3<T> uim, Vec3<T> uip,
+                                            Vec3<T> ujm, Vec3<T> ujp, Vec3<T> uKm, Vec3<T> uKp, V3 cC, V3 eC, V3 zC,
+                                            double aC, Vec3<T> uref, Vec3<T> Uref, S b[14]) {
+    const Vec3<T> cT = cvt3<T>(cC), eT = cvt3<T>(eC), zT = cvt3<T>(zC);
+    const CartT<T> d = du_dxyz(cT, eT, zT, T(aC), du_center_v(g, i, j, K, uK, uim, uip, ujm, ujp, uKm, uKp));
+    const T h = T(0.5);
+    b[0] = uK.x; b[1] = uK.y; b[2] = uK.z;
+    b[3] = uK.x * cT.x + uK.y * cT.y + uK.z * cT.z;
+    b[4] = uK.x * eT.x + uK.y * eT.y + uK.z * eT.z;
+    b[5] = uK.x * zT.x + uK.y * zT.y + uK.z * zT.z;
+    if constexpr (!std::is_same<S, double>::value) {
+        b[0] -= uref.x; b[1] -= uref.y; b[2] -= uref.z;
+        b[3] -= Uref.x; b[4] -= Uref.y; b[5] -= Uref.z;
+    }
+    b[6] = h * (d.du_dx + d.du_dx); b[7] = h * (d.du_dy + d.dv_dx); b[8] = h * (d.du_dz + d.dw_dx);
+    b[9] = h * (d.dv_dy + d.dv_dy); b[10] = h * (d.dv_dz + d.dw_dy); b[11] = h * (d.dw_dz + d.dw_dz);
+    b[12] = strain_mag(d);
+    b[13] = S(1. / aC);
+}
+
+// Filtered quantity n of a point from its base values: u_a (0-2), U_a (3-5), U_a u_b (6-14),
+// S_ij (15-20), S_ij |S| (21-26), 1/aj (27)
+template <class S>
+__device__ __forceinline__ S filter_quantity(int n, const S b[14]) {
+    if (n < 6) return b[n];
+    if (n < 15) return b[3 + (n - 6) / 3] * b[(n - 6) % 3];
+    if (n < 21) return b[6 + (n - 15)];
+    if (n < 27) return b[6 + (n - 21)] * b[12];
+    return b[13];
+}
+
+// The reference-velocity shift of a tile: the velocity and the contravariant velocity of cell cr
+template <class T>
+__device__ __forceinline__ void tile_shift(const T *ucat, const double *csi, const double *eta, const double *zet, int cr,
+                                           Vec3<T> &uref, Vec3<T> &Uref) {
+    uref = ld3(ucat, cr);
+    const Vec3<T> cr3 = cvt3<T>(ld3(csi, cr)), er3 = cvt3<T>(ld3(eta, cr)), zr3 = cvt3<T>(ld3(zet, cr));
+    Uref = Vec3<T>{uref.x * cr3.x + uref.y * cr3.y + uref.z * cr3.z, uref.x * er3.x + uref.y * er3.y + uref.z * er3.z,
+                   uref.x * zr3.x + uref.y * zr3.y + uref.z * zr3.z};
+}
+
 // Type of the filtered quantities and their sums: FP64 with FP64 flow fields.  With FP32 flow
 // fields they are FP32, after the tile's reference velocity is taken out of both factors of the
 // Leonard term (below), so that L = <U u> - <U><u> does not lose its digits to cancellation.
@@ -60,11 +106,7 @@ __global__ void __launch_bounds__(64 * DSM_NR) k_dsm_lm(Geo g, const T *ucat, co
     if constexpr (shifted) {
         const int ic = min(max(bi * DSM_TI + 32, 1), g.mx - 2), jc = min(max(bj * DSM_TJ + 3, 1), g.my - 2);
         const int kc = min(kb, g.mz - 2);
-        const int cr = g.L(kc, jc, ic);
-        uref = ld3(ucat, cr);
-        const Vec3<T> cr3 = cvt3<T>(ld3(csi, cr)), er3 = cvt3<T>(ld3(eta, cr)), zr3 = cvt3<T>(ld3(zet, cr));
-        Uref = Vec3<T>{uref.x * cr3.x + uref.y * cr3.y + uref.z * cr3.z, uref.x * er3.x + uref.y * er3.y + uref.z * er3.z,
-                       uref.x * zr3.x + uref.y * zr3.y + uref.z * zr3.z};
+        tile_shift(ucat, csi, eta, zet, g.L(kc, jc, ic), uref, Uref);
     }
     const bool mi = i >= 1 && i <= g.mx - 2, mj = j >= 1 && j <= g.my - 2;
     const bool outCol = lane >= 1 && lane <= DSM_TI && r >= 1 && r <= DSM_TJ && mi && mj;
@@ -115,35 +157,12 @@ __global__ void __launch_bounds__(64 * DSM_NR) k_dsm_lm(Geo g, const T *ucat, co
         S b[14];
 #pragma unroll
         for (int n = 0; n < 14; n++) b[n] = S(0);
-        if (pt) {
-            // velocity arithmetic in T (the strain, |S| and the contravariant velocity); the
-            // filtered quantities below are formed and summed in S
-            const Vec3<T> cT = cvt3<T>(cC), eT = cvt3<T>(eC), zT = cvt3<T>(zC);
-            const CartT<T> d = du_dxyz(cT, eT, zT, T(aC), du_center_v(g, i, j, K, uK, uim, uip, ujm, ujp, uKm, uKp));
-            const T h = T(0.5);
-            b[0] = uK.x; b[1] = uK.y; b[2] = uK.z;
-            b[3] = uK.x * cT.x + uK.y * cT.y + uK.z * cT.z;
-            b[4] = uK.x * eT.x + uK.y * eT.y + uK.z * eT.z;
-            b[5] = uK.x * zT.x + uK.y * zT.y + uK.z * zT.z;
-            if constexpr (shifted) {
-                b[0] -= uref.x; b[1] -= uref.y; b[2] -= uref.z;
-                b[3] -= Uref.x; b[4] -= Uref.y; b[5] -= Uref.z;
-            }
-            b[6] = h * (d.du_dx + d.du_dx); b[7] = h * (d.du_dy + d.dv_dx); b[8] = h * (d.du_dz + d.dw_dx);
-            b[9] = h * (d.dv_dy + d.dv_dy); b[10] = h * (d.dv_dz + d.dw_dy); b[11] = h * (d.dw_dz + d.dw_dz);
-            b[12] = strain_mag(d);
-            b[13] = S(1. / aC);
-        }
+        if (pt) filter_base<S>(g, i, j, K, uK, uim, uip, ujm, ujp, uKm, uKp, cC, eC, zC, aC, uref, Uref, b);
         const double sabs = b[12];
         // ---- filter along i (lanes), then j (LDS) ---------------------------------------------------
 #pragma unroll
         for (int n = 0; n < NQ; n++) {
-            S v;
-            if (n < 6) v = b[n];
-            else if (n < 15) v = b[3 + (n - 6) / 3] * b[(n - 6) % 3];      // U_a u_b
-            else if (n < 21) v = b[6 + (n - 15)];                            // S_ij
-            else if (n < 27) v = b[6 + (n - 21)] * b[12];                    // S_ij |S|
-            else v = b[13];
+            const S v = filter_quantity(n, b);
             const S lo = shup(v), hi = shdn(v);
             sH[n][r][lane] = (n < 27) ? (lo + S(4.0) * v + hi) : (S(0.25) * lo + S(0.5) * v + S(0.25) * hi);
         }
@@ -307,6 +326,220 @@ __global__ void __launch_bounds__(256) k_nut_fused(Geo g, const T *ucat, const d
     }
 }
 
+// ---- plane-averaged dynamic model (DPASD, les.c:812-1174 and 1979-2048) ----------------------------
+// The second test filter weighs the 5 x 5 x 5 neighbourhood of a cell with the products of the 1-D
+// Simpson rows (1, 4, 2, 4, 1) (integrateTestfilterSimpson5x5, inline.h:3612-3633), over its
+// interior cells only (I, J, K in [1, m-2] whatever the periodicity), normalised by the weights of
+// those cells; its width is (64 sum bin5(R) bin5(Q) bin5(P) / aj)^(1/3) over the same cells, with
+// bin5 = (1, 4, 6, 4, 1)/16 (les.c:961-1030).  Both weights and mask are products of 1-D factors,
+// so the filter is applied one axis at a time:
+//   k_dpasd_ij: a 1024-thread workgroup owns 60 x 12 (i, j) cells plus two halo lanes / rows a side
+//     and marches along k.  Per plane each thread forms the 28 quantities of its point (those of
+//     the 3-point filter), filters them along i across the lanes of the wave (two DPP shifts each
+//     way) and along j through LDS, seven quantities per LDS round, and writes the (i, j)-filtered
+//     sums of the tile's cells to HBM (28 arrays over the interior cells, i fastest);
+//   k_dpasd_k: one thread per interior column marches over a chunk of planes: the k filter of the
+//     five planes (re-read from L2), the Germano contraction (QN, NN), and the cell's LM, MM of the
+//     3-point filter; the 64 cells of a wave (one j row) are summed by a cross-lane butterfly into a
+//     partial per (j, i tile, k chunk);
+//   k_dpasd_plane: the partials of each j plane in a fixed order -> the plane means -> Cs(j);
+//   k_dpasd_cs_nut: Cs and nu_t = Cs Delta^2 |S| of every interior cell.
+// Every sum has a fixed order, so Cs is bitwise reproducible from run to run.
+namespace {
+constexpr int DP_NR = 16;                       // rows of a k_dpasd_ij workgroup
+constexpr int DP_TI = 60, DP_TJ = DP_NR - 4;    // its output cells
+constexpr int DP_NC = 7;                        // quantities per LDS round of the j filter
+constexpr int DK_X = 64, DK_Y = 4;              // k_dpasd_k workgroup: one wave per j row
+
+// sum of the 1-D weights (1, 4, 2, 4, 1) over the interior points c-2 .. c+2 of an axis of size m
+__device__ __forceinline__ double simpson5_mask_sum(int c, int m) {
+    return ((c - 2 >= 1) ? 1.0 : 0.0) + ((c - 1 >= 1) ? 4.0 : 0.0) + 2.0 + ((c + 1 <= m - 2) ? 4.0 : 0.0) +
+           ((c + 2 <= m - 2) ? 1.0 : 0.0);
+}
+
+// 5-point row of quantity n over the values at offsets -2 .. 2: Simpson weights for the 27 Germano
+// quantities, the binomial (1, 4, 6, 4, 1)/16 for the volume term of the filter width
+template <class S>
+__device__ __forceinline__ S row5(int n, S m2, S m1, S c, S p1, S p2) {
+    if (n < 27) return ((m2 + p2) + S(4.0) * (m1 + p1)) + S(2.0) * c;
+    return (((m2 + p2) + S(4.0) * (m1 + p1)) + S(6.0) * c) * S(0.0625);
+}
+}  // namespace
+
+template <class T>
+__global__ void __launch_bounds__(64 * DP_NR) k_dpasd_ij(Geo g, const T *ucat, const double *csi, const double *eta,
+                                                         const double *zet, const double *aj,
+                                                         typename DsmSum<T>::type *X, long nI, int kchunk) {
+    using S = typename DsmSum<T>::type;
+    constexpr bool shifted = !std::is_same<S, double>::value;
+    __shared__ T sU[3][DP_NR][64];            // ucat of the plane, for the j neighbours of the strain
+    __shared__ S sH[DP_NC][DP_NR][64];        // i-filtered quantities of one LDS round
+    const int lane = threadIdx.x, r = threadIdx.y;
+    const int tile = xcd_tile();              // j fastest, then i, then the k chunk
+    const int bj = tile % gridDim.y, bi = (tile / gridDim.y) % gridDim.x, bk = tile / (gridDim.x * gridDim.y);
+    // lanes 2 .. 61 and rows 2 .. DP_NR-3 are the tile's cells, the others its halo
+    const int i = bi * DP_TI + lane - 1, j = bj * DP_TJ + r - 1;
+    const int kb = 1 + bk * kchunk, ke = min(kb + kchunk, g.mz - 1);     // planes [kb, ke)
+    // with FP32 sums every point of the tile takes the velocity and contravariant velocity of one
+    // interior cell out of the two factors of the Leonard term; the cell does not depend on the k
+    // chunk, so the five planes that k_dpasd_k combines for a cell were filtered with one shift
+    Vec3<T> uref{T(0), T(0), T(0)}, Uref{T(0), T(0), T(0)};
+    if constexpr (shifted) {
+        const int ic = min(max(bi * DP_TI + 30, 1), g.mx - 2), jc = min(max(bj * DP_TJ + 6, 1), g.my - 2);
+        tile_shift(ucat, csi, eta, zet, g.L((g.mz - 1) / 2, jc, ic), uref, Uref);
+    }
+    const bool mi = i >= 1 && i <= g.mx - 2, mj = j >= 1 && j <= g.my - 2;
+    const bool outCol = lane >= 2 && lane <= DP_TI + 1 && r >= 2 && r <= DP_TJ + 1 && mi && mj;
+    // i >= -1 and j >= -1 here; addresses of columns past the array are clamped, those points are
+    // outside the filter mask
+    const int ia = min(i, g.mx + 2), ja = min(j, g.my + 2);
+    const int ima = max(ia - 1, -GW), ipa = min(ia + 1, g.mx + 2), jma = max(ja - 1, -GW), jpa = min(ja + 1, g.my + 2);
+    const int SK = g.gxm * g.gym;
+    const int c0 = g.L(0, ja, ia);
+    const int ni = g.mx - 2, nj = g.my - 2;
+    const int x0 = (j - 1) * ni + (i - 1);    // interior index of the column at plane 1 (output cells)
+
+    Vec3<T> uKm = ld3(ucat, c0 + (kb - 1) * SK), uK = ld3(ucat, c0 + kb * SK);
+    for (int K = kb; K < ke; K++) {
+        const int cK = opaque_index(c0 + K * SK);
+        const Vec3<T> uKp = ld3(ucat, cK + SK);
+        const V3 cC = ld3(csi, cK), eC = ld3(eta, cK), zC = ld3(zet, cK);
+        const double aC = aj[cK];
+        sU[0][r][lane] = uK.x; sU[1][r][lane] = uK.y; sU[2][r][lane] = uK.z;
+        Vec3<T> uim = shup3(uK), uip = shdn3(uK);
+        if (lane == 0) uim = ld3(ucat, g.L(K, ja, ima));
+        if (lane == 63) uip = ld3(ucat, g.L(K, ja, ipa));
+        __syncthreads();
+        Vec3<T> ujm, ujp;
+        if (r > 0) ujm = Vec3<T>{sU[0][r - 1][lane], sU[1][r - 1][lane], sU[2][r - 1][lane]};
+        else ujm = ld3(ucat, g.L(K, jma, ia));
+        if (r < DP_NR - 1) ujp = Vec3<T>{sU[0][r + 1][lane], sU[1][r + 1][lane], sU[2][r + 1][lane]};
+        else ujp = ld3(ucat, g.L(K, jpa, ia));
+        S b[14];
+#pragma unroll
+        for (int n = 0; n < 14; n++) b[n] = S(0);
+        if (mi && mj) filter_base<S>(g, i, j, K, uK, uim, uip, ujm, ujp, uKm, uKp, cC, eC, zC, aC, uref, Uref, b);
+        const long xo = (long)(K - 1) * ni * nj + x0;
+#pragma unroll
+        for (int c = 0; c < NQ; c += DP_NC) {
+#pragma unroll
+            for (int q = 0; q < DP_NC; q++) {
+                const S v = filter_quantity(c + q, b);
+                const S l1 = shup(v), h1 = shdn(v);
+                const S l2 = shup(l1), h2 = shdn(h1);
+                sH[q][r][lane] = row5(c + q, l2, l1, v, h1, h2);
+            }
+            __syncthreads();
+            if (outCol) {
+#pragma unroll
+                for (int q = 0; q < DP_NC; q++)
+                    X[(long)(c + q) * nI + xo] = row5(c + q, sH[q][r - 2][lane], sH[q][r - 1][lane], sH[q][r][lane],
+                                                      sH[q][r + 1][lane], sH[q][r + 2][lane]);
+            }
+            // the next round overwrites sH; the last round's reads are ordered before the next
+            // plane's sH writes by that plane's sU barrier
+            if (c + DP_NC < NQ) __syncthreads();
+        }
+        uKm = uK;
+        uK = uKp;
+    }
+}
+
+template <class T>
+__global__ void __launch_bounds__(DK_X * DK_Y) k_dpasd_k(Geo g, const typename DsmSum<T>::type *X, long nI,
+                                                         const double *csi, const double *eta, const double *zet,
+                                                         const double *aj, const double *LM, const double *MM,
+                                                         double *part, int kchunk) {
+    using S = typename DsmSum<T>::type;
+    const int lane = threadIdx.x, r = threadIdx.y;
+    const int tile = xcd_tile();              // i fastest, then j, then the k chunk
+    const int bi = tile % gridDim.x, bj = (tile / gridDim.x) % gridDim.y, bk = tile / (gridDim.x * gridDim.y);
+    const int i = 1 + bi * DK_X + lane, j = 1 + bj * DK_Y + r;
+    const int kb = 1 + bk * kchunk, ke = min(kb + kchunk, g.mz - 1);
+    const int ni = g.mx - 2, nj = g.my - 2;
+    double sLM = 0.0, sMM = 0.0, sQN = 0.0, sNN = 0.0;
+    if (i <= g.mx - 2 && j <= g.my - 2) {
+        const double dij = simpson5_mask_sum(i, g.mx) * simpson5_mask_sum(j, g.my);
+        const long SP = (long)ni * nj;
+        const long x0 = (long)(j - 1) * ni + (i - 1);
+        for (int k = kb; k < ke; k++) {
+            // the five planes of the k filter; planes outside [1, mz-2] are outside the mask (the
+            // conditions are uniform over the workgroup)
+            const bool in0 = k - 2 >= 1, in1 = k - 1 >= 1, in3 = k + 1 <= g.mz - 2, in4 = k + 2 <= g.mz - 2;
+            const S *xk = X + (long)(k - 1) * SP + x0;
+            double F[NQ];
+#pragma unroll
+            for (int n = 0; n < NQ; n++) {
+                const S *xn = xk + (long)n * nI;
+                const S v0 = in0 ? xn[-2 * SP] : S(0), v1 = in1 ? xn[-SP] : S(0), v2 = xn[0];
+                const S v3 = in3 ? xn[SP] : S(0), v4 = in4 ? xn[2 * SP] : S(0);
+                F[n] = double(row5(n, v0, v1, v2, v3, v4));
+            }
+            const double rden = 1.0 / (dij * simpson5_mask_sum(k, g.mz));
+#pragma unroll
+            for (int n = 0; n < 27; n++) F[n] *= rden;
+            double L[3][3], Sh[3][3], SSh[3][3];
+            L[0][0] = F[6] - F[3] * F[0]; L[0][1] = F[7] - F[3] * F[1]; L[0][2] = F[8] - F[3] * F[2];
+            L[1][0] = F[9] - F[4] * F[0]; L[1][1] = F[10] - F[4] * F[1]; L[1][2] = F[11] - F[4] * F[2];
+            L[2][0] = F[12] - F[5] * F[0]; L[2][1] = F[13] - F[5] * F[1]; L[2][2] = F[14] - F[5] * F[2];
+            Sh[0][0] = F[15]; Sh[0][1] = Sh[1][0] = F[16]; Sh[0][2] = Sh[2][0] = F[17];
+            Sh[1][1] = F[18]; Sh[1][2] = Sh[2][1] = F[19]; Sh[2][2] = F[20];
+            SSh[0][0] = F[21]; SSh[0][1] = SSh[1][0] = F[22]; SSh[0][2] = SSh[2][0] = F[23];
+            SSh[1][1] = F[24]; SSh[1][2] = SSh[2][1] = F[25]; SSh[2][2] = F[26];
+            const int c = g.L(k, j, i);
+            const double filter = cube_root(1. / aj[c]);
+            const double test_filter = cube_root(64.0 * F[27]);
+            double qn, nn;
+            germano(L, Sh, SSh, filter, test_filter, ld3(csi, c), ld3(eta, c), ld3(zet, c), qn, nn);
+            sLM += LM[c];
+            sMM += MM[c];
+            sQN += qn;
+            sNN += nn;
+        }
+    }
+    // one wave per j row: the sums of its 64 cells (zero past the array)
+    sLM = wave_sum(sLM);
+    sMM = wave_sum(sMM);
+    sQN = wave_sum(sQN);
+    sNN = wave_sum(sNN);
+    if (lane == 0 && j <= g.my - 2) {
+        double *p = part + (((long)(j - 1) * gridDim.x + bi) * gridDim.z + bk) * 4;
+        p[0] = sLM; p[1] = sMM; p[2] = sQN; p[3] = sNN;
+    }
+}
+
+// plane means of LM, MM, QN, NN over the (k, i) cells of each j plane and the plane's coefficient
+// (les.c:1979-2048): the partials in (i tile, k chunk) order
+__global__ void k_dpasd_plane(const double *part, int nparts, double maxCs, double *csP, Geo g) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    if (j > g.my - 2) return;
+    double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    const double *p = part + (long)(j - 1) * nparts * 4;
+    for (int q = 0; q < nparts; q++) {
+        s0 += p[4 * q]; s1 += p[4 * q + 1]; s2 += p[4 * q + 2]; s3 += p[4 * q + 3];
+    }
+    const double cnt = (double)((long)(g.mz - 2) * (g.mx - 2));
+    const double J_LM = s0 / cnt, J_MM = s1 / cnt, J_QN = s2 / cnt, J_NN = s3 / cnt;
+    const double beta = (J_QN * J_MM) / (J_NN * J_LM + 1e-10);
+    const double v = 0.5 * (J_LM / (J_MM + 1e-10)) / beta;
+    csP[j] = pmin(pmax(v, 0.0), maxCs);
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_dpasd_cs_nut(Geo g, const double *csP, const double *Sabs, const double *aj,
+                                                      double *Cs, T *nut) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y * blockDim.y + threadIdx.y + 1,
+              k = blockIdx.z + 1;
+    if (i > g.mx - 2 || j > g.my - 2 || k > g.mz - 2) return;
+    const int c = g.L(k, j, i);
+    const double cs = csP[j];
+    Cs[c] = cs;
+    if (nut) {
+        const double filter = cube_root(1. / aj[c]);
+        nut[c] = T(cs * (filter * filter) * Sabs[c]);
+    }
+}
+
 // zero six disjoint boxes of an array with row length nx and plane size nx*ny (ncomp values per
 // point) in one launch: blockIdx.y selects the box, the blocks stride over its values
 struct ShellBoxes { int i0[6], j0[6], k0[6], ni[6], nj[6], nk[6]; };
@@ -374,20 +607,77 @@ void zero_outside_interior(T *a, const Geo &g, hipStream_t s) {
     zero_shell(a, 1, g.gxm, g.gym, g.gzm, GW, GW, s);
 }
 
+namespace {
+// k planes per chunk of a k-marching launch over `tiles` column tiles: about `target` workgroups in
+// all, at least 8 planes per chunk
+int plane_chunk(int nk, int tiles, int target) {
+    int chunks = (target + tiles - 1) / tiles;
+    chunks = max(1, min(chunks, nk / 8 > 0 ? nk / 8 : 1));
+    return (nk + chunks - 1) / chunks;
+}
+
+// LM, MM (3-point test filter + Germano contraction) and |S| of every interior cell
+template <class T>
+void launch_dsm_lm(const Geo &g, const tosca_mesh *m, const T *ucat, double *LM, double *MM, double *Sabs, hipStream_t s) {
+    const int nk = g.mz - 2;
+    const int ti = (g.mx - 2 + DSM_TI - 1) / DSM_TI, tj = (g.my - 2 + DSM_TJ - 1) / DSM_TJ;
+    const int kchunk = plane_chunk(nk, ti * tj, 2048);
+    dim3 grid(ti, tj, (nk + kchunk - 1) / kchunk);
+    TOSCA_LAUNCH(sizeof(T) == 8 ? "k_dsm_lm" : "k_dsm_lm_f32", k_dsm_lm<T>, grid, dim3(64, DSM_NR), 0, s, g, ucat, m->csi,
+                 m->eta, m->zet, m->aj, LM, MM, Sabs, kchunk);
+}
+
+// launch shape of k_dpasd_k, which sizes the partials buffer
+struct DpasdK { int ti, tj, chunks, kchunk; };
+DpasdK dpasd_k_shape(const Geo &g) {
+    DpasdK d;
+    const int nk = g.mz - 2;
+    d.ti = (g.mx - 2 + DK_X - 1) / DK_X;
+    d.tj = (g.my - 2 + DK_Y - 1) / DK_Y;
+    d.kchunk = plane_chunk(nk, d.ti * d.tj, 4096);
+    d.chunks = (nk + d.kchunk - 1) / d.kchunk;
+    return d;
+}
+}  // namespace
+
+size_t dpasd_ws_doubles(const Geo &g) {
+    const size_t nI = (size_t)(g.mx - 2) * (g.my - 2) * (g.mz - 2);
+    const DpasdK d = dpasd_k_shape(g);
+    return 3 * (size_t)g.nloc() + NQ * nI + (size_t)(g.my - 2) * d.ti * d.chunks * 4 + (size_t)g.my + 64;
+}
+
+template <class T>
+void fused_dpasd(const Geo &g, const tosca_mesh *m, const T *ucat, double maxCs, double *cs, T *nut, double *ws,
+                 hipStream_t s) {
+    using S = typename DsmSum<T>::type;
+    const long n = g.nloc();
+    const long nI = (long)(g.mx - 2) * (g.my - 2) * (g.mz - 2);
+    double *LM = ws, *MM = ws + n, *Sabs = ws + 2 * n;
+    S *X = reinterpret_cast<S *>(ws + 3 * n);
+    const DpasdK d = dpasd_k_shape(g);
+    double *part = ws + 3 * n + NQ * nI;      // an FP32 X takes half of its share
+    double *csP = part + (long)(g.my - 2) * d.ti * d.chunks * 4;
+    launch_dsm_lm<T>(g, m, ucat, LM, MM, Sabs, s);
+    const int nk = g.mz - 2;
+    const int ti = (g.mx - 2 + DP_TI - 1) / DP_TI, tj = (g.my - 2 + DP_TJ - 1) / DP_TJ;
+    const int kchunk = plane_chunk(nk, ti * tj, 1536);
+    TOSCA_LAUNCH(sizeof(T) == 8 ? "k_dpasd_ij" : "k_dpasd_ij_f32", k_dpasd_ij<T>, dim3(ti, tj, (nk + kchunk - 1) / kchunk),
+                 dim3(64, DP_NR), 0, s, g, ucat, m->csi, m->eta, m->zet, m->aj, X, nI, kchunk);
+    TOSCA_LAUNCH(sizeof(T) == 8 ? "k_dpasd_k" : "k_dpasd_k_f32", k_dpasd_k<T>, dim3(d.ti, d.tj, d.chunks), dim3(DK_X, DK_Y),
+                 0, s, g, X, nI, m->csi, m->eta, m->zet, m->aj, LM, MM, part, d.kchunk);
+    TOSCA_LAUNCH("k_dpasd_plane", k_dpasd_plane, dim3((g.my - 2 + 255) / 256), dim3(256), 0, s, part, d.ti * d.chunks,
+                 maxCs, csP, g);
+    TOSCA_LAUNCH(sizeof(T) == 8 ? "k_dpasd_cs_nut" : "k_dpasd_cs_nut_f32", k_dpasd_cs_nut<T>,
+                 dim3((g.mx - 2 + 63) / 64, (g.my - 2 + 3) / 4, g.mz - 2), dim3(64, 4), 0, s, g, csP, Sabs, m->aj, cs, nut);
+}
+
 template <class T>
 void fused_dsm(const Geo &g, const tosca_mesh *m, const T *ucat, double maxCs, double *cs, T *nut, double *ws,
                hipStream_t s) {
     const long n = g.nloc();
     double *LM = ws, *MM = ws + n, *Sabs = ws + 2 * n;
     const int nk = g.mz - 2;
-    const int tiles = ((g.mx - 2 + DSM_TI - 1) / DSM_TI) * ((g.my - 2 + DSM_TJ - 1) / DSM_TJ);
-    int chunks = (2048 + tiles - 1) / tiles;
-    chunks = max(1, min(chunks, nk / 8 > 0 ? nk / 8 : 1));
-    const int kchunk = (nk + chunks - 1) / chunks;
-    chunks = (nk + kchunk - 1) / kchunk;
-    dim3 grid((g.mx - 2 + DSM_TI - 1) / DSM_TI, (g.my - 2 + DSM_TJ - 1) / DSM_TJ, chunks);
-    TOSCA_LAUNCH(sizeof(T) == 8 ? "k_dsm_lm" : "k_dsm_lm_f32", k_dsm_lm<T>, grid, dim3(64, DSM_NR), 0, s, g, ucat, m->csi, m->eta, m->zet, m->aj, LM, MM,
-                 Sabs, kchunk);
+    launch_dsm_lm<T>(g, m, ucat, LM, MM, Sabs, s);
     const int ctiles = ((g.mx - 2 + CX - 1) / CX) * ((g.my - 2 + CY - 1) / CY);
     int cchunks = max(1, min((2304 + ctiles - 1) / ctiles, nk / 8 > 0 ? nk / 8 : 1));
     const int ckchunk = (nk + cchunks - 1) / cchunks;
@@ -411,6 +701,8 @@ void nut_bcs(T *nut, const tosca_params *p, const Geo &g, hipStream_t s) { launc
     template void zero_outside_interior<T>(T *, const Geo &, hipStream_t);                                  \
     template void fused_dsm<T>(const Geo &, const tosca_mesh *, const T *, double, double *, T *, double *,   \
                                hipStream_t);                                                                 \
+    template void fused_dpasd<T>(const Geo &, const tosca_mesh *, const T *, double, double *, T *, double *, \
+                                 hipStream_t);                                                               \
     template void fused_nut<T>(const Geo &, const tosca_mesh *, const T *, const double *, T *, T *,         \
                                hipStream_t);                                                                 \
     template void nut_bcs<T>(T *, const tosca_params *, const Geo &, hipStream_t);

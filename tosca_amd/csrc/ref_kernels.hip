@@ -355,10 +355,6 @@ __global__ void k_box_clip(const double *LM, const double *MM, const double *aj,
 // fixed order by k_dpasd_cs, so the plane means are bitwise reproducible from run to run.
 constexpr int PLANE_CHUNKS = 16;
 constexpr int PLANE_THREADS = 256;
-__device__ __forceinline__ double wave_sum(double v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
 __global__ void __launch_bounds__(PLANE_THREADS) k_plane_partial(const double *LM, const double *MM, const double *QN,
                                                                  const double *NN, double *part, Geo g) {
     const int j = blockIdx.x + 1, chunk = blockIdx.y;
