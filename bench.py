@@ -216,9 +216,11 @@ def mixed_leg(torch, ctx, mesh, case, ucont, ucat, theta, lay, args, fp64_ms, fa
     return out
 
 
-def vjp_leg(torch, ctx, mesh, case, ucont, ucat, theta, cs, lay, args, step_ms):
+def vjp_leg(torch, ctx, mesh, case, ucont, ucat, theta, cs, lay, args, step_ms, mixed_ms=None):
     """Adjoint cost: tosca_rhs_vjp against the forward it differentiates (UpdateNut + FormU +
-    Buoyancy with the step's Cs) and against the whole step (which also runs the dynamic model)."""
+    Buoyancy with the step's Cs) and against the whole step (which also runs the dynamic model);
+    the mixed adjoint (tosca_rhs_vjp_mixed, FP32 flow fields and packets) against the FP64 adjoint
+    and the mixed step, with its difference from the FP64 adjoint at the same FP32-rounded inputs."""
     dev = "cuda"
     gs, gl = lay.gshape, (lay.mz, lay.my, lay.mx, 3)
     nut = torch.zeros(gs, dtype=torch.float64, device=dev)
@@ -256,9 +258,36 @@ def vjp_leg(torch, ctx, mesh, case, ucont, ucat, theta, cs, lay, args, step_ms):
     ctx.enable_timing(False)
     print(f"# vjp: {vjp_ms:.3f} ms vs forward {fwd_ms:.3f} ms (x{vjp_ms / fwd_ms:.2f}), vs step x{vjp_ms / step_ms:.2f}",
           file=sys.stderr, flush=True)
-    return {"vjp_ms": vjp_ms, "forward_ms": fwd_ms, "vjp_over_forward": vjp_ms / fwd_ms,
-            "vjp_over_step": vjp_ms / step_ms, "kernels_ms": {k: v[0] / v[1] for k, v in tim.items()},
-            "forward": "UpdateNut + FormU + Buoyancy (FP64) with the step's Cs"}
+    out = {"vjp_ms": vjp_ms, "forward_ms": fwd_ms, "vjp_over_forward": vjp_ms / fwd_ms,
+           "vjp_over_step": vjp_ms / step_ms, "kernels_ms": {k: v[0] / v[1] for k, v in tim.items()},
+           "forward": "UpdateNut + FormU + Buoyancy (FP64) with the step's Cs"}
+    # mixed adjoint
+    x32 = [t.float().contiguous() for t in (ucont, ucat, theta)]
+    bars32 = [torch.empty_like(t) for t in x32] + [torch.empty_like(cs)]
+
+    def vjp32():
+        ctx.rhs_vjp_mixed(mesh, case.params, *x32, cs, rbar, 1.0, 1.0, *bars32)
+
+    m_ms = timed(vjp32)
+    ctx.enable_timing(True)
+    ctx.reset_timing()
+    for _ in range(args.steps):
+        vjp32()
+    tim32 = ctx.timings()
+    ctx.enable_timing(False)
+    ctx.rhs_vjp(mesh, case.params, *[t.double() for t in x32], cs, rbar, 1.0, 1.0, *bars)
+    vjp32()
+    torch.cuda.synchronize()
+    err = {}
+    for k, a, b in zip(("ucont", "ucat", "theta", "cs"), bars32, bars):
+        err[k] = float(torch.linalg.vector_norm(a.double() - b)) / max(float(torch.linalg.vector_norm(b)), 1e-300)
+    out["mixed"] = {"vjp_ms": m_ms, "speedup_vs_fp64_vjp": vjp_ms / m_ms,
+                    "vjp_over_mixed_step": (m_ms / mixed_ms) if mixed_ms else None,
+                    "rel_l2_vs_fp64_vjp_same_inputs": err, "kernels_ms": {k: v[0] / v[1] for k, v in tim32.items()}}
+    print(f"# vjp mixed: {m_ms:.3f} ms (x{vjp_ms / m_ms:.2f} vs FP64 adjoint"
+          + (f", x{m_ms / mixed_ms:.2f} the mixed step" if mixed_ms else "") + "); rel L2 vs FP64 adjoint "
+          + ", ".join(f"{k} {v:.1e}" for k, v in err.items()), file=sys.stderr, flush=True)
+    return out
 
 
 def dpasd_leg(torch, ctx, mesh, case, ucont, ucat, theta, lay, args, farm=None):
@@ -852,7 +881,7 @@ def main():
     }
     line["mixed"] = mixed_leg(torch, ctx, mesh, case, ucont, ucat, theta, lay, args, ms, farm)
     line["dpasd"] = dpasd_leg(torch, ctx, mesh, case, ucont, ucat, theta, lay, args, farm)
-    line["vjp"] = vjp_leg(torch, ctx, mesh, case, ucont, ucat, theta, cs, lay, args, ms)
+    line["vjp"] = vjp_leg(torch, ctx, mesh, case, ucont, ucat, theta, cs, lay, args, ms, line["mixed"]["ms_per_step"])
     line["explicit_rhs_u"] = explicit_leg(torch, ctx, mesh, case, ucont, ucat, nut, lay, args)
     line["rk3w_adjoint"] = rk3w_leg(torch, ctx, mesh, case, ucont, ucat, nut, farm, lay, args)
     line["formt"] = formt_leg(torch, ctx, mesh, case, ucont, theta, nut, lay, args)

@@ -270,6 +270,18 @@ int tosca_rhs_vjp(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_params *pa
                   const double *rbar, double scale, double buoy_scale,
                   double *ucont_bar, double *ucat_bar, double *theta_bar, double *cs_bar);
 
+/* Mixed-precision form of tosca_rhs_vjp (no reference counterpart): the adjoint of the function
+ * tosca_rhs_les_step_mixed evaluates, for its FP32 flow fields.  ucont, ucat, theta (local, FP32)
+ * in; cs (local), rbar (global) FP64; ucont_bar, ucat_bar, theta_bar (local, FP32) and cs_bar
+ * (local, FP64) overwritten.  The forward intermediates (nu_t, |S|, the face gradients and
+ * interpolated velocities) are recomputed in FP32 as the mixed step computes them; the cotangent
+ * arithmetic is FP64; the cotangent packets between the kernels are stored in FP32, which halves
+ * their HBM traffic.  Same function, error codes and conventions as tosca_rhs_vjp. */
+int tosca_rhs_vjp_mixed(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_params *params,
+                        const float *ucont, const float *ucat, const float *theta, const double *cs,
+                        const double *rbar, double scale, double buoy_scale,
+                        float *ucont_bar, float *ucat_bar, float *theta_bar, double *cs_bar);
+
 /* ExplicitRhsU (src/solvers/ueqn_solvers.h:24; body src/solvers/ueqn_solvers.c:1276-1407), the
  * immediate caller of FormU, with the
  * terms this library covers, in the reference's order:

@@ -142,6 +142,7 @@ SIGNATURES = {
     "tosca_rhs_les_step": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P, P, D, P, P, P, D]),
     "tosca_rhs_les_step_mixed": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P, P, D, P, P, P, D]),
     "tosca_rhs_vjp": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P, P, P, D, D, P, P, P, P]),
+    "tosca_rhs_vjp_mixed": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P, P, P, D, D, P, P, P, P]),
     "tosca_rhs_les_step_mixed_m32": (I, [P, C.POINTER(Mesh), C.POINTER(MeshF32), C.POINTER(Params), P, P, P, P, D, P, P,
                                          P, D]),
     "tosca_explicit_rhs_u": (I, [P, C.POINTER(Mesh), C.POINTER(Params), P, P, P, P, P, P, I, P, P, P, I]),

@@ -1638,6 +1638,34 @@ int tosca_rhs_vjp(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, cons
     return launched(c, "rhs_vjp");
 }
 
+int tosca_rhs_vjp_mixed(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const float *ucont,
+                        const float *ucat, const float *theta, const double *cs, const double *rbar, double scale,
+                        double buoy_scale, float *ucont_bar, float *ucat_bar, float *theta_bar, double *cs_bar) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if ((rc = check_params(c, p))) return rc;
+    if (p->les_active && p->les_model != TOSCA_LES_SMAGORINSKY && p->les_model != TOSCA_LES_DSM &&
+        p->les_model != TOSCA_LES_DPASD)
+        return set_err(c, TOSCA_ERR_UNSUPPORTED, "LES model %d: nu_t = Cs Delta^2 |S| models only", p->les_model);
+    if (p->div_scheme != TOSCA_DIV_CENTRAL && p->div_scheme != TOSCA_DIV_QUICK)
+        return set_err(c, TOSCA_ERR_UNSUPPORTED, "div_scheme = %d: only central (0) and QUICK (1) are supported", p->div_scheme);
+    if (!(p->t_ref != 0.0)) return set_err(c, TOSCA_ERR_ARG, "t_ref must be non-zero");
+    set_geo_per(c, p);
+    const Geo &g = c->geo;
+    const size_t n = (size_t)g.nloc(), v = 3 * n * sizeof(float), ng = 3 * (size_t)g.mx * g.my * g.mz * sizeof(double);
+    if ((rc = check_mesh(c, m, true, true, true))) return rc;
+    if ((rc = check_dev(c, ucont, v, "ucont")) || (rc = check_dev(c, ucat, v, "ucat")) ||
+        (rc = check_dev(c, theta, n * sizeof(float), "theta")) || (rc = check_dev(c, cs, n * sizeof(double), "cs")) ||
+        (rc = check_dev(c, rbar, ng, "rbar")) || (rc = check_dev(c, ucont_bar, v, "ucont_bar")) ||
+        (rc = check_dev(c, ucat_bar, v, "ucat_bar")) || (rc = check_dev(c, theta_bar, n * sizeof(float), "theta_bar")) ||
+        (rc = check_dev(c, cs_bar, n * sizeof(double), "cs_bar")))
+        return rc;
+    if ((rc = ensure_ws(c, std::max(ws_need(c), 52 * n + 64)))) return rc;
+    tosca::vjp_rhs_mixed(g, m, p, ucont, ucat, cs, rbar, scale, buoy_scale, ucont_bar, ucat_bar, theta_bar, cs_bar,
+                         c->ws, c->stream);
+    return launched(c, "rhs_vjp_mixed");
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------------------------------------

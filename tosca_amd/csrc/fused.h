@@ -79,6 +79,13 @@ void vjp_rhs(const Geo &g, const tosca_mesh *mesh, const tosca_params *p, const 
              const double *theta, const double *cs, const double *rbar, double scale, double buoy_scale,
              double *ucont_bar, double *ucat_bar, double *theta_bar, double *cs_bar, double *ws, hipStream_t s);
 
+// The same adjoint with FP32 flow fields and FP32 cotangent packets (the mixed step's function):
+// the forward intermediates in FP32 as the mixed step computes them, the cotangent arithmetic in
+// FP64, the packets and intermediates stored in FP32, ucont_bar / ucat_bar / theta_bar FP32.
+void vjp_rhs_mixed(const Geo &g, const tosca_mesh *mesh, const tosca_params *p, const float *ucont, const float *ucat,
+                   const double *cs, const double *rbar, double scale, double buoy_scale, float *ucont_bar,
+                   float *ucat_bar, float *theta_bar, double *cs_bar, double *ws, hipStream_t s);
+
 // Adjoint of FormU(ucont, ucat, nu_t; scale, formMode 0) with nu_t an input: ucont_bar and ucat_bar
 // overwritten, the nu_t cotangent added to nut_bar (LES active); ws holds >= 52 * nloc doubles.
 void vjp_formu_nut(const Geo &g, const tosca_mesh *mesh, const tosca_params *p, const double *ucont, const double *ucat,

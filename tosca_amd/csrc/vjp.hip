@@ -28,7 +28,9 @@ namespace tosca {
 namespace {
 
 __device__ __forceinline__ void aadd(double *p, double v) { unsafeAtomicAdd(p, v); }
-__device__ __forceinline__ void aadd3(double *p, int c, V3 v) {
+__device__ __forceinline__ void aadd(float *p, double v) { unsafeAtomicAdd(p, float(v)); }
+template <class T>
+__device__ __forceinline__ void aadd3(T *p, int c, V3 v) {
     aadd(p + 3 * c, v.x);
     aadd(p + 3 * c + 1, v.y);
     aadd(p + 3 * c + 2, v.z);
@@ -38,17 +40,20 @@ __device__ __forceinline__ V3 add3v(V3 a, V3 b) { return V3{a.x + b.x, a.y + b.y
 __device__ __forceinline__ double dot3(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 
 // w * (p0 + p1 - m0 - m1): adjoint of the four-point differences of Compute_du_*
-__device__ __forceinline__ void scat4(double *ub, int p0, int p1, int m0, int m1, double w, V3 gb) {
+template <class T>
+__device__ __forceinline__ void scat4(T *ub, int p0, int p1, int m0, int m1, double w, V3 gb) {
     const V3 a = sc3(w, gb), b = sc3(-w, gb);
     aadd3(ub, p0, a); aadd3(ub, p1, a); aadd3(ub, m0, b); aadd3(ub, m1, b);
 }
-__device__ __forceinline__ void scat2(double *ub, int p, int m, double w, V3 gb) {
+template <class T>
+__device__ __forceinline__ void scat2(T *ub, int p, int m, double w, V3 gb) {
     aadd3(ub, p, sc3(w, gb)); aadd3(ub, m, sc3(-w, gb));
 }
 
 // Cotangents of the nine curvilinear derivatives, grouped by derivative direction: gc = (dudc,
 // dvdc, dwdc), ge = (dude, ...), gz = (dudz, ...).  Adjoint of Compute_du_i (same branches).
-__device__ void du_i_adj(const Geo &g, double *ub, int i, int j, int k, V3 gc, V3 ge, V3 gz) {
+template <class T>
+__device__ void du_i_adj(const Geo &g, T *ub, int i, int j, int k, V3 gc, V3 ge, V3 gz) {
     const bool iN = g.nonper(0), jN = g.nonper(1), kN = g.nonper(2);
     scat2(ub, g.L(k, j, i + 1), g.L(k, j, i), 1.0, gc);
     const bool ib = (i == 0 || i == g.mx - 2);
@@ -61,7 +66,8 @@ __device__ void du_i_adj(const Geo &g, double *ub, int i, int j, int k, V3 gc, V
 }
 
 // adjoint of Compute_du_j
-__device__ void du_j_adj(const Geo &g, double *ub, int i, int j, int k, V3 gc, V3 ge, V3 gz) {
+template <class T>
+__device__ void du_j_adj(const Geo &g, T *ub, int i, int j, int k, V3 gc, V3 ge, V3 gz) {
     const bool iN = g.nonper(0), jN = g.nonper(1), kN = g.nonper(2);
     const bool jb = (j == 0 || j == g.my - 2);
     if (i == g.mx - 2 && jb && (iN || jN)) scat4(ub, g.L(k, j + 1, i), g.L(k, j, i), g.L(k, j + 1, i - 1), g.L(k, j, i - 1), 0.5, gc);
@@ -74,7 +80,8 @@ __device__ void du_j_adj(const Geo &g, double *ub, int i, int j, int k, V3 gc, V
 }
 
 // adjoint of Compute_du_k
-__device__ void du_k_adj(const Geo &g, double *ub, int i, int j, int k, V3 gc, V3 ge, V3 gz) {
+template <class T>
+__device__ void du_k_adj(const Geo &g, T *ub, int i, int j, int k, V3 gc, V3 ge, V3 gz) {
     const bool iN = g.nonper(0), jN = g.nonper(1), kN = g.nonper(2);
     const bool kb = (k == 0 || k == g.mz - 2);
     if (i == g.mx - 2 && kb && (iN || kN)) scat4(ub, g.L(k + 1, j, i), g.L(k, j, i), g.L(k + 1, j, i - 1), g.L(k, j, i - 1), 0.5, gc);
@@ -87,7 +94,8 @@ __device__ void du_k_adj(const Geo &g, double *ub, int i, int j, int k, V3 gc, V
 }
 
 // adjoint of Compute_du_center at cell (i, j, k)
-__device__ void du_center_adj(const Geo &g, double *ub, int i, int j, int k, V3 gc, V3 ge, V3 gz) {
+template <class T>
+__device__ void du_center_adj(const Geo &g, T *ub, int i, int j, int k, V3 gc, V3 ge, V3 gz) {
     const int c = g.L(k, j, i);
     if (g.nonper(0) && i == g.mx - 2) scat2(ub, c, g.L(k, j, i - 1), 1.0, gc);
     else if (g.nonper(0) && i == 1) scat2(ub, g.L(k, j, i + 1), c, 1.0, gc);
@@ -102,17 +110,23 @@ __device__ void du_center_adj(const Geo &g, double *ub, int i, int j, int k, V3 
 
 struct NutBCv { int bc[6]; };
 
+// T: type of the flow fields (ucont, ucat, theta, nu_t) and of their cotangents; P: storage type
+// of the cotangent packets and intermediates (fpbar, nu_eff_bar, the face and cell packets).  The
+// cotangent arithmetic is FP64 in both forms.
+template <class T, class P>
 struct VjpArgs {
-    const double *ucont, *ucat, *theta, *cs, *nut, *rbar;
+    const T *ucont, *ucat, *theta, *nut;
+    const double *cs, *rbar;
     const double *csi, *eta, *zet, *icsi, *ieta, *izet, *jcsi, *jeta, *jzet, *kcsi, *keta, *kzet;
     const double *aj, *iaj, *jaj, *kaj;
     const double *tauI, *tauJ;
-    double *fpbar, *nutbar;
-    double *vbar, *ubar, *tbar, *csbar;
+    P *fpbar;
+    T *vbar, *ubar, *tbar;
+    double *csbar;
     double *nutbar_acc; // nu_t given as an input: its cotangent is accumulated here (local)
-    double *pk[3];      // per face direction: 12 SoA planes of n doubles (cv, gn, ga, gb)
-    double *nueb[3];    // per face direction: cotangent of nu_eff at each face (n doubles)
-    double *spk;        // per interior cell: 9 SoA planes (cotangents of the du_center derivatives)
+    P *pk[3];           // per face direction: 12 SoA planes of n values (cv, gn, ga, gb)
+    P *nueb[3];         // per face direction: cotangent of nu_eff at each face (n values)
+    P *spk;             // per interior cell: 9 SoA planes (cotangents of the du_center derivatives)
     long n;
     double scale, bscale, tRef, nu;
     int quick, les;
@@ -120,17 +134,34 @@ struct VjpArgs {
     int bc_u[6], bc_nut[6];
 };
 
-__device__ __forceinline__ V3 cellS(const VjpArgs &A, int q, int c) {
+template <class A_>
+__device__ __forceinline__ V3 cellS(const A_ &A, int q, int c) {
     return ld3(q == 0 ? A.csi : (q == 1 ? A.eta : A.zet), c);
 }
-__device__ __forceinline__ double faceA(const VjpArgs &A, int q, int c) {
+template <class A_>
+__device__ __forceinline__ double faceA(const A_ &A, int q, int c) {
     return (q == 0 ? A.iaj : (q == 1 ? A.jaj : A.kaj))[c];
+}
+template <class T>
+__device__ __forceinline__ Grad grad64(const GradT<T> &a) {
+    return Grad{double(a.dudc), double(a.dvdc), double(a.dwdc), double(a.dude), double(a.dvde), double(a.dwde),
+                double(a.dudz), double(a.dvdz), double(a.dwdz)};
+}
+template <class P>
+__device__ __forceinline__ void stp3(P *p, int c, V3 v) { p[3 * c] = P(v.x); p[3 * c + 1] = P(v.y); p[3 * c + 2] = P(v.z); }
+template <class P>
+__device__ __forceinline__ V3 ldp3(const P *p, int c) { return V3{double(p[3 * c]), double(p[3 * c + 1]), double(p[3 * c + 2])}; }
+
+template <class T>
+__global__ void k_vjp_zero(T *a, long n) {
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) a[t] = T(0);
 }
 
 }  // namespace
 
 // fpbar at every interior cell (gather; no atomics)
-__global__ void __launch_bounds__(256) k_vjp_fp(VjpArgs A, Geo g) {
+template <class T, class P>
+__global__ void __launch_bounds__(256) k_vjp_fp(VjpArgs<T, P> A, Geo g) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y * blockDim.y + threadIdx.y + 1,
               k = blockIdx.z + 1;
     if (i > g.mx - 2 || j > g.my - 2 || k > g.mz - 2) return;
@@ -167,7 +198,7 @@ __global__ void __launch_bounds__(256) k_vjp_fp(VjpArgs A, Geo g) {
             fb = add3v(fb, sc3(w, Sg));
         }
     }
-    st3(A.fpbar, c, fb);
+    stp3(A.fpbar, c, fb);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -178,8 +209,8 @@ __global__ void __launch_bounds__(256) k_vjp_fp(VjpArgs A, Geo g) {
 // (the only ones with the one-sided and clipped stencils) scatter into ucat_bar directly.
 // Thread x runs along i for every face direction (coalesced).
 // ------------------------------------------------------------------------------------------------
-template <int DIR>
-__global__ void __launch_bounds__(256) k_vjp_face(VjpArgs A, Geo g) {
+template <int DIR, class T, class P>
+__global__ void __launch_bounds__(256) k_vjp_face(VjpArgs<T, P> A, Geo g) {
     const int m[3] = {g.mx, g.my, g.mz};
     int x[3];
     x[0] = blockIdx.x * blockDim.x + threadIdx.x;
@@ -199,16 +230,17 @@ __global__ void __launch_bounds__(256) k_vjp_face(VjpArgs A, Geo g) {
     const bool interior = f >= 1 && f <= m[DIR] - 3;
     // cotangent of the face flux: + fp of the left cell, - fp of the right cell (interior cells)
     V3 Fb{0, 0, 0};
-    if (f >= 1) Fb = add3v(Fb, ld3(A.fpbar, c));
-    if (f + 1 <= m[DIR] - 2) Fb = add3v(Fb, sc3(-1.0, ld3(A.fpbar, up)));
+    if (f >= 1) Fb = add3v(Fb, ldp3(A.fpbar, c));
+    if (f + 1 <= m[DIR] - 2) Fb = add3v(Fb, sc3(-1.0, ldp3(A.fpbar, up)));
     if (per && f == m[DIR] - 2) {                  // face 0 aliases this face: it is cell 1's left face
         int t[3] = {i, j, k};
         t[DIR] = 1;
-        Fb = add3v(Fb, sc3(-1.0, ld3(A.fpbar, g.L(t[2], t[1], t[0]))));
+        Fb = add3v(Fb, sc3(-1.0, ldp3(A.fpbar, g.L(t[2], t[1], t[0]))));
     }
-    const double *u = A.ucat;
+    const T *u = A.ucat;
     // ---- convective: div = -V * u_face ---------------------------------------------------------
-    const double V = A.ucont[3 * c + DIR];
+    const T VT = A.ucont[3 * c + DIR];
+    const double V = double(VT);
     const V3 cv = sc3(-V, Fb);
     if (A.quick) {
         int L, R;
@@ -216,17 +248,17 @@ __global__ void __launch_bounds__(256) k_vjp_face(VjpArgs A, Geo g) {
         int xl[3] = {i, j, k}, xr[3] = {i, j, k};
         xl[DIR] = L; xr[DIR] = R;
         const int cLL = g.L(xl[2], xl[1], xl[0]), cRR = g.L(xr[2], xr[1], xr[0]);
-        const V3 uf = quick(ld3(u, cLL), ld3(u, c), ld3(u, up), ld3(u, cRR), V);
-        A.vbar[3 * c + DIR] = -dot3(Fb, uf);
+        const V3 uf = cvt3<double>(quick(ld3(u, cLL), ld3(u, c), ld3(u, up), ld3(u, cRR), VT));
+        A.vbar[3 * c + DIR] = T(-dot3(Fb, uf));
         if (!interior) {
             // d/dfU = 1/2 + 2/8, d/dfD = 1/2 - 1/8, d/dfUU = -1/8
             if (V > 0) { aadd3(A.ubar, c, sc3(0.75, cv)); aadd3(A.ubar, up, sc3(0.375, cv)); aadd3(A.ubar, cLL, sc3(-0.125, cv)); }
             else { aadd3(A.ubar, up, sc3(0.75, cv)); aadd3(A.ubar, c, sc3(0.375, cv)); aadd3(A.ubar, cRR, sc3(-0.125, cv)); }
         }
     } else {
-        const V3 a0 = ld3(u, c), a1 = ld3(u, up);
-        const V3 uf{0.5 * (a0.x + a1.x), 0.5 * (a0.y + a1.y), 0.5 * (a0.z + a1.z)};
-        A.vbar[3 * c + DIR] = -dot3(Fb, uf);
+        const Vec3<T> a0 = ld3(u, c), a1 = ld3(u, up);
+        const V3 uf{double(T(0.5) * (a0.x + a1.x)), double(T(0.5) * (a0.y + a1.y)), double(T(0.5) * (a0.z + a1.z))};
+        A.vbar[3 * c + DIR] = T(-dot3(Fb, uf));
         if (!interior) { aadd3(A.ubar, c, sc3(0.5, cv)); aadd3(A.ubar, up, sc3(0.5, cv)); }
     }
     // ---- viscous ------------------------------------------------------------------------------------
@@ -239,7 +271,7 @@ __global__ void __launch_bounds__(256) k_vjp_face(VjpArgs A, Geo g) {
         } else if ((bl == TOSCA_BCU_SLIP && f == 0) || (br == TOSCA_BCU_SLIP && f == m[DIR] - 2)) {
             nuEff = 0.0;
         } else {
-            nuEff = A.nu + 0.5 * (A.nut[c] + A.nut[up]);
+            nuEff = A.nu + 0.5 * (double(A.nut[c]) + double(A.nut[up]));
             dep = true;
         }
     }
@@ -251,10 +283,10 @@ __global__ void __launch_bounds__(256) k_vjp_face(VjpArgs A, Geo g) {
     const double ajc = fap[c];
     double nb = 0.0;
     if (dep) {
-        const Grad gr = (DIR == 0) ? du_i(g, u, i, j, k) : (DIR == 1) ? du_j(g, u, i, j, k) : du_k(g, u, i, j, k);
+        const Grad gr = grad64((DIR == 0) ? du_i(g, u, i, j, k) : (DIR == 1) ? du_j(g, u, i, j, k) : du_k(g, u, i, j, k));
         nb = dot3(Fb, visc_flux(gr, fc, fe, fz, DIR, ajc, 1.0));
     }
-    A.nueb[DIR][c] = nb;
+    A.nueb[DIR][c] = P(nb);
     // visc_m = a nu (g_X1 d_X u_m + (M_X . d_X u) n_m) summed over X = csi, eta, zet
     const V3 nrm = (DIR == 0) ? fc : (DIR == 1) ? fe : fz;
     const double g11 = dot3(fc, nrm), g21 = dot3(fe, nrm), g31 = dot3(fz, nrm);
@@ -270,11 +302,11 @@ __global__ void __launch_bounds__(256) k_vjp_face(VjpArgs A, Geo g) {
         const V3 ga = (DIR == 0) ? ge : gc;
         const V3 gb = (DIR == 2) ? ge : gz;
         const V3 pv[4] = {cv, gn, ga, gb};
-        double *pk = A.pk[DIR];
+        P *pk = A.pk[DIR];
         for (int w = 0; w < 4; w++) {
-            pk[(3 * w + 0) * A.n + c] = pv[w].x;
-            pk[(3 * w + 1) * A.n + c] = pv[w].y;
-            pk[(3 * w + 2) * A.n + c] = pv[w].z;
+            pk[(3 * w + 0) * A.n + c] = P(pv[w].x);
+            pk[(3 * w + 1) * A.n + c] = P(pv[w].y);
+            pk[(3 * w + 2) * A.n + c] = P(pv[w].z);
         }
     } else if (nuEff != 0.0) {
         if (DIR == 0) du_i_adj(g, A.ubar, i, j, k, gc, ge, gz);
@@ -287,7 +319,8 @@ __global__ void __launch_bounds__(256) k_vjp_face(VjpArgs A, Geo g) {
 // Interior cells: nu_t_bar (gathered from the nu_eff cotangents of the cell's six faces, plus the
 // ghosts UpdateNutBCs fills from this cell), Cs_bar, and the packet of the |S| path.
 // ------------------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_vjp_nut(VjpArgs A, Geo g) {
+template <class T, class P>
+__global__ void __launch_bounds__(256) k_vjp_nut(VjpArgs<T, P> A, Geo g) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y * blockDim.y + threadIdx.y + 1,
               k = blockIdx.z + 1;
     if (i > g.mx - 2 || j > g.my - 2 || k > g.mz - 2) return;
@@ -296,10 +329,10 @@ __global__ void __launch_bounds__(256) k_vjp_nut(VjpArgs A, Geo g) {
     double nb = 0.0;
     for (int q = 0; q < 3; q++) {
         int t[3] = {i, j, k};
-        nb += 0.5 * A.nueb[q][c];                                      // face q of the cell
+        nb += 0.5 * double(A.nueb[q][c]);                              // face q of the cell
         t[q] = x[q] - 1;
         if (!(x[q] == 1 && g.per[q] != TOSCA_PERIODIC_NONE))          // periodic face 0 is not evaluated
-            nb += 0.5 * A.nueb[q][g.L(t[2], t[1], t[0])];
+            nb += 0.5 * double(A.nueb[q][g.L(t[2], t[1], t[0])]);
         // ghosts filled from this cell: cotangent of nu_t there is 1/2 nu_eff_bar of the boundary face
         const int bl = A.bc_nut[2 * q], br = A.bc_nut[2 * q + 1];
         const bool pq = g.per[q] != TOSCA_PERIODIC_NONE;
@@ -307,15 +340,18 @@ __global__ void __launch_bounds__(256) k_vjp_nut(VjpArgs A, Geo g) {
         const int f0 = g.L(t[2], t[1], t[0]);
         t[q] = m[q] - 2;                                               // face m-2 (right cell ghost m-1)
         const int fm = g.L(t[2], t[1], t[0]);
-        if (bl == TOSCA_BCNUT_ZEROGRADIENT && x[q] == 1) nb += 0.5 * A.nueb[q][f0];
-        if (bl == TOSCA_BCNUT_PERIODIC && pq && x[q] == m[q] - 2) nb += 0.5 * A.nueb[q][f0];
-        if (br == TOSCA_BCNUT_ZEROGRADIENT && x[q] == m[q] - 2) nb += 0.5 * A.nueb[q][fm];
-        if (br == TOSCA_BCNUT_PERIODIC && pq && x[q] == 1) nb += 0.5 * A.nueb[q][fm];
+        if (bl == TOSCA_BCNUT_ZEROGRADIENT && x[q] == 1) nb += 0.5 * double(A.nueb[q][f0]);
+        if (bl == TOSCA_BCNUT_PERIODIC && pq && x[q] == m[q] - 2) nb += 0.5 * double(A.nueb[q][f0]);
+        if (br == TOSCA_BCNUT_ZEROGRADIENT && x[q] == m[q] - 2) nb += 0.5 * double(A.nueb[q][fm]);
+        if (br == TOSCA_BCNUT_PERIODIC && pq && x[q] == 1) nb += 0.5 * double(A.nueb[q][fm]);
     }
     const V3 cc = ld3(A.csi, c), ee = ld3(A.eta, c), zz = ld3(A.zet, c);
     const double a = A.aj[c];
-    const Cart d = du_dxyz(cc, ee, zz, a, du_center(g, A.ucat, i, j, k));
-    const double S = strain_mag(d);
+    // the strain as the forward computes it (in T), the rest in FP64
+    const CartT<T> dT = du_dxyz(cvt3<T>(cc), cvt3<T>(ee), cvt3<T>(zz), T(a), du_center(g, A.ucat, i, j, k));
+    const Cart d{double(dT.du_dx), double(dT.du_dy), double(dT.du_dz), double(dT.dv_dx), double(dT.dv_dy),
+                 double(dT.dv_dz), double(dT.dw_dx), double(dT.dw_dy), double(dT.dw_dz)};
+    const double S = double(strain_mag(dT));
     const double filter = cube_root(1. / a), d2 = filter * filter;
     A.csbar[c] = nb * d2 * S;
     V3 gc{0, 0, 0}, ge{0, 0, 0}, gz{0, 0, 0};
@@ -332,15 +368,16 @@ __global__ void __launch_bounds__(256) k_vjp_nut(VjpArgs A, Geo g) {
     }
     const V3 pv[3] = {gc, ge, gz};
     for (int w = 0; w < 3; w++) {
-        A.spk[(3 * w + 0) * A.n + c] = pv[w].x;
-        A.spk[(3 * w + 1) * A.n + c] = pv[w].y;
-        A.spk[(3 * w + 2) * A.n + c] = pv[w].z;
+        A.spk[(3 * w + 0) * A.n + c] = P(pv[w].x);
+        A.spk[(3 * w + 1) * A.n + c] = P(pv[w].y);
+        A.spk[(3 * w + 2) * A.n + c] = P(pv[w].z);
     }
 }
 
 namespace {
-__device__ __forceinline__ V3 pk3(const double *pk, long n, int w, int c) {
-    return V3{pk[(3 * w + 0) * n + c], pk[(3 * w + 1) * n + c], pk[(3 * w + 2) * n + c]};
+template <class P>
+__device__ __forceinline__ V3 pk3(const P *pk, long n, int w, int c) {
+    return V3{double(pk[(3 * w + 0) * n + c]), double(pk[(3 * w + 1) * n + c]), double(pk[(3 * w + 2) * n + c])};
 }
 }  // namespace
 
@@ -348,17 +385,18 @@ __device__ __forceinline__ V3 pk3(const double *pk, long n, int w, int c) {
 // ucat_bar and theta_bar of every owned point (boundary ghosts included): the regular stencils of
 // the interior faces and interior cells, transposed.  Adds to the boundary-face scatter.
 // ------------------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_vjp_gather(VjpArgs A, Geo g) {
+template <class T, class P>
+__global__ void __launch_bounds__(256) k_vjp_gather(VjpArgs<T, P> A, Geo g) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
     if (i > g.mx - 1) return;
     const int x[3] = {i, j, k}, m[3] = {g.mx, g.my, g.mz};
     const int c = g.L(k, j, i);
     auto Lof = [&](const int *t) { return g.L(t[2], t[1], t[0]); };
-    V3 acc = ld3(A.ubar, c);
+    V3 acc = cvt3<double>(ld3(A.ubar, c));
     double tb = 0.0;
     for (int q = 0; q < 3; q++) {
         const int a = (q == 0) ? 1 : 0, b = (q == 2) ? 1 : 2;          // transverse axes, lower first
-        const double *pk = A.pk[q];
+        const P *pk = A.pk[q];
         // an interior face of direction q at index point t
         auto faceOK = [&](const int *t) {
             return t[q] >= 1 && t[q] <= m[q] - 3 && t[a] >= 1 && t[a] <= m[a] - 2 && t[b] >= 1 && t[b] <= m[b] - 2;
@@ -379,7 +417,7 @@ __global__ void __launch_bounds__(256) k_vjp_gather(VjpArgs A, Geo g) {
             double w;
             if (!A.quick) w = (s == 0 || s == -1) ? 0.5 : 0.0;
             else {
-                const bool pos = A.ucont[3 * fc + q] > 0;
+                const bool pos = A.ucont[3 * fc + q] > T(0);
                 if (s == 0) w = pos ? 0.75 : 0.375;
                 else if (s == -1) w = pos ? 0.375 : 0.75;
                 else if (s == 1) w = pos ? -0.125 : 0.0;
@@ -430,8 +468,8 @@ __global__ void __launch_bounds__(256) k_vjp_gather(VjpArgs A, Geo g) {
             tb += -A.bscale * (-9.81 * face[3 * cb + 2]) * 0.5 / A.tRef * A.rbar[3 * g.G(t[2], t[1], t[0]) + q];
         }
     }
-    st3(A.ubar, c, acc);
-    if (A.tbar) A.tbar[c] = tb;
+    stp3(A.ubar, c, acc);
+    if (A.tbar) A.tbar[c] = T(tb);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -439,37 +477,42 @@ __global__ void __launch_bounds__(256) k_vjp_gather(VjpArgs A, Geo g) {
 // direction q at index point c averages nu_t of cells c and c + e_q, so point c collects half the
 // nu_eff cotangent of faces c and c - e_q (zero where a face is not evaluated).
 // ------------------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_vjp_nutbar(VjpArgs A, Geo g) {
+template <class T, class P>
+__global__ void __launch_bounds__(256) k_vjp_nutbar(VjpArgs<T, P> A, Geo g) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
     if (i > g.mx - 1) return;
     const int c = g.L(k, j, i);
-    const double s = A.nueb[0][c] + A.nueb[0][g.L(k, j, i - 1)] + A.nueb[1][c] + A.nueb[1][g.L(k, j - 1, i)] +
-                     A.nueb[2][c] + A.nueb[2][g.L(k - 1, j, i)];
+    const double s = double(A.nueb[0][c]) + double(A.nueb[0][g.L(k, j, i - 1)]) + double(A.nueb[1][c]) +
+                     double(A.nueb[1][g.L(k, j - 1, i)]) + double(A.nueb[2][c]) + double(A.nueb[2][g.L(k - 1, j, i)]);
     A.nutbar_acc[c] += 0.5 * s;
 }
 
 // The transposed stages on the stream.  cs != nullptr: nu_t = UpdateNut(ucat, cs) is recomputed
 // and differentiated (Cs_bar, |S| path); nut_in != nullptr: nu_t is an input and its cotangent is
-// added to nut_bar.  theta_bar == nullptr: no Buoyancy term.
-void vjp_rhs_impl(const Geo &g, const tosca_mesh *mesh, const tosca_params *p, const double *ucont,
-                  const double *ucat, const double *theta, const double *cs, const double *nut_in, const double *rbar,
-                  double scale, double buoy_scale, double *ucont_bar, double *ucat_bar, double *theta_bar,
-                  double *cs_bar, double *nut_bar, double *ws, hipStream_t s) {
+// added to nut_bar.  theta_bar == nullptr: no Buoyancy term.  ws: n values of T for nu_t, then
+// 51 n values of P for the intermediates (52 n doubles cover both forms).
+template <class T, class P>
+void vjp_rhs_impl(const Geo &g, const tosca_mesh *mesh, const tosca_params *p, const T *ucont, const T *ucat,
+                  const double *cs, const T *nut_in, const double *rbar, double scale, double buoy_scale, T *ucont_bar,
+                  T *ucat_bar, T *theta_bar, double *cs_bar, double *nut_bar, double *ws, hipStream_t s) {
     const long n = g.nloc();
-    double *nut = ws, *fpbar = ws + n, *nueb = ws + 4 * n, *spk = ws + 7 * n, *pk = ws + 16 * n;   // 52 n in all
-    launch_fill(ucont_bar, 3 * n, 0.0, s);
-    launch_fill(ucat_bar, 3 * n, 0.0, s);
-    if (theta_bar) launch_fill(theta_bar, n, 0.0, s);
+    T *nut = reinterpret_cast<T *>(ws);
+    P *base = reinterpret_cast<P *>(ws + n);
+    P *fpbar = base, *nueb = base + 3 * n, *spk = base + 6 * n, *pk = base + 15 * n;   // 51 n values of P
+    const dim3 zb(256), zg(2048);
+    TOSCA_LAUNCH("k_vjp_zero", k_vjp_zero<T>, zg, zb, 0, s, ucont_bar, 3 * n);
+    TOSCA_LAUNCH("k_vjp_zero", k_vjp_zero<T>, zg, zb, 0, s, ucat_bar, 3 * n);
+    if (theta_bar) TOSCA_LAUNCH("k_vjp_zero", k_vjp_zero<T>, zg, zb, 0, s, theta_bar, n);
     if (cs_bar) launch_fill(cs_bar, n, 0.0, s);
-    launch_fill(nueb, 3 * n, 0.0, s);
+    TOSCA_LAUNCH("k_vjp_zero", k_vjp_zero<P>, zg, zb, 0, s, nueb, 3 * n);
     const bool les = p->les_active != 0;
     if (les && !nut_in) {   // recompute nu_t as the forward step does
-        fused_nut<double>(g, mesh, ucat, cs, nut, nullptr, s);
+        fused_nut<T>(g, mesh, ucat, cs, nut, nullptr, s);
         zero_outside_interior(nut, g, s);
         nut_bcs(nut, p, g, s);
     }
-    VjpArgs A;
-    A.ucont = ucont; A.ucat = ucat; A.theta = theta; A.cs = cs; A.rbar = rbar;
+    VjpArgs<T, P> A;
+    A.ucont = ucont; A.ucat = ucat; A.theta = nullptr; A.cs = cs; A.rbar = rbar;
     A.nut = les ? (nut_in ? nut_in : nut) : ucat;
     A.csi = mesh->csi; A.eta = mesh->eta; A.zet = mesh->zet;
     A.icsi = mesh->icsi; A.ieta = mesh->ieta; A.izet = mesh->izet;
@@ -477,7 +520,7 @@ void vjp_rhs_impl(const Geo &g, const tosca_mesh *mesh, const tosca_params *p, c
     A.kcsi = mesh->kcsi; A.keta = mesh->keta; A.kzet = mesh->kzet;
     A.aj = mesh->aj; A.iaj = mesh->iaj; A.jaj = mesh->jaj; A.kaj = mesh->kaj;
     A.tauI = p->tau_wall_i; A.tauJ = p->tau_wall_j;
-    A.fpbar = fpbar; A.nutbar = nullptr; A.nutbar_acc = nut_bar;
+    A.fpbar = fpbar; A.nutbar_acc = nut_bar;
     A.vbar = ucont_bar; A.ubar = ucat_bar; A.tbar = theta_bar; A.csbar = cs_bar;
     for (int q = 0; q < 3; q++) { A.pk[q] = pk + 12 * q * n; A.nueb[q] = nueb + q * n; }
     A.spk = spk;
@@ -487,22 +530,35 @@ void vjp_rhs_impl(const Geo &g, const tosca_mesh *mesh, const tosca_params *p, c
     A.les = les;
     A.nut_in = nut_in != nullptr;
     for (int q = 0; q < 6; q++) { A.bc_u[q] = p->bc_u[q]; A.bc_nut[q] = p->bc_nut[q]; }
+    const bool f32 = sizeof(T) == 4;
     const dim3 cb(64, 4), cg((g.mx - 2 + 63) / 64, (g.my - 2 + 3) / 4, g.mz - 2);
-    TOSCA_LAUNCH("k_vjp_fp", k_vjp_fp, cg, cb, 0, s, A, g);
+    TOSCA_LAUNCH(f32 ? "k_vjp_fp_f32" : "k_vjp_fp", (k_vjp_fp<T, P>), cg, cb, 0, s, A, g);
     // faces: x along i (0 .. mx-2 for i faces, 1 .. mx-2 otherwise), y / z the other two axes
-    TOSCA_LAUNCH("k_vjp_face_i", k_vjp_face<0>, dim3((g.mx - 1 + 255) / 256, g.my - 2, g.mz - 2), dim3(256), 0, s, A, g);
-    TOSCA_LAUNCH("k_vjp_face_j", k_vjp_face<1>, dim3((g.mx - 2 + 255) / 256, g.my - 1, g.mz - 2), dim3(256), 0, s, A, g);
-    TOSCA_LAUNCH("k_vjp_face_k", k_vjp_face<2>, dim3((g.mx - 2 + 255) / 256, g.my - 2, g.mz - 1), dim3(256), 0, s, A, g);
-    if (les && !nut_in) TOSCA_LAUNCH("k_vjp_nut", k_vjp_nut, cg, cb, 0, s, A, g);
-    if (les && nut_in) TOSCA_LAUNCH("k_vjp_nutbar", k_vjp_nutbar, dim3((g.mx + 255) / 256, g.my, g.mz), dim3(256), 0, s, A, g);
-    TOSCA_LAUNCH("k_vjp_gather", k_vjp_gather, dim3((g.mx + 255) / 256, g.my, g.mz), dim3(256), 0, s, A, g);
+    TOSCA_LAUNCH(f32 ? "k_vjp_face_i_f32" : "k_vjp_face_i", (k_vjp_face<0, T, P>), dim3((g.mx - 1 + 255) / 256, g.my - 2, g.mz - 2),
+                 dim3(256), 0, s, A, g);
+    TOSCA_LAUNCH(f32 ? "k_vjp_face_j_f32" : "k_vjp_face_j", (k_vjp_face<1, T, P>), dim3((g.mx - 2 + 255) / 256, g.my - 1, g.mz - 2),
+                 dim3(256), 0, s, A, g);
+    TOSCA_LAUNCH(f32 ? "k_vjp_face_k_f32" : "k_vjp_face_k", (k_vjp_face<2, T, P>), dim3((g.mx - 2 + 255) / 256, g.my - 2, g.mz - 1),
+                 dim3(256), 0, s, A, g);
+    if (les && !nut_in) TOSCA_LAUNCH(f32 ? "k_vjp_nut_f32" : "k_vjp_nut", (k_vjp_nut<T, P>), cg, cb, 0, s, A, g);
+    if (les && nut_in)
+        TOSCA_LAUNCH("k_vjp_nutbar", (k_vjp_nutbar<T, P>), dim3((g.mx + 255) / 256, g.my, g.mz), dim3(256), 0, s, A, g);
+    TOSCA_LAUNCH(f32 ? "k_vjp_gather_f32" : "k_vjp_gather", (k_vjp_gather<T, P>), dim3((g.mx + 255) / 256, g.my, g.mz),
+                 dim3(256), 0, s, A, g);
 }
 
 void vjp_rhs(const Geo &g, const tosca_mesh *mesh, const tosca_params *p, const double *ucont, const double *ucat,
              const double *theta, const double *cs, const double *rbar, double scale, double buoy_scale,
              double *ucont_bar, double *ucat_bar, double *theta_bar, double *cs_bar, double *ws, hipStream_t s) {
-    vjp_rhs_impl(g, mesh, p, ucont, ucat, theta, cs, nullptr, rbar, scale, buoy_scale, ucont_bar, ucat_bar, theta_bar,
-                 cs_bar, nullptr, ws, s);
+    vjp_rhs_impl<double, double>(g, mesh, p, ucont, ucat, cs, nullptr, rbar, scale, buoy_scale, ucont_bar, ucat_bar,
+                                 theta_bar, cs_bar, nullptr, ws, s);
+}
+
+void vjp_rhs_mixed(const Geo &g, const tosca_mesh *mesh, const tosca_params *p, const float *ucont, const float *ucat,
+                   const double *cs, const double *rbar, double scale, double buoy_scale, float *ucont_bar,
+                   float *ucat_bar, float *theta_bar, double *cs_bar, double *ws, hipStream_t s) {
+    vjp_rhs_impl<float, float>(g, mesh, p, ucont, ucat, cs, nullptr, rbar, scale, buoy_scale, ucont_bar, ucat_bar,
+                               theta_bar, cs_bar, nullptr, ws, s);
 }
 
 // FormU(formMode 0, scale) as a function of (ucont, ucat, nu_t): ucont_bar and ucat_bar
@@ -510,8 +566,8 @@ void vjp_rhs(const Geo &g, const tosca_mesh *mesh, const tosca_params *p, const 
 void vjp_formu_nut(const Geo &g, const tosca_mesh *mesh, const tosca_params *p, const double *ucont, const double *ucat,
                    const double *nut, const double *rbar, double scale, double *ucont_bar, double *ucat_bar,
                    double *nut_bar, double *ws, hipStream_t s) {
-    vjp_rhs_impl(g, mesh, p, ucont, ucat, nullptr, nullptr, p->les_active ? nut : nullptr, rbar, scale, 0.0, ucont_bar,
-                 ucat_bar, nullptr, nullptr, nut_bar, ws, s);
+    vjp_rhs_impl<double, double>(g, mesh, p, ucont, ucat, nullptr, p->les_active ? nut : nullptr, rbar, scale, 0.0,
+                                 ucont_bar, ucat_bar, nullptr, nullptr, nut_bar, ws, s);
 }
 
 }  // namespace tosca

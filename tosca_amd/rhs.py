@@ -151,6 +151,16 @@ class RhsContext:
                                            _ptr(cs), _ptr(rbar), float(scale), float(buoy_scale), _ptr(ucont_bar),
                                            _ptr(ucat_bar), _ptr(theta_bar), _ptr(cs_bar)))
 
+    def rhs_vjp_mixed(self, mesh, params, ucont, ucat, theta, cs, rbar, scale, buoy_scale, ucont_bar, ucat_bar,
+                      theta_bar, cs_bar):
+        """rhs_vjp for the mixed step's FP32 flow fields: ucont, ucat, theta and ucont_bar, ucat_bar,
+        theta_bar FP32; cs, rbar and cs_bar FP64."""
+        self._stream()
+        m, p = self._mesh(mesh), abi.make_params(params)
+        self._check(self.lib.tosca_rhs_vjp_mixed(self.h, C.byref(m), C.byref(p), _p32(ucont), _p32(ucat), _p32(theta),
+                                                 _ptr(cs), _ptr(rbar), float(scale), float(buoy_scale),
+                                                 _p32(ucont_bar), _p32(ucat_bar), _p32(theta_bar), _ptr(cs_bar)))
+
     def ExplicitRhsU(self, mesh, params, ucont, ucat, nut, dp, btheta, btheta_o, ab2, farm, ucont_o, rhs,
                      form_mode: int = 0):
         """ExplicitRhsU(ueqn, formMode) (src/solvers/ueqn_solvers.c:1276-1407): no-penetration and
