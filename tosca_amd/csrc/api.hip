@@ -239,11 +239,16 @@ tosca::FormUArgs formu_args(const tosca_mesh *m, const tosca_params *p, const do
     return A;
 }
 
-size_t ws_need(const tosca_ctx *c) {
-    // 22 local fields (FormU: 6 face-flux vectors + fp; UpdateCs: strain 10 + LM/MM/QN/NN)
-    // plus the DPASD plane partials
-    return std::max({22 * (size_t)c->geo.nloc() + (size_t)c->geo.my * 16 * 4 + 64,
-                     3 * (size_t)c->geo.nloc() + tosca::formu_ws_doubles(c->geo) + 64, tosca::dpasd_ws_doubles(c->geo)});
+size_t ws_need(const tosca_ctx *c, const tosca_params *p) {
+    const size_t n = (size_t)c->geo.nloc();
+    // reference-structured path: FormU's 6 face-flux vectors + fp (21 local fields), UpdateCs's
+    // strain rows, LM/MM/QN/NN and the DPASD plane partials
+    if (c->ref_path) return 22 * n + (size_t)c->geo.my * 16 * 4 + 64;
+    // fused path: LM, MM, |S| then FormU's boundary-face and seam buffers; FormT's two face-flux
+    // vectors; the DPASD filter sums when that model runs
+    size_t need = std::max(3 * n + tosca::formu_ws_doubles(c->geo) + 64, 6 * n + 64);
+    if (p && p->les_active && p->les_model == TOSCA_LES_DPASD) need = std::max(need, tosca::dpasd_ws_doubles(c->geo));
+    return need;
 }
 
 // The fused UpdateCs -> UpdateNut -> Buoyancy -> FormU sequence on the context's stream, for
@@ -326,7 +331,7 @@ int check_mixed(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const 
         if ((rc = check_dev(c, p->tau_wall_i, 3 * (size_t)g.mz * g.my * sizeof(double), "tau_wall_i"))) return rc;
     if (p->bc_u[2] == TOSCA_BCU_WALLFUNCTION || p->bc_u[3] == TOSCA_BCU_WALLFUNCTION)
         if ((rc = check_dev(c, p->tau_wall_j, 3 * (size_t)g.mz * g.mx * sizeof(double), "tau_wall_j"))) return rc;
-    if ((rc = ensure_ws(c, ws_need(c)))) return rc;
+    if ((rc = ensure_ws(c, ws_need(c, p)))) return rc;
     return TOSCA_OK;
 }
 
@@ -387,7 +392,7 @@ int check_explicit(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, dou
         if ((rc = check_dev(c, nut, n * sizeof(double), "nut"))) return rc;
     }
     if (visc && (rc = check_wall_tau(c, p))) return rc;
-    return ensure_ws(c, ws_need(c));
+    return ensure_ws(c, ws_need(c, p));
 }
 
 // ExplicitRhsU's kernels in the reference's order, on validated arguments
@@ -454,7 +459,7 @@ int check_imex(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const d
         if ((rc = check_dev(c, nut, n * sizeof(double), "nut"))) return rc;
     }
     if ((rc = check_wall_tau(c, p))) return rc;
-    return ensure_ws(c, ws_need(c));
+    return ensure_ws(c, ws_need(c, p));
 }
 
 int check_krylov(tosca_ctx *c, const tosca_krylov_params *kp, tosca_krylov_result *kr) {
@@ -770,10 +775,10 @@ int tosca_formu(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const 
     }
     if ((rc = check_dev(c, rhs, 3 * (size_t)g.mx * g.my * g.mz * sizeof(double), "rhs"))) return rc;
     if (c->ref_path) {
-        if ((rc = ensure_ws(c, ws_need(c)))) return rc;
+        if ((rc = ensure_ws(c, ws_need(c, p)))) return rc;
         tosca::ref_formu(g, m, p, ucont, ucat, (p->les_active && visc) ? nut : nullptr, rhs, scale, form_mode, c->ws, c->stream);
     } else {
-        if ((rc = ensure_ws(c, ws_need(c)))) return rc;
+        if ((rc = ensure_ws(c, ws_need(c, p)))) return rc;
         tosca::FormUArgs A = formu_args(m, p, ucont, ucat, (p->les_active && visc) ? nut : nullptr, rhs, scale, form_mode);
         tosca::fused_formu(g, A, c->ws + 3 * (size_t)g.nloc(), c->stream);
     }
@@ -938,7 +943,7 @@ int tosca_ueqn_rk3w_adjoint(tosca_ctx *c, const tosca_mesh *m, const tosca_param
     if (ucont_bar == rhs || ucont_bar == rhs_o || ucont_bar == ucont_o)
         return set_err(c, TOSCA_ERR_ARG, "ucont_bar must not alias the work vectors");
     // the FormU adjoint's workspace, then the checkpoints (n_stages + 1 states) and the work vectors
-    if ((rc = ensure_ws(c, std::max(ws_need(c), 52 * n + 64)))) return rc;
+    if ((rc = ensure_ws(c, std::max(ws_need(c, p), 52 * n + 64)))) return rc;
     const size_t need = ((size_t)n_stages + 1) * ng3 + 2 * ng3 + 9 * n;
     if (c->adj_elems < need) {
         if (c->adj) { hipStreamSynchronize(c->stream); hipFree(c->adj); c->adj = nullptr; c->adj_elems = 0; }
@@ -1249,7 +1254,7 @@ int tosca_formt(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const 
             (rc = check_dev(c, tp->q_wall_j, 3 * (size_t)g.mz * g.mx * sizeof(double), "q_wall_j")))
             return rc;
     }
-    if ((rc = ensure_ws(c, ws_need(c)))) return rc;
+    if ((rc = ensure_ws(c, ws_need(c, p)))) return rc;
     tosca::ref_formt(g, m, p, tp, ucont, theta, nut, kt, rhs, scale, form_mode, c->ws, c->stream);
     return launched(c, "FormT");
 }
@@ -1355,7 +1360,7 @@ int tosca_update_cs(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, co
         if ((rc = check_mesh(c, m, false, true, false))) return rc;
         if ((rc = check_dev(c, ucat, 3 * n * sizeof(double), "ucat"))) return rc;
     }
-    if ((rc = ensure_ws(c, ws_need(c)))) return rc;
+    if ((rc = ensure_ws(c, ws_need(c, p)))) return rc;
     if (!c->ref_path && (p->les_model == TOSCA_LES_DSM || p->les_model == TOSCA_LES_DPASD)) {
         if (p->les_model == TOSCA_LES_DSM) tosca::fused_dsm<double>(c->geo, m, ucat, p->max_cs, cs, nullptr, c->ws, c->stream);
         else tosca::fused_dpasd<double>(c->geo, m, ucat, p->max_cs, cs, nullptr, c->ws, c->stream);
@@ -1452,7 +1457,7 @@ int rhs_les_step_impl(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, 
         if ((rc = check_dev(c, p->tau_wall_i, 3 * (size_t)g.mz * g.my * sizeof(double), "tau_wall_i"))) return rc;
     if (p->bc_u[2] == TOSCA_BCU_WALLFUNCTION || p->bc_u[3] == TOSCA_BCU_WALLFUNCTION)
         if ((rc = check_dev(c, p->tau_wall_j, 3 * (size_t)g.mz * g.mx * sizeof(double), "tau_wall_j"))) return rc;
-    if ((rc = ensure_ws(c, ws_need(c)))) return rc;
+    if ((rc = ensure_ws(c, ws_need(c, p)))) return rc;
     queue_step<double>(c, m, p, ucont, ucat, theta, rhs, scale, cs, nut, btheta, buoy_scale, nullptr, farm);
     return launched(c, "rhs_les_step");
 }
@@ -1632,7 +1637,7 @@ int tosca_rhs_vjp(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, cons
         (rc = check_dev(c, cs_bar, n * sizeof(double), "cs_bar")))
         return rc;
     // nu_t, fpbar, the nu_eff cotangents and the ucat-cotangent packets of the faces and cells
-    if ((rc = ensure_ws(c, std::max(ws_need(c), 52 * n + 64)))) return rc;
+    if ((rc = ensure_ws(c, std::max(ws_need(c, p), 52 * n + 64)))) return rc;
     tosca::vjp_rhs(g, m, p, ucont, ucat, theta, cs, rbar, scale, buoy_scale, ucont_bar, ucat_bar, theta_bar, cs_bar,
                    c->ws, c->stream);
     return launched(c, "rhs_vjp");
@@ -1660,7 +1665,7 @@ int tosca_rhs_vjp_mixed(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p
         (rc = check_dev(c, ucat_bar, v, "ucat_bar")) || (rc = check_dev(c, theta_bar, n * sizeof(float), "theta_bar")) ||
         (rc = check_dev(c, cs_bar, n * sizeof(double), "cs_bar")))
         return rc;
-    if ((rc = ensure_ws(c, std::max(ws_need(c), 52 * n + 64)))) return rc;
+    if ((rc = ensure_ws(c, std::max(ws_need(c, p), 52 * n + 64)))) return rc;
     tosca::vjp_rhs_mixed(g, m, p, ucont, ucat, cs, rbar, scale, buoy_scale, ucont_bar, ucat_bar, theta_bar, cs_bar,
                          c->ws, c->stream);
     return launched(c, "rhs_vjp_mixed");
