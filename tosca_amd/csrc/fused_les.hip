@@ -171,8 +171,10 @@ __global__ void __launch_bounds__(64 * DSM_NR) k_dsm_lm(Geo g, const T *ucat, co
             sH[n][r][lane] = (n < 27) ? (lo + S(4.0) * v + hi) : (S(0.25) * lo + S(0.5) * v + S(0.25) * hi);
         }
         __syncthreads();
-        const int rm = r > 0 ? r - 1 : r, rp = r < DSM_NR - 1 ? r + 1 : r;
+        const int rm = r - 1, rp = r + 1;
         // ---- filter along k, and the finished output cell K-1 --------------------------------------
+        // (the halo rows 0 and DSM_NR-1, whole waves, have no output and skip this part)
+        if (r >= 1 && r <= DSM_NR - 2) {
         const bool emit = outCol && K - 1 >= kb && K - 1 < ke;
         S q[NQ];
 #pragma unroll
@@ -213,6 +215,7 @@ __global__ void __launch_bounds__(64 * DSM_NR) k_dsm_lm(Geo g, const T *ucat, co
             LM[c] = nm * rd2;
             MM[c] = dn * rd2;
             Sabs[c] = sabsPrev;
+        }
         }
         sabsPrev = sabs;
         if constexpr (shifted) { cP = cC; eP = eC; zP = zC; aP = aC; }
