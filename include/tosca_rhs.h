@@ -498,10 +498,13 @@ int tosca_update_contravariant_bcs(tosca_ctx *ctx, const tosca_mesh *mesh, const
                                    double *ucont, const double *ucat);
 
 /* The kLeft inletFunction profile of tosca_update_cartesian_bcs (mesh->inletF.kLeft, read in
- * src/inflow.c:40-120; applied at boundary.c:1228-1300): the analytic velocity profiles. */
-enum { TOSCA_INLET_LOGLAW = 2, TOSCA_INLET_SINUSOIDAL = 6 };
+ * src/inflow.c:40-120 and 845-870; applied at boundary.c:1228-1300): the analytic velocity
+ * profiles.  TOSCA_INLET_NIEUWSTADT is NieuwstadtInflowEvaluate (src/inflow.c:2732-2811), the
+ * Ekman-layer profile with veer: (U, V, 0) from the complex velocity W(h) = Wg - dW(h) of a
+ * boundary layer capped at h_inv, rotated so that the profile at h_ref points along udir. */
+enum { TOSCA_INLET_LOGLAW = 2, TOSCA_INLET_NIEUWSTADT = 5, TOSCA_INLET_SINUSOIDAL = 6 };
 typedef struct {
-    int type;                        /* typeU: TOSCA_INLET_LOGLAW or TOSCA_INLET_SINUSOIDAL          */
+    int type;                        /* typeU: TOSCA_INLET_LOGLAW, _NIEUWSTADT or _SINUSOIDAL        */
     const double *cent;              /* mesh->lCent: cell centres, local AoS, device                */
     double grnd_level;               /* mesh->grndLevel (log law: h = cent.z - grnd_level)          */
     double ymin, ly;                 /* mesh->bounds.ymin, mesh->bounds.Ly (sinusoidal)             */
@@ -509,6 +512,8 @@ typedef struct {
     double amplitude, periods;       /* sinusoidal amplitude and number of periods across Ly        */
     double udir[3];                  /* Udir, the unit flow direction                               */
     double uref[3];                  /* Uref (sinusoidal: its magnitude scales the profile)         */
+    double fc;                       /* Nieuwstadt: Coriolis parameter fc = 2 Omega sin(latitude)   */
+    double h_ref;                    /* Nieuwstadt: hReference, the height whose direction is udir  */
 } tosca_inlet;
 
 /* UpdateCartesianBCs (src/boundary.c:824-2050) on lUcat (local AoS, in place): the ghost cells of

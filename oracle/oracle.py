@@ -51,6 +51,10 @@ def load() -> C.CDLL:
         "oracle_contravariant_bcs": (I, [In, M, Pa, P, P]),
         "oracle_cartesian_bcs": (I, [In, M, Pa, P, P, P, P]),
         "oracle_cabot_utau": (D, [D, D, D]),
+        "oracle_cgamma": (None, [D, D, P]),
+        "oracle_cdigamma": (None, [D, D, P]),
+        "oracle_hypergeom": (None, [P, P, P, D, P]),
+        "oracle_nieuwstadt": (None, [P, D, P]),
         "oracle_cabot_table": (D, [C.c_int64]),
         "oracle_temperature_bcs": (I, [In, M, Pa, P, P, P]),
         "oracle_formt": (I, [In, M, Pa, C.POINTER(abi.TeqnParams), P, P, P, P, P, D, I]),
@@ -200,6 +204,36 @@ def cartesian_bcs(info, mesh, params, ucat, u_fixed=None, ustar=None, inlet=None
 def cabot_utau(nu, u, y):
     """uTauCabot (wallfunctions.c:519-543): friction velocity of the smooth-wall Cabot law."""
     return float(load().oracle_cabot_utau(float(nu), float(u), float(y)))
+
+
+def cgamma(z):
+    """Complex gamma of the reference (inline.h:372-427)."""
+    out = np.zeros(2)
+    load().oracle_cgamma(float(z.real), float(z.imag), _p(out))
+    return complex(out[0], out[1])
+
+
+def cdigamma(z):
+    """Complex digamma of the reference (inline.h:429-476)."""
+    out = np.zeros(2)
+    load().oracle_cdigamma(float(z.real), float(z.imag), _p(out))
+    return complex(out[0], out[1])
+
+
+def hypergeom(a, b, c, z):
+    """2F1(a, b; c; z) for complex a, b, c and real z, by the reference's series (inline.h:478-525)."""
+    arr = [np.array([complex(v).real, complex(v).imag]) for v in (a, b, c)]
+    out = np.zeros(2)
+    load().oracle_hypergeom(_p(arr[0]), _p(arr[1]), _p(arr[2]), float(z), _p(out))
+    return complex(out[0], out[1])
+
+
+def nieuwstadt(inlet, h):
+    """NieuwstadtInflowEvaluate (inflow.c:2732-2811) at height h: (U, V, 0)."""
+    il = abi.make_inlet(dict(inlet, cent=None))
+    out = np.zeros(3)
+    load().oracle_nieuwstadt(C.cast(C.pointer(il), C.c_void_p), float(h), _p(out))
+    return out
 
 
 def cabot_table(i):

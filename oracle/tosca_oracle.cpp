@@ -19,6 +19,8 @@
 //   KSPGMRES as kspIMEX runs it (PETSc 3.15: classical Gram-Schmidt, Givens rotations)
 //   UpdateContravariantBCs / UpdateCartesianBCs / UpdateTemperatureBCs   src/boundary.c:614-2628
 //   the Cabot wall law   src/wallfunctions.c:280-585
+//   the Nieuwstadt inflow profile   src/inflow.c:2732-2811, with the complex gamma, digamma and
+//                    hypergeometric functions of src/include/inline.h:372-525
 //   FormT            src/solvers/teqn_solvers.c:6-595
 //   GradP            src/peqn.c:3437-3773
 //   projectBladeForce (uniformADM branch) and bodyForceCartesian2Contravariant
@@ -40,6 +42,7 @@
 // is off so every product and sum is rounded exactly as the reference's gcc build without FMA.
 
 #include <cmath>
+#include <complex>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -2099,6 +2102,138 @@ static double utau_rough(double nu, double ks, double sb, double ut) {
 extern "C" double oracle_cabot_utau(double nu, double u, double y) { return utau_cabot(nu, u, y); }
 extern "C" double oracle_cabot_table(int64_t i) { return cabot_entry(i); }
 
+// ---- Nieuwstadt inflow (src/inflow.c:2732-2811) and its special functions --------------------------
+// The complex gamma, digamma and Gauss hypergeometric functions of src/include/inline.h:372-525,
+// restated step by step, with the reference's coefficient tables (published Lanczos coefficients:
+// g = 9 with 11 terms for gamma, g = 607/128 with 15 terms for digamma), so that the profile
+// matches the reference's to round-off.
+namespace {
+using cplx = std::complex<double>;
+const double kLanczosG9[11] = {1.000000000000000174663,     5716.400188274341379136,    -14815.30426768413909044,
+                               14291.49277657478554025,     -6348.160217641458813289,   1301.608286058321874105,
+                               -108.1767053514369634679,    2.605696505611755827729,    -0.7423452510201416151527e-2,
+                               0.5384136432509564062961e-7, -0.4023533141268236372067e-8};
+const double kLanczosG607[15] = {0.99999999999999709182,     57.156235665862923517,      -59.597960355475491248,
+                                 14.136097974741747174,      -0.49191381609762019978,    .33994649984811888699e-4,
+                                 .46523628927048575665e-4,   -.98374475304879564677e-4,  .15808870322491248884e-3,
+                                 -.21026444172410488319e-3,  .21743961811521264320e-3,   -.16431810653676389022e-3,
+                                 .84418223983852743293e-4,   -.26190838401581408670e-4,  .36899182659531622704e-5};
+
+bool nonpositive_integer(cplx x) { return std::round(x.real()) == x.real() && x.imag() == 0.0 && x.real() <= 0.0; }
+
+// gamma (inline.h:372-427): Lanczos for Re x >= 0 (of -x otherwise, then the reflection formula);
+// exact products for positive integers, 1e20 at the poles
+cplx cgamma(cplx x) {
+    cplx z = x;
+    if (x.real() < 0.0) z = -1.0 * z;
+    const cplx g(9.0, 0.0);
+    cplx t = z + g, s(0.0, 0.0);
+    for (int k = 10; k > 0; k--) {
+        s = s + (cplx)kLanczosG9[k] / t;
+        t = t - 1.0;
+    }
+    s = s + (cplx)kLanczosG9[0];
+    const cplx ss = z + g - 0.5;
+    s = std::log(s * std::sqrt(2.0 * M_PI)) + (z - 0.5) * std::log(ss) - ss;
+    cplx f = std::exp(s);
+    if (x.real() < 0.0) f = -1.0 * M_PI / (x * f * std::sin(M_PI * x));
+    if (nonpositive_integer(x)) f = 1e20;
+    if (std::round(x.real()) == x.real() && x.imag() == 0.0 && x.real() > 0.0) {
+        cplx pp(1.0, 0.0);
+        for (int64_t m = 1; m < (int64_t)x.real(); m++) pp = pp * cplx((double)m, 0.0);
+        f = pp;
+    }
+    return f;
+}
+
+// digamma (inline.h:429-476): the logarithmic derivative of the Lanczos sum, the reflection
+// formula for Re x < 1/2, 1e20 at the poles
+cplx cdigamma(cplx x) {
+    cplx z = x;
+    if (x.real() < 0.5) z = 1.0 - z;
+    const cplx g(607.0 / 128.0, 0.0);
+    cplx n(0.0, 0.0), d = n;
+    for (int k = 14; k > 0; k--) {
+        const cplx dz = 1.0 / (z + (double)(k + 1) - 2.0);
+        const cplx dd = kLanczosG607[k] * dz;
+        d = d + dd;
+        n = n - dd * dz;
+    }
+    d = d + kLanczosG607[0];
+    const cplx gg = z + g - 0.5;
+    cplx f = std::log(gg) + (n / d - g / gg);
+    if (x.real() < 0.5) f = f - M_PI * (1.0 / std::tan(M_PI * x));
+    if (nonpositive_integer(x)) f = 1e20;
+    return f;
+}
+
+// 2F1(a, b; c; z) for real z (inline.h:478-525): the power series, left after the third
+// consecutive term below 1e-15 of the sum (30000 terms at most); NaN when the sum is NaN
+cplx hypergeom(cplx a, cplx b, cplx c, double z) {
+    if (nonpositive_integer(c)) return cplx(1e20, 0.0);
+    const double epsilon = 1e-15;
+    cplx term(1.0, 0.0), result(1.0, 0.0);
+    int termination = 0;
+    for (int64_t k = 0; k < 30000; k++) {
+        term = term * ((a + (double)k) * (b + (double)k) * z / ((c + (double)k) * ((double)k + 1.0)));
+        if (std::abs(term) < std::abs(result * epsilon)) {
+            if (++termination >= 3) return result;
+        } else {
+            termination = 0;
+        }
+        if (term == 0.0) break;
+        result = result + term;
+        if (std::isnan(result.real()) || std::isnan(result.imag())) return cplx(std::nan(""), std::nan(""));
+    }
+    return result;
+}
+
+// W(eta) = Wg - dW(eta), the complex velocity of the capped Ekman layer over u_tau
+cplx nieuwstadt_w(const tosca_inlet *in, double C, double eta) {
+    const double vkConstant = 0.4;
+    const cplx alpha = 0.5 + 0.5 * std::sqrt(cplx(1.0, 4.0 * C));
+    const cplx Const = cdigamma(alpha + 1.0) + cdigamma(alpha - 1.0) - 2.0 * cdigamma(1.0);
+    const cplx Wg = 1.0 / vkConstant * (std::log(in->h_inv / in->roughness) - Const);
+    const cplx deltaW = cplx(0.0, -1.0 / vkConstant) * alpha * alpha * cgamma(alpha) * cgamma(alpha) /
+                        (C * cgamma(2.0 * alpha)) * std::pow(1.0 - eta, alpha - 1.0) *
+                        hypergeom(alpha + 1.0, alpha - 1.0, 2.0 * alpha, 1.0 - eta);
+    return Wg - deltaW;
+}
+
+// NieuwstadtInflowEvaluate (inflow.c:2732-2811): the velocity at height h, eta = min(h / hInv, 1),
+// rotated by the angle between udir and the profile's direction at hReference (whose normalisation
+// divides V_r by the norm formed with the already normalised U_r, as the reference does)
+V3 nieuwstadt(const tosca_inlet *in, double h) {
+    const double vkConstant = 0.4;
+    const double C = in->fc * in->h_inv / (vkConstant * in->u_tau);
+    const cplx Wp = nieuwstadt_w(in, C, h < in->h_inv ? h / in->h_inv : 1.0);
+    const double U_p = Wp.real() * in->u_tau, V_p = Wp.imag() * in->u_tau;
+    const cplx Wr = nieuwstadt_w(in, C, in->h_ref / in->h_inv);
+    double U_r = Wr.real() * in->u_tau, V_r = Wr.imag() * in->u_tau;
+    U_r = U_r / sqrt(U_r * U_r + V_r * V_r);
+    V_r = V_r / sqrt(U_r * U_r + V_r * V_r);
+    const double theta = std::acos(in->udir[0] * U_r + in->udir[1] * V_r);
+    return V3{std::cos(theta) * U_p - std::sin(theta) * V_p, std::sin(theta) * U_p + std::cos(theta) * V_p, 0.0};
+}
+}  // namespace
+
+extern "C" void oracle_cgamma(double re, double im, double *out) {
+    const cplx f = cgamma(cplx(re, im));
+    out[0] = f.real(); out[1] = f.imag();
+}
+extern "C" void oracle_cdigamma(double re, double im, double *out) {
+    const cplx f = cdigamma(cplx(re, im));
+    out[0] = f.real(); out[1] = f.imag();
+}
+extern "C" void oracle_hypergeom(const double *a, const double *b, const double *c, double z, double *out) {
+    const cplx f = hypergeom(cplx(a[0], a[1]), cplx(b[0], b[1]), cplx(c[0], c[1]), z);
+    out[0] = f.real(); out[1] = f.imag();
+}
+extern "C" void oracle_nieuwstadt(const tosca_inlet *in, double h, double *out) {
+    const V3 u = nieuwstadt(in, h);
+    out[0] = u.x; out[1] = u.y; out[2] = u.z;
+}
+
 int oracle_cartesian_bcs(const tosca_dmda_info *info, const tosca_mesh *m, const tosca_params *p, double *ucat_,
                          const double *ufix, double *ustar, const tosca_inlet *inlet) {
     Lay l;
@@ -2120,7 +2255,10 @@ int oracle_cartesian_bcs(const tosca_dmda_info *info, const tosca_mesh *m, const
     const int *wf = p->wall_u;
     const double *aj = m->aj;
     const bool inletK = inlet && bc[4] == TOSCA_BCU_GENERIC;
-    if (inletK && inlet->type != 2 && inlet->type != 6) { g_err = "inlet function type not covered"; return TOSCA_ERR_UNSUPPORTED; }
+    if (inletK && inlet->type != 2 && inlet->type != 5 && inlet->type != 6) {
+        g_err = "inlet function type not covered";
+        return TOSCA_ERR_UNSUPPORTED;
+    }
     V3 *lucat = (V3 *)ucat_;
     const V3 *csi = (const V3 *)m->csi, *eta = (const V3 *)m->eta, *zet = (const V3 *)m->zet;
     const int64_t mx = l.mx, my = l.my, mz = l.mz;
@@ -2194,7 +2332,9 @@ int oracle_cartesian_bcs(const tosca_dmda_info *info, const tosca_mesh *m, const
                 if (k == 1 && inletK) {                          // boundary.c:1228-1300
                     const V3 cc = ((const V3 *)inlet->cent)[c];
                     double uMag = 0;
-                    if (inlet->type == 2) {                      // log-law profile
+                    if (inlet->type == 5) {                      // Nieuwstadt model
+                        ucat[gk0] = nieuwstadt(inlet, cc.z - inlet->grnd_level);
+                    } else if (inlet->type == 2) {               // log-law profile
                         const double h = cc.z - inlet->grnd_level, vkConstant = 0.4;
                         if (h <= inlet->h_inv) {
                             const double v = (inlet->u_tau / vkConstant) * std::log(h / inlet->roughness);
@@ -2208,7 +2348,7 @@ int oracle_cartesian_bcs(const tosca_dmda_info *info, const tosca_mesh *m, const
                         uMag = (1.0 + inlet->amplitude * std::cos(inlet->periods * 2.0 * M_PI * fraction)) *
                                sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
                     }
-                    ucat[gk0] = V3{uMag * inlet->udir[0], uMag * inlet->udir[1], uMag * inlet->udir[2]};
+                    if (inlet->type != 5) ucat[gk0] = V3{uMag * inlet->udir[0], uMag * inlet->udir[1], uMag * inlet->udir[2]};
                 }
                 if (bc[0] == TOSCA_BCU_WALLFUNCTION && wf[0] == -1 && i == 1) cabot(c, 0, gi0, roughI);
                 else if (bc[1] == TOSCA_BCU_WALLFUNCTION && wf[1] == -1 && i == mx - 2) cabot(c, 0, gi1, roughI);

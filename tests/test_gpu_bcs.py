@@ -173,11 +173,12 @@ def test_cartesian_bcs_rejects_wall_function(rhs_mod):
     ctx.close()
 
 
-@pytest.mark.parametrize("kind", ["loglaw", "sinusoidal"])
+@pytest.mark.parametrize("kind", ["loglaw", "sinusoidal", "nieuwstadt"])
 def test_cartesian_bcs_inlet(rhs_mod, kind):
-    """kLeft inletFunction profiles (log law, sinusoidal) on a random curvilinear mesh with the
-    other patches of the other kinds; the cell centres come from the mesh builder.  log and cos
-    may round differently on the two sides: bar 1e-14."""
+    """kLeft inletFunction profiles (log law, sinusoidal, Nieuwstadt) on a random curvilinear mesh
+    with the other patches of the other kinds; the cell centres come from the mesh builder.  log
+    and cos may round differently on the two sides: bar 1e-14; the Nieuwstadt profile's complex
+    log, exp, sqrt, quotients and its series of up to thousands of terms: bar 1e-12."""
     rng = np.random.Generator(np.random.PCG64(61))
     lay, mesh_t = C.mesh_metrics(random_nodes((7, 6, 5), 61), (0, 0, 0))
     cent = to_np(mesh_t["cent"])
@@ -191,6 +192,12 @@ def test_cartesian_bcs_inlet(rhs_mod, kind):
     if kind == "loglaw":
         inlet = dict(type=C.INLET_LOGLAW, grnd_level=float(zc.min()) - 0.05, u_tau=0.4, roughness=0.03,
                      h_inv=float(np.median(zc)), udir=(0.6, 0.8, 0.0))
+    elif kind == "nieuwstadt":
+        # cells below and above the capping height, the lowest within 0.05 of the ground (a series of
+        # thousands of terms); Coriolis parameter for C = fc h_inv / (kappa u_tau) = 0.6
+        h_inv = float(np.median(zc) - zc.min())
+        inlet = dict(type=C.INLET_NIEUWSTADT, grnd_level=float(zc.min()) - 0.05, u_tau=0.4, roughness=0.03,
+                     h_inv=h_inv, h_ref=0.3 * h_inv, fc=0.6 * 0.4 * 0.4 / h_inv, udir=(np.cos(0.3), np.sin(0.3), 0.0))
     else:
         yc = own[..., 1]
         inlet = dict(type=C.INLET_SINUSOIDAL, ymin=float(yc.min()), ly=float(np.ptp(yc)), amplitude=0.25, periods=1.5,
@@ -206,7 +213,11 @@ def test_cartesian_bcs_inlet(rhs_mod, kind):
     torch.cuda.synchronize()
     e = relerr(to_np(Ug), Uo)
     print(f"  ucat relL2={e:.2e} identical={np.array_equal(to_np(Ug), Uo)}", flush=True)
-    assert e <= TOL
+    assert e <= (1e-12 if kind == "nieuwstadt" else TOL)
+    if kind == "nieuwstadt":
+        ghost = to_np(Ug)[3]                        # the k = 0 plane (ghost width 3), where the profile goes
+        print(f"  inlet ghost plane max|relerr|={np.abs(ghost - Uo[3]).max() / np.abs(Uo[3]).max():.2e}", flush=True)
+        assert np.abs(ghost - Uo[3]).max() <= 1e-12 * np.abs(Uo[3]).max()
     from tosca_amd.abi import ToscaError
     with pytest.raises(ToscaError):
         ctx.UpdateCartesianBCs({k: to_dev(v) for k, v in mesh.items()}, params, Ug, None, None,

@@ -69,16 +69,17 @@ class Params(C.Structure):
 class Inlet(C.Structure):
     _fields_ = [("type", C.c_int), ("cent", C.c_void_p), ("grnd_level", C.c_double), ("ymin", C.c_double),
                 ("ly", C.c_double), ("u_tau", C.c_double), ("roughness", C.c_double), ("h_inv", C.c_double),
-                ("amplitude", C.c_double), ("periods", C.c_double), ("udir", C.c_double * 3), ("uref", C.c_double * 3)]
+                ("amplitude", C.c_double), ("periods", C.c_double), ("udir", C.c_double * 3), ("uref", C.c_double * 3),
+                ("fc", C.c_double), ("h_ref", C.c_double)]
 
 
 def make_inlet(d: Mapping) -> "Inlet":
     """tosca_inlet from a dict: type, cent (device or host pointer as int), grnd_level, ymin, ly,
-    u_tau, roughness, h_inv, amplitude, periods, udir, uref."""
+    u_tau, roughness, h_inv, amplitude, periods, udir, uref, fc, h_ref."""
     x = Inlet()
     x.type = int(d["type"])
     x.cent = d.get("cent") or None
-    for n in ("grnd_level", "ymin", "ly", "u_tau", "roughness", "h_inv", "amplitude", "periods"):
+    for n in ("grnd_level", "ymin", "ly", "u_tau", "roughness", "h_inv", "amplitude", "periods", "fc", "h_ref"):
         setattr(x, n, float(d.get(n, 0.0)))
     x.udir[:] = [float(v) for v in d.get("udir", (1.0, 0.0, 0.0))]
     x.uref[:] = [float(v) for v in d.get("uref", (0.0, 0.0, 0.0))]

@@ -1311,9 +1311,10 @@ int tosca_update_cartesian_bcs(tosca_ctx *c, const tosca_mesh *m, const tosca_pa
             return rc;
     }
     if (inlet && p->bc_u[4] == TOSCA_BCU_GENERIC) {
-        if (inlet->type != TOSCA_INLET_LOGLAW && inlet->type != TOSCA_INLET_SINUSOIDAL)
-            return set_err(c, TOSCA_ERR_UNSUPPORTED, "inlet function type %d: only the log law (2) and the sinusoidal "
-                                                     "profile (6) are covered", inlet->type);
+        if (inlet->type != TOSCA_INLET_LOGLAW && inlet->type != TOSCA_INLET_NIEUWSTADT &&
+            inlet->type != TOSCA_INLET_SINUSOIDAL)
+            return set_err(c, TOSCA_ERR_UNSUPPORTED, "inlet function type %d: only the log law (2), the Nieuwstadt "
+                                                     "model (5) and the sinusoidal profile (6) are covered", inlet->type);
         if ((rc = check_dev(c, inlet->cent, 3 * n * sizeof(double), "inlet.cent"))) return rc;
     }
     if (smooth && !c->cabot) {

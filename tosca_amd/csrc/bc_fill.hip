@@ -231,8 +231,160 @@ __device__ V3 unit_normal(V3 cs, V3 et, V3 ze, int a) {
 struct InletArgs {
     int type;
     const double *cent;
-    double grnd, ymin, ly, u_tau, rough, h_inv, amp, periods, udir[3], uref_mag;
+    double grnd, ymin, ly, u_tau, rough, h_inv, amp, periods, udir[3], uref_mag, fc, h_ref;
 };
+
+// ---- the Nieuwstadt inflow profile (src/inflow.c:2732-2811) -----------------------------------------
+// Complex arithmetic of its own (no std::complex in device code): products and sums as written,
+// quotients by Smith's algorithm, log / exp / sqrt / sin / tan from their real and imaginary parts,
+// and a real base raised to a complex power as polar(x^Re w, Im w log x), 0 for x = 0.  Each
+// function follows the steps of the reference's (inline.h:372-525), with its coefficient tables.
+namespace {
+struct cx { double r, i; };
+__device__ __forceinline__ cx operator+(cx a, cx b) { return cx{a.r + b.r, a.i + b.i}; }
+__device__ __forceinline__ cx operator-(cx a, cx b) { return cx{a.r - b.r, a.i - b.i}; }
+__device__ __forceinline__ cx operator+(cx a, double b) { return cx{a.r + b, a.i}; }
+__device__ __forceinline__ cx operator-(cx a, double b) { return cx{a.r - b, a.i}; }
+__device__ __forceinline__ cx operator-(double a, cx b) { return cx{a - b.r, -b.i}; }
+__device__ __forceinline__ cx operator+(double a, cx b) { return cx{a + b.r, b.i}; }
+__device__ __forceinline__ cx operator*(cx a, cx b) { return cx{a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r}; }
+__device__ __forceinline__ cx operator*(double s, cx a) { return cx{s * a.r, s * a.i}; }
+__device__ __forceinline__ cx operator*(cx a, double s) { return cx{a.r * s, a.i * s}; }
+__device__ __forceinline__ cx operator/(cx a, cx b) {
+    if (fabs(b.r) >= fabs(b.i)) {
+        const double t = b.i / b.r, d = b.r + b.i * t;
+        return cx{(a.r + a.i * t) / d, (a.i - a.r * t) / d};
+    }
+    const double t = b.r / b.i, d = b.r * t + b.i;
+    return cx{(a.r * t + a.i) / d, (a.i * t - a.r) / d};
+}
+__device__ __forceinline__ cx operator/(double a, cx b) { return cx{a, 0.0} / b; }
+__device__ __forceinline__ cx operator/(cx a, double s) { return cx{a.r / s, a.i / s}; }
+__device__ __forceinline__ double cabs_(cx a) { return hypot(a.r, a.i); }
+__device__ __forceinline__ cx clog_(cx a) { return cx{log(hypot(a.r, a.i)), atan2(a.i, a.r)}; }
+__device__ __forceinline__ cx cexp_(cx a) {
+    const double e = exp(a.r);
+    return cx{e * cos(a.i), e * sin(a.i)};
+}
+__device__ __forceinline__ cx csqrt_(cx a) {           // principal branch
+    if (a.r == 0.0 && a.i == 0.0) return cx{0.0, a.i};
+    const double t = sqrt(0.5 * (hypot(a.r, a.i) + fabs(a.r)));
+    if (a.r >= 0.0) return cx{t, a.i / (2.0 * t)};
+    return cx{fabs(a.i) / (2.0 * t), copysign(t, a.i)};
+}
+__device__ __forceinline__ cx csin_(cx a) { return cx{sin(a.r) * cosh(a.i), cos(a.r) * sinh(a.i)}; }
+__device__ __forceinline__ cx ctan_(cx a) {
+    const double d = cos(2.0 * a.r) + cosh(2.0 * a.i);
+    return cx{sin(2.0 * a.r) / d, sinh(2.0 * a.i) / d};
+}
+__device__ __forceinline__ cx rpow_(double x, cx w) {   // x >= 0
+    if (!(x > 0.0)) return cx{0.0, 0.0};
+    const double rho = pow(x, w.r), th = w.i * log(x);
+    return cx{rho * cos(th), rho * sin(th)};
+}
+__device__ __forceinline__ bool nonpos_int(cx x) { return round(x.r) == x.r && x.i == 0.0 && x.r <= 0.0; }
+
+__constant__ double kLanczosG9[11] = {1.000000000000000174663,     5716.400188274341379136,
+                                      -14815.30426768413909044,    14291.49277657478554025,
+                                      -6348.160217641458813289,    1301.608286058321874105,
+                                      -108.1767053514369634679,    2.605696505611755827729,
+                                      -0.7423452510201416151527e-2, 0.5384136432509564062961e-7,
+                                      -0.4023533141268236372067e-8};
+__constant__ double kLanczosG607[15] = {0.99999999999999709182,    57.156235665862923517,     -59.597960355475491248,
+                                        14.136097974741747174,     -0.49191381609762019978,   .33994649984811888699e-4,
+                                        .46523628927048575665e-4,  -.98374475304879564677e-4, .15808870322491248884e-3,
+                                        -.21026444172410488319e-3, .21743961811521264320e-3,  -.16431810653676389022e-3,
+                                        .84418223983852743293e-4,  -.26190838401581408670e-4, .36899182659531622704e-5};
+constexpr double kPi = 3.14159265358979323846;
+
+__device__ cx cgamma_(cx x) {                            // inline.h:372-427
+    cx z = x;
+    if (x.r < 0.0) z = -1.0 * z;
+    const cx g{9.0, 0.0};
+    cx t = z + g, s{0.0, 0.0};
+    for (int k = 10; k > 0; k--) {
+        s = s + cx{kLanczosG9[k], 0.0} / t;
+        t = t - 1.0;
+    }
+    s = s + cx{kLanczosG9[0], 0.0};
+    const cx ss = z + g - 0.5;
+    s = clog_(s * sqrt(2.0 * kPi)) + (z - 0.5) * clog_(ss) - ss;
+    cx f = cexp_(s);
+    if (x.r < 0.0) f = -1.0 * kPi / (x * f * csin_(kPi * x));
+    if (nonpos_int(x)) f = cx{1e20, 0.0};
+    if (round(x.r) == x.r && x.i == 0.0 && x.r > 0.0) {
+        cx pp{1.0, 0.0};
+        for (long m = 1; m < (long)x.r; m++) pp = pp * cx{(double)m, 0.0};
+        f = pp;
+    }
+    return f;
+}
+
+__device__ cx cdigamma_(cx x) {                          // inline.h:429-476
+    cx z = x;
+    if (x.r < 0.5) z = 1.0 - z;
+    const cx g{607.0 / 128.0, 0.0};
+    cx n{0.0, 0.0}, d = n;
+    for (int k = 14; k > 0; k--) {
+        const cx dz = 1.0 / (z + (double)(k + 1) - 2.0);
+        const cx dd = kLanczosG607[k] * dz;
+        d = d + dd;
+        n = n - dd * dz;
+    }
+    d = d + kLanczosG607[0];
+    const cx gg = z + g - 0.5;
+    cx f = clog_(gg) + (n / d - g / gg);
+    if (x.r < 0.5) f = f - kPi * (1.0 / ctan_(kPi * x));
+    if (nonpos_int(x)) f = cx{1e20, 0.0};
+    return f;
+}
+
+__device__ cx hypergeom_(cx a, cx b, cx c, double z) {   // inline.h:478-525
+    if (nonpos_int(c)) return cx{1e20, 0.0};
+    const double epsilon = 1e-15;
+    cx term{1.0, 0.0}, result{1.0, 0.0};
+    int termination = 0;
+    for (int k = 0; k < 30000; k++) {
+        term = term * ((a + (double)k) * (b + (double)k) * z / ((c + (double)k) * ((double)k + 1.0)));
+        if (cabs_(term) < cabs_(result * epsilon)) {
+            if (++termination >= 3) return result;
+        } else {
+            termination = 0;
+        }
+        if (term.r == 0.0 && term.i == 0.0) break;
+        result = result + term;
+        if (isnan(result.r) || isnan(result.i)) return cx{nan(""), nan("")};
+    }
+    return result;
+}
+
+// W(eta) = Wg - dW(eta), the capped Ekman layer's complex velocity over u_tau
+__device__ cx nieuwstadt_w(const InletArgs &In, double C, double eta) {
+    const double vk = 0.4;
+    const cx alpha = 0.5 + 0.5 * csqrt_(cx{1.0, 4.0 * C});
+    const cx Const = cdigamma_(alpha + 1.0) + cdigamma_(alpha - 1.0) - 2.0 * cdigamma_(cx{1.0, 0.0});
+    const cx Wg = 1.0 / vk * (cx{log(In.h_inv / In.rough), 0.0} - Const);
+    const cx ga = cgamma_(alpha);
+    const cx deltaW = cx{0.0, -1.0 / vk} * alpha * alpha * ga * ga / (C * cgamma_(2.0 * alpha)) *
+                      rpow_(1.0 - eta, alpha - 1.0) * hypergeom_(alpha + 1.0, alpha - 1.0, 2.0 * alpha, 1.0 - eta);
+    return Wg - deltaW;
+}
+
+// NieuwstadtInflowEvaluate at height h: (U, V, 0), rotated so that the direction at hReference
+// lies along udir (V_r normalised with the already normalised U_r, as in the reference)
+__device__ V3 nieuwstadt(const InletArgs &In, double h) {
+    const double vk = 0.4;
+    const double C = In.fc * In.h_inv / (vk * In.u_tau);
+    const cx Wp = nieuwstadt_w(In, C, h < In.h_inv ? h / In.h_inv : 1.0);
+    const double U_p = Wp.r * In.u_tau, V_p = Wp.i * In.u_tau;
+    const cx Wr = nieuwstadt_w(In, C, In.h_ref / In.h_inv);
+    double U_r = Wr.r * In.u_tau, V_r = Wr.i * In.u_tau;
+    U_r = U_r / sqrt(U_r * U_r + V_r * V_r);
+    V_r = V_r / sqrt(U_r * U_r + V_r * V_r);
+    const double theta = acos(In.udir[0] * U_r + In.udir[1] * V_r);
+    return V3{cos(theta) * U_p - sin(theta) * V_p, sin(theta) * U_p + cos(theta) * V_p, 0.0};
+}
+}  // namespace
 
 // ghost cell of a patch next to an interior boundary cell (transverse indices interior)
 __global__ void k_cart_bcs(double *ucat, const double *csi, const double *eta, const double *zet, const double *aj,
@@ -299,8 +451,10 @@ __global__ void k_cart_bcs(double *ucat, const double *csi, const double *eta, c
         out = V3{gp.x - bN.x, gp.y - bN.y, gp.z - bN.z};
     } else if (b == TOSCA_BCU_GENERIC && q == 4 && In.type != 0) {      // inletFunction, :1228-1300
         const V3 cc = ld3(In.cent, ci);
-        double uMag;
-        if (In.type == TOSCA_INLET_LOGLAW) {                             // :1254-1286
+        double uMag = 0.0;
+        if (In.type == TOSCA_INLET_NIEUWSTADT) {                         // :1287-1292
+            out = nieuwstadt(In, cc.z - In.grnd);
+        } else if (In.type == TOSCA_INLET_LOGLAW) {                      // :1254-1286
             const double h = cc.z - In.grnd, vk = 0.4;
             if (h <= In.h_inv) {
                 const double v = (In.u_tau / vk) * log(h / In.rough);
@@ -312,7 +466,7 @@ __global__ void k_cart_bcs(double *ucat, const double *csi, const double *eta, c
             const double fraction = (cc.y - In.ymin) / In.ly;
             uMag = (1.0 + In.amp * cos(In.periods * 2.0 * 3.14159265358979323846 * fraction)) * In.uref_mag;
         }
-        out = V3{uMag * In.udir[0], uMag * In.udir[1], uMag * In.udir[2]};
+        if (In.type != TOSCA_INLET_NIEUWSTADT) out = V3{uMag * In.udir[0], uMag * In.udir[1], uMag * In.udir[2]};
     } else if (b == TOSCA_BCU_ZEROGRADIENT) {                            // :1869-1928
         const V3 U = ld3(ucat, ci), U2 = ld3(ucat, g.L(c2[2], c2[1], c2[0]));
         out = V3{2 * U.x - U2.x, 2 * U.y - U2.y, 2 * U.z - U2.z};
@@ -407,7 +561,7 @@ void launch_cartesian_bcs(double *ucat, const double *u_fixed, double *ustar, co
     if (inlet) {
         In.type = inlet->type; In.cent = inlet->cent; In.grnd = inlet->grnd_level; In.ymin = inlet->ymin;
         In.ly = inlet->ly; In.u_tau = inlet->u_tau; In.rough = inlet->roughness; In.h_inv = inlet->h_inv;
-        In.amp = inlet->amplitude; In.periods = inlet->periods;
+        In.amp = inlet->amplitude; In.periods = inlet->periods; In.fc = inlet->fc; In.h_ref = inlet->h_ref;
         for (int d = 0; d < 3; d++) In.udir[d] = inlet->udir[d];
         const double *u = inlet->uref;
         In.uref_mag = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);   // nMag(Uref)
