@@ -17,7 +17,7 @@ import math
 import numpy as np
 import pytest
 import scipy.special as sp
-import mpmath
+mpmath = pytest.importorskip("mpmath")
 
 from oracle import oracle as O
 from tosca_amd import case as C
