@@ -9,7 +9,9 @@ sizes in the other GPU tests):
   - the mixed-precision steps against the FP64 step: <= 1e-5 per output field;
   - the adjoint against central finite differences of the forward in a random direction;
   - ExplicitRhsU and the RK4 predictor fused against reference-structured;
-  - GradP exact for a pressure linear in space.
+  - GradP exact for a pressure linear in space;
+  - the plane-averaged dynamic model fused against reference-structured, and the mixed step with it
+    and the mixed adjoint against their FP64 forms at the same FP32-rounded inputs.
 """
 import os
 
@@ -146,6 +148,62 @@ def test_mixed_against_fp64(shard):
             print(f"  mixed ({variant}) {k:7s} relL2={e:.2e}", flush=True)
             assert e <= 1e-5, (variant, k)
     del mesh32
+    ctx.close()
+
+
+def test_dpasd_fused_vs_reference_structured(shard):
+    """the plane-averaged dynamic model: the fused step (k-marching 5-point filter, wave-reduced
+    plane partials) against the reference-structured kernels (gathered 5 x 5 x 5 stencils)"""
+    rhs, case, mesh = shard
+    lay = case.lay
+    params = dict(case.params, les_model=C.LES_DPASD)
+    outs = {}
+    for path in ("fused", "ref"):
+        ctx = _ctx(rhs, lay, path)
+        o = _outs(lay)
+        ctx.rhs_les_step(mesh, params, case.ucont, case.ucat, case.theta, o["rhs"], 1.0, o["cs"], o["nut"],
+                         o["btheta"], 1.0)
+        torch.cuda.synchronize()
+        outs[path] = o
+        ctx.close()
+    for k in ("rhs", "cs", "nut"):
+        e = _rel(outs["fused"][k], outs["ref"][k])
+        print(f"  DPASD {k:4s} fused vs reference-structured relL2={e:.2e}", flush=True)
+        assert e <= 1e-12, k
+
+
+def test_mixed_dpasd_and_adjoint_against_fp64_on_rounded_inputs(shard):
+    """the mixed step with the plane-averaged model, and the mixed adjoint, against their FP64
+    forms evaluated at the same FP32-rounded inputs (the difference is the FP32 arithmetic alone):
+    <= 1e-5 per output"""
+    rhs, case, mesh = shard
+    lay = case.lay
+    params = dict(case.params, les_model=C.LES_DPASD)
+    ctx = _ctx(rhs, lay, "fused")
+    x32 = [t.float().contiguous() for t in (case.ucont, case.ucat, case.theta)]
+    x64 = [t.double() for t in x32]
+    ref = _outs(lay)
+    ctx.rhs_les_step(mesh, params, *x64, ref["rhs"], 1.0, ref["cs"], ref["nut"], ref["btheta"], 1.0)
+    o = _outs(lay, torch.float32)
+    ctx.rhs_les_step_mixed(mesh, params, *x32, o["rhs"], 1.0, o["cs"], o["nut"], o["btheta"], 1.0)
+    torch.cuda.synchronize()
+    for k in ("rhs", "cs", "nut", "btheta"):
+        e = _rel(o[k].double(), ref[k])
+        print(f"  mixed DPASD {k:7s} relL2={e:.2e}", flush=True)
+        assert e <= 1e-5, k
+    del o, ref
+    gen = torch.Generator(device="cuda").manual_seed(7)
+    rbar = torch.randn((lay.mz, lay.my, lay.mx, 3), dtype=torch.float64, device="cuda", generator=gen)
+    cs = torch.rand(lay.gshape, dtype=torch.float64, device="cuda", generator=gen) * 0.05
+    b64 = [torch.empty_like(t) for t in x64] + [torch.empty_like(cs)]
+    b32 = [torch.empty_like(t) for t in x32] + [torch.empty_like(cs)]
+    ctx.rhs_vjp(mesh, case.params, *x64, cs, rbar, 1.0, 1.0, *b64)
+    ctx.rhs_vjp_mixed(mesh, case.params, *x32, cs, rbar, 1.0, 1.0, *b32)
+    torch.cuda.synchronize()
+    for k, a, b in zip(("ucont", "ucat", "theta", "cs"), b32, b64):
+        e = _rel(a.double(), b)
+        print(f"  mixed adjoint {k:6s} relL2={e:.2e}", flush=True)
+        assert e <= 1e-5, k
     ctx.close()
 
 
