@@ -73,6 +73,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU-oracle baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-farm", action="store_true", help="shard workload without its wind-farm source")
+    ap.add_argument("--no-configs", action="store_true", help="skip the step timings of the other configurations")
     return ap.parse_args()
 
 
@@ -381,6 +382,72 @@ def dpasd_leg(torch, ctx, mesh, case, ucont, ucat, theta, lay, args, farm=None):
     print(f"# dpasd mixed x{out['mixed']['speedup_vs_fp64']:.2f} (Cs rel L2 {out['mixed']['cs_rel_l2_vs_fp64']:.1e}); "
           f"UpdateCs fused {ms_f:.2f} ms vs reference-structured {ms_r:.1f} ms (Cs rel L2 {rel:.1e})",
           file=sys.stderr, flush=True)
+    return out
+
+
+def configs_leg(torch, args, skip: int):
+    """The step (FP64 and mixed, FP64 metrics) on the other benchmark configurations at their full
+    sizes, each built on the GPU and timed like the main workload; and the adjoint against the
+    forward it differentiates on the 192x192x128 hill, where the adjoint ratio is quoted."""
+    from tosca_amd import case as C
+    from tosca_amd.rhs import RhsContext
+    out = {}
+    for cfg in (1, 2, 3, 4):
+        if cfg == skip:
+            continue
+        t0 = time.time()
+        case = build_case(cfg, "cuda")
+        lay = case.lay
+        mesh = {k: v.contiguous() for k, v in case.mesh.items() if k != "cent"}
+        ctx = RhsContext(lay.info())
+        ctx.mesh_check(mesh)
+        gs, gl = lay.gshape, (lay.mz, lay.my, lay.mx, 3)
+        rhs = torch.zeros(gl, dtype=torch.float64, device="cuda")
+        bt = torch.zeros_like(rhs)
+        cs = torch.zeros(gs, dtype=torch.float64, device="cuda")
+        nut = torch.zeros_like(cs)
+        x32 = [t.float().contiguous() for t in (case.ucont, case.ucat, case.theta)]
+        nut32 = torch.zeros(gs, dtype=torch.float32, device="cuda")
+
+        def timed(fn, reps):
+            for _ in range(args.warmup):
+                fn()
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                fn()
+            e1.record()
+            torch.cuda.synchronize()
+            return e0.elapsed_time(e1) / reps
+
+        ms = timed(lambda: ctx.rhs_les_step(mesh, case.params, case.ucont, case.ucat, case.theta, rhs, 1.0, cs, nut, bt,
+                                            1.0), args.steps)
+        ms32 = timed(lambda: ctx.rhs_les_step_mixed(mesh, case.params, *x32, rhs, 1.0, cs, nut32, bt, 1.0), args.steps)
+        r = {"workload": workload_name(cfg, case), "cells": lay.ncells, "ms_per_step": ms,
+             "value": lay.ncells / (ms * 1e-3), "unit": "cell-updates/s",
+             "step_frac_of_hbm_peak": STEP_BYTES * lay.ncells / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+             "mixed_ms_per_step": ms32, "mixed_speedup": ms / ms32}
+        if cfg == 4:
+            rbar = torch.randn(gl, dtype=torch.float64, device="cuda", generator=torch.Generator(device="cuda").manual_seed(4))
+            bars = [torch.empty_like(t) for t in (case.ucont, case.ucat, case.theta, cs)]
+
+            def fwd():
+                ctx.UpdateNut(mesh, case.params, case.ucat, cs, nut)
+                ctx.FormU(mesh, case.params, case.ucont, case.ucat, nut, rhs, 1.0, 0)
+                ctx.Buoyancy(mesh, case.params, case.theta, 1.0, bt)
+
+            f_ms = timed(fwd, args.steps)
+            v_ms = timed(lambda: ctx.rhs_vjp(mesh, case.params, case.ucont, case.ucat, case.theta, cs, rbar, 1.0, 1.0,
+                                             *bars), args.steps)
+            r.update({"vjp_ms": v_ms, "forward_ms": f_ms, "vjp_over_forward": v_ms / f_ms})
+        out[WORKLOAD_TAG[cfg]] = r
+        print(f"# config {cfg} ({WORKLOAD_TAG[cfg]}, built in {time.time() - t0:.1f} s): {ms:.3f} ms/step, "
+              f"{r['value']:.3e} cells/s, mixed x{r['mixed_speedup']:.2f}"
+              + (f", vjp/forward x{r['vjp_over_forward']:.2f}" if cfg == 4 else ""), file=sys.stderr, flush=True)
+        ctx.close()
+        del ctx, case, mesh, rhs, bt, cs, nut, x32, nut32
+        torch.cuda.empty_cache()
     return out
 
 
@@ -887,6 +954,8 @@ def main():
     line["formt"] = formt_leg(torch, ctx, mesh, case, ucont, theta, nut, lay, args)
     line["ghost_fills"] = boundary_leg(torch, ctx, mesh, case, ucont, ucat, theta, lay, args)
     line["host_staged"] = host_leg(torch, ctx, mesh, case, ucont, ucat, theta, lay, args, ms)
+    if not args.no_configs:
+        line["other_configs"] = configs_leg(torch, args, args.config)
     if not args.no_cpu:
         spec = C.CONFIGS[args.config]
         line["cpu_baseline"] = cpu_baseline((spec["builder"], spec["kwargs"]), args.cpu_seconds)
