@@ -15,6 +15,8 @@
 // Periodic seams: face 0 of a periodic axis is evaluated as face m-2 and the ghost cell m-1 as
 // cell 1, which is what resetFacePeriodicFluxesVector / resetCellPeriodicFluxes make the
 // reference read.
+#include <type_traits>
+
 #include "fused.h"
 
 namespace tosca {
@@ -83,6 +85,12 @@ template <class MT>
 __device__ __forceinline__ V3 ldnt3(const MT *p, int c) {
     return V3{double(__builtin_nontemporal_load(p + 3 * c)), double(__builtin_nontemporal_load(p + 3 * c + 1)),
               double(__builtin_nontemporal_load(p + 3 * c + 2))};
+}
+// the same, kept in the storage type
+template <class MT>
+__device__ __forceinline__ Vec3<MT> ldnt3s(const MT *p, int c) {
+    return Vec3<MT>{__builtin_nontemporal_load(p + 3 * c), __builtin_nontemporal_load(p + 3 * c + 1),
+                    __builtin_nontemporal_load(p + 3 * c + 2)};
 }
 
 // LDS ring of ucat planes: SoA [slot][component][row][col], rows j0-2 .. j0+NR, cols i0-2 .. i0+65
@@ -208,17 +216,17 @@ __device__ __forceinline__ const MT *face_fz(const MainArgs<T, MT> &A) { return 
 template <int DIR, class T, class MT>
 __device__ __forceinline__ const MT *face_fa(const MainArgs<T, MT> &A) { return DIR == 0 ? A.iaj : DIR == 1 ? A.jaj : A.kaj; }
 
-// metrics and fluxes of one interior face, loaded ahead of use
-template <class T>
-struct FaceIn { V3 fc, fe, fz; double fa; T V, nutR; };
+// metrics and fluxes of one interior face, loaded ahead of use, the metrics in their storage type
+template <class T, class MT>
+struct FaceIn { Vec3<MT> fc, fe, fz; MT fa; T V, nutR; };
 
 // c: local index of the face's cell
 template <int DIR, class T, class MT>
-__device__ __forceinline__ FaceIn<T> face_in(const MainArgs<T, MT> &A, int c, int s) {
-    FaceIn<T> f;
-    f.fc = ldnt3(face_fc<DIR>(A), c);
-    f.fe = ldnt3(face_fe<DIR>(A), c);
-    f.fz = ldnt3(face_fz<DIR>(A), c);
+__device__ __forceinline__ FaceIn<T, MT> face_in(const MainArgs<T, MT> &A, int c, int s) {
+    FaceIn<T, MT> f;
+    f.fc = ldnt3s(face_fc<DIR>(A), c);
+    f.fe = ldnt3s(face_fe<DIR>(A), c);
+    f.fz = ldnt3s(face_fz<DIR>(A), c);
     f.fa = face_fa<DIR>(A)[c];
     f.V = __builtin_nontemporal_load(A.ucont + 3 * c + DIR);
     f.nutR = A.nut[c + s];
@@ -268,7 +276,7 @@ __device__ __forceinline__ Vec3<T> ring_conv(const Ring<NR, T> &R, int row, int 
 
 // interior face flux from the LDS ring and preloaded face data (nut0: nu_t of the face's cell)
 template <int DIR, int NR, class T, class MT>
-__device__ __forceinline__ FluxT<T> face_pre(const MainArgs<T, MT> &A, const FaceIn<T> &in, T nut0, const Ring<NR, T> &R,
+__device__ __forceinline__ FluxT<T> face_pre(const MainArgs<T, MT> &A, const FaceIn<T, MT> &in, T nut0, const Ring<NR, T> &R,
                                              int row, int col, int t) {
     FluxT<T> r;
     r.d = Vec3<T>{0, 0, 0};
@@ -277,7 +285,8 @@ __device__ __forceinline__ FluxT<T> face_pre(const MainArgs<T, MT> &A, const Fac
     if (A.mode != 1) {
         T nuEff = T(A.nu);
         if (A.les) nuEff = T(A.nu) + T(0.5) * (nut0 + in.nutR);
-        r.v = ring_visc<DIR, NR, T>(R, row, col, t, in.fc, in.fe, in.fz, in.fa, nuEff);
+        r.v = ring_visc<DIR, NR, T>(R, row, col, t, cvt3<double>(in.fc), cvt3<double>(in.fe), cvt3<double>(in.fz),
+                                    double(in.fa), nuEff);
     }
     return r;
 }
@@ -406,6 +415,17 @@ __global__ void __launch_bounds__(64 * NR) k_fused_formu(MainArgs<T, MT> A, Geo 
     const int cp0 = c0 + (kb - 1) * SK;
     double fa0p = A.iaj[cp0], fa1p = A.jaj[cp0], fa2p = A.kaj[cp0];
     double gzIp = A.icsi[3 * cp0 + 2], gzJp = A.jeta[3 * cp0 + 2], gzKp = A.kzet[3 * cp0 + 2];
+    // With FP32 metric copies the face data of the next plane step fit in the registers the kernel
+    // has left at its occupancy (2 waves per SIMD), so they are loaded one step ahead; with FP64
+    // metrics they would spill.
+    constexpr bool AHEAD = std::is_same<MT, float>::value;
+    FaceIn<T, MT> nx1{}, nx2{}, nx3{};
+    if constexpr (AHEAD) {
+        const int cn = opaque_index(c0 + min(kb, g.mz - 2) * SK);
+        if (needF1) nx1 = face_in<0>(A, cn, 1);
+        if (needF2) nx2 = face_in<1>(A, cn, SJ);
+        if (needF3) nx3 = face_in<2>(A, cn, SK);
+    }
     for (int t = kb; t <= ke; t++) {
         // ---- every global load of this plane step, issued together ------------------------------
         T pre[PRE];
@@ -414,10 +434,20 @@ __global__ void __launch_bounds__(64 * NR) k_fused_formu(MainArgs<T, MT> A, Geo 
         const bool ghostK = (t == g.mz - 1);
         // faces of plane t (clamped at the ghost plane): local index of the face's cell
         const int cq = opaque_index(c0 + min(t, g.mz - 2) * SK);
-        FaceIn<T> in1{}, in2{}, in3{};
-        if (needF1) in1 = face_in<0>(A, cq, 1);
-        if (needF2) in2 = face_in<1>(A, cq, SJ);
-        if (needF3) in3 = face_in<2>(A, cq, SK);
+        FaceIn<T, MT> in1{}, in2{}, in3{};
+        if constexpr (AHEAD) {
+            in1 = nx1; in2 = nx2; in3 = nx3;
+            if (t + 1 <= ke) {
+                const int cn = opaque_index(c0 + min(t + 1, g.mz - 2) * SK);
+                if (needF1) nx1 = face_in<0>(A, cn, 1);
+                if (needF2) nx2 = face_in<1>(A, cn, SJ);
+                if (needF3) nx3 = face_in<2>(A, cn, SK);
+            }
+        } else {
+            if (needF1) in1 = face_in<0>(A, cq, 1);
+            if (needF2) in2 = face_in<1>(A, cq, SJ);
+            if (needF3) in3 = face_in<2>(A, cq, SK);
+        }
         // nu_t is loaded whether or not the viscous LES term uses it (the loop has no divergent
         // branches): A.nut must never be null; the launcher substitutes a valid local array when
         // the caller passes none (a null nu_t here faulted the GPU once, in formMode 1)
