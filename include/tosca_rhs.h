@@ -498,13 +498,35 @@ int tosca_update_contravariant_bcs(tosca_ctx *ctx, const tosca_mesh *mesh, const
                                    double *ucont, const double *ucat);
 
 /* The kLeft inletFunction profile of tosca_update_cartesian_bcs (mesh->inletF.kLeft, read in
- * src/inflow.c:40-120 and 845-870; applied at boundary.c:1228-1300): the analytic velocity
- * profiles.  TOSCA_INLET_NIEUWSTADT is NieuwstadtInflowEvaluate (src/inflow.c:2732-2811), the
- * Ekman-layer profile with veer: (U, V, 0) from the complex velocity W(h) = Wg - dW(h) of a
- * boundary layer capped at h_inv, rotated so that the profile at h_ref points along udir. */
-enum { TOSCA_INLET_LOGLAW = 2, TOSCA_INLET_NIEUWSTADT = 5, TOSCA_INLET_SINUSOIDAL = 6 };
+ * src/inflow.c:40-120 and 422-870; applied at boundary.c:1228-1600).  h = cent.z - grnd_level.
+ *   TOSCA_INLET_POWERLAW (typeU 1): (h / href)^0.107027 uref (1 + noise / 10000 uprime_rms).  The
+ *     reference draws noise = rand() % 20000 - 10000 per ghost after srand(time(NULL)), so its
+ *     values cannot be reproduced; the caller passes the draws it wants, one per (j, i).
+ *   TOSCA_INLET_LOGLAW (2), TOSCA_INLET_SINUSOIDAL (6): the analytic profiles.
+ *   TOSCA_INLET_NIEUWSTADT (5): NieuwstadtInflowEvaluate (src/inflow.c:2732-2811), the Ekman-layer
+ *     profile with veer: (U, V, 0) from the complex velocity W(h) = Wg - dW(h) of a boundary layer
+ *     capped at h_inv, rotated so that the profile at h_ref points along udir.
+ *   TOSCA_INLET_MAPPED (3): the inflow-database plane repeated prds1 x prds2 times: ghost (j, i)
+ *     takes plane[jif][iif], jif = j mod n1 in 1..n1 and iif = i mod n2 in 1..n2, while
+ *     j <= n1 prds1 and i <= n2 prds2; beyond that the last plane row / column (or, with merge1,
+ *     avg_top_u[9]).  With merge1 the value below the database top is blended into the height
+ *     profile avg_top_u at avg_top_coords (two-point interpolation, findInterpolationWeightsWith-
+ *     Extrap, inline.h:5873-5966) by the tanh weights of scaleHyperTang{Bot,Top} (inline.h:
+ *     5971-5999) with H = avg_top_length and delta = avg_top_delta.
+ *   TOSCA_INLET_INTERP (4): the database plane interpolated onto this mesh with the per-cell
+ *     stencils and weights of SetInflowWeights (src/inflow.c:941-1870): 4 points for every
+ *     component, or with spline the y (z) component from the 6-point stencils cells_2/weights_2
+ *     (cells_1/weights_1); with shift2 the values at i_left = y_ids[i] - 1 and i_right = y_ids[i]
+ *     mixed as (1 - y_weights[i]) left + y_weights[i] right (boundary.c:885-935 computes y_ids and
+ *     y_weights on the host each call); then the merge1 blend of type 3.
+ * plane is ifPtr->ucat_plane after readInflowU (the caller reads and time-interpolates the
+ * database files and copies the plane to the device). */
+enum {
+    TOSCA_INLET_POWERLAW = 1, TOSCA_INLET_LOGLAW = 2, TOSCA_INLET_MAPPED = 3, TOSCA_INLET_INTERP = 4,
+    TOSCA_INLET_NIEUWSTADT = 5, TOSCA_INLET_SINUSOIDAL = 6
+};
 typedef struct {
-    int type;                        /* typeU: TOSCA_INLET_LOGLAW, _NIEUWSTADT or _SINUSOIDAL        */
+    int type;                        /* typeU: one of TOSCA_INLET_*                                 */
     const double *cent;              /* mesh->lCent: cell centres, local AoS, device                */
     double grnd_level;               /* mesh->grndLevel (log law: h = cent.z - grnd_level)          */
     double ymin, ly;                 /* mesh->bounds.ymin, mesh->bounds.Ly (sinusoidal)             */
@@ -514,6 +536,28 @@ typedef struct {
     double uref[3];                  /* Uref (sinusoidal: its magnitude scales the profile)         */
     double fc;                       /* Nieuwstadt: Coriolis parameter fc = 2 Omega sin(latitude)   */
     double h_ref;                    /* Nieuwstadt: hReference, the height whose direction is udir  */
+    /* power law (typeU 1); its velocity at href is uref                                          */
+    double href, uprime_rms;         /* Href, uPrimeRMS                                             */
+    const int *noise;                /* [my][mx] the draws in [-10000, 9999] at the kLeft ghosts     */
+    /* mapped inflow databases (typeU 3 and 4)                                                   */
+    const double *plane;             /* ucat_plane [n1 + 2][n2 + 2][3], device                      */
+    int n1, n2;                      /* database plane size without its side ghosts                 */
+    int prds1, prds2;                /* n1Periods, n2Periods (typeU 3)                              */
+    int merge1;                      /* n1Merge: blend into avg_top_u near the database top         */
+    double avg_top_length;           /* avgTopLength                                                */
+    double avg_top_delta;            /* avgTopDelta                                                 */
+    double avg_top_coords[10];       /* avgTopPointCoords (ascending heights)                       */
+    double avg_top_u[10][3];         /* uBarAvgTopX                                                 */
+    /* typeU 4: stencils into plane as (j, i) pairs, [my][mx][4][2] and [my][mx][6][2], device   */
+    int spline;                      /* interpMethod == "spline"                                    */
+    int shift2;                      /* n2Shift                                                     */
+    const int *cells;                /* closestCells                                                */
+    const double *weights;           /* inflowWeights [my][mx][4]                                   */
+    const int *cells_1, *cells_2;    /* closestCells_1, closestCells_2 (spline only)                */
+    const double *weights_1;         /* inflowWeights_1 [my][mx][6] (spline only)                   */
+    const double *weights_2;         /* inflowWeights_2 [my][mx][6] (spline only)                   */
+    const int *y_ids;                /* yIDs [mx] (shift2 only)                                     */
+    const double *y_weights;         /* yWeights [mx] (shift2 only)                                 */
 } tosca_inlet;
 
 /* UpdateCartesianBCs (src/boundary.c:824-2050) on lUcat (local AoS, in place): the ghost cells of
@@ -537,13 +581,13 @@ typedef struct {
  *                minus Ub_nor (0 and u* = 0 where |Ub_par| < 1e-20).  The right patch of an
  *                axis whose left patch is Cabot too takes the left patch's roughness, as in the
  *                reference (boundary.c:978-979, 1072-1073),
- *   inletFunction on kLeft (bc_u[4] = TOSCA_BCU_GENERIC with inlet != NULL, boundary.c:1228-1300):
- *                the log law max(uTau/0.4 ln(h/roughness), 1e-5) Udir below hInv and its value at
- *                hInv above, h = cent.z - grnd_level; or the sinusoidal profile
- *                (1 + amplitude cos(2 pi periods (cent.y - ymin)/Ly)) |Uref| Udir,
+ *   inletFunction on kLeft (bc_u[4] = TOSCA_BCU_GENERIC with inlet != NULL, boundary.c:1228-1600):
+ *                the profile of tosca_inlet: the power law, the log law max(uTau/0.4 ln(h/roughness),
+ *                1e-5) Udir below hInv and its value at hInv above, h = cent.z - grnd_level, the
+ *                mapped and interpolated inflow-database planes, the Nieuwstadt profile, or the
+ *                sinusoidal profile (1 + amplitude cos(2 pi periods (cent.y - ymin)/Ly)) |Uref| Udir,
  * then the ghost scatter.  ustar: local scalar array (lUstar), device; may be NULL when no patch
- * is Cabot.  inlet: may be NULL when kLeft is not an inlet function.  The other inlet-function
- * profiles (random power law, mapped inflow databases, Nieuwstadt), inlet functions on other
+ * is Cabot.  inlet: may be NULL when kLeft is not an inlet function.  Inlet functions on other
  * patches and wall-function k patches return TOSCA_ERR_UNSUPPORTED.  The first call with a
  * smooth Cabot patch builds the context's copy of the reference's integration table (5e6 + 1
  * doubles). */

@@ -55,6 +55,7 @@ def load() -> C.CDLL:
         "oracle_cdigamma": (None, [D, D, P]),
         "oracle_hypergeom": (None, [P, P, P, D, P]),
         "oracle_nieuwstadt": (None, [P, D, P]),
+        "oracle_interp_weights_extrap": (None, [P, I, D, P, P]),
         "oracle_cabot_table": (D, [C.c_int64]),
         "oracle_temperature_bcs": (I, [In, M, Pa, P, P, P]),
         "oracle_formt": (I, [In, M, Pa, C.POINTER(abi.TeqnParams), P, P, P, P, P, D, I]),
@@ -189,14 +190,19 @@ def contravariant_bcs(info, mesh, params, ucont, ucat):
 def cartesian_bcs(info, mesh, params, ucat, u_fixed=None, ustar=None, inlet=None):
     """UpdateCartesianBCs subset restated; ucat (and ustar, a local scalar array, where Cabot wall
     functions write it) modified in place; u_fixed (6, 3) or None; inlet: dict of the kLeft
-    inlet-function profile (abi.make_inlet keys, cent a numpy array of the local cell centres)."""
+    inlet-function profile (abi.make_inlet keys; its arrays numpy: cent, plane, weights*, y_weights
+    float64, noise, cells*, y_ids int32)."""
     lib = load()
     inf, m, p = abi.make_info(info), _mesh(mesh), abi.make_params(params)
     uf = None if u_fixed is None else np.ascontiguousarray(u_fixed, dtype=np.float64).reshape(18)
     il = None
     if inlet is not None:
-        cent = np.ascontiguousarray(inlet["cent"], dtype=np.float64)
-        il = abi.make_inlet(dict(inlet, cent=cent.ctypes.data))
+        keep = {}
+        for k in abi._INLET_PTRS:
+            if inlet.get(k) is not None:
+                dt = np.int32 if k in ("noise", "cells", "cells_1", "cells_2", "y_ids") else np.float64
+                keep[k] = np.ascontiguousarray(inlet[k], dtype=dt)
+        il = abi.make_inlet(dict(inlet, **{k: (keep[k].ctypes.data if k in keep else None) for k in abi._INLET_PTRS}))
     _check(lib.oracle_cartesian_bcs(C.byref(inf), C.byref(m), C.byref(p), _p(ucat), _p(uf), _p(ustar),
                                     C.cast(C.pointer(il), C.c_void_p) if il is not None else None))
 
@@ -234,6 +240,15 @@ def nieuwstadt(inlet, h):
     out = np.zeros(3)
     load().oracle_nieuwstadt(C.cast(C.pointer(il), C.c_void_p), float(h), _p(out))
     return out
+
+
+def interp_weights_extrap(pvec, pval):
+    """findInterpolationWeightsWithExtrap (inline.h:5943-5966): ((w0, w1), (label0, label1))."""
+    pv = np.ascontiguousarray(pvec, dtype=np.float64)
+    w = np.zeros(2)
+    lab = np.zeros(2, dtype=np.int32)
+    load().oracle_interp_weights_extrap(_p(pv), int(pv.size), float(pval), _p(w), lab.ctypes.data_as(C.c_void_p))
+    return (float(w[0]), float(w[1])), (int(lab[0]), int(lab[1]))
 
 
 def cabot_table(i):

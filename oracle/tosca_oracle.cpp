@@ -21,6 +21,9 @@
 //   the Cabot wall law   src/wallfunctions.c:280-585
 //   the Nieuwstadt inflow profile   src/inflow.c:2732-2811, with the complex gamma, digamma and
 //                    hypergeometric functions of src/include/inline.h:372-525
+//   the power-law and mapped inflow-database ghosts of UpdateCartesianBCs   src/boundary.c:1228-1622,
+//                    with findInterpolationWeights(WithExtrap) and scaleHyperTang{Bot,Top}
+//                    (inline.h:5873-5999)
 //   FormT            src/solvers/teqn_solvers.c:6-595
 //   GradP            src/peqn.c:3437-3773
 //   projectBladeForce (uniformADM branch) and bodyForceCartesian2Contravariant
@@ -2234,6 +2237,99 @@ extern "C" void oracle_nieuwstadt(const tosca_inlet *in, double h, double *out) 
     out[0] = u.x; out[1] = u.y; out[2] = u.z;
 }
 
+// ---- mapped inflow (typeU 3 and 4) helpers ------------------------------------------------------
+// findInterpolationWeights (src/include/inline.h:5873-5938): the two closest of npts points, a
+// point passed over when its distance equals the current runner-up's (or leader's) to 1e-5
+// relative; lower point first; linear weights
+void interp_weights(const double *pvec, int npts, double pval, double *weights, int *labels) {
+    const double eqTol = 1e-5;
+    std::vector<double> diff(npts);
+    int idx_1 = 0, idx_2 = 1;
+    for (int i = 0; i < npts; i++) diff[i] = std::fabs(pvec[i] - pval);
+    for (int i = 0; i < npts; i++) {
+        const double tol1 = std::fabs(diff[i] - diff[idx_2]) / diff[i];
+        if (diff[i] < diff[idx_1] && tol1 > eqTol) { idx_2 = idx_1; idx_1 = i; }
+        const double tol2 = std::fabs(diff[i] - diff[idx_1]) / diff[i];
+        if (diff[i] < diff[idx_2] && i != idx_1 && tol2 > eqTol) idx_2 = i;
+    }
+    if (pvec[idx_2] < pvec[idx_1]) std::swap(idx_1, idx_2);
+    labels[0] = idx_1; labels[1] = idx_2;
+    const double idx = (idx_2 - idx_1) / (pvec[idx_2] - pvec[idx_1]) * (pval - pvec[idx_1]) + idx_1;
+    weights[0] = (idx_2 - idx) / (idx_2 - idx_1);
+    weights[1] = (idx - idx_1) / (idx_2 - idx_1);
+}
+// findInterpolationWeightsWithExtrap (inline.h:5943-5966): the end point, weights 1/2 each, outside
+void interp_weights_extrap(const double *pvec, int npts, double pval, double *weights, int *labels) {
+    if (pval >= pvec[npts - 1]) { weights[0] = weights[1] = 0.5; labels[0] = labels[1] = npts - 1; }
+    else if (pval <= pvec[0]) { weights[0] = weights[1] = 0.5; labels[0] = labels[1] = 0; }
+    else interp_weights(pvec, npts, pval, weights, labels);
+}
+extern "C" void oracle_interp_weights_extrap(const double *pvec, int npts, double pval, double *weights, int *labels) {
+    interp_weights_extrap(pvec, npts, pval, weights, labels);
+}
+// scaleHyperTangBot / scaleHyperTangTop (inline.h:5971-5999)
+double hyper_tang_bot(double h, double H, double delta) { return 0.5 * (1.0 - std::tanh(7.0 * (h - (H - 0.5 * delta)) / delta)); }
+double hyper_tang_top(double h, double H, double delta) { return 0.5 * (1.0 + std::tanh(7.0 * (h - (H - 0.5 * delta)) / delta)); }
+
+// the n1Merge blend at height h (boundary.c:1318-1343, 1580-1617)
+V3 merge_top(const tosca_inlet *in, double height, V3 u) {
+    double Wg[2];
+    int IDs[2];
+    interp_weights_extrap(in->avg_top_coords, 10, height, Wg, IDs);
+    const double bot = hyper_tang_bot(height, in->avg_top_length, in->avg_top_delta);
+    const double top = hyper_tang_top(height, in->avg_top_length, in->avg_top_delta);
+    const double *a = in->avg_top_u[IDs[0]], *b = in->avg_top_u[IDs[1]];
+    return V3{bot * u.x + top * (a[0] * Wg[0] + b[0] * Wg[1]), bot * u.y + top * (a[1] * Wg[0] + b[1] * Wg[1]),
+              bot * u.z + top * (a[2] * Wg[0] + b[2] * Wg[1])};
+}
+
+// one component of a typeU-4 stencil at table entry (j, i) of a [my][mx][np] table
+double stencil(const tosca_inlet *in, const int *cells, const double *w, int np, int64_t jm, int comp) {
+    double s = 0.0;
+    for (int q = 0; q < np; q++) {
+        const int64_t e = jm * np + q;
+        const int64_t pj = cells[2 * e], pi = cells[2 * e + 1];
+        s += w[e] * in->plane[3 * (pj * (in->n2 + 2) + pi) + comp];
+    }
+    return s;
+}
+V3 interp_point(const tosca_inlet *in, int64_t mx, int64_t j, int64_t i) {
+    const int64_t jm = j * mx + i;
+    V3 u;
+    u.x = stencil(in, in->cells, in->weights, 4, jm, 0);
+    u.y = in->spline ? stencil(in, in->cells_2, in->weights_2, 6, jm, 1) : stencil(in, in->cells, in->weights, 4, jm, 1);
+    u.z = in->spline ? stencil(in, in->cells_1, in->weights_1, 6, jm, 2) : stencil(in, in->cells, in->weights, 4, jm, 2);
+    return u;
+}
+
+// the kLeft ghost of a mapped inflow (typeU 3: boundary.c:1302-1371; typeU 4: :1372-1622)
+V3 mapped_inflow(const tosca_inlet *in, int64_t mx, int64_t j, int64_t i, double height) {
+    auto plane = [&](int64_t pj, int64_t pi) {
+        const double *v = in->plane + 3 * (pj * (in->n2 + 2) + pi);
+        return V3{v[0], v[1], v[2]};
+    };
+    if (in->type == 3) {
+        int64_t jif = j % in->n1 == 0 ? in->n1 : j % in->n1;
+        int64_t iif = i % in->n2 == 0 ? in->n2 : i % in->n2;
+        if (j <= in->n1 * in->prds1 && i <= in->n2 * in->prds2)
+            return in->merge1 ? merge_top(in, height, plane(jif, iif)) : plane(jif, iif);
+        if (in->merge1) return V3{in->avg_top_u[9][0], in->avg_top_u[9][1], in->avg_top_u[9][2]};
+        if (j > in->n1 * in->prds1) jif = in->n1;
+        if (i > in->n2 * in->prds2) iif = in->n2;
+        return plane(jif, iif);
+    }
+    V3 g;
+    if (in->shift2) {
+        const int64_t iL = in->y_ids[i] - 1, iR = in->y_ids[i];
+        const V3 uL = interp_point(in, mx, j, iL), uR = interp_point(in, mx, j, iR);
+        const double w = in->y_weights[i];
+        g = V3{(1.0 - w) * uL.x + w * uR.x, (1.0 - w) * uL.y + w * uR.y, (1.0 - w) * uL.z + w * uR.z};
+    } else {
+        g = interp_point(in, mx, j, i);
+    }
+    return in->merge1 ? merge_top(in, height, g) : g;
+}
+
 int oracle_cartesian_bcs(const tosca_dmda_info *info, const tosca_mesh *m, const tosca_params *p, double *ucat_,
                          const double *ufix, double *ustar, const tosca_inlet *inlet) {
     Lay l;
@@ -2255,9 +2351,19 @@ int oracle_cartesian_bcs(const tosca_dmda_info *info, const tosca_mesh *m, const
     const int *wf = p->wall_u;
     const double *aj = m->aj;
     const bool inletK = inlet && bc[4] == TOSCA_BCU_GENERIC;
-    if (inletK && inlet->type != 2 && inlet->type != 5 && inlet->type != 6) {
+    if (inletK && (inlet->type < 1 || inlet->type > 6)) {
         g_err = "inlet function type not covered";
         return TOSCA_ERR_UNSUPPORTED;
+    }
+    if (inletK) {
+        const int t = inlet->type;
+        const bool bad = !inlet->cent || (t == 1 && !inlet->noise) ||
+                         ((t == 3 || t == 4) && (!inlet->plane || inlet->n1 < 1 || inlet->n2 < 1)) ||
+                         (t == 3 && (inlet->prds1 < 1 || inlet->prds2 < 1)) ||
+                         (t == 4 && (!inlet->cells || !inlet->weights)) ||
+                         (t == 4 && inlet->spline && (!inlet->cells_1 || !inlet->cells_2 || !inlet->weights_1 || !inlet->weights_2)) ||
+                         (t == 4 && inlet->shift2 && (!inlet->y_ids || !inlet->y_weights));
+        if (bad) { g_err = "inlet function: missing table or bad plane size"; return TOSCA_ERR_ARG; }
     }
     V3 *lucat = (V3 *)ucat_;
     const V3 *csi = (const V3 *)m->csi, *eta = (const V3 *)m->eta, *zet = (const V3 *)m->zet;
@@ -2334,6 +2440,16 @@ int oracle_cartesian_bcs(const tosca_dmda_info *info, const tosca_mesh *m, const
                     double uMag = 0;
                     if (inlet->type == 5) {                      // Nieuwstadt model
                         ucat[gk0] = nieuwstadt(inlet, cc.z - inlet->grnd_level);
+                    } else if (inlet->type == 1) {               // power law with the caller's draws
+                        const double h = cc.z - inlet->grnd_level, alpha = 0.107027;
+                        const double pw = std::pow(h / inlet->href, alpha);
+                        V3 u{pw * inlet->uref[0], pw * inlet->uref[1], pw * inlet->uref[2]};
+                        const double f = inlet->uprime_rms;
+                        const int n1 = inlet->noise[j * mx + i];
+                        const double turbulence = (1 + ((double)n1) / 10000. * f);
+                        ucat[gk0] = V3{turbulence * u.x, turbulence * u.y, turbulence * u.z};
+                    } else if (inlet->type == 3 || inlet->type == 4) {   // mapped inflow databases
+                        ucat[gk0] = mapped_inflow(inlet, mx, j, i, cc.z - inlet->grnd_level);
                     } else if (inlet->type == 2) {               // log-law profile
                         const double h = cc.z - inlet->grnd_level, vkConstant = 0.4;
                         if (h <= inlet->h_inv) {
@@ -2348,7 +2464,8 @@ int oracle_cartesian_bcs(const tosca_dmda_info *info, const tosca_mesh *m, const
                         uMag = (1.0 + inlet->amplitude * std::cos(inlet->periods * 2.0 * M_PI * fraction)) *
                                sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
                     }
-                    if (inlet->type != 5) ucat[gk0] = V3{uMag * inlet->udir[0], uMag * inlet->udir[1], uMag * inlet->udir[2]};
+                    if (inlet->type == 2 || inlet->type == 6)
+                        ucat[gk0] = V3{uMag * inlet->udir[0], uMag * inlet->udir[1], uMag * inlet->udir[2]};
                 }
                 if (bc[0] == TOSCA_BCU_WALLFUNCTION && wf[0] == -1 && i == 1) cabot(c, 0, gi0, roughI);
                 else if (bc[1] == TOSCA_BCU_WALLFUNCTION && wf[1] == -1 && i == mx - 2) cabot(c, 0, gi1, roughI);

@@ -94,3 +94,41 @@ def out_arrays(lay, seed=7):
     glob = (lay.mz, lay.my, lay.mx, 3)
     return dict(rhs=rng.standard_normal(glob), cs=rng.standard_normal(gs), nut=rng.standard_normal(gs),
                 ksgs=rng.standard_normal(gs), btheta=rng.standard_normal(glob), gcont=rng.standard_normal(glob))
+
+
+def inflow_inlet(lay, kind: str, seed: int, cent: np.ndarray, n1: int = 3, n2: int = 4) -> dict:
+    """A kLeft inlet-function profile of type power law / mapped / interpolated inflow with random
+    inputs (host numpy arrays) for a layout: kind 'powerlaw', 'mapped', 'mapped_merge', 'interp',
+    'interp_spline_shift_merge'.  The mapped plane is (n1 + 2) x (n2 + 2); periods are chosen so that
+    the mesh reaches past them in j and i."""
+    rng = np.random.Generator(np.random.PCG64(900 + seed))
+    mx, my = lay.mx, lay.my
+    own = cent[4:-4, 4:-4, 4:-4]
+    zc = own[..., 2]
+    grnd = float(zc.min()) - 0.05
+    if kind == "powerlaw":
+        return dict(type=C.INLET_POWERLAW, cent=cent, grnd_level=grnd, href=float(np.median(zc) - grnd),
+                    uprime_rms=0.1, uref=(4.0, 3.0, 0.5),
+                    noise=rng.integers(-10000, 10000, size=(my, mx)).astype(np.int32))
+    d = dict(cent=cent, grnd_level=grnd, n1=n1, n2=n2, plane=rng.standard_normal((n1 + 2, n2 + 2, 3)))
+    if kind.endswith("merge"):
+        h = zc - grnd
+        coords = np.sort(rng.uniform(h.min(), h.max(), 10))
+        d.update(merge1=1, avg_top_coords=tuple(coords), avg_top_u=tuple(map(tuple, rng.standard_normal((10, 3)))),
+                 avg_top_length=float(np.median(h)), avg_top_delta=float(0.3 * np.ptp(h)))
+    if kind.startswith("mapped"):
+        d.update(type=C.INLET_MAPPED, prds1=max(1, (my - 3) // (2 * n1)), prds2=max(1, (mx - 3) // n2))
+        return d
+    d.update(type=C.INLET_INTERP,
+             cells=np.stack([rng.integers(0, n1 + 2, size=(my, mx, 4)), rng.integers(0, n2 + 2, size=(my, mx, 4))],
+                            axis=-1).astype(np.int32),
+             weights=rng.uniform(0.0, 1.0, size=(my, mx, 4)))
+    if "spline" in kind:
+        for s in ("_1", "_2"):
+            d["cells" + s] = np.stack([rng.integers(0, n1 + 2, size=(my, mx, 6)), rng.integers(0, n2 + 2, size=(my, mx, 6))],
+                                      axis=-1).astype(np.int32)
+            d["weights" + s] = rng.uniform(-0.2, 1.0, size=(my, mx, 6))
+        d["spline"] = 1
+    if "shift" in kind:
+        d.update(shift2=1, y_ids=rng.integers(1, mx, size=mx).astype(np.int32), y_weights=rng.uniform(0.0, 1.0, size=mx))
+    return d

@@ -46,6 +46,9 @@ def test_struct_sizes_match_header(tmp_path):
 #include <stddef.h>
 #include "tosca_rhs.h"
 int main(void) {
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu\n", offsetof(tosca_inlet, noise), offsetof(tosca_inlet, plane),
+         offsetof(tosca_inlet, merge1), offsetof(tosca_inlet, avg_top_coords), offsetof(tosca_inlet, avg_top_u),
+         offsetof(tosca_inlet, spline), offsetof(tosca_inlet, cells), offsetof(tosca_inlet, y_weights));
   printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(tosca_inlet), offsetof(tosca_inlet, udir), sizeof(tosca_dmda_info), sizeof(tosca_mesh), sizeof(tosca_params),
          offsetof(tosca_params, nut_value), offsetof(tosca_params, nu), offsetof(tosca_params, tau_wall_i),
          offsetof(tosca_params, wall_u), offsetof(tosca_params, wall_rough));
@@ -55,6 +58,9 @@ int main(void) {
     (d / "p.c").write_text(src)
     subprocess.run(["gcc", "-I", str(ROOT / "include"), str(d / "p.c"), "-o", str(d / "p")], check=True)
     vals = list(map(int, subprocess.run([str(d / "p")], capture_output=True, text=True, check=True).stdout.split()))
+    inlet_fields = ("noise", "plane", "merge1", "avg_top_coords", "avg_top_u", "spline", "cells", "y_weights")
+    assert vals[:8] == [getattr(abi.Inlet, f).offset for f in inlet_fields]
+    vals = vals[8:]
     assert vals[0] == C.sizeof(abi.Inlet) and vals[1] == abi.Inlet.udir.offset
     vals = vals[2:]
     assert vals[0] == C.sizeof(abi.DmdaInfo)

@@ -9,7 +9,7 @@ import torch
 
 from tosca_amd import case as C
 from oracle import oracle as O
-from tests._util import np_mesh, to_np, to_dev, relerr, random_nodes
+from tests._util import np_mesh, to_np, to_dev, relerr, random_nodes, inflow_inlet
 
 pytestmark = pytest.mark.gpu
 
@@ -221,5 +221,56 @@ def test_cartesian_bcs_inlet(rhs_mod, kind):
     from tosca_amd.abi import ToscaError
     with pytest.raises(ToscaError):
         ctx.UpdateCartesianBCs({k: to_dev(v) for k, v in mesh.items()}, params, Ug, None, None,
-                               dict(inlet, type=3, cent=to_dev(cent)))
+                               dict(inlet, type=7, cent=to_dev(cent)))
+    ctx.close()
+
+
+def _dev_inlet(inlet):
+    """The inlet dict with its arrays on the device (int32 tables stay int32)."""
+    out = {}
+    for k, v in inlet.items():
+        if isinstance(v, np.ndarray):
+            out[k] = torch.as_tensor(np.ascontiguousarray(v)).cuda()
+        else:
+            out[k] = v
+    return out
+
+
+@pytest.mark.parametrize("kind", ["powerlaw", "mapped", "mapped_merge", "interp", "interp_spline_shift_merge"])
+def test_cartesian_bcs_inflow(rhs_mod, kind):
+    """kLeft power-law (caller's noise draws), mapped (periodized plane, with and past its periods)
+    and interpolated (4-point stencils; 6-point spline stencils for v and w, the shifted left/right
+    mix) inflow ghosts, with and without the blend into the averaged top profile, on a random
+    curvilinear mesh with random planes and tables.  pow and tanh may round differently on the two
+    sides and the stencil sums may be contracted on the device: bar 1e-14 on the ghost plane."""
+    lay, mesh_t = C.mesh_metrics(random_nodes((5, 9, 8), 63), (0, 0, 0))
+    cent = to_np(mesh_t["cent"])
+    mesh = np_mesh(mesh_t)
+    params = C.default_params((0, 0, 0), [C.BC_FIXED] * 6, [C.BC_ZEROGRAD] * 6, div_scheme=C.DIV_CENTRAL,
+                              les_model=C.LES_DSM, nu=0.01)
+    params = dict(params, bc_u=(C.BCU_ZEROGRADIENT, C.BCU_SLIP, C.BCU_NOSLIP, C.BCU_ZEROGRADIENT, C.BCU_GENERIC,
+                                C.BCU_ZEROGRADIENT))
+    inlet = inflow_inlet(lay, kind, 5, cent)
+    print(f"\nUpdateCartesianBCs inflow {kind} mx={lay.mx} my={lay.my} mz={lay.mz}", flush=True)
+    rng = np.random.Generator(np.random.PCG64(64))
+    U0 = rng.standard_normal(lay.gshape + (3,))
+    Uo = U0.copy()
+    O.cartesian_bcs(lay.info(), mesh, params, Uo, None, None, inlet)
+    ctx = rhs_mod.RhsContext(lay.info())
+    Ug = to_dev(U0)
+    ctx.UpdateCartesianBCs({k: to_dev(v) for k, v in mesh.items()}, params, Ug, None, None, _dev_inlet(inlet))
+    torch.cuda.synchronize()
+    got = to_np(Ug)
+    ghost, ref = got[3, 4:lay.my + 2, 4:lay.mx + 2], Uo[3, 4:lay.my + 2, 4:lay.mx + 2]
+    e = np.abs(ghost - ref).max() / np.abs(ref).max()
+    print(f"  ucat relL2={relerr(got, Uo):.2e} ghost plane max|relerr|={e:.2e}", flush=True)
+    assert np.isfinite(ref).all() and np.abs(ref - U0[3, 4:lay.my + 2, 4:lay.mx + 2]).min() > 0
+    assert e <= TOL
+    assert relerr(got, Uo) <= TOL
+    from tosca_amd.abi import ToscaError
+    dev = {k: to_dev(v) for k, v in mesh.items()}
+    missing = {"powerlaw": dict(noise=None), "mapped": dict(n1=0), "mapped_merge": dict(plane=None),
+               "interp": dict(weights=None), "interp_spline_shift_merge": dict(y_ids=None)}[kind]
+    with pytest.raises(ToscaError):
+        ctx.UpdateCartesianBCs(dev, params, Ug, None, None, _dev_inlet(dict(inlet, **missing)))
     ctx.close()

@@ -198,8 +198,10 @@ def test_inlet_sinusoidal_mean():
     c45 = np.sqrt(0.5)
     assert abs(mag.max() - 5.0 * (1 + 0.3 * c45)) <= 1e-12 and abs(mag.min() - 5.0 * (1 - 0.3 * c45)) <= 1e-12
     from tosca_amd.abi import ToscaError
-    with pytest.raises(ToscaError):                        # the random power law is not covered
+    with pytest.raises(ToscaError):                        # the power law needs the caller's noise draws
         O.cartesian_bcs(lay.info(), mesh, params, u, None, None, dict(inlet, type=1))
+    with pytest.raises(ToscaError):                        # no typeU 7
+        O.cartesian_bcs(lay.info(), mesh, params, u, None, None, dict(inlet, type=7))
     with pytest.raises(ToscaError):                        # inlet functions on kLeft only
         O.cartesian_bcs(lay.info(), mesh, dict(params, bc_u=(GEN, ZG, NOSLIP, SLIP, GEN, ZG)), u, None, None, inlet)
 

@@ -46,7 +46,8 @@ def test_oracle_under_asan_ubsan(tmp_path):
     subprocess.run(cmd, cwd=ROOT, check=True, timeout=600)
     env = dict(os.environ, LD_PRELOAD=asan, TOSCA_ORACLE_LIB=str(lib), ASAN_OPTIONS="detect_leaks=0",
                UBSAN_OPTIONS="halt_on_error=1:print_stacktrace=1", OMP_NUM_THREADS="2")
-    tests = ["tests/test_oracle_boundary_pins.py", "tests/test_oracle_bcs.py", "tests/test_oracle_gradp.py",
+    tests = ["tests/test_oracle_boundary_pins.py", "tests/test_oracle_bcs.py", "tests/test_oracle_inflow.py",
+             "tests/test_oracle_gradp.py",
              "tests/test_oracle_krylov.py", "tests/test_oracle_rk3w.py", "tests/test_oracle_tangent.py"]
     rc, out = _run([sys.executable, "-m", "pytest", "-q", "-x", "-p", "no:cacheprovider", "-m", "not gpu", *tests],
                    env, 1500)

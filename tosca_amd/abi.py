@@ -70,19 +70,41 @@ class Inlet(C.Structure):
     _fields_ = [("type", C.c_int), ("cent", C.c_void_p), ("grnd_level", C.c_double), ("ymin", C.c_double),
                 ("ly", C.c_double), ("u_tau", C.c_double), ("roughness", C.c_double), ("h_inv", C.c_double),
                 ("amplitude", C.c_double), ("periods", C.c_double), ("udir", C.c_double * 3), ("uref", C.c_double * 3),
-                ("fc", C.c_double), ("h_ref", C.c_double)]
+                ("fc", C.c_double), ("h_ref", C.c_double),
+                ("href", C.c_double), ("uprime_rms", C.c_double), ("noise", C.c_void_p),
+                ("plane", C.c_void_p), ("n1", C.c_int), ("n2", C.c_int), ("prds1", C.c_int), ("prds2", C.c_int),
+                ("merge1", C.c_int), ("avg_top_length", C.c_double), ("avg_top_delta", C.c_double),
+                ("avg_top_coords", C.c_double * 10), ("avg_top_u", (C.c_double * 3) * 10),
+                ("spline", C.c_int), ("shift2", C.c_int), ("cells", C.c_void_p), ("weights", C.c_void_p),
+                ("cells_1", C.c_void_p), ("cells_2", C.c_void_p), ("weights_1", C.c_void_p),
+                ("weights_2", C.c_void_p), ("y_ids", C.c_void_p), ("y_weights", C.c_void_p)]
+
+
+_INLET_PTRS = ("cent", "noise", "plane", "cells", "weights", "cells_1", "cells_2", "weights_1", "weights_2",
+               "y_ids", "y_weights")
 
 
 def make_inlet(d: Mapping) -> "Inlet":
-    """tosca_inlet from a dict: type, cent (device or host pointer as int), grnd_level, ymin, ly,
-    u_tau, roughness, h_inv, amplitude, periods, udir, uref, fc, h_ref."""
+    """tosca_inlet from a dict: type; the pointers (device or host, as int) cent, noise, plane, cells,
+    weights, cells_1, cells_2, weights_1, weights_2, y_ids, y_weights; the scalars grnd_level, ymin,
+    ly, u_tau, roughness, h_inv, amplitude, periods, fc, h_ref, href, uprime_rms, avg_top_length,
+    avg_top_delta; the integers n1, n2, prds1, prds2, merge1, spline, shift2; the vectors udir,
+    uref; the tables avg_top_coords (10) and avg_top_u (10 x 3)."""
     x = Inlet()
     x.type = int(d["type"])
-    x.cent = d.get("cent") or None
-    for n in ("grnd_level", "ymin", "ly", "u_tau", "roughness", "h_inv", "amplitude", "periods", "fc", "h_ref"):
+    for n in _INLET_PTRS:
+        setattr(x, n, d.get(n) or None)
+    for n in ("grnd_level", "ymin", "ly", "u_tau", "roughness", "h_inv", "amplitude", "periods", "fc", "h_ref",
+              "href", "uprime_rms", "avg_top_length", "avg_top_delta"):
         setattr(x, n, float(d.get(n, 0.0)))
+    for n in ("n1", "n2", "prds1", "prds2", "merge1", "spline", "shift2"):
+        setattr(x, n, int(d.get(n, 0)))
     x.udir[:] = [float(v) for v in d.get("udir", (1.0, 0.0, 0.0))]
     x.uref[:] = [float(v) for v in d.get("uref", (0.0, 0.0, 0.0))]
+    x.avg_top_coords[:] = [float(v) for v in d.get("avg_top_coords", (0.0,) * 10)]
+    top = d.get("avg_top_u", ((0.0, 0.0, 0.0),) * 10)
+    for n in range(10):
+        x.avg_top_u[n][:] = [float(v) for v in top[n]]
     return x
 
 

@@ -34,7 +34,8 @@ PERIODIC_NONE, PERIODIC_COPY, PERIODIC_WRAP = 0, 1, 2
 DIV_CENTRAL, DIV_QUICK = 0, 1
 BCU_GENERIC, BCU_SLIP, BCU_WALLFUNCTION, BCU_PERIODIC, BCU_NOSLIP, BCU_ZEROGRADIENT, BCU_FIXEDVALUE = 0, 1, 2, 3, 4, 5, 6
 BCT_ZEROGRADIENT, BCT_FIXEDVALUE, BCT_FIXEDGRADIENT, BCT_PERIODIC, BCT_SURFACETEMP = 0, 1, 2, 3, 4
-INLET_LOGLAW, INLET_NIEUWSTADT, INLET_SINUSOIDAL = 2, 5, 6      # inletFunction typeU of the kLeft patch
+# inletFunction typeU of the kLeft patch
+INLET_POWERLAW, INLET_LOGLAW, INLET_MAPPED, INLET_INTERP, INLET_NIEUWSTADT, INLET_SINUSOIDAL = 1, 2, 3, 4, 5, 6
 BCNUT_NONE, BCNUT_ZEROGRADIENT, BCNUT_FIXEDVALUE, BCNUT_PERIODIC = 0, 1, 2, 3
 LES_SMAGORINSKY, LES_DSM, LES_DPASD = 0, 2, 5
 LES_STABILITY, LES_DLASI, LES_DLASD, LES_AMD, LES_VREMAN, LES_BV, LES_BAMD = 1, 3, 4, 6, 7, 8, 9

@@ -1311,11 +1311,38 @@ int tosca_update_cartesian_bcs(tosca_ctx *c, const tosca_mesh *m, const tosca_pa
             return rc;
     }
     if (inlet && p->bc_u[4] == TOSCA_BCU_GENERIC) {
-        if (inlet->type != TOSCA_INLET_LOGLAW && inlet->type != TOSCA_INLET_NIEUWSTADT &&
-            inlet->type != TOSCA_INLET_SINUSOIDAL)
-            return set_err(c, TOSCA_ERR_UNSUPPORTED, "inlet function type %d: only the log law (2), the Nieuwstadt "
-                                                     "model (5) and the sinusoidal profile (6) are covered", inlet->type);
+        const int t = inlet->type;
+        if (t < TOSCA_INLET_POWERLAW || t > TOSCA_INLET_SINUSOIDAL)
+            return set_err(c, TOSCA_ERR_UNSUPPORTED, "inlet function type %d: the power law (1), log law (2), mapped (3) "
+                                                     "and interpolated (4) inflow, Nieuwstadt (5) and sinusoidal (6) "
+                                                     "profiles are covered", t);
         if ((rc = check_dev(c, inlet->cent, 3 * n * sizeof(double), "inlet.cent"))) return rc;
+        const size_t plane_pts = (size_t)c->geo.mx * c->geo.my;        // [my][mx] tables
+        if (t == TOSCA_INLET_POWERLAW && (rc = check_dev(c, inlet->noise, plane_pts * sizeof(int), "inlet.noise")))
+            return rc;
+        if (t == TOSCA_INLET_MAPPED || t == TOSCA_INLET_INTERP) {
+            if (inlet->n1 < 1 || inlet->n2 < 1)
+                return set_err(c, TOSCA_ERR_ARG, "inflow plane size n1 = %d, n2 = %d", inlet->n1, inlet->n2);
+            if (t == TOSCA_INLET_MAPPED && (inlet->prds1 < 1 || inlet->prds2 < 1))
+                return set_err(c, TOSCA_ERR_ARG, "inflow periods n1Periods = %d, n2Periods = %d", inlet->prds1, inlet->prds2);
+            const size_t np = (size_t)(inlet->n1 + 2) * (inlet->n2 + 2);
+            if ((rc = check_dev(c, inlet->plane, 3 * np * sizeof(double), "inlet.plane"))) return rc;
+        }
+        if (t == TOSCA_INLET_INTERP) {
+            if ((rc = check_dev(c, inlet->cells, 8 * plane_pts * sizeof(int), "inlet.cells")) ||
+                (rc = check_dev(c, inlet->weights, 4 * plane_pts * sizeof(double), "inlet.weights")))
+                return rc;
+            if (inlet->spline &&
+                ((rc = check_dev(c, inlet->cells_1, 12 * plane_pts * sizeof(int), "inlet.cells_1")) ||
+                 (rc = check_dev(c, inlet->cells_2, 12 * plane_pts * sizeof(int), "inlet.cells_2")) ||
+                 (rc = check_dev(c, inlet->weights_1, 6 * plane_pts * sizeof(double), "inlet.weights_1")) ||
+                 (rc = check_dev(c, inlet->weights_2, 6 * plane_pts * sizeof(double), "inlet.weights_2"))))
+                return rc;
+            if (inlet->shift2 &&
+                ((rc = check_dev(c, inlet->y_ids, (size_t)c->geo.mx * sizeof(int), "inlet.y_ids")) ||
+                 (rc = check_dev(c, inlet->y_weights, (size_t)c->geo.mx * sizeof(double), "inlet.y_weights"))))
+                return rc;
+        }
     }
     if (smooth && !c->cabot) {
         if (hipMalloc(&c->cabot, tosca::cabot_table_doubles() * sizeof(double)) != hipSuccess) {

@@ -232,7 +232,115 @@ struct InletArgs {
     int type;
     const double *cent;
     double grnd, ymin, ly, u_tau, rough, h_inv, amp, periods, udir[3], uref_mag, fc, h_ref;
+    // power law
+    double uref[3], href, uprime;
+    const int *noise;
+    // mapped inflow planes
+    const double *plane;
+    int n1, n2, prds1, prds2, merge1, spline, shift2;
+    double top_len, top_delta, top_z[10], top_u[10][3];
+    const int *cells, *cells1, *cells2, *yids;
+    const double *w, *w1, *w2, *yw;
 };
+
+namespace {
+// findInterpolationWeightsWithExtrap (inline.h:5943-5966) over the 10 ascending heights pv: the
+// two closest points (a point whose distance equals the runner-up's to 1e-5 relative is passed
+// over, findInterpolationWeights, inline.h:5873-5938), ordered by height, and their weights
+__device__ void interp_weights10(const double *pv, double pval, double w[2], int id[2]) {
+    if (pval >= pv[9]) { w[0] = w[1] = 0.5; id[0] = id[1] = 9; return; }
+    if (pval <= pv[0]) { w[0] = w[1] = 0.5; id[0] = id[1] = 0; return; }
+    const double eqTol = 1e-5;
+    int i1 = 0, i2 = 1;
+    double d1 = fabs(pv[0] - pval), d2 = fabs(pv[1] - pval);
+    for (int i = 0; i < 10; i++) {
+        const double di = fabs(pv[i] - pval);
+        const double tol1 = fabs(di - d2) / di;
+        if (di < d1 && tol1 > eqTol) { i2 = i1; d2 = d1; i1 = i; d1 = di; }
+        const double tol2 = fabs(di - d1) / di;
+        if (di < d2 && i != i1 && tol2 > eqTol) { i2 = i; d2 = di; }
+    }
+    if (pv[i2] < pv[i1]) { const int t = i2; i2 = i1; i1 = t; }
+    const double idx = double(i2 - i1) / (pv[i2] - pv[i1]) * (pval - pv[i1]) + double(i1);
+    w[0] = (double(i2) - idx) / double(i2 - i1);
+    w[1] = (idx - double(i1)) / double(i2 - i1);
+    id[0] = i1; id[1] = i2;
+}
+
+// scaleHyperTangBot / scaleHyperTangTop (inline.h:5971-5999)
+__device__ __forceinline__ double tanh_bot(double h, double H, double delta) {
+    const double cb = H - 0.5 * delta;
+    return 0.5 * (1.0 - tanh(7.0 * (h - cb) / delta));
+}
+__device__ __forceinline__ double tanh_top(double h, double H, double delta) {
+    const double cb = H - 0.5 * delta;
+    return 0.5 * (1.0 + tanh(7.0 * (h - cb) / delta));
+}
+
+// the n1Merge blend of a database value u at height h into the averaged top profile
+// (boundary.c:1318-1343, 1580-1617)
+__device__ V3 merge_top(const InletArgs &In, double h, V3 u) {
+    double Wg[2];
+    int ID[2];
+    interp_weights10(In.top_z, h, Wg, ID);
+    const double b = tanh_bot(h, In.top_len, In.top_delta), t = tanh_top(h, In.top_len, In.top_delta);
+    const double *a0 = In.top_u[ID[0]], *a1 = In.top_u[ID[1]];
+    return V3{b * u.x + t * (a0[0] * Wg[0] + a1[0] * Wg[1]), b * u.y + t * (a0[1] * Wg[0] + a1[1] * Wg[1]),
+              b * u.z + t * (a0[2] * Wg[0] + a1[2] * Wg[1])};
+}
+
+// one component of the stencil sum w_0 p_0 + w_1 p_1 + ... over np points of cell (j, i) of the
+// interpolation tables (stencil entries clamped into the plane)
+__device__ double stencil_sum(const InletArgs &In, const int *cells, const double *w, int np, long jm, int comp) {
+    const long base = jm * np;
+    double s = 0.0;
+    for (int q = 0; q < np; q++) {
+        const int pj = min(max(cells[2 * (base + q)], 0), In.n1 + 1), pi = min(max(cells[2 * (base + q) + 1], 0), In.n2 + 1);
+        const double v = In.plane[3 * ((long)pj * (In.n2 + 2) + pi) + comp];
+        s = (q == 0) ? w[base + q] * v : s + w[base + q] * v;
+    }
+    return s;
+}
+
+// the interpolated database velocity of mesh cell (j, i), row-major index jm = j mx + i
+__device__ V3 interp_cell(const InletArgs &In, long jm) {
+    V3 u;
+    u.x = stencil_sum(In, In.cells, In.w, 4, jm, 0);
+    if (In.spline) {
+        u.y = stencil_sum(In, In.cells2, In.w2, 6, jm, 1);
+        u.z = stencil_sum(In, In.cells1, In.w1, 6, jm, 2);
+    } else {
+        u.y = stencil_sum(In, In.cells, In.w, 4, jm, 1);
+        u.z = stencil_sum(In, In.cells, In.w, 4, jm, 2);
+    }
+    return u;
+}
+
+// the mapped-inflow ghost value of cell (j, i) at height h (typeU 3: boundary.c:1302-1371;
+// typeU 4: :1372-1622)
+__device__ V3 mapped_inflow(const InletArgs &In, const Geo &g, int j, int i, double h) {
+    auto P = [&](int pj, int pi) { return ld3(In.plane, pj * (In.n2 + 2) + pi); };
+    if (In.type == TOSCA_INLET_MAPPED) {
+        int jif = j % In.n1 == 0 ? In.n1 : j % In.n1;
+        int iif = i % In.n2 == 0 ? In.n2 : i % In.n2;
+        if (j <= In.n1 * In.prds1 && i <= In.n2 * In.prds2) return In.merge1 ? merge_top(In, h, P(jif, iif)) : P(jif, iif);
+        if (In.merge1) return V3{In.top_u[9][0], In.top_u[9][1], In.top_u[9][2]};
+        if (j > In.n1 * In.prds1) jif = In.n1;
+        if (i > In.n2 * In.prds2) iif = In.n2;
+        return P(jif, iif);
+    }
+    V3 u;
+    if (In.shift2) {
+        const int ir = min(max(In.yids[i], 1), g.mx - 1), il = ir - 1;
+        const V3 uL = interp_cell(In, (long)j * g.mx + il), uR = interp_cell(In, (long)j * g.mx + ir);
+        const double yw = In.yw[i], ow = 1.0 - yw;
+        u = V3{ow * uL.x + yw * uR.x, ow * uL.y + yw * uR.y, ow * uL.z + yw * uR.z};
+    } else {
+        u = interp_cell(In, (long)j * g.mx + i);
+    }
+    return In.merge1 ? merge_top(In, h, u) : u;
+}
+}  // namespace
 
 // ---- the Nieuwstadt inflow profile (src/inflow.c:2732-2811) -----------------------------------------
 // Complex arithmetic of its own (no std::complex in device code): products and sums as written,
@@ -454,6 +562,13 @@ __global__ void k_cart_bcs(double *ucat, const double *csi, const double *eta, c
         double uMag = 0.0;
         if (In.type == TOSCA_INLET_NIEUWSTADT) {                         // :1287-1292
             out = nieuwstadt(In, cc.z - In.grnd);
+        } else if (In.type == TOSCA_INLET_POWERLAW) {                    // :1233-1252
+            const double h = cc.z - In.grnd, alpha = 0.107027;
+            const double pw = pow(h / In.href, alpha);
+            const double turb = 1 + double(In.noise[(long)c[1] * g.mx + c[0]]) / 10000. * In.uprime;
+            out = V3{pw * In.uref[0] * turb, pw * In.uref[1] * turb, pw * In.uref[2] * turb};
+        } else if (In.type == TOSCA_INLET_MAPPED || In.type == TOSCA_INLET_INTERP) {
+            out = mapped_inflow(In, g, c[1], c[0], cc.z - In.grnd);
         } else if (In.type == TOSCA_INLET_LOGLAW) {                      // :1254-1286
             const double h = cc.z - In.grnd, vk = 0.4;
             if (h <= In.h_inv) {
@@ -466,7 +581,8 @@ __global__ void k_cart_bcs(double *ucat, const double *csi, const double *eta, c
             const double fraction = (cc.y - In.ymin) / In.ly;
             uMag = (1.0 + In.amp * cos(In.periods * 2.0 * 3.14159265358979323846 * fraction)) * In.uref_mag;
         }
-        if (In.type != TOSCA_INLET_NIEUWSTADT) out = V3{uMag * In.udir[0], uMag * In.udir[1], uMag * In.udir[2]};
+        if (In.type == TOSCA_INLET_LOGLAW || In.type == TOSCA_INLET_SINUSOIDAL)
+            out = V3{uMag * In.udir[0], uMag * In.udir[1], uMag * In.udir[2]};
     } else if (b == TOSCA_BCU_ZEROGRADIENT) {                            // :1869-1928
         const V3 U = ld3(ucat, ci), U2 = ld3(ucat, g.L(c2[2], c2[1], c2[0]));
         out = V3{2 * U.x - U2.x, 2 * U.y - U2.y, 2 * U.z - U2.z};
@@ -565,6 +681,17 @@ void launch_cartesian_bcs(double *ucat, const double *u_fixed, double *ustar, co
         for (int d = 0; d < 3; d++) In.udir[d] = inlet->udir[d];
         const double *u = inlet->uref;
         In.uref_mag = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);   // nMag(Uref)
+        for (int d = 0; d < 3; d++) In.uref[d] = u[d];
+        In.href = inlet->href; In.uprime = inlet->uprime_rms; In.noise = inlet->noise;
+        In.plane = inlet->plane; In.n1 = inlet->n1; In.n2 = inlet->n2; In.prds1 = inlet->prds1; In.prds2 = inlet->prds2;
+        In.merge1 = inlet->merge1; In.spline = inlet->spline; In.shift2 = inlet->shift2;
+        In.top_len = inlet->avg_top_length; In.top_delta = inlet->avg_top_delta;
+        for (int n = 0; n < 10; n++) {
+            In.top_z[n] = inlet->avg_top_coords[n];
+            for (int d = 0; d < 3; d++) In.top_u[n][d] = inlet->avg_top_u[n][d];
+        }
+        In.cells = inlet->cells; In.cells1 = inlet->cells_1; In.cells2 = inlet->cells_2; In.yids = inlet->y_ids;
+        In.w = inlet->weights; In.w1 = inlet->weights_1; In.w2 = inlet->weights_2; In.yw = inlet->y_weights;
     }
     const int n = maxdim(g);
     PatchSet P = patches(p->bc_u, u_fixed, 18);

@@ -183,7 +183,8 @@ class RhsContext:
     def UpdateCartesianBCs(self, mesh, params, ucat, u_fixed=None, ustar=None, inlet=None):
         """UpdateCartesianBCs(ueqn): ghost cells of ucat, in place; u_fixed: 6 x 3 fixedValue vectors;
         ustar: local scalar array (lUstar) the Cabot wall-function patches write; inlet: dict of the
-        kLeft inlet-function profile (abi.make_inlet keys, cent a CUDA tensor)."""
+        kLeft inlet-function profile (abi.make_inlet keys; its arrays CUDA tensors: cent float64,
+        plane / weights* / y_weights float64, noise / cells* / y_ids int32)."""
         self._stream()
         m, p = self._mesh(mesh), abi.make_params(params)
         uf = None
@@ -191,7 +192,9 @@ class RhsContext:
             uf = (C.c_double * 18)(*[float(x) for row in u_fixed for x in row])
         il = None
         if inlet is not None:
-            il = abi.make_inlet(dict(inlet, cent=_ptr(inlet.get("cent"))))
+            ints = ("noise", "cells", "cells_1", "cells_2", "y_ids")
+            il = abi.make_inlet(dict(inlet, **{k: _ptr(inlet.get(k), torch.int32 if k in ints else torch.float64)
+                                               for k in abi._INLET_PTRS}))
         self._check(self.lib.tosca_update_cartesian_bcs(self.h, C.byref(m), C.byref(p), _ptr(ucat),
                                                         C.cast(uf, C.c_void_p) if uf is not None else None,
                                                         _ptr(ustar), C.cast(C.pointer(il), C.c_void_p) if il is not None else None))
