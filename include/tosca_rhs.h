@@ -157,7 +157,13 @@ int tosca_ctx_timing_entry(tosca_ctx *ctx, int idx, char *name, int name_len, do
  * entry point below that takes a mesh returns TOSCA_ERR_MESH unless a mesh with the same nvert,
  * meshtag, aj, csi, eta and zet pointers passed this check on the same context (the context keeps
  * the last 64 such meshes; a failed check removes the mesh).  The check does not see later writes
- * into those arrays: the mesh is static in the reference (SetMeshMetrics runs once). */
+ * into those arrays: the mesh is static in the reference (SetMeshMetrics runs once).
+ * When icsi, jeta and kzet are given it also records whether every interior cell's csi, eta, zet
+ * equal 0.5 f + 0.5 f' of the face bases icsi, jeta, kzet on its two faces of that direction, bit
+ * for bit, as SetMeshMetrics forms them (mesh.c:925-960, the face bases and the cells' dummy bases
+ * being the same cross products).  For such a mesh FormU's face dot on FP64 metrics forms the cell
+ * bases from the face bases it reads anyway instead of loading them (the same values); for any
+ * other mesh, and on FP32 metric copies, it loads them. */
 int tosca_mesh_check(tosca_ctx *ctx, const tosca_mesh *mesh);
 
 /* FormU (src/solvers/ueqn_solvers.c:6-1083).

@@ -42,6 +42,9 @@ struct tosca_ctx {
     // meshes tosca_mesh_check accepted (their nvert, meshtag, aj, csi, eta, zet pointers): the
     // kernels do not read nvert / meshTag, so every entry point that takes a mesh requires one
     std::vector<std::array<const void *, 6>> checked;
+    // the checked meshes (with their icsi, jeta, kzet pointers) whose cell-centred csi, eta, zet are
+    // the averages of the face bases on their two faces, bit for bit: FormU forms them itself
+    std::vector<std::array<const void *, 9>> derivable;
     tosca::KernelTimer timer;
     bool ref_path = false;   // TOSCA_PATH=ref at creation: reference-structured kernels only
     double *scratch = nullptr;   // this device's copy of `scratch` (tosca_formu_noalloc), or null
@@ -140,6 +143,18 @@ int require_checked(tosca_ctx *c, const tosca_mesh *m) {
                                       "are validated there once, not read per call)");
 }
 
+std::array<const void *, 9> derive_key(const tosca_mesh *m) {
+    return {m->nvert, m->meshtag, m->aj, m->csi, m->eta, m->zet, m->icsi, m->jeta, m->kzet};
+}
+
+// 1 when tosca_mesh_check found this mesh's cell bases to be the face averages
+int cc_derivable(const tosca_ctx *c, const tosca_mesh *m) {
+    const auto key = derive_key(m);
+    for (const auto &k : c->derivable)
+        if (k == key) return 1;
+    return 0;
+}
+
 int check_ctx(tosca_ctx *c) {
     if (!c) return set_err(nullptr, TOSCA_ERR_ARG, "context is NULL");
     if (hipSetDevice(c->device) != hipSuccess) return set_err(c, TOSCA_ERR_DEVICE, "hipSetDevice(%d) failed", c->device);
@@ -220,9 +235,10 @@ void set_geo_per(tosca_ctx *c, const tosca_params *p) {
     for (int a = 0; a < 3; a++) c->geo.per[a] = p->periodic[a];
 }
 
-tosca::FormUArgs formu_args(const tosca_mesh *m, const tosca_params *p, const double *ucont, const double *ucat,
-                            const double *nut, double *rhs, double scale, int mode) {
+tosca::FormUArgs formu_args(const tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const double *ucont,
+                            const double *ucat, const double *nut, double *rhs, double scale, int mode) {
     tosca::FormUArgs A{};
+    A.derive_cc = cc_derivable(c, m);
     A.ucont = ucont; A.ucat = ucat; A.nut = nut;
     A.csi = m->csi; A.eta = m->eta; A.zet = m->zet;
     A.icsi = m->icsi; A.ieta = m->ieta; A.izet = m->izet;
@@ -302,6 +318,8 @@ void queue_step(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const 
     A.btheta = btheta;
     A.bscale = buoy_scale;
     A.tRef = p->t_ref;
+    // FP32 metric copies: the cell bases are loaded (forming them there measured slower)
+    A.derive_cc = std::is_same<MT, double>::value ? cc_derivable(c, m) : 0;
     tosca::fused_formu<T, MT>(g, A, ws + 3 * (size_t)g.nloc(), s);
 }
 
@@ -411,7 +429,7 @@ void queue_explicit(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, do
     if (c->ref_path) {
         tosca::ref_formu(g, m, p, ucont, ucat, nu_t, rhs, 1.0, form_mode, c->ws, s);
     } else {
-        tosca::FormUArgs A = formu_args(m, p, ucont, ucat, nu_t, rhs, 1.0, form_mode);
+        tosca::FormUArgs A = formu_args(c, m, p, ucont, ucat, nu_t, rhs, 1.0, form_mode);
         tosca::fused_formu(g, A, c->ws + 3 * (size_t)g.nloc(), s);
     }
     tosca::launch_non_resolved(rhs, ucont_o, p, g, s);               // :1403
@@ -434,7 +452,7 @@ void queue_imex(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const 
     if (c->ref_path) {                                                     // :1794
         tosca::ref_formu(g, m, p, ucont, ucat, nu_t, av, -1.0 * coef, 2, c->ws, s);
     } else {
-        tosca::FormUArgs A = formu_args(m, p, ucont, ucat, nu_t, av, -1.0 * coef, 2);
+        tosca::FormUArgs A = formu_args(c, m, p, ucont, ucat, nu_t, av, -1.0 * coef, 2);
         tosca::fused_formu(g, A, c->ws + 3 * (size_t)g.nloc(), s);
     }
     tosca::launch_non_resolved(av, ucont_o, p, g, s);                      // :1795
@@ -724,29 +742,43 @@ int tosca_mesh_check(tosca_ctx *c, const tosca_mesh *m) {
     if ((rc = check_mesh(c, m, false, true, false, false))) return rc;
     const size_t s = (size_t)c->geo.nloc() * sizeof(double);
     if ((rc = check_dev(c, m->nvert, s, "mesh.nvert")) || (rc = check_dev(c, m->meshtag, s, "mesh.meshtag"))) return rc;
+    tosca_mesh mc = *m;             // the face bases of the cell-basis identity, when all three are given
+    if (m->icsi && m->jeta && m->kzet) {
+        if ((rc = check_dev(c, m->icsi, 3 * s, "mesh.icsi")) || (rc = check_dev(c, m->jeta, 3 * s, "mesh.jeta")) ||
+            (rc = check_dev(c, m->kzet, 3 * s, "mesh.kzet")))
+            return rc;
+    } else {
+        mc.icsi = mc.jeta = mc.kzet = nullptr;
+    }
     const auto key = mesh_key(m);
+    const auto dkey = derive_key(m);
     c->checked.erase(std::remove(c->checked.begin(), c->checked.end(), key), c->checked.end());
-    const size_t need = 4 * (size_t)(c->geo.mz) + 8;
+    c->derivable.erase(std::remove(c->derivable.begin(), c->derivable.end(), dkey), c->derivable.end());
+    const size_t need = 5 * (size_t)(c->geo.mz) + 8;
     if (c->chk_elems < need) {
         if (c->chk) hipFree(c->chk);
         c->chk = nullptr;
         if (hipMalloc(&c->chk, need * sizeof(int)) != hipSuccess) return set_err(c, TOSCA_ERR_DEVICE, "allocation failed");
         c->chk_elems = need;
     }
-    tosca::launch_mesh_check(c->geo, m, c->chk, c->stream);
+    tosca::launch_mesh_check(c->geo, &mc, c->chk, c->stream);
     if ((rc = launched(c, "mesh check"))) return rc;
-    std::vector<int> h(4 * c->geo.mz);
+    std::vector<int> h(5 * c->geo.mz);
     hipMemcpyAsync(h.data(), c->chk, h.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream);
     if (hipStreamSynchronize(c->stream) != hipSuccess) return set_err(c, TOSCA_ERR_DEVICE, "mesh check failed");
-    long cnt[4] = {0, 0, 0, 0};
+    long cnt[5] = {0, 0, 0, 0, 0};
     for (int k = 0; k < c->geo.mz; k++)
-        for (int q = 0; q < 4; q++) cnt[q] += h[4 * k + q];
+        for (int q = 0; q < 5; q++) cnt[q] += h[5 * k + q];
     if (cnt[0]) return set_err(c, TOSCA_ERR_MESH, "%ld points with nvert != 0 (immersed boundaries are not supported)", cnt[0]);
     if (cnt[1]) return set_err(c, TOSCA_ERR_MESH, "%ld points with meshTag != 0 (overset meshes are not supported)", cnt[1]);
     if (cnt[2]) return set_err(c, TOSCA_ERR_MESH, "%ld interior cells with negative or close-to-zero volume (aj < 1e-20)", cnt[2]);
     if (cnt[3]) return set_err(c, TOSCA_ERR_MESH, "%ld interior cells with a close-to-zero face area (|csi|,|eta|,|zet| < 1e-20)", cnt[3]);
     if (c->checked.size() >= 64) c->checked.erase(c->checked.begin());
     c->checked.push_back(key);
+    if (m->icsi && m->jeta && m->kzet && cnt[4] == 0) {
+        if (c->derivable.size() >= 64) c->derivable.erase(c->derivable.begin());
+        c->derivable.push_back(dkey);
+    }
     return TOSCA_OK;
 }
 
@@ -779,7 +811,7 @@ int tosca_formu(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const 
         tosca::ref_formu(g, m, p, ucont, ucat, (p->les_active && visc) ? nut : nullptr, rhs, scale, form_mode, c->ws, c->stream);
     } else {
         if ((rc = ensure_ws(c, ws_need(c, p)))) return rc;
-        tosca::FormUArgs A = formu_args(m, p, ucont, ucat, (p->les_active && visc) ? nut : nullptr, rhs, scale, form_mode);
+        tosca::FormUArgs A = formu_args(c, m, p, ucont, ucat, (p->les_active && visc) ? nut : nullptr, rhs, scale, form_mode);
         tosca::fused_formu(g, A, c->ws + 3 * (size_t)g.nloc(), c->stream);
     }
     return launched(c, "FormU");
@@ -815,7 +847,7 @@ int tosca_formu_noalloc(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p
     }
     if ((rc = check_dev(c, rhs, 3 * (size_t)g.mx * g.my * g.mz * sizeof(double), "rhs"))) return rc;
     // the fused kernels write every scratch element they read before reading it: nothing to clear
-    tosca::FormUArgs A = formu_args(m, p, ucont, ucat, (p->les_active && visc) ? nut : nullptr, rhs, scale, form_mode);
+    tosca::FormUArgs A = formu_args(c, m, p, ucont, ucat, (p->les_active && visc) ? nut : nullptr, rhs, scale, form_mode);
     tosca::fused_formu(g, A, c->scratch, c->stream);
     return launched(c, "FormU (allocation-free)");
 }
@@ -1056,7 +1088,7 @@ int tosca_ueqn_rk3cn(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, i
         if (c->ref_path) {
             tosca::ref_formu(g, m, p, ucont, ucat, nu_t, visc, 1.0, 2, c->ws, s);
         } else {
-            tosca::FormUArgs A = formu_args(m, p, ucont, ucat, nu_t, visc, 1.0, 2);
+            tosca::FormUArgs A = formu_args(c, m, p, ucont, ucat, nu_t, visc, 1.0, 2);
             tosca::fused_formu(g, A, c->ws + 3 * (size_t)g.nloc(), s);
         }
         tosca::launch_non_resolved(visc, ucont_o, p, g, s);
@@ -1114,7 +1146,7 @@ int tosca_ueqn_abcn(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, do
         if (c->ref_path) {
             tosca::ref_formu(g, m, p, ucont, ucat, nu_t, out, 1.0, mode, c->ws, s);
         } else {
-            tosca::FormUArgs A = formu_args(m, p, ucont, ucat, nu_t, out, 1.0, mode);
+            tosca::FormUArgs A = formu_args(c, m, p, ucont, ucat, nu_t, out, 1.0, mode);
             tosca::fused_formu(g, A, c->ws + 3 * (size_t)g.nloc(), s);
         }
         tosca::launch_non_resolved(out, ucont_o, p, g, s);
@@ -1194,7 +1226,7 @@ int tosca_snes_func_eval(tosca_ctx *c, const tosca_mesh *m, const tosca_params *
     if (c->ref_path) {                                                        // :1230
         tosca::ref_formu(g, m, p, ucont, ucat, nu_t, F, scale, 0, c->ws, s);
     } else {
-        tosca::FormUArgs A = formu_args(m, p, ucont, ucat, nu_t, F, scale, 0);
+        tosca::FormUArgs A = formu_args(c, m, p, ucont, ucat, nu_t, F, scale, 0);
         tosca::fused_formu(g, A, c->ws + 3 * (size_t)g.nloc(), s);
     }
     tosca::launch_snes_finish(F, u, ucont_o, dt, p, g, s);                    // :1264-1271
@@ -1706,10 +1738,14 @@ int tosca_rhs_vjp_mixed(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p
 // ---------------------------------------------------------------------------------------------
 namespace tosca {
 
+// counters per plane: nvert != 0, meshTag != 0, aj < 1e-20 and a close-to-zero face area at interior
+// cells, and interior cells whose csi, eta, zet differ from 0.5 a + 0.5 b of the face bases on
+// their two faces (or every interior cell, when icsi / jeta / kzet are not given)
 __global__ void k_mesh_check(Geo g, const double *nvert, const double *meshtag, const double *aj, const double *csi,
-                             const double *eta, const double *zet, int *out) {
+                             const double *eta, const double *zet, const double *icsi, const double *jeta,
+                             const double *kzet, int *out) {
     const int k = blockIdx.x;
-    int cnt[4] = {0, 0, 0, 0};
+    int cnt[5] = {0, 0, 0, 0, 0};
     const int n = g.my * g.mx;
     for (int t = threadIdx.x; t < n; t += blockDim.x) {
         int j = t / g.mx, i = t % g.mx;
@@ -1723,22 +1759,33 @@ __global__ void k_mesh_check(Geo g, const double *nvert, const double *meshtag, 
             double na = sqrt(a.x * a.x + a.y * a.y + a.z * a.z), nb = sqrt(b.x * b.x + b.y * b.y + b.z * b.z),
                    nd = sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
             if (!(na >= 1e-20) || !(nb >= 1e-20) || !(nd >= 1e-20)) cnt[3]++;
+            if (!icsi || !jeta || !kzet) {
+                cnt[4]++;
+            } else {
+                auto avg_is = [](V3 cc, V3 f, V3 fl) {
+                    return cc.x == 0.5 * f.x + 0.5 * fl.x && cc.y == 0.5 * f.y + 0.5 * fl.y && cc.z == 0.5 * f.z + 0.5 * fl.z;
+                };
+                if (!avg_is(a, ld3(icsi, c), ld3(icsi, g.L(k, j, i - 1))) || !avg_is(b, ld3(jeta, c), ld3(jeta, g.L(k, j - 1, i))) ||
+                    !avg_is(d, ld3(kzet, c), ld3(kzet, g.L(k - 1, j, i))))
+                    cnt[4]++;
+            }
         }
     }
-    __shared__ int red[4][256];
-    for (int q = 0; q < 4; q++) red[q][threadIdx.x] = cnt[q];
+    __shared__ int red[5][256];
+    for (int q = 0; q < 5; q++) red[q][threadIdx.x] = cnt[q];
     __syncthreads();
     for (int w = blockDim.x / 2; w > 0; w >>= 1) {
         if (threadIdx.x < w)
-            for (int q = 0; q < 4; q++) red[q][threadIdx.x] += red[q][threadIdx.x + w];
+            for (int q = 0; q < 5; q++) red[q][threadIdx.x] += red[q][threadIdx.x + w];
         __syncthreads();
     }
     if (threadIdx.x == 0)
-        for (int q = 0; q < 4; q++) out[4 * k + q] = red[q][0];
+        for (int q = 0; q < 5; q++) out[5 * k + q] = red[q][0];
 }
 
 void launch_mesh_check(const Geo &g, const tosca_mesh *m, int *out, hipStream_t s) {
-    TOSCA_LAUNCH("k_mesh_check", k_mesh_check, dim3(g.mz), dim3(256), 0, s, g, m->nvert, m->meshtag, m->aj, m->csi, m->eta, m->zet, out);
+    TOSCA_LAUNCH("k_mesh_check", k_mesh_check, dim3(g.mz), dim3(256), 0, s, g, m->nvert, m->meshtag, m->aj, m->csi, m->eta,
+                 m->zet, m->icsi, m->jeta, m->kzet, out);
 }
 
 }  // namespace tosca

@@ -28,6 +28,11 @@ struct FormUArgsT {
     const T *theta;
     double *btheta;
     double bscale, tRef;
+    // 1: the cell-centred csi, eta, zet of every interior cell equal 0.5 a + 0.5 b of the face
+    // bases icsi, jeta, kzet on its two faces of that direction (how SetMeshMetrics forms them,
+    // mesh.c:925-960, checked bit for bit by tosca_mesh_check); the face dot then forms them from
+    // the face data it already holds instead of loading them
+    int derive_cc;
 };
 using FormUArgs = FormUArgsT<double>;
 

@@ -308,7 +308,19 @@ __device__ __forceinline__ FluxT<T> face_main(const MainArgs<T, MT> &A, const Ri
     return r;
 }
 
-template <int NR, class T, class MT>
+// 0.5 a + 0.5 b in double: a cell-centred basis vector from the face bases on its two faces
+// (AxByC(0.5, own face, 0.5, lower face), mesh.c:938-954; 0.5 x is exact, so the FMA the compiler
+// may form rounds the same)
+template <class MT>
+__device__ __forceinline__ V3 face_avg(Vec3<MT> a, Vec3<MT> b) {
+    return V3{0.5 * double(a.x) + 0.5 * double(b.x), 0.5 * double(a.y) + 0.5 * double(b.y),
+              0.5 * double(a.z) + 0.5 * double(b.z)};
+}
+
+// DER: the cell-centred csi, eta, zet of the face dot are formed from the face bases icsi, jeta,
+// kzet the plane steps load anyway (valid for meshes whose cell bases are those averages; ghost
+// neighbours still load theirs)
+template <int NR, class T, class MT, bool DER>
 __global__ void __launch_bounds__(64 * NR) k_fused_formu(MainArgs<T, MT> A, Geo g, int kchunk) {
     // Every thread evaluates its three faces and fp on every plane (threads outside the tile's
     // useful range work on clamped, in-bounds addresses and their results are discarded), so the
@@ -317,6 +329,8 @@ __global__ void __launch_bounds__(64 * NR) k_fused_formu(MainArgs<T, MT> A, Geo 
     constexpr int TI = 62, TJ = NR - 1;
     __shared__ T sF2[6][NR][64];          // j-face fluxes of the plane (div xyz, visc xyz)
     __shared__ double sFp[2][3][NR][64];  // fp of the previous plane (double-buffered)
+    __shared__ MT sEt[2][3][NR][DER ? 64 : 1];   // jeta of the plane (DER; double-buffered)
+    __shared__ MT sEc[3][NR][DER ? 64 : 1];      // icsi of the plane (DER; read by its own wave only)
     __shared__ Ring<NR, T> ring;          // ucat planes t-1 .. t+2
     using RingT = Ring<NR, T>;
     constexpr int PLANE = RingT::RJ * RingT::RI * 3;
@@ -415,6 +429,10 @@ __global__ void __launch_bounds__(64 * NR) k_fused_formu(MainArgs<T, MT> A, Geo 
     const int cp0 = c0 + (kb - 1) * SK;
     double fa0p = A.iaj[cp0], fa1p = A.jaj[cp0], fa2p = A.kaj[cp0];
     double gzIp = A.icsi[3 * cp0 + 2], gzJp = A.jeta[3 * cp0 + 2], gzKp = A.kzet[3 * cp0 + 2];
+    // DER: kzet of plane t-1 (of plane kb-1 for the first step's zet); icsi and jeta of plane t-1
+    // are read back from their LDS copies
+    Vec3<MT> fzP{0, 0, 0};
+    if constexpr (DER) fzP = ld3(A.kzet, cp0);
     // With FP32 metric copies the face data of the next plane step fit in the registers the kernel
     // has left at its occupancy (2 waves per SIMD), so they are loaded one step ahead; with FP64
     // metrics they would spill.
@@ -473,12 +491,20 @@ __global__ void __launch_bounds__(64 * NR) k_fused_formu(MainArgs<T, MT> A, Geo 
         double th = 0, thI = 0, thJ = 0, thK = 0;
         if (out) {
             // the upper cell's zet and theta: this step's k neighbours, the next step's own values
-            zeB = cvt3<double>(ld3(A.zet, cpq + SK));
+            if constexpr (DER) zeB = ghostK ? cvt3<double>(ld3(A.zet, cpq + SK)) : face_avg(in3.fz, fzP);
+            else zeB = cvt3<double>(ld3(A.zet, cpq + SK));
             if (buoy) thK = A.theta[cpq + SK];
         }
         if (rhsStep) {
-            csA = cvt3<double>(ld3(A.csi, cpq)); csB = cvt3<double>(ld3(A.csi, cpq + 1));
-            etA = cvt3<double>(ld3(A.eta, cpq)); etB = cvt3<double>(ld3(A.eta, cpq + SJ));
+            if constexpr (DER) {
+                // the ghost neighbours of the last interior cells keep their own (mirrored or
+                // periodic) bases; the other bases are formed below
+                if (i == g.mx - 2) csB = cvt3<double>(ld3(A.csi, cpq + 1));
+                if (j == g.my - 2) etB = cvt3<double>(ld3(A.eta, cpq + SJ));
+            } else {
+                csA = cvt3<double>(ld3(A.csi, cpq)); csB = cvt3<double>(ld3(A.csi, cpq + 1));
+                etA = cvt3<double>(ld3(A.eta, cpq)); etB = cvt3<double>(ld3(A.eta, cpq + SJ));
+            }
             zeA = zeNext;
             r0 = A.rhs[ro]; r1 = A.rhs[ro + 1]; r2 = A.rhs[ro + 2];
             if (A.farm) {
@@ -492,6 +518,7 @@ __global__ void __launch_bounds__(64 * NR) k_fused_formu(MainArgs<T, MT> A, Geo 
         const Vec3<T> F1dm = shup3(F1.d), F1vm = shup3(F1.v);
         sF2[0][r][lane] = F2.d.x; sF2[1][r][lane] = F2.d.y; sF2[2][r][lane] = F2.d.z;
         sF2[3][r][lane] = F2.v.x; sF2[4][r][lane] = F2.v.y; sF2[5][r][lane] = F2.v.z;
+        if constexpr (DER) { sEt[buf][0][r][lane] = in2.fe.x; sEt[buf][1][r][lane] = in2.fe.y; sEt[buf][2][r][lane] = in2.fe.z; }
         __syncthreads();
         const int rm = r > 0 ? r - 1 : 0;
         const Vec3<T> F2dm{sF2[0][rm][lane], sF2[1][rm][lane], sF2[2][rm][lane]};
@@ -520,6 +547,23 @@ __global__ void __launch_bounds__(64 * NR) k_fused_formu(MainArgs<T, MT> A, Geo 
             const double *qq = A.sJ + 3 * (ia + (t - 1) * g.mx);
             fJ = V3{qq[0], qq[1], qq[2]};
         }
+        Vec3<MT> fcP{0, 0, 0}, feP{0, 0, 0};
+        if constexpr (DER) {
+            // csi of cells i, i+1 from icsi of faces i-1, i, i+1 (lanes), eta of rows j, j+1 from
+            // jeta of rows j-1, j, j+1 (the previous step's LDS copy), all of plane t-1
+            fcP = Vec3<MT>{sEc[0][r][lane], sEc[1][r][lane], sEc[2][r][lane]};
+            const Vec3<MT> fcL = shup3(fcP), fcU = shdn3(fcP);
+            const int rmE = r > 0 ? r - 1 : 0, rpE = r + 1 < NR ? r + 1 : r;
+            feP = Vec3<MT>{sEt[buf ^ 1][0][r][lane], sEt[buf ^ 1][1][r][lane], sEt[buf ^ 1][2][r][lane]};
+            const Vec3<MT> feM{sEt[buf ^ 1][0][rmE][lane], sEt[buf ^ 1][1][rmE][lane], sEt[buf ^ 1][2][rmE][lane]};
+            const Vec3<MT> feN{sEt[buf ^ 1][0][rpE][lane], sEt[buf ^ 1][1][rpE][lane], sEt[buf ^ 1][2][rpE][lane]};
+            if (rhsStep) {
+                csA = face_avg(fcP, fcL);
+                if (i != g.mx - 2) csB = face_avg(fcU, fcP);
+                etA = face_avg(feP, feM);
+                if (j != g.my - 2) etB = face_avg(feN, feP);
+            }
+        }
         if (rhsStep) {
             const V3 fK = fp;
             double v1 = csA.x * fpPrev.x + csA.y * fpPrev.y + csA.z * fpPrev.z;
@@ -541,9 +585,11 @@ __global__ void __launch_bounds__(64 * NR) k_fused_formu(MainArgs<T, MT> A, Geo 
                 // Boussinesq source on the three + faces of the cell (ueqn_sources.c:2812-2870):
                 // g . S with g = (0, 0, -9.81) is -9.81 * S.z exactly
                 const double tI = 0.5 * (th + thI), tJ = 0.5 * (th + thJ), tK = 0.5 * (th + thK);
-                A.btheta[ro] = A.bscale * ((-9.81 * gzIp) * (A.tRef - tI) / A.tRef);
-                A.btheta[ro + 1] = A.bscale * ((-9.81 * gzJp) * (A.tRef - tJ) / A.tRef);
-                A.btheta[ro + 2] = A.bscale * ((-9.81 * gzKp) * (A.tRef - tK) / A.tRef);
+                const double gzI = DER ? double(fcP.z) : gzIp, gzJ = DER ? double(feP.z) : gzJp,
+                             gzK = DER ? double(fzP.z) : gzKp;
+                A.btheta[ro] = A.bscale * ((-9.81 * gzI) * (A.tRef - tI) / A.tRef);
+                A.btheta[ro + 1] = A.bscale * ((-9.81 * gzJ) * (A.tRef - tJ) / A.tRef);
+                A.btheta[ro + 2] = A.bscale * ((-9.81 * gzK) * (A.tRef - tK) / A.tRef);
             }
         }
         sFp[buf][0][r][lane] = fp.x; sFp[buf][1][r][lane] = fp.y; sFp[buf][2][r][lane] = fp.z;
@@ -553,7 +599,12 @@ __global__ void __launch_bounds__(64 * NR) k_fused_formu(MainArgs<T, MT> A, Geo 
         fpPrev = fp;
         F3p = F3;
         fa0p = in1.fa; fa1p = in2.fa; fa2p = in3.fa;
-        gzIp = in1.fc.z; gzJp = in2.fe.z; gzKp = in3.fz.z;
+        if constexpr (DER) {
+            // this wave's row only: its reads of the plane's icsi above precede this write
+            sEc[0][r][lane] = in1.fc.x; sEc[1][r][lane] = in1.fc.y; sEc[2][r][lane] = in1.fc.z;
+            fzP = in3.fz;
+        }
+        else { gzIp = in1.fc.z; gzJp = in2.fe.z; gzKp = in3.fz.z; }
         zeNext = zeB;
         thNext = thK;
         nutNext = in3.nutR;
@@ -614,7 +665,15 @@ void fused_formu(const Geo &g, const FormUArgsT<T, MT> &A, double *ws, hipStream
     const int kchunk = (nk + chunks - 1) / chunks;
     chunks = (nk + kchunk - 1) / kchunk;
     dim3 grid((g.mx - 2 + TI - 1) / TI, (g.my - 2 + TJ - 1) / TJ, chunks);
-    TOSCA_LAUNCH(sizeof(T) == 8 ? "k_fused_formu" : sizeof(MT) == 8 ? "k_fused_formu_f32" : "k_fused_formu_f32m", (k_fused_formu<NR, T, MT>), grid, dim3(64, NR), 0, s, M, g, kchunk);
+    const char *kname = sizeof(T) == 8 ? "k_fused_formu" : sizeof(MT) == 8 ? "k_fused_formu_f32" : "k_fused_formu_f32m";
+    // the cell bases formed from the face bases on FP64 metrics only (on the FP32 copies, whose
+    // kernel is latency-bound, that measured slower than loading them)
+    bool derive = false;
+    if constexpr (std::is_same<MT, double>::value) derive = A.derive_cc != 0;
+    if constexpr (std::is_same<MT, double>::value) {
+        if (derive) TOSCA_LAUNCH(kname, (k_fused_formu<NR, T, MT, true>), grid, dim3(64, NR), 0, s, M, g, kchunk);
+    }
+    if (!derive) TOSCA_LAUNCH(kname, (k_fused_formu<NR, T, MT, false>), grid, dim3(64, NR), 0, s, M, g, kchunk);
     // the farm AXPY on the owned points outside the interior cells (the kernel covers the interior)
     if (A.farm) shell_add(A.rhs, A.farm, g.mx, g.my, g.mz, s);
     if (nsj > 1)
