@@ -387,8 +387,12 @@ __device__ __forceinline__ V3 pk3(const P *pk, long n, int w, int c) {
 // ------------------------------------------------------------------------------------------------
 template <class T, class P>
 __global__ void __launch_bounds__(256) k_vjp_gather(VjpArgs<T, P> A, Geo g) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
-    if (i > g.mx - 1) return;
+    // a workgroup is 64 points along i on 4 consecutive k planes (which read each other's k
+    // neighbours' packets); consecutive workgroups on one XCD take consecutive j rows
+    const int tile = xcd_tile();
+    const int bx = tile % gridDim.x, j = (tile / gridDim.x) % g.my, bk = tile / (gridDim.x * g.my);
+    const int i = bx * 64 + threadIdx.x, k = bk * 4 + threadIdx.y;
+    if (i > g.mx - 1 || k > g.mz - 1) return;
     const int x[3] = {i, j, k}, m[3] = {g.mx, g.my, g.mz};
     const int c = g.L(k, j, i);
     auto Lof = [&](const int *t) { return g.L(t[2], t[1], t[0]); };
@@ -543,8 +547,8 @@ void vjp_rhs_impl(const Geo &g, const tosca_mesh *mesh, const tosca_params *p, c
     if (les && !nut_in) TOSCA_LAUNCH(f32 ? "k_vjp_nut_f32" : "k_vjp_nut", (k_vjp_nut<T, P>), cg, cb, 0, s, A, g);
     if (les && nut_in)
         TOSCA_LAUNCH("k_vjp_nutbar", (k_vjp_nutbar<T, P>), dim3((g.mx + 255) / 256, g.my, g.mz), dim3(256), 0, s, A, g);
-    TOSCA_LAUNCH(f32 ? "k_vjp_gather_f32" : "k_vjp_gather", (k_vjp_gather<T, P>), dim3((g.mx + 255) / 256, g.my, g.mz),
-                 dim3(256), 0, s, A, g);
+    TOSCA_LAUNCH(f32 ? "k_vjp_gather_f32" : "k_vjp_gather", (k_vjp_gather<T, P>), dim3((g.mx + 63) / 64, (g.mz + 3) / 4, g.my),
+                 dim3(64, 4), 0, s, A, g);
 }
 
 void vjp_rhs(const Geo &g, const tosca_mesh *mesh, const tosca_params *p, const double *ucont, const double *ucat,
