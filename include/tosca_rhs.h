@@ -254,8 +254,10 @@ typedef struct {
 /* tosca_rhs_les_step_mixed with FormU reading the FP32 metric copies of mesh32 instead of the FP64
  * metrics (no reference counterpart).  FormU's flux arithmetic already runs in FP32 on FP32-rounded
  * face metrics; here the face dot into rhs and the buoyancy source also see FP32-rounded metrics
- * and Jacobians (arithmetic still FP64).  UpdateCs / UpdateNut read the FP64 mesh.  FormU's
- * metric bytes halve (312 -> 156 B/cell). */
+ * and Jacobians (arithmetic still FP64).  The dynamic Smagorinsky filter reads the FP32 csi, eta,
+ * zet too (its strain already used them rounded to FP32; its Germano contraction now sees them
+ * rounded); aj, the box average, DPASD and UpdateNut read the FP64 mesh.  FormU's metric bytes
+ * halve (312 -> 156 B/cell). */
 int tosca_rhs_les_step_mixed_m32(tosca_ctx *ctx, const tosca_mesh *mesh, const tosca_mesh_f32 *mesh32,
                                  const tosca_params *params, const float *ucont, const float *ucat,
                                  const float *theta, double *rhs, double scale, double *cs, float *nut,

@@ -278,7 +278,8 @@ void queue_step(tosca_ctx *c, const tosca_mesh *m, const tosca_params *p, const 
     if (!ws) ws = c->ws;
     // UpdateCs (+ the interior of UpdateNut)
     if (p->les_model == TOSCA_LES_DSM) {
-        tosca::fused_dsm<T>(g, m, ucat, p->max_cs, cs, nut, ws, s);
+        // with FormU on the FP32 metric copies, the filter kernel reads their csi, eta, zet too
+        tosca::fused_dsm<T>(g, m, ucat, p->max_cs, cs, nut, ws, s, std::is_same<MT, float>::value ? m32 : nullptr);
     } else if (p->les_model == TOSCA_LES_DPASD) {
         tosca::fused_dpasd<T>(g, m, ucat, p->max_cs, cs, nut, ws, s);
     } else {

@@ -44,10 +44,11 @@ size_t formu_ws_doubles(const Geo &g);
 
 // Dynamic Smagorinsky: LM, MM and |S| of every interior cell into ws (3 local scalar fields),
 // then the box average + clip into cs (interior) and, when nut != nullptr, nut = Cs Delta^2 |S|
-// (interior).  Ghost handling is left to the caller.  T: type of ucat and nut.
+// (interior).  Ghost handling is left to the caller.  T: type of ucat and nut.  m32 (FP32 flow
+// fields only): FP32 copies of csi, eta, zet the filter kernel reads instead of the mesh's.
 template <class T>
 void fused_dsm(const Geo &g, const tosca_mesh *m, const T *ucat, double maxCs, double *cs, T *nut, double *ws,
-               hipStream_t s);
+               hipStream_t s, const tosca_mesh_f32 *m32 = nullptr);
 
 // Plane-averaged dynamic model (DPASD): LM, MM, |S| (3-point filter) and the 5-point filter's
 // QN, NN summed over every j plane into Cs(j), written to cs (interior) with, when nut != nullptr,

@@ -229,7 +229,7 @@ __device__ __forceinline__ FaceIn<T, MT> face_in(const MainArgs<T, MT> &A, int c
     f.fz = ldnt3s(face_fz<DIR>(A), c);
     f.fa = face_fa<DIR>(A)[c];
     f.V = __builtin_nontemporal_load(A.ucont + 3 * c + DIR);
-    f.nutR = A.nut[c + s];
+    f.nutR = A.les ? A.nut[c + s] : T(0);      // nu_t is read only when the LES term uses it
     return f;
 }
 
@@ -466,10 +466,10 @@ __global__ void __launch_bounds__(64 * NR) k_fused_formu(MainArgs<T, MT> A, Geo 
             if (needF2) in2 = face_in<1>(A, cq, SJ);
             if (needF3) in3 = face_in<2>(A, cq, SK);
         }
-        // nu_t is loaded whether or not the viscous LES term uses it (the loop has no divergent
-        // branches): A.nut must never be null; the launcher substitutes a valid local array when
-        // the caller passes none (a null nu_t here faulted the GPU once, in formMode 1)
-        const T nut0 = (needF3 && t > kb) ? nutNext : A.nut[cq];
+        // nu_t is read only when the LES term uses it (A.les, uniform across the launch); A.nut
+        // still must never be null: the launcher substitutes a valid local array when the caller
+        // passes none (a null nu_t loaded here unconditionally faulted the GPU once, in formMode 1)
+        const T nut0 = (needF3 && t > kb) ? nutNext : (A.les ? A.nut[cq] : T(0));
         // ---- face fluxes ---------------------------------------------------------------------------
         const Vec3<T> z3{0, 0, 0};
         FluxT<T> F1{z3, z3}, F2{z3, z3}, F3{z3, z3};

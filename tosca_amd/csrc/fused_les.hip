@@ -38,10 +38,10 @@ __device__ __forceinline__ double simpson_mask_sum(int c, int m) {
 // contravariant velocity u.csi, u.eta, u.zet), Sxx Sxy Sxz Syy Syz Szz, |S|, 1/aj.  The velocity
 // arithmetic (strain, |S|, U) is in the flow type T; the shift (uref, Uref) is subtracted from
 // u and U when the sums are FP32.
-template <class S, class T>
+template <class S, class T, class MT>
 __device__ __forceinline__ void filter_base(const Geo &g, int i, int j, int K, Vec3<T> uK, Vec3<T> uim, Vec3<T> uip,
-                                            Vec3<T> ujm, Vec3<T> ujp, Vec3<T> uKm, Vec3<T> uKp, V3 cC, V3 eC, V3 zC,
-                                            double aC, Vec3<T> uref, Vec3<T> Uref, S b[14]) {
+                                            Vec3<T> ujm, Vec3<T> ujp, Vec3<T> uKm, Vec3<T> uKp, Vec3<MT> cC, Vec3<MT> eC,
+                                            Vec3<MT> zC, double aC, Vec3<T> uref, Vec3<T> Uref, S b[14]) {
     const Vec3<T> cT = cvt3<T>(cC), eT = cvt3<T>(eC), zT = cvt3<T>(zC);
     const CartT<T> d = du_dxyz(cT, eT, zT, T(aC), du_center_v(g, i, j, K, uK, uim, uip, ujm, ujp, uKm, uKp));
     const T h = T(0.5);
@@ -71,8 +71,8 @@ __device__ __forceinline__ S filter_quantity(int n, const S b[14]) {
 }
 
 // The reference-velocity shift of a tile: the velocity and the contravariant velocity of cell cr
-template <class T>
-__device__ __forceinline__ void tile_shift(const T *ucat, const double *csi, const double *eta, const double *zet, int cr,
+template <class T, class MT>
+__device__ __forceinline__ void tile_shift(const T *ucat, const MT *csi, const MT *eta, const MT *zet, int cr,
                                            Vec3<T> &uref, Vec3<T> &Uref) {
     uref = ld3(ucat, cr);
     const Vec3<T> cr3 = cvt3<T>(ld3(csi, cr)), er3 = cvt3<T>(ld3(eta, cr)), zr3 = cvt3<T>(ld3(zet, cr));
@@ -86,9 +86,11 @@ __device__ __forceinline__ void tile_shift(const T *ucat, const double *csi, con
 template <class T> struct DsmSum { using type = double; };
 template <> struct DsmSum<float> { using type = float; };
 
-template <class T>
-__global__ void __launch_bounds__(64 * DSM_NR) k_dsm_lm(Geo g, const T *ucat, const double *csi,
-                                                        const double *eta, const double *zet, const double *aj,
+// MT: storage type of csi, eta, zet (float: the FP32 metric copies of the mixed step that runs
+// FormU on them; with FP32 flow fields the strain uses FP32 metrics either way)
+template <class T, class MT = double>
+__global__ void __launch_bounds__(64 * DSM_NR) k_dsm_lm(Geo g, const T *ucat, const MT *csi,
+                                                        const MT *eta, const MT *zet, const double *aj,
                                                         double *LM, double *MM, double *Sabs, int kchunk) {
     using S = typename DsmSum<T>::type;
     constexpr bool shifted = !std::is_same<S, double>::value;
@@ -123,7 +125,7 @@ __global__ void __launch_bounds__(64 * DSM_NR) k_dsm_lm(Geo g, const T *ucat, co
     Vec3<T> uKm = ld3(ucat, c0 + (kb - 2) * SK), uK = ld3(ucat, c0 + (kb - 1) * SK);
     // the metrics of the previous plane's point, the cell the finished output belongs to: carried in
     // registers where the FP32 sums leave room for them, loaded again with FP64 sums
-    V3 cP{0, 0, 0}, eP{0, 0, 0}, zP{0, 0, 0};
+    Vec3<MT> cP{0, 0, 0}, eP{0, 0, 0}, zP{0, 0, 0};
     double aP = 1.0;
 
     // 1 / (the mask weight of the output cell's 3 x 3 x 3 neighbourhood), for each k weight sum
@@ -141,7 +143,7 @@ __global__ void __launch_bounds__(64 * DSM_NR) k_dsm_lm(Geo g, const T *ucat, co
         // plane spills at this register budget and measured slower)
         const int cK = opaque_index(c0 + K * SK);
         const Vec3<T> uKp = ld3(ucat, cK + SK);
-        const V3 cC = ld3(csi, cK), eC = ld3(eta, cK), zC = ld3(zet, cK);
+        const Vec3<MT> cC = ld3(csi, cK), eC = ld3(eta, cK), zC = ld3(zet, cK);
         const double aC = aj[cK];
         // ---- neighbours of the point (i, j, K) ----------------------------------------------------
         sU[0][r][lane] = uK.x; sU[1][r][lane] = uK.y; sU[2][r][lane] = uK.z;
@@ -210,7 +212,8 @@ __global__ void __launch_bounds__(64 * DSM_NR) k_dsm_lm(Geo g, const T *ucat, co
             // Delta^2 = (1/aj)^(2/3) = r(aj)^2 and Delta~^2 = (8 q)^(2/3) = 4 q r(q), r(x) = x^(-1/3)
             const double rf = rcbrt_pos(aP), q27 = double(q[27]);
             double nm, dn;
-            germano_sq(L, Sh, SSh, rf * rf, 4.0 * q27 * rcbrt_pos(q27) * rden, cP, eP, zP, nm, dn);
+            germano_sq(L, Sh, SSh, rf * rf, 4.0 * q27 * rcbrt_pos(q27) * rden, cvt3<double>(cP), cvt3<double>(eP),
+                       cvt3<double>(zP), nm, dn);
             const double rd2 = rden * rden;
             LM[c] = nm * rd2;
             MM[c] = dn * rd2;
@@ -648,12 +651,21 @@ int plane_chunk(int nk, int tiles, int target) {
 }
 
 // LM, MM (3-point test filter + Germano contraction) and |S| of every interior cell
+// m32: FP32 copies of csi, eta, zet to read instead of the mesh's (FP32 flow fields only)
 template <class T>
-void launch_dsm_lm(const Geo &g, const tosca_mesh *m, const T *ucat, double *LM, double *MM, double *Sabs, hipStream_t s) {
+void launch_dsm_lm(const Geo &g, const tosca_mesh *m, const T *ucat, double *LM, double *MM, double *Sabs, hipStream_t s,
+                   const tosca_mesh_f32 *m32 = nullptr) {
     const int nk = g.mz - 2;
     const int ti = (g.mx - 2 + DSM_TI - 1) / DSM_TI, tj = (g.my - 2 + DSM_TJ - 1) / DSM_TJ;
     const int kchunk = plane_chunk(nk, ti * tj, 2048);
     dim3 grid(ti, tj, (nk + kchunk - 1) / kchunk);
+    if constexpr (std::is_same<T, float>::value) {
+        if (m32) {
+            TOSCA_LAUNCH("k_dsm_lm_f32m", (k_dsm_lm<float, float>), grid, dim3(64, DSM_NR), 0, s, g, ucat, m32->csi,
+                         m32->eta, m32->zet, m->aj, LM, MM, Sabs, kchunk);
+            return;
+        }
+    }
     TOSCA_LAUNCH(sizeof(T) == 8 ? "k_dsm_lm" : "k_dsm_lm_f32", k_dsm_lm<T>, grid, dim3(64, DSM_NR), 0, s, g, ucat, m->csi,
                  m->eta, m->zet, m->aj, LM, MM, Sabs, kchunk);
 }
@@ -704,11 +716,11 @@ void fused_dpasd(const Geo &g, const tosca_mesh *m, const T *ucat, double maxCs,
 
 template <class T>
 void fused_dsm(const Geo &g, const tosca_mesh *m, const T *ucat, double maxCs, double *cs, T *nut, double *ws,
-               hipStream_t s) {
+               hipStream_t s, const tosca_mesh_f32 *m32) {
     const long n = g.nloc();
     double *LM = ws, *MM = ws + n, *Sabs = ws + 2 * n;
     const int nk = g.mz - 2;
-    launch_dsm_lm<T>(g, m, ucat, LM, MM, Sabs, s);
+    launch_dsm_lm<T>(g, m, ucat, LM, MM, Sabs, s, m32);
     const int ctiles = ((g.mx - 2 + CX - 1) / CX) * ((g.my - 2 + CY - 1) / CY);
     int cchunks = max(1, min((2304 + ctiles - 1) / ctiles, nk / 8 > 0 ? nk / 8 : 1));
     const int ckchunk = (nk + cchunks - 1) / cchunks;
@@ -731,7 +743,7 @@ void nut_bcs(T *nut, const tosca_params *p, const Geo &g, hipStream_t s) { launc
     template void zero_shell<T>(T *, int, int, int, int, int, int, hipStream_t);                            \
     template void zero_outside_interior<T>(T *, const Geo &, hipStream_t);                                  \
     template void fused_dsm<T>(const Geo &, const tosca_mesh *, const T *, double, double *, T *, double *,   \
-                               hipStream_t);                                                                 \
+                               hipStream_t, const tosca_mesh_f32 *);                                         \
     template void fused_dpasd<T>(const Geo &, const tosca_mesh *, const T *, double, double *, T *, double *, \
                                  hipStream_t);                                                               \
     template void fused_nut<T>(const Geo &, const tosca_mesh *, const T *, const double *, T *, T *,         \
