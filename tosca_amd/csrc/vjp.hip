@@ -116,6 +116,7 @@ struct NutBCv { int bc[6]; };
 template <class T, class P>
 struct VjpArgs {
     const T *ucont, *ucat, *theta, *nut;
+    T *nut_w;           // where k_vjp_pre writes the recomputed nu_t (the array nut reads)
     const double *cs, *rbar;
     const double *csi, *eta, *zet, *icsi, *ieta, *izet, *jcsi, *jeta, *jzet, *kcsi, *keta, *kzet;
     const double *aj, *iaj, *jaj, *kaj;
@@ -159,18 +160,15 @@ __global__ void k_vjp_zero(T *a, long n) {
 
 }  // namespace
 
-// fpbar at every interior cell (gather; no atomics)
+// fpbar of interior cell (i, j, k) (a gather; no atomics); S: its csi, eta, zet
 template <class T, class P>
-__global__ void __launch_bounds__(256) k_vjp_fp(VjpArgs<T, P> A, Geo g) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y * blockDim.y + threadIdx.y + 1,
-              k = blockIdx.z + 1;
-    if (i > g.mx - 2 || j > g.my - 2 || k > g.mz - 2) return;
+__device__ __forceinline__ void fp_cell(const VjpArgs<T, P> &A, const Geo &g, int i, int j, int k, const V3 *S) {
     const int idx[3] = {i, j, k}, m[3] = {g.mx, g.my, g.mz};
     const int c = g.L(k, j, i);
     V3 fb{0, 0, 0};
     for (int q = 0; q < 3; q++) {
         const int dq[3] = {q == 0, q == 1, q == 2};
-        const V3 Sc = cellS(A, q, c);
+        const V3 Sc = S[q];
         // face q of this cell (this cell on the left)
         double w = A.scale * faceA(A, q, c) * 0.5 * A.rbar[3 * g.G(k, j, i) + q];
         fb = add3v(fb, sc3(w, Sc));
@@ -199,6 +197,33 @@ __global__ void __launch_bounds__(256) k_vjp_fp(VjpArgs<T, P> A, Geo g) {
         }
     }
     stp3(A.fpbar, c, fb);
+}
+
+// fpbar at every interior cell
+template <class T, class P>
+__global__ void __launch_bounds__(256) k_vjp_fp(VjpArgs<T, P> A, Geo g) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y * blockDim.y + threadIdx.y + 1,
+              k = blockIdx.z + 1;
+    if (i > g.mx - 2 || j > g.my - 2 || k > g.mz - 2) return;
+    const int c = g.L(k, j, i);
+    const V3 S[3] = {ld3(A.csi, c), ld3(A.eta, c), ld3(A.zet, c)};
+    fp_cell(A, g, i, j, k, S);
+}
+
+// fpbar and the recomputed nu_t = Cs Delta^2 |S| (UpdateNut's interior, as k_nut_fused forms it,
+// les.c:2510-2514) of every interior cell, from one load of its csi, eta, zet
+template <class T, class P>
+__global__ void __launch_bounds__(256) k_vjp_pre(VjpArgs<T, P> A, Geo g) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y * blockDim.y + threadIdx.y + 1,
+              k = blockIdx.z + 1;
+    if (i > g.mx - 2 || j > g.my - 2 || k > g.mz - 2) return;
+    const int c = g.L(k, j, i);
+    const V3 S[3] = {ld3(A.csi, c), ld3(A.eta, c), ld3(A.zet, c)};
+    const double a = A.aj[c];
+    const double Sm = strain_mag(du_dxyz(cvt3<T>(S[0]), cvt3<T>(S[1]), cvt3<T>(S[2]), T(a), du_center(g, A.ucat, i, j, k)));
+    const double filter = cube_root(1. / a);
+    A.nut_w[c] = T(A.cs[c] * (filter * filter) * Sm);
+    fp_cell(A, g, i, j, k, S);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -510,14 +535,11 @@ void vjp_rhs_impl(const Geo &g, const tosca_mesh *mesh, const tosca_params *p, c
     if (cs_bar) launch_fill(cs_bar, n, 0.0, s);
     TOSCA_LAUNCH("k_vjp_zero", k_vjp_zero<P>, zg, zb, 0, s, nueb, 3 * n);
     const bool les = p->les_active != 0;
-    if (les && !nut_in) {   // recompute nu_t as the forward step does
-        fused_nut<T>(g, mesh, ucat, cs, nut, nullptr, s);
-        zero_outside_interior(nut, g, s);
-        nut_bcs(nut, p, g, s);
-    }
+    const bool renut = les && !nut_in;      // nu_t recomputed as the forward step does (k_vjp_pre)
     VjpArgs<T, P> A;
     A.ucont = ucont; A.ucat = ucat; A.theta = nullptr; A.cs = cs; A.rbar = rbar;
     A.nut = les ? (nut_in ? nut_in : nut) : ucat;
+    A.nut_w = nut;
     A.csi = mesh->csi; A.eta = mesh->eta; A.zet = mesh->zet;
     A.icsi = mesh->icsi; A.ieta = mesh->ieta; A.izet = mesh->izet;
     A.jcsi = mesh->jcsi; A.jeta = mesh->jeta; A.jzet = mesh->jzet;
@@ -536,7 +558,13 @@ void vjp_rhs_impl(const Geo &g, const tosca_mesh *mesh, const tosca_params *p, c
     for (int q = 0; q < 6; q++) { A.bc_u[q] = p->bc_u[q]; A.bc_nut[q] = p->bc_nut[q]; }
     const bool f32 = sizeof(T) == 4;
     const dim3 cb(64, 4), cg((g.mx - 2 + 63) / 64, (g.my - 2 + 3) / 4, g.mz - 2);
-    TOSCA_LAUNCH(f32 ? "k_vjp_fp_f32" : "k_vjp_fp", (k_vjp_fp<T, P>), cg, cb, 0, s, A, g);
+    if (renut) {
+        TOSCA_LAUNCH(f32 ? "k_vjp_pre_f32" : "k_vjp_pre", (k_vjp_pre<T, P>), cg, cb, 0, s, A, g);
+        zero_outside_interior(nut, g, s);
+        nut_bcs(nut, p, g, s);
+    } else {
+        TOSCA_LAUNCH(f32 ? "k_vjp_fp_f32" : "k_vjp_fp", (k_vjp_fp<T, P>), cg, cb, 0, s, A, g);
+    }
     // faces: x along i (0 .. mx-2 for i faces, 1 .. mx-2 otherwise), y / z the other two axes
     TOSCA_LAUNCH(f32 ? "k_vjp_face_i_f32" : "k_vjp_face_i", (k_vjp_face<0, T, P>), dim3((g.mx - 1 + 255) / 256, g.my - 2, g.mz - 2),
                  dim3(256), 0, s, A, g);
