@@ -268,25 +268,55 @@ __global__ void __launch_bounds__(CX * CY) k_dsm_cs_nut(Geo g, const double *LM,
     const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
     const bool own = i <= g.mx - 2 && j <= g.my - 2;
     double pw[2] = {0, 0}, pl[2] = {0, 0}, pm[2] = {0, 0};     // 2-D sums of planes K-2, K-1
-    int b = 0;
-    for (int K = kb - 1; K <= ke; K++) {
-        for (int idx = tid; idx < HP; idx += CX * CY) {
-            int I = i0 - 1 + idx % HX, J = j0 - 1 + idx / HX, KK = K;
-            double w = 0.0, l = 0.0, mm = 0.0;
-            if (I <= g.mx - 1 && J <= g.my - 1) {     // points past the owned range feed no output
-                w = 1. / aj[g.L(KK, J, I)];
-                bool ghost = false;
-                bool ok = remap_axis(I, 1, g.mx, g.per[0], false, ghost);
-                ok = remap_axis(J, 1, g.my, g.per[1], true, ghost) && ok;
-                ok = remap_axis(KK, 1, g.mz, g.per[2], false, ghost) && ok;
-                if (!ok) w = 0;
-                if (!ghost && w != 0) { const int c = g.L(KK, J, I); l = LM[c]; mm = MM[c]; }
+    // The staged values of the next plane are loaded into registers while this plane's sums are
+    // formed (raw loads: aj at the point, LM / MM at its remapped point; flags: in range, remap ok,
+    // ghost), and written to the other LDS buffer at the start of the next step.
+    constexpr int NPT = (HP + CX * CY - 1) / (CX * CY);
+    double ra[NPT], rl[NPT], rm[NPT];
+    int rf[NPT];
+    double ajN = 1.0, sabsN = 0.0;                // the output cell of the step fetch(K) serves, own threads
+    auto fetch = [&](int K) {
+#pragma unroll
+        for (int u = 0; u < NPT; u++) {
+            const int idx = tid + u * CX * CY;
+            ra[u] = 1.0; rl[u] = 0.0; rm[u] = 0.0; rf[u] = 0;
+            if (idx < HP) {
+                int I = i0 - 1 + idx % HX, J = j0 - 1 + idx / HX, KK = K;
+                if (I <= g.mx - 1 && J <= g.my - 1) {     // points past the owned range feed no output
+                    ra[u] = aj[g.L(KK, J, I)];
+                    bool ghost = false;
+                    bool ok = remap_axis(I, 1, g.mx, g.per[0], false, ghost);
+                    ok = remap_axis(J, 1, g.my, g.per[1], true, ghost) && ok;
+                    ok = remap_axis(KK, 1, g.mz, g.per[2], false, ghost) && ok;
+                    rf[u] = 1 | (ok ? 2 : 0) | (ghost ? 4 : 0);
+                    if (!ghost && ok) { const int c = g.L(KK, J, I); rl[u] = LM[c]; rm[u] = MM[c]; }
+                }
             }
-            W[b][idx] = w;
-            Lw[b][idx] = l;
-            Mw[b][idx] = mm;
         }
+        // the cell the step that stages plane K emits: k = K-1
+        if (own && K - 1 >= kb && K - 1 < ke) { const int c = g.L(K - 1, j, i); ajN = aj[c]; sabsN = Sabs[c]; }
+    };
+    int b = 0;
+    fetch(kb - 1);
+    for (int K = kb - 1; K <= ke; K++) {
+#pragma unroll
+        for (int u = 0; u < NPT; u++) {
+            const int idx = tid + u * CX * CY;
+            if (idx < HP) {
+                double w = 0.0, l = 0.0, mm = 0.0;
+                if (rf[u] & 1) {
+                    w = 1. / ra[u];
+                    if (!(rf[u] & 2)) w = 0;
+                    if (!(rf[u] & 4) && w != 0) { l = rl[u]; mm = rm[u]; }
+                }
+                W[b][idx] = w;
+                Lw[b][idx] = l;
+                Mw[b][idx] = mm;
+            }
+        }
+        const double ajC = ajN, sabsC = sabsN;     // the output cell of this step (k = K-1)
         __syncthreads();
+        if (K + 1 <= ke) fetch(K + 1);
         // 3 x 3 sums of this plane at (i, j), Q (j) outer, P (i) inner, as the reference's loops
         double sw = 0, sl = 0, sm = 0;
         for (int Q = 0; Q < 3; Q++)
@@ -307,8 +337,8 @@ __global__ void __launch_bounds__(CX * CY) k_dsm_cs_nut(Geo g, const double *LM,
             const double cs = pmin(pmax(C, 0.0), maxCs);
             Cs[c] = cs;
             if (nut) {
-                const double filter = cube_root(1. / aj[c]);
-                nut[c] = T(cs * (filter * filter) * Sabs[c]);
+                const double filter = cube_root(1. / ajC);
+                nut[c] = T(cs * (filter * filter) * sabsC);
             }
         }
         pw[0] = pw[1]; pw[1] = sw;
