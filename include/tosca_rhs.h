@@ -38,7 +38,8 @@ extern "C" {
 
 /* ---- error codes ------------------------------------------------------------------------ */
 #define TOSCA_OK               0
-#define TOSCA_ERR_ARG          1   /* null / misaligned pointer, bad enum value, bad form_mode   */
+#define TOSCA_ERR_ARG          1   /* null / misaligned pointer (device arrays: 8-byte aligned,
+                                        int tables 4-byte), bad enum value, bad form_mode        */
 #define TOSCA_ERR_UNSUPPORTED  2   /* IBM, overset, structural LES, schemes other than central/QUICK */
 #define TOSCA_ERR_LAYOUT       3   /* ghost width != 3 or not a whole-domain single-GPU layout   */
 #define TOSCA_ERR_DEVICE       4   /* HIP runtime error or device unavailable                   */

@@ -273,4 +273,14 @@ def test_cartesian_bcs_inflow(rhs_mod, kind):
                "interp": dict(weights=None), "interp_spline_shift_merge": dict(y_ids=None)}[kind]
     with pytest.raises(ToscaError):
         ctx.UpdateCartesianBCs(dev, params, Ug, None, None, _dev_inlet(dict(inlet, **missing)))
+    if kind == "powerlaw":
+        # an int table needs 4-byte alignment only: the draws one element into a larger array
+        di = _dev_inlet(inlet)
+        off = torch.zeros(di["noise"].numel() + 1, dtype=torch.int32, device="cuda")[1:]
+        off.copy_(di["noise"].reshape(-1))
+        assert off.data_ptr() % 8 == 4
+        U2 = to_dev(U0)
+        ctx.UpdateCartesianBCs(dev, params, U2, None, None, dict(di, noise=off.view(di["noise"].shape)))
+        torch.cuda.synchronize()
+        assert np.array_equal(to_np(U2), got)
     ctx.close()

@@ -104,10 +104,12 @@ bool device_forced_off() {
     return e && e[0] && strcmp(e, "0") != 0;
 }
 
-// `need` bytes starting at p must lie inside one device allocation.
-int check_dev(tosca_ctx *c, const void *p, size_t need, const char *name) {
+// `need` bytes starting at p must lie inside one device allocation; p aligned to `align` bytes (the
+// element size: 8 for double arrays, 4 for int and float arrays)
+int check_dev(tosca_ctx *c, const void *p, size_t need, const char *name, size_t align = 8) {
     if (!p) return set_err(c, TOSCA_ERR_ARG, "%s is NULL", name);
-    if (reinterpret_cast<uintptr_t>(p) % 8) return set_err(c, TOSCA_ERR_ARG, "%s is not 8-byte aligned", name);
+    if (reinterpret_cast<uintptr_t>(p) % align)
+        return set_err(c, TOSCA_ERR_ARG, "%s is not %zu-byte aligned", name, align);
     hipPointerAttribute_t attr;
     if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
         (void)hipGetLastError();
@@ -1351,7 +1353,7 @@ int tosca_update_cartesian_bcs(tosca_ctx *c, const tosca_mesh *m, const tosca_pa
                                                      "profiles are covered", t);
         if ((rc = check_dev(c, inlet->cent, 3 * n * sizeof(double), "inlet.cent"))) return rc;
         const size_t plane_pts = (size_t)c->geo.mx * c->geo.my;        // [my][mx] tables
-        if (t == TOSCA_INLET_POWERLAW && (rc = check_dev(c, inlet->noise, plane_pts * sizeof(int), "inlet.noise")))
+        if (t == TOSCA_INLET_POWERLAW && (rc = check_dev(c, inlet->noise, plane_pts * sizeof(int), "inlet.noise", 4)))
             return rc;
         if (t == TOSCA_INLET_MAPPED || t == TOSCA_INLET_INTERP) {
             if (inlet->n1 < 1 || inlet->n2 < 1)
@@ -1362,17 +1364,17 @@ int tosca_update_cartesian_bcs(tosca_ctx *c, const tosca_mesh *m, const tosca_pa
             if ((rc = check_dev(c, inlet->plane, 3 * np * sizeof(double), "inlet.plane"))) return rc;
         }
         if (t == TOSCA_INLET_INTERP) {
-            if ((rc = check_dev(c, inlet->cells, 8 * plane_pts * sizeof(int), "inlet.cells")) ||
+            if ((rc = check_dev(c, inlet->cells, 8 * plane_pts * sizeof(int), "inlet.cells", 4)) ||
                 (rc = check_dev(c, inlet->weights, 4 * plane_pts * sizeof(double), "inlet.weights")))
                 return rc;
             if (inlet->spline &&
-                ((rc = check_dev(c, inlet->cells_1, 12 * plane_pts * sizeof(int), "inlet.cells_1")) ||
-                 (rc = check_dev(c, inlet->cells_2, 12 * plane_pts * sizeof(int), "inlet.cells_2")) ||
+                ((rc = check_dev(c, inlet->cells_1, 12 * plane_pts * sizeof(int), "inlet.cells_1", 4)) ||
+                 (rc = check_dev(c, inlet->cells_2, 12 * plane_pts * sizeof(int), "inlet.cells_2", 4)) ||
                  (rc = check_dev(c, inlet->weights_1, 6 * plane_pts * sizeof(double), "inlet.weights_1")) ||
                  (rc = check_dev(c, inlet->weights_2, 6 * plane_pts * sizeof(double), "inlet.weights_2"))))
                 return rc;
             if (inlet->shift2 &&
-                ((rc = check_dev(c, inlet->y_ids, (size_t)c->geo.mx * sizeof(int), "inlet.y_ids")) ||
+                ((rc = check_dev(c, inlet->y_ids, (size_t)c->geo.mx * sizeof(int), "inlet.y_ids", 4)) ||
                  (rc = check_dev(c, inlet->y_weights, (size_t)c->geo.mx * sizeof(double), "inlet.y_weights"))))
                 return rc;
         }
