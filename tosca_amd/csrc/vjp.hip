@@ -9,7 +9,7 @@
 //   k_vjp_fp      face dot (ueqn_solvers.c:1006-1035) -> fpbar per interior cell (a gather: the
 //                 cell's own faces, the faces of its lower neighbours, and the ghost m-1 aliases:
 //                 cell 1 on a periodic axis, the copied cell m-2 otherwise)
-//   k_vjp_face    cell divergence (ueqn_solvers.c:871-993) -> face-flux cotangent, then the
+//   k_vjp_faces   (the i, j and k faces of each index point) cell divergence (ueqn_solvers.c:871-993) -> face-flux cotangent, then the
 //                 convective flux (central / QUICK, inline.h:4314-4324, 5089-5109) -> ucont_bar and
 //                 a packet of ucat cotangents, the viscous flux (ueqn_solvers.c:184-216...) -> the
 //                 cotangents of its nine derivatives (Compute_du_i/j/k, inline.h:2748-2931) into the
@@ -234,18 +234,11 @@ __global__ void __launch_bounds__(256) k_vjp_pre(VjpArgs<T, P> A, Geo g) {
 // (the only ones with the one-sided and clipped stencils) scatter into ucat_bar directly.
 // Thread x runs along i for every face direction (coalesced).
 // ------------------------------------------------------------------------------------------------
+// one face of direction DIR: x = (i, j, k) its index point, f = x[DIR] in 0 .. m-2, the other two
+// in 1 .. m-2
 template <int DIR, class T, class P>
-__global__ void __launch_bounds__(256) k_vjp_face(VjpArgs<T, P> A, Geo g) {
+__device__ __forceinline__ void face_adj(const VjpArgs<T, P> &A, const Geo &g, const int *x) {
     const int m[3] = {g.mx, g.my, g.mz};
-    int x[3];
-    x[0] = blockIdx.x * blockDim.x + threadIdx.x;
-    x[1] = blockIdx.y;
-    x[2] = blockIdx.z;
-    // faces f = 0 .. m-2 along DIR (thread index as is), transverse indices 1 .. m-2 (index + 1)
-    for (int d = 0; d < 3; d++) {
-        if (d != DIR) x[d] += 1;
-        if (x[d] > m[d] - 2) return;
-    }
     const int f = x[DIR];
     const bool per = g.per[DIR] != TOSCA_PERIODIC_NONE;
     if (per && f == 0) return;                     // face 0 of a periodic axis is face m-2
@@ -338,6 +331,18 @@ __global__ void __launch_bounds__(256) k_vjp_face(VjpArgs<T, P> A, Geo g) {
         else if (DIR == 1) du_j_adj(g, A.ubar, i, j, k, gc, ge, gz);
         else du_k_adj(g, A.ubar, i, j, k, gc, ge, gz);
     }
+}
+
+// the i, j and k faces of index point (i, j, k), i in 0 .. mx-2, j in 0 .. my-2, k in 0 .. mz-2
+// (each face where it exists): the three share the point's fpbar, ucont, nu_t and ucat loads
+template <class T, class P>
+__global__ void __launch_bounds__(256) k_vjp_faces(VjpArgs<T, P> A, Geo g) {
+    const int x[3] = {(int)(blockIdx.x * blockDim.x + threadIdx.x), (int)blockIdx.y, (int)blockIdx.z};
+    if (x[0] > g.mx - 2) return;
+    const bool in0 = x[0] >= 1, in1 = x[1] >= 1, in2 = x[2] >= 1;
+    if (in1 && in2) face_adj<0>(A, g, x);
+    if (in0 && in2) face_adj<1>(A, g, x);
+    if (in0 && in1) face_adj<2>(A, g, x);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -566,11 +571,7 @@ void vjp_rhs_impl(const Geo &g, const tosca_mesh *mesh, const tosca_params *p, c
         TOSCA_LAUNCH(f32 ? "k_vjp_fp_f32" : "k_vjp_fp", (k_vjp_fp<T, P>), cg, cb, 0, s, A, g);
     }
     // faces: x along i (0 .. mx-2 for i faces, 1 .. mx-2 otherwise), y / z the other two axes
-    TOSCA_LAUNCH(f32 ? "k_vjp_face_i_f32" : "k_vjp_face_i", (k_vjp_face<0, T, P>), dim3((g.mx - 1 + 255) / 256, g.my - 2, g.mz - 2),
-                 dim3(256), 0, s, A, g);
-    TOSCA_LAUNCH(f32 ? "k_vjp_face_j_f32" : "k_vjp_face_j", (k_vjp_face<1, T, P>), dim3((g.mx - 2 + 255) / 256, g.my - 1, g.mz - 2),
-                 dim3(256), 0, s, A, g);
-    TOSCA_LAUNCH(f32 ? "k_vjp_face_k_f32" : "k_vjp_face_k", (k_vjp_face<2, T, P>), dim3((g.mx - 2 + 255) / 256, g.my - 2, g.mz - 1),
+    TOSCA_LAUNCH(f32 ? "k_vjp_faces_f32" : "k_vjp_faces", (k_vjp_faces<T, P>), dim3((g.mx - 1 + 255) / 256, g.my - 1, g.mz - 1),
                  dim3(256), 0, s, A, g);
     if (les && !nut_in) TOSCA_LAUNCH(f32 ? "k_vjp_nut_f32" : "k_vjp_nut", (k_vjp_nut<T, P>), cg, cb, 0, s, A, g);
     if (les && nut_in)
