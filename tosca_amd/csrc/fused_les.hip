@@ -525,6 +525,43 @@ __global__ void __launch_bounds__(DK_X * DK_Y) k_dpasd_k(Geo g, const typename D
         const double dij = simpson5_mask_sum(i, g.mx) * simpson5_mask_sum(j, g.my);
         const long SP = (long)ni * nj;
         const long x0 = (long)(j - 1) * ni + (i - 1);
+        // FP32 sums: four output planes per step (planes k-2 .. k+5 read once for the four, two
+        // planes per output), the filtered values kept in FP32 until each output's contraction;
+        // the same values and the same order of the sums as two per step
+        if constexpr (std::is_same<S, float>::value) {
+            for (int k = kb; k < ke; k += 4) {
+                const int nout = min(4, ke - k);
+                bool in[8];
+#pragma unroll
+                for (int o = 0; o < 8; o++) in[o] = k - 2 + o >= 1 && k - 2 + o <= g.mz - 2 && o < nout + 4;
+                const S *xk = X + (long)(k - 1) * SP + x0;
+                S F4[4][NQ];
+#pragma unroll
+                for (int n = 0; n < NQ; n++) {
+                    const S *xn = xk + (long)n * nI;
+                    S v[8];
+#pragma unroll
+                    for (int o = 0; o < 8; o++) v[o] = in[o] ? xn[(o - 2) * SP] : S(0);
+#pragma unroll
+                    for (int u = 0; u < 4; u++) F4[u][n] = row5(n, v[u], v[u + 1], v[u + 2], v[u + 3], v[u + 4]);
+                }
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    if (u < nout) {
+                        double F[NQ];
+#pragma unroll
+                        for (int n = 0; n < NQ; n++) F[n] = double(F4[u][n]);
+                        double qn, nn;
+                        const int c = g.L(k + u, j, i);
+                        dpasd_germano(F, 1.0 / (dij * simpson5_mask_sum(k + u, g.mz)), csi, eta, zet, aj, c, qn, nn);
+                        sLM += LM[c];
+                        sMM += MM[c];
+                        sQN += qn;
+                        sNN += nn;
+                    }
+                }
+            }
+        } else
         // two output planes per step: planes k-2 .. k+3 are read once for both, so each output reads
         // three planes from L2 instead of five.  Planes outside [1, mz-2] are outside the mask; the
         // conditions are uniform over the workgroup.
