@@ -250,17 +250,24 @@ def vjp_leg(torch, ctx, mesh, case, ucont, ucat, theta, cs, lay, args, step_ms, 
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / args.steps
 
+    def kernel_times(fn):
+        # per-kernel device time over `steps` calls (no host gaps between the calls)
+        ctx.enable_timing(True)
+        ctx.reset_timing()
+        for _ in range(args.steps):
+            fn()
+        t = ctx.timings()
+        ctx.enable_timing(False)
+        return t, sum(v[0] for v in t.values()) / args.steps
+
     fwd_ms, vjp_ms = timed(fwd), timed(vjp)
-    ctx.enable_timing(True)
-    ctx.reset_timing()
-    for _ in range(args.steps):
-        vjp()
-    tim = ctx.timings()
-    ctx.enable_timing(False)
-    print(f"# vjp: {vjp_ms:.3f} ms vs forward {fwd_ms:.3f} ms (x{vjp_ms / fwd_ms:.2f}), vs step x{vjp_ms / step_ms:.2f}",
-          file=sys.stderr, flush=True)
+    tim, vjp_k = kernel_times(vjp)
+    _, fwd_k = kernel_times(fwd)
+    print(f"# vjp: {vjp_ms:.3f} ms vs forward {fwd_ms:.3f} ms (x{vjp_ms / fwd_ms:.2f}), vs step x{vjp_ms / step_ms:.2f}; "
+          f"kernel time {vjp_k:.3f} vs {fwd_k:.3f} ms (x{vjp_k / fwd_k:.2f})", file=sys.stderr, flush=True)
     out = {"vjp_ms": vjp_ms, "forward_ms": fwd_ms, "vjp_over_forward": vjp_ms / fwd_ms,
            "vjp_over_step": vjp_ms / step_ms, "kernels_ms": {k: v[0] / v[1] for k, v in tim.items()},
+           "vjp_kernels_ms": vjp_k, "forward_kernels_ms": fwd_k, "vjp_over_forward_kernels": vjp_k / fwd_k,
            "forward": "UpdateNut + FormU + Buoyancy (FP64) with the step's Cs"}
     # mixed adjoint
     x32 = [t.float().contiguous() for t in (ucont, ucat, theta)]
