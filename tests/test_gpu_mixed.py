@@ -51,6 +51,8 @@ CASES = [
     ("shard dsm", lambda: C.shard_case(nx=64, ny=64, nz=32)),
     ("hill dpasd", lambda: C.hill_case(nx=24, ny=12, nz=10, les_model=C.LES_DPASD)),
     ("hill dpasd multitile", lambda: C.hill_case(nx=40, ny=130, nz=28, les_model=C.LES_DPASD)),
+    # 25 k planes in chunks of 9, 9, 7: the FP32 k pass's last step of four has one or three outputs
+    ("hill dpasd odd chunks", lambda: C.hill_case(nx=25, ny=12, nz=10, les_model=C.LES_DPASD)),
 ]
 
 
