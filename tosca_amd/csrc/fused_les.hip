@@ -138,13 +138,29 @@ __global__ void __launch_bounds__(64 * DSM_NR) k_dsm_lm(Geo g, const T *ucat, co
     for (int n = 0; n < NQ; n++) { A[n] = S(0); B[n] = S(0); }
     double sabsPrev = 0.0;
 
+    // FP32 sums: the next plane's metrics and ucat are loaded one plane ahead (the ghost planes
+    // keep K + 2 <= mz + 1 inside the local array)
+    Vec3<T> uN{T(0), T(0), T(0)};
+    Vec3<MT> cN{0, 0, 0}, eN{0, 0, 0}, zN{0, 0, 0};
+    double aN = 1.0;
+    if constexpr (shifted) {
+        const int c1 = c0 + (kb - 1) * SK;
+        uN = ld3(ucat, c1 + SK); cN = ld3(csi, c1); eN = ld3(eta, c1); zN = ld3(zet, c1); aN = aj[c1];
+    }
     for (int K = kb - 1; K <= ke; K++) {
-        // loads of this plane, issued before the exchange barrier (a register prefetch of the next
-        // plane spills at this register budget and measured slower)
+        // loads of this plane, issued before the exchange barrier (with FP64 sums a register
+        // prefetch of the next plane spills at this register budget and measured slower)
         const int cK = opaque_index(c0 + K * SK);
-        const Vec3<T> uKp = ld3(ucat, cK + SK);
-        const Vec3<MT> cC = ld3(csi, cK), eC = ld3(eta, cK), zC = ld3(zet, cK);
-        const double aC = aj[cK];
+        Vec3<T> uKp;
+        Vec3<MT> cC, eC, zC;
+        double aC;
+        if constexpr (shifted) {
+            uKp = uN; cC = cN; eC = eN; zC = zN; aC = aN;
+        } else {
+            uKp = ld3(ucat, cK + SK);
+            cC = ld3(csi, cK); eC = ld3(eta, cK); zC = ld3(zet, cK);
+            aC = aj[cK];
+        }
         // ---- neighbours of the point (i, j, K) ----------------------------------------------------
         sU[0][r][lane] = uK.x; sU[1][r][lane] = uK.y; sU[2][r][lane] = uK.z;
         const bool pt = mi && mj && K >= 1 && K <= g.mz - 2;     // point inside the filter mask
@@ -157,6 +173,11 @@ __global__ void __launch_bounds__(64 * DSM_NR) k_dsm_lm(Geo g, const T *ucat, co
         else ujm = ld3(ucat, g.L(K, jma, ia));
         if (r < DSM_NR - 1) ujp = Vec3<T>{sU[0][r + 1][lane], sU[1][r + 1][lane], sU[2][r + 1][lane]};
         else ujp = ld3(ucat, g.L(K, jpa, ia));
+        if constexpr (shifted) {
+            // issued after this plane's own loads, so that waiting for those does not wait for these
+            const int cn = opaque_index(cK + SK);
+            uN = ld3(ucat, cn + SK); cN = ld3(csi, cn); eN = ld3(eta, cn); zN = ld3(zet, cn); aN = aj[cn];
+        }
         // ---- point quantities (les.c:466-611), zero outside the mask -------------------------------
         // base values b: u v w, U V W, Sxx Sxy Sxz Syy Syz Szz, |S|, 1/aj; the 28 filtered quantities
         // are formed from them one at a time, right before their i filter
